@@ -1,0 +1,223 @@
+#!/usr/bin/env python
+"""Benchmark of the TT hot path on one MI355X.
+
+Headline: ALS_SPD full sweeps per second on the d=16, n=16 Dirichlet-Laplacian + U(0,1) diagonal potential
+operator (rank-3 channel form), random rank-4 right-hand side, solution rank 64; one step = one full sweep
+(ALS_SPD with numHalfSweeps=2, convergenceEpsilon=0) followed by round(1e-12). The mixed-precision path
+(FP32 or BF16 local matvecs with FP64 refinement) is the reported value; the FP64 device path runs the same
+steps from the same start for the ratio, and the relative residual ‖Ax−b‖/‖b‖ of both is reported.
+Secondary: ADF gradients per second (d=10, n=10, rank 8, 200k sampled entries, FP32, all cores).
+
+Prints one JSON line. `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+SEED = 0xBAADF00D
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
+PEAK_HBM_GBS = 8000.0
+
+
+def build_config2(xe, np):
+    from xerus_amd.workloads import laplace_cores
+    d, n = 16, 16
+    pots = np.random.default_rng(SEED).random((d, n))
+    A = xe.TTOperator.from_components(laplace_cores(d, n, list(pots), rank3=True))
+    xe.seed(SEED)
+    b = xe.TTTensor.random([n] * d, [4] * (d - 1))
+    x0 = xe.TTTensor.random([n] * d, [64] * (d - 1))
+    return A, b, x0, pots
+
+
+def run_als(xe, torch, A, b, x0, precision, steps):
+    x = x0.copy()
+    als = xe.ALS_SPD.copy()
+    als.convergenceEpsilon = 0.0
+    als.precision = precision
+    stats = {"cgIterations": 0, "localSolves": 0, "refinements": 0, "flopsMatvec": 0.0, "flopsEnv": 0.0}
+    torch.cuda.synchronize()
+    xe.lib().xt_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        als(A, x, b, 2)
+        x.round(1e-12)
+        for k in stats:
+            stats[k] += als.last_stats[k]
+    xe.lib().xt_synchronize()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, x, stats
+
+
+def adf_workload(xe, np):
+    d, n, r, P = 10, 10, 8, 200_000
+    rng = np.random.default_rng(SEED + 5)
+    xe.seed(SEED)
+
+    def tt(scale):
+        t = xe.TTTensor([n] * d)
+        for k in range(d):
+            rl = 1 if k == 0 else r
+            rr = 1 if k == d - 1 else r
+            t.set_component(k, xe.normal_draws(rl * n * rr).reshape(rl, n, rr) * scale)
+        return t
+
+    truth = tt(1 / np.sqrt(8))
+    x = tt(1 / np.sqrt(8))
+    m = xe.SinglePointMeasurementSet.random(P, [n] * d)
+    m.measure(truth)
+    # "N(0, 1e-4)" noise read with the variance convention of the cores: σ = 1e-2
+    m.measuredValues = m.measuredValues + rng.normal(0.0, 1e-2, P)
+    return d, n, r, P, x, m
+
+
+def bench_adf(xe, torch, np, steps):
+    import ctypes as C
+    from xerus_amd._lib import call
+    d, n, r, P, x, m = adf_workload(xe, np)
+    cores = [x.get_component(k) for k in range(d)]
+    flat = torch.tensor(np.concatenate([c.ravel() for c in cores]), dtype=torch.float32, device="cuda")
+    vals_b = torch.tensor(m.measuredValues, dtype=torch.float32, device="cuda")
+    vals = torch.empty(P, dtype=torch.float32, device="cuda")
+    grad = torch.empty_like(flat)
+    ns = (C.c_int * d)(*[n] * d)
+    rs = (C.c_int * (d + 1))(*([1] + [r] * (d - 1) + [1]))
+    pos = m.positions_device_ptr()
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def one():
+        call("xt_tt_sample_f32", d, ns, rs, flat.data_ptr(), pos, P, vals.data_ptr(), stream)
+        resid = vals - vals_b
+        call("xt_tt_sample_adjoint_f32", d, ns, rs, flat.data_ptr(), pos, P, resid.data_ptr(), grad.data_ptr(), stream)
+
+    for _ in range(3):
+        one()
+    torch.cuda.synchronize()
+    xe._lib.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    prof = xe._lib.profile_read()
+    xe._lib.profile_enable(False)
+    # algorithmic bytes per full gradient: int32 indices + FP32 value in + FP32 residual, cores in + grads out
+    alg_bytes = P * (4 * d + 4 + 4) + 2 * flat.numel() * 4
+    grad_launch = prof.get("adf_grad_kernel<float>")
+    return {"grads_per_s": steps / dt, "ms_per_grad": 1e3 * dt / steps, "P": P, "alg_bytes": alg_bytes,
+            "grad_kernel_ms": grad_launch[1] / grad_launch[0] if grad_launch else None,
+            "residual_kernel_ms": (prof["adf_residual_kernel<float>"][1] / prof["adf_residual_kernel<float>"][0]
+                                   if "adf_residual_kernel<float>" in prof else None)}
+
+
+def cpu_baseline(np):
+    """The oracle (FP64 CPU restatement of xerus's ALS_SPD, matrix-free CG local solver) on the same
+    config-2 problem, bounded to the first half-sweep."""
+    from oracle import oracle as O
+    from xerus_amd.workloads import laplace_cores
+    d, n = 16, 16
+    pots = np.random.default_rng(SEED).random((d, n))
+    A = O.TT.from_cores(laplace_cores(d, n, list(pots), rank3=True), op=True)
+    O.seed(SEED)
+    b = O.TT.random([n] * d, [4] * (d - 1))
+    x = O.TT.random([n] * d, [64] * (d - 1))
+    t0 = time.perf_counter()
+    O.als(A, x, b, num_half_sweeps=1, eps=0.0, local_solver="cg", cg_tol=1e-12)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 0.5 / dt, "unit": "sweeps/s", "cores": threads, "kind": "port",
+            "sample": f"first half-sweep (14 local CG solves, tol 1e-12) of the d16 n16 r64 problem, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", choices=["f32", "bf16"], default="f32")
+    ap.add_argument("--adf-steps", type=int, default=200)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is described in DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import xerus_amd as xe
+
+    torch.cuda.set_device(0)
+    prec = xe.XT_F32 if args.precision == "f32" else xe.XT_BF16
+    A, b, x0, _ = build_config2(xe, np)
+    print(f"# config2 built: x0 ranks {x0.ranks()}", file=sys.stderr, flush=True)
+
+    if args.warmup > 0:
+        run_als(xe, torch, A, b, x0, prec, args.warmup)
+        print("# warmup done", file=sys.stderr, flush=True)
+    xe._lib.profile_enable(True)
+    t_mixed, x_mixed, st_mixed = run_als(xe, torch, A, b, x0, prec, args.steps)
+    prof = xe._lib.profile_read()
+    xe._lib.profile_enable(False)
+    res_mixed = xe.residual(A, x_mixed, b)
+    print(f"# mixed: {t_mixed:.2f} s, residual {res_mixed:.3e}, ranks {x_mixed.ranks()}", file=sys.stderr, flush=True)
+
+    fp64 = None
+    if not args.no_fp64:
+        t64, x64, st64 = run_als(xe, torch, A, b, x0, xe.XT_F64, args.steps)
+        res64 = xe.residual(A, x64, b)
+        diff = (x_mixed - x64).frob_norm() / x64.frob_norm()
+        fp64 = {"sweeps_per_s": args.steps / t64, "ms_per_step": 1e3 * t64 / args.steps, "rel_residual": res64,
+                "cg_iterations": st64["cgIterations"], "rel_diff_mixed_vs_fp64": diff}
+        print(f"# fp64: {t64:.2f} s, residual {res64:.3e}", file=sys.stderr, flush=True)
+
+    # dominant kernel of the timed region
+    dom_tag, (dom_n, dom_ms, dom_flops) = max(prof.items(), key=lambda kv: kv[1][1])
+    dtype_key = "bf16" if "bf16" in dom_tag else ("f64" if "double" in dom_tag else "f32")
+    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "kernel": dom_tag, "launches": dom_n, "avg_launch_us": 1e3 * dom_ms / dom_n,
+                "flops_per_launch": dom_flops / dom_n, "achieved": achieved, "peak": PEAK_TFLOPS[dtype_key],
+                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[dtype_key], "traffic": None}
+
+    adf = bench_adf(xe, torch, np, args.adf_steps) if args.adf_steps > 0 else None
+    if adf and adf["grad_kernel_ms"]:
+        adf["grad_kernel_GBs"] = adf["alg_bytes"] / (adf["grad_kernel_ms"] * 1e-3) / 1e9
+        adf["grad_kernel_hbm_frac"] = adf["grad_kernel_GBs"] / PEAK_HBM_GBS
+
+    cpu = None if args.no_cpu else cpu_baseline(np)
+
+    value = args.steps / t_mixed
+    out = {
+        "metric": "ALS sweep/s d16 n16 r64 mixed vs FP64-GPU @rel.res 1e-10; ADF grads/s",
+        "value": value,
+        "unit": "sweeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_mixed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision + "+f64-refinement",
+        "data": "synthetic (seeded Laplacian+potential operator, random TT right-hand side and start)",
+        "config": {"workload": "ALS_SPD d=16 n=16 x-rank 64, A rank 3 (Laplacian + U(0,1) potential), b rank 4; "
+                               "step = 2 half-sweeps + round(1e-12)",
+                   "precision": args.precision, "rel_residual": res_mixed, "ranks_final": max(x_mixed.ranks()),
+                   "cg_iterations": st_mixed["cgIterations"], "refinements": st_mixed["refinements"]},
+        "fp64_gpu": fp64,
+        "speedup_vs_fp64": (value / fp64["sweeps_per_s"]) if fp64 else None,
+        "adf": adf,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

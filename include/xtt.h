@@ -1,0 +1,157 @@
+/*
+ * libxtt — MI355X (gfx950) Tensor-Train engine behind xerus's TT API. C ABI: plain pointers, sizes and
+ * opaque handles; every function returns 0 (XT_OK) or an XT_ERR_* code, with a message in xt_last_error().
+ *
+ * Layout conventions are xerus's: row-major, last index fastest (include/xerus/tensor.h:84-88); a TT
+ * component k has dims (r_{k-1}, n_k, r_k), an operator component (r_{k-1}, n_k, m_k, r_k)
+ * (include/xerus/ttNetwork.h:316-321); operator dims are [n_0..n_{d-1}, m_0..m_{d-1}] (ttNetwork.cpp:70-80).
+ * All device work is ordered on the library's stream of the current device unless a `stream` argument is
+ * given; entry points taking a stream return only after that stream has finished the call's work.
+ */
+#ifndef XTT_H
+#define XTT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+	XT_OK = 0,
+	XT_ERR_INVALID = 1,     /* bad argument / shape: xerus REQUIRE → generic_error (misc/check.h:60-65) */
+	XT_ERR_HIP = 2,         /* HIP runtime failure */
+	XT_ERR_NUMERIC = 3,     /* numerical breakdown */
+	XT_ERR_NOMEM = 4,       /* device allocation failed */
+	XT_ERR_UNSUPPORTED = 5  /* shape outside what the kernels handle (e.g. ADF rank > 16) */
+};
+
+enum { XT_F64 = 0, XT_F32 = 1, XT_BF16 = 2 };
+
+typedef struct xt_tt_s* xt_tt;      /* TTTensor / TTOperator resident on the device */
+typedef struct xt_meas_s* xt_meas;  /* SinglePointMeasurementSet */
+
+/* ---------------------------------------------------------------- runtime */
+const char* xt_last_error(void);
+const char* xt_version(void);
+int xt_synchronize(void);
+int xt_release_cache(void);
+/* Launch profiler (no counterpart in xerus; replaces its XERUS_PA_START/END timers, misc/performanceAnalysis.h:33-56):
+ * while enabled, GEMM and ADF launches record HIP events on their own stream. xt_profile_read synchronises
+ * the device, returns per-kernel (tag, launches, total ms, algorithmic flops) and clears the records. */
+int xt_profile_enable(int on);
+int xt_profile_read(size_t max_records, char* tags, size_t tag_len, size_t* launches, double* ms, double* flops, size_t* count);
+/* misc::randomEngine.seed(s) (src/xerus/misc/random.cpp:29-30); also resets the shared normal distribution */
+int xt_seed(uint64_t seed);
+int xt_normal_draws(size_t n, double* out);                 /* defaultNormalDistribution(randomEngine) */
+int xt_uniform_draws(size_t n, size_t hi, size_t* out);     /* uniform_int_distribution<size_t>(0, hi) */
+
+/* ------------------------------------------- boundary 1: blasWrapper and Tensor-level ops (device FP64)
+ * Replaces xerus::blasWrapper (include/xerus/blasLapackWrapper.h:37-146) under xerus::contract /
+ * calculate_* (include/xerus/tensor.h:948-1016). Pointers are device pointers. */
+
+/* C = alpha·op(A)·op(B) (β = 0): blasWrapper::matrix_matrix_product, blasLapackWrapper.h:61-85 */
+int xt_gemm(int transA, int transB, size_t m, size_t n, size_t k, double alpha, const double* A, const double* B, double* C,
+            void* stream);
+/* the same with FP32 operands (FP32 MFMA) and with FP32 storage / BF16 products / FP32 accumulation */
+int xt_gemm_f32(int transA, int transB, size_t m, size_t n, size_t k, float alpha, const float* A, const float* B, float* C,
+                void* stream);
+int xt_gemm_bf16(int transA, int transB, size_t m, size_t n, size_t k, float alpha, const float* A, const float* B, float* C,
+                 void* stream);
+/* Q (m×min(m,n)), R (min×n): blasWrapper::qr, blasLapackWrapper.h:109-112 */
+int xt_qr(const double* A, size_t m, size_t n, double* Q, double* R);
+/* R (m×min), Q (min×n): blasWrapper::rq, blasLapackWrapper.h:118-121 */
+int xt_rq(const double* A, size_t m, size_t n, double* R, double* Q);
+/* rank-revealing QR; Q packed m×rank, C packed rank×n (buffers sized for min(m,n)): blasWrapper::qc,
+ * blasLapackWrapper.h:97-100, rank rule of blasLapackWrapper.cpp:268-272 */
+int xt_qc(const double* A, size_t m, size_t n, double* Q, double* C, size_t* rank);
+/* C packed m×rank, Q packed rank×n: blasWrapper::cq, blasLapackWrapper.h:103-106 */
+int xt_cq(const double* A, size_t m, size_t n, double* C, double* Q, size_t* rank);
+/* thin SVD, S descending (device, min(m,n)): blasWrapper::svd, blasLapackWrapper.h:88-91 */
+int xt_svd(const double* A, size_t m, size_t n, double* U, double* S, double* Vt);
+/* xerus::reshuffle (indexedTensor_tensor_evaluate.cpp:55-137): shuffle[i] = new position of mode i */
+int xt_reshuffle(size_t degree, const size_t* dims, const size_t* shuffle, const double* in, double* out);
+/* blasWrapper::dot_product (blasLapackWrapper.h:55-58); result to host */
+int xt_dot(const double* x, const double* y, size_t n, double* result);
+
+/* ------------------------------------------------------ boundary 2: TTTensor / TTOperator (ttNetwork.h) */
+/* TTNetwork(dims): rank-1 zero TT, canonical at 0 (ttNetwork.cpp:57-108) */
+int xt_tt_new(int is_operator, size_t ndims, const size_t* dims, xt_tt* out);
+/* TTNetwork::random(dims, ranks) with the shared engine (ttNetwork.h:128-155) */
+int xt_tt_random(int is_operator, size_t ndims, const size_t* dims, const size_t* ranks, xt_tt* out);
+int xt_tt_copy(xt_tt src, xt_tt* out);
+int xt_tt_free(xt_tt t);
+int xt_tt_order(xt_tt t, size_t* order);
+int xt_tt_is_operator(xt_tt t, int* is_operator);
+int xt_tt_dims(xt_tt t, size_t* dims);                       /* order (or 2·order) entries */
+int xt_tt_ranks(xt_tt t, size_t* ranks);                     /* TTNetwork::ranks, ttNetwork.cpp:716-731 */
+int xt_tt_component_dims(xt_tt t, size_t k, size_t* dims);   /* 3 (or 4) entries */
+int xt_tt_get_component(xt_tt t, size_t k, double* host_out);  /* TTNetwork::get_component, ttNetwork.cpp:466-470 */
+int xt_tt_set_component(xt_tt t, size_t k, size_t ndim, const size_t* dims, const double* host_data); /* :475-492 */
+int xt_tt_component_ptr(xt_tt t, size_t k, double** device_ptr);  /* valid until the TT changes shape */
+int xt_tt_canonicalization(xt_tt t, int* canonicalized, size_t* core_position);
+int xt_tt_assume_core_position(xt_tt t, size_t pos);          /* ttNetwork.cpp:734-738 */
+int xt_tt_move_core(xt_tt t, size_t pos, int keep_rank);      /* TTNetwork::move_core, ttNetwork.cpp:581-628 */
+int xt_tt_round(xt_tt t, const size_t* max_ranks, double eps);  /* TTNetwork::round, ttNetwork.cpp:643-665 */
+int xt_tt_frob_norm(xt_tt t, double* out);                    /* ttNetwork.cpp:781-789 */
+int xt_tt_dot(xt_tt a, xt_tt b, double* out);
+int xt_tt_add(xt_tt a, xt_tt b, xt_tt* out);                  /* a + b, ttNetwork.cpp:796-847 */
+int xt_tt_scale(xt_tt t, double alpha);
+int xt_tt_apply(xt_tt A, xt_tt x, xt_tt* out);                /* A·x as TT (TTStack), ttStack.cpp:197-310 */
+int xt_tt_full(xt_tt t, double* host_out);                    /* dense tensor (small TTs only) */
+/* ‖A·x − b‖/‖b‖ from the TT of A·x − b (the SPD residual_f, als.cpp:483-486) */
+int xt_tt_residual(xt_tt A, xt_tt x, xt_tt b, double* out);
+
+/* ------------------------------------------------------------ ALSVariant (als.h:37-223, als.cpp:708-778) */
+typedef struct {
+	unsigned sites;                 /* 1: ALS / ALS_SPD, 2: DMRG / DMRG_SPD */
+	size_t numHalfSweeps;           /* 0 = until convergence */
+	double convergenceEpsilon;      /* on the energy, default 1e-6 */
+	int useResidualForEndCriterion;
+	int preserveCorePosition;
+	int assumeSPD;
+	int precision;                  /* XT_F64, XT_F32 or XT_BF16 local matvecs (FP64 refinement for the latter two) */
+	double localTolerance;          /* relative residual of each local solve */
+	int maxLocalIterations;
+	double innerTolerance;          /* mixed precision inner CG tolerance (0 = default) */
+	int maxRefinements;
+} xt_als_options;
+typedef struct {
+	size_t halfSweeps, localSolves, cgIterations, refinements, matvecs64, matvecsLow;
+	double flopsMatvec, flopsEnv;   /* algorithmic flops: 2·a·n·c·(a·p + n·p·q + c·q) per matvec / stack push */
+} xt_als_stats;
+/* the presets ALS, ALS_SPD, DMRG, DMRG_SPD (als.cpp:781-788): sites ∈ {1,2}, assumeSPD ∈ {0,1} */
+int xt_als_default_options(xt_als_options* o, unsigned sites, int assumeSPD);
+/* ALSVariant::operator()(A, x, b, …) (als.h:152-207); A may be NULL (x ≈ b). Returns the energy. */
+int xt_als_solve(const xt_als_options* o, xt_tt A, xt_tt x, xt_tt b, double* energy, xt_als_stats* stats);
+
+/* ------------------------------------------- SinglePointMeasurementSet (measurments.h:43-91) and ADF */
+int xt_meas_random(size_t P, size_t order, const size_t* dims, xt_meas* out);  /* measurments.cpp:222-248 */
+int xt_meas_new(size_t P, size_t order, const uint32_t* positions, const double* values, xt_meas* out);
+int xt_meas_free(xt_meas m);
+int xt_meas_size(xt_meas m, size_t* P);
+int xt_meas_positions(xt_meas m, uint32_t* host_out);
+int xt_meas_positions_ptr(xt_meas m, const uint32_t** device_ptr);
+int xt_meas_values(xt_meas m, double* host_out);
+int xt_meas_set_values(xt_meas m, const double* host_values);
+int xt_meas_measure(xt_meas m, xt_tt x);                                       /* measurments.cpp:125-153 */
+/* xerus's projected gradient Δ_k = Σ (b_i − x_i)·∂x_i/∂G_k for all k (adf.cpp:314-396), concatenated cores */
+int xt_adf_gradient(xt_tt x, xt_meas m, int precision, double* host_grad, double* residual_norm);
+/* ADFVariant::operator() (adf.h:248-266); max_ranks NULL = x's ranks */
+int xt_adf_solve(xt_tt x, xt_meas m, const size_t* max_ranks, size_t max_iterations, double target_residual,
+                 double min_decrease, int precision, double* residual, size_t* iterations);
+
+/* Raw sampled-entry kernels on caller-owned device buffers (FP32 cores, all cores concatenated in TT order;
+ * dims n[d] and ranks r[d+1] given on the host; positions P×d uint32 on the device).
+ * values[i] = x(position_i); grads = Σ_i seed_i·∂x_i/∂G (the adjoint of the sampling). */
+int xt_tt_sample_f32(int d, const int* n, const int* r, const float* cores, const uint32_t* pos, int P, float* values,
+                     void* stream);
+int xt_tt_sample_adjoint_f32(int d, const int* n, const int* r, const float* cores, const uint32_t* pos, int P,
+                             const float* seed, float* grads, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XTT_H */

@@ -1,0 +1,71 @@
+"""CPU checks of the C ABI: libxtt.so loads, exports every function declared in include/xtt.h, and its
+host-side pieces (argument validation, option defaults, the shared random engine) behave like xerus's —
+none of these touch the GPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from helpers import SEED
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "xtt.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(xt_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_symbols_exported(xe):
+    from xerus_amd import _lib
+    names = declared_functions()
+    assert len(names) >= 50
+    L = xe.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(_lib.SIGNATURES), "ctypes signatures out of sync with include/xtt.h"
+
+
+def test_null_arguments_rejected(xe):
+    L = xe.lib()
+    assert L.xt_tt_new(0, 3, None, None) == xe._lib.XT_ERR_INVALID
+    assert b"must not be NULL" in L.xt_last_error()
+    assert L.xt_tt_order(None, None) == xe._lib.XT_ERR_INVALID
+    assert L.xt_als_default_options(None, 1, 1) == xe._lib.XT_ERR_INVALID
+
+
+def test_als_presets_match_reference(xe):
+    """ALS/ALS_SPD sites 1, DMRG/DMRG_SPD sites 2, convergenceEpsilon 1e-6, preserveCorePosition true
+    (als.h:134-141, als.cpp:781-788); ADF = ADFVariant(0, 1e-8, 0.999) (adf.cpp:610)."""
+    assert (xe.ALS.sites, xe.ALS.assumeSPD) == (1, False)
+    assert (xe.ALS_SPD.sites, xe.ALS_SPD.assumeSPD) == (1, True)
+    assert (xe.DMRG.sites, xe.DMRG.assumeSPD) == (2, False)
+    assert (xe.DMRG_SPD.sites, xe.DMRG_SPD.assumeSPD) == (2, True)
+    for v in (xe.ALS, xe.ALS_SPD, xe.DMRG, xe.DMRG_SPD):
+        assert v.convergenceEpsilon == 1e-6 and v.preserveCorePosition and v.numHalfSweeps == 0
+        assert not v.useResidualForEndCriterion
+    assert (xe.ADF.maxIterations, xe.ADF.targetResidualNorm, xe.ADF.minimalResidualNormDecrease) == (0, 1e-8, 0.999)
+
+
+def test_engine_matches_oracle(xe, oracle):
+    """libxtt's host engine and the oracle draw identical streams (same mt19937_64 and libstdc++ distributions)."""
+    xe.seed(SEED)
+    oracle.seed(SEED)
+    assert np.array_equal(xe.normal_draws(500), oracle.normal_draws(500))
+    assert np.array_equal(xe.uniform_draws(500, 15), oracle.uniform_draws(500, 15))
+
+
+def test_reduce_to_maximal_ranks(xe):
+    f = xe.TTTensor.reduce_to_maximal_ranks
+    assert f([8] * 7, [4] * 8) == [4, 8, 8, 8, 8, 8, 4]
+    assert f([64] * 15, [16] * 16) == [16] + [64] * 13 + [16]
+    assert f([100, 100], [2, 3, 4]) == [2, 4]
+
+
+def test_gpus_flag_refuses_multi_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
+    assert r.returncode == 2 and "one GPU" in r.stderr

@@ -1,0 +1,169 @@
+"""GPU parity of the boundary-1 kernels (xerus::blasWrapper replacement) against the oracle and closed forms.
+
+GEMM: every transpose combination, ragged shapes and large K against float64 numpy (FP64 bit-close, FP32
+and BF16 to their stated tolerances). QR/RQ/QC/CQ/SVD: reconstruction, orthogonality, triangularity and the
+integer rank decisions of xerus's QC rule (blasLapackWrapper.cpp:268-272), compared with the oracle.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
+
+
+def ptr(t):
+    return t.data_ptr()
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 5), (64, 64, 64), (192, 1024, 64), (1024, 64, 192), (300, 130, 700)])
+def test_gemm_f64(xe, ta, tb, m, n, k):
+    from xerus_amd._lib import call
+    print(f"gemm_f64 ta={ta} tb={tb} m={m} n={n} k={k}", flush=True)
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    A = rng.standard_normal((k, m) if ta else (m, k))
+    B = rng.standard_normal((n, k) if tb else (k, n))
+    ref = 0.5 * (A.T if ta else A) @ (B.T if tb else B)
+    dA, dB = dev(A), dev(B)
+    dC = torch.empty((m, n), dtype=torch.float64, device=DEV)
+    call("xt_gemm", ta, tb, m, n, k, 0.5, ptr(dA), ptr(dB), ptr(dC), None)
+    err = np.abs(dC.cpu().numpy() - ref).max() / max(1.0, np.abs(ref).max())
+    assert err < 1e-14 * max(1, k) ** 0.5 * 10
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 2e-6), ("bf16", 2e-2)])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1)])
+@pytest.mark.parametrize("m,n,k", [(48, 64, 48), (1024, 64, 192), (33, 17, 100)])
+def test_gemm_lowp(xe, prec, tol, ta, tb, m, n, k):
+    from xerus_amd._lib import call
+    print(f"gemm_{prec} ta={ta} tb={tb} m={m} n={n} k={k}", flush=True)
+    rng = np.random.default_rng(k)
+    A = rng.standard_normal((k, m) if ta else (m, k)).astype(np.float32)
+    B = rng.standard_normal((n, k) if tb else (k, n)).astype(np.float32)
+    ref = (A.T if ta else A).astype(np.float64) @ (B.T if tb else B).astype(np.float64)
+    dA, dB = dev(A, torch.float32), dev(B, torch.float32)
+    dC = torch.empty((m, n), dtype=torch.float32, device=DEV)
+    call("xt_gemm_f32" if prec == "f32" else "xt_gemm_bf16", ta, tb, m, n, k, 1.0, ptr(dA), ptr(dB), ptr(dC), None)
+    # relative to the scale of a length-k dot of unit normals
+    err = np.abs(dC.cpu().numpy() - ref).max() / np.sqrt(k)
+    assert err < tol
+
+
+def _qr_checks(Q, R, A, tol=1e-13):
+    k = min(A.shape)
+    assert np.abs(Q @ R - A).max() < tol * max(1, np.abs(A).max()) * A.shape[1]
+    assert np.abs(Q.T @ Q - np.eye(k)).max() < tol * 10
+
+
+@pytest.mark.parametrize("m,n", [(32, 8), (8, 32), (1024, 64), (64, 1024), (7, 7), (1, 5), (5, 1)])
+def test_qr_rq(xe, oracle, m, n):
+    from xerus_amd._lib import call
+    print(f"qr/rq m={m} n={n}", flush=True)
+    rng = np.random.default_rng(m + n)
+    A = rng.standard_normal((m, n))
+    k = min(m, n)
+    dA = dev(A)
+    dQ, dR = torch.empty((m, k), dtype=torch.float64, device=DEV), torch.empty((k, n), dtype=torch.float64, device=DEV)
+    call("xt_qr", ptr(dA), m, n, ptr(dQ), ptr(dR))
+    Q, R = dQ.cpu().numpy(), dR.cpu().numpy()
+    _qr_checks(Q, R, A)
+    assert np.abs(np.tril(R, -1)).max() == 0
+    Qo, Ro = oracle.qr(A)
+    # same Householder convention ⇒ the same factors up to rounding
+    assert np.abs(np.abs(R) - np.abs(Ro)).max() < 1e-10 * np.abs(Ro).max()
+    dR2, dQ2 = torch.empty((m, k), dtype=torch.float64, device=DEV), torch.empty((k, n), dtype=torch.float64, device=DEV)
+    call("xt_rq", ptr(dA), m, n, ptr(dR2), ptr(dQ2))
+    R2, Q2 = dR2.cpu().numpy(), dQ2.cpu().numpy()
+    assert np.abs(R2 @ Q2 - A).max() < 1e-12 * n
+    assert np.abs(Q2 @ Q2.T - np.eye(k)).max() < 1e-12
+    Ro2, Qo2 = oracle.rq(A)
+    assert np.abs(np.abs(R2) - np.abs(Ro2)).max() < 1e-10 * np.abs(Ro2).max()
+
+
+@pytest.mark.parametrize("m,n,r", [(40, 12, 12), (40, 12, 5), (12, 40, 3), (1024, 64, 64), (1024, 64, 20), (64, 64, 1)])
+def test_qc_cq_rank(xe, oracle, m, n, r):
+    """Rank decisions must equal the oracle's (xerus's 16·ε·R_00 rule with LAPACK signs)."""
+    from xerus_amd._lib import call
+    print(f"qc/cq m={m} n={n} r={r}", flush=True)
+    rng = np.random.default_rng(m * n + r)
+    A = rng.standard_normal((m, r)) @ rng.standard_normal((r, n))
+    k = min(m, n)
+    dA = dev(A)
+    dQ, dC = torch.zeros(m * k, dtype=torch.float64, device=DEV), torch.zeros(k * n, dtype=torch.float64, device=DEV)
+    rank = __import__("ctypes").c_size_t()
+    call("xt_qc", ptr(dA), m, n, ptr(dQ), ptr(dC), __import__("ctypes").byref(rank))
+    rk = rank.value
+    Qo, Co, ro = oracle.qc(A)
+    assert rk == ro
+    Q = dQ.cpu().numpy()[: m * rk].reshape(m, rk)
+    Cm = dC.cpu().numpy()[: rk * n].reshape(rk, n)
+    assert np.abs(Q @ Cm - A).max() < 1e-11 * np.abs(A).max()
+    assert np.abs(Q.T @ Q - np.eye(rk)).max() < 1e-12
+    dC2, dQ2 = torch.zeros(m * k, dtype=torch.float64, device=DEV), torch.zeros(k * n, dtype=torch.float64, device=DEV)
+    call("xt_cq", ptr(dA), m, n, ptr(dC2), ptr(dQ2), __import__("ctypes").byref(rank))
+    Co2, Qo2, ro2 = oracle.cq(A)
+    assert rank.value == ro2
+    rk = rank.value
+    C2 = dC2.cpu().numpy()[: m * rk].reshape(m, rk)
+    Q2 = dQ2.cpu().numpy()[: rk * n].reshape(rk, n)
+    assert np.abs(C2 @ Q2 - A).max() < 1e-11 * np.abs(A).max()
+    assert np.abs(Q2 @ Q2.T - np.eye(rk)).max() < 1e-12
+
+
+@pytest.mark.parametrize("m,n", [(64, 64), (100, 30), (30, 100), (1, 1), (9, 4)])
+def test_svd(xe, oracle, m, n):
+    from xerus_amd._lib import call
+    print(f"svd m={m} n={n}", flush=True)
+    rng = np.random.default_rng(m * 31 + n)
+    # graded spectrum down to 1e-14 so that the small singular values are resolved
+    k = min(m, n)
+    U0, _ = np.linalg.qr(rng.standard_normal((m, k)))
+    V0, _ = np.linalg.qr(rng.standard_normal((n, k)))
+    s0 = np.logspace(0, -14, k)
+    A = (U0 * s0) @ V0.T
+    dA = dev(A)
+    dU, dS, dV = (torch.zeros(s, dtype=torch.float64, device=DEV) for s in ((m, k), (k,), (k, n)))
+    call("xt_svd", ptr(dA), m, n, ptr(dU), ptr(dS), ptr(dV))
+    U, S, Vt = dU.cpu().numpy(), dS.cpu().numpy(), dV.cpu().numpy()
+    assert np.all(np.diff(S) <= 0)
+    _, So, _ = oracle.svd(A)
+    # one-sided Jacobi: small singular values to high relative accuracy
+    assert np.all(np.abs(S - So) <= 1e-13 * S[0] + 1e-6 * So)
+    assert np.abs((U * S) @ Vt - A).max() < 1e-13
+    assert np.abs(U.T @ U - np.eye(k)).max() < 1e-10
+    assert np.abs(Vt @ Vt.T - np.eye(k)).max() < 1e-10
+
+
+def test_svd_closed_forms(xe):
+    """SVD of the identity and of the zero matrix (the reference's Tensor:SVD_Identity / SVD_zero cases)."""
+    from xerus_amd._lib import call
+    for A in (np.eye(6), np.zeros((5, 3))):
+        m, n = A.shape
+        k = min(m, n)
+        dA = dev(A)
+        dU, dS, dV = (torch.zeros(s, dtype=torch.float64, device=DEV) for s in ((m, k), (k,), (k, n)))
+        call("xt_svd", ptr(dA), m, n, ptr(dU), ptr(dS), ptr(dV))
+        S = dS.cpu().numpy()
+        assert np.allclose(S, np.linalg.svd(A, compute_uv=False), atol=1e-15)
+
+
+def test_reshuffle(xe, oracle):
+    from xerus_amd._lib import call
+    import ctypes as C
+    rng = np.random.default_rng(3)
+    T = rng.standard_normal((2, 3, 4, 5))
+    for sh in ([0, 1, 2, 3], [3, 2, 1, 0], [1, 3, 0, 2], [2, 0, 3, 1]):
+        dims = np.array(T.shape, dtype=np.uintp)
+        shuf = np.array(sh, dtype=np.uintp)
+        out = torch.empty(T.size, dtype=torch.float64, device=DEV)
+        call("xt_reshuffle", 4, dims.ctypes.data_as(C.POINTER(C.c_size_t)), shuf.ctypes.data_as(C.POINTER(C.c_size_t)),
+             ptr(dev(T)), ptr(out))
+        ref = oracle.reshuffle(T, sh)
+        assert np.array_equal(out.cpu().numpy(), ref.ravel())  # data movement: bit-exact

@@ -1,0 +1,246 @@
+"""GPU parity of the TT path (TTTensor/TTOperator, move_core, round, ALS/DMRG, ADF) against the oracle.
+
+Integer results (ranks, core positions, measurement index sets, RNG draws) must be bit-exact; floating point
+results are compared through gauge-invariant quantities (full tensors, norms, energies, residuals) at the
+tolerance written in each test.
+"""
+import numpy as np
+import pytest
+
+from helpers import SEED, apply_dense_laplace, exact_laplace_solve, laplace_cores
+
+pytestmark = pytest.mark.gpu
+
+
+def twin_random(xe, oracle, dims, ranks, op=False, seed=SEED):
+    xe.seed(seed)
+    oracle.seed(seed)
+    cls = xe.TTOperator if op else xe.TTTensor
+    return cls.random(dims, ranks), oracle.TT.random(dims, ranks, op=op)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_rng_streams_match(xe, oracle):
+    print("rng draws", flush=True)
+    xe.seed(SEED); oracle.seed(SEED)
+    assert np.array_equal(xe.normal_draws(1000), oracle.normal_draws(1000))
+    assert np.array_equal(xe.uniform_draws(1000, 9), oracle.uniform_draws(1000, 9))
+
+
+@pytest.mark.parametrize("dims,ranks", [([4] * 8, [8] * 7), ([16] * 16, [64] * 15), ([2, 3, 4, 5], [3, 10, 2]), ([5], [])])
+def test_random_ranks_and_values(xe, oracle, dims, ranks):
+    print(f"random {dims} {ranks}", flush=True)
+    x, xo = twin_random(xe, oracle, dims, ranks)
+    assert x.ranks() == xo.ranks()
+    assert x.canonicalized and x.corePosition == 0
+    if np.prod(dims) <= 65536:
+        assert rel(x.full(), xo.full()) < 1e-12
+    else:
+        assert abs(x.frob_norm() - xo.frob_norm()) < 1e-10 * xo.frob_norm()
+
+
+@pytest.mark.parametrize("keep", [True, False])
+def test_move_core(xe, oracle, keep):
+    print(f"move_core keep={keep}", flush=True)
+    x, xo = twin_random(xe, oracle, [3, 4, 5, 4, 3], [3, 7, 7, 3])
+    ref = xo.full()
+    for pos in (4, 1, 3, 0, 2):
+        x.move_core(pos, keep)
+        xo.move_core(pos, keep)
+        assert x.ranks() == xo.ranks()
+        assert x.corePosition == pos and x.canonicalized
+        assert rel(x.full(), ref) < 1e-13
+        for k in range(x.order()):  # left- / right-orthogonality of the non-core components
+            c = x.get_component(k)
+            if k < pos:
+                M = c.reshape(-1, c.shape[-1]); assert np.abs(M.T @ M - np.eye(M.shape[1])).max() < 1e-12
+            elif k > pos:
+                M = c.reshape(c.shape[0], -1); assert np.abs(M @ M.T - np.eye(M.shape[0])).max() < 1e-12
+
+
+def test_round_eps_and_maxrank(xe, oracle):
+    print("round", flush=True)
+    # a TT whose sum with itself has exactly half the rank
+    a, ao = twin_random(xe, oracle, [4, 4, 4, 4, 4], [4, 8, 8, 4])
+    s, so = a + a, ao + ao
+    for t, to in ((s, so),):
+        t.round(1e-12)
+        to.round(eps=1e-12)
+        assert t.ranks() == to.ranks()
+        assert rel(t.full(), to.full()) < 1e-12
+    b, bo = twin_random(xe, oracle, [4, 4, 4, 4, 4], [4, 16, 16, 4])
+    b.round(5)
+    bo.round(5)
+    assert b.ranks() == bo.ranks() == [4, 5, 5, 4]
+    assert rel(b.full(), bo.full()) < 1e-10
+
+
+def test_tt_sum_and_difference(xe, oracle):
+    """TT:real_difference pins (ttArithmetic.cxx:74-115): sum ranks {8,8,8,8}; a − a ≈ 0."""
+    print("tt sum", flush=True)
+    a, _ = twin_random(xe, oracle, [10] * 5, [4] * 4)
+    b, _ = twin_random(xe, oracle, [10] * 5, [4] * 4, seed=SEED + 1)
+    c = a + b
+    assert c.ranks() == [8, 8, 8, 8]
+    assert (a - a).frob_norm() < 1e-11
+    assert ((a + b) - (b + a)).frob_norm() < 1e-11
+    assert ((73 * a + b) - (b + 73 * a)).frob_norm() < 1e-9
+
+
+def test_operator_apply_and_residual(xe, oracle):
+    print("apply/residual", flush=True)
+    d, n = 5, 4
+    rng = np.random.default_rng(0)
+    pots = [rng.random(n) for _ in range(d)]
+    cores = laplace_cores(d, n, pots)
+    A, Ao = xe.TTOperator.from_components(cores), oracle.TT.from_cores(cores, op=True)
+    x, xo = twin_random(xe, oracle, [n] * d, [3] * (d - 1))
+    y = A(x)
+    ref = apply_dense_laplace(xo.full(), d, n, pots)
+    assert rel(y.full(), ref) < 1e-13
+    b, bo = twin_random(xe, oracle, [n] * d, [2] * (d - 1), seed=SEED + 5)
+    r = xe.residual(A, x, b)
+    ro = oracle.residual(Ao, xo, bo)
+    dense = np.linalg.norm(ref - bo.full()) / np.linalg.norm(bo.full())
+    assert abs(r - ro) < 1e-12 * ro and abs(r - dense) < 1e-12 * dense
+
+
+def config1(xe, oracle, d=8, n=4):
+    """d=8, n=4 Laplacian (rank 2), b random rank 2 normalised, x0 random rank 8 — both engines, same draws."""
+    cores = laplace_cores(d, n)
+    A, Ao = xe.TTOperator.from_components(cores), oracle.TT.from_cores(cores, op=True)
+    xe.seed(SEED); oracle.seed(SEED)
+    b, bo = xe.TTTensor.random([n] * d, [2] * (d - 1)), oracle.TT.random([n] * d, [2] * (d - 1))
+    nb = bo.frob_norm()
+    b = b / nb
+    bo.scale(1.0 / nb)
+    x, xo = xe.TTTensor.random([n] * d, [8] * (d - 1)), oracle.TT.random([n] * d, [8] * (d - 1))
+    return A, Ao, b, bo, x, xo
+
+
+def test_als_spd_config1(xe, oracle):
+    """ALS_SPD, 5 sweeps (numHalfSweeps=10, eps=0): energy, residual, solution and ranks vs the oracle's
+    dense-Cholesky ALS; the residual also checked against the exact (fast-diagonalisation) solution."""
+    print("als_spd config1", flush=True)
+    A, Ao, b, bo, x, xo = config1(xe, oracle)
+    assert x.ranks() == xo.ranks() == [4, 8, 8, 8, 8, 8, 4]
+    als = xe.ALS_SPD.copy()
+    als.convergenceEpsilon = 0.0
+    als.localTolerance = 1e-14
+    e = als(A, x, b, 10)
+    eo, st = oracle.als(Ao, xo, bo, num_half_sweeps=10, eps=0.0)
+    assert als.last_stats["halfSweeps"] == st["half_sweeps"] == 10
+    assert als.last_stats["localSolves"] == st["local_solves"] == 51
+    assert abs(e - eo) < 1e-12 * abs(eo)
+    assert x.ranks() == xo.ranks() and x.corePosition == xo.state()[1]
+    assert rel(x.full(), xo.full()) < 1e-9
+    r, ro = xe.residual(A, x, b), oracle.residual(Ao, xo, bo)
+    assert abs(r - ro) < 1e-6 * ro
+    xs = exact_laplace_solve(bo.full(), 8, 4)
+    dense = np.linalg.norm(apply_dense_laplace(x.full(), 8, 4) - bo.full()) / np.linalg.norm(bo.full())
+    assert abs(dense - r) < 1e-9 * dense
+    assert rel(x.full(), xs) < 1e-4  # rank-8 ALS is within its approximation error of the exact solution
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 1e-9), ("bf16", 1e-9)])
+def test_als_mixed_precision_config3(xe, oracle, prec, tol):
+    """Config-1 seed and sizes with FP32 / BF16 local matvecs and FP64 refinement: same fixed point as FP64."""
+    print(f"als mixed {prec}", flush=True)
+    A, Ao, b, bo, x, xo = config1(xe, oracle)
+    als = xe.ALS_SPD.copy()
+    als.convergenceEpsilon = 0.0
+    als.localTolerance = 1e-13
+    als.precision = xe.XT_F32 if prec == "f32" else xe.XT_BF16
+    als(A, x, b, 10)
+    oracle.als(Ao, xo, bo, num_half_sweeps=10, eps=0.0)
+    err = rel(x.full(), xo.full())
+    print(f"  ‖x − x_oracle‖/‖x_oracle‖ = {err:.3e}, refinements {als.last_stats['refinements']}", flush=True)
+    assert err < tol
+
+
+def test_als_identity(xe, oracle):
+    """ALS:identity (als.cxx:28-67): with A = I the solution is b."""
+    print("als identity", flush=True)
+    d, n = 3, 10
+    I = [np.eye(n).reshape(1, n, n, 1)] * d
+    A = xe.TTOperator.from_components(I)
+    xe.seed(SEED)
+    b = xe.TTTensor.random([n] * d, [4, 4])
+    x = xe.TTTensor.random([n] * d, [4, 4])
+    als = xe.ALS_SPD.copy()
+    als.useResidualForEndCriterion = True
+    res = als(A, x, b, 1e-4)
+    assert res < 0.01
+    assert (x - b).frob_norm() < 1e-9
+
+
+@pytest.mark.parametrize("variant", ["DMRG_SPD", "ALS", "DMRG"])
+def test_other_variants(xe, oracle, variant):
+    print(f"variant {variant}", flush=True)
+    d, n = 5, 3
+    rng = np.random.default_rng(7)
+    pots = [rng.random(n) for _ in range(d)]
+    cores = laplace_cores(d, n, pots)
+    A, Ao = xe.TTOperator.from_components(cores), oracle.TT.from_cores(cores, op=True)
+    xe.seed(SEED); oracle.seed(SEED)
+    b, bo = xe.TTTensor.random([n] * d, [2] * (d - 1)), oracle.TT.random([n] * d, [2] * (d - 1))
+    x, xo = xe.TTTensor.random([n] * d, [4] * (d - 1)), oracle.TT.random([n] * d, [4] * (d - 1))
+    als = getattr(xe, variant).copy()
+    als.convergenceEpsilon = 0.0
+    als.localTolerance = 1e-14
+    e = als(A, x, b, 4)
+    eo, st = oracle.als(Ao, xo, bo, sites=als.sites, spd=als.assumeSPD, num_half_sweeps=4, eps=0.0)
+    assert x.ranks() == xo.ranks()
+    assert abs(e - eo) < 1e-8 * max(abs(eo), 1e-12)
+    assert rel(x.full(), xo.full()) < 1e-7
+
+
+def test_measurement_sets_bit_exact(xe, oracle):
+    print("measurements", flush=True)
+    dims = [10] * 10
+    xe.seed(SEED); oracle.seed(SEED)
+    m = xe.SinglePointMeasurementSet.random(20000, dims)
+    mo = oracle.Measurements.random(20000, dims)
+    assert np.array_equal(m.positions(10), mo.positions(10))
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_adf_gradient(xe, oracle, prec):
+    """The adjoint kernel's full gradient vs the oracle's xerus projected gradient (adf.cpp:314-396)."""
+    print(f"adf gradient {prec}", flush=True)
+    d, n, r = 6, 5, 4
+    truth, trutho = twin_random(xe, oracle, [n] * d, [r] * (d - 1))
+    x, xo = twin_random(xe, oracle, [n] * d, [r] * (d - 1), seed=SEED + 9)
+    xe.seed(SEED); oracle.seed(SEED)
+    m = xe.SinglePointMeasurementSet.random(3000, [n] * d)
+    mo = oracle.Measurements.random(3000, [n] * d)
+    m.measure(truth)
+    mo.measure(trutho)
+    assert np.abs(m.measuredValues - mo.values()).max() < 1e-12
+    g, rn = xe.adf_gradient(x, m, xe.XT_F64 if prec == "f64" else xe.XT_F32)
+    go, rno = oracle.adf_gradient(xo, mo)
+    tol = 1e-11 if prec == "f64" else 2e-5
+    assert abs(rn - rno) < tol * rno
+    for k in range(d):
+        assert np.abs(g[k] - go[k]).max() < tol * np.abs(go[k]).max()
+
+
+def test_adf_solve(xe, oracle):
+    print("adf solve", flush=True)
+    d, n, r = 5, 6, 3
+    truth, trutho = twin_random(xe, oracle, [n] * d, [r] * (d - 1))
+    x, xo = twin_random(xe, oracle, [n] * d, [r] * (d - 1), seed=SEED + 3)
+    xe.seed(SEED + 1); oracle.seed(SEED + 1)
+    m = xe.SinglePointMeasurementSet.random(2000, [n] * d)
+    mo = oracle.Measurements.random(2000, [n] * d)
+    m.measure(truth)
+    mo.measure(trutho)
+    adf = xe.ADFVariant(8, 1e-8, 0.999)
+    res = adf(x, m)
+    reso, ito = oracle.adf_solve(xo, mo, max_iterations=8)
+    assert adf.last_iterations == ito
+    assert abs(res - reso) < 1e-6 * reso
+    assert rel(x.full(), xo.full()) < 1e-6

@@ -1,0 +1,166 @@
+"""ctypes binding of libxtt.so (the C ABI declared in include/xtt.h).
+
+The library is built in-tree (``make -C xerus_amd/csrc``); loading fails loudly when it is missing — there is
+no CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxtt.so")
+
+XT_OK, XT_ERR_INVALID, XT_ERR_HIP, XT_ERR_NUMERIC, XT_ERR_NOMEM, XT_ERR_UNSUPPORTED = range(6)
+XT_F64, XT_F32, XT_BF16 = 0, 1, 2
+
+_sz = C.c_size_t
+_ps = C.POINTER(C.c_size_t)
+_pd = C.POINTER(C.c_double)
+_pf = C.POINTER(C.c_float)
+_pi = C.POINTER(C.c_int)
+_pu32 = C.POINTER(C.c_uint32)
+_vp = C.c_void_p
+
+
+class XtALSOptions(C.Structure):
+    _fields_ = [
+        ("sites", C.c_uint),
+        ("numHalfSweeps", C.c_size_t),
+        ("convergenceEpsilon", C.c_double),
+        ("useResidualForEndCriterion", C.c_int),
+        ("preserveCorePosition", C.c_int),
+        ("assumeSPD", C.c_int),
+        ("precision", C.c_int),
+        ("localTolerance", C.c_double),
+        ("maxLocalIterations", C.c_int),
+        ("innerTolerance", C.c_double),
+        ("maxRefinements", C.c_int),
+    ]
+
+
+class XtALSStats(C.Structure):
+    _fields_ = [
+        ("halfSweeps", C.c_size_t),
+        ("localSolves", C.c_size_t),
+        ("cgIterations", C.c_size_t),
+        ("refinements", C.c_size_t),
+        ("matvecs64", C.c_size_t),
+        ("matvecsLow", C.c_size_t),
+        ("flopsMatvec", C.c_double),
+        ("flopsEnv", C.c_double),
+    ]
+
+
+# name → (restype, argtypes); every entry point returns an int status
+SIGNATURES = {
+    "xt_last_error": (C.c_char_p, []),
+    "xt_version": (C.c_char_p, []),
+    "xt_synchronize": (C.c_int, []),
+    "xt_release_cache": (C.c_int, []),
+    "xt_profile_enable": (C.c_int, [C.c_int]),
+    "xt_profile_read": (C.c_int, [_sz, C.c_char_p, _sz, _ps, _pd, _pd, _ps]),
+    "xt_seed": (C.c_int, [C.c_uint64]),
+    "xt_normal_draws": (C.c_int, [_sz, _pd]),
+    "xt_uniform_draws": (C.c_int, [_sz, _sz, _ps]),
+    "xt_gemm": (C.c_int, [C.c_int, C.c_int, _sz, _sz, _sz, C.c_double, _vp, _vp, _vp, _vp]),
+    "xt_gemm_f32": (C.c_int, [C.c_int, C.c_int, _sz, _sz, _sz, C.c_float, _vp, _vp, _vp, _vp]),
+    "xt_gemm_bf16": (C.c_int, [C.c_int, C.c_int, _sz, _sz, _sz, C.c_float, _vp, _vp, _vp, _vp]),
+    "xt_qr": (C.c_int, [_vp, _sz, _sz, _vp, _vp]),
+    "xt_rq": (C.c_int, [_vp, _sz, _sz, _vp, _vp]),
+    "xt_qc": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _ps]),
+    "xt_cq": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _ps]),
+    "xt_svd": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp]),
+    "xt_reshuffle": (C.c_int, [_sz, _ps, _ps, _vp, _vp]),
+    "xt_dot": (C.c_int, [_vp, _vp, _sz, _pd]),
+    "xt_tt_new": (C.c_int, [C.c_int, _sz, _ps, C.POINTER(_vp)]),
+    "xt_tt_random": (C.c_int, [C.c_int, _sz, _ps, _ps, C.POINTER(_vp)]),
+    "xt_tt_copy": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "xt_tt_free": (C.c_int, [_vp]),
+    "xt_tt_order": (C.c_int, [_vp, _ps]),
+    "xt_tt_is_operator": (C.c_int, [_vp, _pi]),
+    "xt_tt_dims": (C.c_int, [_vp, _ps]),
+    "xt_tt_ranks": (C.c_int, [_vp, _ps]),
+    "xt_tt_component_dims": (C.c_int, [_vp, _sz, _ps]),
+    "xt_tt_get_component": (C.c_int, [_vp, _sz, _pd]),
+    "xt_tt_set_component": (C.c_int, [_vp, _sz, _sz, _ps, _pd]),
+    "xt_tt_component_ptr": (C.c_int, [_vp, _sz, C.POINTER(_vp)]),
+    "xt_tt_canonicalization": (C.c_int, [_vp, _pi, _ps]),
+    "xt_tt_assume_core_position": (C.c_int, [_vp, _sz]),
+    "xt_tt_move_core": (C.c_int, [_vp, _sz, C.c_int]),
+    "xt_tt_round": (C.c_int, [_vp, _ps, C.c_double]),
+    "xt_tt_frob_norm": (C.c_int, [_vp, _pd]),
+    "xt_tt_dot": (C.c_int, [_vp, _vp, _pd]),
+    "xt_tt_add": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
+    "xt_tt_scale": (C.c_int, [_vp, C.c_double]),
+    "xt_tt_apply": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
+    "xt_tt_full": (C.c_int, [_vp, _pd]),
+    "xt_tt_residual": (C.c_int, [_vp, _vp, _vp, _pd]),
+    "xt_als_default_options": (C.c_int, [C.POINTER(XtALSOptions), C.c_uint, C.c_int]),
+    "xt_als_solve": (C.c_int, [C.POINTER(XtALSOptions), _vp, _vp, _vp, _pd, C.POINTER(XtALSStats)]),
+    "xt_meas_random": (C.c_int, [_sz, _sz, _ps, C.POINTER(_vp)]),
+    "xt_meas_new": (C.c_int, [_sz, _sz, _pu32, _pd, C.POINTER(_vp)]),
+    "xt_meas_free": (C.c_int, [_vp]),
+    "xt_meas_size": (C.c_int, [_vp, _ps]),
+    "xt_meas_positions": (C.c_int, [_vp, _pu32]),
+    "xt_meas_positions_ptr": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "xt_meas_values": (C.c_int, [_vp, _pd]),
+    "xt_meas_set_values": (C.c_int, [_vp, _pd]),
+    "xt_meas_measure": (C.c_int, [_vp, _vp]),
+    "xt_adf_gradient": (C.c_int, [_vp, _vp, C.c_int, _pd, _pd]),
+    "xt_adf_solve": (C.c_int, [_vp, _vp, _ps, _sz, C.c_double, C.c_double, C.c_int, _pd, _ps]),
+    "xt_tt_sample_f32": (C.c_int, [C.c_int, _pi, _pi, _vp, _vp, C.c_int, _vp, _vp]),
+    "xt_tt_sample_adjoint_f32": (C.c_int, [C.c_int, _pi, _pi, _vp, _vp, C.c_int, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+class XttError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[xtt error {code}] {msg}")
+        self.code = code
+
+
+def lib() -> C.CDLL:
+    """Load libxtt.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libxtt.so not built: {LIB_PATH} missing (run `make -C xerus_amd/csrc`)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != XT_OK:
+        raise XttError(rc, lib().xt_last_error().decode())
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args))
+
+
+def profile_enable(on: bool) -> None:
+    call("xt_profile_enable", int(on))
+
+
+def profile_read(max_records: int = 64):
+    """Per-kernel {tag: (launches, total_ms, algorithmic_flops)} since the last read (synchronises)."""
+    tag_len = 96
+    tags = C.create_string_buffer(max_records * tag_len)
+    launches = (C.c_size_t * max_records)()
+    ms = (C.c_double * max_records)()
+    flops = (C.c_double * max_records)()
+    count = C.c_size_t()
+    check(lib().xt_profile_read(max_records, tags, tag_len, launches, ms, flops, C.byref(count)))
+    out = {}
+    for i in range(min(count.value, max_records)):
+        tag = tags.raw[i * tag_len:(i + 1) * tag_len].split(b"\0", 1)[0].decode()
+        out[tag] = (int(launches[i]), float(ms[i]), float(flops[i]))
+    return out

@@ -1,0 +1,266 @@
+// SinglePointMeasurementSet (src/xerus/measurments.cpp) and ADFVariant (src/xerus/algorithms/adf.cpp) on
+// the device. Positions live in HBM as uint32 (P×d, lexicographically sorted like xerus); TT cores are
+// flattened per step into one buffer of the working precision (FP64 for parity runs, FP32 for the
+// throughput configuration) and every sampled-entry contraction runs in the ADF kernels.
+#include "adf_driver.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <set>
+
+namespace xtt {
+
+Measurements meas_random(size_t P, const dims_t& dims) {  // measurments.cpp:222-248
+	size_t total = 1;
+	bool overflow = false;
+	for (size_t n : dims) {
+		XTT_REQUIRE(n > 0, "dimension 0 is not valid");
+		if (total > SIZE_MAX / n) overflow = true; else total *= n;
+	}
+	XTT_REQUIRE(overflow || total >= P, "It's impossible to perform as many measurements as requested.");
+	std::set<size_t> seen;
+	std::vector<std::vector<size_t>> positions;
+	std::vector<size_t> idx(dims.size());
+	while (positions.size() < P) {
+		size_t pos = 0;
+		for (size_t i = 0; i < dims.size(); ++i) { idx[i] = rng_uniform(dims[i] - 1); pos *= dims[i]; pos += idx[i]; }
+		if (seen.insert(pos).second) positions.push_back(idx);
+	}
+	std::sort(positions.begin(), positions.end());  // measurments.cpp:89-105
+	Measurements m;
+	m.d = dims.size();
+	m.pos.resize(P * m.d);
+	for (size_t i = 0; i < P; ++i) for (size_t k = 0; k < m.d; ++k) m.pos[i * m.d + k] = (uint32_t)positions[i][k];
+	m.values.assign(P, 0.0);
+	m.upload();
+	return m;
+}
+
+void Measurements::check_against(const TT& x) const {
+	XTT_REQUIRE(d == x.order(), "Measurment degree must coincide with x degree.");
+	for (size_t i = 0; i < pos.size(); ++i) XTT_REQUIRE(pos[i] < x.dims[i % d], "measurement position outside the tensor");
+}
+
+void Measurements::upload() {
+	dpos.resize(pos.size());
+	if (!pos.empty()) XTT_HIP(hipMemcpyAsync(dpos.get(), pos.data(), pos.size() * 4, hipMemcpyHostToDevice, stream()));
+	sync();
+}
+
+template <class T>
+void Flat<T>::build(const TT& x) {
+	const size_t d = x.order();
+	std::vector<int> n(d), rank(d + 1);
+	std::vector<long long> off(d);
+	long long o = 0;
+	int maxR = 1;
+	rank[0] = 1;
+	for (size_t k = 0; k < d; ++k) {
+		const DTensor& c = x.cores[k];
+		XTT_REQUIRE(c.dims[0] <= 16 && c.dims[2] <= 16, "ADF: ranks above 16 are not supported");
+		n[k] = (int)c.dims[1];
+		rank[k + 1] = (int)c.dims[2];
+		off[k] = o;
+		o += (long long)c.size();
+		maxR = std::max(maxR, (int)std::max(c.dims[0], c.dims[2]));
+	}
+	if (dn.size() < d) { dn.resize(d); drank.resize(d + 1); doff.resize(d); }
+	if (cores.size() < (size_t)o) cores.resize(o);
+	hn = n; hrank = rank; hoff = off;
+	XTT_HIP(hipMemcpyAsync(dn.get(), n.data(), d * 4, hipMemcpyHostToDevice, stream()));
+	XTT_HIP(hipMemcpyAsync(drank.get(), rank.data(), (d + 1) * 4, hipMemcpyHostToDevice, stream()));
+	XTT_HIP(hipMemcpyAsync(doff.get(), off.data(), d * 8, hipMemcpyHostToDevice, stream()));
+	desc.d = (int)d;
+	desc.n = dn.get();
+	desc.rank = drank.get();
+	desc.coreOff = doff.get();
+	desc.maxR = maxR;
+	desc.gradSize = (int)o;
+	refresh(x);
+	sync();  // the host arrays above are locals
+}
+
+template <class T>
+void Flat<T>::refresh(const TT& x) {
+	for (size_t k = 0; k < x.order(); ++k) {
+		const DTensor& c = x.cores[k];
+		if constexpr (sizeof(T) == 8) {
+			XTT_HIP(hipMemcpyAsync(cores.get() + hoff[k], c.data(), c.size() * 8, hipMemcpyDeviceToDevice, stream()));
+		} else {
+			convert_f64_f32(c.data(), reinterpret_cast<float*>(cores.get()) + hoff[k], c.size(), stream());
+		}
+	}
+}
+
+template struct Flat<double>;
+template struct Flat<float>;
+
+namespace {
+template <class T>
+struct AdfWork {
+	DBuf<T> vals, resid, grad, partials;
+	DBuf<double> norms, dpart, slices;
+	DBuf<unsigned> counter;
+	void ensure(size_t P, size_t gradSize, size_t nslice) {
+		if (resid.size() < P) { resid.resize(P); vals.resize(P); }
+		if (grad.size() < gradSize) grad.resize(gradSize);
+		const size_t pb = (size_t)adf_grad_blocks((int)P) * gradSize;
+		if (partials.size() < pb) partials.resize(pb);
+		if (!norms.get()) { norms.resize(2); counter.resize(1); XTT_HIP(hipMemsetAsync(counter.get(), 0, 4, stream())); }
+		const size_t sp = std::max<size_t>((size_t)adf_slice_blocks((int)P) * std::max<size_t>(nslice, 1), 2 * 1024);
+		if (dpart.size() < sp) dpart.resize(sp);
+		if (slices.size() < nslice) slices.resize(std::max<size_t>(nslice, 1));
+	}
+};
+
+template <class T>
+void upload_values(const Measurements& m, DBuf<T>& v) {
+	std::vector<T> h(m.values.begin(), m.values.end());
+	v.resize(h.size());
+	XTT_HIP(hipMemcpyAsync(v.get(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream()));
+	sync();
+}
+}  // namespace
+
+void meas_measure(Measurements& m, const TT& x) {  // measurments.cpp:125-153 (FP64)
+	XTT_REQUIRE(m.d == x.order(), "Degrees of solution and measurements must match!");
+	m.check_against(x);
+	const size_t P = m.size();
+	if (P == 0) return;
+	Flat<double> f;
+	f.build(x);
+	AdfWork<double> w;
+	w.ensure(P, f.desc.gradSize, 1);
+	XTT_HIP(hipMemsetAsync(w.vals.get(), 0, P * 8, stream()));
+	AdfDesc a = f.desc;
+	a.P = (int)P;
+	adf_residual<double>(a, f.cores.get(), m.dpos.get(), w.vals.get(), w.resid.get(), w.norms.get(), w.dpart.get(), w.counter.get(), stream());
+	XTT_HIP(hipMemcpyAsync(m.values.data(), w.resid.get(), P * 8, hipMemcpyDeviceToHost, stream()));
+	sync();
+}
+
+template <class T>
+std::vector<std::vector<double>> adf_full_gradient(const TT& x, const Measurements& m, double* residualNorm) {
+	XTT_REQUIRE(m.d == x.order() && m.size() > 0, "ADF: measurement set does not match x");
+	m.check_against(x);
+	const size_t P = m.size();
+	Flat<T> f;
+	f.build(x);
+	AdfWork<T> w;
+	w.ensure(P, f.desc.gradSize, 1);
+	upload_values<T>(m, w.vals);
+	AdfDesc a = f.desc;
+	a.P = (int)P;
+	adf_residual<T>(a, f.cores.get(), m.dpos.get(), w.vals.get(), w.resid.get(), w.norms.get(), w.dpart.get(), w.counter.get(), stream());
+	adf_gradient<T>(a, f.cores.get(), m.dpos.get(), w.resid.get(), w.grad.get(), w.partials.get(), -1, stream());
+	std::vector<T> g(f.desc.gradSize);
+	double norms[2];
+	XTT_HIP(hipMemcpyAsync(g.data(), w.grad.get(), g.size() * sizeof(T), hipMemcpyDeviceToHost, stream()));
+	XTT_HIP(hipMemcpyAsync(norms, w.norms.get(), 16, hipMemcpyDeviceToHost, stream()));
+	sync();
+	if (residualNorm) *residualNorm = std::sqrt(norms[0] / norms[1]);
+	std::vector<std::vector<double>> out(x.order());
+	for (size_t k = 0; k < x.order(); ++k) {
+		const size_t sz = x.cores[k].size();
+		out[k].resize(sz);
+		// xerus's projected gradient is Σ (b − x)·∂x = −Σ resid·∂x
+		for (size_t e = 0; e < sz; ++e) out[k][e] = -(double)g[f.hoff[k] + e];
+	}
+	return out;
+}
+template std::vector<std::vector<double>> adf_full_gradient<double>(const TT&, const Measurements&, double*);
+template std::vector<std::vector<double>> adf_full_gradient<float>(const TT&, const Measurements&, double*);
+
+// adf.cpp:489-604 with the stack contractions replaced by the sampled-entry kernels.
+template <class T>
+double adf_solve(const ADFOptions& o, TT& x, const Measurements& m, const dims_t& maxRanksIn, size_t* iterations) {
+	const size_t d = x.order(), P = m.size();
+	XTT_REQUIRE(P > 0, "Need at very least one measurment.");
+	XTT_REQUIRE(m.d == d, "Measurment degree must coincide with x degree.");
+	m.check_against(x);
+	const dims_t maxRanks = reduce_to_maximal_ranks(maxRanksIn, x.dims, false);
+	double normMeasured = 0;
+	for (double v : m.values) normMeasured += v * v;
+	normMeasured = std::sqrt(normMeasured);
+	AdfWork<T> w;
+	upload_values<T>(m, w.vals);
+	Flat<T> f;
+	size_t iteration = 0;
+	double residualNorm = DBL_MAX, lastResidualNorm = DBL_MAX;
+
+	auto residual = [&]() {  // resid = x − b at the current x; returns Σ resid²
+		f.build(x);
+		w.ensure(P, f.desc.gradSize, 64);
+		AdfDesc a = f.desc;
+		a.P = (int)P;
+		adf_residual<T>(a, f.cores.get(), m.dpos.get(), w.vals.get(), w.resid.get(), w.norms.get(), w.dpart.get(), w.counter.get(), stream());
+		double norms[2];
+		XTT_HIP(hipMemcpyAsync(norms, w.norms.get(), 16, hipMemcpyDeviceToHost, stream()));
+		sync();
+		return norms[0];
+	};
+
+	auto solve_with_current_ranks = [&]() {
+		double resDec1 = 0.0, resDec2 = 0.0, resDec3 = 0.0;
+		for (; o.maxIterations == 0 || iteration < o.maxIterations; ++iteration) {
+			move_core(x, 0, true);
+			const double rr = residual();
+			lastResidualNorm = residualNorm;
+			residualNorm = std::sqrt(rr) / normMeasured;
+			const double resDec4 = resDec3;
+			resDec3 = resDec2;
+			resDec2 = resDec1;
+			resDec1 = residualNorm / lastResidualNorm;
+			if (residualNorm < o.targetResidualNorm || resDec1 * resDec2 * resDec3 * resDec4 > std::pow(o.minimalResidualNormDecrease, 4)) break;
+			for (size_t k = 0; k < d; ++k) {
+				if (k > 0) residual();
+				AdfDesc a = f.desc;
+				a.P = (int)P;
+				const DTensor& core = x.cores[k];
+				const size_t rl = core.dims[0], n = core.dims[1], rr2 = core.dims[2], sz = core.size();
+				// gradient of ½Σ(x−b)² w.r.t. core k; xerus's Δ (adf.cpp:359-396) is its negative
+				adf_gradient<T>(a, f.cores.get(), m.dpos.get(), w.resid.get(), w.grad.get(), w.partials.get(), (int)k, stream());
+				adf_slice_norms<T>(a, f.cores.get(), m.dpos.get(), (int)k, w.grad.get() + f.hoff[k], (int)n, w.slices.get(),
+				                   w.dpart.get(), stream());
+				std::vector<T> g(sz);
+				std::vector<double> normA(n);
+				XTT_HIP(hipMemcpyAsync(g.data(), w.grad.get() + f.hoff[k], sz * sizeof(T), hipMemcpyDeviceToHost, stream()));
+				XTT_HIP(hipMemcpyAsync(normA.data(), w.slices.get(), n * 8, hipMemcpyDeviceToHost, stream()));
+				sync();
+				// update_x (adf.cpp:468-478): x_k[:, j, :] += (‖Δ_j‖²/‖P_j A Δ‖²)·Δ_j with Δ = −g
+				std::vector<double> upd(sz, 0.0);
+				for (size_t j = 0; j < n; ++j) {
+					double pyr = 0;
+					for (size_t al = 0; al < rl; ++al)
+						for (size_t c = 0; c < rr2; ++c) { const double t = (double)g[(al * n + j) * rr2 + c]; pyr += t * t; }
+					const double step = pyr / normA[j];
+					for (size_t al = 0; al < rl; ++al)
+						for (size_t c = 0; c < rr2; ++c) upd[(al * n + j) * rr2 + c] = -step * (double)g[(al * n + j) * rr2 + c];
+				}
+				DTensor u = dt_upload(core.dims, upd.data());
+				axpy_f64(x.cores[k].data(), 1.0, u.data(), sz, stream());
+				sync();
+				if (k + 1 < d) move_core(x, k + 1, true);
+			}
+		}
+	};
+
+	move_core(x, 0, false);  // canonicalize_left
+	solve_with_current_ranks();
+	while (residualNorm > o.targetResidualNorm && x.ranks() != maxRanks && (o.maxIterations == 0 || iteration < o.maxIterations)) {
+		move_core(x, 0, true);  // adf.cpp:590-602
+		TT rnd = tt_random(x.dims, dims_t(d - 1, 1), false);
+		const double scale = 1e-6 * tt_frob_norm(x) / tt_frob_norm(rnd);
+		tt_scale(rnd, scale);
+		x = tt_add(x, rnd);
+		round(x, maxRanks, 8 * DBL_EPSILON);
+		solve_with_current_ranks();
+	}
+	if (iterations) *iterations = iteration;
+	return residualNorm;
+}
+template double adf_solve<double>(const ADFOptions&, TT&, const Measurements&, const dims_t&, size_t*);
+template double adf_solve<float>(const ADFOptions&, TT&, const Measurements&, const dims_t&, size_t*);
+
+}  // namespace xtt

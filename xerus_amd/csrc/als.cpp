@@ -1,0 +1,499 @@
+// ALSVariant::solve on the device (src/xerus/algorithms/als.cpp). The sweep control (prepare_x_for_als,
+// prepare_stacks, check_for_end_of_sweep, move_to_next_index) follows xerus step by step; the stacks and
+// local problems are device tensors and the local solve is a matrix-free CG instead of xerus's dense
+// LAPACK solve (the dense local operator is 34 GB at d=16, n=16, r=64).
+//
+// Local operator of a site (a = left x-rank, n = mode size, c = right x-rank, p/q = operator ranks):
+//   y(a,n,c) = Σ L(a,p,a')·A(p,n,n',q)·R(c,q,c')·x(a',n',c')
+// evaluated as three MFMA GEMMs without any permutation of the CG vectors:
+//   T1[(a,p)][(n',c')] = L[(a,p)][a']·x[a'][(n',c')]                      M=a·p  N=n·c  K=a
+//   T2[a][(n,q)][c']   = Am[(n,q)][(p,n')]·T1[a][(p,n')][c']  (batched over a)  M=n·q  N=c  K=p·n
+//   y[(a,n)][c]        = T2[(a,n)][(q,c')]·R[c][(q,c')]ᵀ                   M=a·n  N=c    K=q·c
+// The left stack push reuses T1/T2 with x = X_k and finishes with L'[c][(q,c')] = X_kᵀ·T2; the right push
+// is the left push on reversed cores (X(a,n,c) → (c,n,a), A(p,n,n',q) → (q,n,n',p)).
+//
+// Precision: FP64 runs CG on FP64 GEMMs. FP32 / BF16 run FP64 iterative refinement: r = b − A·x with the
+// FP64 operator, an inner CG on FP32 vectors with FP32 (v_mfma_f32_16x16x4_f32) or BF16
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulate) GEMMs solves A·e = r/‖r‖, and x += ‖r‖·e in FP64.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+#include "tt.hpp"
+
+namespace xtt {
+
+namespace {
+
+constexpr double XERUS_EPSILON = 8 * DBL_EPSILON;  // include/xerus/basic.h:50
+
+DTensor ones(const dims_t& d) {
+	DTensor t(d);
+	fill_f64(t.data(), 1.0, t.size(), stream());
+	return t;
+}
+
+// B_k[(p1,p2), n, n', (q1,q2)] = Σ_m A[p1,m,n,q1]·A[p2,m,n',q2]  (the AᵀA stacks of als.cpp:422-428, 617-623)
+TT op_gram(const TT& A) {
+	TT B;
+	B.op = true; B.dims = A.dims; B.canonicalized = false; B.corePosition = 0;
+	for (size_t k = 0; k < A.order(); ++k) {
+		const DTensor& a = A.cores[k];
+		const size_t p = a.dims[0], n = a.dims[2], q = a.dims[3];
+		DTensor am = reshuffle(a, {1, 0, 2, 3});                 // (m, p, n, q)
+		DTensor g = contract(am, true, am, false, 1);            // (p, n, q, p', n', q')
+		DTensor h = reshuffle(g, {0, 2, 4, 1, 3, 5});            // (p, p', n, n', q, q')
+		h.reshape({p * p, n, n, q * q});
+		B.cores.push_back(std::move(h));
+		B.dims[k] = n; B.dims[A.order() + k] = n;
+	}
+	return B;
+}
+
+// (Aᵀb)_k[(p,β), n', (q,β')] = Σ_m A[p,m,n',q]·b[β,m,β']  (als.cpp:432-437, 639-645)
+TT op_transpose_apply(const TT& A, const TT& b) {
+	TT c;
+	c.op = false; c.canonicalized = false; c.corePosition = 0;
+	for (size_t k = 0; k < b.order(); ++k) {
+		const DTensor& a = A.cores[k];
+		const DTensor& bb = b.cores[k];
+		const size_t p = a.dims[0], n = a.dims[2], q = a.dims[3], r = bb.dims[0], s = bb.dims[2];
+		DTensor am = reshuffle(a, {1, 0, 2, 3});   // (m, p, n', q)
+		DTensor bm = reshuffle(bb, {1, 0, 2});     // (m, β, β')
+		DTensor g = contract(am, true, bm, false, 1);  // (p, n', q, β, β')
+		DTensor h = reshuffle(g, {0, 2, 3, 1, 4});     // (p, β, n', q, β')
+		h.reshape({p * r, n, q * s});
+		c.cores.push_back(std::move(h));
+		c.dims.push_back(n);
+	}
+	return c;
+}
+
+DTensor merge_op(const DTensor& B1, const DTensor& B2) {  // (p, n1·n2, n1'·n2', q)
+	DTensor t = contract(B1, false, B2, false, 1);        // (p, n1, n1', n2, n2', q)
+	DTensor u = reshuffle(t, {0, 1, 3, 2, 4, 5});
+	u.reshape({B1.dims[0], B1.dims[1] * B2.dims[1], B1.dims[2] * B2.dims[2], B2.dims[3]});
+	return u;
+}
+DTensor merge_vec(const DTensor& x1, const DTensor& x2) {  // (a, n1·n2, c)
+	DTensor t = contract(x1, false, x2, false, 1);
+	t.reshape({x1.dims[0], x1.dims[1] * x2.dims[1], x2.dims[2]});
+	return t;
+}
+
+// Operator pieces of one site in the GEMM-friendly layouts, FP64 plus optional FP32 copies.
+struct SiteOp {
+	const DTensor* L = nullptr;  // (a, p, a')
+	const DTensor* R = nullptr;  // (c, q, c')
+	DTensor Am;                  // (n, q, p, n')
+	size_t a = 0, n = 0, c = 0, P = 0, Q = 0;
+	DBuf<float> L32, R32, Am32;
+	size_t N() const { return a * n * c; }
+};
+
+void site_from(SiteOp& s, const DTensor& L, const DTensor& Acore, const DTensor& R) {
+	s.L = &L; s.R = &R;
+	s.P = Acore.dims[0]; s.n = Acore.dims[1]; s.Q = Acore.dims[3];
+	s.a = L.dims[0]; s.c = R.dims[0];
+	XTT_REQUIRE(Acore.dims[1] == Acore.dims[2], "ALS: local operator must be square");
+	XTT_REQUIRE(L.dims[1] == s.P && R.dims[1] == s.Q, "ALS: stack/operator rank mismatch");
+	s.Am = reshuffle(Acore, {2, 0, 3, 1});
+}
+void site_lowp(SiteOp& s) {
+	s.L32.resize(s.L->size()); s.R32.resize(s.R->size()); s.Am32.resize(s.Am.size());
+	convert_f64_f32(s.L->data(), s.L32.get(), s.L->size(), stream());
+	convert_f64_f32(s.R->data(), s.R32.get(), s.R->size(), stream());
+	convert_f64_f32(s.Am.data(), s.Am32.get(), s.Am.size(), stream());
+}
+
+GemmDesc stage2_desc(size_t a, size_t n, size_t c, size_t P, size_t Q, const int* skip) {
+	GemmDesc g = gemm_desc(0, 0, n * Q, c, P * n);
+	g.batch = a;
+	g.sA = 0;
+	g.sB = (long long)(P * n * c);
+	g.sC = (long long)(n * Q * c);
+	g.skip = skip;
+	return g;
+}
+
+template <class T>
+void gemm_t(const GemmDesc& g, const T* A, const T* B, T* C, int prec);
+template <>
+void gemm_t<double>(const GemmDesc& g, const double* A, const double* B, double* C, int) { gemm_f64(g, A, B, C, stream()); }
+template <>
+void gemm_t<float>(const GemmDesc& g, const float* A, const float* B, float* C, int prec) {
+	if (prec == P_BF16) gemm_bf16(g, A, B, C, stream()); else gemm_f32(g, A, B, C, stream());
+}
+
+// y = A_loc·x
+template <class T>
+void local_matvec(const T* L, const T* Am, const T* R, size_t a, size_t n, size_t c, size_t P, size_t Q, const T* x, T* y,
+                  T* T1, T* T2, const int* skip, int prec) {
+	GemmDesc g1 = gemm_desc(0, 0, a * P, n * c, a); g1.skip = skip;
+	gemm_t<T>(g1, L, x, T1, prec);
+	gemm_t<T>(stage2_desc(a, n, c, P, Q, skip), Am, T1, T2, prec);
+	GemmDesc g3 = gemm_desc(0, 1, a * n, c, Q * c); g3.skip = skip;
+	gemm_t<T>(g3, T2, R, y, prec);
+}
+
+// L'(c,q,c') = Σ L(a,p,a')·X(a,n,c)·A(p,n,n',q)·X(a',n',c')
+DTensor op_left(const DTensor& L, const DTensor& X, const DTensor& Acore) {
+	const size_t a = X.dims[0], n = X.dims[1], c = X.dims[2], P = Acore.dims[0], Q = Acore.dims[3];
+	XTT_REQUIRE(L.dims[0] == a && L.dims[2] == a && L.dims[1] == P, "stack push: shape mismatch");
+	DTensor Am = reshuffle(Acore, {2, 0, 3, 1});
+	DBuf<double> T1(a * P * n * c), T2(a * n * Q * c);
+	gemm_f64(gemm_desc(0, 0, a * P, n * c, a), L.data(), X.data(), T1.get(), stream());
+	gemm_f64(stage2_desc(a, n, c, P, Q, nullptr), Am.data(), T1.get(), T2.get(), stream());
+	DTensor out({c, Q, c});
+	gemm_f64(gemm_desc(1, 0, c, Q * c, a * n), X.data(), T2.get(), out.data(), stream());
+	return out;
+}
+DTensor op_right(const DTensor& R, const DTensor& X, const DTensor& Acore) {
+	DTensor Xr = reshuffle(X, {2, 1, 0});
+	DTensor Ar = reshuffle(Acore, {3, 1, 2, 0});
+	return op_left(R, Xr, Ar);
+}
+// Lb'(β',c) = Σ Lb(β,a)·b(β,n,β')·X(a,n,c)
+DTensor rhs_left(const DTensor& Lb, const DTensor& b, const DTensor& X) {
+	DTensor T = contract(Lb, true, b, false, 1);   // (a, n, β')
+	return contract(T, true, X, false, 2);         // (β', c)
+}
+DTensor rhs_right(const DTensor& Rb, const DTensor& b, const DTensor& X) {
+	DTensor br = reshuffle(b, {2, 1, 0});
+	DTensor Xr = reshuffle(X, {2, 1, 0});
+	return rhs_left(Rb, br, Xr);                   // (β, a)
+}
+
+struct Workspace {
+	DBuf<double> r, p, q, T1, T2, bl;
+	DBuf<float> r32, p32, q32, e32, z32, T1f, T2f;
+	DBuf<double> e64;
+	DBuf<CGState> st;
+	DBuf<double> partials;
+	void ensure(size_t N, size_t t1, size_t t2, bool lowp) {
+		if (r.size() < N) { r.resize(N); p.resize(N); q.resize(N); bl.resize(N); e64.resize(N); }
+		if (T1.size() < t1) T1.resize(t1);
+		if (T2.size() < t2) T2.resize(t2);
+		if (!st.get()) { st.resize(1); partials.resize(2 * 1024); }
+		if (lowp) {
+			if (r32.size() < N) { r32.resize(N); p32.resize(N); q32.resize(N); e32.resize(N); z32.resize(N); }
+			if (T1f.size() < t1) T1f.resize(t1);
+			if (T2f.size() < t2) T2f.resize(t2);
+		}
+	}
+};
+
+struct ALSData {
+	const ALSOptions& o;
+	const TT* A;
+	TT B;      // operator of the stacks: A (SPD) or AᵀA
+	TT rhs;    // b or Aᵀb
+	TT& x;
+	const TT& b;
+	dims_t targetRank;
+	double normB;
+	bool canonAtEnd;
+	size_t corePosAtEnd;
+	std::vector<DTensor> opL, opR, rhsL, rhsR;
+	size_t first = 0, second = 0, curr = 0;
+	double lastEnergy2 = 1e102, lastEnergy = 1e101, energy = 1e100;
+	size_t halfSweepCount = 0;
+	bool increasing = true;
+	ALSStats* stats;
+	Workspace ws;
+
+	ALSData(const ALSOptions& o_, const TT* A_, TT& x_, const TT& b_, ALSStats* s)
+	    : o(o_), A(A_), x(x_), b(b_), targetRank(x_.ranks()), normB(tt_frob_norm(b_)), canonAtEnd(x_.canonicalized),
+	      corePosAtEnd(x_.corePosition), stats(s) {
+		if (A) {
+			if (o.assumeSPD) { B = tt_copy(*A); rhs = tt_copy(b); } else { B = op_gram(*A); rhs = op_transpose_apply(*A, b); }
+		} else {
+			rhs = tt_copy(b);
+		}
+		prepare_x();
+		prepare_stacks();
+		curr = first;
+	}
+
+	void prepare_x() {  // als.cpp:330-407
+		const size_t d = x.order();
+		size_t f = 0, dimProd = 1;
+		std::vector<double> h;
+		while (f + 1 < d) {
+			const size_t localDim = x.dims[f], newProd = dimProd * localDim;
+			if (x.rank(f) < newProd) break;
+			DTensor cur = dt_copy(x.cores[f]);
+			cur.reshape({cur.dims[0] * cur.dims[1], cur.dims[2]});
+			x.set_component(f + 1, contract(cur, false, x.cores[f + 1], false, 1));
+			h.assign(dimProd * localDim * newProd, 0.0);
+			for (size_t i = 0; i < dimProd; ++i) for (size_t j = 0; j < localDim; ++j) h[(i * localDim + j) * newProd + i * localDim + j] = 1.0;
+			x.set_component(f, dt_upload({dimProd, localDim, newProd}, h.data()));
+			sync();
+			f += 1; dimProd = newProd;
+		}
+		size_t l = d;
+		dimProd = 1;
+		while (l > f + o.sites) {
+			const size_t localDim = x.dims[l - 1], newProd = dimProd * localDim;
+			if (x.rank(l - 2) < newProd) break;
+			DTensor cur = dt_copy(x.cores[l - 1]);
+			cur.reshape({cur.dims[0], cur.dims[1] * cur.dims[2]});
+			x.set_component(l - 2, contract(x.cores[l - 2], false, cur, false, 1));
+			h.assign(newProd * localDim * dimProd, 0.0);
+			for (size_t j = 0; j < localDim; ++j) for (size_t k = 0; k < dimProd; ++k) h[((j * dimProd + k) * localDim + j) * dimProd + k] = 1.0;
+			x.set_component(l - 1, dt_upload({newProd, localDim, dimProd}, h.data()));
+			sync();
+			l -= 1; dimProd = newProd;
+		}
+		if (canonAtEnd && corePosAtEnd < f) {
+			x.corePosition = f; x.canonicalized = true;
+		} else {
+			if (canonAtEnd && corePosAtEnd >= l) { x.corePosition = l - 1; x.canonicalized = true; }
+			move_core(x, f, true);
+		}
+		first = f; second = l;
+	}
+
+	void prepare_stacks() {  // als.cpp:442-478
+		const size_t d = x.order();
+		opL.clear(); opR.clear(); rhsL.clear(); rhsR.clear();
+		opL.push_back(ones({1, 1, 1})); opR.push_back(ones({1, 1, 1}));
+		rhsL.push_back(ones({1, 1})); rhsR.push_back(ones({1, 1}));
+		for (size_t i = d - 1; i > first + o.sites - 1; --i) {
+			if (A) push_right(i);
+			rhsR.push_back(rhs_right(rhsR.back(), rhs.cores[i], x.cores[i]));
+		}
+		for (size_t i = 0; i < first; ++i) {
+			if (A) push_left(i);
+			rhsL.push_back(rhs_left(rhsL.back(), rhs.cores[i], x.cores[i]));
+		}
+	}
+	void count_env(const DTensor& X, const DTensor& Acore) {
+		if (!stats) return;
+		const double a = (double)X.dims[0], n = (double)X.dims[1], c = (double)X.dims[2], p = (double)Acore.dims[0], q = (double)Acore.dims[3];
+		stats->flopsEnv += 2 * a * n * c * (a * p + n * p * q + c * q);
+	}
+	void push_left(size_t i) { opL.push_back(op_left(opL.back(), x.cores[i], B.cores[i])); count_env(x.cores[i], B.cores[i]); }
+	void push_right(size_t i) { opR.push_back(op_right(opR.back(), x.cores[i], B.cores[i])); count_env(x.cores[i], B.cores[i]); }
+
+	DTensor merged_op() const { DTensor m = dt_copy(B.cores[curr]); for (size_t p = 1; p < o.sites; ++p) m = merge_op(m, B.cores[curr + p]); return m; }
+	DTensor merged(const TT& t) const { DTensor m = dt_copy(t.cores[curr]); for (size_t p = 1; p < o.sites; ++p) m = merge_vec(m, t.cores[curr + p]); return m; }
+
+	DTensor local_rhs() const {  // als.cpp:629-648
+		DTensor bm = merged(rhs);
+		DTensor T = contract(rhsL.back(), true, bm, false, 1);  // (a, N, β')
+		return contract(T, false, rhsR.back(), false, 1);       // (a, N, c)
+	}
+
+	void matvec64(const SiteOp& s, const double* in, double* out, const int* skip) {
+		ws.ensure(s.N(), s.a * s.P * s.n * s.c, s.a * s.n * s.Q * s.c, false);
+		local_matvec<double>(s.L->data(), s.Am.data(), s.R->data(), s.a, s.n, s.c, s.P, s.Q, in, out, ws.T1.get(), ws.T2.get(), skip, P_F64);
+		if (stats) { stats->matvecs64++; stats->flopsMatvec += 2.0 * s.a * s.n * s.c * (s.a * s.P + s.n * s.P * s.Q + s.c * s.Q); }
+	}
+
+	double energy_f() {  // als.cpp:480-545, evaluated from the local quantities at curr
+		if (o.useResidualForEndCriterion && A && o.assumeSPD) return tt_residual(*A, x, b);
+		DTensor xm = merged(x);
+		DTensor bl = local_rhs();
+		const double xb = dt_dot(xm, bl);
+		if (!A) return 0.5 * dt_dot(xm, xm) - xb;
+		DTensor Bm = merged_op();
+		SiteOp s;
+		site_from(s, opL.back(), Bm, opR.back());
+		DTensor y(xm.dims);
+		matvec64(s, xm.data(), y.data(), nullptr);
+		const double xAx = dt_dot(xm, y);
+		if (o.assumeSPD) return std::abs(0.5 * xAx - xb);
+		return std::sqrt(xAx - 2 * xb + normB * normB) / normB;
+	}
+
+	CGState read_state() {
+		CGState h{};
+		XTT_HIP(hipMemcpyAsync(&h, ws.st.get(), sizeof(CGState), hipMemcpyDeviceToHost, stream()));
+		sync();
+		return h;
+	}
+
+	// FP64 CG on A_loc·x = b, warm start from x (device vector, overwritten).
+	void cg64(SiteOp& s, double* x, const double* bl) {
+		const size_t N = s.N();
+		ws.ensure(N, s.a * s.P * s.n * s.c, s.a * s.n * s.Q * s.c, false);
+		matvec64(s, x, ws.q.get(), nullptr);
+		cg_state_init(ws.st.get(), o.localTol * o.localTol, o.maxLocalIter, stream());
+		cg_start<double>(ws.r.get(), ws.p.get(), bl, ws.q.get(), N, ws.st.get(), ws.partials.get(), stream());
+		const int* done = &ws.st.get()->done;
+		for (int it = 0; it < o.maxLocalIter;) {
+			const int chunk = std::min(16, o.maxLocalIter - it);
+			for (int c = 0; c < chunk; ++c) {
+				matvec64(s, ws.p.get(), ws.q.get(), done);
+				cg_alpha<double>(ws.p.get(), ws.q.get(), N, ws.st.get(), ws.partials.get(), stream());
+				cg_update<double>(x, ws.r.get(), ws.p.get(), ws.q.get(), N, ws.st.get(), ws.partials.get(), stream());
+				cg_direction<double>(ws.p.get(), ws.r.get(), N, ws.st.get(), stream());
+			}
+			it += chunk;
+			const CGState h = read_state();
+			if (h.done) { if (stats) stats->cgIterations += h.iter; return; }
+		}
+		if (stats) stats->cgIterations += read_state().iter;
+	}
+
+	// Mixed precision: FP64 iterative refinement around an FP32 / BF16 inner CG.
+	void cg_mixed(SiteOp& s, double* x, const double* bl) {
+		const size_t N = s.N();
+		ws.ensure(N, s.a * s.P * s.n * s.c, s.a * s.n * s.Q * s.c, true);
+		site_lowp(s);
+		const double inner = o.innerTol > 0 ? o.innerTol : (o.precision == P_BF16 ? 3e-2 : 1e-5);
+		XTT_HIP(hipMemsetAsync(ws.z32.get(), 0, N * sizeof(float), stream()));
+		Scratch64 sc;
+		const double bb = sc.dot(bl, bl, N);
+		const double tol2 = o.localTol * o.localTol * bb;
+		for (int ref = 0; ref <= o.maxRefine; ++ref) {
+			matvec64(s, x, ws.q.get(), nullptr);
+			XTT_HIP(hipMemcpyAsync(ws.r.get(), bl, N * 8, hipMemcpyDeviceToDevice, stream()));
+			axpy_f64(ws.r.get(), -1.0, ws.q.get(), N, stream());
+			const double rr = sc.dot(ws.r.get(), ws.r.get(), N);
+			if (rr <= tol2 || ref == o.maxRefine) return;
+			if (stats) stats->refinements++;
+			const double rn = std::sqrt(rr);
+			scale_f64(ws.r.get(), 1.0 / rn, N, stream());
+			convert_f64_f32(ws.r.get(), ws.r32.get(), N, stream());
+			XTT_HIP(hipMemsetAsync(ws.e32.get(), 0, N * sizeof(float), stream()));
+			cg_state_init(ws.st.get(), inner * inner, o.maxLocalIter, stream());
+			// r32 doubles as the right-hand side: cg_start copies it into r/p before they are updated
+			XTT_HIP(hipMemcpyAsync(ws.q32.get(), ws.r32.get(), N * 4, hipMemcpyDeviceToDevice, stream()));
+			cg_start<float>(ws.r32.get(), ws.p32.get(), ws.q32.get(), ws.z32.get(), N, ws.st.get(), ws.partials.get(), stream());
+			const int* done = &ws.st.get()->done;
+			for (int it = 0; it < o.maxLocalIter;) {
+				const int chunk = std::min(16, o.maxLocalIter - it);
+				for (int c = 0; c < chunk; ++c) {
+					local_matvec<float>(s.L32.get(), s.Am32.get(), s.R32.get(), s.a, s.n, s.c, s.P, s.Q, ws.p32.get(), ws.q32.get(),
+					                    ws.T1f.get(), ws.T2f.get(), done, o.precision);
+					cg_alpha<float>(ws.p32.get(), ws.q32.get(), N, ws.st.get(), ws.partials.get(), stream());
+					cg_update<float>(ws.e32.get(), ws.r32.get(), ws.p32.get(), ws.q32.get(), N, ws.st.get(), ws.partials.get(), stream());
+					cg_direction<float>(ws.p32.get(), ws.r32.get(), N, ws.st.get(), stream());
+				}
+				it += chunk;
+				const CGState h = read_state();
+				if (h.done) { if (stats) { stats->cgIterations += h.iter; stats->matvecsLow += h.iter; } break; }
+			}
+			convert_f32_f64(ws.e32.get(), ws.e64.get(), N, stream());
+			axpy_f64(x, rn, ws.e64.get(), N, stream());
+		}
+	}
+
+	struct Scratch64 {
+		DBuf<double> partials{2 * 1024}, out{1};
+		DBuf<unsigned> counter{1};
+		Scratch64() { XTT_HIP(hipMemsetAsync(counter.get(), 0, sizeof(unsigned), stream())); }
+		double dot(const double* a, const double* b, size_t n) {
+			dot_f64(a, b, n, out.get(), partials.get(), counter.get(), stream());
+			double r = 0;
+			XTT_HIP(hipMemcpyAsync(&r, out.get(), 8, hipMemcpyDeviceToHost, stream()));
+			sync();
+			return r;
+		}
+	};
+
+	void local_solve() {
+		DTensor xm = merged(x);
+		const size_t a = xm.dims[0], c = xm.dims[2];
+		DTensor bl = local_rhs();
+		if (!A) {
+			xm = std::move(bl);
+		} else {
+			DTensor Bm = merged_op();
+			SiteOp s;
+			site_from(s, opL.back(), Bm, opR.back());
+			if (o.precision == P_F64) cg64(s, xm.data(), bl.data()); else cg_mixed(s, xm.data(), bl.data());
+		}
+		if (stats) stats->localSolves++;
+		if (o.sites == 1) {
+			xm.reshape(x.cores[curr].dims);
+			x.set_component(curr, std::move(xm));
+			return;
+		}
+		// sites == 2: SVD split capped at the target rank (als.cpp:276-295)
+		xm.reshape({a, x.cores[curr].dims[1], x.cores[curr + 1].dims[1], c});
+		DTensor U, Vt;
+		std::vector<double> S;
+		calculate_svd(U, S, Vt, xm, 2, targetRank[curr], XERUS_EPSILON);
+		const size_t rk = S.size(), cols = Vt.size() / std::max<size_t>(rk, 1), rows = U.size() / std::max<size_t>(rk, 1);
+		if (increasing) {
+			for (size_t j = 0; j < rk; ++j) scale_f64(Vt.data() + j * cols, S[j], cols, stream());
+			x.set_component(curr, std::move(U));
+			x.set_component(curr + 1, std::move(Vt));
+			x.corePosition = curr + 1;
+		} else {
+			// U·S: scale column j of U by s_j
+			DTensor Ut = transpose2(U, U.degree() - 1);
+			for (size_t j = 0; j < rk; ++j) scale_f64(Ut.data() + j * rows, S[j], rows, stream());
+			DTensor US = transpose2(Ut, 1);
+			US.reshape(U.dims);
+			x.set_component(curr + 1, std::move(Vt));
+			x.set_component(curr, std::move(US));
+			x.corePosition = curr;
+		}
+		x.canonicalized = true;
+	}
+
+	bool end_of_sweep() {  // als.cpp:651-700
+		if ((!increasing && curr == first) || (increasing && curr == second - o.sites)) {
+			halfSweepCount += 1;
+			lastEnergy2 = lastEnergy;
+			lastEnergy = energy;
+			energy = energy_f();
+			if (stats) { stats->halfSweeps = halfSweepCount; stats->energies.push_back(energy); }
+			if (halfSweepCount == o.numHalfSweeps || std::abs(lastEnergy - energy) < o.convergenceEpsilon ||
+			    std::abs(lastEnergy2 - energy) < o.convergenceEpsilon || second - first <= o.sites) {
+				if (canonAtEnd && o.preserveCorePosition) move_core(x, corePosAtEnd, true);
+				return true;
+			}
+			increasing = !increasing;
+		}
+		return false;
+	}
+
+	// als.cpp:565-605; for sites >= 2 the pushed stack is that of the site leaving the window, after the core has
+	// moved into the window (for sites = 1 this is exactly xerus's step).
+	void next() {
+		if (increasing) {
+			move_core(x, curr + 1, true);
+			if (A) { opR.pop_back(); push_left(curr); }
+			rhsR.pop_back();
+			rhsL.push_back(rhs_left(rhsL.back(), rhs.cores[curr], x.cores[curr]));
+			curr++;
+		} else {
+			const size_t leaving = curr + o.sites - 1;
+			move_core(x, leaving - 1, true);
+			if (A) { opL.pop_back(); push_right(leaving); }
+			rhsL.pop_back();
+			rhsR.push_back(rhs_right(rhsR.back(), rhs.cores[leaving], x.cores[leaving]));
+			curr--;
+		}
+	}
+};
+
+}  // namespace
+
+double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats* stats) {  // als.cpp:708-778
+	XTT_REQUIRE(o.sites > 0, "ALSVariant: sites must be > 0");
+	XTT_REQUIRE(x.order() > 0, "ALS: x must have degree > 0");
+	XTT_REQUIRE(x.dims == b.dims, "ALS: x and b dimensions differ");
+	XTT_REQUIRE(!x.op && !b.op, "ALS: x and b must be TTTensors");
+	if (A) {
+		XTT_REQUIRE(A->op && A->dims.size() == 2 * b.dims.size(), "ALS: operator degree mismatch");
+		for (size_t i = 0; i < x.order(); ++i)
+			XTT_REQUIRE(A->dims[i] == x.dims[i] && A->dims[i + x.order()] == x.dims[i], "ALS: operator dimension mismatch");
+	} else {
+		XTT_REQUIRE(o.sites == 1, "approximation dmrg not implemented yet");
+	}
+	ALSData data(o, A, x, b, stats);
+	data.energy = data.energy_f();
+	while (true) {
+		data.local_solve();
+		if (data.end_of_sweep()) return data.energy;
+		data.next();
+	}
+}
+
+}  // namespace xtt

@@ -1,0 +1,308 @@
+// MFMA GEMM for gfx950: C[b] = alpha·op(A[b])·op(B[b]) + beta·C[b], row-major, batched with element strides.
+//
+// One workgroup computes a BM×BN tile with WM×WN waves; each wave owns TM×TN 16×16 MFMA tiles. The K
+// extent is staged through LDS in chunks of up to KMAX (our TT shapes have K <= 256, so one chunk: one
+// global-load phase per tile). Staging loads are issued 8 at a time before the LDS stores so the wave keeps
+// many loads in flight. Ragged edges are predicated (zero fill), so any m, n, k works.
+//
+// FP64 uses v_mfma_f64_16x16x4_f64, FP32 v_mfma_f32_16x16x4_f32 and the BF16 path
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation (operands are held as FP32 in memory and rounded to bf16
+// while being staged). Operand lane layouts (16×16 tile, lane l):
+//   x4 shapes : A[i=l%16][k=l/16], B[k=l/16][j=l%16]
+//   x32 bf16  : A[i=l%16][k=8(l/16)+e], B[k=8(l/16)+e][j=l%16], e = 0..7
+//   result    : D[i=4(l/16)+r][j=l%16], r = 0..3
+#include "kernels.hpp"
+#include "xtt_common.hpp"
+
+namespace xtt {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+struct Args {
+	const void* A;
+	const void* B;
+	void* C;
+	int m, n, k;
+	int lda, ldb, ldc;
+	long long sA, sB, sC;
+	int ta, tb;
+	double alpha, beta;
+	const int* skip;
+	int kc;  // K chunk staged in LDS
+};
+
+__device__ inline d4 mfma_x4(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ inline f4 mfma_x4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+template <class T> struct AccOf;
+template <> struct AccOf<double> { using type = d4; };
+template <> struct AccOf<float> { using type = f4; };
+
+// Stage op(X) restricted to rows [r0, r0+R) × k-range [k0, k0+kc) into LDS as S[kk*(R+PAD) + i] (k-major).
+// trans = 0: X[i*ld + k] (k contiguous); trans = 1: X[k*ld + i] (i contiguous). Out-of-range → 0.
+template <class T, int R, int PAD, int NT>
+__device__ inline void stage_kmajor(T* S, const T* X, int ld, int trans, int r0, int rows, int k0, int kc, int klim, int tid) {
+	const int total = R * kc;
+	for (int base = 0; base < total; base += NT * 8) {
+		T v[8];
+		int li[8], lk[8];
+#pragma unroll
+		for (int u = 0; u < 8; ++u) {
+			const int e = base + u * NT + tid;
+			int i, kk;
+			if (!trans) { i = e / kc; kk = e - i * kc; } else { kk = e / R; i = e - kk * R; }
+			li[u] = i; lk[u] = kk;
+			const int gi = r0 + i, gk = k0 + kk;
+			v[u] = T(0);
+			if (e < total && gi < rows && gk < klim)
+				v[u] = trans ? X[(size_t)gk * ld + gi] : X[(size_t)gi * ld + gk];
+		}
+#pragma unroll
+		for (int u = 0; u < 8; ++u) {
+			const int e = base + u * NT + tid;
+			if (e < total) S[lk[u] * (R + PAD) + li[u]] = v[u];
+		}
+	}
+}
+
+template <class T, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_x4_kernel(Args g) {
+	if (g.skip && *g.skip) return;
+	constexpr int NT = 64 * WM * WN, TM = BM / WM / 16, TN = BN / WN / 16, PAD = 1;
+	using Acc = typename AccOf<T>::type;
+	extern __shared__ __align__(16) unsigned char smem_raw[];
+	T* As = reinterpret_cast<T*>(smem_raw);
+	T* Bs = As + (size_t)g.kc * (BM + PAD);
+	const long long bz = blockIdx.z;
+	const T* A = static_cast<const T*>(g.A) + bz * g.sA;
+	const T* B = static_cast<const T*>(g.B) + bz * g.sB;
+	T* C = static_cast<T*>(g.C) + bz * g.sC;
+	const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int wm = wave / WN, wn = wave % WN;
+	const int li = lane & 15, lk = lane >> 4;
+	Acc acc[TM][TN];
+#pragma unroll
+	for (int a = 0; a < TM; ++a)
+#pragma unroll
+		for (int b = 0; b < TN; ++b) acc[a][b] = Acc{0, 0, 0, 0};
+
+	for (int k0 = 0; k0 < g.k; k0 += g.kc) {
+		const int kc = min(g.kc, g.k - k0);
+		const int kcp = (kc + 3) & ~3;  // MFMA consumes K in steps of 4; staging zero-fills the tail
+		stage_kmajor<T, BM, PAD, NT>(As, A, g.lda, g.ta, m0, g.m, k0, kcp, g.k, tid);
+		// B is staged as op(B)ᵀ rows: element (j, kk) = op(B)[kk][j]; tb = 0 means k-major in memory.
+		stage_kmajor<T, BN, PAD, NT>(Bs, B, g.ldb, !g.tb, n0, g.n, k0, kcp, g.k, tid);
+		__syncthreads();
+		for (int kk = 0; kk < kcp; kk += 4) {
+			T av[TM], bv[TN];
+#pragma unroll
+			for (int a = 0; a < TM; ++a) av[a] = As[(kk + lk) * (BM + PAD) + (wm * TM + a) * 16 + li];
+#pragma unroll
+			for (int b = 0; b < TN; ++b) bv[b] = Bs[(kk + lk) * (BN + PAD) + (wn * TN + b) * 16 + li];
+#pragma unroll
+			for (int a = 0; a < TM; ++a)
+#pragma unroll
+				for (int b = 0; b < TN; ++b) acc[a][b] = mfma_x4(av[a], bv[b], acc[a][b]);
+		}
+		__syncthreads();
+	}
+	const T alpha = T(g.alpha), beta = T(g.beta);
+#pragma unroll
+	for (int a = 0; a < TM; ++a)
+#pragma unroll
+		for (int b = 0; b < TN; ++b)
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				const int i = m0 + (wm * TM + a) * 16 + 4 * lk + r;
+				const int j = n0 + (wn * TN + b) * 16 + li;
+				if (i < g.m && j < g.n) {
+					T* c = C + (size_t)i * g.ldc + j;
+					const T v = alpha * acc[a][b][r];
+					*c = (g.beta != 0.0) ? v + beta * *c : v;
+				}
+			}
+}
+
+// BF16 compute: LDS holds bf16 row-major tiles As[i][k], Bs[j][k] (k contiguous, 8 elements per lane read).
+template <int R, int NT>
+__device__ inline void stage_rowk_bf16(__bf16* S, const float* X, int ld, int trans, int r0, int rows, int k0, int kc, int klim,
+                                       int tid) {
+	constexpr int KP = 8;  // row padding (elements) keeps 16-byte alignment and breaks bank aliasing
+	const int total = R * kc;
+	for (int base = 0; base < total; base += NT * 8) {
+		float v[8];
+		int li[8], lk[8];
+#pragma unroll
+		for (int u = 0; u < 8; ++u) {
+			const int e = base + u * NT + tid;
+			int i, kk;
+			if (!trans) { i = e / kc; kk = e - i * kc; } else { kk = e / R; i = e - kk * R; }
+			li[u] = i; lk[u] = kk;
+			const int gi = r0 + i, gk = k0 + kk;
+			v[u] = 0.f;
+			if (e < total && gi < rows && gk < klim) v[u] = trans ? X[(size_t)gk * ld + gi] : X[(size_t)gi * ld + gk];
+		}
+#pragma unroll
+		for (int u = 0; u < 8; ++u) {
+			const int e = base + u * NT + tid;
+			if (e < total) S[li[u] * (kc + KP) + lk[u]] = static_cast<__bf16>(v[u]);
+		}
+	}
+}
+
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_bf16_kernel(Args g) {
+	if (g.skip && *g.skip) return;
+	constexpr int NT = 64 * WM * WN, TM = BM / WM / 16, TN = BN / WN / 16, KP = 8;
+	extern __shared__ __align__(16) unsigned char smem_raw[];
+	const long long bz = blockIdx.z;
+	const float* A = static_cast<const float*>(g.A) + bz * g.sA;
+	const float* B = static_cast<const float*>(g.B) + bz * g.sB;
+	float* C = static_cast<float*>(g.C) + bz * g.sC;
+	const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int wm = wave / WN, wn = wave % WN;
+	const int li = lane & 15, lk = lane >> 4;
+	f4 acc[TM][TN];
+#pragma unroll
+	for (int a = 0; a < TM; ++a)
+#pragma unroll
+		for (int b = 0; b < TN; ++b) acc[a][b] = f4{0, 0, 0, 0};
+	for (int k0 = 0; k0 < g.k; k0 += g.kc) {
+		const int kc = min(g.kc, g.k - k0);
+		const int kcp = (kc + 31) & ~31;
+		__bf16* As = reinterpret_cast<__bf16*>(smem_raw);
+		__bf16* Bs = As + (size_t)BM * (kcp + KP);
+		stage_rowk_bf16<BM, NT>(As, A, g.lda, g.ta, m0, g.m, k0, kcp, g.k, tid);
+		stage_rowk_bf16<BN, NT>(Bs, B, g.ldb, !g.tb, n0, g.n, k0, kcp, g.k, tid);
+		__syncthreads();
+		for (int kk = 0; kk < kcp; kk += 32) {
+			bf8 av[TM], bv[TN];
+#pragma unroll
+			for (int a = 0; a < TM; ++a) av[a] = *reinterpret_cast<const bf8*>(As + ((wm * TM + a) * 16 + li) * (kcp + KP) + kk + 8 * lk);
+#pragma unroll
+			for (int b = 0; b < TN; ++b) bv[b] = *reinterpret_cast<const bf8*>(Bs + ((wn * TN + b) * 16 + li) * (kcp + KP) + kk + 8 * lk);
+#pragma unroll
+			for (int a = 0; a < TM; ++a)
+#pragma unroll
+				for (int b = 0; b < TN; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+		}
+		__syncthreads();
+	}
+	const float alpha = float(g.alpha), beta = float(g.beta);
+#pragma unroll
+	for (int a = 0; a < TM; ++a)
+#pragma unroll
+		for (int b = 0; b < TN; ++b)
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				const int i = m0 + (wm * TM + a) * 16 + 4 * lk + r;
+				const int j = n0 + (wn * TN + b) * 16 + li;
+				if (i < g.m && j < g.n) {
+					float* c = C + (size_t)i * g.ldc + j;
+					const float v = alpha * acc[a][b][r];
+					*c = (g.beta != 0.0) ? v + beta * *c : v;
+				}
+			}
+}
+
+Args make_args(const GemmDesc& d, const void* A, const void* B, void* C) {
+	XTT_REQUIRE(d.m <= INT32_MAX && d.n <= INT32_MAX && d.k <= INT32_MAX, "gemm: dimension too large");
+	XTT_REQUIRE(d.batch >= 1 && d.batch <= 65535, "gemm: batch out of range");
+	XTT_REQUIRE(d.lda >= (d.ta ? d.m : d.k) && d.ldb >= (d.tb ? d.k : d.n) && d.ldc >= d.n, "gemm: leading dimension too small");
+	Args a{};
+	a.A = A; a.B = B; a.C = C;
+	a.m = (int)d.m; a.n = (int)d.n; a.k = (int)d.k;
+	a.lda = (int)d.lda; a.ldb = (int)d.ldb; a.ldc = (int)d.ldc;
+	a.sA = d.sA; a.sB = d.sB; a.sC = d.sC;
+	a.ta = d.ta; a.tb = d.tb; a.alpha = d.alpha; a.beta = d.beta; a.skip = d.skip;
+	return a;
+}
+
+// LDS budget per workgroup (gfx950 has 160 KiB per CU); kernels needing more than the 64 KiB default opt in.
+constexpr size_t kLdsBudget = 150 * 1024;
+
+template <class K>
+void allow_lds(K kernel, size_t bytes) {
+	if (bytes > 64 * 1024) XTT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+// Tile choice: 64×64 tiles once they still give >= 256 workgroups, else 32×32. The K chunk is the largest
+// multiple of 4 (x4 MFMA) whose two operand tiles fit the LDS budget.
+template <class T>
+void launch_x4(const GemmDesc& d, const T* A, const T* B, T* C, hipStream_t s) {
+	if (d.m == 0 || d.n == 0) return;
+	Args a = make_args(d, A, B, C);
+	const size_t tiles64 = ((d.m + 63) / 64) * ((d.n + 63) / 64) * d.batch;
+	const bool big = tiles64 >= 256;
+	const size_t BMp = big ? 65 : 33;
+	size_t kmax = kLdsBudget / (2 * BMp * sizeof(T));
+	kmax &= ~size_t(3);
+	a.kc = (int)std::max<size_t>(4, std::min<size_t>((d.k + 3) & ~size_t(3), kmax));
+	const size_t lds = (size_t)a.kc * BMp * 2 * sizeof(T);
+	const double flops = 2.0 * d.m * d.n * d.k * d.batch;
+	const int slot = prof_begin(s);
+	if (big) {
+		auto kern = gemm_x4_kernel<T, 64, 64, 2, 2>;
+		allow_lds(kern, lds);
+		dim3 grid((unsigned)((d.n + 63) / 64), (unsigned)((d.m + 63) / 64), (unsigned)d.batch);
+		hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a);
+		XTT_LAUNCH_CHECK();
+		prof_end(slot, s, sizeof(T) == 8 ? "gemm_x4_kernel<double,64,64>" : "gemm_x4_kernel<float,64,64>", flops);
+	} else {
+		auto kern = gemm_x4_kernel<T, 32, 32, 2, 2>;
+		allow_lds(kern, lds);
+		dim3 grid((unsigned)((d.n + 31) / 32), (unsigned)((d.m + 31) / 32), (unsigned)d.batch);
+		hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a);
+		XTT_LAUNCH_CHECK();
+		prof_end(slot, s, sizeof(T) == 8 ? "gemm_x4_kernel<double,32,32>" : "gemm_x4_kernel<float,32,32>", flops);
+	}
+}
+
+}  // namespace
+
+GemmDesc gemm_desc(int ta, int tb, size_t m, size_t n, size_t k) {
+	GemmDesc d;
+	d.ta = ta; d.tb = tb; d.m = m; d.n = n; d.k = k;
+	d.lda = ta ? m : k; d.ldb = tb ? k : n; d.ldc = n;
+	d.sA = (long long)(m * k); d.sB = (long long)(k * n); d.sC = (long long)(m * n);
+	return d;
+}
+
+void gemm_f64(const GemmDesc& g, const double* A, const double* B, double* C, hipStream_t s) { launch_x4<double>(g, A, B, C, s); }
+void gemm_f32(const GemmDesc& g, const float* A, const float* B, float* C, hipStream_t s) { launch_x4<float>(g, A, B, C, s); }
+
+void gemm_bf16(const GemmDesc& d, const float* A, const float* B, float* C, hipStream_t s) {
+	if (d.m == 0 || d.n == 0) return;
+	Args a = make_args(d, A, B, C);
+	const size_t tiles64 = ((d.m + 63) / 64) * ((d.n + 63) / 64) * d.batch;
+	const bool big = tiles64 >= 256;
+	const size_t rows = big ? 128 : 64;
+	size_t kmax = kLdsBudget / (rows * sizeof(__bf16)) - 8;
+	kmax &= ~size_t(31);
+	a.kc = (int)std::max<size_t>(32, std::min<size_t>((d.k + 31) & ~size_t(31), kmax));
+	const size_t lds = rows * (size_t)(a.kc + 8) * sizeof(__bf16);
+	const double flops = 2.0 * d.m * d.n * d.k * d.batch;
+	const int slot = prof_begin(s);
+	if (big) {
+		auto kern = gemm_bf16_kernel<64, 64, 2, 2>;
+		allow_lds(kern, lds);
+		dim3 grid((unsigned)((d.n + 63) / 64), (unsigned)((d.m + 63) / 64), (unsigned)d.batch);
+		hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a);
+		XTT_LAUNCH_CHECK();
+		prof_end(slot, s, "gemm_bf16_kernel<64,64>", flops);
+	} else {
+		auto kern = gemm_bf16_kernel<32, 32, 2, 2>;
+		allow_lds(kern, lds);
+		dim3 grid((unsigned)((d.n + 31) / 32), (unsigned)((d.m + 31) / 32), (unsigned)d.batch);
+		hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a);
+		XTT_LAUNCH_CHECK();
+		prof_end(slot, s, "gemm_bf16_kernel<32,32>", flops);
+	}
+}
+
+}  // namespace xtt

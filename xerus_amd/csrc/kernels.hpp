@@ -1,0 +1,102 @@
+// Launchers of libxtt's gfx950 kernels. Every launcher enqueues on the given stream and checks the launch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace xtt {
+
+// C[b] = alpha·op(A[b])·op(B[b]) + beta·C[b], row-major, op(A) m×k, op(B) k×n. Batch strides in elements.
+// `skip` (device int, may be null): when *skip != 0 the launch does nothing (predicated CG tails).
+struct GemmDesc {
+	int ta = 0, tb = 0;
+	size_t m = 0, n = 0, k = 0;
+	size_t lda = 0, ldb = 0, ldc = 0;
+	long long sA = 0, sB = 0, sC = 0;
+	size_t batch = 1;
+	double alpha = 1.0, beta = 0.0;
+	const int* skip = nullptr;
+};
+GemmDesc gemm_desc(int ta, int tb, size_t m, size_t n, size_t k);  // packed leading dimensions, batch 1
+void gemm_f64(const GemmDesc& g, const double* A, const double* B, double* C, hipStream_t s);
+void gemm_f32(const GemmDesc& g, const float* A, const float* B, float* C, hipStream_t s);
+// BF16 products with FP32 accumulation; operands are stored as FP32 and rounded to bf16 when staged in LDS.
+void gemm_bf16(const GemmDesc& g, const float* A, const float* B, float* C, hipStream_t s);
+
+// out = reshuffle(in, shuffle): shuffle[i] = new position of old mode i (xerus::reshuffle semantics).
+void permute_f64(size_t degree, const size_t* dims, const size_t* shuffle, const double* in, double* out, hipStream_t s);
+void revtrans_f64(const double* in, size_t rows, size_t cols, double* out, hipStream_t s);  // out[i][j] = in[rows-1-j][cols-1-i]
+void convert_f64_f32(const double* in, float* out, size_t n, hipStream_t s);
+void convert_f32_f64(const float* in, double* out, size_t n, hipStream_t s);
+void fill_f64(double* p, double v, size_t n, hipStream_t s);
+void scale_f64(double* p, double alpha, size_t n, hipStream_t s);
+void axpy_f64(double* y, double alpha, const double* x, size_t n, hipStream_t s);        // y += alpha·x
+// Deterministic FP64 reductions into *out (device). `partials` holds >= reduce_blocks(n) doubles, `counter`
+// one zero-initialised unsigned that the kernel returns to zero.
+size_t reduce_blocks(size_t n);
+void dot_f64(const double* x, const double* y, size_t n, double* out, double* partials, unsigned* counter, hipStream_t s);
+void dot_f32_acc64(const float* x, const float* y, size_t n, double* out, double* partials, unsigned* counter, hipStream_t s);
+
+// ---- local CG on the device; all scalars live in CGState (device) -------------------------------------
+struct CGState {
+	double rr;        // current ‖r‖²
+	double rr0;       // ‖b‖² of the system being solved (tolerance reference)
+	double tol2;      // stop when rr <= tol2·rr0
+	double alpha, beta, pq;
+	int done;         // 1 once converged (every later CG launch is a no-op)
+	int iter;         // iterations performed
+	int maxIter;
+	int pad;
+	unsigned counter; // reduction ticket
+	unsigned pad2;
+};
+template <class T> void cg_start(T* r, T* p, const T* b, const T* q, size_t n, CGState* st, double* partials, hipStream_t s);
+template <class T> void cg_alpha(const T* p, const T* q, size_t n, CGState* st, double* partials, hipStream_t s);
+template <class T> void cg_update(T* x, T* r, const T* p, const T* q, size_t n, CGState* st, double* partials, hipStream_t s);
+template <class T> void cg_direction(T* p, const T* r, size_t n, CGState* st, hipStream_t s);
+void cg_state_init(CGState* st, double tol2, int maxIter, hipStream_t s);
+
+// ---- dense factorisations (one workgroup per matrix, FP64, matrix in device memory) -------------------
+// In-place Householder QR (pivot = 0) or column-pivoted QR (pivot = 1) of a row-major m×n matrix, LAPACK
+// dgeqr2 / dlaqp2 conventions: R on and above the diagonal, reflectors below, tau[min(m,n)], jpvt[n]
+// (original column index of each position). `ws` needs 2n doubles. *rank receives xerus's QC rank
+// (blasLapackWrapper.cpp:268-272) when pivot = 1, min(m,n) otherwise.
+void geqp2(double* A, size_t m, size_t n, double* tau, int* jpvt, int pivot, double* ws, int* rank, hipStream_t s);
+// Q (m×k, row-major, k <= min(m,n)) = first k columns of H_0…H_{min-1}·[I;0] from the reflectors in A.
+void org2r(const double* A, size_t m, size_t n, const double* tau, double* Q, size_t k, hipStream_t s);
+// C (rank×n, row-major) from the pivoted R in A: C[row][jpvt[col]] = R[row][col] for row <= col.
+void extract_c(const double* A, size_t m, size_t n, const int* jpvt, const int* rank, double* C, hipStream_t s);
+// R (k×n) = upper triangle of A (row-major), k = min(m,n).
+void extract_r(const double* A, size_t m, size_t n, double* R, hipStream_t s);
+// One-sided Jacobi SVD of a row-major m×n matrix with m >= n: U (m×n), S (n, descending), Vt (n×n).
+// Workspace: m·n + n·n + 2n doubles.
+void jacobi_svd(const double* A, size_t m, size_t n, double* U, double* S, double* Vt, double* ws, hipStream_t s);
+
+// ---- ADF sampled-entry kernels (FP32 cores / values, FP64 norms) ---------------------------------------
+struct AdfDesc {
+	int d;                    // order
+	int P;                    // number of measurements
+	const int* n;             // device: mode sizes [d]
+	const int* rank;          // device: ranks r_0..r_d (r_0 = r_d = 1) [d+1]
+	const long long* coreOff; // device: element offset of core k in `cores` [d]
+	int maxR;                 // max rank (<= 16)
+	int gradSize;             // total elements of all cores
+};
+// value_i = x(position_i); resid_i = value_i - b_i; Σ resid² and Σ b² in FP64 into norms[0..1].
+template <class T>
+void adf_residual(const AdfDesc& a, const T* cores, const uint32_t* pos, const T* values, T* resid, double* norms,
+                  double* partials, unsigned* counter, hipStream_t s);
+// grad (layout of `cores`) = Σ_i resid_i·∂x_i/∂G, restricted to core kSel when kSel >= 0 (other entries 0).
+// `partials` holds adf_grad_blocks(P)·gradSize elements.
+int adf_grad_blocks(int P);
+template <class T>
+void adf_gradient(const AdfDesc& a, const T* cores, const uint32_t* pos, const T* resid, T* grad, T* partials, int kSel,
+                  hipStream_t s);
+// out[j] = Σ_{i: i_k = j} (x_i with core k replaced by D)²; `partials` holds adf_slice_blocks(P)·nslices doubles.
+int adf_slice_blocks(int P);
+template <class T>
+void adf_slice_norms(const AdfDesc& a, const T* cores, const uint32_t* pos, int k, const T* D, int nslices, double* out,
+                     double* partials, hipStream_t s);
+
+}  // namespace xtt

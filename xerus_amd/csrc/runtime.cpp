@@ -1,0 +1,158 @@
+// libxtt runtime: thread-local error text, the per-device library stream and the caching device allocator.
+#include <mutex>
+#include <unordered_map>
+
+#include "xtt_common.hpp"
+
+namespace xtt {
+
+namespace {
+thread_local std::string g_error;
+
+struct DeviceState {
+	hipStream_t stream = nullptr;
+};
+std::mutex g_mu;
+std::unordered_map<int, DeviceState> g_dev;
+
+// Allocation cache: size class (rounded to 256 B below 1 MiB, to 64 KiB above) → free blocks.
+struct Block { size_t bytes; int device; };
+std::mutex g_amu;
+std::multimap<std::pair<int, size_t>, void*> g_free;
+std::unordered_map<void*, Block> g_live;
+size_t g_inUse = 0;
+
+size_t size_class(size_t b) {
+	if (b <= (1u << 20)) return (b + 255) & ~size_t(255);
+	return (b + 65535) & ~size_t(65535);
+}
+}  // namespace
+
+void set_error(const std::string& s) { g_error = s; }
+const std::string& last_error() { return g_error; }
+
+hipStream_t stream() {
+	int dev = 0;
+	XTT_HIP(hipGetDevice(&dev));
+	std::lock_guard<std::mutex> lk(g_mu);
+	DeviceState& d = g_dev[dev];
+	if (!d.stream) XTT_HIP(hipStreamCreateWithFlags(&d.stream, hipStreamDefault));
+	return d.stream;
+}
+
+void sync() { XTT_HIP(hipStreamSynchronize(stream())); }
+
+void* dmalloc(size_t bytes) {
+	if (bytes == 0) bytes = 1;
+	int dev = 0;
+	XTT_HIP(hipGetDevice(&dev));
+	const size_t cls = size_class(bytes);
+	{
+		std::lock_guard<std::mutex> lk(g_amu);
+		auto it = g_free.find({dev, cls});
+		if (it != g_free.end()) {
+			void* p = it->second;
+			g_free.erase(it);
+			g_live[p] = {cls, dev};
+			g_inUse += cls;
+			return p;
+		}
+	}
+	void* p = nullptr;
+	hipError_t e = hipMalloc(&p, cls);
+	if (e == hipErrorOutOfMemory) {  // give cached blocks back and retry once
+		(void)hipGetLastError();
+		release_cache();
+		e = hipMalloc(&p, cls);
+	}
+	if (e != hipSuccess) throw Error(e == hipErrorOutOfMemory ? ERR_NOMEM : ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+	std::lock_guard<std::mutex> lk(g_amu);
+	g_live[p] = {cls, dev};
+	g_inUse += cls;
+	return p;
+}
+
+void dfree(void* p) {
+	if (!p) return;
+	std::lock_guard<std::mutex> lk(g_amu);
+	auto it = g_live.find(p);
+	if (it == g_live.end()) return;
+	g_free.insert({{it->second.device, it->second.bytes}, p});
+	g_inUse -= it->second.bytes;
+	g_live.erase(it);
+}
+
+void release_cache() {
+	std::multimap<std::pair<int, size_t>, void*> blocks;
+	{
+		std::lock_guard<std::mutex> lk(g_amu);
+		blocks.swap(g_free);
+	}
+	if (!blocks.empty()) (void)hipDeviceSynchronize();
+	for (auto& kv : blocks) (void)hipFree(kv.second);
+}
+
+namespace {
+struct Slot { hipEvent_t a, b; std::string tag; double flops; };
+std::mutex g_pmu;
+bool g_prof = false;
+std::vector<Slot> g_slots;
+}  // namespace
+
+bool prof_enabled() { return g_prof; }
+void prof_enable(bool on) { std::lock_guard<std::mutex> lk(g_pmu); g_prof = on; }
+
+int prof_begin(hipStream_t s) {
+	if (!g_prof) return -1;
+	Slot sl;
+	XTT_HIP(hipEventCreate(&sl.a));
+	XTT_HIP(hipEventCreate(&sl.b));
+	XTT_HIP(hipEventRecord(sl.a, s));
+	std::lock_guard<std::mutex> lk(g_pmu);
+	g_slots.push_back(sl);
+	return (int)g_slots.size() - 1;
+}
+
+void prof_end(int slot, hipStream_t s, const char* tag, double flops) {
+	if (slot < 0) return;
+	std::lock_guard<std::mutex> lk(g_pmu);
+	Slot& sl = g_slots[slot];
+	sl.tag = tag;
+	sl.flops = flops;
+	XTT_HIP(hipEventRecord(sl.b, s));
+}
+
+std::vector<ProfRecord> prof_collect() {
+	XTT_HIP(hipDeviceSynchronize());
+	std::lock_guard<std::mutex> lk(g_pmu);
+	std::map<std::string, ProfRecord> agg;
+	for (Slot& sl : g_slots) {
+		float ms = 0;
+		XTT_HIP(hipEventElapsedTime(&ms, sl.a, sl.b));
+		ProfRecord& pr = agg[sl.tag];
+		pr.tag = sl.tag;
+		pr.launches++;
+		pr.ms += ms;
+		pr.flops += sl.flops;
+		(void)hipEventDestroy(sl.a);
+		(void)hipEventDestroy(sl.b);
+	}
+	g_slots.clear();
+	std::vector<ProfRecord> out;
+	for (auto& kv : agg) out.push_back(kv.second);
+	return out;
+}
+
+size_t bytes_in_use() {
+	std::lock_guard<std::mutex> lk(g_amu);
+	return g_inUse;
+}
+
+void DTensor::reshape(std::vector<size_t> d) {
+	size_t p = 1;
+	for (size_t x : d) p *= x;
+	XTT_REQUIRE(p == count(), "reshape: element count changes");
+	dims = std::move(d);
+}
+
+}  // namespace xtt

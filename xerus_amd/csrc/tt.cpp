@@ -1,0 +1,481 @@
+// Tensor-level operations and TTNetwork algorithms (move_core, round, sum, operator application, stable
+// residual) on device-resident FP64 cores. Control flow follows xerus (file:line cited per function); the
+// arithmetic runs in libxtt's gfx950 kernels.
+#include "tt.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <random>
+
+namespace xtt {
+
+namespace {
+size_t prodr(const dims_t& d, size_t b, size_t e) { size_t p = 1; for (size_t i = b; i < e; ++i) p *= d[i]; return p; }
+dims_t cat(dims_t a, const dims_t& b) { a.insert(a.end(), b.begin(), b.end()); return a; }
+dims_t sl(const dims_t& d, size_t b, size_t e) { return dims_t(d.begin() + b, d.begin() + e); }
+
+// Reduction scratch shared by the host-side helpers (one stream, so reuse is ordered).
+struct Scratch {
+	DBuf<double> partials;
+	DBuf<unsigned> counter;
+	DBuf<double> scalar;
+	Scratch() : partials(2 * 1024), counter(1), scalar(4) {
+		XTT_HIP(hipMemsetAsync(counter.get(), 0, sizeof(unsigned), stream()));
+	}
+};
+Scratch& scratch() { static thread_local Scratch s; return s; }
+
+std::mt19937_64& engine() { static std::mt19937_64 e(0xBAADF00Dull); return e; }
+std::normal_distribution<double>& normal() { static std::normal_distribution<double> d; return d; }
+}  // namespace
+
+void rng_seed(uint64_t s) { engine().seed(s); normal().reset(); }
+double rng_normal() { return normal()(engine()); }
+size_t rng_uniform(size_t hi) { std::uniform_int_distribution<size_t> u(0, hi); return u(engine()); }
+
+// ------------------------------------------------------------------------------- Tensor level
+
+DTensor dt_upload(const dims_t& dims, const double* host) {
+	DTensor t(dims);
+	if (t.size()) XTT_HIP(hipMemcpyAsync(t.data(), host, t.size() * 8, hipMemcpyHostToDevice, stream()));
+	return t;
+}
+void dt_download(const DTensor& t, double* host) {
+	if (t.size()) XTT_HIP(hipMemcpyAsync(host, t.data(), t.size() * 8, hipMemcpyDeviceToHost, stream()));
+	sync();
+}
+DTensor dt_copy(const DTensor& t) {
+	DTensor o(t.dims);
+	if (t.size()) XTT_HIP(hipMemcpyAsync(o.data(), t.data(), t.size() * 8, hipMemcpyDeviceToDevice, stream()));
+	return o;
+}
+DTensor dt_zeros(const dims_t& dims) {
+	DTensor t(dims);
+	if (t.size()) XTT_HIP(hipMemsetAsync(t.data(), 0, t.size() * 8, stream()));
+	return t;
+}
+
+DTensor contract(const DTensor& lhs, bool lt, const DTensor& rhs, bool rt, size_t modes) {
+	const size_t ld = lhs.degree(), rd = rhs.degree();
+	XTT_REQUIRE(modes <= ld && modes <= rd, "contract: too many modes");
+	dims_t lrem = lt ? sl(lhs.dims, modes, ld) : sl(lhs.dims, 0, ld - modes);
+	dims_t lcon = lt ? sl(lhs.dims, 0, modes) : sl(lhs.dims, ld - modes, ld);
+	dims_t rrem = rt ? sl(rhs.dims, 0, rd - modes) : sl(rhs.dims, modes, rd);
+	dims_t rcon = rt ? sl(rhs.dims, rd - modes, rd) : sl(rhs.dims, 0, modes);
+	XTT_REQUIRE(lcon == rcon, "contract: dimension mismatch");
+	DTensor out(cat(lrem, rrem));
+	const size_t M = prodr(lrem, 0, lrem.size()), N = prodr(rrem, 0, rrem.size()), K = prodr(lcon, 0, lcon.size());
+	if (M * N == 0) return out;
+	if (K == 0) { XTT_HIP(hipMemsetAsync(out.data(), 0, out.size() * 8, stream())); return out; }
+	gemm_f64(gemm_desc(lt, rt, M, N, K), lhs.data(), rhs.data(), out.data(), stream());
+	return out;
+}
+
+DTensor reshuffle(const DTensor& t, const std::vector<size_t>& shuffle) {
+	XTT_REQUIRE(shuffle.size() == t.degree(), "reshuffle: wrong permutation size");
+	dims_t nd(t.degree());
+	std::vector<bool> seen(t.degree(), false);
+	for (size_t i = 0; i < t.degree(); ++i) {
+		XTT_REQUIRE(shuffle[i] < t.degree() && !seen[shuffle[i]], "reshuffle: not a permutation");
+		seen[shuffle[i]] = true;
+		nd[shuffle[i]] = t.dims[i];
+	}
+	DTensor o(nd);
+	permute_f64(t.degree(), t.dims.data(), shuffle.data(), t.data(), o.data(), stream());
+	return o;
+}
+
+DTensor transpose2(const DTensor& t, size_t split) {
+	const size_t m = prodr(t.dims, 0, split), n = prodr(t.dims, split, t.degree());
+	DTensor o(cat(sl(t.dims, split, t.degree()), sl(t.dims, 0, split)));
+	const size_t d2[2] = {m, n}, sh[2] = {1, 0};
+	permute_f64(2, d2, sh, t.data(), o.data(), stream());
+	return o;
+}
+
+namespace {
+struct Factor {  // work buffers of one Householder factorisation
+	DBuf<double> a, tau;
+	DBuf<int> jpvt, rank;
+};
+// Factor the m×n matrix view of `in`; returns the rank decided on the device (synchronises).
+int factor(Factor& f, const DTensor& in, size_t m, size_t n, bool pivot) {
+	f.a.resize(m * n);
+	f.tau.resize(std::min(m, n));
+	f.jpvt.resize(n);
+	f.rank.resize(1);
+	XTT_HIP(hipMemcpyAsync(f.a.get(), in.data(), m * n * 8, hipMemcpyDeviceToDevice, stream()));
+	geqp2(f.a.get(), m, n, f.tau.get(), f.jpvt.get(), pivot ? 1 : 0, nullptr, f.rank.get(), stream());
+	if (!pivot) return (int)std::min(m, n);
+	int r = 0;
+	XTT_HIP(hipMemcpyAsync(&r, f.rank.get(), sizeof(int), hipMemcpyDeviceToHost, stream()));
+	sync();
+	return r;
+}
+}  // namespace
+
+// tensor.cpp:1492-1507 + blasLapackWrapper.cpp:374-431
+void calculate_qr(DTensor& Q, DTensor& R, const DTensor& in, size_t split) {
+	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree()), k = std::min(m, n);
+	XTT_REQUIRE(m > 0 && n > 0, "Dimension must be larger than zero");
+	Factor f;
+	factor(f, in, m, n, false);
+	Q = DTensor(cat(sl(in.dims, 0, split), {k}));
+	R = DTensor(cat({k}, sl(in.dims, split, in.degree())));
+	org2r(f.a.get(), m, n, f.tau.get(), Q.data(), k, stream());
+	extract_r(f.a.get(), m, n, R.data(), stream());
+}
+
+// tensor.cpp:1510-1525 + blasLapackWrapper.cpp:442-498, through QR(J·Aᵀ·J) = Q̃R̃ ⇒ A = (J·R̃ᵀ·J)(J·Q̃ᵀ·J).
+void calculate_rq(DTensor& R, DTensor& Q, const DTensor& in, size_t split) {
+	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree()), k = std::min(m, n);
+	XTT_REQUIRE(m > 0 && n > 0, "Dimension must be larger than zero");
+	DTensor B({n, m});
+	revtrans_f64(in.data(), m, n, B.data(), stream());
+	DTensor Qt, Rt;
+	calculate_qr(Qt, Rt, B, 1);  // Qt n×k, Rt k×m
+	R = DTensor(cat(sl(in.dims, 0, split), {k}));
+	Q = DTensor(cat({k}, sl(in.dims, split, in.degree())));
+	revtrans_f64(Rt.data(), k, m, R.data(), stream());
+	revtrans_f64(Qt.data(), n, k, Q.data(), stream());
+}
+
+// tensor.cpp:1528-1545 + blasLapackWrapper.cpp:235-305
+void calculate_qc(DTensor& Q, DTensor& C, const DTensor& in, size_t split) {
+	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree());
+	XTT_REQUIRE(m > 0 && n > 0, "Dimension must be larger than zero");
+	Factor f;
+	const size_t r = (size_t)factor(f, in, m, n, true);
+	Q = DTensor(cat(sl(in.dims, 0, split), {r}));
+	C = DTensor(cat({r}, sl(in.dims, split, in.degree())));
+	org2r(f.a.get(), m, n, f.tau.get(), Q.data(), r, stream());
+	extract_c(f.a.get(), m, n, f.jpvt.get(), f.rank.get(), C.data(), stream());
+}
+
+// tensor.cpp:1548-1565 + blasLapackWrapper.cpp:316-371: pivoted QR of Aᵀ, A = (R·Pᵀ)ᵀ·Qᵀ.
+void calculate_cq(DTensor& C, DTensor& Q, const DTensor& in, size_t split) {
+	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree());
+	XTT_REQUIRE(m > 0 && n > 0, "Dimension must be larger than zero");
+	DTensor At({n, m});
+	{ const size_t d2[2] = {m, n}, sh[2] = {1, 0}; permute_f64(2, d2, sh, in.data(), At.data(), stream()); }
+	DTensor Qt, Ct;
+	calculate_qc(Qt, Ct, At, 1);  // Qt n×r, Ct r×m
+	const size_t r = Ct.dims[0];
+	C = DTensor(cat(sl(in.dims, 0, split), {r}));
+	Q = DTensor(cat({r}, sl(in.dims, split, in.degree())));
+	{ const size_t d2[2] = {r, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, Ct.data(), C.data(), stream()); }
+	{ const size_t d2[2] = {n, r}, sh[2] = {1, 0}; permute_f64(2, d2, sh, Qt.data(), Q.data(), stream()); }
+}
+
+// tensor.cpp:1424-1489 (dense branch); truncation rank = min(k, maxRank) then first j with s_j <= eps·s_0.
+void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTensor& in, size_t split, size_t maxRank, double eps) {
+	XTT_REQUIRE(eps >= 0 && eps < 1, "Epsilon must be fullfill 0 <= _eps < 1.");
+	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree()), k = std::min(m, n);
+	XTT_REQUIRE(m > 0 && n > 0, "Dimension must be larger than zero");
+	DBuf<double> u(m * k), s(k), vt(k * n);
+	const size_t M = std::max(m, n), N = k, np = N + (N & 1);
+	DBuf<double> ws(M * np + np * np + 2 * np);
+	if (m >= n) {
+		jacobi_svd(in.data(), m, n, u.get(), s.get(), vt.get(), ws.get(), stream());
+	} else {  // Aᵀ = U'·S·V'ᵀ  ⇒  A = V'·S·U'ᵀ
+		DBuf<double> at(n * m), u2(n * m), vt2(m * m);
+		{ const size_t d2[2] = {m, n}, sh[2] = {1, 0}; permute_f64(2, d2, sh, in.data(), at.get(), stream()); }
+		jacobi_svd(at.get(), n, m, u2.get(), s.get(), vt2.get(), ws.get(), stream());
+		{ const size_t d2[2] = {m, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, vt2.get(), u.get(), stream()); }
+		{ const size_t d2[2] = {n, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, u2.get(), vt.get(), stream()); }
+	}
+	std::vector<double> sh(k);
+	XTT_HIP(hipMemcpyAsync(sh.data(), s.get(), k * 8, hipMemcpyDeviceToHost, stream()));
+	sync();
+	size_t rank = k;
+	if (maxRank != 0) rank = std::min(rank, maxRank);
+	for (size_t j = 1; j < rank; ++j) if (sh[j] <= eps * sh[0]) { rank = j; break; }
+	U = DTensor(cat(sl(in.dims, 0, split), {rank}));
+	Vt = DTensor(cat({rank}, sl(in.dims, split, in.degree())));
+	XTT_HIP(hipMemcpy2DAsync(U.data(), rank * 8, u.get(), k * 8, rank * 8, m, hipMemcpyDeviceToDevice, stream()));
+	XTT_HIP(hipMemcpyAsync(Vt.data(), vt.get(), rank * n * 8, hipMemcpyDeviceToDevice, stream()));
+	S.assign(sh.begin(), sh.begin() + rank);
+}
+
+DTensor qr_r_only(const DTensor& in, size_t split) {
+	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree()), k = std::min(m, n);
+	Factor f;
+	factor(f, in, m, n, false);
+	DTensor R({k, n});
+	extract_r(f.a.get(), m, n, R.data(), stream());
+	return R;
+}
+
+double dt_dot(const DTensor& a, const DTensor& b) {
+	XTT_REQUIRE(a.size() == b.size(), "dot: size mismatch");
+	Scratch& s = scratch();
+	dot_f64(a.data(), b.data(), a.size(), s.scalar.get(), s.partials.get(), s.counter.get(), stream());
+	double r = 0;
+	XTT_HIP(hipMemcpyAsync(&r, s.scalar.get(), 8, hipMemcpyDeviceToHost, stream()));
+	sync();
+	return r;
+}
+double dt_norm(const DTensor& t) { return std::sqrt(std::max(0.0, dt_dot(t, t))); }
+void dt_scale(DTensor& t, double s) { scale_f64(t.data(), s, t.size(), stream()); }
+
+// ------------------------------------------------------------------------------- TTNetwork
+
+dims_t TT::ranks() const {
+	dims_t r;
+	for (size_t k = 0; k + 1 < cores.size(); ++k) r.push_back(cores[k].dims.back());
+	return r;
+}
+
+void TT::set_component(size_t k, DTensor t) {
+	XTT_REQUIRE(k < order(), "Illegal index in TTNetwork::set_component");
+	XTT_REQUIRE(t.degree() == (op ? 4u : 3u), "Component must have degree N+2");
+	dims[k] = t.dims[1];
+	if (op) dims[order() + k] = t.dims[2];
+	cores[k] = std::move(t);
+	canonicalized = canonicalized && (corePosition == k);
+}
+
+TT tt_zero(const dims_t& dims, bool op) {  // ttNetwork.cpp:57-108
+	TT x;
+	x.op = op;
+	x.dims = dims;
+	const size_t d = dims.size() / (op ? 2 : 1);
+	XTT_REQUIRE(dims.size() % (op ? 2 : 1) == 0, "Illegal degree for TTOperator.");
+	for (size_t n : dims) XTT_REQUIRE(n > 0, "Zero is no valid dimension.");
+	for (size_t k = 0; k < d; ++k) x.cores.push_back(dt_zeros(op ? dims_t{1, dims[k], dims[d + k], 1} : dims_t{1, dims[k], 1}));
+	x.canonicalized = true;
+	x.corePosition = 0;
+	return x;
+}
+
+TT tt_copy(const TT& x) {
+	TT y;
+	y.op = x.op; y.dims = x.dims; y.canonicalized = x.canonicalized; y.corePosition = x.corePosition;
+	for (const auto& c : x.cores) y.cores.push_back(dt_copy(c));
+	return y;
+}
+
+dims_t reduce_to_maximal_ranks(dims_t ranks, const dims_t& dims, bool op) {  // ttNetwork.cpp:369-402
+	const size_t d = dims.size() / (op ? 2 : 1);
+	XTT_REQUIRE(d == ranks.size() + 1, "Invalid number of ranks or dimensions given.");
+	size_t cur = 1;
+	for (size_t i = 0; i + 1 < d; ++i) {
+		cur *= dims[i]; if (op) cur *= dims[d + i];
+		if (cur < ranks[i]) ranks[i] = cur; else cur = ranks[i];
+	}
+	cur = 1;
+	for (size_t i = 1; i < d; ++i) {
+		cur *= dims[d - i]; if (op) cur *= dims[2 * d - i];
+		if (cur < ranks[d - i - 1]) ranks[d - i - 1] = cur; else cur = ranks[d - i - 1];
+	}
+	return ranks;
+}
+
+TT tt_random(const dims_t& dims, const dims_t& ranks, bool op) {  // ttNetwork.h:128-155
+	const size_t d = dims.size() / (op ? 2 : 1);
+	XTT_REQUIRE(ranks.size() + 1 == d, "Non-matching amount of ranks given to TTNetwork::random.");
+	for (size_t r : ranks) XTT_REQUIRE(r > 0, "Trying to construct random TTTensor with rank 0 is illegal.");
+	TT x = tt_zero(dims_t(dims.size(), 1), op);
+	const dims_t target = reduce_to_maximal_ranks(ranks, dims, op);
+	std::vector<double> h;
+	for (size_t i = 0; i < d; ++i) {
+		const size_t l = i == 0 ? 1 : target[i - 1], r = i + 1 == d ? 1 : target[i];
+		const dims_t cd = op ? dims_t{l, dims[i], dims[d + i], r} : dims_t{l, dims[i], r};
+		h.resize(prodr(cd, 0, cd.size()));
+		for (double& v : h) v = rng_normal();
+		x.set_component(i, dt_upload(cd, h.data()));
+		sync();  // h is reused
+	}
+	move_core(x, 0, false);
+	return x;
+}
+
+// tensorNetwork.cpp:821-909 for adjacent components.
+void transfer_core(TT& x, size_t from, size_t to, bool allowRankReduction) {
+	DTensor& F = x.cores[from];
+	DTensor& T = x.cores[to];
+	DTensor Q, R;
+	if (to == from + 1) {
+		if (allowRankReduction) calculate_qc(Q, R, F, F.degree() - 1); else calculate_qr(Q, R, F, F.degree() - 1);
+		F = std::move(Q);
+		T = contract(R, false, T, false, 1);
+	} else {
+		XTT_REQUIRE(to + 1 == from, "transfer_core: components not adjacent");
+		if (allowRankReduction) calculate_cq(R, Q, F, 1); else calculate_rq(R, Q, F, 1);
+		F = std::move(Q);
+		T = contract(T, false, R, false, 1);
+	}
+}
+
+namespace {
+bool exceeds_maximal_ranks(const TT& x) {  // ttNetwork.cpp:349-359
+	for (size_t i = 0; i < x.order(); ++i) {
+		const DTensor& c = x.cores[i];
+		const size_t e = x.ext(i);
+		if (c.dims.front() > e * c.dims.back() || c.dims.back() > e * c.dims.front()) return true;
+	}
+	return false;
+}
+}  // namespace
+
+void move_core(TT& x, size_t pos, bool keepRank) {  // ttNetwork.cpp:581-628
+	const size_t d = x.order();
+	XTT_REQUIRE(pos < d, "Illegal core-position");
+	const bool red = !keepRank;
+	if (x.canonicalized) {
+		for (size_t n = x.corePosition; n < pos; ++n) transfer_core(x, n, n + 1, red);
+		for (size_t n = x.corePosition; n > pos; --n) transfer_core(x, n, n - 1, red);
+	} else {
+		for (size_t n = 0; n < pos; ++n) transfer_core(x, n, n + 1, red);
+		for (size_t n = d; n > pos + 1; --n) transfer_core(x, n - 1, n - 2, red);
+	}
+	while (exceeds_maximal_ranks(x)) {
+		for (size_t n = pos; n > 0; --n) transfer_core(x, n, n - 1, red);
+		for (size_t n = 0; n + 1 < d; ++n) transfer_core(x, n, n + 1, red);
+		for (size_t n = d; n > pos + 1; --n) transfer_core(x, n - 1, n - 2, red);
+	}
+	x.canonicalized = true;
+	x.corePosition = pos;
+}
+
+// tensorNetwork.cpp:678-818 for (from, to) = (k, k-1): the shared edge is from's first and to's last mode.
+void round_edge(TT& x, size_t from, size_t to, size_t maxRank, double eps) {
+	XTT_REQUIRE(to + 1 == from, "round_edge: expects (k, k-1)");
+	DTensor& F = x.cores[from];
+	DTensor& T = x.cores[to];
+	const double r = (double)F.dims[0];
+	std::vector<double> S;
+	if (5.0 * (double)F.size() * (double)T.size() >= 6.0 * r * r * r * r) {
+		DTensor coreA, coreB, Fq, Tq, U, Vt;
+		calculate_cq(coreA, Fq, F, 1);
+		calculate_qc(Tq, coreB, T, T.degree() - 1);
+		DTensor X = contract(coreA, true, coreB, true, 1);
+		calculate_svd(U, S, Vt, X, 1, maxRank, eps);
+		// S·Vt: scale row j of Vt by s_j
+		const size_t rk = S.size(), cols = Vt.size() / std::max<size_t>(rk, 1);
+		for (size_t j = 0; j < rk; ++j) scale_f64(Vt.data() + j * cols, S[j], cols, stream());
+		F = contract(U, true, Fq, false, 1);
+		T = contract(Tq, false, Vt, true, 1);
+	} else {
+		DTensor X = contract(F, true, T, true, 1);
+		DTensor U, Vt;
+		calculate_svd(U, S, Vt, X, F.degree() - 1, maxRank, eps);
+		const size_t rk = S.size(), cols = Vt.size() / std::max<size_t>(rk, 1);
+		for (size_t j = 0; j < rk; ++j) scale_f64(Vt.data() + j * cols, S[j], cols, stream());
+		T = transpose2(Vt, 1);
+		std::vector<size_t> back(U.degree());
+		for (size_t i = 0; i + 1 < U.degree(); ++i) back[i] = i + 1;
+		back[U.degree() - 1] = 0;
+		F = reshuffle(U, back);
+	}
+}
+
+void round(TT& x, const dims_t& maxRanks, double eps) {  // ttNetwork.cpp:643-665
+	const size_t d = x.order();
+	XTT_REQUIRE(eps < 1, "_eps must be smaller than one.");
+	XTT_REQUIRE(maxRanks.size() + 1 == d, "There must be exactly degree/N-1 maxRanks.");
+	for (size_t r : maxRanks) XTT_REQUIRE(r > 0, "Trying to round a TTTensor to rank 0 is not possible.");
+	const bool initCanon = x.canonicalized;
+	const size_t initPos = x.corePosition;
+	move_core(x, d - 1, false);
+	for (size_t i = 0; i + 1 < d; ++i) round_edge(x, d - 1 - i, d - 2 - i, maxRanks[d - 2 - i], eps);
+	x.corePosition = 0;
+	x.canonicalized = true;
+	if (initCanon) move_core(x, initPos, false);
+}
+
+double tt_dot(const TT& a, const TT& b) {
+	XTT_REQUIRE(a.dims == b.dims, "dot: dimensions differ");
+	DTensor E = dt_upload({1, 1}, std::vector<double>{1.0}.data());
+	sync();
+	for (size_t k = 0; k < a.order(); ++k) {
+		const DTensor& A = a.cores[k];
+		const DTensor& B = b.cores[k];
+		const size_t e = a.ext(k);
+		DTensor T({E.dims[0], e * B.dims.back()});
+		gemm_f64(gemm_desc(0, 0, E.dims[0], e * B.dims.back(), E.dims[1]), E.data(), B.data(), T.data(), stream());
+		DTensor En({A.dims.back(), B.dims.back()});
+		gemm_f64(gemm_desc(1, 0, A.dims.back(), B.dims.back(), E.dims[0] * e), A.data(), T.data(), En.data(), stream());
+		E = std::move(En);
+	}
+	double r = 0;
+	XTT_HIP(hipMemcpyAsync(&r, E.data(), 8, hipMemcpyDeviceToHost, stream()));
+	sync();
+	return r;
+}
+
+double tt_frob_norm(const TT& x) {  // ttNetwork.cpp:781-789
+	if (x.canonicalized) return dt_norm(x.cores[x.corePosition]);
+	return std::sqrt(std::max(0.0, tt_dot(x, x)));
+}
+
+void tt_scale(TT& x, double s) { dt_scale(x.cores[x.canonicalized ? x.corePosition : 0], s); }
+
+// ttNetwork.cpp:796-847: block-diagonal cores, first core concatenated horizontally, last vertically.
+TT tt_add(const TT& a, const TT& b) {
+	XTT_REQUIRE(a.dims == b.dims, "The dimensions in TT sum must coincide.");
+	TT x = tt_copy(a);
+	const size_t d = x.order();
+	const bool initCanon = x.canonicalized;
+	const size_t initPos = x.corePosition;
+	if (d <= 1) { axpy_f64(x.cores[0].data(), 1.0, b.cores[0].data(), x.cores[0].size(), stream()); return x; }
+	for (size_t k = 0; k < d; ++k) {
+		const DTensor& A = a.cores[k];
+		const DTensor& B = b.cores[k];
+		const size_t la = A.dims.front(), lb = B.dims.front(), ra = A.dims.back(), rb = B.dims.back(), e = x.ext(k);
+		const size_t L = k == 0 ? 1 : la + lb, R = k + 1 == d ? 1 : ra + rb;
+		dims_t nd = A.dims; nd.front() = L; nd.back() = R;
+		DTensor t = dt_zeros(nd);
+		const size_t lo = k == 0 ? 0 : la, ro = k + 1 == d ? 0 : ra;
+		XTT_HIP(hipMemcpy2DAsync(t.data(), R * 8, A.data(), ra * 8, ra * 8, la * e, hipMemcpyDeviceToDevice, stream()));
+		XTT_HIP(hipMemcpy2DAsync(t.data() + lo * e * R + ro, R * 8, B.data(), rb * 8, rb * 8, lb * e, hipMemcpyDeviceToDevice, stream()));
+		x.set_component(k, std::move(t));
+	}
+	if (initCanon) move_core(x, initPos, false);
+	return x;
+}
+
+// A·x (ttNetwork.cpp:888-967, ttStack.cpp:197-310): Z[(α,β), n, (α',β')] = Σ_m A[α,n,m,α']·x[β,m,β'].
+TT tt_apply(const TT& A, const TT& x) {
+	const size_t d = x.order();
+	XTT_REQUIRE(A.op && A.order() == d, "tt_apply: operator/tensor mismatch");
+	TT z;
+	z.op = false; z.dims = sl(A.dims, 0, d); z.canonicalized = false; z.corePosition = 0;
+	for (size_t k = 0; k < d; ++k) {
+		const DTensor& a = A.cores[k];
+		const DTensor& c = x.cores[k];
+		const size_t p = a.dims[0], n = a.dims[1], m = a.dims[2], q = a.dims[3], r = c.dims[0], s = c.dims[2];
+		XTT_REQUIRE(c.dims[1] == m, "tt_apply: mode mismatch");
+		DTensor ap = reshuffle(a, {0, 1, 3, 2});          // (p, n, q, m)
+		DTensor cp = reshuffle(c, {1, 0, 2});             // (m, r, s)
+		DTensor T = contract(ap, false, cp, false, 1);   // (p, n, q, r, s)
+		DTensor Z = reshuffle(T, {0, 2, 3, 1, 4});        // (p, r, n, q, s)
+		Z.reshape({p * r, n, q * s});
+		z.cores.push_back(std::move(Z));
+	}
+	return z;
+}
+
+// ‖Ax − b‖/‖b‖: TT of Ax − b (rank r_A·r_x + r_b), left-to-right Householder R sweep, norm of the last core
+// (the SPD residual_f of als.cpp:483-486 evaluates frob_norm of the canonicalized TT difference).
+double tt_residual(const TT& A, const TT& x, const TT& b) {
+	TT z = tt_apply(A, x);
+	TT nb = tt_copy(b);
+	nb.canonicalized = false;
+	dt_scale(nb.cores[0], -1.0);
+	TT w = tt_add(z, nb);
+	DTensor R;
+	for (size_t k = 0; k < w.order(); ++k) {
+		DTensor c = k == 0 ? dt_copy(w.cores[0]) : contract(R, false, w.cores[k], false, 1);
+		if (k + 1 == w.order()) {
+			TT bb = tt_copy(b);
+			bb.canonicalized = false;
+			return dt_norm(c) / std::sqrt(tt_dot(bb, bb));
+		}
+		R = qr_r_only(c, c.degree() - 1);
+	}
+	return 0.0;
+}
+
+}  // namespace xtt

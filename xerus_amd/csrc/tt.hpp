@@ -1,0 +1,90 @@
+// Device-resident TT tensors / operators and the Tensor-level operations xerus's TT path is built from.
+// All FP64; cores stay in HBM, only integer rank decisions and scalar norms cross to the host.
+#pragma once
+#include <functional>
+#include <vector>
+
+#include "kernels.hpp"
+#include "xtt_common.hpp"
+
+namespace xtt {
+
+using dims_t = std::vector<size_t>;
+
+// ---- Tensor level (tensor.cpp / blasLapackWrapper.cpp equivalents) ------------------------------------
+DTensor dt_upload(const dims_t& dims, const double* host);
+void dt_download(const DTensor& t, double* host);
+DTensor dt_copy(const DTensor& t);
+DTensor dt_zeros(const dims_t& dims);
+// xerus::contract (tensor.cpp:1252-1352): lhsTrans ⇒ lhs's first `modes` contracted, rhsTrans ⇒ rhs's last.
+DTensor contract(const DTensor& lhs, bool lhsTrans, const DTensor& rhs, bool rhsTrans, size_t modes);
+DTensor reshuffle(const DTensor& t, const std::vector<size_t>& shuffle);  // shuffle[i] = new position of mode i
+DTensor transpose2(const DTensor& t, size_t split);                       // (rows…, cols…) → (cols…, rows…)
+void calculate_qr(DTensor& Q, DTensor& R, const DTensor& in, size_t split);
+void calculate_rq(DTensor& R, DTensor& Q, const DTensor& in, size_t split);
+void calculate_qc(DTensor& Q, DTensor& C, const DTensor& in, size_t split);
+void calculate_cq(DTensor& C, DTensor& Q, const DTensor& in, size_t split);
+// Thin SVD with xerus's truncation (tensor.cpp:1463-1474); S returned as host singular values (kept ones).
+void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTensor& in, size_t split, size_t maxRank, double eps);
+// R factor only (k×n) of a Householder QR of the m×n matrix view (used by the stable residual).
+DTensor qr_r_only(const DTensor& in, size_t split);
+double dt_norm(const DTensor& t);
+double dt_dot(const DTensor& a, const DTensor& b);
+void dt_scale(DTensor& t, double s);
+
+// ---- TTNetwork ---------------------------------------------------------------------------------------
+struct TT {
+	bool op = false;
+	dims_t dims;  // d (tensor) or 2d (operator: row dims then column dims)
+	std::vector<DTensor> cores;
+	bool canonicalized = true;
+	size_t corePosition = 0;
+	size_t order() const { return cores.size(); }
+	size_t ext(size_t k) const { return op ? dims[k] * dims[order() + k] : dims[k]; }
+	size_t rank(size_t i) const { return cores[i].dims.back(); }
+	dims_t ranks() const;
+	void set_component(size_t k, DTensor t);  // ttNetwork.cpp:475-492
+};
+TT tt_zero(const dims_t& dims, bool op);
+TT tt_copy(const TT& x);
+dims_t reduce_to_maximal_ranks(dims_t ranks, const dims_t& dims, bool op);
+TT tt_random(const dims_t& dims, const dims_t& ranks, bool op);  // host RNG draws as xerus, then move_core(0)
+void transfer_core(TT& x, size_t from, size_t to, bool allowRankReduction);
+void move_core(TT& x, size_t pos, bool keepRank);
+void round_edge(TT& x, size_t from, size_t to, size_t maxRank, double eps);
+void round(TT& x, const dims_t& maxRanks, double eps);
+double tt_frob_norm(const TT& x);
+double tt_dot(const TT& a, const TT& b);
+TT tt_add(const TT& a, const TT& b);
+void tt_scale(TT& x, double s);
+TT tt_apply(const TT& A, const TT& x);   // rank r_A·r_x, uncanonicalized
+double tt_residual(const TT& A, const TT& x, const TT& b);  // ‖Ax − b‖/‖b‖, stable (QR sweep of the TT of Ax − b)
+
+// Shared RNG mirroring xerus::misc::randomEngine / defaultNormalDistribution (misc/random.cpp:29-30).
+void rng_seed(uint64_t s);
+double rng_normal();
+size_t rng_uniform(size_t hi);  // uniform_int_distribution<size_t>(0, hi)
+
+// ---- ALS --------------------------------------------------------------------------------------------
+enum Precision { P_F64 = 0, P_F32 = 1, P_BF16 = 2 };
+struct ALSOptions {
+	unsigned sites = 1;
+	size_t numHalfSweeps = 0;
+	double convergenceEpsilon = 1e-6;
+	bool useResidualForEndCriterion = false;
+	bool preserveCorePosition = true;
+	bool assumeSPD = true;
+	int precision = P_F64;      // arithmetic of the local matvecs
+	double localTol = 1e-13;    // relative local residual target of the local solve
+	int maxLocalIter = 2000;    // CG iterations per local solve (per refinement step in mixed precision)
+	double innerTol = 0;        // mixed precision: inner relative tolerance (0: 1e-5 FP32, 3e-2 BF16)
+	int maxRefine = 30;         // mixed precision: FP64 refinement steps
+};
+struct ALSStats {
+	size_t halfSweeps = 0, localSolves = 0, cgIterations = 0, refinements = 0, matvecs64 = 0, matvecsLow = 0;
+	double flopsMatvec = 0, flopsEnv = 0;  // algorithmic flop counts
+	std::vector<double> energies;
+};
+double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats* stats);
+
+}  // namespace xtt

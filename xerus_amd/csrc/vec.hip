@@ -1,0 +1,293 @@
+// Element-wise, permutation and reduction kernels of libxtt, plus the device-resident CG step kernels.
+//
+// Reductions are deterministic: a fixed grid writes one partial per block, and the last block to finish
+// (ticket from an agent-scope atomic, after an agent-scope fence) sums the partials in index order. No
+// float atomics, so repeated runs are bit-identical.
+#include "kernels.hpp"
+#include "xtt_common.hpp"
+
+namespace xtt {
+namespace {
+
+constexpr int RB = 256;  // threads per reduction block
+
+struct PermArgs {
+	int deg;
+	long long n;
+	long long inStride[8];   // stride of old mode i in the input
+	long long outStride[8];  // stride (in the output) of old mode i
+	long long dims[8];
+};
+
+__global__ void permute_kernel(PermArgs a, const double* __restrict__ in, double* __restrict__ out) {
+	for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < a.n; e += (long long)gridDim.x * blockDim.x) {
+		long long rem = e, o = 0;
+		for (int i = a.deg - 1; i >= 0; --i) {
+			const long long idx = rem % a.dims[i];
+			rem /= a.dims[i];
+			o += idx * a.outStride[i];
+		}
+		out[o] = in[e];
+	}
+}
+
+__global__ void convert_kernel_64_32(const double* in, float* out, size_t n) {
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = (float)in[i];
+}
+__global__ void convert_kernel_32_64(const float* in, double* out, size_t n) {
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = (double)in[i];
+}
+__global__ void fill_kernel(double* p, double v, size_t n) {
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void scale_kernel(double* p, double a, size_t n) {
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] *= a;
+}
+__global__ void axpy_kernel(double* y, double a, const double* x, size_t n) {
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] += a * x[i];
+}
+
+// out (cols×rows) = J·inᵀ·J for a rows×cols input: out[i][j] = in[rows-1-j][cols-1-i].
+__global__ void revtrans_kernel(const double* in, size_t rows, size_t cols, double* out) {
+	const size_t n = rows * cols;
+	for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+		const size_t i = e / rows, j = e % rows;
+		out[e] = in[(rows - 1 - j) * cols + (cols - 1 - i)];
+	}
+}
+
+unsigned grid_for(size_t n, unsigned block = 256) {
+	size_t g = (n + block - 1) / block;
+	if (g > 2048) g = 2048;
+	if (g == 0) g = 1;
+	return (unsigned)g;
+}
+
+// ---- block reductions (two running sums) ---------------------------------------------------------------
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+	return v;
+}
+
+// Sum over the block; result valid in every thread. `ws` needs RB/64 * 2 doubles.
+__device__ inline void block_sum2(double& a, double& b, double* ws) {
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	a = wave_sum(a);
+	b = wave_sum(b);
+	__syncthreads();
+	if (lane == 0) { ws[2 * wave] = a; ws[2 * wave + 1] = b; }
+	__syncthreads();
+	a = 0; b = 0;
+	for (int w = 0; w < RB / 64; ++w) { a += ws[2 * w]; b += ws[2 * w + 1]; }
+	__syncthreads();
+}
+
+// Publishes this block's (a, b) into partials[2*block..] and returns true in every thread of the block that
+// finishes last; that block then holds the in-order totals in (a, b).
+__device__ inline bool last_block_sum2(double& a, double& b, double* partials, unsigned* counter, double* ws) {
+	__shared__ int amLast;
+	block_sum2(a, b, ws);
+	if (threadIdx.x == 0) {
+		partials[2 * blockIdx.x] = a;
+		partials[2 * blockIdx.x + 1] = b;
+		__threadfence();
+		const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+		amLast = (t == gridDim.x - 1);
+	}
+	__syncthreads();
+	if (!amLast) return false;
+	__threadfence();
+	a = 0; b = 0;
+	for (unsigned i = threadIdx.x; i < gridDim.x; i += RB) {
+		a += __hip_atomic_load(partials + 2 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		b += __hip_atomic_load(partials + 2 * i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	}
+	block_sum2(a, b, ws);
+	if (threadIdx.x == 0) *counter = 0;
+	return true;
+}
+
+template <class T>
+__global__ __launch_bounds__(RB) void dot_kernel(const T* x, const T* y, size_t n, double* out, double* partials, unsigned* counter) {
+	__shared__ double ws[RB / 64 * 2];
+	double a = 0, b = 0;
+	for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) a += (double)x[i] * (double)y[i];
+	if (last_block_sum2(a, b, partials, counter, ws) && threadIdx.x == 0) *out = a;
+}
+
+// ---- CG steps ------------------------------------------------------------------------------------------
+
+template <class T>
+__global__ __launch_bounds__(RB) void cg_start_kernel(T* r, T* p, const T* b, const T* q, size_t n, CGState* st, double* partials) {
+	__shared__ double ws[RB / 64 * 2];
+	double rr = 0, bb = 0;
+	for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+		const T ri = b[i] - q[i];
+		r[i] = ri;
+		p[i] = ri;
+		rr += (double)ri * (double)ri;
+		bb += (double)b[i] * (double)b[i];
+	}
+	if (last_block_sum2(rr, bb, partials, &st->counter, ws) && threadIdx.x == 0) {
+		st->rr = rr;
+		st->rr0 = bb;
+		st->iter = 0;
+		st->done = (rr <= st->tol2 * bb || st->maxIter <= 0) ? 1 : 0;
+	}
+}
+
+template <class T>
+__global__ __launch_bounds__(RB) void cg_alpha_kernel(const T* p, const T* q, size_t n, CGState* st, double* partials) {
+	if (st->done) return;
+	__shared__ double ws[RB / 64 * 2];
+	double pq = 0, z = 0;
+	for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) pq += (double)p[i] * (double)q[i];
+	if (last_block_sum2(pq, z, partials, &st->counter, ws) && threadIdx.x == 0) {
+		st->pq = pq;
+		if (pq > 0) {
+			st->alpha = st->rr / pq;
+		} else {  // breakdown (non-positive curvature or exact solution): stop where we are
+			st->alpha = 0;
+			st->done = 1;
+		}
+	}
+}
+
+template <class T>
+__global__ __launch_bounds__(RB) void cg_update_kernel(T* x, T* r, const T* p, const T* q, size_t n, CGState* st, double* partials) {
+	if (st->done) return;
+	__shared__ double ws[RB / 64 * 2];
+	const T alpha = (T)st->alpha;
+	double rr = 0, z = 0;
+	for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+		x[i] += alpha * p[i];
+		const T ri = r[i] - alpha * q[i];
+		r[i] = ri;
+		rr += (double)ri * (double)ri;
+	}
+	if (last_block_sum2(rr, z, partials, &st->counter, ws) && threadIdx.x == 0) {
+		st->beta = rr / st->rr;
+		st->rr = rr;
+		st->iter += 1;
+		if (rr <= st->tol2 * st->rr0 || st->iter >= st->maxIter) st->done = 1;
+	}
+}
+
+template <class T>
+__global__ __launch_bounds__(RB) void cg_direction_kernel(T* p, const T* r, size_t n, CGState* st) {
+	if (st->done) return;
+	const T beta = (T)st->beta;
+	for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) p[i] = r[i] + beta * p[i];
+}
+
+__global__ void cg_state_init_kernel(CGState* st, double tol2, int maxIter) {
+	if (threadIdx.x == 0 && blockIdx.x == 0) {
+		st->rr = 0; st->rr0 = 0; st->tol2 = tol2; st->alpha = 0; st->beta = 0; st->pq = 0;
+		st->done = 0; st->iter = 0; st->maxIter = maxIter; st->pad = 0; st->counter = 0; st->pad2 = 0;
+	}
+}
+
+}  // namespace
+
+size_t reduce_blocks(size_t n) {
+	size_t g = (n + RB * 4 - 1) / (RB * 4);
+	if (g > 512) g = 512;
+	if (g == 0) g = 1;
+	return g;
+}
+
+void permute_f64(size_t degree, const size_t* dims, const size_t* shuffle, const double* in, double* out, hipStream_t s) {
+	XTT_REQUIRE(degree <= 8, "permute: at most 8 modes");
+	PermArgs a{};
+	a.deg = (int)degree;
+	long long n = 1;
+	for (size_t i = 0; i < degree; ++i) { a.dims[i] = (long long)dims[i]; n *= (long long)dims[i]; }
+	a.n = n;
+	if (n == 0) return;
+	long long newDims[8];
+	for (size_t i = 0; i < degree; ++i) { XTT_REQUIRE(shuffle[i] < degree, "permute: bad shuffle"); newDims[shuffle[i]] = (long long)dims[i]; }
+	long long newStride[8];
+	long long st = 1;
+	for (size_t j = degree; j-- > 0;) { newStride[j] = st; st *= newDims[j]; }
+	for (size_t i = 0; i < degree; ++i) a.outStride[i] = newStride[shuffle[i]];
+	hipLaunchKernelGGL(permute_kernel, dim3(grid_for((size_t)n)), dim3(256), 0, s, a, in, out);
+	XTT_LAUNCH_CHECK();
+}
+
+void revtrans_f64(const double* in, size_t rows, size_t cols, double* out, hipStream_t s) {
+	if (!rows || !cols) return;
+	hipLaunchKernelGGL(revtrans_kernel, dim3(grid_for(rows * cols)), dim3(256), 0, s, in, rows, cols, out);
+	XTT_LAUNCH_CHECK();
+}
+
+void convert_f64_f32(const double* in, float* out, size_t n, hipStream_t s) {
+	if (!n) return;
+	hipLaunchKernelGGL(convert_kernel_64_32, dim3(grid_for(n)), dim3(256), 0, s, in, out, n);
+	XTT_LAUNCH_CHECK();
+}
+void convert_f32_f64(const float* in, double* out, size_t n, hipStream_t s) {
+	if (!n) return;
+	hipLaunchKernelGGL(convert_kernel_32_64, dim3(grid_for(n)), dim3(256), 0, s, in, out, n);
+	XTT_LAUNCH_CHECK();
+}
+void fill_f64(double* p, double v, size_t n, hipStream_t s) {
+	if (!n) return;
+	hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, s, p, v, n);
+	XTT_LAUNCH_CHECK();
+}
+void scale_f64(double* p, double a, size_t n, hipStream_t s) {
+	if (!n) return;
+	hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, s, p, a, n);
+	XTT_LAUNCH_CHECK();
+}
+void axpy_f64(double* y, double a, const double* x, size_t n, hipStream_t s) {
+	if (!n) return;
+	hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(n)), dim3(256), 0, s, y, a, x, n);
+	XTT_LAUNCH_CHECK();
+}
+void dot_f64(const double* x, const double* y, size_t n, double* out, double* partials, unsigned* counter, hipStream_t s) {
+	hipLaunchKernelGGL(dot_kernel<double>, dim3((unsigned)reduce_blocks(n)), dim3(RB), 0, s, x, y, n, out, partials, counter);
+	XTT_LAUNCH_CHECK();
+}
+void dot_f32_acc64(const float* x, const float* y, size_t n, double* out, double* partials, unsigned* counter, hipStream_t s) {
+	hipLaunchKernelGGL(dot_kernel<float>, dim3((unsigned)reduce_blocks(n)), dim3(RB), 0, s, x, y, n, out, partials, counter);
+	XTT_LAUNCH_CHECK();
+}
+
+template <class T>
+void cg_start(T* r, T* p, const T* b, const T* q, size_t n, CGState* st, double* partials, hipStream_t s) {
+	hipLaunchKernelGGL(cg_start_kernel<T>, dim3((unsigned)reduce_blocks(n)), dim3(RB), 0, s, r, p, b, q, n, st, partials);
+	XTT_LAUNCH_CHECK();
+}
+template <class T>
+void cg_alpha(const T* p, const T* q, size_t n, CGState* st, double* partials, hipStream_t s) {
+	hipLaunchKernelGGL(cg_alpha_kernel<T>, dim3((unsigned)reduce_blocks(n)), dim3(RB), 0, s, p, q, n, st, partials);
+	XTT_LAUNCH_CHECK();
+}
+template <class T>
+void cg_update(T* x, T* r, const T* p, const T* q, size_t n, CGState* st, double* partials, hipStream_t s) {
+	hipLaunchKernelGGL(cg_update_kernel<T>, dim3((unsigned)reduce_blocks(n)), dim3(RB), 0, s, x, r, p, q, n, st, partials);
+	XTT_LAUNCH_CHECK();
+}
+template <class T>
+void cg_direction(T* p, const T* r, size_t n, CGState* st, hipStream_t s) {
+	hipLaunchKernelGGL(cg_direction_kernel<T>, dim3((unsigned)reduce_blocks(n)), dim3(RB), 0, s, p, r, n, st);
+	XTT_LAUNCH_CHECK();
+}
+void cg_state_init(CGState* st, double tol2, int maxIter, hipStream_t s) {
+	hipLaunchKernelGGL(cg_state_init_kernel, dim3(1), dim3(64), 0, s, st, tol2, maxIter);
+	XTT_LAUNCH_CHECK();
+}
+
+template void cg_start<double>(double*, double*, const double*, const double*, size_t, CGState*, double*, hipStream_t);
+template void cg_start<float>(float*, float*, const float*, const float*, size_t, CGState*, double*, hipStream_t);
+template void cg_alpha<double>(const double*, const double*, size_t, CGState*, double*, hipStream_t);
+template void cg_alpha<float>(const float*, const float*, size_t, CGState*, double*, hipStream_t);
+template void cg_update<double>(double*, double*, const double*, const double*, size_t, CGState*, double*, hipStream_t);
+template void cg_update<float>(float*, float*, const float*, const float*, size_t, CGState*, double*, hipStream_t);
+template void cg_direction<double>(double*, const double*, size_t, CGState*, hipStream_t);
+template void cg_direction<float>(float*, const float*, size_t, CGState*, hipStream_t);
+
+}  // namespace xtt

@@ -1,0 +1,99 @@
+// Shared runtime pieces of libxtt: status handling, the library stream, a caching device allocator and the
+// device tensor type used by the TT layer. Everything here is host code compiled by hipcc for gfx950.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace xtt {
+
+enum Status : int { OK = 0, ERR_INVALID = 1, ERR_HIP = 2, ERR_NUMERIC = 3, ERR_NOMEM = 4, ERR_UNSUPPORTED = 5 };
+
+struct Error : std::runtime_error {
+	int code;
+	Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define XTT_REQUIRE(cond, msg)                                                                          \
+	do {                                                                                                \
+		if (!(cond)) throw ::xtt::Error(::xtt::ERR_INVALID, std::string(msg));                          \
+	} while (0)
+
+#define XTT_HIP(call)                                                                                   \
+	do {                                                                                                \
+		hipError_t e_ = (call);                                                                         \
+		if (e_ != hipSuccess)                                                                           \
+			throw ::xtt::Error(e_ == hipErrorOutOfMemory ? ::xtt::ERR_NOMEM : ::xtt::ERR_HIP,           \
+			                   std::string(#call) + ": " + hipGetErrorString(e_));                     \
+	} while (0)
+
+// Kernel launch check: a launch error surfaces on the next runtime call, so query it right away.
+#define XTT_LAUNCH_CHECK() XTT_HIP(hipGetLastError())
+
+void set_error(const std::string& s);
+
+// Optional launch profiler: while enabled, instrumented launches record a HIP event pair on their own stream,
+// tagged with the kernel instantiation and its algorithmic flop count. Read back after a synchronisation.
+struct ProfRecord { std::string tag; size_t launches = 0; double ms = 0, flops = 0; };
+bool prof_enabled();
+void prof_enable(bool on);
+int prof_begin(hipStream_t s);                                     // returns a slot (or -1 when disabled)
+void prof_end(int slot, hipStream_t s, const char* tag, double flops);
+std::vector<ProfRecord> prof_collect();                            // synchronises, aggregates, clears
+
+// The library's stream on the current device (default priority, blocking: it orders with the legacy null
+// stream torch uses), created on first use.
+hipStream_t stream();
+// Stream used by an entry point: the caller's if given, else the library stream.
+inline hipStream_t pick(void* s) { return s ? static_cast<hipStream_t>(s) : stream(); }
+void sync();
+
+// Caching allocator: freed blocks are kept per size class and reused, so ALS / round / ADF loops do not call
+// hipMalloc once warm. Blocks are 256-byte aligned (hipMalloc's alignment).
+void* dmalloc(size_t bytes);
+void dfree(void* p);
+void release_cache();
+size_t bytes_in_use();
+
+template <class T>
+struct DBuf {  // owning device buffer of T
+	T* p = nullptr;
+	size_t n = 0;
+	DBuf() = default;
+	explicit DBuf(size_t count) { resize(count); }
+	DBuf(const DBuf&) = delete;
+	DBuf& operator=(const DBuf&) = delete;
+	DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+	DBuf& operator=(DBuf&& o) noexcept { if (this != &o) { reset(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; } return *this; }
+	~DBuf() { reset(); }
+	void reset() { if (p) dfree(p); p = nullptr; n = 0; }
+	void resize(size_t count) {
+		if (count == n && p) return;
+		reset();
+		if (count) { p = static_cast<T*>(dmalloc(count * sizeof(T))); n = count; }
+	}
+	T* get() const { return p; }
+	size_t size() const { return n; }
+};
+
+// Dense row-major FP64 tensor resident on the device (xerus::Tensor's dense representation,
+// include/xerus/tensor.h:84-123, without the lazy scalar factor).
+struct DTensor {
+	std::vector<size_t> dims;
+	DBuf<double> buf;
+	DTensor() = default;
+	explicit DTensor(std::vector<size_t> d) : dims(std::move(d)) { buf.resize(count()); }
+	size_t count() const { size_t p = 1; for (size_t x : dims) p *= x; return p; }
+	size_t size() const { return buf.size(); }
+	size_t degree() const { return dims.size(); }
+	double* data() const { return buf.get(); }
+	void reshape(std::vector<size_t> d);  // same element count
+};
+
+}  // namespace xtt

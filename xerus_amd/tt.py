@@ -1,0 +1,460 @@
+"""xerus-compatible Python surface of the TT path, backed by libxtt (device-resident cores on the MI355X).
+
+Names and argument meaning follow xerus's Python module (src/xerus/python/ttnetwork.cpp:32-165,
+leastSquares.cpp:97-138, recovery.cpp:31-160): ``TTTensor``/``TTOperator`` with ``random``,
+``get_component``/``set_component``, ``ranks``, ``move_core``, ``round``, ``frob_norm``; the ``ALSVariant``
+presets ``ALS``, ``ALS_SPD``, ``DMRG``, ``DMRG_SPD``; ``SinglePointMeasurementSet``; ``ADFVariant`` / ``ADF``.
+Errors raise :class:`XttError` (xerus throws ``generic_error``).
+"""
+from __future__ import annotations
+
+import copy as _copy
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import XT_BF16, XT_F32, XT_F64, XttError, call
+
+EPSILON = 8 * np.finfo(float).eps  # include/xerus/basic.h:50
+_SIZE_MAX = np.iinfo(np.uintp).max
+
+
+def _sizes(seq):
+    a = np.ascontiguousarray(list(seq), dtype=np.uintp)
+    return a, a.ctypes.data_as(C.POINTER(C.c_size_t))
+
+
+def seed(s: int) -> None:
+    """misc::randomEngine.seed(s) (the seed the reference's tests use is 0xBAADF00D)."""
+    call("xt_seed", int(s))
+
+
+def normal_draws(n: int) -> np.ndarray:
+    out = np.empty(n)
+    call("xt_normal_draws", n, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def uniform_draws(n: int, hi: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.uintp)
+    call("xt_uniform_draws", n, hi, out.ctypes.data_as(C.POINTER(C.c_size_t)))
+    return out
+
+
+class _TTNetwork:
+    _is_operator = False
+
+    def __init__(self, arg=None, *, _handle=None):
+        if _handle is not None:
+            self._h = _handle
+            return
+        if arg is None:
+            arg = [1]
+        if isinstance(arg, (int, np.integer)):
+            arg = [1] * int(arg)
+        dims = list(arg)
+        _, p = _sizes(dims)
+        h = C.c_void_p()
+        call("xt_tt_new", int(self._is_operator), len(dims), p, C.byref(h))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib._lib is not None:
+            _lib._lib.xt_tt_free(h)
+            self._h = None
+
+    # ------------------------------------------------------------------ construction
+    @classmethod
+    def random(cls, dims, ranks):
+        """TTNetwork::random(dims, ranks) (ttNetwork.h:128-155): N(0,1) cores from the shared engine."""
+        dims = list(dims)
+        if isinstance(ranks, (int, np.integer)):
+            ranks = [int(ranks)] * (len(dims) // (2 if cls._is_operator else 1) - 1)
+        _, pd = _sizes(dims)
+        _, pr = _sizes(list(ranks) or [0])
+        h = C.c_void_p()
+        call("xt_tt_random", int(cls._is_operator), len(dims), pd, pr, C.byref(h))
+        return cls(_handle=h)
+
+    @classmethod
+    def from_components(cls, cores, core_position=None):
+        """Build from a list of component arrays; canonical at `core_position` if given (else not canonical)."""
+        cores = [np.asarray(c, dtype=np.float64) for c in cores]
+        if cls._is_operator:
+            dims = [c.shape[1] for c in cores] + [c.shape[2] for c in cores]
+        else:
+            dims = [c.shape[1] for c in cores]
+        t = cls(dims)
+        for k, c in enumerate(cores):
+            t.set_component(k, c)
+        if core_position is not None:
+            t.assume_core_position(core_position)
+        return t
+
+    def copy(self):
+        h = C.c_void_p()
+        call("xt_tt_copy", self._h, C.byref(h))
+        return type(self)(_handle=h)
+
+    __copy__ = copy
+
+    def __deepcopy__(self, memo):
+        return self.copy()
+
+    # ------------------------------------------------------------------ structure
+    def order(self) -> int:
+        o = C.c_size_t()
+        call("xt_tt_order", self._h, C.byref(o))
+        return o.value
+
+    def degree(self) -> int:
+        return self.order() * (2 if self._is_operator else 1)
+
+    @property
+    def dimensions(self):
+        out = np.empty(self.degree(), dtype=np.uintp)
+        call("xt_tt_dims", self._h, out.ctypes.data_as(C.POINTER(C.c_size_t)))
+        return [int(v) for v in out]
+
+    def ranks(self):
+        d = self.order()
+        out = np.zeros(max(d - 1, 1), dtype=np.uintp)
+        call("xt_tt_ranks", self._h, out.ctypes.data_as(C.POINTER(C.c_size_t)))
+        return [int(v) for v in out[: d - 1]]
+
+    def rank(self, i: int) -> int:
+        return self.ranks()[i]
+
+    def component_dims(self, k):
+        out = np.empty(4 if self._is_operator else 3, dtype=np.uintp)
+        call("xt_tt_component_dims", self._h, k, out.ctypes.data_as(C.POINTER(C.c_size_t)))
+        return tuple(int(v) for v in out)
+
+    def get_component(self, k: int) -> np.ndarray:
+        out = np.empty(self.component_dims(k))
+        call("xt_tt_get_component", self._h, k, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def set_component(self, k: int, T) -> None:
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        _, p = _sizes(T.shape)
+        call("xt_tt_set_component", self._h, k, T.ndim, p, T.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def component_ptr(self, k: int) -> int:
+        p = C.c_void_p()
+        call("xt_tt_component_ptr", self._h, k, C.byref(p))
+        return p.value
+
+    @property
+    def canonicalized(self) -> bool:
+        return self._state()[0]
+
+    @property
+    def corePosition(self) -> int:
+        return self._state()[1]
+
+    def _state(self):
+        c, p = C.c_int(), C.c_size_t()
+        call("xt_tt_canonicalization", self._h, C.byref(c), C.byref(p))
+        return bool(c.value), p.value
+
+    def assume_core_position(self, pos: int) -> None:
+        call("xt_tt_assume_core_position", self._h, pos)
+
+    # ------------------------------------------------------------------ TT algorithms
+    def move_core(self, position: int, keepRank: bool = False) -> None:
+        """TTNetwork::move_core (ttNetwork.cpp:581-628)."""
+        call("xt_tt_move_core", self._h, position, int(keepRank))
+
+    def canonicalize_left(self) -> None:
+        self.move_core(0)
+
+    def canonicalize_right(self) -> None:
+        self.move_core(self.order() - 1)
+
+    def round(self, arg=None, eps: float | None = None) -> None:
+        """TTNetwork::round (ttNetwork.cpp:643-684): round(eps: float), round(maxRank: int) or
+        round(maxRanks: list[, eps])."""
+        d = self.order()
+        if isinstance(arg, float) and eps is None:
+            ranks, e = [_SIZE_MAX] * (d - 1), arg
+        elif isinstance(arg, (int, np.integer)):
+            if int(arg) <= 0:
+                raise XttError(_lib.XT_ERR_INVALID, "MaxRank must be positive")
+            ranks, e = [int(arg)] * (d - 1), EPSILON if eps is None else eps
+        else:
+            ranks = list(arg) if arg is not None else [_SIZE_MAX] * (d - 1)
+            e = EPSILON if eps is None else eps
+        _, p = _sizes(ranks or [0])
+        call("xt_tt_round", self._h, p, float(e))
+
+    def frob_norm(self) -> float:
+        out = C.c_double()
+        call("xt_tt_frob_norm", self._h, C.byref(out))
+        return out.value
+
+    def dot(self, other) -> float:
+        out = C.c_double()
+        call("xt_tt_dot", self._h, other._h, C.byref(out))
+        return out.value
+
+    def full(self) -> np.ndarray:
+        dims = self.dimensions
+        out = np.empty(int(np.prod(dims)))
+        call("xt_tt_full", self._h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out.reshape(dims)
+
+    # ------------------------------------------------------------------ arithmetic
+    def __add__(self, other):
+        h = C.c_void_p()
+        call("xt_tt_add", self._h, other._h, C.byref(h))
+        return type(self)(_handle=h)
+
+    def __sub__(self, other):
+        return self + other * -1.0
+
+    def __mul__(self, s):
+        if isinstance(s, (int, float, np.floating, np.integer)):
+            t = self.copy()
+            call("xt_tt_scale", t._h, float(s))
+            return t
+        return NotImplemented
+
+    __rmul__ = __mul__
+
+    def __truediv__(self, s):
+        return self * (1.0 / float(s))
+
+    @staticmethod
+    def reduce_to_maximal_ranks(ranks, dims, is_operator=False):
+        """ttNetwork.cpp:369-402."""
+        ranks = list(ranks)
+        d = len(dims) // (2 if is_operator else 1)
+        cur = 1
+        for i in range(d - 1):
+            cur *= dims[i] * (dims[d + i] if is_operator else 1)
+            if cur < ranks[i]:
+                ranks[i] = cur
+            else:
+                cur = ranks[i]
+        cur = 1
+        for i in range(1, d):
+            cur *= dims[d - i] * (dims[2 * d - i] if is_operator else 1)
+            if cur < ranks[d - i - 1]:
+                ranks[d - i - 1] = cur
+            else:
+                cur = ranks[d - i - 1]
+        return ranks
+
+
+class TTTensor(_TTNetwork):
+    _is_operator = False
+
+
+class TTOperator(_TTNetwork):
+    _is_operator = True
+
+    def __call__(self, x: TTTensor) -> TTTensor:
+        """A·x as a TT of rank r_A·r_x (ttStack.cpp:197-310), not canonicalized."""
+        h = C.c_void_p()
+        call("xt_tt_apply", self._h, x._h, C.byref(h))
+        return TTTensor(_handle=h)
+
+
+def frob_norm(t) -> float:
+    return t.frob_norm()
+
+
+def residual(A: TTOperator, x: TTTensor, b: TTTensor) -> float:
+    """‖A·x − b‖/‖b‖ computed stably on the device (TT of A·x − b, Householder R sweep)."""
+    out = C.c_double()
+    call("xt_tt_residual", A._h, x._h, b._h, C.byref(out))
+    return out.value
+
+
+# ---------------------------------------------------------------------- ALS
+
+
+class ALSVariant:
+    """ALSVariant (als.h:37-223). Extra fields choose the local solve on the device: ``precision``
+    (XT_F64 / XT_F32 / XT_BF16 local matvecs, the latter two with FP64 refinement), ``localTolerance``,
+    ``maxLocalIterations``, ``innerTolerance``, ``maxRefinements``."""
+
+    def __init__(self, sites: int, numHalfSweeps: int, assumeSPD: bool, useResidual: bool = False):
+        if sites <= 0:
+            raise XttError(_lib.XT_ERR_INVALID, "sites must be > 0")
+        o = _lib.XtALSOptions()
+        call("xt_als_default_options", C.byref(o), sites, int(assumeSPD))
+        self.sites = sites
+        self.numHalfSweeps = numHalfSweeps
+        self.convergenceEpsilon = o.convergenceEpsilon
+        self.useResidualForEndCriterion = useResidual
+        self.preserveCorePosition = True
+        self.assumeSPD = assumeSPD
+        self.precision = XT_F64
+        self.localTolerance = o.localTolerance
+        self.maxLocalIterations = o.maxLocalIterations
+        self.innerTolerance = 0.0
+        self.maxRefinements = o.maxRefinements
+        self.last_stats = None
+
+    def copy(self):
+        return _copy.copy(self)
+
+    def _options(self):
+        o = _lib.XtALSOptions()
+        o.sites = self.sites
+        o.numHalfSweeps = self.numHalfSweeps
+        o.convergenceEpsilon = self.convergenceEpsilon
+        o.useResidualForEndCriterion = int(self.useResidualForEndCriterion)
+        o.preserveCorePosition = int(self.preserveCorePosition)
+        o.assumeSPD = int(self.assumeSPD)
+        o.precision = self.precision
+        o.localTolerance = self.localTolerance
+        o.maxLocalIterations = self.maxLocalIterations
+        o.innerTolerance = self.innerTolerance
+        o.maxRefinements = self.maxRefinements
+        return o
+
+    def __call__(self, *args, **kw):
+        """(A, x, b[, epsilon | numHalfSweeps]) or (x, b[, epsilon | numHalfSweeps]); x is updated in place,
+        the energy is returned (als.h:152-207)."""
+        args = list(args)
+        A = args.pop(0) if isinstance(args[0], TTOperator) else None
+        x, b = args[0], args[1]
+        o = self._options()
+        extra = args[2] if len(args) > 2 else kw.get("epsilon", kw.get("numHalfSweeps"))
+        if "numHalfSweeps" in kw:
+            o.numHalfSweeps = int(kw["numHalfSweeps"])
+        elif "epsilon" in kw:
+            o.convergenceEpsilon = float(kw["epsilon"])
+        elif extra is not None:
+            if isinstance(extra, (int, np.integer)) and not isinstance(extra, bool):
+                o.numHalfSweeps = int(extra)
+            else:
+                o.convergenceEpsilon = float(extra)
+        energy = C.c_double()
+        st = _lib.XtALSStats()
+        call("xt_als_solve", C.byref(o), A._h if A is not None else None, x._h, b._h, C.byref(energy), C.byref(st))
+        self.last_stats = {f: getattr(st, f) for f, _ in _lib.XtALSStats._fields_}
+        return energy.value
+
+
+ALS = ALSVariant(1, 0, False)
+ALS_SPD = ALSVariant(1, 0, True)
+DMRG = ALSVariant(2, 0, False)
+DMRG_SPD = ALSVariant(2, 0, True)
+
+
+# ---------------------------------------------------------------------- measurements and ADF
+
+
+class SinglePointMeasurementSet:
+    """SinglePointMeasurementSet (measurments.h:43-91): sorted unique positions and measured values."""
+
+    def __init__(self, _handle=None):
+        self._h = _handle
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib._lib is not None:
+            _lib._lib.xt_meas_free(h)
+            self._h = None
+
+    @classmethod
+    def random(cls, numMeasurements: int, arg):
+        """random(P, dims) draws positions with the shared engine (measurments.cpp:222-248);
+        random(P, solution) additionally measures the TT `solution`."""
+        dims = arg.dimensions if isinstance(arg, _TTNetwork) else list(arg)
+        _, p = _sizes(dims)
+        h = C.c_void_p()
+        call("xt_meas_random", numMeasurements, len(dims), p, C.byref(h))
+        m = cls(h)
+        if isinstance(arg, _TTNetwork):
+            m.measure(arg)
+        return m
+
+    @classmethod
+    def from_arrays(cls, positions, values):
+        pos = np.ascontiguousarray(positions, dtype=np.uint32)
+        val = np.ascontiguousarray(values, dtype=np.float64)
+        h = C.c_void_p()
+        call("xt_meas_new", pos.shape[0], pos.shape[1], pos.ctypes.data_as(C.POINTER(C.c_uint32)),
+             val.ctypes.data_as(C.POINTER(C.c_double)), C.byref(h))
+        return cls(h)
+
+    def size(self) -> int:
+        n = C.c_size_t()
+        call("xt_meas_size", self._h, C.byref(n))
+        return n.value
+
+    __len__ = size
+
+    def positions(self, order: int) -> np.ndarray:
+        out = np.empty((self.size(), order), dtype=np.uint32)
+        call("xt_meas_positions", self._h, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return out
+
+    def positions_device_ptr(self) -> int:
+        p = C.c_void_p()
+        call("xt_meas_positions_ptr", self._h, C.byref(p))
+        return p.value
+
+    @property
+    def measuredValues(self) -> np.ndarray:
+        out = np.empty(self.size())
+        call("xt_meas_values", self._h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    @measuredValues.setter
+    def measuredValues(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        call("xt_meas_set_values", self._h, v.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def measure(self, solution: TTTensor) -> None:
+        call("xt_meas_measure", self._h, solution._h)
+
+    def frob_norm(self) -> float:
+        return float(np.linalg.norm(self.measuredValues))
+
+
+def adf_gradient(x: TTTensor, measurements: SinglePointMeasurementSet, precision: int = XT_F64):
+    """xerus's projected gradients Δ_k = Σ_i (b_i − x_i)·∂x_i/∂G_k for every core at x (adf.cpp:314-396),
+    computed by the device adjoint kernel. Returns (list of arrays shaped like the cores, ‖P_Ω(x) − b‖/‖b‖)."""
+    shapes = [x.component_dims(k) for k in range(x.order())]
+    out = np.empty(sum(int(np.prod(s)) for s in shapes))
+    rn = C.c_double()
+    call("xt_adf_gradient", x._h, measurements._h, precision, out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(rn))
+    grads, o = [], 0
+    for s in shapes:
+        n = int(np.prod(s))
+        grads.append(out[o: o + n].reshape(s))
+        o += n
+    return grads, rn.value
+
+
+class ADFVariant:
+    """ADFVariant (adf.h:230-266): ``maxIterations``, ``targetResidualNorm``, ``minimalResidualNormDecrease``."""
+
+    def __init__(self, maxIteration: int, targetResidual: float, minimalResidualDecrease: float):
+        self.maxIterations = maxIteration
+        self.targetResidualNorm = targetResidual
+        self.minimalResidualNormDecrease = minimalResidualDecrease
+        self.precision = XT_F64
+        self.last_iterations = None
+
+    def __call__(self, x: TTTensor, measurements: SinglePointMeasurementSet, maxRanks=None) -> float:
+        """Returns ‖P_Ω(x) − b‖/‖b‖; x is updated in place."""
+        p = None
+        if maxRanks is not None:
+            _arr, p = _sizes(maxRanks)
+        res, it = C.c_double(), C.c_size_t()
+        call("xt_adf_solve", x._h, measurements._h, p, self.maxIterations, self.targetResidualNorm,
+             self.minimalResidualNormDecrease, self.precision, C.byref(res), C.byref(it))
+        self.last_iterations = it.value
+        return res.value
+
+
+ADF = ADFVariant(0, 1e-8, 0.999)  # adf.cpp:610

@@ -1,0 +1,44 @@
+"""Synthetic workloads of the TT benchmark: Dirichlet-Laplacian (+ diagonal potential) TT operators.
+
+The operator Σ_k I⊗…⊗H_k⊗…⊗I with H_k = tridiag(-1,2,-1) + diag(v_k) has minimal TT rank 2; the rank-3
+channel form is the non-minimal assembly with separate L and V channels.
+"""
+import numpy as np
+
+
+def laplace_cores(d, n, potentials=None, rank3=False):
+    """TT cores (r, n, n, r) of Σ_k I⊗…⊗H_k⊗…⊗I with H_k = tridiag(-1,2,-1) (+ diag(v_k)).
+    rank3=True builds the non-minimal rank-3 channel form [[I,0,0],[0,I,0],[L,V_k,I]]."""
+    L = 2 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+    I = np.eye(n)
+    cores = []
+    for k in range(d):
+        V = np.diag(potentials[k]) if potentials is not None else np.zeros((n, n))
+        if rank3:
+            # channels: 0 = L placed, 1 = V placed, 2 = nothing placed yet
+            if d == 1:
+                cores.append((L + V).reshape(1, n, n, 1))
+            elif k == 0:
+                c = np.zeros((1, n, n, 3)); c[0, :, :, 0] = L; c[0, :, :, 1] = V; c[0, :, :, 2] = I
+                cores.append(c)
+            elif k == d - 1:
+                c = np.zeros((3, n, n, 1)); c[0, :, :, 0] = I; c[1, :, :, 0] = I; c[2, :, :, 0] = L + V
+                cores.append(c)
+            else:
+                c = np.zeros((3, n, n, 3))
+                c[0, :, :, 0] = I; c[1, :, :, 1] = I; c[2, :, :, 0] = L; c[2, :, :, 1] = V; c[2, :, :, 2] = I
+                cores.append(c)
+            continue
+        H = L + V
+        if d == 1:
+            cores.append(H.reshape(1, n, n, 1))
+        elif k == 0:
+            c = np.zeros((1, n, n, 2)); c[0, :, :, 0] = H; c[0, :, :, 1] = I
+            cores.append(c)
+        elif k == d - 1:
+            c = np.zeros((2, n, n, 1)); c[0, :, :, 0] = I; c[1, :, :, 0] = H
+            cores.append(c)
+        else:
+            c = np.zeros((2, n, n, 2)); c[0, :, :, 0] = I; c[1, :, :, 0] = H; c[1, :, :, 1] = I
+            cores.append(c)
+    return cores
