@@ -36,6 +36,11 @@ typedef struct xt_meas_s* xt_meas;  /* SinglePointMeasurementSet */
 const char* xt_last_error(void);
 const char* xt_version(void);
 int xt_synchronize(void);
+/* plain device buffers for callers without their own allocator (ordered on the library stream) */
+int xt_malloc(size_t bytes, void** device_ptr);
+int xt_free(void* device_ptr);
+int xt_memcpy_htod(void* device_dst, const void* host_src, size_t bytes);
+int xt_memcpy_dtoh(void* host_dst, const void* device_src, size_t bytes);
 int xt_release_cache(void);
 /* Launch profiler (no counterpart in xerus; replaces its XERUS_PA_START/END timers, misc/performanceAnalysis.h:33-56):
  * while enabled, GEMM and ADF launches record HIP events on their own stream. xt_profile_read synchronises
@@ -147,6 +152,9 @@ int xt_adf_solve(xt_tt x, xt_meas m, const size_t* max_ranks, size_t max_iterati
  * values[i] = x(position_i); grads = Σ_i seed_i·∂x_i/∂G (the adjoint of the sampling). */
 int xt_tt_sample_f32(int d, const int* n, const int* r, const float* cores, const uint32_t* pos, int P, float* values,
                      void* stream);
+/* resid[i] = x(position_i) − b[i]; sums (host, nullable) = {Σ resid², Σ b²} accumulated in FP64 */
+int xt_tt_sample_residual_f32(int d, const int* n, const int* r, const float* cores, const uint32_t* pos, int P, const float* b,
+                              float* resid, double* sums, void* stream);
 int xt_tt_sample_adjoint_f32(int d, const int* n, const int* r, const float* cores, const uint32_t* pos, int P,
                              const float* seed, float* grads, void* stream);
 

@@ -58,6 +58,10 @@ SIGNATURES = {
     "xt_version": (C.c_char_p, []),
     "xt_synchronize": (C.c_int, []),
     "xt_release_cache": (C.c_int, []),
+    "xt_malloc": (C.c_int, [_sz, C.POINTER(_vp)]),
+    "xt_free": (C.c_int, [_vp]),
+    "xt_memcpy_htod": (C.c_int, [_vp, _vp, _sz]),
+    "xt_memcpy_dtoh": (C.c_int, [_vp, _vp, _sz]),
     "xt_profile_enable": (C.c_int, [C.c_int]),
     "xt_profile_read": (C.c_int, [_sz, C.c_char_p, _sz, _ps, _pd, _pd, _ps]),
     "xt_seed": (C.c_int, [C.c_uint64]),
@@ -110,6 +114,7 @@ SIGNATURES = {
     "xt_adf_gradient": (C.c_int, [_vp, _vp, C.c_int, _pd, _pd]),
     "xt_adf_solve": (C.c_int, [_vp, _vp, _ps, _sz, C.c_double, C.c_double, C.c_int, _pd, _ps]),
     "xt_tt_sample_f32": (C.c_int, [C.c_int, _pi, _pi, _vp, _vp, C.c_int, _vp, _vp]),
+    "xt_tt_sample_residual_f32": (C.c_int, [C.c_int, _pi, _pi, _vp, _vp, C.c_int, _vp, _vp, _pd, _vp]),
     "xt_tt_sample_adjoint_f32": (C.c_int, [C.c_int, _pi, _pi, _vp, _vp, C.c_int, _vp, _vp, _vp]),
 }
 
@@ -164,3 +169,36 @@ def profile_read(max_records: int = 64):
         tag = tags.raw[i * tag_len:(i + 1) * tag_len].split(b"\0", 1)[0].decode()
         out[tag] = (int(launches[i]), float(ms[i]), float(flops[i]))
     return out
+
+
+class DeviceArray:
+    """A device buffer owned through libxtt (xt_malloc), for callers that do not bring their own allocator."""
+
+    def __init__(self, shape, dtype):
+        import numpy as np
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = C.c_void_p()
+        call("xt_malloc", self.nbytes, C.byref(p))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        call("xt_memcpy_htod", d.ptr, a.ctypes.data, d.nbytes)
+        return d
+
+    def numpy(self):
+        import numpy as np
+        out = np.empty(self.shape, dtype=self.dtype)
+        call("xt_memcpy_dtoh", out.ctypes.data, self.ptr, self.nbytes)
+        return out
+
+    def __del__(self):
+        p = getattr(self, "ptr", None)
+        if p and _lib is not None:
+            _lib.xt_free(p)
+            self.ptr = None

@@ -18,6 +18,9 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
+#include <map>
+#include <set>
 
 #include "tt.hpp"
 
@@ -87,7 +90,6 @@ struct SiteOp {
 	const DTensor* R = nullptr;  // (c, q, c')
 	DTensor Am;                  // (n, q, p, n')
 	size_t a = 0, n = 0, c = 0, P = 0, Q = 0;
-	DBuf<float> L32, R32, Am32;
 	size_t N() const { return a * n * c; }
 };
 
@@ -99,13 +101,6 @@ void site_from(SiteOp& s, const DTensor& L, const DTensor& Acore, const DTensor&
 	XTT_REQUIRE(L.dims[1] == s.P && R.dims[1] == s.Q, "ALS: stack/operator rank mismatch");
 	s.Am = reshuffle(Acore, {2, 0, 3, 1});
 }
-void site_lowp(SiteOp& s) {
-	s.L32.resize(s.L->size()); s.R32.resize(s.R->size()); s.Am32.resize(s.Am.size());
-	convert_f64_f32(s.L->data(), s.L32.get(), s.L->size(), stream());
-	convert_f64_f32(s.R->data(), s.R32.get(), s.R->size(), stream());
-	convert_f64_f32(s.Am.data(), s.Am32.get(), s.Am.size(), stream());
-}
-
 GemmDesc stage2_desc(size_t a, size_t n, size_t c, size_t P, size_t Q, const int* skip) {
 	GemmDesc g = gemm_desc(0, 0, n * Q, c, P * n);
 	g.batch = a;
@@ -164,24 +159,68 @@ DTensor rhs_right(const DTensor& Rb, const DTensor& b, const DTensor& X) {
 	return rhs_left(Rb, br, Xr);                   // (β, a)
 }
 
+// Persistent CG workspace of the library (one per process). Every buffer a CG chunk touches has a fixed
+// address while the capacities do not grow, so a chunk of CG iterations recorded once per local shape into a
+// hipGraph is replayed for every site of that shape (and across ALS calls). Growing a buffer drops the graphs.
 struct Workspace {
-	DBuf<double> r, p, q, T1, T2, bl;
-	DBuf<float> r32, p32, q32, e32, z32, T1f, T2f;
-	DBuf<double> e64;
+	size_t capN = 0, capT1 = 0, capT2 = 0, capL = 0, capR = 0, capAm = 0;
+	DBuf<double> x, bl, r, p, q, T1, T2, L, R, Am, e64;
+	DBuf<float> r32, p32, q32, e32, z32, T1f, T2f, L32, R32, Am32;
 	DBuf<CGState> st;
 	DBuf<double> partials;
-	void ensure(size_t N, size_t t1, size_t t2, bool lowp) {
-		if (r.size() < N) { r.resize(N); p.resize(N); q.resize(N); bl.resize(N); e64.resize(N); }
-		if (T1.size() < t1) T1.resize(t1);
-		if (T2.size() < t2) T2.resize(t2);
+	std::map<std::vector<size_t>, hipGraphExec_t> graphs;
+	std::set<std::vector<size_t>> seen;
+
+	void drop_graphs() {
+		for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+		graphs.clear();
+		seen.clear();
+	}
+	void ensure(size_t N, size_t t1, size_t t2, size_t nl, size_t nr, size_t nam) {
+		if (N <= capN && t1 <= capT1 && t2 <= capT2 && nl <= capL && nr <= capR && nam <= capAm && st.get()) return;
+		sync();
+		drop_graphs();
+		capN = std::max(capN, N); capT1 = std::max(capT1, t1); capT2 = std::max(capT2, t2);
+		capL = std::max(capL, nl); capR = std::max(capR, nr); capAm = std::max(capAm, nam);
+		for (DBuf<double>* b : {&x, &bl, &r, &p, &q, &e64}) b->resize(capN);
+		for (DBuf<float>* b : {&r32, &p32, &q32, &e32, &z32}) b->resize(capN);
+		T1.resize(capT1); T2.resize(capT2); T1f.resize(capT1); T2f.resize(capT2);
+		L.resize(capL); L32.resize(capL); R.resize(capR); R32.resize(capR); Am.resize(capAm); Am32.resize(capAm);
 		if (!st.get()) { st.resize(1); partials.resize(2 * 1024); }
-		if (lowp) {
-			if (r32.size() < N) { r32.resize(N); p32.resize(N); q32.resize(N); e32.resize(N); z32.resize(N); }
-			if (T1f.size() < t1) T1f.resize(t1);
-			if (T2f.size() < t2) T2f.resize(t2);
+		XTT_HIP(hipMemsetAsync(z32.get(), 0, capN * sizeof(float), stream()));
+	}
+	// Runs `body` (which enqueues kernels on the library stream): eagerly the first time a key is seen,
+	// from a recorded hipGraph afterwards. No graphs while the launch profiler is on.
+	template <class F>
+	void run(const std::vector<size_t>& key, F&& body) {
+		if (prof_enabled() || getenv("XTT_NO_GRAPHS")) { body(); return; }
+		auto it = graphs.find(key);
+		if (it == graphs.end()) {
+			if (!seen.count(key)) { seen.insert(key); body(); return; }
+			hipGraph_t g = nullptr;
+			XTT_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
+			try {
+				body();
+			} catch (...) {
+				(void)hipStreamEndCapture(stream(), &g);
+				if (g) (void)hipGraphDestroy(g);
+				throw;
+			}
+			XTT_HIP(hipStreamEndCapture(stream(), &g));
+			hipGraphExec_t e = nullptr;
+			const hipError_t ie = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+			(void)hipGraphDestroy(g);
+			XTT_HIP(ie);
+			it = graphs.emplace(key, e).first;
 		}
+		XTT_HIP(hipGraphLaunch(it->second, stream()));
 	}
 };
+
+Workspace& workspace() {
+	static Workspace w;
+	return w;
+}
 
 struct ALSData {
 	const ALSOptions& o;
@@ -200,7 +239,7 @@ struct ALSData {
 	size_t halfSweepCount = 0;
 	bool increasing = true;
 	ALSStats* stats;
-	Workspace ws;
+	Workspace& ws = workspace();
 
 	ALSData(const ALSOptions& o_, const TT* A_, TT& x_, const TT& b_, ALSStats* s)
 	    : o(o_), A(A_), x(x_), b(b_), targetRank(x_.ranks()), normB(tt_frob_norm(b_)), canonAtEnd(x_.canonicalized),
@@ -285,8 +324,15 @@ struct ALSData {
 		return contract(T, false, rhsR.back(), false, 1);       // (a, N, c)
 	}
 
+	void reserve(const SiteOp& s) {
+		ws.ensure(s.N(), s.a * s.P * s.n * s.c, s.a * s.n * s.Q * s.c, s.L->size(), s.R->size(), s.Am.size());
+	}
+	void count_matvec(const SiteOp& s) {
+		if (stats) { stats->matvecs64++; stats->flopsMatvec += 2.0 * s.a * s.n * s.c * (s.a * s.P + s.n * s.P * s.Q + s.c * s.Q); }
+	}
+	// matvec with the site's own (non-workspace) operator pieces; used outside the CG loops
 	void matvec64(const SiteOp& s, const double* in, double* out, const int* skip) {
-		ws.ensure(s.N(), s.a * s.P * s.n * s.c, s.a * s.n * s.Q * s.c, false);
+		reserve(s);
 		local_matvec<double>(s.L->data(), s.Am.data(), s.R->data(), s.a, s.n, s.c, s.P, s.Q, in, out, ws.T1.get(), ws.T2.get(), skip, P_F64);
 		if (stats) { stats->matvecs64++; stats->flopsMatvec += 2.0 * s.a * s.n * s.c * (s.a * s.P + s.n * s.P * s.Q + s.c * s.Q); }
 	}
@@ -314,71 +360,105 @@ struct ALSData {
 		return h;
 	}
 
+	// Copies the site's operator pieces into the workspace (fixed addresses for the recorded chunks).
+	void stage_site(const SiteOp& s, bool lowp) {
+		reserve(s);
+		XTT_HIP(hipMemcpyAsync(ws.L.get(), s.L->data(), s.L->size() * 8, hipMemcpyDeviceToDevice, stream()));
+		XTT_HIP(hipMemcpyAsync(ws.R.get(), s.R->data(), s.R->size() * 8, hipMemcpyDeviceToDevice, stream()));
+		XTT_HIP(hipMemcpyAsync(ws.Am.get(), s.Am.data(), s.Am.size() * 8, hipMemcpyDeviceToDevice, stream()));
+		if (lowp) {
+			convert_f64_f32(ws.L.get(), ws.L32.get(), s.L->size(), stream());
+			convert_f64_f32(ws.R.get(), ws.R32.get(), s.R->size(), stream());
+			convert_f64_f32(ws.Am.get(), ws.Am32.get(), s.Am.size(), stream());
+		}
+	}
+	void ws_matvec64(const SiteOp& s, const double* in, double* out, const int* skip) {
+		local_matvec<double>(ws.L.get(), ws.Am.get(), ws.R.get(), s.a, s.n, s.c, s.P, s.Q, in, out, ws.T1.get(), ws.T2.get(), skip, P_F64);
+	}
+
+	static constexpr int kChunk = 16;
+
 	// FP64 CG on A_loc·x = b, warm start from x (device vector, overwritten).
-	void cg64(SiteOp& s, double* x, const double* bl) {
+	void cg64(SiteOp& s, double* xout, const double* blin) {
 		const size_t N = s.N();
-		ws.ensure(N, s.a * s.P * s.n * s.c, s.a * s.n * s.Q * s.c, false);
-		matvec64(s, x, ws.q.get(), nullptr);
+		stage_site(s, false);
+		double* x = ws.x.get();
+		const double* bl = ws.bl.get();
+		XTT_HIP(hipMemcpyAsync(x, xout, N * 8, hipMemcpyDeviceToDevice, stream()));
+		XTT_HIP(hipMemcpyAsync(ws.bl.get(), blin, N * 8, hipMemcpyDeviceToDevice, stream()));
+		ws_matvec64(s, x, ws.q.get(), nullptr);
+		count_matvec(s);
 		cg_state_init(ws.st.get(), o.localTol * o.localTol, o.maxLocalIter, stream());
 		cg_start<double>(ws.r.get(), ws.p.get(), bl, ws.q.get(), N, ws.st.get(), ws.partials.get(), stream());
 		const int* done = &ws.st.get()->done;
-		for (int it = 0; it < o.maxLocalIter;) {
-			const int chunk = std::min(16, o.maxLocalIter - it);
-			for (int c = 0; c < chunk; ++c) {
-				matvec64(s, ws.p.get(), ws.q.get(), done);
-				cg_alpha<double>(ws.p.get(), ws.q.get(), N, ws.st.get(), ws.partials.get(), stream());
-				cg_update<double>(x, ws.r.get(), ws.p.get(), ws.q.get(), N, ws.st.get(), ws.partials.get(), stream());
-				cg_direction<double>(ws.p.get(), ws.r.get(), N, ws.st.get(), stream());
-			}
-			it += chunk;
+		const std::vector<size_t> key{64, s.a, s.n, s.c, s.P, s.Q};
+		int iters = 0;
+		for (int it = 0; it < o.maxLocalIter; it += kChunk) {
+			ws.run(key, [&] {
+				for (int c = 0; c < kChunk; ++c) {
+					ws_matvec64(s, ws.p.get(), ws.q.get(), done);
+					cg_alpha<double>(ws.p.get(), ws.q.get(), N, ws.st.get(), ws.partials.get(), stream());
+					cg_update<double>(x, ws.r.get(), ws.p.get(), ws.q.get(), N, ws.st.get(), ws.partials.get(), stream());
+					cg_direction<double>(ws.p.get(), ws.r.get(), N, ws.st.get(), stream());
+				}
+			});
 			const CGState h = read_state();
-			if (h.done) { if (stats) stats->cgIterations += h.iter; return; }
+			iters = h.iter;
+			if (h.done) break;
 		}
-		if (stats) stats->cgIterations += read_state().iter;
+		if (stats) { stats->cgIterations += iters; for (int i = 0; i < iters; ++i) count_matvec(s); }
+		XTT_HIP(hipMemcpyAsync(xout, x, N * 8, hipMemcpyDeviceToDevice, stream()));
 	}
 
 	// Mixed precision: FP64 iterative refinement around an FP32 / BF16 inner CG.
-	void cg_mixed(SiteOp& s, double* x, const double* bl) {
+	void cg_mixed(SiteOp& s, double* xout, const double* blin) {
 		const size_t N = s.N();
-		ws.ensure(N, s.a * s.P * s.n * s.c, s.a * s.n * s.Q * s.c, true);
-		site_lowp(s);
+		stage_site(s, true);
+		double* x = ws.x.get();
+		const double* bl = ws.bl.get();
+		XTT_HIP(hipMemcpyAsync(x, xout, N * 8, hipMemcpyDeviceToDevice, stream()));
+		XTT_HIP(hipMemcpyAsync(ws.bl.get(), blin, N * 8, hipMemcpyDeviceToDevice, stream()));
 		const double inner = o.innerTol > 0 ? o.innerTol : (o.precision == P_BF16 ? 3e-2 : 1e-5);
-		XTT_HIP(hipMemsetAsync(ws.z32.get(), 0, N * sizeof(float), stream()));
 		Scratch64 sc;
 		const double bb = sc.dot(bl, bl, N);
 		const double tol2 = o.localTol * o.localTol * bb;
+		const int* done = &ws.st.get()->done;
+		const std::vector<size_t> key{(size_t)o.precision, s.a, s.n, s.c, s.P, s.Q};
 		for (int ref = 0; ref <= o.maxRefine; ++ref) {
-			matvec64(s, x, ws.q.get(), nullptr);
+			ws_matvec64(s, x, ws.q.get(), nullptr);
+			count_matvec(s);
 			XTT_HIP(hipMemcpyAsync(ws.r.get(), bl, N * 8, hipMemcpyDeviceToDevice, stream()));
 			axpy_f64(ws.r.get(), -1.0, ws.q.get(), N, stream());
 			const double rr = sc.dot(ws.r.get(), ws.r.get(), N);
-			if (rr <= tol2 || ref == o.maxRefine) return;
+			if (rr <= tol2 || ref == o.maxRefine) break;
 			if (stats) stats->refinements++;
 			const double rn = std::sqrt(rr);
 			scale_f64(ws.r.get(), 1.0 / rn, N, stream());
-			convert_f64_f32(ws.r.get(), ws.r32.get(), N, stream());
-			XTT_HIP(hipMemsetAsync(ws.e32.get(), 0, N * sizeof(float), stream()));
+			convert_f64_f32(ws.r.get(), ws.q32.get(), N, stream());  // right-hand side r/‖r‖
 			cg_state_init(ws.st.get(), inner * inner, o.maxLocalIter, stream());
-			// r32 doubles as the right-hand side: cg_start copies it into r/p before they are updated
-			XTT_HIP(hipMemcpyAsync(ws.q32.get(), ws.r32.get(), N * 4, hipMemcpyDeviceToDevice, stream()));
+			// e = 0: r32 = p32 = rhs
 			cg_start<float>(ws.r32.get(), ws.p32.get(), ws.q32.get(), ws.z32.get(), N, ws.st.get(), ws.partials.get(), stream());
-			const int* done = &ws.st.get()->done;
-			for (int it = 0; it < o.maxLocalIter;) {
-				const int chunk = std::min(16, o.maxLocalIter - it);
-				for (int c = 0; c < chunk; ++c) {
-					local_matvec<float>(s.L32.get(), s.Am32.get(), s.R32.get(), s.a, s.n, s.c, s.P, s.Q, ws.p32.get(), ws.q32.get(),
-					                    ws.T1f.get(), ws.T2f.get(), done, o.precision);
-					cg_alpha<float>(ws.p32.get(), ws.q32.get(), N, ws.st.get(), ws.partials.get(), stream());
-					cg_update<float>(ws.e32.get(), ws.r32.get(), ws.p32.get(), ws.q32.get(), N, ws.st.get(), ws.partials.get(), stream());
-					cg_direction<float>(ws.p32.get(), ws.r32.get(), N, ws.st.get(), stream());
-				}
-				it += chunk;
+			XTT_HIP(hipMemcpyAsync(ws.e32.get(), ws.z32.get(), N * 4, hipMemcpyDeviceToDevice, stream()));
+			int iters = 0;
+			for (int it = 0; it < o.maxLocalIter; it += kChunk) {
+				ws.run(key, [&] {
+					for (int c = 0; c < kChunk; ++c) {
+						local_matvec<float>(ws.L32.get(), ws.Am32.get(), ws.R32.get(), s.a, s.n, s.c, s.P, s.Q, ws.p32.get(), ws.q32.get(),
+						                    ws.T1f.get(), ws.T2f.get(), done, o.precision);
+						cg_alpha<float>(ws.p32.get(), ws.q32.get(), N, ws.st.get(), ws.partials.get(), stream());
+						cg_update<float>(ws.e32.get(), ws.r32.get(), ws.p32.get(), ws.q32.get(), N, ws.st.get(), ws.partials.get(), stream());
+						cg_direction<float>(ws.p32.get(), ws.r32.get(), N, ws.st.get(), stream());
+					}
+				});
 				const CGState h = read_state();
-				if (h.done) { if (stats) { stats->cgIterations += h.iter; stats->matvecsLow += h.iter; } break; }
+				iters = h.iter;
+				if (h.done) break;
 			}
+			if (stats) { stats->cgIterations += iters; stats->matvecsLow += iters; }
 			convert_f32_f64(ws.e32.get(), ws.e64.get(), N, stream());
 			axpy_f64(x, rn, ws.e64.get(), N, stream());
 		}
+		XTT_HIP(hipMemcpyAsync(xout, x, N * 8, hipMemcpyDeviceToDevice, stream()));
 	}
 
 	struct Scratch64 {

@@ -14,6 +14,7 @@ namespace xtt { const std::string& last_error(); }
 namespace {
 template <class F>
 int guard(F&& f) {
+	(void)hipGetLastError();  // a failed call before this one must not be reported by this one's launch checks
 	try {
 		f();
 		return XT_OK;
@@ -52,6 +53,14 @@ int xt_profile_read(size_t max_records, char* tags, size_t tag_len, size_t* laun
 			if (flops) flops[i] = recs[i].flops;
 		}
 	});
+}
+int xt_malloc(size_t bytes, void** ptr) { return guard([&] { NN(ptr); *ptr = nullptr; XTT_HIP(hipMalloc(ptr, bytes ? bytes : 1)); }); }
+int xt_free(void* ptr) { return guard([&] { if (ptr) { XTT_HIP(hipDeviceSynchronize()); XTT_HIP(hipFree(ptr)); } }); }
+int xt_memcpy_htod(void* dst, const void* src, size_t bytes) {
+	return guard([&] { if (bytes) { NN(dst); NN(src); XTT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, xtt::stream())); xtt::sync(); } });
+}
+int xt_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+	return guard([&] { if (bytes) { NN(dst); NN(src); XTT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, xtt::stream())); xtt::sync(); } });
 }
 int xt_seed(uint64_t s) { return guard([&] { xtt::rng_seed(s); }); }
 int xt_normal_draws(size_t n, double* out) { return guard([&] { NN(out); for (size_t i = 0; i < n; ++i) out[i] = xtt::rng_normal(); }); }
@@ -381,6 +390,21 @@ int xt_tt_sample_f32(int d, const int* n, const int* r, const float* cores, cons
 		xtt::DBuf<unsigned> counter(1);
 		XTT_HIP(hipMemsetAsync(counter.get(), 0, 4, s));
 		xtt::adf_residual<float>(rd.desc, cores, pos, nullptr, values, norms.get(), partials.get(), counter.get(), s);
+		XTT_HIP(hipStreamSynchronize(s));
+	});
+}
+int xt_tt_sample_residual_f32(int d, const int* n, const int* r, const float* cores, const uint32_t* pos, int P, const float* b,
+                               float* resid, double* sums, void* stream) {
+	return guard([&] {
+		NN(n); NN(r); NN(cores); NN(resid); XTT_REQUIRE(P == 0 || (pos && b), "pos/b must not be NULL");
+		hipStream_t s = xtt::pick(stream);
+		RawDesc rd(d, n, r, P, s);
+		if (P == 0) { if (sums) sums[0] = sums[1] = 0; return; }
+		xtt::DBuf<double> norms(2), partials(2 * 1024);
+		xtt::DBuf<unsigned> counter(1);
+		XTT_HIP(hipMemsetAsync(counter.get(), 0, 4, s));
+		xtt::adf_residual<float>(rd.desc, cores, pos, b, resid, norms.get(), partials.get(), counter.get(), s);
+		if (sums) XTT_HIP(hipMemcpyAsync(sums, norms.get(), 16, hipMemcpyDeviceToHost, s));
 		XTT_HIP(hipStreamSynchronize(s));
 	});
 }

@@ -10,7 +10,10 @@
 // while being staged). Operand lane layouts (16×16 tile, lane l):
 //   x4 shapes : A[i=l%16][k=l/16], B[k=l/16][j=l%16]
 //   x32 bf16  : A[i=l%16][k=8(l/16)+e], B[k=8(l/16)+e][j=l%16], e = 0..7
-//   result    : D[i=4(l/16)+r][j=l%16], r = 0..3
+//   result    : FP32 / BF16: D[i=4(l/16)+r][j=l%16]; FP64: D[i=4r+l/16][j=l%16], r = 0..3 (register r)
+#include <mutex>
+#include <set>
+
 #include "kernels.hpp"
 #include "xtt_common.hpp"
 
@@ -36,6 +39,12 @@ struct Args {
 
 __device__ inline d4 mfma_x4(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 __device__ inline f4 mfma_x4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+// Row of result register r in lane group lk (= lane/16) of a 16×16 MFMA tile: the FP64 instruction interleaves
+// rows across lane groups (row 4r + lk), the FP32 one gives each lane group 4 consecutive rows (row 4lk + r).
+template <class T> struct RowOf;
+template <> struct RowOf<double> { __device__ static int row(int lk, int r) { return 4 * r + lk; } };
+template <> struct RowOf<float> { __device__ static int row(int lk, int r) { return 4 * lk + r; } };
 
 template <class T> struct AccOf;
 template <> struct AccOf<double> { using type = d4; };
@@ -117,7 +126,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_x4_kernel(Args g) {
 		for (int b = 0; b < TN; ++b)
 #pragma unroll
 			for (int r = 0; r < 4; ++r) {
-				const int i = m0 + (wm * TM + a) * 16 + 4 * lk + r;
+				const int i = m0 + (wm * TM + a) * 16 + RowOf<T>::row(lk, r);
 				const int j = n0 + (wn * TN + b) * 16 + li;
 				if (i < g.m && j < g.n) {
 					T* c = C + (size_t)i * g.ldc + j;
@@ -226,9 +235,17 @@ Args make_args(const GemmDesc& d, const void* A, const void* B, void* C) {
 // LDS budget per workgroup (gfx950 has 160 KiB per CU); kernels needing more than the 64 KiB default opt in.
 constexpr size_t kLdsBudget = 150 * 1024;
 
+// Raised once per instantiation to the full budget, so launches recorded into a hipGraph never need it.
 template <class K>
 void allow_lds(K kernel, size_t bytes) {
-	if (bytes > 64 * 1024) XTT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+	static std::mutex mu;
+	static std::set<const void*> raised;
+	if (bytes <= 64 * 1024) return;
+	const void* f = reinterpret_cast<const void*>(kernel);
+	std::lock_guard<std::mutex> lk(mu);
+	if (raised.count(f)) return;
+	XTT_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget));
+	raised.insert(f);
 }
 
 // Tile choice: 64×64 tiles once they still give >= 256 workgroups, else 32×32. The K chunk is the largest

@@ -65,7 +65,10 @@ void* dmalloc(size_t bytes) {
 		release_cache();
 		e = hipMalloc(&p, cls);
 	}
-	if (e != hipSuccess) throw Error(e == hipErrorOutOfMemory ? ERR_NOMEM : ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+	if (e != hipSuccess) {
+		(void)hipGetLastError();  // clear the sticky error so later launch checks do not report it
+		throw Error(e == hipErrorOutOfMemory ? ERR_NOMEM : ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+	}
 	std::lock_guard<std::mutex> lk(g_amu);
 	g_live[p] = {cls, dev};
 	g_inUse += cls;
