@@ -496,21 +496,19 @@ struct ALSData {
 		xm.reshape({a, x.cores[curr].dims[1], x.cores[curr + 1].dims[1], c});
 		DTensor U, Vt;
 		std::vector<double> S;
-		calculate_svd(U, S, Vt, xm, 2, targetRank[curr], XERUS_EPSILON);
+		DBuf<double> sd;
+		calculate_svd(U, S, Vt, xm, 2, targetRank[curr], XERUS_EPSILON, &sd);
 		const size_t rk = S.size(), cols = Vt.size() / std::max<size_t>(rk, 1), rows = U.size() / std::max<size_t>(rk, 1);
 		if (increasing) {
-			for (size_t j = 0; j < rk; ++j) scale_f64(Vt.data() + j * cols, S[j], cols, stream());
+			scale_by_f64(Vt.data(), sd.get(), rk, cols, true, stream());
 			x.set_component(curr, std::move(U));
 			x.set_component(curr + 1, std::move(Vt));
 			x.corePosition = curr + 1;
 		} else {
 			// U·S: scale column j of U by s_j
-			DTensor Ut = transpose2(U, U.degree() - 1);
-			for (size_t j = 0; j < rk; ++j) scale_f64(Ut.data() + j * rows, S[j], rows, stream());
-			DTensor US = transpose2(Ut, 1);
-			US.reshape(U.dims);
+			scale_by_f64(U.data(), sd.get(), rows, rk, false, stream());
 			x.set_component(curr + 1, std::move(Vt));
-			x.set_component(curr, std::move(US));
+			x.set_component(curr, std::move(U));
 			x.corePosition = curr;
 		}
 		x.canonicalized = true;

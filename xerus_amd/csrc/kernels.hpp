@@ -31,7 +31,9 @@ void convert_f64_f32(const double* in, float* out, size_t n, hipStream_t s);
 void convert_f32_f64(const float* in, double* out, size_t n, hipStream_t s);
 void fill_f64(double* p, double v, size_t n, hipStream_t s);
 void scale_f64(double* p, double alpha, size_t n, hipStream_t s);
-void axpy_f64(double* y, double alpha, const double* x, size_t n, hipStream_t s);        // y += alpha·x
+void axpy_f64(double* y, double alpha, const double* x, size_t n, hipStream_t s);
+// m (rows×cols) *= sv[row] (byRow) or sv[col], sv on the device
+void scale_by_f64(double* m, const double* sv, size_t rows, size_t cols, bool byRow, hipStream_t s);        // y += alpha·x
 // Deterministic FP64 reductions into *out (device). `partials` holds >= reduce_blocks(n) doubles, `counter`
 // one zero-initialised unsigned that the kernel returns to zero.
 size_t reduce_blocks(size_t n);
@@ -69,6 +71,14 @@ void org2r(const double* A, size_t m, size_t n, const double* tau, double* Q, si
 void extract_c(const double* A, size_t m, size_t n, const int* jpvt, const int* rank, double* C, hipStream_t s);
 // R (k×n) = upper triangle of A (row-major), k = min(m,n).
 void extract_r(const double* A, size_t m, size_t n, double* R, hipStream_t s);
+// Register-resident Householder QR for n <= 64 columns (any row count, tree-reduced above 256 rows).
+bool hh_supported(size_t m, size_t n);
+// A (m×n row-major, or n×m row-major read transposed when `trans`) = Q·R: Q m×k (may be null), R k×n upper,
+// k = min(m, n).
+void hh_qr(const double* A, size_t m, size_t n, bool trans, double* Q, double* R, hipStream_t s);
+// xerus's QC: A·P = Q·R with column pivoting, rank by the 16·ε·R_00 rule. Writes *rank (device), Q (m×rank,
+// ld rank, capacity m×k), C (rank×n, capacity k×n) and jpvt[n].
+void hh_qc(const double* A, size_t m, size_t n, double* Q, double* C, int* jpvt, int* rank, hipStream_t s);
 // One-sided Jacobi SVD of a row-major m×n matrix with m >= n: U (m×n), S (n, descending), Vt (n×n).
 // Workspace: m·n + n·n + 2n doubles.
 void jacobi_svd(const double* A, size_t m, size_t n, double* U, double* S, double* Vt, double* ws, hipStream_t s);

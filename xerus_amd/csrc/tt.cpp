@@ -119,10 +119,14 @@ int factor(Factor& f, const DTensor& in, size_t m, size_t n, bool pivot) {
 void calculate_qr(DTensor& Q, DTensor& R, const DTensor& in, size_t split) {
 	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree()), k = std::min(m, n);
 	XTT_REQUIRE(m > 0 && n > 0, "Dimension must be larger than zero");
-	Factor f;
-	factor(f, in, m, n, false);
 	Q = DTensor(cat(sl(in.dims, 0, split), {k}));
 	R = DTensor(cat({k}, sl(in.dims, split, in.degree())));
+	if (hh_supported(m, n)) {
+		hh_qr(in.data(), m, n, false, Q.data(), R.data(), stream());
+		return;
+	}
+	Factor f;
+	factor(f, in, m, n, false);
 	org2r(f.a.get(), m, n, f.tau.get(), Q.data(), k, stream());
 	extract_r(f.a.get(), m, n, R.data(), stream());
 }
@@ -145,6 +149,18 @@ void calculate_rq(DTensor& R, DTensor& Q, const DTensor& in, size_t split) {
 void calculate_qc(DTensor& Q, DTensor& C, const DTensor& in, size_t split) {
 	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree());
 	XTT_REQUIRE(m > 0 && n > 0, "Dimension must be larger than zero");
+	if (hh_supported(m, n)) {
+		const size_t k = std::min(m, n);
+		DBuf<double> q(m * k), c(k * n);
+		DBuf<int> jp(n), rk(1);
+		hh_qc(in.data(), m, n, q.get(), c.get(), jp.get(), rk.get(), stream());
+		int r = 0;
+		XTT_HIP(hipMemcpyAsync(&r, rk.get(), sizeof(int), hipMemcpyDeviceToHost, stream()));
+		sync();
+		Q = DTensor(cat(sl(in.dims, 0, split), {(size_t)r}), std::move(q));
+		C = DTensor(cat({(size_t)r}, sl(in.dims, split, in.degree())), std::move(c));
+		return;
+	}
 	Factor f;
 	const size_t r = (size_t)factor(f, in, m, n, true);
 	Q = DTensor(cat(sl(in.dims, 0, split), {r}));
@@ -169,7 +185,8 @@ void calculate_cq(DTensor& C, DTensor& Q, const DTensor& in, size_t split) {
 }
 
 // tensor.cpp:1424-1489 (dense branch); truncation rank = min(k, maxRank) then first j with s_j <= eps·s_0.
-void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTensor& in, size_t split, size_t maxRank, double eps) {
+void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTensor& in, size_t split, size_t maxRank, double eps,
+                   DBuf<double>* sDev) {
 	XTT_REQUIRE(eps >= 0 && eps < 1, "Epsilon must be fullfill 0 <= _eps < 1.");
 	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree()), k = std::min(m, n);
 	XTT_REQUIRE(m > 0 && n > 0, "Dimension must be larger than zero");
@@ -196,13 +213,18 @@ void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTenso
 	XTT_HIP(hipMemcpy2DAsync(U.data(), rank * 8, u.get(), k * 8, rank * 8, m, hipMemcpyDeviceToDevice, stream()));
 	XTT_HIP(hipMemcpyAsync(Vt.data(), vt.get(), rank * n * 8, hipMemcpyDeviceToDevice, stream()));
 	S.assign(sh.begin(), sh.begin() + rank);
+	if (sDev) *sDev = std::move(s);
 }
 
 DTensor qr_r_only(const DTensor& in, size_t split) {
 	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree()), k = std::min(m, n);
+	DTensor R({k, n});
+	if (hh_supported(m, n)) {
+		hh_qr(in.data(), m, n, false, nullptr, R.data(), stream());
+		return R;
+	}
 	Factor f;
 	factor(f, in, m, n, false);
-	DTensor R({k, n});
 	extract_r(f.a.get(), m, n, R.data(), stream());
 	return R;
 }
@@ -340,29 +362,37 @@ void move_core(TT& x, size_t pos, bool keepRank) {  // ttNetwork.cpp:581-628
 }
 
 // tensorNetwork.cpp:678-818 for (from, to) = (k, k-1): the shared edge is from's first and to's last mode.
-void round_edge(TT& x, size_t from, size_t to, size_t maxRank, double eps) {
+void round_edge(TT& x, size_t from, size_t to, size_t maxRank, double eps, bool toIsLeftOrthogonal) {
 	XTT_REQUIRE(to + 1 == from, "round_edge: expects (k, k-1)");
 	DTensor& F = x.cores[from];
 	DTensor& T = x.cores[to];
 	const double r = (double)F.dims[0];
 	std::vector<double> S;
+	DBuf<double> sd;
 	if (5.0 * (double)F.size() * (double)T.size() >= 6.0 * r * r * r * r) {
-		DTensor coreA, coreB, Fq, Tq, U, Vt;
+		DTensor coreA, Fq, U, Vt, X;
 		calculate_cq(coreA, Fq, F, 1);
-		calculate_qc(Tq, coreB, T, T.degree() - 1);
-		DTensor X = contract(coreA, true, coreB, true, 1);
-		calculate_svd(U, S, Vt, X, 1, maxRank, eps);
-		// S·Vt: scale row j of Vt by s_j
-		const size_t rk = S.size(), cols = Vt.size() / std::max<size_t>(rk, 1);
-		for (size_t j = 0; j < rk; ++j) scale_f64(Vt.data() + j * cols, S[j], cols, stream());
-		F = contract(U, true, Fq, false, 1);
-		T = contract(Tq, false, Vt, true, 1);
+		if (toIsLeftOrthogonal) {
+			// QC(T) = (T, I): X = coreAᵀ, and T·Vtᵀ replaces Tq·(coreB·…)
+			X = transpose2(coreA, 1);
+			calculate_svd(U, S, Vt, X, 1, maxRank, eps, &sd);
+			scale_by_f64(Vt.data(), sd.get(), S.size(), Vt.size() / std::max<size_t>(S.size(), 1), true, stream());
+			F = contract(U, true, Fq, false, 1);
+			T = contract(T, false, Vt, true, 1);
+		} else {
+			DTensor coreB, Tq;
+			calculate_qc(Tq, coreB, T, T.degree() - 1);
+			X = contract(coreA, true, coreB, true, 1);
+			calculate_svd(U, S, Vt, X, 1, maxRank, eps, &sd);
+			scale_by_f64(Vt.data(), sd.get(), S.size(), Vt.size() / std::max<size_t>(S.size(), 1), true, stream());
+			F = contract(U, true, Fq, false, 1);
+			T = contract(Tq, false, Vt, true, 1);
+		}
 	} else {
 		DTensor X = contract(F, true, T, true, 1);
 		DTensor U, Vt;
-		calculate_svd(U, S, Vt, X, F.degree() - 1, maxRank, eps);
-		const size_t rk = S.size(), cols = Vt.size() / std::max<size_t>(rk, 1);
-		for (size_t j = 0; j < rk; ++j) scale_f64(Vt.data() + j * cols, S[j], cols, stream());
+		calculate_svd(U, S, Vt, X, F.degree() - 1, maxRank, eps, &sd);
+		scale_by_f64(Vt.data(), sd.get(), S.size(), Vt.size() / std::max<size_t>(S.size(), 1), true, stream());
 		T = transpose2(Vt, 1);
 		std::vector<size_t> back(U.degree());
 		for (size_t i = 0; i + 1 < U.degree(); ++i) back[i] = i + 1;
@@ -379,7 +409,8 @@ void round(TT& x, const dims_t& maxRanks, double eps) {  // ttNetwork.cpp:643-66
 	const bool initCanon = x.canonicalized;
 	const size_t initPos = x.corePosition;
 	move_core(x, d - 1, false);
-	for (size_t i = 0; i + 1 < d; ++i) round_edge(x, d - 1 - i, d - 2 - i, maxRanks[d - 2 - i], eps);
+	// after move_core(d-1) every core left of the edge being rounded is left-orthogonal
+	for (size_t i = 0; i + 1 < d; ++i) round_edge(x, d - 1 - i, d - 2 - i, maxRanks[d - 2 - i], eps, true);
 	x.corePosition = 0;
 	x.canonicalized = true;
 	if (initCanon) move_core(x, initPos, false);

@@ -25,7 +25,9 @@ void calculate_rq(DTensor& R, DTensor& Q, const DTensor& in, size_t split);
 void calculate_qc(DTensor& Q, DTensor& C, const DTensor& in, size_t split);
 void calculate_cq(DTensor& C, DTensor& Q, const DTensor& in, size_t split);
 // Thin SVD with xerus's truncation (tensor.cpp:1463-1474); S returned as host singular values (kept ones).
-void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTensor& in, size_t split, size_t maxRank, double eps);
+// sDev, when given, receives the kept singular values on the device as well.
+void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTensor& in, size_t split, size_t maxRank, double eps,
+                   DBuf<double>* sDev = nullptr);
 // R factor only (k×n) of a Householder QR of the m×n matrix view (used by the stable residual).
 DTensor qr_r_only(const DTensor& in, size_t split);
 double dt_norm(const DTensor& t);
@@ -51,7 +53,9 @@ dims_t reduce_to_maximal_ranks(dims_t ranks, const dims_t& dims, bool op);
 TT tt_random(const dims_t& dims, const dims_t& ranks, bool op);  // host RNG draws as xerus, then move_core(0)
 void transfer_core(TT& x, size_t from, size_t to, bool allowRankReduction);
 void move_core(TT& x, size_t pos, bool keepRank);
-void round_edge(TT& x, size_t from, size_t to, size_t maxRank, double eps);
+// toIsLeftOrthogonal: core `to` has orthonormal columns (true inside round), so its QC is skipped (it would
+// return the core itself up to a signed permutation and full rank).
+void round_edge(TT& x, size_t from, size_t to, size_t maxRank, double eps, bool toIsLeftOrthogonal = false);
 void round(TT& x, const dims_t& maxRanks, double eps);
 double tt_frob_norm(const TT& x);
 double tt_dot(const TT& a, const TT& b);

@@ -43,6 +43,12 @@ __global__ void fill_kernel(double* p, double v, size_t n) {
 __global__ void scale_kernel(double* p, double a, size_t n) {
 	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] *= a;
 }
+// M (rows×cols, row-major) scaled by a device vector: along rows (M[r][c] *= s[r]) or columns (*= s[c]).
+__global__ void scale_by_kernel(double* m, const double* sv, size_t rows, size_t cols, int byRow) {
+	const size_t n = rows * cols;
+	for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+		m[e] *= sv[byRow ? e / cols : e % cols];
+}
 __global__ void axpy_kernel(double* y, double a, const double* x, size_t n) {
 	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] += a * x[i];
 }
@@ -240,6 +246,11 @@ void fill_f64(double* p, double v, size_t n, hipStream_t s) {
 void scale_f64(double* p, double a, size_t n, hipStream_t s) {
 	if (!n) return;
 	hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, s, p, a, n);
+	XTT_LAUNCH_CHECK();
+}
+void scale_by_f64(double* m, const double* sv, size_t rows, size_t cols, bool byRow, hipStream_t s) {
+	if (!rows || !cols) return;
+	hipLaunchKernelGGL(scale_by_kernel, dim3(grid_for(rows * cols)), dim3(256), 0, s, m, sv, rows, cols, byRow ? 1 : 0);
 	XTT_LAUNCH_CHECK();
 }
 void axpy_f64(double* y, double a, const double* x, size_t n, hipStream_t s) {

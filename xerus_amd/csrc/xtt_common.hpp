@@ -89,6 +89,11 @@ struct DTensor {
 	DBuf<double> buf;
 	DTensor() = default;
 	explicit DTensor(std::vector<size_t> d) : dims(std::move(d)) { buf.resize(count()); }
+	// Adopt a device buffer holding at least count() elements (the first count() are the tensor).
+	DTensor(std::vector<size_t> d, DBuf<double>&& b) : dims(std::move(d)), buf(std::move(b)) {
+		if (count() > buf.n) throw Error(ERR_INVALID, "DTensor: adopted buffer too small");
+		buf.n = count();
+	}
 	size_t count() const { size_t p = 1; for (size_t x : dims) p *= x; return p; }
 	size_t size() const { return buf.size(); }
 	size_t degree() const { return dims.size(); }
