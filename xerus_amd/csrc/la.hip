@@ -7,6 +7,7 @@
 // first index on ties; recompute when the downdated norm falls below sqrt(eps)). These fix the integer rank
 // decisions that xerus takes from dgeqp3 (blasLapackWrapper.cpp:262-272).
 #include <cfloat>
+#include <mutex>
 
 #include "kernels.hpp"
 #include "xtt_common.hpp"
@@ -295,6 +296,105 @@ __global__ __launch_bounds__(QT) void jacobi_kernel(const double* A, int m, int 
 	for (int j = tid; j < np; j += QT) if (ord[j] < n) S[ord[j]] = sv[j];
 }
 
+// DPP move of a 64-bit value inside 16-lane rows (butterfly partners only: every lane of a row ends with
+// the same bits after a full butterfly).
+template <int CTRL>
+__device__ inline double dpp64(double v) {
+	const long long b = __double_as_longlong(v);
+	const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+	const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+	return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// Sum over a 16-lane row through the partners l^1, l^2, l^7, l^15: identical in all 16 lanes.
+__device__ inline double row16_sum(double v) {
+	v += dpp64<0xB1>(v);   // quad_perm [1,0,3,2]
+	v += dpp64<0x4E>(v);   // quad_perm [2,3,0,1]
+	v += dpp64<0x141>(v);  // row_half_mirror
+	v += dpp64<0x140>(v);  // row_mirror
+	return v;
+}
+
+constexpr int JG = 16;            // lanes per column pair
+constexpr int JT = 512;           // threads: 32 pairs per round
+constexpr int JMAXM = 128, JMAXN = 64;
+
+// One-sided Jacobi for m <= 128, n <= 64 with W (m×np, column-major) and V (np×np) in LDS. Same round-robin
+// ordering, rotation formula and stopping rule as jacobi_kernel; the pair sums are DPP butterflies.
+__global__ __launch_bounds__(JT) void jacobi_lds_kernel(const double* A, int m, int n, int np, double* U, double* S, double* Vt) {
+	extern __shared__ double sm[];
+	double* W = sm;                       // m × np, column stride m
+	double* V = sm + (size_t)m * np;      // np × np
+	__shared__ double sv[JMAXN];
+	__shared__ int ord[JMAXN];
+	__shared__ int rotated;
+	const int tid = threadIdx.x;
+	for (int e = tid; e < m * np; e += JT) {
+		const int c = e / m, r = e % m;
+		W[e] = c < n ? A[(size_t)r * n + c] : 0.0;
+	}
+	for (int e = tid; e < np * np; e += JT) V[e] = (e / np == e % np) ? 1.0 : 0.0;
+	if (tid == 0) rotated = 0;
+	__syncthreads();
+	const int npairs = np / 2;
+	const int grp = tid / JG, gl = tid % JG;
+	for (int sweep = 0; sweep < 60; ++sweep) {
+		for (int t = 0; t < np - 1; ++t) {
+			if (grp < npairs) {
+				const int posA = grp, posB = np - 1 - grp;
+				const int pa = posA == 0 ? 0 : 1 + (posA - 1 + t) % (np - 1);
+				const int pb = posB == 0 ? 0 : 1 + (posB - 1 + t) % (np - 1);
+				const int p = min(pa, pb), q = max(pa, pb);
+				double* wp = W + (size_t)p * m;
+				double* wq = W + (size_t)q * m;
+				double al = 0, be = 0, ga = 0;
+				for (int r = gl; r < m; r += JG) { const double x = wp[r], y = wq[r]; al = fma(x, x, al); be = fma(y, y, be); ga = fma(x, y, ga); }
+				al = row16_sum(al);
+				be = row16_sum(be);
+				ga = row16_sum(ga);
+				if (ga != 0.0 && fabs(ga) > DBL_EPSILON * sqrt(al * be)) {
+					if (gl == 0) rotated = 1;
+					const double zeta = (be - al) / (2.0 * ga);
+					const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+					const double c = 1.0 / sqrt(1.0 + tt * tt), s = c * tt;
+					for (int r = gl; r < m; r += JG) { const double x = wp[r], y = wq[r]; wp[r] = c * x - s * y; wq[r] = s * x + c * y; }
+					double* vp = V + (size_t)p * np;
+					double* vq = V + (size_t)q * np;
+					for (int r = gl; r < np; r += JG) { const double x = vp[r], y = vq[r]; vp[r] = c * x - s * y; vq[r] = s * x + c * y; }
+				}
+			}
+			__syncthreads();
+		}
+		const int any = rotated;
+		__syncthreads();
+		if (!any) break;
+		if (tid == 0) rotated = 0;
+		__syncthreads();
+	}
+	for (int j = tid; j < np; j += JT) {
+		double s = 0;
+		for (int r = 0; r < m; ++r) { const double x = W[(size_t)j * m + r]; s += x * x; }
+		sv[j] = sqrt(s);
+	}
+	__syncthreads();
+	for (int j = tid; j < np; j += JT) {
+		int rk = 0;
+		for (int i = 0; i < np; ++i) rk += (sv[i] > sv[j]) || (sv[i] == sv[j] && i < j);
+		ord[j] = rk;
+	}
+	__syncthreads();
+	for (int e = tid; e < np * m; e += JT) {
+		const int j = e / m, r = e % m;
+		const int rk = ord[j];
+		if (rk < n) U[(size_t)r * n + rk] = sv[j] > 0 ? W[e] / sv[j] : 0.0;
+	}
+	for (int e = tid; e < np * np; e += JT) {
+		const int j = e / np, c = e % np;
+		const int rk = ord[j];
+		if (rk < n && c < n) Vt[(size_t)rk * n + c] = V[e];
+	}
+	for (int j = tid; j < np; j += JT) if (ord[j] < n) S[ord[j]] = sv[j];
+}
+
 }  // namespace
 
 void geqp2(double* A, size_t m, size_t n, double* tau, int* jpvt, int pivot, double* ws, int* rank, hipStream_t s) {
@@ -332,6 +432,16 @@ void extract_r(const double* A, size_t m, size_t n, double* R, hipStream_t s) {
 void jacobi_svd(const double* A, size_t m, size_t n, double* U, double* S, double* Vt, double* ws, hipStream_t s) {
 	XTT_REQUIRE(m >= n && n > 0, "jacobi_svd expects m >= n > 0");
 	const size_t np = n + (n & 1);
+	if (m <= (size_t)JMAXM && np <= (size_t)JMAXN) {
+		static std::once_flag once;
+		const size_t lds = ((size_t)JMAXM * JMAXN + (size_t)JMAXN * JMAXN) * sizeof(double);
+		std::call_once(once, [&] {
+			XTT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(jacobi_lds_kernel), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+		});
+		hipLaunchKernelGGL(jacobi_lds_kernel, dim3(1), dim3(JT), (m * np + np * np) * sizeof(double), s, A, (int)m, (int)n, (int)np, U, S, Vt);
+		XTT_LAUNCH_CHECK();
+		return;
+	}
 	double* W = ws;
 	double* V = W + m * np;
 	double* sv = V + np * np;

@@ -186,14 +186,14 @@ void calculate_cq(DTensor& C, DTensor& Q, const DTensor& in, size_t split) {
 
 // tensor.cpp:1424-1489 (dense branch); truncation rank = min(k, maxRank) then first j with s_j <= eps·s_0.
 void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTensor& in, size_t split, size_t maxRank, double eps,
-                   DBuf<double>* sDev) {
+                   DBuf<double>* sDev, bool jacobiOnTranspose) {
 	XTT_REQUIRE(eps >= 0 && eps < 1, "Epsilon must be fullfill 0 <= _eps < 1.");
 	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree()), k = std::min(m, n);
 	XTT_REQUIRE(m > 0 && n > 0, "Dimension must be larger than zero");
 	DBuf<double> u(m * k), s(k), vt(k * n);
 	const size_t M = std::max(m, n), N = k, np = N + (N & 1);
 	DBuf<double> ws(M * np + np * np + 2 * np);
-	if (m >= n) {
+	if (m > n || (m == n && !jacobiOnTranspose)) {
 		jacobi_svd(in.data(), m, n, u.get(), s.get(), vt.get(), ws.get(), stream());
 	} else {  // Aᵀ = U'·S·V'ᵀ  ⇒  A = V'·S·U'ᵀ
 		DBuf<double> at(n * m), u2(n * m), vt2(m * m);
@@ -375,7 +375,7 @@ void round_edge(TT& x, size_t from, size_t to, size_t maxRank, double eps, bool 
 		if (toIsLeftOrthogonal) {
 			// QC(T) = (T, I): X = coreAᵀ, and T·Vtᵀ replaces Tq·(coreB·…)
 			X = transpose2(coreA, 1);
-			calculate_svd(U, S, Vt, X, 1, maxRank, eps, &sd);
+			calculate_svd(U, S, Vt, X, 1, maxRank, eps, &sd, true);
 			scale_by_f64(Vt.data(), sd.get(), S.size(), Vt.size() / std::max<size_t>(S.size(), 1), true, stream());
 			F = contract(U, true, Fq, false, 1);
 			T = contract(T, false, Vt, true, 1);
@@ -383,7 +383,7 @@ void round_edge(TT& x, size_t from, size_t to, size_t maxRank, double eps, bool 
 			DTensor coreB, Tq;
 			calculate_qc(Tq, coreB, T, T.degree() - 1);
 			X = contract(coreA, true, coreB, true, 1);
-			calculate_svd(U, S, Vt, X, 1, maxRank, eps, &sd);
+			calculate_svd(U, S, Vt, X, 1, maxRank, eps, &sd, true);
 			scale_by_f64(Vt.data(), sd.get(), S.size(), Vt.size() / std::max<size_t>(S.size(), 1), true, stream());
 			F = contract(U, true, Fq, false, 1);
 			T = contract(Tq, false, Vt, true, 1);

@@ -25,9 +25,11 @@ void calculate_rq(DTensor& R, DTensor& Q, const DTensor& in, size_t split);
 void calculate_qc(DTensor& Q, DTensor& C, const DTensor& in, size_t split);
 void calculate_cq(DTensor& C, DTensor& Q, const DTensor& in, size_t split);
 // Thin SVD with xerus's truncation (tensor.cpp:1463-1474); S returned as host singular values (kept ones).
-// sDev, when given, receives the kept singular values on the device as well.
+// sDev, when given, receives the kept singular values on the device as well. jacobiOnTranspose: for a
+// square matrix whose rows are nearly orthogonal (Rᵀ of a pivoted QR, as in round's precompression), the
+// one-sided Jacobi runs on the transpose, where it needs fewer sweeps (Drmač–Veselić preconditioning).
 void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTensor& in, size_t split, size_t maxRank, double eps,
-                   DBuf<double>* sDev = nullptr);
+                   DBuf<double>* sDev = nullptr, bool jacobiOnTranspose = false);
 // R factor only (k×n) of a Householder QR of the m×n matrix view (used by the stable residual).
 DTensor qr_r_only(const DTensor& in, size_t split);
 double dt_norm(const DTensor& t);
