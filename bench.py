@@ -106,7 +106,7 @@ def bench_adf(xe, np, steps):
     xe._lib.profile_enable(False)
     # algorithmic bytes per full gradient: int32 indices + FP32 value in + FP32 residual, cores in + grads out
     alg_bytes = P * (4 * d + 4 + 4) + 2 * int(np.prod(flat.shape)) * 4
-    g = prof.get("adf_grad_kernel<float>")
+    g = prof.get("adf_grad_fast_kernel<float>") or prof.get("adf_grad_kernel<float>")
     rk = prof.get("adf_residual_kernel<float>")
     return {"grads_per_s": steps / dt, "ms_per_grad": 1e3 * dt / steps, "P": P, "alg_bytes": alg_bytes,
             "residual_norm": float(np.sqrt(sums[0] / sums[1])),

@@ -131,6 +131,15 @@ int xt_als_default_options(xt_als_options* o, unsigned sites, int assumeSPD);
 /* ALSVariant::operator()(A, x, b, …) (als.h:152-207); A may be NULL (x ≈ b). Returns the energy. */
 int xt_als_solve(const xt_als_options* o, xt_tt A, xt_tt x, xt_tt b, double* energy, xt_als_stats* stats);
 
+/* PerformanceData (performanceData.h:43-157, PerformanceData::add at performanceData.cpp:34-80). While a sink
+ * is set on the calling thread, xt_als_solve adds a point after the initial energy (flags 3), after every local
+ * step (flags 0) and at the end of every half sweep (1 after a left-to-right, 3 after a right-to-left one), as
+ * als.cpp:426-480,515-526 do; xt_adf_solve adds one per iteration (adf.cpp:514). elapsed_us excludes the
+ * residual evaluation (stop_timer / continue_timer). Pass NULL to clear. */
+typedef void (*xt_perf_callback)(void* user, size_t iteration, double elapsed_us, double residual, const size_t* ranks,
+                                 size_t nranks, size_t flags);
+int xt_perf_sink(xt_perf_callback cb, void* user);
+
 /* ------------------------------------------- SinglePointMeasurementSet (measurments.h:43-91) and ADF */
 int xt_meas_random(size_t P, size_t order, const size_t* dims, xt_meas* out);  /* measurments.cpp:222-248 */
 int xt_meas_new(size_t P, size_t order, const uint32_t* positions, const double* values, xt_meas* out);

@@ -229,6 +229,27 @@ def test_adf_gradient(xe, oracle, prec):
         assert np.abs(g[k] - go[k]).max() < tol * np.abs(go[k]).max()
 
 
+@pytest.mark.parametrize("d,n,r,P", [(10, 10, 8, 20000), (6, 4, 12, 2000), (16, 3, 8, 5000)])
+def test_adf_gradient_f32_shapes(xe, oracle, d, n, r, P):
+    """FP32 adjoint kernels at the ADF benchmark's shape (d=10, n=10, r=8), above the rank-8 fast path (r=12) and
+    at the largest order of the fast path (d=16), against the FP64 oracle; repeated calls are bit-identical."""
+    print(f"adf gradient f32 d={d} n={n} r={r} P={P}", flush=True)
+    truth, trutho = twin_random(xe, oracle, [n] * d, [r] * (d - 1))
+    x, xo = twin_random(xe, oracle, [n] * d, [r] * (d - 1), seed=SEED + 4)
+    xe.seed(SEED); oracle.seed(SEED)
+    m = xe.SinglePointMeasurementSet.random(P, [n] * d)
+    mo = oracle.Measurements.random(P, [n] * d)
+    m.measure(truth)
+    mo.measure(trutho)
+    g, rn = xe.adf_gradient(x, m, xe.XT_F32)
+    g2, rn2 = xe.adf_gradient(x, m, xe.XT_F32)
+    go, rno = oracle.adf_gradient(xo, mo)
+    assert abs(rn - rno) < 1e-5 * rno
+    for k in range(d):
+        assert np.array_equal(g[k], g2[k])
+        assert np.abs(g[k] - go[k]).max() < 3e-5 * max(np.abs(go[k]).max(), 1e-30)
+
+
 def test_adf_solve(xe, oracle):
     print("adf solve", flush=True)
     d, n, r = 5, 6, 3
@@ -246,3 +267,37 @@ def test_adf_solve(xe, oracle):
     assert abs(res - reso) < 1e-6 * reso
     assert rel(x.full(), xo.full()) < 1e-6
 
+
+
+def test_performance_data(xe, oracle):
+    """PerformanceData points where xerus records them (als.cpp:426-480,515-526; adf.cpp:514): one after the initial
+    energy, one per local step with flags 1/3 at half-sweep ends, residuals equal to the stable TT residual."""
+    print("performance data", flush=True)
+    d, n = 6, 4
+    from helpers import laplace_cores
+    A = xe.TTOperator.from_components(laplace_cores(d, n))
+    xe.seed(SEED)
+    b = xe.TTTensor.random([n] * d, [2] * (d - 1))
+    x = xe.TTTensor.random([n] * d, [3] * (d - 1))
+    perf = xe.PerformanceData(errorFunction=lambda t: xe.frob_norm(t))
+    als = xe.ALS_SPD.copy()
+    als.convergenceEpsilon = 0.0
+    als(A, x, b, 2, perf)
+    flags = [p.flags for p in perf.data]
+    # optimizedRange [1, 5) for d=6, n=4, ranks 3 ≤ 4: 4 steps per half sweep
+    assert flags[0] == 3 and flags.count(1) == 1 and flags[-1] == 3
+    assert [p.iterationCount for p in perf.data] == list(range(len(perf.data)))
+    assert all(b2.elapsedTime >= a2.elapsedTime for a2, b2 in zip(perf.data, perf.data[1:]))
+    assert abs(perf.data[-1].residual - xe.residual(A, x, b)) < 1e-12
+    assert abs(perf.data[-1].error - xe.frob_norm(x)) < 1e-12 * xe.frob_norm(x)
+    assert perf.data[-1].ranks == x.ranks()
+    # ADF: one point per iteration
+    truth = xe.TTTensor.random([n] * d, [2] * (d - 1))
+    m = xe.SinglePointMeasurementSet.random(1500, [n] * d)
+    m.measure(truth)
+    y = xe.TTTensor.random([n] * d, [2] * (d - 1))
+    pa = xe.PerformanceData()
+    adf = xe.ADFVariant(5, 1e-12, 0.999999)
+    res = adf(y, m, pa)
+    assert [p.iterationCount for p in pa.data] == list(range(len(pa.data)))
+    assert abs(pa.data[-1].residual - res) < 1e-12 * max(res, 1e-30)

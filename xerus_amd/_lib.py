@@ -21,6 +21,8 @@ _pf = C.POINTER(C.c_float)
 _pi = C.POINTER(C.c_int)
 _pu32 = C.POINTER(C.c_uint32)
 _vp = C.c_void_p
+# void (*)(void* user, size_t iteration, double elapsed_us, double residual, const size_t* ranks, size_t nranks, size_t flags)
+PERF_CALLBACK = C.CFUNCTYPE(None, _vp, C.c_size_t, C.c_double, C.c_double, C.POINTER(C.c_size_t), C.c_size_t, C.c_size_t)
 
 
 class XtALSOptions(C.Structure):
@@ -113,6 +115,7 @@ SIGNATURES = {
     "xt_meas_measure": (C.c_int, [_vp, _vp]),
     "xt_adf_gradient": (C.c_int, [_vp, _vp, C.c_int, _pd, _pd]),
     "xt_adf_solve": (C.c_int, [_vp, _vp, _ps, _sz, C.c_double, C.c_double, C.c_int, _pd, _ps]),
+    "xt_perf_sink": (C.c_int, [PERF_CALLBACK, _vp]),
     "xt_tt_sample_f32": (C.c_int, [C.c_int, _pi, _pi, _vp, _vp, C.c_int, _vp, _vp]),
     "xt_tt_sample_residual_f32": (C.c_int, [C.c_int, _pi, _pi, _vp, _vp, C.c_int, _vp, _vp, _pd, _vp]),
     "xt_tt_sample_adjoint_f32": (C.c_int, [C.c_int, _pi, _pi, _vp, _vp, C.c_int, _vp, _vp, _vp]),

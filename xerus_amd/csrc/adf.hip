@@ -11,6 +11,9 @@
 //                second kernel, so the result is bit-reproducible (no float atomics).
 // adf_slice_norms : Σ_{i: i_k = j} (F_{k-1}[i]·D[:, j, :]·B_{k+1}[i])² for a direction D of core k (the
 //                denominator of ADF's slice-wise step, adf.cpp:413-465).
+#include <mutex>
+#include <type_traits>
+
 #include "kernels.hpp"
 #include "xtt_common.hpp"
 
@@ -26,40 +29,57 @@ __device__ inline double wsum64(double v) {
 	return v;
 }
 
-// Left-to-right walk of one measurement; returns x_i.
-template <class T>
-__device__ inline T tt_value(const AdfDesc& a, const T* cores, const uint32_t* p, int kSel = -1, const T* D = nullptr) {
-	T f[RMAX];
+// Left-to-right walk of one measurement; returns x_i. RM bounds the ranks (registers), `cores` may point to
+// an LDS copy.
+template <class T, int RM = RMAX>
+__device__ inline T tt_value(const AdfDesc& a, const T* cores, const uint32_t* p, int kSel = -1, const T* D = nullptr,
+                             const int* nS = nullptr, const int* rS = nullptr, const long long* offS = nullptr) {
+	T f[RM];
 	f[0] = 1.f;
 	int rl = 1;
 	for (int k = 0; k < a.d; ++k) {
-		const int n = a.n[k], rr = a.rank[k + 1];
+		const int n = nS ? nS[k] : a.n[k], rr = rS ? rS[k + 1] : a.rank[k + 1];
 		if (p[k] >= (uint32_t)n) return T(NAN);  // position outside the tensor: no memory access
-		const T* G = (k == kSel ? D : cores + a.coreOff[k]) + (size_t)p[k] * rr;
-		T g[RMAX];
+		const T* G = (k == kSel ? D : cores + (offS ? offS[k] : a.coreOff[k])) + (size_t)p[k] * rr;
+		T g[RM];
 #pragma unroll
-		for (int c = 0; c < RMAX; ++c) g[c] = T(0);
-		for (int al = 0; al < rl; ++al) {
-			const T fa = f[al];
-			const T* row = G + (size_t)al * n * rr;
+		for (int c = 0; c < RM; ++c) g[c] = T(0);
 #pragma unroll
-			for (int c = 0; c < RMAX; ++c) if (c < rr) g[c] += fa * row[c];
+		for (int al = 0; al < RM; ++al) {
+			if (al < rl) {
+				const T fa = f[al];
+				const T* row = G + (size_t)al * n * rr;
+#pragma unroll
+				for (int c = 0; c < RM; ++c) if (c < rr) g[c] = fma(fa, row[c], g[c]);
+			}
 		}
 #pragma unroll
-		for (int c = 0; c < RMAX; ++c) f[c] = g[c];
+		for (int c = 0; c < RM; ++c) f[c] = g[c];
 		rl = rr;
 	}
 	return f[0];
 }
 
-template <class T>
+template <class T, int RM, bool LDSCORES>
 __global__ __launch_bounds__(AB) void adf_residual_kernel(AdfDesc a, const T* cores, const uint32_t* pos, const T* values,
                                                           T* resid, double* norms, double* partials, unsigned* counter) {
+	extern __shared__ __align__(16) unsigned char rsm[];
 	__shared__ double ws[2 * AB / 64];
 	__shared__ int amLast;
+	__shared__ int sN[32], sRk[33];
+	__shared__ long long sOff[32];
+	if (threadIdx.x < a.d) { sN[threadIdx.x] = a.n[threadIdx.x]; sOff[threadIdx.x] = a.coreOff[threadIdx.x]; }
+	if (threadIdx.x <= a.d) sRk[threadIdx.x] = a.rank[threadIdx.x];
+	const T* src = cores;
+	if (LDSCORES) {
+		T* sc = reinterpret_cast<T*>(rsm);
+		for (int e = threadIdx.x; e < a.gradSize; e += AB) sc[e] = cores[e];
+		src = sc;
+	}
+	__syncthreads();
 	double rr = 0, bb = 0;
 	for (int i = blockIdx.x * AB + threadIdx.x; i < a.P; i += gridDim.x * AB) {
-		const T v = tt_value<T>(a, cores, pos + (size_t)i * a.d);
+		const T v = tt_value<T, RM>(a, src, pos + (size_t)i * a.d, -1, nullptr, sN, sRk, sOff);
 		const T bi = values ? values[i] : T(0);
 		const T r = v - bi;
 		resid[i] = r;
@@ -80,16 +100,293 @@ __global__ __launch_bounds__(AB) void adf_residual_kernel(AdfDesc a, const T* co
 		amLast = (t == gridDim.x - 1);
 	}
 	__syncthreads();
-	if (amLast && threadIdx.x == 0) {
+	if (amLast) {  // every thread sums a fixed stride of the partials, then a fixed-order block sum
 		__threadfence();
 		double s0 = 0, s1 = 0;
-		for (unsigned b = 0; b < gridDim.x; ++b) {
+		for (unsigned b = threadIdx.x; b < gridDim.x; b += AB) {
 			s0 += __hip_atomic_load(partials + 2 * b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 			s1 += __hip_atomic_load(partials + 2 * b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		}
-		norms[0] = s0;
-		norms[1] = s1;
-		*counter = 0;
+		s0 = wsum64(s0); s1 = wsum64(s1);
+		__syncthreads();
+		if (lane == 0) { ws[2 * wave] = s0; ws[2 * wave + 1] = s1; }
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			double t0 = 0, t1 = 0;
+			for (int w = 0; w < AB / 64; ++w) { t0 += ws[2 * w]; t1 += ws[2 * w + 1]; }
+			norms[0] = t0;
+			norms[1] = t1;
+			*counter = 0;
+		}
+	}
+}
+
+// ---- fast FP32 path for ranks <= 8 and order <= 16 ---------------------------------------------------------
+// The cores are first copied into a zero-padded layout Gp[k] = (n_k, 8, 8) (slice j of core k is one
+// contiguous 8×8 block, blocks PB floats apart), so the per-sample chains are branch-free 8×8 products with
+// 16-byte loads; padded entries are zero and contribute nothing.
+// Gradient, phase 1 (per wave, 16 samples): lanes 0..15 walk the left stacks F_k, lanes 16..31 the right
+// stacks B_k of the same samples, into the wave's LDS slot. Phase 2: lane (a, c) = (lane/8, lane%8) owns entry
+// (a, ·, c) of every core and adds res_s·F_{k-1}[s][a]·B_{k+1}[s][c] to its own accumulator (k, i_k(s)) in the
+// wave's LDS region: every address has one owner lane, so the adds need no ordering between lanes and the
+// result is deterministic (samples in order, waves summed in order, blocks by the reduce kernel).
+constexpr int FW = 4;        // waves per block
+constexpr int FR = 8;        // rank bound
+constexpr int PB = FR * FR + 4;  // floats per padded 8×8 block: the extra 4 spread blocks over the LDS banks
+constexpr int FD = 16;       // order bound
+
+__global__ void pad_cores_kernel(AdfDesc a, const float* cores, long long* padOff, float* Gp, int total) {
+	if (blockIdx.x == 0 && threadIdx.x < a.d) {
+		long long o = 0;
+		for (int k = 0; k < (int)threadIdx.x; ++k) o += (long long)a.n[k] * PB;
+		padOff[threadIdx.x] = o;
+	}
+	for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+		int k = 0, o = 0;
+		while (k + 1 < a.d && o + a.n[k] * PB <= e) { o += a.n[k] * PB; ++k; }
+		const int rem = e - o;
+		const int j = rem / PB, w = rem % PB, al = w / FR, c = w % FR;
+		const int rl = a.rank[k], rr = a.rank[k + 1], n = a.n[k];
+		Gp[e] = (w < FR * FR && al < rl && c < rr) ? cores[a.coreOff[k] + ((size_t)al * n + j) * rr + c] : 0.f;
+	}
+}
+
+// v ← v·G (row vector times the 8×8 block), or v ← G·v
+__device__ inline void mul_row8(float (&v)[FR], const float* G) {
+	float g[FR];
+#pragma unroll
+	for (int c = 0; c < FR; ++c) g[c] = 0.f;
+#pragma unroll
+	for (int al = 0; al < FR; ++al) {
+		const float4 lo = *reinterpret_cast<const float4*>(G + al * FR);
+		const float4 hi = *reinterpret_cast<const float4*>(G + al * FR + 4);
+		g[0] = fmaf(v[al], lo.x, g[0]); g[1] = fmaf(v[al], lo.y, g[1]); g[2] = fmaf(v[al], lo.z, g[2]); g[3] = fmaf(v[al], lo.w, g[3]);
+		g[4] = fmaf(v[al], hi.x, g[4]); g[5] = fmaf(v[al], hi.y, g[5]); g[6] = fmaf(v[al], hi.z, g[6]); g[7] = fmaf(v[al], hi.w, g[7]);
+	}
+#pragma unroll
+	for (int c = 0; c < FR; ++c) v[c] = g[c];
+}
+__device__ inline void mul_col8(float (&v)[FR], const float* G) {
+	float g[FR];
+#pragma unroll
+	for (int al = 0; al < FR; ++al) {
+		const float4 lo = *reinterpret_cast<const float4*>(G + al * FR);
+		const float4 hi = *reinterpret_cast<const float4*>(G + al * FR + 4);
+		g[al] = lo.x * v[0] + lo.y * v[1] + lo.z * v[2] + lo.w * v[3] + hi.x * v[4] + hi.y * v[5] + hi.z * v[6] + hi.w * v[7];
+	}
+#pragma unroll
+	for (int al = 0; al < FR; ++al) v[al] = g[al];
+}
+
+template <class T>
+__global__ __launch_bounds__(AB) void adf_residual_fast_kernel(AdfDesc a, const float* Gp, const long long* padOff, const uint32_t* pos,
+                                                               const T* values, T* resid, double* norms, double* partials,
+                                                               unsigned* counter) {
+	extern __shared__ __align__(16) float gsm[];  // padded cores
+	__shared__ double ws[2 * AB / 64];
+	__shared__ int amLast;
+	__shared__ int sN[FD];
+	__shared__ long long sPo[FD];
+	if (threadIdx.x < a.d) { sN[threadIdx.x] = a.n[threadIdx.x]; sPo[threadIdx.x] = padOff[threadIdx.x]; }
+	for (int e = threadIdx.x; e < a.sumN * PB / 4; e += AB)
+		reinterpret_cast<float4*>(gsm)[e] = reinterpret_cast<const float4*>(Gp)[e];
+	__syncthreads();
+	double rr = 0, bb = 0;
+	for (int i = blockIdx.x * AB + threadIdx.x; i < a.P; i += gridDim.x * AB) {
+		const uint32_t* p = pos + (size_t)i * a.d;
+		float v[FR];
+#pragma unroll
+		for (int c = 0; c < FR; ++c) v[c] = c == 0 ? 1.f : 0.f;
+		bool valid = true;
+		uint32_t pk[FD];
+#pragma unroll
+		for (int k = 0; k < FD; ++k) pk[k] = p[k < a.d ? k : a.d - 1];
+#pragma unroll
+		for (int k = 0; k < FD; ++k) {
+			if (k < a.d) {
+				valid = valid && pk[k] < (uint32_t)sN[k];
+				mul_row8(v, gsm + sPo[k] + (size_t)(valid ? pk[k] : 0) * PB);
+			}
+		}
+		const T val = valid ? T(v[0]) : T(NAN);
+		const T bi = values ? values[i] : T(0);
+		const T r = val - bi;
+		resid[i] = r;
+		rr += (double)r * r;
+		bb += (double)bi * bi;
+	}
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	rr = wsum64(rr); bb = wsum64(bb);
+	if (lane == 0) { ws[2 * wave] = rr; ws[2 * wave + 1] = bb; }
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double s0 = 0, s1 = 0;
+		for (int w = 0; w < AB / 64; ++w) { s0 += ws[2 * w]; s1 += ws[2 * w + 1]; }
+		partials[2 * blockIdx.x] = s0;
+		partials[2 * blockIdx.x + 1] = s1;
+		__threadfence();
+		const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+		amLast = (t == gridDim.x - 1);
+	}
+	__syncthreads();
+	if (amLast) {  // every thread sums a fixed stride of the partials, then a fixed-order block sum
+		__threadfence();
+		double s0 = 0, s1 = 0;
+		for (unsigned b = threadIdx.x; b < gridDim.x; b += AB) {
+			s0 += __hip_atomic_load(partials + 2 * b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			s1 += __hip_atomic_load(partials + 2 * b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		}
+		s0 = wsum64(s0); s1 = wsum64(s1);
+		__syncthreads();
+		if (lane == 0) { ws[2 * wave] = s0; ws[2 * wave + 1] = s1; }
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			double t0 = 0, t1 = 0;
+			for (int w = 0; w < AB / 64; ++w) { t0 += ws[2 * w]; t1 += ws[2 * w + 1]; }
+			norms[0] = t0;
+			norms[1] = t1;
+			*counter = 0;
+		}
+	}
+}
+
+// Gradient: one block of FW = 4 waves takes rounds of FS = 128 samples. Phase 1: waves 0-1 walk the left
+// stacks F_k of the round's samples (one per lane), waves 2-3 the right stacks B_k, from the LDS copy of the
+// padded cores, into the block's staging area. Phase 2: wave w owns cores k ≡ w (mod 4) and adds the round's
+// contribution to its gradient with MFMA: G_k[a, j, c] += Σ_s (res_s·F_{k-1}[s][a]) · (B_{k+1}[s][c]·[i_k(s) = j])
+// is an (8 × 128)·(128 × 8n_k) product, run as v_mfma_f32_16x16x4_f32 over K = samples with the accumulators
+// in registers. The summation order is fixed, so the result is deterministic (the blocks' partials are summed in
+// block order by the reduce kernel).
+constexpr int FS = 128;
+constexpr int FSP = FS + 1;  // staging row stride: lanes reading rows a = 0..7 of one sample hit different banks
+
+constexpr int GNT = 8;       // 16-column MFMA tiles per core (8·n_k <= 128, n_k <= 16)
+constexpr int GKC = FD / FW; // cores per wave
+
+__host__ __device__ inline size_t fast_grad_floats(int d, int sumN) {
+	return (size_t)sumN * PB + (size_t)FR * FSP                      // padded cores, e_0 rows
+	       + (size_t)2 * (d - 1) * FR * FSP + (size_t)d * FS + FS;  // F, B, positions, residuals
+}
+
+typedef float gf4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const float* Gp, const long long* padOff, const uint32_t* pos,
+                                                                const float* resid, float* partials, int kSel) {
+	extern __shared__ __align__(16) float fsm[];
+	__shared__ int sN[FD], sRk[FD + 1], sSlot[FD + 1];
+	__shared__ long long sOff[FD], sPo[FD];
+	const int tid = threadIdx.x, lane = tid & 63;
+	const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+	const int d = a.d;
+	if (tid < d) { sN[tid] = a.n[tid]; sOff[tid] = a.coreOff[tid]; sPo[tid] = padOff[tid]; }
+	if (tid <= d) sRk[tid] = a.rank[tid];
+	__syncthreads();
+	if (tid == 0) {
+		int acc0 = 0;
+		for (int k = 0; k < d; ++k) { sSlot[k] = acc0; acc0 += sN[k]; }
+		sSlot[d] = acc0;
+	}
+	__syncthreads();
+	const int nslots = sSlot[d];
+	float* Gs = fsm;                                                    // padded cores
+	float* Es = Gs + (size_t)nslots * PB;                               // rows e_0: F_{-1} and B_d
+	float* Fs = Es + (size_t)FR * FSP;                                  // F_k at [(k·FR + c)·FSP + s], k = 0..d-2
+	float* Bs = Fs + (size_t)(d - 1) * FR * FSP;                        // B_k at [((k-1)·FR + a)·FSP + s], k = 1..d-1
+	int* Is = reinterpret_cast<int*>(Bs + (size_t)(d - 1) * FR * FSP);  // i_k at [k·FS + s]
+	float* Rs = reinterpret_cast<float*>(Is + (size_t)d * FS);         // res at [s]
+	for (int e = tid; e < nslots * PB / 4; e += FW * 64)
+		reinterpret_cast<float4*>(Gs)[e] = reinterpret_cast<const float4*>(Gp)[e];
+	for (int e = tid; e < FR * FSP; e += FW * 64) Es[e] = e < FSP ? 1.f : 0.f;
+	const int li = lane & 15, lk = lane >> 4;  // MFMA: A row / B column li, K index lk
+	gf4 acc[GKC][GNT];
+#pragma unroll
+	for (int q = 0; q < GKC; ++q)
+#pragma unroll
+		for (int t = 0; t < GNT; ++t) acc[q][t] = gf4{0.f, 0.f, 0.f, 0.f};
+	const int per = (a.P + gridDim.x - 1) / gridDim.x;
+	const int s0 = blockIdx.x * per, s1 = min(a.P, s0 + per);
+	__syncthreads();
+	for (int base = s0; base < s1; base += FS) {
+		const int cnt = min(FS, s1 - base);
+		{
+			const int s = tid & (FS - 1);
+			const bool fwd = w < 2;  // wave-uniform
+			bool valid = s < cnt;
+			const uint32_t* p = pos + (size_t)(base + (valid ? s : 0)) * d;
+			uint32_t pk[FD];
+#pragma unroll
+			for (int k = 0; k < FD; ++k) pk[k] = p[k < d ? k : d - 1];
+#pragma unroll
+			for (int k = 0; k < FD; ++k) valid = valid && (k >= d || pk[k] < (uint32_t)sN[k]);
+			if (fwd) {
+				for (int k = 0; k < d; ++k) Is[k * FS + s] = valid ? (int)pk[k] : 0;
+				Rs[s] = valid ? resid[base + s] : 0.f;
+			}
+			float v[FR];
+#pragma unroll
+			for (int c = 0; c < FR; ++c) v[c] = c == 0 ? 1.f : 0.f;
+			if (fwd) {
+#pragma unroll
+				for (int k = 0; k + 1 < FD; ++k) {
+					if (k + 1 < d) {
+						mul_row8(v, Gs + sPo[k] + (size_t)(valid ? pk[k] : 0) * PB);
+#pragma unroll
+						for (int c = 0; c < FR; ++c) Fs[(k * FR + c) * FSP + s] = v[c];
+					}
+				}
+			} else {
+#pragma unroll
+				for (int k = FD - 1; k >= 1; --k) {
+					if (k < d) {
+						mul_col8(v, Gs + sPo[k] + (size_t)(valid ? pk[k] : 0) * PB);
+#pragma unroll
+						for (int al = 0; al < FR; ++al) Bs[((k - 1) * FR + al) * FSP + s] = v[al];
+					}
+				}
+			}
+		}
+		__syncthreads();
+#pragma unroll
+		for (int q = 0; q < GKC; ++q) {
+			const int k = w + q * FW;
+			if (k >= d || (kSel >= 0 && k != kSel)) continue;
+			const int nt = (sN[k] * FR + 15) / 16;
+			const float* fk = (k > 0 ? Fs + (k - 1) * FR * FSP : Es) + (li < FR ? li : 0) * FSP;
+			const float* bk = (k + 1 < d ? Bs + k * FR * FSP : Es) + (li & 7) * FSP;
+			const int* ik = Is + k * FS;
+			for (int kb = 0; kb < FS; kb += 4) {
+				const int sidx = kb + lk;
+				const float av = li < FR ? Rs[sidx] * fk[sidx] : 0.f;
+				const int j = ik[sidx];
+				const float bv = bk[sidx];
+#pragma unroll
+				for (int t = 0; t < GNT; ++t) {
+					if (t < nt) {
+						const float b = (2 * t + (li >> 3) == j) ? bv : 0.f;
+						acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b, acc[q][t], 0, 0, 0);
+					}
+				}
+			}
+		}
+		__syncthreads();
+	}
+	// the block's accumulators, written in the gradient layout: register r of tile t holds
+	// G_k[a = 4·lk + r, j = (16t + li)/8, c = li%8]
+	float* dst = partials + (size_t)blockIdx.x * a.gradSize;
+#pragma unroll
+	for (int q = 0; q < GKC; ++q) {
+		const int k = w + q * FW;
+		if (k >= d) continue;
+		const int rl = sRk[k], rr = sRk[k + 1], n = sN[k];
+#pragma unroll
+		for (int t = 0; t < GNT; ++t) {
+			const int j = (16 * t + li) >> 3, c = li & 7;
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				const int al = 4 * lk + r;
+				if (j < n && al < rl && c < rr) dst[sOff[k] + ((size_t)al * n + j) * rr + c] = acc[q][t][r];
+			}
+		}
 	}
 }
 
@@ -204,13 +501,19 @@ __global__ __launch_bounds__(AB) void adf_grad_kernel(AdfDesc a, const T* cores,
 	for (int e = tid; e < a.gradSize; e += AB) dst[e] = acc[e];
 }
 
+// grad[e] = Σ_b partials[b][e]: 64 entries per block, four threads per entry each summing every fourth block
+// partial, then the four sums in a fixed order (deterministic).
 template <class T>
-__global__ void adf_grad_reduce_kernel(const T* partials, int nblocks, int gradSize, T* grad) {
-	for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < gradSize; e += gridDim.x * blockDim.x) {
-		T s = T(0);
-		for (int b = 0; b < nblocks; ++b) s += partials[(size_t)b * gradSize + e];
-		grad[e] = s;
-	}
+__global__ __launch_bounds__(256) void adf_grad_reduce_kernel(const T* partials, int nblocks, int gradSize, T* grad) {
+	__shared__ T part[4][64];
+	const int o = threadIdx.x & 63, q = threadIdx.x >> 6;
+	const int e = blockIdx.x * 64 + o;
+	T s = T(0);
+	if (e < gradSize)
+		for (int b = q; b < nblocks; b += 4) s += partials[(size_t)b * gradSize + e];
+	part[q][o] = s;
+	__syncthreads();
+	if (q == 0 && e < gradSize) grad[e] = (part[0][o] + part[1][o]) + (part[2][o] + part[3][o]);
 }
 
 GradLayout grad_layout(int d, int maxR, int gradSize, size_t E) {
@@ -267,14 +570,52 @@ __global__ void adf_slice_reduce_kernel(const double* partials, int nblocks, int
 
 }  // namespace
 
+namespace {
+// ranks <= 8, order <= 16 and the padded cores within 48 KB of LDS
+bool fast_ok(const AdfDesc& a) { return a.maxR <= FR && a.d >= 1 && a.d <= FD && (size_t)a.sumN * PB * sizeof(float) <= 48 * 1024; }
+struct Padded {
+	DBuf<float> Gp;
+	DBuf<long long> off;
+};
+// zero-padded (n_k, 8, 8) copy of the cores (and its core offsets) for the fast kernels
+Padded pad_cores(const AdfDesc& a, const float* cores, hipStream_t s) {
+	Padded pc;
+	const int total = a.sumN * PB;
+	pc.Gp.resize((size_t)total);
+	pc.off.resize(FD);
+	hipLaunchKernelGGL(pad_cores_kernel, dim3(std::max(1, std::min(1024, (total + 255) / 256))), dim3(256), 0, s, a, cores, pc.off.get(),
+	                   pc.Gp.get(), total);
+	XTT_LAUNCH_CHECK();
+	return pc;
+}
+}  // namespace
+
 template <class T>
 void adf_residual(const AdfDesc& a, const T* cores, const uint32_t* pos, const T* values, T* resid, double* norms,
                   double* partials, unsigned* counter, hipStream_t s) {
 	XTT_REQUIRE(a.maxR <= RMAX, "ADF: ranks above 16 are not supported");
 	XTT_REQUIRE(a.d >= 1 && a.d <= 32, "ADF: order must be between 1 and 32");
 	const int blocks = std::max(1, std::min(1024, (a.P + AB - 1) / AB));
+	if (std::is_same<T, float>::value && fast_ok(a)) {
+		Padded pc = pad_cores(a, reinterpret_cast<const float*>(cores), s);
+		const int slot = prof_begin(s);
+		hipLaunchKernelGGL(adf_residual_fast_kernel<T>, dim3(blocks), dim3(AB), (size_t)a.sumN * PB * sizeof(float), s, a, pc.Gp.get(),
+		                   pc.off.get(), pos, values, resid,
+		                   norms, partials, counter);
+		XTT_LAUNCH_CHECK();
+		prof_end(slot, s, "adf_residual_fast_kernel<float>", 0.0);
+		return;
+	}
 	const int slot = prof_begin(s);
-	hipLaunchKernelGGL(adf_residual_kernel<T>, dim3(blocks), dim3(AB), 0, s, a, cores, pos, values, resid, norms, partials, counter);
+	const size_t coreBytes = (size_t)a.gradSize * sizeof(T);
+	const bool lds = coreBytes <= 48 * 1024;
+	if (a.maxR <= 8) {
+		if (lds) hipLaunchKernelGGL((adf_residual_kernel<T, 8, true>), dim3(blocks), dim3(AB), coreBytes, s, a, cores, pos, values, resid, norms, partials, counter);
+		else hipLaunchKernelGGL((adf_residual_kernel<T, 8, false>), dim3(blocks), dim3(AB), 0, s, a, cores, pos, values, resid, norms, partials, counter);
+	} else {
+		if (lds) hipLaunchKernelGGL((adf_residual_kernel<T, RMAX, true>), dim3(blocks), dim3(AB), coreBytes, s, a, cores, pos, values, resid, norms, partials, counter);
+		else hipLaunchKernelGGL((adf_residual_kernel<T, RMAX, false>), dim3(blocks), dim3(AB), 0, s, a, cores, pos, values, resid, norms, partials, counter);
+	}
 	XTT_LAUNCH_CHECK();
 	prof_end(slot, s, sizeof(T) == 8 ? "adf_residual_kernel<double>" : "adf_residual_kernel<float>", 0.0);
 }
@@ -286,6 +627,23 @@ void adf_gradient(const AdfDesc& a, const T* cores, const uint32_t* pos, const T
                   hipStream_t s) {
 	XTT_REQUIRE(a.maxR <= RMAX, "ADF: ranks above 16 are not supported");
 	XTT_REQUIRE(a.d >= 1 && a.d <= 32, "ADF: order must be between 1 and 32");
+	const size_t fastLds = fast_grad_floats(a.d, a.sumN) * sizeof(float);
+	if (std::is_same<T, float>::value && fast_ok(a) && a.d >= 2 && a.maxN * FR <= GNT * 16 && fastLds <= 150 * 1024) {
+		Padded pc = pad_cores(a, reinterpret_cast<const float*>(cores), s);
+		static std::once_flag once;
+		std::call_once(once, [&] {
+			XTT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(adf_grad_fast_kernel), hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+		});
+		const int blocks = std::min(adf_grad_blocks(a.P), 256);
+		const int slot = prof_begin(s);
+		hipLaunchKernelGGL(adf_grad_fast_kernel, dim3(blocks), dim3(FW * 64), fastLds, s, a, pc.Gp.get(), pc.off.get(), pos,
+		                   reinterpret_cast<const float*>(resid), reinterpret_cast<float*>(partials), kSel);
+		XTT_LAUNCH_CHECK();
+		prof_end(slot, s, "adf_grad_fast_kernel<float>", 0.0);
+		hipLaunchKernelGGL(adf_grad_reduce_kernel<T>, dim3((a.gradSize + 63) / 64), dim3(256), 0, s, partials, blocks, a.gradSize, grad);
+		XTT_LAUNCH_CHECK();
+		return;
+	}
 	const GradLayout L = grad_layout(a.d, a.maxR, a.gradSize, sizeof(T));
 	const size_t lds = L.total * sizeof(T);
 	if (lds > 64 * 1024)
@@ -295,8 +653,7 @@ void adf_gradient(const AdfDesc& a, const T* cores, const uint32_t* pos, const T
 	hipLaunchKernelGGL(adf_grad_kernel<T>, dim3(blocks), dim3(AB), lds, s, a, cores, pos, resid, partials, L, kSel);
 	XTT_LAUNCH_CHECK();
 	prof_end(slot, s, sizeof(T) == 8 ? "adf_grad_kernel<double>" : "adf_grad_kernel<float>", 0.0);
-	const int rb = std::max(1, std::min(256, (a.gradSize + 255) / 256));
-	hipLaunchKernelGGL(adf_grad_reduce_kernel<T>, dim3(rb), dim3(256), 0, s, partials, blocks, a.gradSize, grad);
+	hipLaunchKernelGGL(adf_grad_reduce_kernel<T>, dim3((a.gradSize + 63) / 64), dim3(256), 0, s, partials, blocks, a.gradSize, grad);
 	XTT_LAUNCH_CHECK();
 }
 

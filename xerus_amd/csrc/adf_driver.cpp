@@ -77,6 +77,9 @@ void Flat<T>::build(const TT& x) {
 	desc.coreOff = doff.get();
 	desc.maxR = maxR;
 	desc.gradSize = (int)o;
+	desc.sumN = 0;
+	desc.maxN = 0;
+	for (int v : n) { desc.sumN += v; desc.maxN = std::max(desc.maxN, v); }
 	refresh(x);
 	sync();  // the host arrays above are locals
 }
@@ -188,6 +191,8 @@ double adf_solve(const ADFOptions& o, TT& x, const Measurements& m, const dims_t
 	Flat<T> f;
 	size_t iteration = 0;
 	double residualNorm = DBL_MAX, lastResidualNorm = DBL_MAX;
+	PerfClock clock;
+	clock.begin();
 
 	auto residual = [&]() {  // resid = x − b at the current x; returns Σ resid²
 		f.build(x);
@@ -208,6 +213,10 @@ double adf_solve(const ADFOptions& o, TT& x, const Measurements& m, const dims_t
 			const double rr = residual();
 			lastResidualNorm = residualNorm;
 			residualNorm = std::sqrt(rr) / normMeasured;
+			if (PerfSink& ps = perf_sink(); ps.cb) {  // adf.cpp:514, perfData.add(iteration, residualNorm, x, 0)
+				const dims_t rk = x.ranks();
+				ps.cb(ps.user, iteration, clock.elapsed(), residualNorm, rk.data(), rk.size(), 0);
+			}
 			const double resDec4 = resDec3;
 			resDec3 = resDec2;
 			resDec2 = resDec1;

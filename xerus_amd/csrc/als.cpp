@@ -514,6 +514,39 @@ struct ALSData {
 		x.canonicalized = true;
 	}
 
+	// residual_f of als.cpp:240-320: ‖Ax − b‖/‖b‖ (SPD, computed stably as a TT), the energy functional of the
+	// non-SPD case (which is its residual), ‖x − b‖ without an operator.
+	double residual_f() {
+		if (!A) {
+			TT nb = tt_copy(b);
+			nb.canonicalized = false;
+			dt_scale(nb.cores[0], -1.0);
+			return tt_frob_norm(tt_add(x, nb));
+		}
+		if (o.assumeSPD) return tt_residual(*A, x, b);
+		return energy_f();
+	}
+
+	// PerformanceData::add(residual, x, flags) (performanceData.cpp:50-80): residual with the clock paused
+	void perf_add(double residual, size_t flags) {
+		PerfSink& ps = perf_sink();
+		if (!ps.cb) return;
+		clock.pause();
+		const dims_t r = x.ranks();
+		ps.cb(ps.user, perfIteration++, clock.elapsed(), residual, r.data(), r.size(), flags);
+		clock.resume();
+	}
+	void perf_point(size_t flags, bool residualIsEnergy) {
+		if (!perf_sink().cb) return;
+		sync();
+		clock.pause();
+		const double res = residualIsEnergy ? energy : residual_f();
+		clock.resume();
+		perf_add(res, flags);
+	}
+	PerfClock clock;
+	size_t perfIteration = 0;
+
 	bool end_of_sweep() {  // als.cpp:651-700
 		if ((!increasing && curr == first) || (increasing && curr == second - o.sites)) {
 			halfSweepCount += 1;
@@ -521,12 +554,15 @@ struct ALSData {
 			lastEnergy = energy;
 			energy = energy_f();
 			if (stats) { stats->halfSweeps = halfSweepCount; stats->energies.push_back(energy); }
+			perf_point(increasing ? 1 : 3, o.useResidualForEndCriterion);
 			if (halfSweepCount == o.numHalfSweeps || std::abs(lastEnergy - energy) < o.convergenceEpsilon ||
 			    std::abs(lastEnergy2 - energy) < o.convergenceEpsilon || second - first <= o.sites) {
 				if (canonAtEnd && o.preserveCorePosition) move_core(x, corePosAtEnd, true);
 				return true;
 			}
 			increasing = !increasing;
+		} else {
+			perf_point(0, false);
 		}
 		return false;
 	}
@@ -566,7 +602,9 @@ double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats*
 		XTT_REQUIRE(o.sites == 1, "approximation dmrg not implemented yet");
 	}
 	ALSData data(o, A, x, b, stats);
+	data.clock.begin();
 	data.energy = data.energy_f();
+	data.perf_point(3, false);
 	while (true) {
 		data.local_solve();
 		if (data.end_of_sweep()) return data.energy;

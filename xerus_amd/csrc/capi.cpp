@@ -332,6 +332,14 @@ int xt_adf_gradient(xt_tt x, xt_meas m, int precision, double* host, double* rn)
 		for (auto& c : g) { std::memcpy(host + o, c.data(), c.size() * 8); o += c.size(); }
 	});
 }
+int xt_perf_sink(xt_perf_callback cb, void* user) {
+	return guard([&] {
+		xtt::PerfSink& ps = xtt::perf_sink();
+		ps.cb = cb;
+		ps.user = user;
+	});
+}
+
 int xt_adf_solve(xt_tt x, xt_meas m, const size_t* maxRanks, size_t maxIt, double target, double minDec, int precision, double* res,
                  size_t* its) {
 	return guard([&] {
@@ -374,6 +382,9 @@ struct RawDesc {
 		XTT_HIP(hipMemcpyAsync(off.get(), o.data(), d * 8, hipMemcpyHostToDevice, s));
 		XTT_HIP(hipStreamSynchronize(s));
 		desc.d = d; desc.P = P; desc.n = n.get(); desc.rank = r.get(); desc.coreOff = off.get(); desc.maxR = maxR; desc.gradSize = (int)acc;
+		desc.sumN = 0;
+		desc.maxN = 0;
+		for (int k = 0; k < d; ++k) { desc.sumN += hn[k]; desc.maxN = std::max(desc.maxN, hn[k]); }
 	}
 };
 }  // namespace

@@ -92,6 +92,8 @@ struct AdfDesc {
 	const long long* coreOff; // device: element offset of core k in `cores` [d]
 	int maxR;                 // max rank (<= 16)
 	int gradSize;             // total elements of all cores
+	int sumN;                 // Σ_k n_k
+	int maxN;                 // max_k n_k
 };
 // value_i = x(position_i); resid_i = value_i - b_i; Σ resid² and Σ b² in FP64 into norms[0..1].
 template <class T>

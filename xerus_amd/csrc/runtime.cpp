@@ -1,5 +1,6 @@
 // libxtt runtime: thread-local error text, the per-device library stream and the caching device allocator.
 #include <mutex>
+#include <chrono>
 #include <unordered_map>
 
 #include "xtt_common.hpp"
@@ -156,6 +157,14 @@ void DTensor::reshape(std::vector<size_t> d) {
 	for (size_t x : d) p *= x;
 	XTT_REQUIRE(p == count(), "reshape: element count changes");
 	dims = std::move(d);
+}
+
+PerfSink& perf_sink() {
+	static thread_local PerfSink s;
+	return s;
+}
+double PerfClock::now_us() {
+	return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 }  // namespace xtt

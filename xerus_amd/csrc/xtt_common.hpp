@@ -47,6 +47,26 @@ int prof_begin(hipStream_t s);                                     // returns a 
 void prof_end(int slot, hipStream_t s, const char* tag, double flops);
 std::vector<ProfRecord> prof_collect();                            // synchronises, aggregates, clears
 
+// PerformanceData sink (performanceData.h:43-157): while a callback is set on the calling thread, the ALS and
+// ADF drivers report data points (iteration, elapsed µs with the residual evaluation excluded, residual,
+// ranks, flags) as xerus's PerformanceData::add does.
+using PerfCallback = void (*)(void* user, size_t iteration, double elapsedMicros, double residual, const size_t* ranks,
+                              size_t nranks, size_t flags);
+struct PerfSink {
+	PerfCallback cb = nullptr;
+	void* user = nullptr;
+};
+PerfSink& perf_sink();  // thread-local
+// Wall clock of one solve, with pauses (xerus's stop_timer / continue_timer).
+struct PerfClock {
+	double start = 0, pausedAt = -1;
+	static double now_us();
+	void begin() { start = now_us(); pausedAt = -1; }
+	void pause() { pausedAt = now_us(); }
+	void resume() { if (pausedAt >= 0) { start += now_us() - pausedAt; pausedAt = -1; } }
+	double elapsed() const { return (pausedAt >= 0 ? pausedAt : now_us()) - start; }
+};
+
 // The library's stream on the current device (default priority, blocking: it orders with the legacy null
 // stream torch uses), created on first use.
 hipStream_t stream();

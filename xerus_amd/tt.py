@@ -277,10 +277,81 @@ def residual(A: TTOperator, x: TTTensor, b: TTTensor) -> float:
 # ---------------------------------------------------------------------- ALS
 
 
+# ---------------------------------------------------------------------- PerformanceData
+
+
+class PerformanceData:
+    """PerformanceData (performanceData.h:43-157): the optional trace argument of ALSVariant and ADFVariant.
+
+    ``data`` collects DataPoint(iterationCount, elapsedTime [µs], residual, error, ranks, flags) as xerus's
+    PerformanceData::add does (performanceData.cpp:34-80); ``errorFunction(x)``, when given, is evaluated at every
+    point with the clock paused. Points come from the library through a callback on the calling thread.
+    """
+
+    class DataPoint(tuple):
+        __slots__ = ()
+        _names = ("iterationCount", "elapsedTime", "residual", "error", "ranks", "flags")
+
+        def __new__(cls, *vals):
+            return tuple.__new__(cls, vals)
+
+        def __getattr__(self, name):
+            try:
+                return self[self._names.index(name)]
+            except ValueError:
+                raise AttributeError(name) from None
+
+    def __init__(self, printProgress=False, active=True, errorFunction=None):
+        self.printProgress = printProgress
+        self.active = active
+        self.errorFunction = errorFunction
+        self.data = []
+        self.additionalInformation = ""
+
+    def __bool__(self):
+        return self.active
+
+    def reset(self):
+        self.data = []
+        self.additionalInformation = ""
+
+    def _run(self, fn, x):
+        """Run fn() with this object as the thread's sink; x is the tensor the error function is evaluated on."""
+        if not self.active:
+            return fn()
+
+        def cb(_user, it, us, res, ranks, nranks, flags):
+            err = 0.0
+            if self.errorFunction is not None:
+                err = float(self.errorFunction(x))
+            rk = [ranks[i] for i in range(nranks)]
+            self.data.append(PerformanceData.DataPoint(int(it), float(us), float(res), err, rk, int(flags)))
+            if self.printProgress:
+                print(f"Iteration {it:4d} Time: {us * 1e-6:6.2f}s Residual: {res:11.6e} Flags: {flags} Ranks: {rk}")
+
+        hook = _lib.PERF_CALLBACK(cb)
+        call("xt_perf_sink", hook, None)
+        try:
+            return fn()
+        finally:
+            call("xt_perf_sink", _lib.PERF_CALLBACK(), None)
+
+    def dump_to_file(self, path):
+        """One line per point: iteration, time (µs), residual, error, flags, ranks (whitespace separated)."""
+        with open(path, "w") as f:
+            f.write(self.additionalInformation)
+            for p in self.data:
+                f.write(f"{p.iterationCount} {p.elapsedTime:.0f} {p.residual!r} {p.error!r} {p.flags} {' '.join(map(str, p.ranks))}\n")
+
+
 class ALSVariant:
     """ALSVariant (als.h:37-223). Extra fields choose the local solve on the device: ``precision``
     (XT_F64 / XT_F32 / XT_BF16 local matvecs, the latter two with FP64 refinement), ``localTolerance``,
-    ``maxLocalIterations``, ``innerTolerance``, ``maxRefinements``."""
+    ``maxLocalIterations``, ``innerTolerance``, ``maxRefinements``. A PerformanceData passed as the fourth
+    argument (or ``perfData=``) records the run."""
+
+    FLAG_FINISHED_HALFSWEEP = 1  # als.h:109-110
+    FLAG_FINISHED_FULLSWEEP = 3
 
     def __init__(self, sites: int, numHalfSweeps: int, assumeSPD: bool, useResidual: bool = False):
         if sites <= 0:
@@ -337,7 +408,13 @@ class ALSVariant:
                 o.convergenceEpsilon = float(extra)
         energy = C.c_double()
         st = _lib.XtALSStats()
-        call("xt_als_solve", C.byref(o), A._h if A is not None else None, x._h, b._h, C.byref(energy), C.byref(st))
+        perf = kw.get("perfData", args[3] if len(args) > 3 else None)
+        run = lambda: call("xt_als_solve", C.byref(o), A._h if A is not None else None, x._h, b._h, C.byref(energy),
+                           C.byref(st))
+        if perf is not None:
+            perf._run(run, x)
+        else:
+            run()
         self.last_stats = {f: getattr(st, f) for f, _ in _lib.XtALSStats._fields_}
         return energy.value
 
@@ -445,14 +522,20 @@ class ADFVariant:
         self.precision = XT_F64
         self.last_iterations = None
 
-    def __call__(self, x: TTTensor, measurements: SinglePointMeasurementSet, maxRanks=None) -> float:
-        """Returns ‖P_Ω(x) − b‖/‖b‖; x is updated in place."""
+    def __call__(self, x: TTTensor, measurements: SinglePointMeasurementSet, maxRanks=None, perfData=None) -> float:
+        """Returns ‖P_Ω(x) − b‖/‖b‖; x is updated in place (adf.h:248-266)."""
+        if isinstance(maxRanks, PerformanceData):
+            maxRanks, perfData = None, maxRanks
         p = None
         if maxRanks is not None:
             _arr, p = _sizes(maxRanks)
         res, it = C.c_double(), C.c_size_t()
-        call("xt_adf_solve", x._h, measurements._h, p, self.maxIterations, self.targetResidualNorm,
-             self.minimalResidualNormDecrease, self.precision, C.byref(res), C.byref(it))
+        run = lambda: call("xt_adf_solve", x._h, measurements._h, p, self.maxIterations, self.targetResidualNorm,
+                           self.minimalResidualNormDecrease, self.precision, C.byref(res), C.byref(it))
+        if perfData is not None:
+            perfData._run(run, x)
+        else:
+            run()
         self.last_iterations = it.value
         return res.value
 
