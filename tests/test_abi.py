@@ -69,3 +69,20 @@ def test_gpus_flag_refuses_multi_gpu():
     import sys
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_refuses_foreign_hip_runtime():
+    """With torch (and its bundled HIP runtime) imported first, loading libxtt must fail loudly instead of binding
+    libxtt to that runtime; loading libxtt first keeps it on the ROCm runtime it was built for."""
+    import subprocess
+    import sys
+    torch_first = ("import sys; sys.path.insert(0, %r); import torch\n"
+                   "try:\n    import xerus_amd\n    xerus_amd.lib()\nexcept RuntimeError as e:\n"
+                   "    print('refused', 'HIP runtime' in str(e))\n") % ROOT
+    out = subprocess.run([sys.executable, "-c", torch_first], capture_output=True, text=True, timeout=300)
+    assert "refused True" in out.stdout, out.stdout + out.stderr
+    xtt_first = ("import sys; sys.path.insert(0, %r); import xerus_amd; xerus_amd.lib(); import torch\n"
+                 "maps = open('/proc/self/maps').read()\n"
+                 "print('ok', '/opt/rocm' in maps and 'libxtt' in maps)\n") % ROOT
+    out = subprocess.run([sys.executable, "-c", xtt_first], capture_output=True, text=True, timeout=300)
+    assert "ok True" in out.stdout, out.stdout + out.stderr

@@ -130,12 +130,33 @@ class XttError(RuntimeError):
         self.code = code
 
 
+def _foreign_hip_runtime():
+    """Path of a HIP runtime mapped into this process from outside the ROCm installation (torch bundles one),
+    or None. libxtt resolves libamdhip64.so.7 to an already mapped runtime of that name, so it must be loaded
+    before such a runtime appears."""
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                path = line.split()[-1] if line.strip() else ""
+                if "libamdhip64" in path and not path.startswith("/opt/rocm"):
+                    return path
+    except OSError:
+        pass
+    return None
+
+
 def lib() -> C.CDLL:
-    """Load libxtt.so once; raises if it has not been built."""
+    """Load libxtt.so once; raises if it has not been built, or if another HIP runtime (torch's bundled copy)
+    is already mapped, which libxtt would otherwise bind to."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"libxtt.so not built: {LIB_PATH} missing (run `make -C xerus_amd/csrc`)")
+        foreign = _foreign_hip_runtime()
+        if foreign is not None:
+            raise RuntimeError(f"a HIP runtime from {foreign} is already loaded (torch imported first?); import xerus_amd "
+                               "and call xerus_amd.lib() before importing torch, so libxtt runs on the ROCm runtime it was "
+                               "built for")
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
