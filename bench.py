@@ -113,6 +113,28 @@ def bench_adf(xe, np, steps):
             "grad_kernel_ms": g[1] / g[0] if g else None, "residual_kernel_ms": rk[1] / rk[0] if rk else None}
 
 
+def bench_batch(xe, np, count, sweeps, threads, precision):
+    """Many independent ALS_SPD + round problems of uneven size (d ~ U{6..20}, n ∈ {2,4,8,16}, r ~ U{4..64},
+    rank-3 Laplacian + potential operators, rank-4 right-hand sides) through the batched driver: worker threads
+    with a stream each. Returns aggregate sweeps/s (count·sweeps / wall time) and residual statistics."""
+    from xerus_amd.workloads import batch_problems
+    probs = batch_problems(count)
+    als = xe.ALS_SPD.copy()
+    als.convergenceEpsilon = 0.0
+    als.numHalfSweeps = 2
+    als.precision = precision
+    xe.lib().xt_synchronize()
+    t0 = time.perf_counter()
+    res, st = xe.als_round_batch(als, [p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs], sweeps, 1e-12,
+                                 threads=threads)
+    dt = time.perf_counter() - t0
+    shapes = np.array([p[3] for p in probs])
+    return {"problems": count, "sweeps_per_problem": sweeps, "threads": threads, "seconds": dt,
+            "sweeps_per_s": count * sweeps / dt, "residual_max": float(np.max(res)), "residual_median": float(np.median(res)),
+            "d_mean": float(shapes[:, 0].mean()), "n_mean": float(shapes[:, 1].mean()), "r_mean": float(shapes[:, 2].mean()),
+            "cg_iterations": st["cgIterations"], "local_solves": st["localSolves"]}
+
+
 def cpu_baseline(np):
     """The oracle (FP64 CPU restatement of xerus's ALS_SPD, matrix-free CG local solver) on the same
     config-2 problem, bounded to the first half-sweep."""
@@ -141,6 +163,9 @@ def main():
     ap.add_argument("--adf-steps", type=int, default=200)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-problems", type=int, default=512)
+    ap.add_argument("--batch-sweeps", type=int, default=2)
+    ap.add_argument("--batch-threads", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is described in DESIGN.md",
@@ -189,6 +214,11 @@ def main():
         adf["grad_kernel_GBs"] = adf["alg_bytes"] / (adf["grad_kernel_ms"] * 1e-3) / 1e9
         adf["grad_kernel_hbm_frac"] = adf["grad_kernel_GBs"] / PEAK_HBM_GBS
 
+    batch = None
+    if args.batch_problems > 0:
+        batch = bench_batch(xe, np, args.batch_problems, args.batch_sweeps, args.batch_threads, prec)
+        print(f"# batch: {batch['sweeps_per_s']:.1f} sweeps/s over {args.batch_problems} problems", file=sys.stderr, flush=True)
+
     cpu = None if args.no_cpu else cpu_baseline(np)
 
     value = args.steps / t_mixed
@@ -212,6 +242,7 @@ def main():
         "fp64_gpu": fp64,
         "speedup_vs_fp64": (value / fp64["sweeps_per_s"]) if fp64 else None,
         "adf": adf,
+        "batch": batch,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -140,6 +140,13 @@ typedef void (*xt_perf_callback)(void* user, size_t iteration, double elapsed_us
                                  size_t nranks, size_t flags);
 int xt_perf_sink(xt_perf_callback cb, void* user);
 
+/* Many independent problems (no single xerus call; a loop over ALSVariant::operator() and TTNetwork::round,
+ * als.h:152-207, ttNetwork.cpp:643-684): problem p runs `sweeps` × (ALS with options o, then round(eps)) on one
+ * of `threads` worker threads, each with a stream of its own. A[p] may be NULL (x ≈ b). residuals (optional, one
+ * per problem) receives ‖A x − b‖/‖b‖; st (optional) the summed counters. */
+int xt_als_round_batch(const xt_als_options* o, size_t count, const xt_tt* A, const xt_tt* x, const xt_tt* b, size_t sweeps,
+                       double eps, int threads, double* residuals, xt_als_stats* st);
+
 /* ------------------------------------------- SinglePointMeasurementSet (measurments.h:43-91) and ADF */
 int xt_meas_random(size_t P, size_t order, const size_t* dims, xt_meas* out);  /* measurments.cpp:222-248 */
 int xt_meas_new(size_t P, size_t order, const uint32_t* positions, const double* values, xt_meas* out);

@@ -301,3 +301,26 @@ def test_performance_data(xe, oracle):
     res = adf(y, m, pa)
     assert [p.iterationCount for p in pa.data] == list(range(len(pa.data)))
     assert abs(pa.data[-1].residual - res) < 1e-12 * max(res, 1e-30)
+
+
+def test_batch_matches_sequential(xe):
+    """Problems run by the batched driver (worker threads, one stream each) give the same TTs, ranks and
+    residuals as the same problems run one after the other on the library stream."""
+    print("batch vs sequential", flush=True)
+    from xerus_amd.workloads import batch_problems
+    probs = batch_problems(6, d_range=(5, 8), n_choices=(2, 4), r_range=(3, 9))
+    twins = batch_problems(6, d_range=(5, 8), n_choices=(2, 4), r_range=(3, 9))
+    als = xe.ALS_SPD.copy()
+    als.convergenceEpsilon = 0.0
+    als.numHalfSweeps = 2
+    als.precision = xe.XT_F32
+    res, stats = xe.als_round_batch(als, [p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs], 2, 1e-12,
+                                    threads=3)
+    assert stats["localSolves"] > 0
+    for (A, x, b, shape), (A2, x2, b2, _), r in zip(probs, twins, res):
+        for _ in range(2):
+            als(A2, x2, b2, 2)
+            x2.round(1e-12)
+        assert x.ranks() == x2.ranks(), shape
+        assert rel(x.full(), x2.full()) < 1e-10, shape
+        assert abs(r - xe.residual(A2, x2, b2)) < 1e-9 * max(r, 1e-12)

@@ -4,10 +4,10 @@ The product is libxtt.so (HIP/C++ for gfx950, C ABI in include/xtt.h); this pack
 """
 from ._lib import XT_BF16, XT_F32, XT_F64, XttError, lib
 from .tt import (ADF, ALS, ALS_SPD, DMRG, DMRG_SPD, EPSILON, ADFVariant, ALSVariant, PerformanceData, SinglePointMeasurementSet, TTOperator,
-                 TTTensor, adf_gradient, frob_norm, normal_draws, residual, seed, uniform_draws)
+                 TTTensor, adf_gradient, als_round_batch, frob_norm, normal_draws, residual, seed, uniform_draws)
 
 __all__ = [
     "ADF", "ALS", "ALS_SPD", "DMRG", "DMRG_SPD", "EPSILON", "ADFVariant", "ALSVariant", "PerformanceData", "SinglePointMeasurementSet",
-    "TTOperator", "TTTensor", "adf_gradient", "frob_norm", "normal_draws", "residual", "seed", "uniform_draws",
+    "TTOperator", "TTTensor", "adf_gradient", "als_round_batch", "frob_norm", "normal_draws", "residual", "seed", "uniform_draws",
     "XT_BF16", "XT_F32", "XT_F64", "XttError", "lib",
 ]

@@ -104,6 +104,8 @@ SIGNATURES = {
     "xt_tt_residual": (C.c_int, [_vp, _vp, _vp, _pd]),
     "xt_als_default_options": (C.c_int, [C.POINTER(XtALSOptions), C.c_uint, C.c_int]),
     "xt_als_solve": (C.c_int, [C.POINTER(XtALSOptions), _vp, _vp, _vp, _pd, C.POINTER(XtALSStats)]),
+    "xt_als_round_batch": (C.c_int, [C.POINTER(XtALSOptions), _sz, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), _sz,
+                                     C.c_double, C.c_int, _pd, C.POINTER(XtALSStats)]),
     "xt_meas_random": (C.c_int, [_sz, _sz, _ps, C.POINTER(_vp)]),
     "xt_meas_new": (C.c_int, [_sz, _sz, _pu32, _pd, C.POINTER(_vp)]),
     "xt_meas_free": (C.c_int, [_vp]),

@@ -19,8 +19,12 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
+#include <atomic>
+#include <exception>
 #include <map>
+#include <mutex>
 #include <set>
+#include <thread>
 
 #include "tt.hpp"
 
@@ -218,7 +222,7 @@ struct Workspace {
 };
 
 Workspace& workspace() {
-	static Workspace w;
+	static thread_local Workspace w;  // per thread: its graphs replay on that thread's stream
 	return w;
 }
 
@@ -610,6 +614,63 @@ double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats*
 		if (data.end_of_sweep()) return data.energy;
 		data.next();
 	}
+}
+
+// Many independent problems at once: problem p runs `sweeps` × (ALS with options o, then round(eps)) on one of
+// `threads` worker threads. Every worker binds a stream of its own (default priority), so its allocations,
+// captured graphs and kernels stay on that stream and the latency-bound launches of different problems overlap
+// on the device. residuals[p] (optional) = ‖A x − b‖/‖b‖ afterwards.
+void als_round_batch(const ALSOptions& o, const std::vector<const TT*>& A, const std::vector<TT*>& x,
+                     const std::vector<const TT*>& b, size_t sweeps, double eps, int threads, double* residuals,
+                     ALSStats* total) {
+	const size_t count = x.size();
+	XTT_REQUIRE(A.size() == count && b.size() == count, "batch: A, x and b lists differ in length");
+	XTT_REQUIRE(threads >= 1 && threads <= 64, "batch: 1..64 worker threads");
+	int dev = 0;
+	XTT_HIP(hipGetDevice(&dev));
+	sync();  // inputs built on the library stream are complete before workers use them
+	std::atomic<size_t> next{0};
+	std::mutex mu;
+	std::exception_ptr err;
+	auto worker = [&]() {
+		hipStream_t s = nullptr;
+		try {
+			XTT_HIP(hipSetDevice(dev));
+			XTT_HIP(hipStreamCreateWithFlags(&s, hipStreamDefault));
+			bind_thread_stream(s);
+			ALSStats st;
+			for (size_t p = next++; p < count; p = next++) {
+				TT& xp = *x[p];
+				const dims_t noCap(xp.order() > 0 ? xp.order() - 1 : 0, SIZE_MAX);
+				for (size_t sw = 0; sw < sweeps; ++sw) {
+					als_solve(o, A[p], xp, *b[p], &st);
+					round(xp, noCap, eps);
+				}
+				if (residuals) residuals[p] = A[p] ? tt_residual(*A[p], xp, *b[p]) : 0.0;
+			}
+			sync();
+			if (total) {
+				std::lock_guard<std::mutex> lk(mu);
+				total->halfSweeps += st.halfSweeps; total->localSolves += st.localSolves;
+				total->cgIterations += st.cgIterations; total->refinements += st.refinements;
+				total->matvecs64 += st.matvecs64; total->matvecsLow += st.matvecsLow;
+				total->flopsMatvec += st.flopsMatvec; total->flopsEnv += st.flopsEnv;
+			}
+		} catch (...) {
+			std::lock_guard<std::mutex> lk(mu);
+			if (!err) err = std::current_exception();
+			next = count;  // the other workers stop after their current problem
+		}
+		if (s) {
+			(void)hipStreamSynchronize(s);
+			bind_thread_stream(nullptr);
+			(void)hipStreamDestroy(s);
+		}
+	};
+	std::vector<std::thread> pool;
+	for (int t = 0; t < threads; ++t) pool.emplace_back(worker);
+	for (auto& t : pool) t.join();
+	if (err) std::rethrow_exception(err);
 }
 
 }  // namespace xtt

@@ -272,6 +272,51 @@ int xt_als_default_options(xt_als_options* o, unsigned sites, int spd) {
 		o->maxRefinements = d.maxRefine;
 	});
 }
+extern "C++" {
+namespace {
+xtt::ALSOptions als_options(const xt_als_options* o) {
+	XTT_REQUIRE(o->precision >= XT_F64 && o->precision <= XT_BF16, "unknown precision");
+	xtt::ALSOptions p;
+	p.sites = o->sites;
+	p.numHalfSweeps = o->numHalfSweeps;
+	p.convergenceEpsilon = o->convergenceEpsilon;
+	p.useResidualForEndCriterion = o->useResidualForEndCriterion;
+	p.preserveCorePosition = o->preserveCorePosition;
+	p.assumeSPD = o->assumeSPD;
+	p.precision = o->precision;
+	p.localTol = o->localTolerance;
+	p.maxLocalIter = o->maxLocalIterations;
+	p.innerTol = o->innerTolerance;
+	p.maxRefine = o->maxRefinements;
+	return p;
+}
+void put_stats(const xtt::ALSStats& s, xt_als_stats* st) {
+	if (!st) return;
+	st->halfSweeps = s.halfSweeps; st->localSolves = s.localSolves; st->cgIterations = s.cgIterations;
+	st->refinements = s.refinements; st->matvecs64 = s.matvecs64; st->matvecsLow = s.matvecsLow;
+	st->flopsMatvec = s.flopsMatvec; st->flopsEnv = s.flopsEnv;
+}
+}  // namespace
+}  // extern "C++"
+
+int xt_als_round_batch(const xt_als_options* o, size_t count, const xt_tt* A, const xt_tt* x, const xt_tt* b, size_t sweeps,
+                       double eps, int threads, double* residuals, xt_als_stats* st) {
+	return guard([&] {
+		NN(o); NN(x); NN(b);
+		std::vector<const xtt::TT*> av(count), bv(count);
+		std::vector<xtt::TT*> xv(count);
+		for (size_t p = 0; p < count; ++p) {
+			NN(x[p]); NN(b[p]);
+			av[p] = A && A[p] ? &A[p]->t : nullptr;
+			xv[p] = &x[p]->t;
+			bv[p] = &b[p]->t;
+		}
+		xtt::ALSStats s;
+		xtt::als_round_batch(als_options(o), av, xv, bv, sweeps, eps, threads, residuals, &s);
+		put_stats(s, st);
+	});
+}
+
 int xt_als_solve(const xt_als_options* o, xt_tt A, xt_tt x, xt_tt b, double* energy, xt_als_stats* st) {
 	return guard([&] {
 		NN(o); NN(x); NN(b); NN(energy);

@@ -1,6 +1,8 @@
 // libxtt runtime: thread-local error text, the per-device library stream and the caching device allocator.
-#include <mutex>
 #include <chrono>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <unordered_map>
 
 #include "xtt_common.hpp"
@@ -16,10 +18,13 @@ struct DeviceState {
 std::mutex g_mu;
 std::unordered_map<int, DeviceState> g_dev;
 
-// Allocation cache: size class (rounded to 256 B below 1 MiB, to 64 KiB above) → free blocks.
+// Allocation cache: (device, stream, size class rounded to 256 B below 1 MiB, to 64 KiB above) → free blocks.
+// A block freed while a stream is current goes back to that stream's pool, so reuse stays ordered by the
+// stream that last used it.
 struct Block { size_t bytes; int device; };
+using PoolKey = std::tuple<int, hipStream_t, size_t>;
 std::mutex g_amu;
-std::multimap<std::pair<int, size_t>, void*> g_free;
+std::multimap<PoolKey, void*> g_free;
 std::unordered_map<void*, Block> g_live;
 size_t g_inUse = 0;
 
@@ -32,7 +37,13 @@ size_t size_class(size_t b) {
 void set_error(const std::string& s) { g_error = s; }
 const std::string& last_error() { return g_error; }
 
+namespace {
+thread_local hipStream_t t_stream = nullptr;
+}
+void bind_thread_stream(hipStream_t s) { t_stream = s; }
+
 hipStream_t stream() {
+	if (t_stream) return t_stream;
 	int dev = 0;
 	XTT_HIP(hipGetDevice(&dev));
 	std::lock_guard<std::mutex> lk(g_mu);
@@ -48,9 +59,10 @@ void* dmalloc(size_t bytes) {
 	int dev = 0;
 	XTT_HIP(hipGetDevice(&dev));
 	const size_t cls = size_class(bytes);
+	const hipStream_t st = stream();
 	{
 		std::lock_guard<std::mutex> lk(g_amu);
-		auto it = g_free.find({dev, cls});
+		auto it = g_free.find(PoolKey{dev, st, cls});
 		if (it != g_free.end()) {
 			void* p = it->second;
 			g_free.erase(it);
@@ -78,16 +90,17 @@ void* dmalloc(size_t bytes) {
 
 void dfree(void* p) {
 	if (!p) return;
+	const hipStream_t st = stream();
 	std::lock_guard<std::mutex> lk(g_amu);
 	auto it = g_live.find(p);
 	if (it == g_live.end()) return;
-	g_free.insert({{it->second.device, it->second.bytes}, p});
+	g_free.insert({PoolKey{it->second.device, st, it->second.bytes}, p});
 	g_inUse -= it->second.bytes;
 	g_live.erase(it);
 }
 
 void release_cache() {
-	std::multimap<std::pair<int, size_t>, void*> blocks;
+	std::multimap<PoolKey, void*> blocks;
 	{
 		std::lock_guard<std::mutex> lk(g_amu);
 		blocks.swap(g_free);

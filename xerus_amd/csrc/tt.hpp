@@ -92,5 +92,9 @@ struct ALSStats {
 	std::vector<double> energies;
 };
 double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats* stats);
+// problem p: `sweeps` × (als_solve, round(eps)) on one of `threads` worker threads with streams of their own
+void als_round_batch(const ALSOptions& o, const std::vector<const TT*>& A, const std::vector<TT*>& x,
+                     const std::vector<const TT*>& b, size_t sweeps, double eps, int threads, double* residuals,
+                     ALSStats* total);
 
 }  // namespace xtt

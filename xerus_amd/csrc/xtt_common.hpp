@@ -70,6 +70,9 @@ struct PerfClock {
 // The library's stream on the current device (default priority, blocking: it orders with the legacy null
 // stream torch uses), created on first use.
 hipStream_t stream();
+// Binds the calling thread to a stream of its own (nullptr: back to the device's library stream). Work and
+// cached allocations of that thread then stay on it; used by the batched drivers, one stream per worker.
+void bind_thread_stream(hipStream_t s);
 // Stream used by an entry point: the caller's if given, else the library stream.
 inline hipStream_t pick(void* s) { return s ? static_cast<hipStream_t>(s) : stream(); }
 void sync();

@@ -419,6 +419,24 @@ class ALSVariant:
         return energy.value
 
 
+def als_round_batch(variant: "ALSVariant", As, xs, bs, sweeps: int, eps: float, threads: int = 8, residuals: bool = True):
+    """Many independent problems: problem p runs ``sweeps`` × (variant(As[p], xs[p], bs[p]), xs[p].round(eps)) on
+    one of ``threads`` worker threads, each with its own stream (xs updated in place). Returns the relative
+    residuals ‖A x − b‖/‖b‖ (or None) and the summed solver counters."""
+    n = len(xs)
+    if not (len(As) == n and len(bs) == n):
+        raise ValueError("As, xs and bs must have the same length")
+    o = variant._options()
+    A_arr = (C.c_void_p * n)(*[a._h if a is not None else None for a in As])
+    x_arr = (C.c_void_p * n)(*[x._h for x in xs])
+    b_arr = (C.c_void_p * n)(*[b._h for b in bs])
+    res = np.zeros(n) if residuals else None
+    st = _lib.XtALSStats()
+    call("xt_als_round_batch", C.byref(o), n, A_arr, x_arr, b_arr, sweeps, eps, threads,
+         res.ctypes.data_as(C.POINTER(C.c_double)) if residuals else None, C.byref(st))
+    return res, {f: getattr(st, f) for f, _ in _lib.XtALSStats._fields_}
+
+
 ALS = ALSVariant(1, 0, False)
 ALS_SPD = ALSVariant(1, 0, True)
 DMRG = ALSVariant(2, 0, False)

@@ -42,3 +42,25 @@ def laplace_cores(d, n, potentials=None, rank3=False):
             c = np.zeros((2, n, n, 2)); c[0, :, :, 0] = I; c[1, :, :, 0] = H; c[1, :, :, 1] = I
             cores.append(c)
     return cores
+
+
+def batch_problems(count, base_seed=0xBAADF00D, d_range=(6, 20), n_choices=(2, 4, 8, 16), r_range=(4, 64), rhs_rank=4):
+    """Independent ALS_SPD problems of uneven size: problem p draws d ~ U{d_range}, n from n_choices and the
+    solution rank r ~ U{r_range} (then clipped by TTTensor.random's reduce_to_maximal_ranks) with numpy's
+    default_rng(base_seed + p); the operator is the rank-3 Laplacian + U(0,1) potential of laplace_cores, and b
+    (rank rhs_rank) and x0 (rank r) are TTTensor.random draws after seed(base_seed + p).
+    Returns a list of (A, x0, b, (d, n, r))."""
+    from . import tt as _tt
+    out = []
+    for p in range(count):
+        rng = np.random.default_rng(base_seed + p)
+        d = int(rng.integers(d_range[0], d_range[1] + 1))
+        n = int(rng.choice(n_choices))
+        r = int(rng.integers(r_range[0], r_range[1] + 1))
+        pots = rng.random((d, n))
+        A = _tt.TTOperator.from_components(laplace_cores(d, n, list(pots), rank3=True))
+        _tt.seed(base_seed + p)
+        b = _tt.TTTensor.random([n] * d, [rhs_rank] * (d - 1))
+        x = _tt.TTTensor.random([n] * d, [r] * (d - 1))
+        out.append((A, x, b, (d, n, r)))
+    return out
