@@ -598,26 +598,22 @@ void adf_residual(const AdfDesc& a, const T* cores, const uint32_t* pos, const T
 	const int blocks = std::max(1, std::min(1024, (a.P + AB - 1) / AB));
 	if (std::is_same<T, float>::value && fast_ok(a)) {
 		Padded pc = pad_cores(a, reinterpret_cast<const float*>(cores), s);
-		const int slot = prof_begin(s);
-		hipLaunchKernelGGL(adf_residual_fast_kernel<T>, dim3(blocks), dim3(AB), (size_t)a.sumN * PB * sizeof(float), s, a, pc.Gp.get(),
-		                   pc.off.get(), pos, values, resid,
-		                   norms, partials, counter);
-		XTT_LAUNCH_CHECK();
-		prof_end(slot, s, "adf_residual_fast_kernel<float>", 0.0);
+		launch_timed("adf_residual_fast_kernel<float>", 0.0, adf_residual_fast_kernel<T>, dim3(blocks), dim3(AB),
+		             (size_t)a.sumN * PB * sizeof(float), s, a, (const float*)pc.Gp.get(), (const long long*)pc.off.get(), pos, values,
+		             resid, norms, partials, counter);
 		return;
 	}
-	const int slot = prof_begin(s);
 	const size_t coreBytes = (size_t)a.gradSize * sizeof(T);
 	const bool lds = coreBytes <= 48 * 1024;
+	const char* tag = sizeof(T) == 8 ? "adf_residual_kernel<double>" : "adf_residual_kernel<float>";
+	const dim3 g(blocks), t(AB);
 	if (a.maxR <= 8) {
-		if (lds) hipLaunchKernelGGL((adf_residual_kernel<T, 8, true>), dim3(blocks), dim3(AB), coreBytes, s, a, cores, pos, values, resid, norms, partials, counter);
-		else hipLaunchKernelGGL((adf_residual_kernel<T, 8, false>), dim3(blocks), dim3(AB), 0, s, a, cores, pos, values, resid, norms, partials, counter);
+		if (lds) launch_timed(tag, 0.0, adf_residual_kernel<T, 8, true>, g, t, coreBytes, s, a, cores, pos, values, resid, norms, partials, counter);
+		else launch_timed(tag, 0.0, adf_residual_kernel<T, 8, false>, g, t, 0, s, a, cores, pos, values, resid, norms, partials, counter);
 	} else {
-		if (lds) hipLaunchKernelGGL((adf_residual_kernel<T, RMAX, true>), dim3(blocks), dim3(AB), coreBytes, s, a, cores, pos, values, resid, norms, partials, counter);
-		else hipLaunchKernelGGL((adf_residual_kernel<T, RMAX, false>), dim3(blocks), dim3(AB), 0, s, a, cores, pos, values, resid, norms, partials, counter);
+		if (lds) launch_timed(tag, 0.0, adf_residual_kernel<T, RMAX, true>, g, t, coreBytes, s, a, cores, pos, values, resid, norms, partials, counter);
+		else launch_timed(tag, 0.0, adf_residual_kernel<T, RMAX, false>, g, t, 0, s, a, cores, pos, values, resid, norms, partials, counter);
 	}
-	XTT_LAUNCH_CHECK();
-	prof_end(slot, s, sizeof(T) == 8 ? "adf_residual_kernel<double>" : "adf_residual_kernel<float>", 0.0);
 }
 
 int adf_grad_blocks(int P) { return std::max(1, std::min(512, (P + 255) / 256)); }
@@ -635,11 +631,9 @@ void adf_gradient(const AdfDesc& a, const T* cores, const uint32_t* pos, const T
 			XTT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(adf_grad_fast_kernel), hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
 		});
 		const int blocks = std::min(adf_grad_blocks(a.P), 256);
-		const int slot = prof_begin(s);
-		hipLaunchKernelGGL(adf_grad_fast_kernel, dim3(blocks), dim3(FW * 64), fastLds, s, a, pc.Gp.get(), pc.off.get(), pos,
-		                   reinterpret_cast<const float*>(resid), reinterpret_cast<float*>(partials), kSel);
-		XTT_LAUNCH_CHECK();
-		prof_end(slot, s, "adf_grad_fast_kernel<float>", 0.0);
+		launch_timed("adf_grad_fast_kernel<float>", 0.0, adf_grad_fast_kernel, dim3(blocks), dim3(FW * 64), fastLds, s, a,
+		             (const float*)pc.Gp.get(), (const long long*)pc.off.get(), pos, reinterpret_cast<const float*>(resid),
+		             reinterpret_cast<float*>(partials), kSel);
 		hipLaunchKernelGGL(adf_grad_reduce_kernel<T>, dim3((a.gradSize + 63) / 64), dim3(256), 0, s, partials, blocks, a.gradSize, grad);
 		XTT_LAUNCH_CHECK();
 		return;
@@ -649,10 +643,8 @@ void adf_gradient(const AdfDesc& a, const T* cores, const uint32_t* pos, const T
 	if (lds > 64 * 1024)
 		XTT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(adf_grad_kernel<T>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 	const int blocks = adf_grad_blocks(a.P);
-	const int slot = prof_begin(s);
-	hipLaunchKernelGGL(adf_grad_kernel<T>, dim3(blocks), dim3(AB), lds, s, a, cores, pos, resid, partials, L, kSel);
-	XTT_LAUNCH_CHECK();
-	prof_end(slot, s, sizeof(T) == 8 ? "adf_grad_kernel<double>" : "adf_grad_kernel<float>", 0.0);
+	launch_timed(sizeof(T) == 8 ? "adf_grad_kernel<double>" : "adf_grad_kernel<float>", 0.0, adf_grad_kernel<T>, dim3(blocks),
+	             dim3(AB), lds, s, a, cores, pos, resid, partials, L, kSel);
 	hipLaunchKernelGGL(adf_grad_reduce_kernel<T>, dim3((a.gradSize + 63) / 64), dim3(256), 0, s, partials, blocks, a.gradSize, grad);
 	XTT_LAUNCH_CHECK();
 }

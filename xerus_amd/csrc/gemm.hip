@@ -262,21 +262,18 @@ void launch_x4(const GemmDesc& d, const T* A, const T* B, T* C, hipStream_t s) {
 	a.kc = (int)std::max<size_t>(4, std::min<size_t>((d.k + 3) & ~size_t(3), kmax));
 	const size_t lds = (size_t)a.kc * BMp * 2 * sizeof(T);
 	const double flops = 2.0 * d.m * d.n * d.k * d.batch;
-	const int slot = prof_begin(s);
 	if (big) {
 		auto kern = gemm_x4_kernel<T, 64, 64, 2, 2>;
 		allow_lds(kern, lds);
 		dim3 grid((unsigned)((d.n + 63) / 64), (unsigned)((d.m + 63) / 64), (unsigned)d.batch);
-		hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a);
-		XTT_LAUNCH_CHECK();
-		prof_end(slot, s, sizeof(T) == 8 ? "gemm_x4_kernel<double,64,64>" : "gemm_x4_kernel<float,64,64>", flops);
+		launch_timed(sizeof(T) == 8 ? "gemm_x4_kernel<double,64,64>" : "gemm_x4_kernel<float,64,64>", flops, kern, grid,
+		             dim3(256), lds, s, a);
 	} else {
 		auto kern = gemm_x4_kernel<T, 32, 32, 2, 2>;
 		allow_lds(kern, lds);
 		dim3 grid((unsigned)((d.n + 31) / 32), (unsigned)((d.m + 31) / 32), (unsigned)d.batch);
-		hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a);
-		XTT_LAUNCH_CHECK();
-		prof_end(slot, s, sizeof(T) == 8 ? "gemm_x4_kernel<double,32,32>" : "gemm_x4_kernel<float,32,32>", flops);
+		launch_timed(sizeof(T) == 8 ? "gemm_x4_kernel<double,32,32>" : "gemm_x4_kernel<float,32,32>", flops, kern, grid,
+		             dim3(256), lds, s, a);
 	}
 }
 
@@ -304,21 +301,16 @@ void gemm_bf16(const GemmDesc& d, const float* A, const float* B, float* C, hipS
 	a.kc = (int)std::max<size_t>(32, std::min<size_t>((d.k + 31) & ~size_t(31), kmax));
 	const size_t lds = rows * (size_t)(a.kc + 8) * sizeof(__bf16);
 	const double flops = 2.0 * d.m * d.n * d.k * d.batch;
-	const int slot = prof_begin(s);
 	if (big) {
 		auto kern = gemm_bf16_kernel<64, 64, 2, 2>;
 		allow_lds(kern, lds);
 		dim3 grid((unsigned)((d.n + 63) / 64), (unsigned)((d.m + 63) / 64), (unsigned)d.batch);
-		hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a);
-		XTT_LAUNCH_CHECK();
-		prof_end(slot, s, "gemm_bf16_kernel<64,64>", flops);
+		launch_timed("gemm_bf16_kernel<64,64>", flops, kern, grid, dim3(256), lds, s, a);
 	} else {
 		auto kern = gemm_bf16_kernel<32, 32, 2, 2>;
 		allow_lds(kern, lds);
 		dim3 grid((unsigned)((d.n + 31) / 32), (unsigned)((d.m + 31) / 32), (unsigned)d.batch);
-		hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a);
-		XTT_LAUNCH_CHECK();
-		prof_end(slot, s, "gemm_bf16_kernel<32,32>", flops);
+		launch_timed("gemm_bf16_kernel<32,32>", flops, kern, grid, dim3(256), lds, s, a);
 	}
 }
 

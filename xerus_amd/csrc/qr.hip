@@ -426,13 +426,34 @@ __global__ __launch_bounds__(HT) void hh_apply_kernel(ApplyArgs a) {
 
 int rows_per_thread(int rpb) { return rpb <= 8 * HW ? 8 : (rpb <= 16 * HW ? 16 : 32); }
 
+// Householder flops of one level (dgeqrf's count per block: 2n²(m − n/3) for m >= n, 2m²(n − m/3) otherwise),
+// plus, with FORMQ, k reflectors applied to m × nc: 4·nc·(m·k − k²/2).
+double factor_flops(const FactorArgs& a, int blocks, bool formq) {
+	double f = 0;
+	for (int b = 0; b < blocks; ++b) {
+		const double m = std::min(a.rpb, a.rows - b * a.rpb), n = a.n, k = std::min(m, n);
+		f += m >= n ? 2.0 * n * n * (m - n / 3.0) : 2.0 * m * m * (n - m / 3.0);
+		if (formq) f += 4.0 * a.nc * (m * k - k * k / 2.0);
+	}
+	return f;
+}
+
 template <bool PIV, bool FORMQ>
 void factor_variant(const FactorArgs& a, int blocks, hipStream_t s) {
 	const int rpt = rows_per_thread(a.rpb);
 	const dim3 g(blocks), t(HT);
-	if (rpt == 8) hipLaunchKernelGGL((hh_factor_kernel<8, PIV, FORMQ>), g, t, 0, s, a);
-	else if (rpt == 16) hipLaunchKernelGGL((hh_factor_kernel<16, PIV, FORMQ>), g, t, 0, s, a);
-	else hipLaunchKernelGGL((hh_factor_kernel<32, PIV, FORMQ>), g, t, 0, s, a);
+	static const char* tags[3][2] = {{"hh_factor_kernel<8,0,0>", "hh_factor_kernel<8,0,1>"},
+	                                 {"hh_factor_kernel<16,0,0>", "hh_factor_kernel<16,0,1>"},
+	                                 {"hh_factor_kernel<32,0,0>", "hh_factor_kernel<32,0,1>"}};
+	static const char* ptags[3][2] = {{"hh_factor_kernel<8,1,0>", "hh_factor_kernel<8,1,1>"},
+	                                  {"hh_factor_kernel<16,1,0>", "hh_factor_kernel<16,1,1>"},
+	                                  {"hh_factor_kernel<32,1,0>", "hh_factor_kernel<32,1,1>"}};
+	const int ri = rpt == 8 ? 0 : (rpt == 16 ? 1 : 2);
+	const char* tag = (PIV ? ptags : tags)[ri][FORMQ ? 1 : 0];
+	const double flops = prof_enabled() ? factor_flops(a, blocks, FORMQ) : 0.0;
+	if (rpt == 8) launch_timed(tag, flops, hh_factor_kernel<8, PIV, FORMQ>, g, t, 0, s, a);
+	else if (rpt == 16) launch_timed(tag, flops, hh_factor_kernel<16, PIV, FORMQ>, g, t, 0, s, a);
+	else launch_timed(tag, flops, hh_factor_kernel<32, PIV, FORMQ>, g, t, 0, s, a);
 }
 
 void launch_factor(const FactorArgs& a, int blocks, bool piv, hipStream_t s) {
@@ -454,7 +475,14 @@ void apply_rpt(const ApplyArgs& a, int blocks, hipStream_t s) {
 		XTT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&hh_apply_kernel<RPT>),
 		                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 	});
-	hipLaunchKernelGGL((hh_apply_kernel<RPT>), dim3(blocks), dim3(HT), lds, s, a);
+	double f = 0;
+	if (prof_enabled())
+		for (int b = 0; b < blocks; ++b) {
+			const double m = std::min(a.rpb, a.rows - b * a.rpb), k = std::min<double>(m, a.n);
+			f += 4.0 * a.nc * (m * k - k * k / 2.0);
+		}
+	launch_timed(RPT == 8 ? "hh_apply_kernel<8>" : (RPT == 16 ? "hh_apply_kernel<16>" : "hh_apply_kernel<32>"), f,
+	             hh_apply_kernel<RPT>, dim3(blocks), dim3(HT), lds, s, a);
 }
 
 void launch_apply(const ApplyArgs& a, int blocks, hipStream_t s) {

@@ -119,24 +119,19 @@ std::vector<Slot> g_slots;
 bool prof_enabled() { return g_prof; }
 void prof_enable(bool on) { std::lock_guard<std::mutex> lk(g_pmu); g_prof = on; }
 
-int prof_begin(hipStream_t s) {
-	if (!g_prof) return -1;
+ProfEvents prof_events(const char* tag, double flops) {
+	ProfEvents e;
+	if (!g_prof) return e;
 	Slot sl;
 	XTT_HIP(hipEventCreate(&sl.a));
 	XTT_HIP(hipEventCreate(&sl.b));
-	XTT_HIP(hipEventRecord(sl.a, s));
-	std::lock_guard<std::mutex> lk(g_pmu);
-	g_slots.push_back(sl);
-	return (int)g_slots.size() - 1;
-}
-
-void prof_end(int slot, hipStream_t s, const char* tag, double flops) {
-	if (slot < 0) return;
-	std::lock_guard<std::mutex> lk(g_pmu);
-	Slot& sl = g_slots[slot];
 	sl.tag = tag;
 	sl.flops = flops;
-	XTT_HIP(hipEventRecord(sl.b, s));
+	e.start = sl.a;
+	e.stop = sl.b;
+	std::lock_guard<std::mutex> lk(g_pmu);
+	g_slots.push_back(sl);
+	return e;
 }
 
 std::vector<ProfRecord> prof_collect() {

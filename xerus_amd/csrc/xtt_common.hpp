@@ -2,6 +2,7 @@
 // device tensor type used by the TT layer. Everything here is host code compiled by hipcc for gfx950.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstddef>
 #include <cstdint>
@@ -38,14 +39,27 @@ struct Error : std::runtime_error {
 
 void set_error(const std::string& s);
 
-// Optional launch profiler: while enabled, instrumented launches record a HIP event pair on their own stream,
-// tagged with the kernel instantiation and its algorithmic flop count. Read back after a synchronisation.
+// Optional launch profiler: while enabled, launches made through launch_timed hand a HIP event pair to the
+// dispatch itself (hipExtLaunchKernel), so the pair brackets the kernel's execution and not the host's
+// submission gap. Each record carries the kernel instantiation's tag and its algorithmic flop count; read
+// back after a synchronisation.
 struct ProfRecord { std::string tag; size_t launches = 0; double ms = 0, flops = 0; };
+struct ProfEvents { hipEvent_t start = nullptr, stop = nullptr; };
 bool prof_enabled();
 void prof_enable(bool on);
-int prof_begin(hipStream_t s);                                     // returns a slot (or -1 when disabled)
-void prof_end(int slot, hipStream_t s, const char* tag, double flops);
+ProfEvents prof_events(const char* tag, double flops);            // registers one record; null events when disabled
 std::vector<ProfRecord> prof_collect();                            // synchronises, aggregates, clears
+
+template <class F, class... Args>
+void launch_timed(const char* tag, double flops, F kern, dim3 grid, dim3 block, size_t lds, hipStream_t s, Args... args) {
+	if (prof_enabled()) {
+		const ProfEvents e = prof_events(tag, flops);
+		hipExtLaunchKernelGGL(kern, grid, block, (std::uint32_t)lds, s, e.start, e.stop, 0u, args...);
+	} else {
+		hipLaunchKernelGGL(kern, grid, block, lds, s, args...);
+	}
+	XTT_LAUNCH_CHECK();
+}
 
 // PerformanceData sink (performanceData.h:43-157): while a callback is set on the calling thread, the ALS and
 // ADF drivers report data points (iteration, elapsed µs with the residual evaluation excluded, residual,
