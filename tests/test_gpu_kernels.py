@@ -31,7 +31,8 @@ def host(t):
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 5), (64, 64, 64), (192, 1024, 64), (1024, 64, 192), (300, 130, 700)])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 5), (64, 64, 64), (192, 1024, 64), (1024, 64, 192), (300, 130, 700),
+                                   (64, 192, 1024), (4, 64, 1024), (33, 17, 2001), (36, 108, 576)])
 def test_gemm_f64(xe, ta, tb, m, n, k):
     from xerus_amd._lib import call
     print(f"gemm_f64 ta={ta} tb={tb} m={m} n={n} k={k}", flush=True)
@@ -48,7 +49,7 @@ def test_gemm_f64(xe, ta, tb, m, n, k):
 
 @pytest.mark.parametrize("prec,tol", [("f32", 1e-5), ("bf16", 2e-2)])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1)])
-@pytest.mark.parametrize("m,n,k", [(48, 64, 48), (1024, 64, 192), (33, 17, 100)])
+@pytest.mark.parametrize("m,n,k", [(48, 64, 48), (1024, 64, 192), (33, 17, 100), (64, 192, 1024), (5, 36, 577)])
 def test_gemm_lowp(xe, prec, tol, ta, tb, m, n, k):
     from xerus_amd._lib import call
     print(f"gemm_{prec} ta={ta} tb={tb} m={m} n={n} k={k}", flush=True)

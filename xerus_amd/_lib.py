@@ -181,7 +181,7 @@ def profile_enable(on: bool) -> None:
     call("xt_profile_enable", int(on))
 
 
-def profile_read(max_records: int = 64):
+def profile_read(max_records: int = 4096):
     """Per-kernel {tag: (launches, total_ms, algorithmic_flops)} since the last read (synchronises)."""
     tag_len = 96
     tags = C.create_string_buffer(max_records * tag_len)

@@ -382,18 +382,20 @@ struct ALSData {
 
 	static constexpr int kChunk = 16;
 
-	// FP64 CG on A_loc·x = b, warm start from x (device vector, overwritten).
+	// FP64 CG on A_loc·x = b, warm start from x (device vector, overwritten). The start residual is read back
+	// first, so a site that is already converged costs one matvec and one synchronisation.
 	void cg64(SiteOp& s, double* xout, const double* blin) {
 		const size_t N = s.N();
-		stage_site(s, false);
+		reserve(s);
 		double* x = ws.x.get();
 		const double* bl = ws.bl.get();
 		XTT_HIP(hipMemcpyAsync(x, xout, N * 8, hipMemcpyDeviceToDevice, stream()));
 		XTT_HIP(hipMemcpyAsync(ws.bl.get(), blin, N * 8, hipMemcpyDeviceToDevice, stream()));
-		ws_matvec64(s, x, ws.q.get(), nullptr);
-		count_matvec(s);
+		matvec64(s, x, ws.q.get(), nullptr);
 		cg_state_init(ws.st.get(), o.localTol * o.localTol, o.maxLocalIter, stream());
 		cg_start<double>(ws.r.get(), ws.p.get(), bl, ws.q.get(), N, ws.st.get(), ws.partials.get(), stream());
+		if (read_state().done) return;  // x unchanged
+		stage_site(s, false);
 		const int* done = &ws.st.get()->done;
 		const std::vector<size_t> key{64, s.a, s.n, s.c, s.P, s.Q};
 		int iters = 0;
@@ -414,27 +416,29 @@ struct ALSData {
 		XTT_HIP(hipMemcpyAsync(xout, x, N * 8, hipMemcpyDeviceToDevice, stream()));
 	}
 
-	// Mixed precision: FP64 iterative refinement around an FP32 / BF16 inner CG.
-	void cg_mixed(SiteOp& s, double* xout, const double* blin) {
+	// Mixed precision: FP64 iterative refinement around an FP32 / BF16 inner CG. The refinement works on the
+	// caller's vectors directly; the low-precision operator pieces are staged only once a correction is needed,
+	// and each residual check is one fused kernel and one synchronisation.
+	void cg_mixed(SiteOp& s, double* x, const double* bl) {
 		const size_t N = s.N();
-		stage_site(s, true);
-		double* x = ws.x.get();
-		const double* bl = ws.bl.get();
-		XTT_HIP(hipMemcpyAsync(x, xout, N * 8, hipMemcpyDeviceToDevice, stream()));
-		XTT_HIP(hipMemcpyAsync(ws.bl.get(), blin, N * 8, hipMemcpyDeviceToDevice, stream()));
+		reserve(s);
 		const double inner = o.innerTol > 0 ? o.innerTol : (o.precision == P_BF16 ? 3e-2 : 1e-5);
-		Scratch64 sc;
-		const double bb = sc.dot(bl, bl, N);
-		const double tol2 = o.localTol * o.localTol * bb;
+		Scratch64& sc = scratch;
 		const int* done = &ws.st.get()->done;
 		const std::vector<size_t> key{(size_t)o.precision, s.a, s.n, s.c, s.P, s.Q};
+		bool staged = false;
 		for (int ref = 0; ref <= o.maxRefine; ++ref) {
-			ws_matvec64(s, x, ws.q.get(), nullptr);
-			count_matvec(s);
-			XTT_HIP(hipMemcpyAsync(ws.r.get(), bl, N * 8, hipMemcpyDeviceToDevice, stream()));
-			axpy_f64(ws.r.get(), -1.0, ws.q.get(), N, stream());
-			const double rr = sc.dot(ws.r.get(), ws.r.get(), N);
+			matvec64(s, x, ws.q.get(), nullptr);
+			double rb[2];
+			sc.residual(ws.r.get(), bl, ws.q.get(), N, rb);
+			const double rr = rb[0], tol2 = o.localTol * o.localTol * rb[1];
 			if (rr <= tol2 || ref == o.maxRefine) break;
+			if (!staged) {
+				convert_f64_f32(s.L->data(), ws.L32.get(), s.L->size(), stream());
+				convert_f64_f32(s.R->data(), ws.R32.get(), s.R->size(), stream());
+				convert_f64_f32(s.Am.data(), ws.Am32.get(), s.Am.size(), stream());
+				staged = true;
+			}
 			if (stats) stats->refinements++;
 			const double rn = std::sqrt(rr);
 			scale_f64(ws.r.get(), 1.0 / rn, N, stream());
@@ -462,21 +466,20 @@ struct ALSData {
 			convert_f32_f64(ws.e32.get(), ws.e64.get(), N, stream());
 			axpy_f64(x, rn, ws.e64.get(), N, stream());
 		}
-		XTT_HIP(hipMemcpyAsync(xout, x, N * 8, hipMemcpyDeviceToDevice, stream()));
 	}
 
 	struct Scratch64 {
-		DBuf<double> partials{2 * 1024}, out{1};
+		DBuf<double> partials{2 * 1024}, out{2};
 		DBuf<unsigned> counter{1};
 		Scratch64() { XTT_HIP(hipMemsetAsync(counter.get(), 0, sizeof(unsigned), stream())); }
-		double dot(const double* a, const double* b, size_t n) {
-			dot_f64(a, b, n, out.get(), partials.get(), counter.get(), stream());
-			double r = 0;
-			XTT_HIP(hipMemcpyAsync(&r, out.get(), 8, hipMemcpyDeviceToHost, stream()));
+		// r = b − q; rb = {‖r‖², ‖b‖²}
+		void residual(double* r, const double* b, const double* q, size_t n, double* rb) {
+			residual_norms_f64(r, b, q, n, out.get(), partials.get(), counter.get(), stream());
+			XTT_HIP(hipMemcpyAsync(rb, out.get(), 16, hipMemcpyDeviceToHost, stream()));
 			sync();
-			return r;
 		}
 	};
+	Scratch64 scratch;  // one per solve: the reduction counter is zeroed once and returned to zero by each use
 
 	void local_solve() {
 		DTensor xm = merged(x);

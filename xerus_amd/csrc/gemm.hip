@@ -13,6 +13,7 @@
 //   result    : FP32 / BF16: D[i=4(l/16)+r][j=l%16]; FP64: D[i=4r+l/16][j=l%16], r = 0..3 (register r)
 #include <mutex>
 #include <set>
+#include <string>
 
 #include "kernels.hpp"
 #include "xtt_common.hpp"
@@ -34,7 +35,8 @@ struct Args {
 	int ta, tb;
 	double alpha, beta;
 	const int* skip;
-	int kc;  // K chunk staged in LDS
+	int kc;      // K chunk staged in LDS
+	int ksplit;  // split-K: blockIdx.z = K slice of this length, C = slice partials (m × n each); 0 = off
 };
 
 __device__ inline d4 mfma_x4(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
@@ -86,9 +88,19 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_x4_kernel(Args g) {
 	T* As = reinterpret_cast<T*>(smem_raw);
 	T* Bs = As + (size_t)g.kc * (BM + PAD);
 	const long long bz = blockIdx.z;
-	const T* A = static_cast<const T*>(g.A) + bz * g.sA;
-	const T* B = static_cast<const T*>(g.B) + bz * g.sB;
+	const T* A = static_cast<const T*>(g.A);
+	const T* B = static_cast<const T*>(g.B);
 	T* C = static_cast<T*>(g.C) + bz * g.sC;
+	int K = g.k;
+	if (g.ksplit) {  // K slice bz of op(A)·op(B)
+		const int kb = (int)bz * g.ksplit;
+		K = min(g.ksplit, g.k - kb);
+		A += g.ta ? (long long)kb * g.lda : (long long)kb;
+		B += g.tb ? (long long)kb : (long long)kb * g.ldb;
+	} else {
+		A += bz * g.sA;
+		B += bz * g.sB;
+	}
 	const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int wm = wave / WN, wn = wave % WN;
@@ -99,12 +111,12 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_x4_kernel(Args g) {
 #pragma unroll
 		for (int b = 0; b < TN; ++b) acc[a][b] = Acc{0, 0, 0, 0};
 
-	for (int k0 = 0; k0 < g.k; k0 += g.kc) {
-		const int kc = min(g.kc, g.k - k0);
+	for (int k0 = 0; k0 < K; k0 += g.kc) {
+		const int kc = min(g.kc, K - k0);
 		const int kcp = (kc + 3) & ~3;  // MFMA consumes K in steps of 4; staging zero-fills the tail
-		stage_kmajor<T, BM, PAD, NT>(As, A, g.lda, g.ta, m0, g.m, k0, kcp, g.k, tid);
+		stage_kmajor<T, BM, PAD, NT>(As, A, g.lda, g.ta, m0, g.m, k0, kcp, K, tid);
 		// B is staged as op(B)ᵀ rows: element (j, kk) = op(B)[kk][j]; tb = 0 means k-major in memory.
-		stage_kmajor<T, BN, PAD, NT>(Bs, B, g.ldb, !g.tb, n0, g.n, k0, kcp, g.k, tid);
+		stage_kmajor<T, BN, PAD, NT>(Bs, B, g.ldb, !g.tb, n0, g.n, k0, kcp, K, tid);
 		__syncthreads();
 		for (int kk = 0; kk < kcp; kk += 4) {
 			T av[TM], bv[TN];
@@ -134,6 +146,22 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_x4_kernel(Args g) {
 					*c = (g.beta != 0.0) ? v + beta * *c : v;
 				}
 			}
+}
+
+// Split-K epilogue: C = alpha·Σ_s P_s + beta·C, slices summed in index order (deterministic).
+template <class T>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const T* P, int slices, int m, int n, T* C, int ldc, double alpha,
+                                                           double beta, const int* skip) {
+	if (skip && *skip) return;
+	const size_t mn = (size_t)m * n;
+	for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < mn; e += (size_t)gridDim.x * blockDim.x) {
+		T acc = P[e];
+		for (int q = 1; q < slices; ++q) acc += P[(size_t)q * mn + e];
+		const size_t i = e / n, j = e - i * n;
+		T* c = C + i * ldc + j;
+		const T v = T(alpha) * acc;
+		*c = beta != 0.0 ? v + T(beta) * *c : v;
+	}
 }
 
 // BF16 compute: LDS holds bf16 row-major tiles As[i][k], Bs[j][k] (k contiguous, 8 elements per lane read).
@@ -248,6 +276,12 @@ void allow_lds(K kernel, size_t bytes) {
 	raised.insert(f);
 }
 
+// GEMM shape for profiler tags
+std::string shape_suffix(const GemmDesc& d) {
+	return " m=" + std::to_string(d.m) + " n=" + std::to_string(d.n) + " k=" + std::to_string(d.k) +
+	       " batch=" + std::to_string(d.batch) + " t=" + std::to_string(d.ta) + std::to_string(d.tb);
+}
+
 // Tile choice: 64×64 tiles once they still give >= 256 workgroups, else 32×32. The K chunk is the largest
 // multiple of 4 (x4 MFMA) whose two operand tiles fit the LDS budget.
 template <class T>
@@ -262,18 +296,51 @@ void launch_x4(const GemmDesc& d, const T* A, const T* B, T* C, hipStream_t s) {
 	a.kc = (int)std::max<size_t>(4, std::min<size_t>((d.k + 3) & ~size_t(3), kmax));
 	const size_t lds = (size_t)a.kc * BMp * 2 * sizeof(T);
 	const double flops = 2.0 * d.m * d.n * d.k * d.batch;
+	std::string tag = big ? (sizeof(T) == 8 ? "gemm_x4_kernel<double,64,64>" : "gemm_x4_kernel<float,64,64>")
+	                      : (sizeof(T) == 8 ? "gemm_x4_kernel<double,32,32>" : "gemm_x4_kernel<float,32,32>");
+	if (prof_shapes()) tag += shape_suffix(d);
+	// Split-K for few output tiles over a long K (the stack contractions Xᵀ·T over a·n rows): K slices of >= 64
+	// run as blockIdx.z, partials go to a stream-ordered scratch buffer and one epilogue launch sums them.
+	// Not while the stream is being captured (the scratch buffer would outlive its allocation in the graph).
+	const size_t tiles32 = ((d.m + 31) / 32) * ((d.n + 31) / 32);
+	hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+	if (!big && d.batch == 1 && tiles32 < 32 && d.k >= 256) XTT_HIP(hipStreamIsCapturing(s, &cap));
+	if (!big && d.batch == 1 && tiles32 < 32 && d.k >= 256 && cap == hipStreamCaptureStatusNone) {
+		const size_t want = std::max<size_t>(2, 128 / tiles32);
+		const size_t slices0 = std::min<size_t>(d.k / 64, want);
+		const size_t ks = (((d.k + slices0 - 1) / slices0) + 3) & ~size_t(3);
+		const size_t slices = (d.k + ks - 1) / ks;
+		DBuf<T> part(slices * d.m * d.n);
+		Args p = a;
+		p.C = part.get();
+		p.ldc = (int)d.n;
+		p.sC = (long long)(d.m * d.n);
+		p.alpha = 1.0;
+		p.beta = 0.0;
+		p.ksplit = (int)ks;
+		p.kc = (int)std::min<size_t>(ks, (size_t)a.kc);
+		const size_t plds = (size_t)p.kc * BMp * 2 * sizeof(T);
+		auto kern = gemm_x4_kernel<T, 32, 32, 2, 2>;
+		allow_lds(kern, plds);
+		dim3 grid((unsigned)((d.n + 31) / 32), (unsigned)((d.m + 31) / 32), (unsigned)slices);
+		launch_timed((tag + " splitk").c_str(), flops, kern, grid, dim3(256), plds, s, p);
+		const size_t mn = d.m * d.n;
+		const unsigned rb = (unsigned)std::min<size_t>(1024, (mn + 255) / 256);
+		hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3(rb), dim3(256), 0, s, (const T*)part.get(), (int)slices, (int)d.m,
+		                   (int)d.n, C, (int)d.ldc, d.alpha, d.beta, d.skip);
+		XTT_LAUNCH_CHECK();
+		return;
+	}
 	if (big) {
 		auto kern = gemm_x4_kernel<T, 64, 64, 2, 2>;
 		allow_lds(kern, lds);
 		dim3 grid((unsigned)((d.n + 63) / 64), (unsigned)((d.m + 63) / 64), (unsigned)d.batch);
-		launch_timed(sizeof(T) == 8 ? "gemm_x4_kernel<double,64,64>" : "gemm_x4_kernel<float,64,64>", flops, kern, grid,
-		             dim3(256), lds, s, a);
+		launch_timed(tag.c_str(), flops, kern, grid, dim3(256), lds, s, a);
 	} else {
 		auto kern = gemm_x4_kernel<T, 32, 32, 2, 2>;
 		allow_lds(kern, lds);
 		dim3 grid((unsigned)((d.n + 31) / 32), (unsigned)((d.m + 31) / 32), (unsigned)d.batch);
-		launch_timed(sizeof(T) == 8 ? "gemm_x4_kernel<double,32,32>" : "gemm_x4_kernel<float,32,32>", flops, kern, grid,
-		             dim3(256), lds, s, a);
+		launch_timed(tag.c_str(), flops, kern, grid, dim3(256), lds, s, a);
 	}
 }
 

@@ -38,6 +38,9 @@ void scale_by_f64(double* m, const double* sv, size_t rows, size_t cols, bool by
 // one zero-initialised unsigned that the kernel returns to zero.
 size_t reduce_blocks(size_t n);
 void dot_f64(const double* x, const double* y, size_t n, double* out, double* partials, unsigned* counter, hipStream_t s);
+// r = b − q; out2[0] = ‖r‖², out2[1] = ‖b‖² (same partials/counter contract as dot_f64)
+void residual_norms_f64(double* r, const double* b, const double* q, size_t n, double* out2, double* partials, unsigned* counter,
+                        hipStream_t s);
 void dot_f32_acc64(const float* x, const float* y, size_t n, double* out, double* partials, unsigned* counter, hipStream_t s);
 
 // ---- local CG on the device; all scalars live in CGState (device) -------------------------------------

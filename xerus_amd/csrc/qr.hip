@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <mutex>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -449,11 +450,12 @@ void factor_variant(const FactorArgs& a, int blocks, hipStream_t s) {
 	                                  {"hh_factor_kernel<16,1,0>", "hh_factor_kernel<16,1,1>"},
 	                                  {"hh_factor_kernel<32,1,0>", "hh_factor_kernel<32,1,1>"}};
 	const int ri = rpt == 8 ? 0 : (rpt == 16 ? 1 : 2);
-	const char* tag = (PIV ? ptags : tags)[ri][FORMQ ? 1 : 0];
+	std::string tag = (PIV ? ptags : tags)[ri][FORMQ ? 1 : 0];
+	if (prof_shapes()) tag += " rows=" + std::to_string(a.rows) + " n=" + std::to_string(a.n) + " blocks=" + std::to_string(blocks);
 	const double flops = prof_enabled() ? factor_flops(a, blocks, FORMQ) : 0.0;
-	if (rpt == 8) launch_timed(tag, flops, hh_factor_kernel<8, PIV, FORMQ>, g, t, 0, s, a);
-	else if (rpt == 16) launch_timed(tag, flops, hh_factor_kernel<16, PIV, FORMQ>, g, t, 0, s, a);
-	else launch_timed(tag, flops, hh_factor_kernel<32, PIV, FORMQ>, g, t, 0, s, a);
+	if (rpt == 8) launch_timed(tag.c_str(), flops, hh_factor_kernel<8, PIV, FORMQ>, g, t, 0, s, a);
+	else if (rpt == 16) launch_timed(tag.c_str(), flops, hh_factor_kernel<16, PIV, FORMQ>, g, t, 0, s, a);
+	else launch_timed(tag.c_str(), flops, hh_factor_kernel<32, PIV, FORMQ>, g, t, 0, s, a);
 }
 
 void launch_factor(const FactorArgs& a, int blocks, bool piv, hipStream_t s) {
@@ -481,8 +483,9 @@ void apply_rpt(const ApplyArgs& a, int blocks, hipStream_t s) {
 			const double m = std::min(a.rpb, a.rows - b * a.rpb), k = std::min<double>(m, a.n);
 			f += 4.0 * a.nc * (m * k - k * k / 2.0);
 		}
-	launch_timed(RPT == 8 ? "hh_apply_kernel<8>" : (RPT == 16 ? "hh_apply_kernel<16>" : "hh_apply_kernel<32>"), f,
-	             hh_apply_kernel<RPT>, dim3(blocks), dim3(HT), lds, s, a);
+	std::string tag = RPT == 8 ? "hh_apply_kernel<8>" : (RPT == 16 ? "hh_apply_kernel<16>" : "hh_apply_kernel<32>");
+	if (prof_shapes()) tag += " rows=" + std::to_string(a.rows) + " n=" + std::to_string(a.n) + " nc=" + std::to_string(a.nc);
+	launch_timed(tag.c_str(), f, hh_apply_kernel<RPT>, dim3(blocks), dim3(HT), lds, s, a);
 }
 
 void launch_apply(const ApplyArgs& a, int blocks, hipStream_t s) {

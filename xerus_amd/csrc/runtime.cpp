@@ -117,6 +117,10 @@ std::vector<Slot> g_slots;
 }  // namespace
 
 bool prof_enabled() { return g_prof; }
+bool prof_shapes() {
+	static const bool on = getenv("XTT_PROFILE_SHAPES") != nullptr;
+	return on;
+}
 void prof_enable(bool on) { std::lock_guard<std::mutex> lk(g_pmu); g_prof = on; }
 
 ProfEvents prof_events(const char* tag, double flops) {

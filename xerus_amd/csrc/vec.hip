@@ -123,6 +123,23 @@ __global__ __launch_bounds__(RB) void dot_kernel(const T* x, const T* y, size_t 
 	if (last_block_sum2(a, b, partials, counter, ws) && threadIdx.x == 0) *out = a;
 }
 
+// r = b − q with out[0] = ‖r‖², out[1] = ‖b‖² (one pass, deterministic)
+__global__ __launch_bounds__(RB) void residual_norms_kernel(double* r, const double* b, const double* q, size_t n, double* out,
+                                                            double* partials, unsigned* counter) {
+	__shared__ double ws[RB / 64 * 2];
+	double rr = 0, bb = 0;
+	for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+		const double ri = b[i] - q[i];
+		r[i] = ri;
+		rr = fma(ri, ri, rr);
+		bb = fma(b[i], b[i], bb);
+	}
+	if (last_block_sum2(rr, bb, partials, counter, ws) && threadIdx.x == 0) {
+		out[0] = rr;
+		out[1] = bb;
+	}
+}
+
 // ---- CG steps ------------------------------------------------------------------------------------------
 
 template <class T>
@@ -260,6 +277,11 @@ void axpy_f64(double* y, double a, const double* x, size_t n, hipStream_t s) {
 }
 void dot_f64(const double* x, const double* y, size_t n, double* out, double* partials, unsigned* counter, hipStream_t s) {
 	hipLaunchKernelGGL(dot_kernel<double>, dim3((unsigned)reduce_blocks(n)), dim3(RB), 0, s, x, y, n, out, partials, counter);
+	XTT_LAUNCH_CHECK();
+}
+void residual_norms_f64(double* r, const double* b, const double* q, size_t n, double* out2, double* partials, unsigned* counter,
+                        hipStream_t s) {
+	hipLaunchKernelGGL(residual_norms_kernel, dim3((unsigned)reduce_blocks(n)), dim3(RB), 0, s, r, b, q, n, out2, partials, counter);
 	XTT_LAUNCH_CHECK();
 }
 void dot_f32_acc64(const float* x, const float* y, size_t n, double* out, double* partials, unsigned* counter, hipStream_t s) {

@@ -46,6 +46,7 @@ void set_error(const std::string& s);
 struct ProfRecord { std::string tag; size_t launches = 0; double ms = 0, flops = 0; };
 struct ProfEvents { hipEvent_t start = nullptr, stop = nullptr; };
 bool prof_enabled();
+bool prof_shapes();                                                // XTT_PROFILE_SHAPES set: tags carry problem shapes
 void prof_enable(bool on);
 ProfEvents prof_events(const char* tag, double flops);            // registers one record; null events when disabled
 std::vector<ProfRecord> prof_collect();                            // synchronises, aggregates, clears
