@@ -105,6 +105,13 @@ int xt_tt_add(xt_tt a, xt_tt b, xt_tt* out);                  /* a + b, ttNetwor
 int xt_tt_scale(xt_tt t, double alpha);
 int xt_tt_apply(xt_tt A, xt_tt x, xt_tt* out);                /* A·x as TT (TTStack), ttStack.cpp:197-310 */
 int xt_tt_full(xt_tt t, double* host_out);                    /* dense tensor (small TTs only) */
+/* xerus datafiles: misc::save_to_file(TTNetwork, filename, format) (misc/fileIO.h:105-121, stream writers
+   ttNetwork.cpp:1454-1470, tensorNetwork.cpp:1429-1469, tensor.cpp:1781-1806); format 0 = BINARY, 1 = TSV
+   (misc::FileFormat). Files are byte compatible with xerus in both directions. */
+int xt_tt_save(xt_tt t, const char* filename, int format);
+/* misc::load_from_file<TTTensor|TTOperator> (fileIO.h:125-160, readers ttNetwork.cpp:1474-1488,
+   tensorNetwork.cpp:1471-1504, tensor.cpp:1810-1843); the header decides tensor vs operator. */
+int xt_tt_load(const char* filename, xt_tt* out);
 /* ‖A·x − b‖/‖b‖ from the TT of A·x − b (the SPD residual_f, als.cpp:483-486) */
 int xt_tt_residual(xt_tt A, xt_tt x, xt_tt b, double* out);
 

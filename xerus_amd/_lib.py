@@ -102,6 +102,8 @@ SIGNATURES = {
     "xt_tt_apply": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
     "xt_tt_full": (C.c_int, [_vp, _pd]),
     "xt_tt_residual": (C.c_int, [_vp, _vp, _vp, _pd]),
+    "xt_tt_save": (C.c_int, [_vp, C.c_char_p, C.c_int]),
+    "xt_tt_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
     "xt_als_default_options": (C.c_int, [C.POINTER(XtALSOptions), C.c_uint, C.c_int]),
     "xt_als_solve": (C.c_int, [C.POINTER(XtALSOptions), _vp, _vp, _vp, _pd, C.POINTER(XtALSStats)]),
     "xt_als_round_batch": (C.c_int, [C.POINTER(XtALSOptions), _sz, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), _sz,

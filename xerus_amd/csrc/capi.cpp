@@ -250,6 +250,12 @@ int xt_tt_full(xt_tt t, double* host) {
 		xtt::dt_download(acc, host);
 	});
 }
+int xt_tt_save(xt_tt t, const char* filename, int format) {
+	return guard([&] { NN(t); NN(filename); xtt::tt_save(t->t, filename, format); });
+}
+int xt_tt_load(const char* filename, xt_tt* out) {
+	return guard([&] { NN(filename); NN(out); *out = nullptr; *out = new xt_tt_s{xtt::tt_load(filename)}; });
+}
 int xt_tt_residual(xt_tt A, xt_tt x, xt_tt b, double* o) {
 	return guard([&] { NN(A); NN(x); NN(b); NN(o); *o = xtt::tt_residual(A->t, x->t, b->t); });
 }

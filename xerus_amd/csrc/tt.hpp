@@ -2,6 +2,7 @@
 // All FP64; cores stay in HBM, only integer rank decisions and scalar norms cross to the host.
 #pragma once
 #include <functional>
+#include <string>
 #include <vector>
 
 #include "kernels.hpp"
@@ -65,6 +66,10 @@ TT tt_add(const TT& a, const TT& b);
 void tt_scale(TT& x, double s);
 TT tt_apply(const TT& A, const TT& x);   // rank r_A·r_x, uncanonicalized
 double tt_residual(const TT& A, const TT& x, const TT& b);  // ‖Ax − b‖/‖b‖, stable (QR sweep of the TT of Ax − b)
+
+// xerus datafiles (misc::save_to_file / load_from_file of a TTNetwork): format 0 = BINARY, 1 = TSV.
+void tt_save(const TT& x, const std::string& filename, int format);
+TT tt_load(const std::string& filename);  // TTTensor or TTOperator, as the header says
 
 // Shared RNG mirroring xerus::misc::randomEngine / defaultNormalDistribution (misc/random.cpp:29-30).
 void rng_seed(uint64_t s);

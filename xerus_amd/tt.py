@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import copy as _copy
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -261,6 +262,29 @@ class TTOperator(_TTNetwork):
         h = C.c_void_p()
         call("xt_tt_apply", self._h, x._h, C.byref(h))
         return TTTensor(_handle=h)
+
+
+class FileFormat:
+    """misc::FileFormat (misc/fileIO.h:41): BINARY = exact storage, TSV = tab-separated text."""
+    BINARY = 0
+    TSV = 1
+
+
+def save_to_file(obj, filename: str, format: int = FileFormat.BINARY) -> None:
+    """misc::save_to_file for a TTTensor / TTOperator (python/misc.cpp:84-90): a xerus datafile, byte compatible
+    with xerus's own writer (misc/fileIO.h:105-121)."""
+    if not isinstance(obj, _TTNetwork):
+        raise TypeError("save_to_file: only TTTensor and TTOperator are stored by this build")
+    call("xt_tt_save", obj._h, os.fsencode(filename), int(format))
+
+
+def load_from_file(filename: str):
+    """misc::load_from_file (python/misc.cpp:92-116): the TTTensor or TTOperator a xerus datafile holds."""
+    h = C.c_void_p()
+    call("xt_tt_load", os.fsencode(filename), C.byref(h))
+    is_op = C.c_int()
+    call("xt_tt_is_operator", h, C.byref(is_op))
+    return TTOperator(_handle=h) if is_op.value else TTTensor(_handle=h)
 
 
 def frob_norm(t) -> float:
