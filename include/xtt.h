@@ -31,6 +31,7 @@ enum { XT_F64 = 0, XT_F32 = 1, XT_BF16 = 2 };
 
 typedef struct xt_tt_s* xt_tt;      /* TTTensor / TTOperator resident on the device */
 typedef struct xt_meas_s* xt_meas;  /* SinglePointMeasurementSet */
+typedef struct xt_r1meas_s* xt_r1meas;  /* RankOneMeasurementSet */
 
 /* ---------------------------------------------------------------- runtime */
 const char* xt_last_error(void);
@@ -169,6 +170,27 @@ int xt_adf_gradient(xt_tt x, xt_meas m, int precision, double* host_grad, double
 /* ADFVariant::operator() (adf.h:248-266); max_ranks NULL = x's ranks */
 int xt_adf_solve(xt_tt x, xt_meas m, const size_t* max_ranks, size_t max_iterations, double target_residual,
                  double min_decrease, int precision, double* residual, size_t* iterations);
+
+/* ------------------------------------------- RankOneMeasurementSet (measurments.h:94-150) and its ADF path.
+ * Measurement i is u_{i,0} ⊗ … ⊗ u_{i,d-1}; `vectors` holds P rows of Σ_k n_k doubles (mode k's vector at
+ * column offset Σ_{j<k} n_j). */
+int xt_r1meas_random(size_t P, size_t order, const size_t* dims, xt_r1meas* out);  /* random(P, dims), measurments.cpp:267-272,513-536 */
+int xt_r1meas_from_single_point(xt_meas m, size_t order, const size_t* dims, xt_r1meas* out);  /* measurments.cpp:253-265 */
+int xt_r1meas_new(size_t P, size_t order, const size_t* dims, const double* vectors, const double* values, xt_r1meas* out); /* add */
+int xt_r1meas_free(xt_r1meas m);
+int xt_r1meas_size(xt_r1meas m, size_t* P);
+int xt_r1meas_vectors(xt_r1meas m, double* host_out);
+int xt_r1meas_values(xt_r1meas m, double* host_out);
+int xt_r1meas_set_values(xt_r1meas m, const double* host_values);
+int xt_r1meas_sort(xt_r1meas m, int positions_only);   /* measurments.cpp:329-344 (internal::comp order) */
+int xt_r1meas_normalize(xt_r1meas m);                  /* measurments.cpp:346-355 */
+int xt_r1meas_measure(xt_r1meas m, xt_tt x);           /* measurments.cpp:392-424 */
+int xt_r1meas_test(xt_r1meas m, xt_tt x, double* out); /* measurments.cpp:469-505: ‖b − x(u)‖/‖b‖ */
+/* Δ_k = Σ (b_i − x(u_i))·∂x(u_i)/∂G_k for all k (adf.cpp:314-396 with the rank-one dyadic product :333-357) */
+int xt_adf_gradient_r1(xt_tt x, xt_r1meas m, int precision, double* host_grad, double* residual_norm);
+/* ADFVariant::operator() on a RankOneMeasurementSet (adf.h:248-266; update_x adf.cpp:481-487) */
+int xt_adf_solve_r1(xt_tt x, xt_r1meas m, const size_t* max_ranks, size_t max_iterations, double target_residual,
+                    double min_decrease, int precision, double* residual, size_t* iterations);
 
 /* Raw sampled-entry kernels on caller-owned device buffers (FP32 cores, all cores concatenated in TT order;
  * dims n[d] and ranks r[d+1] given on the host; positions P×d uint32 on the device).

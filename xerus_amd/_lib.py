@@ -108,6 +108,20 @@ SIGNATURES = {
     "xt_als_solve": (C.c_int, [C.POINTER(XtALSOptions), _vp, _vp, _vp, _pd, C.POINTER(XtALSStats)]),
     "xt_als_round_batch": (C.c_int, [C.POINTER(XtALSOptions), _sz, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), _sz,
                                      C.c_double, C.c_int, _pd, C.POINTER(XtALSStats)]),
+    "xt_r1meas_random": (C.c_int, [_sz, _sz, _ps, C.POINTER(_vp)]),
+    "xt_r1meas_from_single_point": (C.c_int, [_vp, _sz, _ps, C.POINTER(_vp)]),
+    "xt_r1meas_new": (C.c_int, [_sz, _sz, _ps, _pd, _pd, C.POINTER(_vp)]),
+    "xt_r1meas_free": (C.c_int, [_vp]),
+    "xt_r1meas_size": (C.c_int, [_vp, _ps]),
+    "xt_r1meas_vectors": (C.c_int, [_vp, _pd]),
+    "xt_r1meas_values": (C.c_int, [_vp, _pd]),
+    "xt_r1meas_set_values": (C.c_int, [_vp, _pd]),
+    "xt_r1meas_sort": (C.c_int, [_vp, C.c_int]),
+    "xt_r1meas_normalize": (C.c_int, [_vp]),
+    "xt_r1meas_measure": (C.c_int, [_vp, _vp]),
+    "xt_r1meas_test": (C.c_int, [_vp, _vp, _pd]),
+    "xt_adf_gradient_r1": (C.c_int, [_vp, _vp, C.c_int, _pd, _pd]),
+    "xt_adf_solve_r1": (C.c_int, [_vp, _vp, _ps, _sz, C.c_double, C.c_double, C.c_int, _pd, _ps]),
     "xt_meas_random": (C.c_int, [_sz, _sz, _ps, C.POINTER(_vp)]),
     "xt_meas_new": (C.c_int, [_sz, _sz, _pu32, _pd, C.POINTER(_vp)]),
     "xt_meas_free": (C.c_int, [_vp]),

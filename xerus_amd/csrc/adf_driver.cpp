@@ -118,8 +118,8 @@ struct AdfWork {
 };
 
 template <class T>
-void upload_values(const Measurements& m, DBuf<T>& v) {
-	std::vector<T> h(m.values.begin(), m.values.end());
+void upload_values(const std::vector<double>& values, DBuf<T>& v) {
+	std::vector<T> h(values.begin(), values.end());
 	v.resize(h.size());
 	XTT_HIP(hipMemcpyAsync(v.get(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream()));
 	sync();
@@ -143,20 +143,245 @@ void meas_measure(Measurements& m, const TT& x) {  // measurments.cpp:125-153 (F
 	sync();
 }
 
-template <class T>
-std::vector<std::vector<double>> adf_full_gradient(const TT& x, const Measurements& m, double* residualNorm) {
-	XTT_REQUIRE(m.d == x.order() && m.size() > 0, "ADF: measurement set does not match x");
+// ---- RankOneMeasurementSet ------------------------------------------------------------------------------
+
+size_t RankOneMeasurements::sumN() const {
+	size_t s = 0;
+	for (size_t n : dims) s += n;
+	return s;
+}
+
+void RankOneMeasurements::check_against(const TT& x) const {
+	XTT_REQUIRE(d() == x.order(), "Measurment degree must coincide with x degree.");
+	for (size_t k = 0; k < d(); ++k) XTT_REQUIRE(dims[k] == x.dims[k], "rank-one measurement dimensions differ from x's");
+	XTT_REQUIRE(u.size() == size() * sumN(), "rank-one measurement set is inconsistent");
+}
+
+template <>
+const double* RankOneMeasurements::device_u<double>() const {
+	if (dirty) {
+		du64.resize(std::max<size_t>(u.size(), 1));
+		if (!u.empty()) XTT_HIP(hipMemcpyAsync(du64.get(), u.data(), u.size() * 8, hipMemcpyHostToDevice, stream()));
+		sync();
+		dirty = false;
+		dirty32 = true;
+	}
+	return du64.get();
+}
+template <>
+const float* RankOneMeasurements::device_u<float>() const {
+	const double* d64 = device_u<double>();
+	if (dirty32) {
+		du32.resize(std::max<size_t>(u.size(), 1));
+		if (!u.empty()) convert_f64_f32(d64, du32.get(), u.size(), stream());
+		dirty32 = false;
+	}
+	return du32.get();
+}
+
+// xerus's internal::comp on dense vectors (measurments.cpp:549-558): the first differing entry decides, and the
+// comparator of sort() puts the vector with the LARGER entry first.
+static bool r1_before(const double* a, const double* b, size_t len) {
+	for (size_t k = 0; k < len; ++k) {
+		if (a[k] < b[k]) return false;
+		if (a[k] > b[k]) return true;
+	}
+	return false;
+}
+
+void r1_sort(RankOneMeasurements& m, bool positionsOnly) {
+	const size_t P = m.size(), S = m.sumN();
+	std::vector<size_t> order(P);
+	for (size_t i = 0; i < P; ++i) order[i] = i;
+	// mode-by-mode comparison equals comparing the concatenated rows (all vectors of a mode have one length)
+	std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return r1_before(&m.u[a * S], &m.u[b * S], S); });
+	std::vector<double> u(P * S), v(P);
+	for (size_t i = 0; i < P; ++i) {
+		std::copy(&m.u[order[i] * S], &m.u[order[i] * S] + S, &u[i * S]);
+		v[i] = m.values[positionsOnly ? i : order[i]];
+	}
+	m.u.swap(u);
+	m.values.swap(v);
+	m.invalidate();
+}
+
+RankOneMeasurements r1_random(size_t P, const dims_t& dims) {  // create_random_positions, measurments.cpp:513-536
+	size_t total = 1;
+	bool overflow = false;
+	for (size_t n : dims) {
+		XTT_REQUIRE(n > 0, "dimension 0 is not valid");
+		if (total > SIZE_MAX / n) overflow = true; else total *= n;
+	}
+	XTT_REQUIRE(overflow || total >= P, "It's impossible to perform as many measurements as requested.");
+	RankOneMeasurements m;
+	m.dims = dims;
+	const size_t S = m.sumN();
+	m.u.resize(P * S);
+	for (size_t i = 0; i < P * S; ++i) m.u[i] = rng_normal();  // Tensor::random({n_k}) per mode, measurement-major
+	m.values.assign(P, 0.0);
+	r1_sort(m, true);
+	return m;
+}
+
+RankOneMeasurements r1_from_single_point(const Measurements& sp, const dims_t& dims) {
+	XTT_REQUIRE(sp.d == dims.size(), "Inconsistent degrees.");
+	RankOneMeasurements m;
+	m.dims = dims;
+	const size_t S = m.sumN(), P = sp.size();
+	m.u.assign(P * S, 0.0);
+	for (size_t i = 0; i < P; ++i) {
+		size_t o = 0;
+		for (size_t k = 0; k < dims.size(); ++k) {
+			XTT_REQUIRE(sp.pos[i * sp.d + k] < dims[k], "measurement position outside the dimensions");
+			m.u[i * S + o + sp.pos[i * sp.d + k]] = 1.0;
+			o += dims[k];
+		}
+	}
+	m.values = sp.values;
+	return m;
+}
+
+void r1_normalize(RankOneMeasurements& m) {
+	const size_t S = m.sumN();
+	for (size_t i = 0; i < m.size(); ++i) {
+		size_t o = 0;
+		for (size_t k = 0; k < m.d(); ++k) {
+			double nrm = 0;
+			for (size_t j = 0; j < m.dims[k]; ++j) nrm += m.u[i * S + o + j] * m.u[i * S + o + j];
+			nrm = std::sqrt(nrm);
+			for (size_t j = 0; j < m.dims[k]; ++j) m.u[i * S + o + j] /= nrm;
+			m.values[i] /= nrm;
+			o += m.dims[k];
+		}
+	}
+	m.invalidate();
+}
+
+namespace {
+// Σ (x(u_i) − b_i)² and Σ b_i² in FP64; the values x(u_i) go to vals when given.
+void r1_eval(const RankOneMeasurements& m, const TT& x, double* norms2, std::vector<double>* vals) {
 	m.check_against(x);
 	const size_t P = m.size();
+	Flat<double> f;
+	f.build(x);
+	AdfDesc a = f.desc;
+	a.P = (int)P;
+	AdfWork<double> w;
+	w.ensure(P, f.desc.gradSize, 1);
+	DBuf<double> b(std::max<size_t>(P, 1));
+	XTT_HIP(hipMemcpyAsync(b.get(), m.values.data(), P * 8, hipMemcpyHostToDevice, stream()));
+	r1_residual<double>(a, f.cores.get(), m.device_u<double>(), vals ? nullptr : b.get(), w.resid.get(), w.norms.get(), w.dpart.get(), -1,
+	                    nullptr, stream());
+	XTT_HIP(hipMemcpyAsync(norms2, w.norms.get(), 16, hipMemcpyDeviceToHost, stream()));
+	if (vals) {
+		vals->resize(P);
+		XTT_HIP(hipMemcpyAsync(vals->data(), w.resid.get(), P * 8, hipMemcpyDeviceToHost, stream()));
+	}
+	sync();
+}
+}  // namespace
+
+void r1_measure(RankOneMeasurements& m, const TT& x) {
+	if (m.size() == 0) return;
+	double n2[2];
+	std::vector<double> v;
+	r1_eval(m, x, n2, &v);
+	m.values = v;
+}
+
+double r1_test(const RankOneMeasurements& m, const TT& x) {
+	XTT_REQUIRE(m.size() > 0, "empty measurement set");
+	double n2[2];
+	r1_eval(m, x, n2, nullptr);
+	return std::sqrt(n2[0] / n2[1]);
+}
+
+// ---- gradient and solver, shared by both measurement kinds ----------------------------------------------------
+
+namespace {
+// The per-kind device work of one ADF step. Single points: slice-wise steps (adf.cpp:468-478); rank-one: one
+// step for the whole core (adf.cpp:481-487).
+template <class T>
+struct SinglePointOps {
+	const Measurements& m;
+	void check(const TT& x) const { m.check_against(x); }
+	size_t size() const { return m.size(); }
+	const std::vector<double>& values() const { return m.values; }
+	void residual(const AdfDesc& a, const Flat<T>& f, AdfWork<T>& w) const {
+		adf_residual<T>(a, f.cores.get(), m.dpos.get(), w.vals.get(), w.resid.get(), w.norms.get(), w.dpart.get(), w.counter.get(),
+		                stream());
+	}
+	void gradient(const AdfDesc& a, const Flat<T>& f, AdfWork<T>& w, int k) const {
+		adf_gradient<T>(a, f.cores.get(), m.dpos.get(), w.resid.get(), w.grad.get(), w.partials.get(), k, stream());
+	}
+	// x_k += Δ-step, from the gradient g of core k (host) and the slice norms
+	void step(const AdfDesc& a, const Flat<T>& f, AdfWork<T>& w, size_t k, const DTensor& core, const std::vector<T>& g,
+	          std::vector<double>& upd) const {
+		const size_t rl = core.dims[0], n = core.dims[1], rr = core.dims[2];
+		adf_slice_norms<T>(a, f.cores.get(), m.dpos.get(), (int)k, w.grad.get() + f.hoff[k], (int)n, w.slices.get(), w.dpart.get(),
+		                   stream());
+		std::vector<double> normA(n);
+		XTT_HIP(hipMemcpyAsync(normA.data(), w.slices.get(), n * 8, hipMemcpyDeviceToHost, stream()));
+		sync();
+		// update_x (adf.cpp:468-478): x_k[:, j, :] += (‖Δ_j‖²/‖P_j A Δ‖²)·Δ_j with Δ = −g
+		for (size_t j = 0; j < n; ++j) {
+			double pyr = 0;
+			for (size_t al = 0; al < rl; ++al)
+				for (size_t c = 0; c < rr; ++c) { const double t = (double)g[(al * n + j) * rr + c]; pyr += t * t; }
+			const double st = pyr / normA[j];
+			for (size_t al = 0; al < rl; ++al)
+				for (size_t c = 0; c < rr; ++c) upd[(al * n + j) * rr + c] = -st * (double)g[(al * n + j) * rr + c];
+		}
+	}
+};
+
+template <class T>
+struct RankOneOps {
+	const RankOneMeasurements& m;
+	void check(const TT& x) const { m.check_against(x); }
+	size_t size() const { return m.size(); }
+	const std::vector<double>& values() const { return m.values; }
+	void residual(const AdfDesc& a, const Flat<T>& f, AdfWork<T>& w) const {
+		r1_residual<T>(a, f.cores.get(), m.device_u<T>(), w.vals.get(), w.resid.get(), w.norms.get(), w.dpart.get(), -1, nullptr,
+		               stream());
+	}
+	void gradient(const AdfDesc& a, const Flat<T>& f, AdfWork<T>& w, int k) const {
+		const int d = a.d;
+		for (int q = (k < 0 ? 0 : k); q < (k < 0 ? d : k + 1); ++q)
+			r1_gradient<T>(a, f.cores.get(), m.device_u<T>(), w.resid.get(), q, f.hrank[q], f.hn[q], f.hrank[q + 1], w.grad.get() + f.hoff[q],
+			               w.partials.get(), stream());
+	}
+	void step(const AdfDesc& a, const Flat<T>& f, AdfWork<T>& w, size_t k, const DTensor& core, const std::vector<T>& g,
+	          std::vector<double>& upd) const {
+		(void)core;
+		// Σ_i (x(u_i) with core k replaced by Δ)² (calculate_slicewise_norm_A_projGrad, adf.cpp:413-465, everything at slot 0)
+		DBuf<double> n2(2);
+		r1_residual<T>(a, f.cores.get(), m.device_u<T>(), nullptr, nullptr, n2.get(), w.dpart.get(), (int)k, w.grad.get() + f.hoff[k],
+		               stream());
+		double h[2];
+		XTT_HIP(hipMemcpyAsync(h, n2.get(), 16, hipMemcpyDeviceToHost, stream()));
+		sync();
+		double pyr = 0;
+		for (const T& t : g) pyr += (double)t * (double)t;
+		const double st = pyr / h[0];  // update_x (adf.cpp:481-487): ‖Δ‖² / Σ_j normAProjGrad[j]
+		for (size_t e = 0; e < g.size(); ++e) upd[e] = -st * (double)g[e];
+	}
+};
+
+template <class T, class Ops>
+std::vector<std::vector<double>> full_gradient(const TT& x, const Ops& ops, double* residualNorm) {
+	XTT_REQUIRE(ops.size() > 0, "ADF: empty measurement set");
+	ops.check(x);
+	const size_t P = ops.size();
 	Flat<T> f;
 	f.build(x);
 	AdfWork<T> w;
 	w.ensure(P, f.desc.gradSize, 1);
-	upload_values<T>(m, w.vals);
+	upload_values<T>(ops.values(), w.vals);
 	AdfDesc a = f.desc;
 	a.P = (int)P;
-	adf_residual<T>(a, f.cores.get(), m.dpos.get(), w.vals.get(), w.resid.get(), w.norms.get(), w.dpart.get(), w.counter.get(), stream());
-	adf_gradient<T>(a, f.cores.get(), m.dpos.get(), w.resid.get(), w.grad.get(), w.partials.get(), -1, stream());
+	ops.residual(a, f, w);
+	ops.gradient(a, f, w, -1);
 	std::vector<T> g(f.desc.gradSize);
 	double norms[2];
 	XTT_HIP(hipMemcpyAsync(g.data(), w.grad.get(), g.size() * sizeof(T), hipMemcpyDeviceToHost, stream()));
@@ -172,22 +397,19 @@ std::vector<std::vector<double>> adf_full_gradient(const TT& x, const Measuremen
 	}
 	return out;
 }
-template std::vector<std::vector<double>> adf_full_gradient<double>(const TT&, const Measurements&, double*);
-template std::vector<std::vector<double>> adf_full_gradient<float>(const TT&, const Measurements&, double*);
 
 // adf.cpp:489-604 with the stack contractions replaced by the sampled-entry kernels.
-template <class T>
-double adf_solve(const ADFOptions& o, TT& x, const Measurements& m, const dims_t& maxRanksIn, size_t* iterations) {
-	const size_t d = x.order(), P = m.size();
+template <class T, class Ops>
+double solve(const ADFOptions& o, TT& x, const Ops& ops, const dims_t& maxRanksIn, size_t* iterations) {
+	const size_t d = x.order(), P = ops.size();
 	XTT_REQUIRE(P > 0, "Need at very least one measurment.");
-	XTT_REQUIRE(m.d == d, "Measurment degree must coincide with x degree.");
-	m.check_against(x);
+	ops.check(x);
 	const dims_t maxRanks = reduce_to_maximal_ranks(maxRanksIn, x.dims, false);
 	double normMeasured = 0;
-	for (double v : m.values) normMeasured += v * v;
+	for (double v : ops.values()) normMeasured += v * v;
 	normMeasured = std::sqrt(normMeasured);
 	AdfWork<T> w;
-	upload_values<T>(m, w.vals);
+	upload_values<T>(ops.values(), w.vals);
 	Flat<T> f;
 	size_t iteration = 0;
 	double residualNorm = DBL_MAX, lastResidualNorm = DBL_MAX;
@@ -199,7 +421,7 @@ double adf_solve(const ADFOptions& o, TT& x, const Measurements& m, const dims_t
 		w.ensure(P, f.desc.gradSize, 64);
 		AdfDesc a = f.desc;
 		a.P = (int)P;
-		adf_residual<T>(a, f.cores.get(), m.dpos.get(), w.vals.get(), w.resid.get(), w.norms.get(), w.dpart.get(), w.counter.get(), stream());
+		ops.residual(a, f, w);
 		double norms[2];
 		XTT_HIP(hipMemcpyAsync(norms, w.norms.get(), 16, hipMemcpyDeviceToHost, stream()));
 		sync();
@@ -227,26 +449,14 @@ double adf_solve(const ADFOptions& o, TT& x, const Measurements& m, const dims_t
 				AdfDesc a = f.desc;
 				a.P = (int)P;
 				const DTensor& core = x.cores[k];
-				const size_t rl = core.dims[0], n = core.dims[1], rr2 = core.dims[2], sz = core.size();
+				const size_t sz = core.size();
 				// gradient of ½Σ(x−b)² w.r.t. core k; xerus's Δ (adf.cpp:359-396) is its negative
-				adf_gradient<T>(a, f.cores.get(), m.dpos.get(), w.resid.get(), w.grad.get(), w.partials.get(), (int)k, stream());
-				adf_slice_norms<T>(a, f.cores.get(), m.dpos.get(), (int)k, w.grad.get() + f.hoff[k], (int)n, w.slices.get(),
-				                   w.dpart.get(), stream());
+				ops.gradient(a, f, w, (int)k);
 				std::vector<T> g(sz);
-				std::vector<double> normA(n);
 				XTT_HIP(hipMemcpyAsync(g.data(), w.grad.get() + f.hoff[k], sz * sizeof(T), hipMemcpyDeviceToHost, stream()));
-				XTT_HIP(hipMemcpyAsync(normA.data(), w.slices.get(), n * 8, hipMemcpyDeviceToHost, stream()));
 				sync();
-				// update_x (adf.cpp:468-478): x_k[:, j, :] += (‖Δ_j‖²/‖P_j A Δ‖²)·Δ_j with Δ = −g
 				std::vector<double> upd(sz, 0.0);
-				for (size_t j = 0; j < n; ++j) {
-					double pyr = 0;
-					for (size_t al = 0; al < rl; ++al)
-						for (size_t c = 0; c < rr2; ++c) { const double t = (double)g[(al * n + j) * rr2 + c]; pyr += t * t; }
-					const double step = pyr / normA[j];
-					for (size_t al = 0; al < rl; ++al)
-						for (size_t c = 0; c < rr2; ++c) upd[(al * n + j) * rr2 + c] = -step * (double)g[(al * n + j) * rr2 + c];
-				}
+				ops.step(a, f, w, k, core, g, upd);
 				DTensor u = dt_upload(core.dims, upd.data());
 				axpy_f64(x.cores[k].data(), 1.0, u.data(), sz, stream());
 				sync();
@@ -269,7 +479,31 @@ double adf_solve(const ADFOptions& o, TT& x, const Measurements& m, const dims_t
 	if (iterations) *iterations = iteration;
 	return residualNorm;
 }
+}  // namespace
+
+template <class T>
+std::vector<std::vector<double>> adf_full_gradient(const TT& x, const Measurements& m, double* residualNorm) {
+	return full_gradient<T>(x, SinglePointOps<T>{m}, residualNorm);
+}
+template <class T>
+std::vector<std::vector<double>> adf_full_gradient(const TT& x, const RankOneMeasurements& m, double* residualNorm) {
+	return full_gradient<T>(x, RankOneOps<T>{m}, residualNorm);
+}
+template <class T>
+double adf_solve(const ADFOptions& o, TT& x, const Measurements& m, const dims_t& maxRanks, size_t* iterations) {
+	return solve<T>(o, x, SinglePointOps<T>{m}, maxRanks, iterations);
+}
+template <class T>
+double adf_solve(const ADFOptions& o, TT& x, const RankOneMeasurements& m, const dims_t& maxRanks, size_t* iterations) {
+	return solve<T>(o, x, RankOneOps<T>{m}, maxRanks, iterations);
+}
+template std::vector<std::vector<double>> adf_full_gradient<double>(const TT&, const Measurements&, double*);
+template std::vector<std::vector<double>> adf_full_gradient<float>(const TT&, const Measurements&, double*);
+template std::vector<std::vector<double>> adf_full_gradient<double>(const TT&, const RankOneMeasurements&, double*);
+template std::vector<std::vector<double>> adf_full_gradient<float>(const TT&, const RankOneMeasurements&, double*);
 template double adf_solve<double>(const ADFOptions&, TT&, const Measurements&, const dims_t&, size_t*);
 template double adf_solve<float>(const ADFOptions&, TT&, const Measurements&, const dims_t&, size_t*);
+template double adf_solve<double>(const ADFOptions&, TT&, const RankOneMeasurements&, const dims_t&, size_t*);
+template double adf_solve<float>(const ADFOptions&, TT&, const RankOneMeasurements&, const dims_t&, size_t*);
 
 }  // namespace xtt

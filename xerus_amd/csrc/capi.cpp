@@ -8,6 +8,7 @@
 
 struct xt_tt_s { xtt::TT t; };
 struct xt_meas_s { xtt::Measurements m; };
+struct xt_r1meas_s { xtt::RankOneMeasurements m; };
 
 namespace xtt { const std::string& last_error(); }
 
@@ -381,6 +382,59 @@ int xt_adf_gradient(xt_tt x, xt_meas m, int precision, double* host, double* rn)
 		auto g = precision == XT_F64 ? xtt::adf_full_gradient<double>(x->t, m->m, rn) : xtt::adf_full_gradient<float>(x->t, m->m, rn);
 		size_t o = 0;
 		for (auto& c : g) { std::memcpy(host + o, c.data(), c.size() * 8); o += c.size(); }
+	});
+}
+int xt_r1meas_random(size_t P, size_t d, const size_t* dims, xt_r1meas* out) {
+	return guard([&] { NN(dims); NN(out); *out = new xt_r1meas_s{xtt::r1_random(P, mk(dims, d))}; });
+}
+int xt_r1meas_from_single_point(xt_meas m, size_t d, const size_t* dims, xt_r1meas* out) {
+	return guard([&] { NN(m); NN(dims); NN(out); *out = new xt_r1meas_s{xtt::r1_from_single_point(m->m, mk(dims, d))}; });
+}
+int xt_r1meas_new(size_t P, size_t d, const size_t* dims, const double* vectors, const double* values, xt_r1meas* out) {
+	return guard([&] {
+		NN(dims); NN(out);
+		auto* h = new xt_r1meas_s;
+		h->m.dims = mk(dims, d);
+		const size_t S = h->m.sumN();
+		XTT_REQUIRE(P == 0 || (vectors && values), "vectors/values must not be NULL");
+		h->m.u.assign(vectors, vectors + P * S);
+		h->m.values.assign(values, values + P);
+		*out = h;
+	});
+}
+int xt_r1meas_free(xt_r1meas m) { return guard([&] { delete m; }); }
+int xt_r1meas_size(xt_r1meas m, size_t* P) { return guard([&] { NN(m); NN(P); *P = m->m.size(); }); }
+int xt_r1meas_vectors(xt_r1meas m, double* o) { return guard([&] { NN(m); NN(o); std::memcpy(o, m->m.u.data(), m->m.u.size() * 8); }); }
+int xt_r1meas_values(xt_r1meas m, double* o) { return guard([&] { NN(m); NN(o); std::memcpy(o, m->m.values.data(), m->m.values.size() * 8); }); }
+int xt_r1meas_set_values(xt_r1meas m, const double* v) {
+	return guard([&] { NN(m); NN(v); std::memcpy(m->m.values.data(), v, m->m.values.size() * 8); });
+}
+int xt_r1meas_sort(xt_r1meas m, int positions_only) { return guard([&] { NN(m); xtt::r1_sort(m->m, positions_only != 0); }); }
+int xt_r1meas_normalize(xt_r1meas m) { return guard([&] { NN(m); xtt::r1_normalize(m->m); }); }
+int xt_r1meas_measure(xt_r1meas m, xt_tt x) { return guard([&] { NN(m); NN(x); xtt::r1_measure(m->m, x->t); }); }
+int xt_r1meas_test(xt_r1meas m, xt_tt x, double* out) { return guard([&] { NN(m); NN(x); NN(out); *out = xtt::r1_test(m->m, x->t); }); }
+int xt_adf_gradient_r1(xt_tt x, xt_r1meas m, int precision, double* host, double* rn) {
+	return guard([&] {
+		NN(x); NN(m); NN(host);
+		XTT_REQUIRE(precision == XT_F64 || precision == XT_F32, "ADF gradient precision must be XT_F64 or XT_F32");
+		auto g = precision == XT_F64 ? xtt::adf_full_gradient<double>(x->t, m->m, rn) : xtt::adf_full_gradient<float>(x->t, m->m, rn);
+		size_t o = 0;
+		for (auto& c : g) { std::memcpy(host + o, c.data(), c.size() * 8); o += c.size(); }
+	});
+}
+int xt_adf_solve_r1(xt_tt x, xt_r1meas m, const size_t* maxRanks, size_t maxIt, double target, double minDec, int precision,
+                    double* res, size_t* its) {
+	return guard([&] {
+		NN(x); NN(m); NN(res);
+		XTT_REQUIRE(precision == XT_F64 || precision == XT_F32, "ADF precision must be XT_F64 or XT_F32");
+		xtt::ADFOptions o;
+		o.maxIterations = maxIt; o.targetResidualNorm = target; o.minimalResidualNormDecrease = minDec;
+		const size_t d = x->t.order();
+		xtt::dims_t mr = maxRanks ? mk(maxRanks, d - 1) : x->t.ranks();
+		size_t it = 0;
+		*res = precision == XT_F64 ? xtt::adf_solve<double>(o, x->t, m->m, mr, &it) : xtt::adf_solve<float>(o, x->t, m->m, mr, &it);
+		if (its) *its = it;
+		xtt::sync();
 	});
 }
 int xt_perf_sink(xt_perf_callback cb, void* user) {

@@ -108,6 +108,17 @@ int adf_grad_blocks(int P);
 template <class T>
 void adf_gradient(const AdfDesc& a, const T* cores, const uint32_t* pos, const T* resid, T* grad, T* partials, int kSel,
                   hipStream_t s);
+// Rank-one measurements (U: P × sumN, measurement i's vectors u_{i,0..d-1} concatenated):
+// value_i = x(u_i) with core kSel replaced by D (kSel < 0: none); resid_i = value_i − b_i (values/resid may be
+// null: b = 0 / not stored); norms2 = {Σ resid², Σ b²}. `partials` holds 2·r1_blocks(P) doubles.
+int r1_blocks(int P);
+template <class T>
+void r1_residual(const AdfDesc& a, const T* cores, const T* U, const T* values, T* resid, double* norms2, double* partials,
+                 int kSel, const T* D, hipStream_t s);
+// grad (core k's layout, rl × n × rr) = Σ_i resid_i·∂x(u_i)/∂G_k; `partials` holds r1_blocks(P)·rl·n·rr elements.
+template <class T>
+void r1_gradient(const AdfDesc& a, const T* cores, const T* U, const T* resid, int k, int rl, int n, int rr, T* grad,
+                 T* partials, hipStream_t s);
 // out[j] = Σ_{i: i_k = j} (x_i with core k replaced by D)²; `partials` holds adf_slice_blocks(P)·nslices doubles.
 int adf_slice_blocks(int P);
 template <class T>

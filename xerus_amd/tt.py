@@ -539,13 +539,143 @@ class SinglePointMeasurementSet:
         return float(np.linalg.norm(self.measuredValues))
 
 
-def adf_gradient(x: TTTensor, measurements: SinglePointMeasurementSet, precision: int = XT_F64):
+class RankOneMeasurementSet:
+    """RankOneMeasurementSet (measurments.h:94-150, python/recovery.cpp:84-124): measurement i is the rank-one
+    tensor u_{i,0} ⊗ … ⊗ u_{i,d-1} with measured value b_i. ``RankOneMeasurementSet(sp, dims)`` converts a
+    SinglePointMeasurementSet (unit vectors, measurments.cpp:253-265)."""
+
+    def __init__(self, other=None, dims=None, *, _handle=None, _dims=None):
+        self._h = None
+        if _handle is not None:
+            self._h, self._dims = _handle, list(_dims)
+            return
+        if isinstance(other, RankOneMeasurementSet):  # copy
+            self._dims = list(other._dims)
+            self._build(other.vectors(), other.measuredValues)
+            return
+        if isinstance(other, SinglePointMeasurementSet):
+            if dims is None:
+                raise ValueError("RankOneMeasurementSet(SinglePointMeasurementSet, dims): dims are required")
+            _, p = _sizes(dims)
+            h = C.c_void_p()
+            call("xt_r1meas_from_single_point", other._h, len(dims), p, C.byref(h))
+            self._h, self._dims = h, list(dims)
+            return
+        self._dims = list(dims) if dims is not None else []
+        self._build(np.zeros((0, sum(self._dims))), np.zeros(0))
+
+    def _build(self, vectors, values):
+        vec = np.ascontiguousarray(vectors, dtype=np.float64)
+        val = np.ascontiguousarray(values, dtype=np.float64)
+        _, p = _sizes(self._dims)
+        h = C.c_void_p()
+        call("xt_r1meas_new", val.size, len(self._dims), p, vec.ctypes.data_as(C.POINTER(C.c_double)),
+             val.ctypes.data_as(C.POINTER(C.c_double)), C.byref(h))
+        old, self._h = self._h, h
+        if old and _lib._lib is not None:
+            _lib._lib.xt_r1meas_free(old)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib._lib is not None:
+            _lib._lib.xt_r1meas_free(h)
+            self._h = None
+
+    @classmethod
+    def random(cls, numMeasurements: int, arg):
+        """random(P, dims): every vector drawn N(0,1) from the shared engine, then sorted (measurments.cpp:513-536);
+        random(P, solution) additionally measures the TT `solution`."""
+        dims = arg.dimensions if isinstance(arg, _TTNetwork) else list(arg)
+        _, p = _sizes(dims)
+        h = C.c_void_p()
+        call("xt_r1meas_random", numMeasurements, len(dims), p, C.byref(h))
+        m = cls(_handle=h, _dims=dims)
+        if isinstance(arg, _TTNetwork):
+            m.measure(arg)
+        return m
+
+    def size(self) -> int:
+        n = C.c_size_t()
+        call("xt_r1meas_size", self._h, C.byref(n))
+        return n.value
+
+    __len__ = size
+
+    def degree(self) -> int:
+        return len(self._dims) if self.size() else 0
+
+    def vectors(self) -> np.ndarray:
+        """All positions as a P × Σn_k array (mode k's vector at column offset Σ_{j<k} n_j)."""
+        out = np.empty((self.size(), sum(self._dims)))
+        call("xt_r1meas_vectors", self._h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def get_position(self, i: int):
+        row = self.vectors()[i]
+        offs = np.cumsum([0] + self._dims)
+        return [row[offs[k]:offs[k + 1]].copy() for k in range(len(self._dims))]
+
+    def set_position(self, i: int, pos) -> None:
+        vec = self.vectors()
+        vec[i] = np.concatenate([np.asarray(v, dtype=np.float64).ravel() for v in pos])
+        self._build(vec, self.measuredValues)
+
+    def get_measuredValue(self, i: int) -> float:
+        return float(self.measuredValues[i])
+
+    def set_measuredValue(self, i: int, val: float) -> None:
+        v = self.measuredValues
+        v[i] = val
+        self.measuredValues = v
+
+    def add(self, position, measuredValue: float) -> None:
+        """RankOneMeasurementSet::add (measurments.cpp:309-326)."""
+        pos = [np.asarray(v, dtype=np.float64).ravel() for v in position]
+        if self.size() == 0 and not self._dims:
+            self._dims = [len(v) for v in pos]
+        if [len(v) for v in pos] != self._dims:
+            raise XttError(1, "Inconsitend dimensions obtained.")
+        vec = np.vstack([self.vectors(), np.concatenate(pos)[None, :]])
+        self._build(vec, np.append(self.measuredValues, measuredValue))
+
+    @property
+    def measuredValues(self) -> np.ndarray:
+        out = np.empty(self.size())
+        call("xt_r1meas_values", self._h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    @measuredValues.setter
+    def measuredValues(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        call("xt_r1meas_set_values", self._h, v.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def sort(self, positionsOnly: bool = False) -> None:
+        call("xt_r1meas_sort", self._h, int(positionsOnly))
+
+    def normalize(self) -> None:
+        call("xt_r1meas_normalize", self._h)
+
+    def measure(self, solution: TTTensor) -> None:
+        call("xt_r1meas_measure", self._h, solution._h)
+
+    def test(self, solution: TTTensor) -> float:
+        out = C.c_double()
+        call("xt_r1meas_test", self._h, solution._h, C.byref(out))
+        return out.value
+
+    def frob_norm(self) -> float:
+        return float(np.linalg.norm(self.measuredValues))
+
+
+def adf_gradient(x: TTTensor, measurements, precision: int = XT_F64):
     """xerus's projected gradients Δ_k = Σ_i (b_i − x_i)·∂x_i/∂G_k for every core at x (adf.cpp:314-396),
-    computed by the device adjoint kernel. Returns (list of arrays shaped like the cores, ‖P_Ω(x) − b‖/‖b‖)."""
+    computed by the device adjoint kernel, for a SinglePointMeasurementSet or a RankOneMeasurementSet.
+    Returns (list of arrays shaped like the cores, ‖P_Ω(x) − b‖/‖b‖)."""
     shapes = [x.component_dims(k) for k in range(x.order())]
     out = np.empty(sum(int(np.prod(s)) for s in shapes))
     rn = C.c_double()
-    call("xt_adf_gradient", x._h, measurements._h, precision, out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(rn))
+    fn = "xt_adf_gradient_r1" if isinstance(measurements, RankOneMeasurementSet) else "xt_adf_gradient"
+    call(fn, x._h, measurements._h, precision, out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(rn))
     grads, o = [], 0
     for s in shapes:
         n = int(np.prod(s))
@@ -564,15 +694,17 @@ class ADFVariant:
         self.precision = XT_F64
         self.last_iterations = None
 
-    def __call__(self, x: TTTensor, measurements: SinglePointMeasurementSet, maxRanks=None, perfData=None) -> float:
-        """Returns ‖P_Ω(x) − b‖/‖b‖; x is updated in place (adf.h:248-266)."""
+    def __call__(self, x: TTTensor, measurements, maxRanks=None, perfData=None) -> float:
+        """Returns ‖P_Ω(x) − b‖/‖b‖; x is updated in place (adf.h:248-266). `measurements` is a
+        SinglePointMeasurementSet or a RankOneMeasurementSet."""
         if isinstance(maxRanks, PerformanceData):
             maxRanks, perfData = None, maxRanks
         p = None
         if maxRanks is not None:
             _arr, p = _sizes(maxRanks)
         res, it = C.c_double(), C.c_size_t()
-        run = lambda: call("xt_adf_solve", x._h, measurements._h, p, self.maxIterations, self.targetResidualNorm,
+        fn = "xt_adf_solve_r1" if isinstance(measurements, RankOneMeasurementSet) else "xt_adf_solve"
+        run = lambda: call(fn, x._h, measurements._h, p, self.maxIterations, self.targetResidualNorm,
                            self.minimalResidualNormDecrease, self.precision, C.byref(res), C.byref(it))
         if perfData is not None:
             perfData._run(run, x)
