@@ -109,6 +109,23 @@ def test_operator_apply_and_residual(xe, oracle):
     assert abs(r - ro) < 1e-12 * ro and abs(r - dense) < 1e-12 * dense
 
 
+@pytest.mark.parametrize("n,rx,rb", [(6, 24, 4), (8, 40, 5), (5, 25, 3)])
+def test_residual_high_rank(xe, oracle, n, rx, rb):
+    """The stable residual when the TT of Ax − b has ranks above 64 (3·rx + rb columns): the blocked TSQR
+    R-only path (64-column panels) against the oracle's Householder residual and a dense evaluation."""
+    print(f"residual high rank n={n} rx={rx} rb={rb}", flush=True)
+    d = 4
+    rng = np.random.default_rng(n)
+    pots = [rng.random(n) for _ in range(d)]
+    cores = laplace_cores(d, n, pots, rank3=True)
+    A, Ao = xe.TTOperator.from_components(cores), oracle.TT.from_cores(cores, op=True)
+    x, xo = twin_random(xe, oracle, [n] * d, [rx] * (d - 1), seed=SEED + n)
+    b, bo = twin_random(xe, oracle, [n] * d, [rb] * (d - 1), seed=SEED + 2 * n)
+    r, ro = xe.residual(A, x, b), oracle.residual(Ao, xo, bo)
+    dense = np.linalg.norm(apply_dense_laplace(xo.full(), d, n, pots) - bo.full()) / np.linalg.norm(bo.full())
+    assert abs(r - ro) < 1e-11 * ro and abs(r - dense) < 1e-11 * dense
+
+
 def config1(xe, oracle, d=8, n=4):
     """d=8, n=4 Laplacian (rank 2), b random rank 2 normalised, x0 random rank 8 — both engines, same draws."""
     cores = laplace_cores(d, n)

@@ -76,6 +76,9 @@ void extract_c(const double* A, size_t m, size_t n, const int* jpvt, const int* 
 void extract_r(const double* A, size_t m, size_t n, double* R, hipStream_t s);
 // Register-resident Householder QR for n <= 64 columns (any row count, tree-reduced above 256 rows).
 bool hh_supported(size_t m, size_t n);
+// R factor (min(m,n) × n, zero below the diagonal) of a row-major m × n matrix of any width: 64-column panels of
+// register-resident TSQR with the trailing columns updated per tree node.
+void hh_r_blocked(const double* A, size_t m, size_t n, double* R, hipStream_t s);
 // A (m×n row-major, or n×m row-major read transposed when `trans`) = Q·R: Q m×k (may be null), R k×n upper,
 // k = min(m, n).
 void hh_qr(const double* A, size_t m, size_t n, bool trans, double* Q, double* R, hipStream_t s);

@@ -496,6 +496,152 @@ void launch_apply(const ApplyArgs& a, int blocks, hipStream_t s) {
 	XTT_LAUNCH_CHECK();
 }
 
+// C_b ← Q_bᵀ·C_b = H_{k-1}·…·H_0·C_b for every block b of a factored level, in place: C holds the level's rows
+// (ld ldc), 64 columns per grid.y chunk. The trailing-column update of a panel factorisation; reflectors are
+// staged in LDS as in hh_apply_kernel and applied in forward order.
+struct ApplyTArgs {
+	const double* V;
+	const double* tau;
+	int rows, rpb, n;
+	double* C;
+	long long ldc;
+	int nc;
+};
+
+template <int RPT>
+__global__ __launch_bounds__(HT) void hh_applyT_kernel(ApplyTArgs a) {
+	extern __shared__ double sV[];   // rows × HN
+	__shared__ double sW[2][HW][HN];
+	__shared__ double sTau[HN];
+	const int lane = threadIdx.x & 63;
+	const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int b = blockIdx.x;
+	const int r0 = b * a.rpb;
+	const int rows = min(a.rpb, a.rows - r0);
+	const int n = a.n;
+	const int k = min(rows, n);
+	const int c0 = blockIdx.y * HN;
+	const int ncc = min(HN, a.nc - c0);
+	for (int e = threadIdx.x; e < RPT * HW * HN; e += HT) {
+		const int r = e / HN, c = e % HN;
+		double z = 0.0;
+		if (r < rows && c < k) z = r > c ? a.V[(size_t)(r0 + r) * n + c] : (r == c ? 1.0 : 0.0);
+		sV[e] = z;
+	}
+	if (threadIdx.x < k) sTau[threadIdx.x] = a.tau[(size_t)b * n + threadIdx.x];
+	double q[RPT], u[RPT];
+#pragma unroll
+	for (int t = 0; t < RPT; ++t) {
+		const int r = t * HW + w;
+		q[t] = (r < rows && lane < ncc) ? a.C[(size_t)(r0 + r) * a.ldc + c0 + lane] : 0.0;
+	}
+	__syncthreads();
+	int ph = 0;
+	// in chunk C every reflector i >= C·HW is zero on the rows of registers t < C
+	auto chunk = [&](auto cc) {
+		constexpr int C = decltype(cc)::value;
+		if (C * HW >= k) return;
+		const int iend = min(k, (C + 1) * HW);
+		for (int i = C * HW; i < iend; ++i) {
+			const double ti = sTau[i];
+			if (ti == 0.0) continue;
+			double c0s = 0.0, c1s = 0.0;
+#pragma unroll
+			for (int t = C; t < RPT; ++t) {
+				u[t] = sV[(t * HW + w) * HN + i];
+				if ((t - C) & 1) c1s = fma(u[t], q[t], c1s);
+				else c0s = fma(u[t], q[t], c0s);
+			}
+			sW[ph][w][lane] = c0s + c1s;
+			__syncthreads();
+			double wa = 0.0, wb = 0.0;
+#pragma unroll
+			for (int s2 = 0; s2 < HW; s2 += 2) {
+				wa += sW[ph][s2][lane];
+				wb += sW[ph][s2 + 1][lane];
+			}
+			const double wj = (wa + wb) * ti;
+#pragma unroll
+			for (int t = C; t < RPT; ++t) q[t] = fma(-u[t], wj, q[t]);
+			ph ^= 1;
+		}
+	};
+	chunk(std::integral_constant<int, 0>{});
+	chunk(std::integral_constant<int, 1>{});
+	chunk(std::integral_constant<int, 2>{});
+	chunk(std::integral_constant<int, 3>{});
+	chunk(std::integral_constant<int, 4>{});
+	chunk(std::integral_constant<int, 5>{});
+	chunk(std::integral_constant<int, 6>{});
+	chunk(std::integral_constant<int, 7>{});
+#pragma unroll
+	for (int t = 0; t < RPT; ++t) {
+		const int r = t * HW + w;
+		if (r < rows && lane < ncc) a.C[(size_t)(r0 + r) * a.ldc + c0 + lane] = q[t];
+	}
+}
+
+// After a level of a panel factorisation: level row r of block b (rb = r − b·rpb, k_b = min(rows_b, w) R rows)
+// goes to the next level as [R_b row rb | C row r] when rb < k_b, else its C part joins the next panel's rows.
+// Every block but the last is full, so the next-level row is b·w + rb and the remainder row b·(rpb − w) + rb − k_b.
+__global__ void panel_split_kernel(const double* Rb, int w, const double* C, long long ldc, int nT, int rows, int rpb,
+                                   double* next, long long ldNext, double* rem, long long remRow0) {
+	const long long cols = w + nT;
+	const long long total = (long long)rows * cols;
+	for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
+		const int r = (int)(e / cols), c = (int)(e - (long long)r * cols);
+		const int b = r / rpb, rb = r - b * rpb;
+		const int kb = min(min(rpb, rows - b * rpb), w);
+		if (rb < kb) {
+			const double v = c < w ? Rb[((size_t)b * w + rb) * w + c] : C[(size_t)r * ldc + (c - w)];
+			next[(size_t)(b * w + rb) * ldNext + c] = v;
+		} else if (c >= w) {
+			rem[(size_t)(remRow0 + (long long)b * (rpb - w) + (rb - kb)) * nT + (c - w)] = C[(size_t)r * ldc + (c - w)];
+		}
+	}
+}
+
+template <int RPT>
+void applyT_rpt(const ApplyTArgs& a, int blocks, hipStream_t s) {
+	static std::once_flag once;
+	const size_t lds = (size_t)RPT * HW * HN * sizeof(double);
+	std::call_once(once, [&] {
+		XTT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&hh_applyT_kernel<RPT>),
+		                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+	});
+	double f = 0;
+	if (prof_enabled())
+		for (int b = 0; b < blocks; ++b) {
+			const double m = std::min(a.rpb, a.rows - b * a.rpb), k = std::min<double>(m, a.n);
+			f += 4.0 * a.nc * (m * k - k * k / 2.0);
+		}
+	std::string tag = RPT == 8 ? "hh_applyT_kernel<8>" : (RPT == 16 ? "hh_applyT_kernel<16>" : "hh_applyT_kernel<32>");
+	if (prof_shapes()) tag += " rows=" + std::to_string(a.rows) + " n=" + std::to_string(a.n) + " nc=" + std::to_string(a.nc);
+	launch_timed(tag.c_str(), f, hh_applyT_kernel<RPT>, dim3(blocks, (a.nc + HN - 1) / HN), dim3(HT), lds, s, a);
+}
+
+void launch_applyT(const ApplyTArgs& a, int blocks, hipStream_t s) {
+	const int rpt = rows_per_thread(a.rpb);
+	if (rpt == 8) applyT_rpt<8>(a, blocks, s);
+	else if (rpt == 16) applyT_rpt<16>(a, blocks, s);
+	else applyT_rpt<32>(a, blocks, s);
+}
+
+// Row blocks of one tree level (rows of the level matrix, n columns): a single block up to HBR rows; leaves of
+// equal size (>= HBR/2 >= n rows each); inner levels of whole children (n rows each).
+void level_geometry(int rows, bool leaves, int n, int& blocks, int& rpb) {
+	if (rows <= HBR) {
+		blocks = 1;
+		rpb = rows;
+	} else if (leaves) {
+		blocks = (rows + HBR - 1) / HBR;
+		rpb = (rows + blocks - 1) / blocks;
+	} else {
+		rpb = (HBR / n) * n;
+		blocks = (rows + rpb - 1) / rpb;
+	}
+}
+
 struct Level {
 	int rows = 0, rpb = 0, blocks = 0;
 	DBuf<double> V, tau, R;
@@ -511,16 +657,7 @@ std::vector<Level> tree_factor(const double* A, long long ld, bool trans, int m,
 	for (;;) {
 		Level L;
 		L.rows = rows;
-		if (rows <= HBR) {
-			L.blocks = 1;
-			L.rpb = rows;
-		} else if (lv.empty()) {  // leaves: equal row blocks, each >= HBR/2 >= n rows
-			L.blocks = (rows + HBR - 1) / HBR;
-			L.rpb = (rows + L.blocks - 1) / L.blocks;
-		} else {                  // inner nodes: whole children
-			L.rpb = (HBR / n) * n;
-			L.blocks = (rows + L.rpb - 1) / L.rpb;
-		}
+		level_geometry(rows, lv.empty(), n, L.blocks, L.rpb);
 		const bool root = L.blocks == 1;
 		FactorArgs f{};
 		f.in = in; f.ld = lv.empty() ? ld : n; f.trans = lv.empty() && trans;
@@ -573,7 +710,76 @@ void check_shape(size_t m, size_t n) {
 	XTT_REQUIRE(hh_supported(m, n), "register-resident QR: more than 64 columns");
 }
 
+// R factor (min(m,n) × n, upper triangular) of a row-major m × n matrix with more than 64 columns: panels of
+// 64 columns, each factored as a TSQR tree whose nodes also apply their reflectors to the panel's trailing
+// columns (hh_applyT_kernel). A node's R rows travel up with their trailing parts; all other rows form the next
+// panel's matrix. Only R is kept, so the order of those rows is free.
+void blocked_r(const double* A, long long m, long long n, double* R, hipStream_t s) {
+	fill_f64(R, 0.0, (size_t)std::min(m, n) * n, s);
+	DBuf<double> cur((size_t)m * n);
+	XTT_HIP(hipMemcpyAsync(cur.get(), A, (size_t)m * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+	long long mp = m;
+	for (long long c0 = 0; c0 < n && mp > 0; c0 += HN) {
+		const int np = (int)(n - c0), w = std::min(np, HN), nT = np - w;
+		// remainder rows of this panel: everything but the root's R rows
+		const long long remRows = mp - std::min<long long>(mp, w);
+		DBuf<double> rem((size_t)std::max<long long>(remRows, 1) * std::max(nT, 1));
+		long long remRow = 0;
+		DBuf<double> lvl, nextLvl;
+		const double* L = cur.get();
+		int rows = (int)mp;
+		bool leaves = true;
+		for (;;) {
+			int blocks, rpb;
+			level_geometry(rows, leaves, w, blocks, rpb);
+			DBuf<double> V((size_t)rows * w), tau((size_t)blocks * w), Rb((size_t)blocks * w * w);
+			FactorArgs f{};
+			f.in = L; f.ld = np; f.trans = 0;
+			f.rows = rows; f.rpb = rpb; f.n = w;
+			f.V = V.get(); f.tau = tau.get(); f.R = Rb.get();
+			launch_factor(f, blocks, false, s);
+			if (nT > 0) {
+				ApplyTArgs t{};
+				t.V = V.get(); t.tau = tau.get(); t.rows = rows; t.rpb = rpb; t.n = w;
+				t.C = const_cast<double*>(L) + w; t.ldc = np; t.nc = nT;
+				launch_applyT(t, blocks, s);
+			}
+			const bool root = blocks == 1;
+			const int nextRows = root ? std::min(rows, w) : (blocks - 1) * w + std::min(rows - (blocks - 1) * rpb, w);
+			double* next;
+			long long ldNext;
+			if (root) {
+				next = R + (size_t)c0 * n + c0;
+				ldNext = n;
+			} else {
+				nextLvl.resize((size_t)nextRows * np);
+				next = nextLvl.get();
+				ldNext = np;
+			}
+			const long long total = (long long)rows * np;
+			const unsigned g = (unsigned)std::min<long long>(4096, (total + 255) / 256);
+			hipLaunchKernelGGL(panel_split_kernel, dim3(g), dim3(256), 0, s, (const double*)Rb.get(), w, L + w, (long long)np, nT, rows,
+			                   rpb, next, ldNext, rem.get(), remRow);
+			XTT_LAUNCH_CHECK();
+			remRow += rows - nextRows;
+			if (root) break;
+			std::swap(lvl, nextLvl);  // the level just written becomes the input
+			L = lvl.get();
+			rows = nextRows;
+			leaves = false;
+		}
+		XTT_REQUIRE(remRow == remRows, "blocked QR: row bookkeeping");
+		std::swap(cur, rem);
+		mp = remRows;
+	}
+}
+
 }  // namespace
+
+void hh_r_blocked(const double* A, size_t m, size_t n, double* R, hipStream_t s) {
+	XTT_REQUIRE(m > 0 && n > 0 && m <= (size_t)INT32_MAX / HN && n <= (size_t)INT32_MAX / HN, "blocked QR: matrix size");
+	blocked_r(A, (long long)m, (long long)n, R, s);
+}
 
 bool hh_supported(size_t m, size_t n) { return n >= 1 && n <= (size_t)HN && m >= 1 && m <= (size_t)(INT32_MAX / HN); }
 

@@ -219,13 +219,8 @@ void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTenso
 DTensor qr_r_only(const DTensor& in, size_t split) {
 	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree()), k = std::min(m, n);
 	DTensor R({k, n});
-	if (hh_supported(m, n)) {
-		hh_qr(in.data(), m, n, false, nullptr, R.data(), stream());
-		return R;
-	}
-	Factor f;
-	factor(f, in, m, n, false);
-	extract_r(f.a.get(), m, n, R.data(), stream());
+	if (hh_supported(m, n)) hh_qr(in.data(), m, n, false, nullptr, R.data(), stream());
+	else hh_r_blocked(in.data(), m, n, R.data(), stream());
 	return R;
 }
 
