@@ -107,7 +107,7 @@ def bench_adf(xe, np, steps):
     # algorithmic bytes per full gradient: int32 indices + FP32 value in + FP32 residual, cores in + grads out
     alg_bytes = P * (4 * d + 4 + 4) + 2 * int(np.prod(flat.shape)) * 4
     g = prof.get("adf_grad_fast_kernel<float>") or prof.get("adf_grad_kernel<float>")
-    rk = prof.get("adf_residual_kernel<float>")
+    rk = prof.get("adf_residual_fast_kernel<float>") or prof.get("adf_residual_kernel<float>")
     return {"grads_per_s": steps / dt, "ms_per_grad": 1e3 * dt / steps, "P": P, "alg_bytes": alg_bytes,
             "residual_norm": float(np.sqrt(sums[0] / sums[1])),
             "grad_kernel_ms": g[1] / g[0] if g else None, "residual_kernel_ms": rk[1] / rk[0] if rk else None}
@@ -158,6 +158,21 @@ def cpu_baseline(np, steps):
                       f"local CG tol 1e-12) on the host, {dt:.1f} s"}
 
 
+def measured_traffic(tag):
+    """HBM bytes per launch of a kernel (FETCH_SIZE + WRITE_SIZE per dispatch, calibrated against a copy of known
+    size) from the committed rocprofv3 counter summary of this workload, or (None, None) when it has no entry."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_als_d16_n16_r64.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except OSError:
+        return None, None
+    entry = table.get("kernels", {}).get(tag.split(" ")[0])
+    if not entry:
+        return None, None
+    return entry["fetch_bytes"] + entry["write_bytes"], os.path.relpath(path, ROOT)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +185,9 @@ def main():
     ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--batch-sweeps", type=int, default=2)
     ap.add_argument("--batch-threads", type=int, default=16)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="before the workload, copy MIB MiB three times with the library's permute kernel (identity "
+                         "shuffle: reads and writes exactly MIB MiB), the known-size reference for HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is described in DESIGN.md",
@@ -178,6 +196,18 @@ def main():
 
     import numpy as np
     import xerus_amd as xe
+
+    if args.copy_probe > 0:
+        import ctypes as C
+        from xerus_amd._lib import DeviceArray, call
+        count = args.copy_probe * (1 << 20) // 8
+        src, dst = DeviceArray.from_numpy(np.ones(count)), DeviceArray((count,), np.float64)
+        dims, ident = (C.c_size_t * 1)(count), (C.c_size_t * 1)(0)
+        for _ in range(3):
+            call("xt_reshuffle", 1, dims, ident, src.ptr, dst.ptr)
+        print(f"# copy probe: 3 x permute_kernel over {args.copy_probe} MiB (read {args.copy_probe} MiB, write "
+              f"{args.copy_probe} MiB each)", file=sys.stderr, flush=True)
+        del src, dst
 
     prec = xe.XT_F32 if args.precision == "f32" else xe.XT_BF16
     A, b, x0, _ = build_config2(xe, np)
@@ -206,13 +236,22 @@ def main():
                 "cg_iterations": st64["cgIterations"], "rel_diff_mixed_vs_fp64": diff}
         print(f"# fp64: {t64:.2f} s, residual {res64:.3e}", file=sys.stderr, flush=True)
 
-    # dominant kernel of the timed workload (largest total time among the profiled launches)
-    dom_tag, (dom_n, dom_ms, dom_flops) = max(prof.items(), key=lambda kv: kv[1][1])
+    # dominant kernel of the timed workload (largest total time among the profiled launches); its bound is the
+    # roofline side its algorithmic intensity (flops / compulsory bytes) falls on for its dtype
+    dom_tag, (dom_n, dom_ms, dom_flops, dom_bytes) = max(prof.items(), key=lambda kv: kv[1][1])
     dtype_key = "bf16" if "bf16" in dom_tag else ("f32" if "float" in dom_tag else "f64")
-    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "kernel": dom_tag, "launches": dom_n, "avg_launch_us": 1e3 * dom_ms / dom_n,
-                "flops_per_launch": dom_flops / dom_n, "achieved": achieved, "peak": PEAK_TFLOPS[dtype_key],
-                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[dtype_key], "traffic": None,
+    secs = dom_ms * 1e-3
+    ridge = PEAK_TFLOPS[dtype_key] * 1e12 / (PEAK_HBM_GBS * 1e9)
+    intensity = dom_flops / dom_bytes if dom_bytes else float("inf")
+    if intensity < ridge:
+        bound, achieved, peak, unit = "hbm", dom_bytes / secs / 1e9, PEAK_HBM_GBS, "GB/s"
+    else:
+        bound, achieved, peak, unit = "mfma", dom_flops / secs / 1e12, PEAK_TFLOPS[dtype_key], "TFLOP/s"
+    traffic, traffic_src = measured_traffic(dom_tag)
+    roofline = {"bound": bound, "kernel": dom_tag, "launches": dom_n, "avg_launch_us": 1e3 * dom_ms / dom_n,
+                "flops_per_launch": dom_flops / dom_n, "bytes_per_launch": dom_bytes / dom_n,
+                "intensity_flop_per_byte": intensity, "achieved": achieved, "peak": peak, "unit": unit,
+                "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_src,
                 "kernels_ms": {k: round(v[1], 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])[:8]}}
 
     adf = bench_adf(xe, np, args.adf_steps) if args.adf_steps > 0 else None

@@ -44,10 +44,12 @@ int xt_memcpy_htod(void* device_dst, const void* host_src, size_t bytes);
 int xt_memcpy_dtoh(void* host_dst, const void* device_src, size_t bytes);
 int xt_release_cache(void);
 /* Launch profiler (no counterpart in xerus; replaces its XERUS_PA_START/END timers, misc/performanceAnalysis.h:33-56):
- * while enabled, GEMM and ADF launches record HIP events on their own stream. xt_profile_read synchronises
- * the device, returns per-kernel (tag, launches, total ms, algorithmic flops) and clears the records. */
+ * while enabled, GEMM, Householder, Jacobi and ADF launches get a HIP event pair handed to the dispatch on their
+ * own stream. xt_profile_read synchronises the device, returns per-kernel (tag, launches, total ms, algorithmic
+ * flops, algorithmic bytes) and clears the records. */
 int xt_profile_enable(int on);
-int xt_profile_read(size_t max_records, char* tags, size_t tag_len, size_t* launches, double* ms, double* flops, size_t* count);
+int xt_profile_read(size_t max_records, char* tags, size_t tag_len, size_t* launches, double* ms, double* flops,
+                    double* bytes, size_t* count);
 /* misc::randomEngine.seed(s) (src/xerus/misc/random.cpp:29-30); also resets the shared normal distribution */
 int xt_seed(uint64_t seed);
 int xt_normal_draws(size_t n, double* out);                 /* defaultNormalDistribution(randomEngine) */

@@ -65,7 +65,7 @@ SIGNATURES = {
     "xt_memcpy_htod": (C.c_int, [_vp, _vp, _sz]),
     "xt_memcpy_dtoh": (C.c_int, [_vp, _vp, _sz]),
     "xt_profile_enable": (C.c_int, [C.c_int]),
-    "xt_profile_read": (C.c_int, [_sz, C.c_char_p, _sz, _ps, _pd, _pd, _ps]),
+    "xt_profile_read": (C.c_int, [_sz, C.c_char_p, _sz, _ps, _pd, _pd, _pd, _ps]),
     "xt_seed": (C.c_int, [C.c_uint64]),
     "xt_normal_draws": (C.c_int, [_sz, _pd]),
     "xt_uniform_draws": (C.c_int, [_sz, _sz, _ps]),
@@ -198,18 +198,19 @@ def profile_enable(on: bool) -> None:
 
 
 def profile_read(max_records: int = 4096):
-    """Per-kernel {tag: (launches, total_ms, algorithmic_flops)} since the last read (synchronises)."""
+    """Per-kernel {tag: (launches, total_ms, algorithmic_flops, algorithmic_bytes)} since the last read (synchronises)."""
     tag_len = 96
     tags = C.create_string_buffer(max_records * tag_len)
     launches = (C.c_size_t * max_records)()
     ms = (C.c_double * max_records)()
     flops = (C.c_double * max_records)()
+    nbytes = (C.c_double * max_records)()
     count = C.c_size_t()
-    check(lib().xt_profile_read(max_records, tags, tag_len, launches, ms, flops, C.byref(count)))
+    check(lib().xt_profile_read(max_records, tags, tag_len, launches, ms, flops, nbytes, C.byref(count)))
     out = {}
     for i in range(min(count.value, max_records)):
         tag = tags.raw[i * tag_len:(i + 1) * tag_len].split(b"\0", 1)[0].decode()
-        out[tag] = (int(launches[i]), float(ms[i]), float(flops[i]))
+        out[tag] = (int(launches[i]), float(ms[i]), float(flops[i]), float(nbytes[i]))
     return out
 
 

@@ -598,7 +598,8 @@ void adf_residual(const AdfDesc& a, const T* cores, const uint32_t* pos, const T
 	const int blocks = std::max(1, std::min(1024, (a.P + AB - 1) / AB));
 	if (std::is_same<T, float>::value && fast_ok(a)) {
 		Padded pc = pad_cores(a, reinterpret_cast<const float*>(cores), s);
-		launch_timed("adf_residual_fast_kernel<float>", 0.0, adf_residual_fast_kernel<T>, dim3(blocks), dim3(AB),
+		// positions, values and residuals of every sample, and the cores once
+		launch_timed("adf_residual_fast_kernel<float>", Work(0.0, (double)a.P * (4.0 * a.d + 8.0) + 4.0 * a.gradSize), adf_residual_fast_kernel<T>, dim3(blocks), dim3(AB),
 		             (size_t)a.sumN * PB * sizeof(float), s, a, (const float*)pc.Gp.get(), (const long long*)pc.off.get(), pos, values,
 		             resid, norms, partials, counter);
 		return;
@@ -631,7 +632,8 @@ void adf_gradient(const AdfDesc& a, const T* cores, const uint32_t* pos, const T
 			XTT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(adf_grad_fast_kernel), hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
 		});
 		const int blocks = std::min(adf_grad_blocks(a.P), 256);
-		launch_timed("adf_grad_fast_kernel<float>", 0.0, adf_grad_fast_kernel, dim3(blocks), dim3(FW * 64), fastLds, s, a,
+		// positions and residuals of every sample, the cores in and the gradient out
+		launch_timed("adf_grad_fast_kernel<float>", Work(0.0, (double)a.P * (4.0 * a.d + 4.0) + 8.0 * a.gradSize), adf_grad_fast_kernel, dim3(blocks), dim3(FW * 64), fastLds, s, a,
 		             (const float*)pc.Gp.get(), (const long long*)pc.off.get(), pos, reinterpret_cast<const float*>(resid),
 		             reinterpret_cast<float*>(partials), kSel);
 		hipLaunchKernelGGL(adf_grad_reduce_kernel<T>, dim3((a.gradSize + 63) / 64), dim3(256), 0, s, partials, blocks, a.gradSize, grad);

@@ -42,7 +42,8 @@ const char* xt_version(void) { return "xtt 0.1 (gfx950)"; }
 int xt_synchronize(void) { return guard([] { xtt::sync(); }); }
 int xt_release_cache(void) { return guard([] { xtt::sync(); xtt::release_cache(); }); }
 int xt_profile_enable(int on) { return guard([&] { xtt::prof_enable(on != 0); }); }
-int xt_profile_read(size_t max_records, char* tags, size_t tag_len, size_t* launches, double* ms, double* flops, size_t* count) {
+int xt_profile_read(size_t max_records, char* tags, size_t tag_len, size_t* launches, double* ms, double* flops, double* bytes,
+                    size_t* count) {
 	return guard([&] {
 		NN(count);
 		auto recs = xtt::prof_collect();
@@ -52,6 +53,7 @@ int xt_profile_read(size_t max_records, char* tags, size_t tag_len, size_t* laun
 			if (launches) launches[i] = recs[i].launches;
 			if (ms) ms[i] = recs[i].ms;
 			if (flops) flops[i] = recs[i].flops;
+			if (bytes) bytes[i] = recs[i].bytes;
 		}
 	});
 }

@@ -296,6 +296,7 @@ void launch_x4(const GemmDesc& d, const T* A, const T* B, T* C, hipStream_t s) {
 	a.kc = (int)std::max<size_t>(4, std::min<size_t>((d.k + 3) & ~size_t(3), kmax));
 	const size_t lds = (size_t)a.kc * BMp * 2 * sizeof(T);
 	const double flops = 2.0 * d.m * d.n * d.k * d.batch;
+	const Work work(flops, (double)sizeof(T) * d.batch * ((double)d.m * d.k + (double)d.k * d.n + (d.beta != 0.0 ? 2.0 : 1.0) * d.m * d.n));
 	std::string tag = big ? (sizeof(T) == 8 ? "gemm_x4_kernel<double,64,64>" : "gemm_x4_kernel<float,64,64>")
 	                      : (sizeof(T) == 8 ? "gemm_x4_kernel<double,32,32>" : "gemm_x4_kernel<float,32,32>");
 	if (prof_shapes()) tag += shape_suffix(d);
@@ -323,7 +324,7 @@ void launch_x4(const GemmDesc& d, const T* A, const T* B, T* C, hipStream_t s) {
 		auto kern = gemm_x4_kernel<T, 32, 32, 2, 2>;
 		allow_lds(kern, plds);
 		dim3 grid((unsigned)((d.n + 31) / 32), (unsigned)((d.m + 31) / 32), (unsigned)slices);
-		launch_timed((tag + " splitk").c_str(), flops, kern, grid, dim3(256), plds, s, p);
+		launch_timed((tag + " splitk").c_str(), work, kern, grid, dim3(256), plds, s, p);
 		const size_t mn = d.m * d.n;
 		const unsigned rb = (unsigned)std::min<size_t>(1024, (mn + 255) / 256);
 		hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3(rb), dim3(256), 0, s, (const T*)part.get(), (int)slices, (int)d.m,
@@ -335,12 +336,12 @@ void launch_x4(const GemmDesc& d, const T* A, const T* B, T* C, hipStream_t s) {
 		auto kern = gemm_x4_kernel<T, 64, 64, 2, 2>;
 		allow_lds(kern, lds);
 		dim3 grid((unsigned)((d.n + 63) / 64), (unsigned)((d.m + 63) / 64), (unsigned)d.batch);
-		launch_timed(tag.c_str(), flops, kern, grid, dim3(256), lds, s, a);
+		launch_timed(tag.c_str(), work, kern, grid, dim3(256), lds, s, a);
 	} else {
 		auto kern = gemm_x4_kernel<T, 32, 32, 2, 2>;
 		allow_lds(kern, lds);
 		dim3 grid((unsigned)((d.n + 31) / 32), (unsigned)((d.m + 31) / 32), (unsigned)d.batch);
-		launch_timed(tag.c_str(), flops, kern, grid, dim3(256), lds, s, a);
+		launch_timed(tag.c_str(), work, kern, grid, dim3(256), lds, s, a);
 	}
 }
 
@@ -367,17 +368,18 @@ void gemm_bf16(const GemmDesc& d, const float* A, const float* B, float* C, hipS
 	kmax &= ~size_t(31);
 	a.kc = (int)std::max<size_t>(32, std::min<size_t>((d.k + 31) & ~size_t(31), kmax));
 	const size_t lds = rows * (size_t)(a.kc + 8) * sizeof(__bf16);
-	const double flops = 2.0 * d.m * d.n * d.k * d.batch;
+	const Work work(2.0 * d.m * d.n * d.k * d.batch,
+	                4.0 * d.batch * ((double)d.m * d.k + (double)d.k * d.n + (d.beta != 0.0 ? 2.0 : 1.0) * d.m * d.n));
 	if (big) {
 		auto kern = gemm_bf16_kernel<64, 64, 2, 2>;
 		allow_lds(kern, lds);
 		dim3 grid((unsigned)((d.n + 63) / 64), (unsigned)((d.m + 63) / 64), (unsigned)d.batch);
-		launch_timed("gemm_bf16_kernel<64,64>", flops, kern, grid, dim3(256), lds, s, a);
+		launch_timed("gemm_bf16_kernel<64,64>", work, kern, grid, dim3(256), lds, s, a);
 	} else {
 		auto kern = gemm_bf16_kernel<32, 32, 2, 2>;
 		allow_lds(kern, lds);
 		dim3 grid((unsigned)((d.n + 31) / 32), (unsigned)((d.m + 31) / 32), (unsigned)d.batch);
-		launch_timed("gemm_bf16_kernel<32,32>", flops, kern, grid, dim3(256), lds, s, a);
+		launch_timed("gemm_bf16_kernel<32,32>", work, kern, grid, dim3(256), lds, s, a);
 	}
 }
 

@@ -439,7 +439,7 @@ void jacobi_svd(const double* A, size_t m, size_t n, double* U, double* S, doubl
 			XTT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(jacobi_lds_kernel), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 		});
 		// one sweep's rotations, 6m + 6n flops per pair and n(n-1)/2 pairs (the sweep count is data dependent)
-		launch_timed("jacobi_lds_kernel", 3.0 * (m + np) * np * (np - 1), jacobi_lds_kernel, dim3(1), dim3(JT),
+		launch_timed("jacobi_lds_kernel", Work(3.0 * (m + np) * np * (np - 1), 8.0 * (2.0 * m * n + n + (double)n * n)), jacobi_lds_kernel, dim3(1), dim3(JT),
 		             (m * np + np * np) * sizeof(double), s, A, (int)m, (int)n, (int)np, U, S, Vt);
 		return;
 	}

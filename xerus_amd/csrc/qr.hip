@@ -452,10 +452,17 @@ void factor_variant(const FactorArgs& a, int blocks, hipStream_t s) {
 	const int ri = rpt == 8 ? 0 : (rpt == 16 ? 1 : 2);
 	std::string tag = (PIV ? ptags : tags)[ri][FORMQ ? 1 : 0];
 	if (prof_shapes()) tag += " rows=" + std::to_string(a.rows) + " n=" + std::to_string(a.n) + " blocks=" + std::to_string(blocks);
-	const double flops = prof_enabled() ? factor_flops(a, blocks, FORMQ) : 0.0;
-	if (rpt == 8) launch_timed(tag.c_str(), flops, hh_factor_kernel<8, PIV, FORMQ>, g, t, 0, s, a);
-	else if (rpt == 16) launch_timed(tag.c_str(), flops, hh_factor_kernel<16, PIV, FORMQ>, g, t, 0, s, a);
-	else launch_timed(tag.c_str(), flops, hh_factor_kernel<32, PIV, FORMQ>, g, t, 0, s, a);
+	Work work;
+	if (prof_enabled()) {
+		// level matrix in; reflectors, τ, the blocks' R rows and (FORMQ) Q out
+		const double k = std::min(a.rpb, a.n);
+		work.flops = factor_flops(a, blocks, FORMQ);
+		work.bytes = 8.0 * ((double)a.rows * a.n + (a.V ? (double)a.rows * a.n + (double)blocks * a.n : 0.0) +
+		                    (a.R ? (double)blocks * k * a.n : 0.0) + (FORMQ ? (double)a.rows * a.nc : 0.0));
+	}
+	if (rpt == 8) launch_timed(tag.c_str(), work, hh_factor_kernel<8, PIV, FORMQ>, g, t, 0, s, a);
+	else if (rpt == 16) launch_timed(tag.c_str(), work, hh_factor_kernel<16, PIV, FORMQ>, g, t, 0, s, a);
+	else launch_timed(tag.c_str(), work, hh_factor_kernel<32, PIV, FORMQ>, g, t, 0, s, a);
 }
 
 void launch_factor(const FactorArgs& a, int blocks, bool piv, hipStream_t s) {
@@ -485,7 +492,9 @@ void apply_rpt(const ApplyArgs& a, int blocks, hipStream_t s) {
 		}
 	std::string tag = RPT == 8 ? "hh_apply_kernel<8>" : (RPT == 16 ? "hh_apply_kernel<16>" : "hh_apply_kernel<32>");
 	if (prof_shapes()) tag += " rows=" + std::to_string(a.rows) + " n=" + std::to_string(a.n) + " nc=" + std::to_string(a.nc);
-	launch_timed(tag.c_str(), f, hh_apply_kernel<RPT>, dim3(blocks), dim3(HT), lds, s, a);
+	// reflectors, τ and the blocks' start rows in; the level's Q rows out
+	const Work work(f, 8.0 * ((double)a.rows * a.n + (double)blocks * a.n + (double)blocks * a.n * a.nc + (double)a.rows * a.nc));
+	launch_timed(tag.c_str(), work, hh_apply_kernel<RPT>, dim3(blocks), dim3(HT), lds, s, a);
 }
 
 void launch_apply(const ApplyArgs& a, int blocks, hipStream_t s) {
@@ -617,7 +626,8 @@ void applyT_rpt(const ApplyTArgs& a, int blocks, hipStream_t s) {
 		}
 	std::string tag = RPT == 8 ? "hh_applyT_kernel<8>" : (RPT == 16 ? "hh_applyT_kernel<16>" : "hh_applyT_kernel<32>");
 	if (prof_shapes()) tag += " rows=" + std::to_string(a.rows) + " n=" + std::to_string(a.n) + " nc=" + std::to_string(a.nc);
-	launch_timed(tag.c_str(), f, hh_applyT_kernel<RPT>, dim3(blocks, (a.nc + HN - 1) / HN), dim3(HT), lds, s, a);
+	const Work work(f, 8.0 * ((double)a.rows * a.n + (double)blocks * a.n + 2.0 * a.rows * a.nc));
+	launch_timed(tag.c_str(), work, hh_applyT_kernel<RPT>, dim3(blocks, (a.nc + HN - 1) / HN), dim3(HT), lds, s, a);
 }
 
 void launch_applyT(const ApplyTArgs& a, int blocks, hipStream_t s) {

@@ -110,7 +110,7 @@ void release_cache() {
 }
 
 namespace {
-struct Slot { hipEvent_t a, b; std::string tag; double flops; };
+struct Slot { hipEvent_t a, b; std::string tag; double flops, bytes; };
 std::mutex g_pmu;
 bool g_prof = false;
 std::vector<Slot> g_slots;
@@ -123,14 +123,15 @@ bool prof_shapes() {
 }
 void prof_enable(bool on) { std::lock_guard<std::mutex> lk(g_pmu); g_prof = on; }
 
-ProfEvents prof_events(const char* tag, double flops) {
+ProfEvents prof_events(const char* tag, const Work& w) {
 	ProfEvents e;
 	if (!g_prof) return e;
 	Slot sl;
 	XTT_HIP(hipEventCreate(&sl.a));
 	XTT_HIP(hipEventCreate(&sl.b));
 	sl.tag = tag;
-	sl.flops = flops;
+	sl.flops = w.flops;
+	sl.bytes = w.bytes;
 	e.start = sl.a;
 	e.stop = sl.b;
 	std::lock_guard<std::mutex> lk(g_pmu);
@@ -150,6 +151,7 @@ std::vector<ProfRecord> prof_collect() {
 		pr.launches++;
 		pr.ms += ms;
 		pr.flops += sl.flops;
+		pr.bytes += sl.bytes;
 		(void)hipEventDestroy(sl.a);
 		(void)hipEventDestroy(sl.b);
 	}

@@ -43,18 +43,23 @@ void set_error(const std::string& s);
 // dispatch itself (hipExtLaunchKernel), so the pair brackets the kernel's execution and not the host's
 // submission gap. Each record carries the kernel instantiation's tag and its algorithmic flop count; read
 // back after a synchronisation.
-struct ProfRecord { std::string tag; size_t launches = 0; double ms = 0, flops = 0; };
+struct ProfRecord { std::string tag; size_t launches = 0; double ms = 0, flops = 0, bytes = 0; };
 struct ProfEvents { hipEvent_t start = nullptr, stop = nullptr; };
+// Algorithmic work of one launch: flops and the HBM bytes its inputs and outputs occupy (compulsory traffic).
+struct Work {
+	double flops = 0, bytes = 0;
+	Work(double f = 0, double b = 0) : flops(f), bytes(b) {}
+};
 bool prof_enabled();
 bool prof_shapes();                                                // XTT_PROFILE_SHAPES set: tags carry problem shapes
 void prof_enable(bool on);
-ProfEvents prof_events(const char* tag, double flops);            // registers one record; null events when disabled
+ProfEvents prof_events(const char* tag, const Work& w);           // registers one record; null events when disabled
 std::vector<ProfRecord> prof_collect();                            // synchronises, aggregates, clears
 
 template <class F, class... Args>
-void launch_timed(const char* tag, double flops, F kern, dim3 grid, dim3 block, size_t lds, hipStream_t s, Args... args) {
+void launch_timed(const char* tag, const Work& work, F kern, dim3 grid, dim3 block, size_t lds, hipStream_t s, Args... args) {
 	if (prof_enabled()) {
-		const ProfEvents e = prof_events(tag, flops);
+		const ProfEvents e = prof_events(tag, work);
 		hipExtLaunchKernelGGL(kern, grid, block, (std::uint32_t)lds, s, e.start, e.stop, 0u, args...);
 	} else {
 		hipLaunchKernelGGL(kern, grid, block, lds, s, args...);
