@@ -359,8 +359,7 @@ struct ALSData {
 
 	CGState read_state() {
 		CGState h{};
-		XTT_HIP(hipMemcpyAsync(&h, ws.st.get(), sizeof(CGState), hipMemcpyDeviceToHost, stream()));
-		sync();
+		read_back(&h, ws.st.get(), sizeof(CGState));
 		return h;
 	}
 
@@ -475,8 +474,7 @@ struct ALSData {
 		// r = b − q; rb = {‖r‖², ‖b‖²}
 		void residual(double* r, const double* b, const double* q, size_t n, double* rb) {
 			residual_norms_f64(r, b, q, n, out.get(), partials.get(), counter.get(), stream());
-			XTT_HIP(hipMemcpyAsync(rb, out.get(), 16, hipMemcpyDeviceToHost, stream()));
-			sync();
+			read_back(rb, out.get(), 16);
 		}
 	};
 	Scratch64 scratch;  // one per solve: the reduction counter is zeroed once and returned to zero by each use

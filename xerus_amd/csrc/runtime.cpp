@@ -1,5 +1,6 @@
 // libxtt runtime: thread-local error text, the per-device library stream and the caching device allocator.
 #include <chrono>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -53,6 +54,39 @@ hipStream_t stream() {
 }
 
 void sync() { XTT_HIP(hipStreamSynchronize(stream())); }
+
+namespace {
+// Pinned staging buffers for read_back, shared by all threads; kept until the process ends.
+constexpr size_t kPinnedBytes = 64 * 1024;
+std::mutex g_pinMu;
+std::vector<void*> g_pinFree;
+}  // namespace
+
+void read_back(void* host, const void* dev, size_t bytes) {
+	if (bytes == 0) return;
+	if (bytes > kPinnedBytes) {
+		XTT_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream()));
+		sync();
+		return;
+	}
+	void* buf = nullptr;
+	{
+		std::lock_guard<std::mutex> lk(g_pinMu);
+		if (!g_pinFree.empty()) {
+			buf = g_pinFree.back();
+			g_pinFree.pop_back();
+		}
+	}
+	if (!buf) XTT_HIP(hipHostMalloc(&buf, kPinnedBytes, hipHostMallocDefault));
+	hipError_t e = hipMemcpyAsync(buf, dev, bytes, hipMemcpyDeviceToHost, stream());
+	if (e == hipSuccess) e = hipStreamSynchronize(stream());
+	if (e == hipSuccess) std::memcpy(host, buf, bytes);
+	{
+		std::lock_guard<std::mutex> lk(g_pinMu);
+		g_pinFree.push_back(buf);
+	}
+	XTT_HIP(e);
+}
 
 void* dmalloc(size_t bytes) {
 	if (bytes == 0) bytes = 1;

@@ -109,8 +109,7 @@ int factor(Factor& f, const DTensor& in, size_t m, size_t n, bool pivot) {
 	geqp2(f.a.get(), m, n, f.tau.get(), f.jpvt.get(), pivot ? 1 : 0, nullptr, f.rank.get(), stream());
 	if (!pivot) return (int)std::min(m, n);
 	int r = 0;
-	XTT_HIP(hipMemcpyAsync(&r, f.rank.get(), sizeof(int), hipMemcpyDeviceToHost, stream()));
-	sync();
+	read_back(&r, f.rank.get(), sizeof(int));
 	return r;
 }
 }  // namespace
@@ -155,8 +154,7 @@ void calculate_qc(DTensor& Q, DTensor& C, const DTensor& in, size_t split) {
 		DBuf<int> jp(n), rk(1);
 		hh_qc(in.data(), m, n, q.get(), c.get(), jp.get(), rk.get(), stream());
 		int r = 0;
-		XTT_HIP(hipMemcpyAsync(&r, rk.get(), sizeof(int), hipMemcpyDeviceToHost, stream()));
-		sync();
+		read_back(&r, rk.get(), sizeof(int));
 		Q = DTensor(cat(sl(in.dims, 0, split), {(size_t)r}), std::move(q));
 		C = DTensor(cat({(size_t)r}, sl(in.dims, split, in.degree())), std::move(c));
 		return;
@@ -203,8 +201,7 @@ void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTenso
 		{ const size_t d2[2] = {n, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, u2.get(), vt.get(), stream()); }
 	}
 	std::vector<double> sh(k);
-	XTT_HIP(hipMemcpyAsync(sh.data(), s.get(), k * 8, hipMemcpyDeviceToHost, stream()));
-	sync();
+	read_back(sh.data(), s.get(), k * 8);
 	size_t rank = k;
 	if (maxRank != 0) rank = std::min(rank, maxRank);
 	for (size_t j = 1; j < rank; ++j) if (sh[j] <= eps * sh[0]) { rank = j; break; }
@@ -229,8 +226,7 @@ double dt_dot(const DTensor& a, const DTensor& b) {
 	Scratch& s = scratch();
 	dot_f64(a.data(), b.data(), a.size(), s.scalar.get(), s.partials.get(), s.counter.get(), stream());
 	double r = 0;
-	XTT_HIP(hipMemcpyAsync(&r, s.scalar.get(), 8, hipMemcpyDeviceToHost, stream()));
-	sync();
+	read_back(&r, s.scalar.get(), 8);
 	return r;
 }
 double dt_norm(const DTensor& t) { return std::sqrt(std::max(0.0, dt_dot(t, t))); }
@@ -426,8 +422,7 @@ double tt_dot(const TT& a, const TT& b) {
 		E = std::move(En);
 	}
 	double r = 0;
-	XTT_HIP(hipMemcpyAsync(&r, E.data(), 8, hipMemcpyDeviceToHost, stream()));
-	sync();
+	read_back(&r, E.data(), 8);
 	return r;
 }
 
