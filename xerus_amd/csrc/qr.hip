@@ -1,9 +1,9 @@
 // Householder QR for the tall-skinny cores of move_core / round (n <= 64 columns), register resident.
 //
 // One workgroup (8 waves) owns a block of up to 256 rows: lane j of every wave holds column j, wave w holds
-// rows w, w+8, w+16, … of the block in registers. In a Householder step the pivot column is broadcast to the
-// wave through a small LDS slot (lane i writes it, all lanes read it); the per-wave partial sums of vᵀA
-// (whose pivot lane is the column's ‖x‖²) cross waves through LDS: one barrier per column. The reflector is
+// rows w, w+8, w+16, … of the block in registers. In a Householder step every wave reads the pivot column's
+// values on its rows from lane i into scalar registers (v_readlane, no LDS traffic); the per-wave partial sums
+// of vᵀA (whose pivot lane is the column's ‖x‖²) cross waves through LDS: one barrier per column. The reflector is
 // applied unscaled and scaled once at the end (u = scal·a below the diagonal).
 //
 // Matrices taller than one block are factored as a reduction tree (TSQR): the leaves factor row blocks, each
@@ -84,7 +84,6 @@ struct FactorArgs {
 
 template <int RPT, bool PIV, bool FORMQ>
 __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
-	__shared__ double sCol[HW][RPT];   // column i of the wave's own rows below the diagonal, zeros above
 	__shared__ double sW[2][HW][HN];   // per-wave Σ_{own r > i} a_ri·a_rj (lane i: the norm² partial)
 	__shared__ double sRow[2][HN];     // row i
 	__shared__ double sR[HN][HN];      // finished rows of R (lanes >= row)
@@ -156,17 +155,17 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 					}
 				}
 			}
-			// phase A: d_j = Σ_{own r > i} a_ri·a_rj (unscaled reflector; lane i gets the norm² partial), row i
-			if (lane == i) {
-				sCol[w][C] = w > wi ? x[C] : 0.0;
+			// phase A: d_j = Σ_{own r > i} a_ri·a_rj (unscaled reflector; lane i gets the norm² partial), row i.
+			// Column i of the wave's rows is read from lane i into scalar registers (v_readlane).
+			{
+				const double vc = lane_value(x[C], i);
+				v[C] = w > wi ? vc : 0.0;
 #pragma unroll
-				for (int t = C + 1; t < RPT; ++t) sCol[w][t] = x[t];
+				for (int t = C + 1; t < RPT; ++t) v[t] = lane_value(x[t], i);
 			}
-			__builtin_amdgcn_wave_barrier();
 			double d0 = 0.0, d1 = 0.0;
 #pragma unroll
 			for (int t = C; t < RPT; ++t) {
-				v[t] = sCol[w][t];
 				if ((t - C) & 1) d1 = fma(v[t], x[t], d1);
 				else d0 = fma(v[t], x[t], d0);
 			}
@@ -300,15 +299,10 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 		for (int i = min(k, (C + 1) * HW) - 1; i >= C * HW; --i) {
 			const double ti = sTau[i];
 			if (ti == 0.0) continue;
-			if (lane == i) {
-#pragma unroll
-				for (int t = C; t < RPT; ++t) sCol[w][t] = x[t];
-			}
-			__builtin_amdgcn_wave_barrier();
 			double c0 = 0.0, c1 = 0.0;
 #pragma unroll
 			for (int t = C; t < RPT; ++t) {
-				v[t] = sCol[w][t];
+				v[t] = lane_value(x[t], i);  // reflector i (lane i's column) on the wave's rows
 				if ((t - C) & 1) c1 = fma(v[t], q[t], c1);
 				else c0 = fma(v[t], q[t], c0);
 			}
@@ -351,11 +345,10 @@ struct ApplyArgs {
 	const int* ncDev;     // when set, the column count is read here (<= nc)
 };
 
-// out_b = H_0 · … · H_{k-1} · [init_b; 0] for every block b of the level. The block's reflectors are staged
-// in LDS as full columns u_i (zero above row i, one on it) and read as broadcasts.
+// out_b = H_0 · … · H_{k-1} · [init_b; 0] for every block b of the level. Lane i holds the block's reflector
+// column u_i (zero above row i, one on it) for the wave's rows; the waves read it with v_readlane.
 template <int RPT>
 __global__ __launch_bounds__(HT) void hh_apply_kernel(ApplyArgs a) {
-	extern __shared__ double sV[];   // rows × HN
 	__shared__ double sW[2][HW][HN];
 	__shared__ double sTau[HN];
 	const int lane = threadIdx.x & 63;
@@ -366,11 +359,13 @@ __global__ __launch_bounds__(HT) void hh_apply_kernel(ApplyArgs a) {
 	const int n = a.n;
 	const int k = min(rows, n);
 	const int nc = a.ncDev ? __builtin_amdgcn_readfirstlane(*a.ncDev) : a.nc;
-	for (int e = threadIdx.x; e < RPT * HW * HN; e += HT) {
-		const int r = e / HN, c = e % HN;
+	double vr[RPT];
+#pragma unroll
+	for (int t = 0; t < RPT; ++t) {
+		const int r = t * HW + w;
 		double z = 0.0;
-		if (r < rows && c < k) z = r > c ? a.V[(size_t)(r0 + r) * n + c] : (r == c ? 1.0 : 0.0);
-		sV[e] = z;
+		if (r < rows && lane < k) z = r > lane ? a.V[(size_t)(r0 + r) * n + lane] : (r == lane ? 1.0 : 0.0);
+		vr[t] = z;
 	}
 	if (threadIdx.x < k) sTau[threadIdx.x] = a.tau[(size_t)b * n + threadIdx.x];
 	double q[RPT], u[RPT];
@@ -392,7 +387,7 @@ __global__ __launch_bounds__(HT) void hh_apply_kernel(ApplyArgs a) {
 			double c0 = 0.0, c1 = 0.0;
 #pragma unroll
 			for (int t = C; t < RPT; ++t) {
-				u[t] = sV[(t * HW + w) * HN + i];
+				u[t] = lane_value(vr[t], i);
 				if ((t - C) & 1) c1 = fma(u[t], q[t], c1);
 				else c0 = fma(u[t], q[t], c0);
 			}
@@ -478,12 +473,7 @@ void launch_factor(const FactorArgs& a, int blocks, bool piv, hipStream_t s) {
 
 template <int RPT>
 void apply_rpt(const ApplyArgs& a, int blocks, hipStream_t s) {
-	static std::once_flag once;
-	const size_t lds = (size_t)RPT * HW * HN * sizeof(double);
-	std::call_once(once, [&] {
-		XTT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&hh_apply_kernel<RPT>),
-		                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-	});
+	const size_t lds = 0;
 	double f = 0;
 	if (prof_enabled())
 		for (int b = 0; b < blocks; ++b) {
@@ -507,7 +497,7 @@ void launch_apply(const ApplyArgs& a, int blocks, hipStream_t s) {
 
 // C_b ← Q_bᵀ·C_b = H_{k-1}·…·H_0·C_b for every block b of a factored level, in place: C holds the level's rows
 // (ld ldc), 64 columns per grid.y chunk. The trailing-column update of a panel factorisation; reflectors are
-// staged in LDS as in hh_apply_kernel and applied in forward order.
+// held and read as in hh_apply_kernel and applied in forward order.
 struct ApplyTArgs {
 	const double* V;
 	const double* tau;
@@ -519,7 +509,6 @@ struct ApplyTArgs {
 
 template <int RPT>
 __global__ __launch_bounds__(HT) void hh_applyT_kernel(ApplyTArgs a) {
-	extern __shared__ double sV[];   // rows × HN
 	__shared__ double sW[2][HW][HN];
 	__shared__ double sTau[HN];
 	const int lane = threadIdx.x & 63;
@@ -531,11 +520,13 @@ __global__ __launch_bounds__(HT) void hh_applyT_kernel(ApplyTArgs a) {
 	const int k = min(rows, n);
 	const int c0 = blockIdx.y * HN;
 	const int ncc = min(HN, a.nc - c0);
-	for (int e = threadIdx.x; e < RPT * HW * HN; e += HT) {
-		const int r = e / HN, c = e % HN;
+	double vr[RPT];
+#pragma unroll
+	for (int t = 0; t < RPT; ++t) {
+		const int r = t * HW + w;
 		double z = 0.0;
-		if (r < rows && c < k) z = r > c ? a.V[(size_t)(r0 + r) * n + c] : (r == c ? 1.0 : 0.0);
-		sV[e] = z;
+		if (r < rows && lane < k) z = r > lane ? a.V[(size_t)(r0 + r) * n + lane] : (r == lane ? 1.0 : 0.0);
+		vr[t] = z;
 	}
 	if (threadIdx.x < k) sTau[threadIdx.x] = a.tau[(size_t)b * n + threadIdx.x];
 	double q[RPT], u[RPT];
@@ -557,7 +548,7 @@ __global__ __launch_bounds__(HT) void hh_applyT_kernel(ApplyTArgs a) {
 			double c0s = 0.0, c1s = 0.0;
 #pragma unroll
 			for (int t = C; t < RPT; ++t) {
-				u[t] = sV[(t * HW + w) * HN + i];
+				u[t] = lane_value(vr[t], i);
 				if ((t - C) & 1) c1s = fma(u[t], q[t], c1s);
 				else c0s = fma(u[t], q[t], c0s);
 			}
@@ -612,12 +603,7 @@ __global__ void panel_split_kernel(const double* Rb, int w, const double* C, lon
 
 template <int RPT>
 void applyT_rpt(const ApplyTArgs& a, int blocks, hipStream_t s) {
-	static std::once_flag once;
-	const size_t lds = (size_t)RPT * HW * HN * sizeof(double);
-	std::call_once(once, [&] {
-		XTT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&hh_applyT_kernel<RPT>),
-		                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-	});
+	const size_t lds = 0;
 	double f = 0;
 	if (prof_enabled())
 		for (int b = 0; b < blocks; ++b) {
