@@ -173,23 +173,31 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 			if (w == wi) sRow[p][lane] = x[C];
 			__syncthreads();
 			// phase B: reflector (dlarfg) and A(i:, i+1:) -= u·wᵀ with u = [1; scal·a(i+1:, i)]
-			double xn2 = 0.0, sa = 0.0, sb = 0.0;
+			double sa = 0.0, sb = 0.0;
 #pragma unroll
 			for (int q = 0; q < HW; q += 2) {
-				xn2 += sW[p][q][i] + sW[p][q + 1][i];
 				sa += sW[p][q][lane];
 				sb += sW[p][q + 1][lane];
 			}
 			const double s = sa + sb;
+			const double xn2 = lane_value(s, i);  // lane i's sum is ‖a(i+1:, i)‖²
 			const double alpha = sRow[p][i];
 			double beta = alpha, tau = 0.0, scal = 1.0;
 			if (xn2 != 0.0) {
 				const double a2 = fma(alpha, alpha, xn2);
 				// sqrt(alpha² + ‖x‖²) directly unless it leaves the safe range (then dlapy2's scaled form)
-				const double nrm = (a2 < 1e300 && a2 > 1e-300) ? sqrt(a2) : hypot(alpha, sqrt(xn2));
+				const bool safe = a2 < 1e300 && a2 > 1e-300;
+				const double nrm = safe ? sqrt(a2) : hypot(alpha, sqrt(xn2));
 				beta = -copysign(nrm, alpha);
-				tau = (beta - alpha) / beta;
-				scal = 1.0 / (alpha - beta);
+				const double amb = alpha - beta;
+				if (safe) {  // tau = (beta − alpha)/beta and scal = 1/(alpha − beta) from one reciprocal
+					const double rc = 1.0 / (beta * amb);
+					tau = -amb * amb * rc;
+					scal = beta * rc;
+				} else {
+					tau = (beta - alpha) / beta;
+					scal = 1.0 / amb;
+				}
 			}
 			const double wj = (lane > i && lane < n) ? tau * fma(scal, s, sRow[p][lane]) : 0.0;
 			const double rij = sRow[p][lane] - wj;  // row i after the update (lanes > i)
