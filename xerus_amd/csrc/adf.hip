@@ -130,7 +130,7 @@ __global__ __launch_bounds__(AB) void adf_residual_kernel(AdfDesc a, const T* co
 // (a, ·, c) of every core and adds res_s·F_{k-1}[s][a]·B_{k+1}[s][c] to its own accumulator (k, i_k(s)) in the
 // wave's LDS region: every address has one owner lane, so the adds need no ordering between lanes and the
 // result is deterministic (samples in order, waves summed in order, blocks by the reduce kernel).
-constexpr int FW = 4;        // waves per block
+constexpr int FW = 8;        // waves per gradient block (two per SIMD); the residual kernel uses AB threads
 constexpr int FR = 8;        // rank bound
 constexpr int PB = FR * FR + 4;  // floats per padded 8×8 block: the extra 4 spread blocks over the LDS banks
 constexpr int FD = 16;       // order bound
@@ -250,22 +250,26 @@ __global__ __launch_bounds__(AB) void adf_residual_fast_kernel(AdfDesc a, const 
 	}
 }
 
-// Gradient: one block of FW = 4 waves takes rounds of FS = 128 samples. Phase 1: waves 0-1 walk the left
+// Gradient: one block of FW = 8 waves takes rounds of FS = 128 samples. Phase 1: waves 0-1 walk the left
 // stacks F_k of the round's samples (one per lane), waves 2-3 the right stacks B_k, from the LDS copy of the
-// padded cores, into the block's staging area. Phase 2: wave w owns cores k ≡ w (mod 4) and adds the round's
-// contribution to its gradient with MFMA: G_k[a, j, c] += Σ_s (res_s·F_{k-1}[s][a]) · (B_{k+1}[s][c]·[i_k(s) = j])
-// is an (8 × 128)·(128 × 8n_k) product, run as v_mfma_f32_16x16x4_f32 over K = samples with the accumulators
-// in registers. The summation order is fixed, so the result is deterministic (the blocks' partials are summed in
-// block order by the reduce kernel).
+// padded cores, into the block's staging area. Phase 2: wave w owns cores k ≡ w (mod 8) and adds the round's
+// contribution to its gradient with MFMA: G_k[a, j, c] += Σ_{s: i_k(s) = j} (res_s·F_{k-1}[s][a]) · B_{k+1}[s][c].
+// The wave first counting-sorts the round's samples by i_k (ballot / mbcnt per bin, each bin padded to a
+// multiple of 4), so every v_mfma_f32_16x16x4_f32 step takes 4 samples of one bin and updates only that bin's
+// 16-column tile (bins 2t and 2t+1 share tile t). The summation order is fixed (bins in order, samples in round
+// order), so the result is deterministic; the blocks' partials are summed in block order by the reduce kernel.
 constexpr int FS = 128;
 constexpr int FSP = FS + 1;  // staging row stride: lanes reading rows a = 0..7 of one sample hit different banks
 
 constexpr int GNT = 8;       // 16-column MFMA tiles per core (8·n_k <= 128, n_k <= 16)
 constexpr int GKC = FD / FW; // cores per wave
 
+constexpr int FPERM = FS + 3 * 16;  // per-wave sorted sample slots: a round plus the padding of up to 16 bins
+
 __host__ __device__ inline size_t fast_grad_floats(int d, int sumN) {
 	return (size_t)sumN * PB + (size_t)FR * FSP                      // padded cores, e_0 rows
-	       + (size_t)2 * (d - 1) * FR * FSP + (size_t)d * FS + FS;  // F, B, positions, residuals
+	       + (size_t)2 * (d - 1) * FR * FSP + (size_t)d * FS + FS    // F, B, positions, residuals
+	       + (size_t)FW * FPERM;                                     // per-wave sorted order
 }
 
 typedef float gf4 __attribute__((ext_vector_type(4)));
@@ -294,6 +298,7 @@ __global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const
 	float* Bs = Fs + (size_t)(d - 1) * FR * FSP;                        // B_k at [((k-1)·FR + a)·FSP + s], k = 1..d-1
 	int* Is = reinterpret_cast<int*>(Bs + (size_t)(d - 1) * FR * FSP);  // i_k at [k·FS + s]
 	float* Rs = reinterpret_cast<float*>(Is + (size_t)d * FS);         // res at [s]
+	int* Ps = reinterpret_cast<int*>(Rs + FS) + w * FPERM;              // this wave's sorted sample slots (-1: padding)
 	for (int e = tid; e < nslots * PB / 4; e += FW * 64)
 		reinterpret_cast<float4*>(Gs)[e] = reinterpret_cast<const float4*>(Gp)[e];
 	for (int e = tid; e < FR * FSP; e += FW * 64) Es[e] = e < FSP ? 1.f : 0.f;
@@ -308,9 +313,9 @@ __global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const
 	__syncthreads();
 	for (int base = s0; base < s1; base += FS) {
 		const int cnt = min(FS, s1 - base);
-		{
+		if (w < 4) {  // phase 1 walkers (wave-uniform)
 			const int s = tid & (FS - 1);
-			const bool fwd = w < 2;  // wave-uniform
+			const bool fwd = w < 2;
 			bool valid = s < cnt;
 			const uint32_t* p = pos + (size_t)(base + (valid ? s : 0)) * d;
 			uint32_t pk[FD];
@@ -350,20 +355,43 @@ __global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const
 		for (int q = 0; q < GKC; ++q) {
 			const int k = w + q * FW;
 			if (k >= d || (kSel >= 0 && k != kSel)) continue;
-			const int nt = (sN[k] * FR + 15) / 16;
+			const int nk = sN[k];
 			const float* fk = (k > 0 ? Fs + (k - 1) * FR * FSP : Es) + (li < FR ? li : 0) * FSP;
 			const float* bk = (k + 1 < d ? Bs + k * FR * FSP : Es) + (li & 7) * FSP;
 			const int* ik = Is + k * FS;
-			for (int kb = 0; kb < FS; kb += 4) {
-				const int sidx = kb + lk;
-				const float av = li < FR ? Rs[sidx] * fk[sidx] : 0.f;
-				const int j = ik[sidx];
-				const float bv = bk[sidx];
+			// counting sort of the round's samples by i_k: lane holds samples lane and lane + 64
+			const int ja = lane < cnt ? ik[lane] : -1;
+			const int jb = lane + 64 < cnt ? ik[lane + 64] : -1;
+			for (int e = lane; e < FPERM; e += 64) Ps[e] = -1;
+			__builtin_amdgcn_wave_barrier();
+			int bstart[16], bend[16];
+			int st = 0;
 #pragma unroll
-				for (int t = 0; t < GNT; ++t) {
-					if (t < nt) {
-						const float b = (2 * t + (li >> 3) == j) ? bv : 0.f;
-						acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b, acc[q][t], 0, 0, 0);
+			for (int bin = 0; bin < 16; ++bin) {
+				const unsigned long long ma = __ballot(ja == bin), mb = __ballot(jb == bin);
+				const int ca = __popcll(ma), cb = __popcll(mb);
+				const int below_a = __builtin_amdgcn_mbcnt_hi((unsigned)(ma >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ma, 0));
+				const int below_b = __builtin_amdgcn_mbcnt_hi((unsigned)(mb >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mb, 0));
+				if (ja == bin) Ps[st + below_a] = lane;
+				if (jb == bin) Ps[st + ca + below_b] = lane + 64;
+				bstart[bin] = st;
+				bend[bin] = st + ((ca + cb + 3) & ~3);
+				st = bend[bin];
+			}
+			__builtin_amdgcn_wave_barrier();
+#pragma unroll
+			for (int t = 0; t < GNT; ++t) {
+				if (2 * t >= nk) continue;
+#pragma unroll
+				for (int h = 0; h < 2; ++h) {
+					const int bin = 2 * t + h;
+					for (int ks = bstart[bin]; ks < bend[bin]; ks += 4) {
+						const int sidx = Ps[ks + lk];
+						const bool ok = sidx >= 0;
+						const int sx = ok ? sidx : 0;
+						const float av = (li < FR && ok) ? Rs[sx] * fk[sx] : 0.f;
+						const float bv = (ok && (li >> 3) == h) ? bk[sx] : 0.f;
+						acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[q][t], 0, 0, 0);
 					}
 				}
 			}
