@@ -14,8 +14,8 @@
 // the tree, not LAPACK's single sweep, which only changes the TT gauge.
 //
 // Column pivoting (xerus's QC, blasLapackWrapper.cpp:244-305 with dgeqp3) runs on one block: directly on
-// the matrix when it has at most 256 rows, else on the tree's n×n root R (same column norms and the same
-// partial-norm downdating, since Q is orthogonal). The pivot rule and downdating are dlaqp2's: largest
+// the matrix when it has at most 256 rows, else as the tree's root on the stacked R factors of the level
+// below (same column norms and the same partial-norm downdating, since the levels' Q are orthogonal). The pivot rule and downdating are dlaqp2's: largest
 // partial norm, first index on ties, recompute when the downdated norm falls below sqrt(eps/2). The rank rule
 // is xerus's: the first r with |R_rr| < 16·ε·R_00, where R_00 carries dlarfg's sign for the original first
 // row (blasLapackWrapper.cpp:266-272).
@@ -185,18 +185,25 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 			double beta = alpha, tau = 0.0, scal = 1.0;
 			if (xn2 != 0.0) {
 				const double a2 = fma(alpha, alpha, xn2);
-				// sqrt(alpha² + ‖x‖²) directly unless it leaves the safe range (then dlapy2's scaled form)
-				const bool safe = a2 < 1e300 && a2 > 1e-300;
-				const double nrm = safe ? sqrt(a2) : hypot(alpha, sqrt(xn2));
-				beta = -copysign(nrm, alpha);
-				const double amb = alpha - beta;
-				if (safe) {  // tau = (beta − alpha)/beta and scal = 1/(alpha − beta) from one reciprocal
-					const double rc = 1.0 / (beta * amb);
-					tau = -amb * amb * rc;
-					scal = beta * rc;
-				} else {
+				if (a2 < 1e300 && a2 > 1e-300) {
+					// With y = 1/‖(alpha, x)‖: beta = −sign(alpha)/y, tau = (beta − alpha)/beta = 1 + |alpha|·y and
+					// scal = 1/(alpha − beta) = sign(alpha)·y/tau. y is v_rsq_f64 refined by two Newton steps
+					// y ← y + y·(1 − a2·y²)/2, 1/tau (tau in [1, 2]) is v_rcp_f64 refined by two steps r ← r + r·(1 − tau·r).
+					double y = __builtin_amdgcn_rsq(a2);
+					y = fma(0.5 * y, fma(-(a2 * y), y, 1.0), y);
+					y = fma(0.5 * y, fma(-(a2 * y), y, 1.0), y);
+					const double aa = fabs(alpha);
+					tau = fma(aa, y, 1.0);
+					double rt = __builtin_amdgcn_rcp(tau);
+					rt = fma(rt, fma(-tau, rt, 1.0), rt);
+					rt = fma(rt, fma(-tau, rt, 1.0), rt);
+					beta = -copysign(a2 * y, alpha);
+					scal = copysign(y * rt, alpha);
+				} else {  // outside the safe range: dlapy2's scaled norm and plain divisions
+					const double nrm = hypot(alpha, sqrt(xn2));
+					beta = -copysign(nrm, alpha);
 					tau = (beta - alpha) / beta;
-					scal = 1.0 / amb;
+					scal = 1.0 / (alpha - beta);
 				}
 			}
 			const double wj = (lane > i && lane < n) ? tau * fma(scal, s, sRow[p][lane]) : 0.0;
@@ -652,9 +659,11 @@ struct Level {
 };
 
 // Factor the m×n matrix through the tree. The root's R (min(m,n)×n) goes to rootR when given; when rootQ is
-// given the root also forms its Q (root rows × min(m,n), identity start) in the same launch.
+// given the root also forms its Q (root rows × min(m,n), identity start) in the same launch. With rootIn set,
+// the root is left to the caller: the levels below it are factored and *rootIn / *rootRows receive the root's
+// input (the stacked R factors of the last level, ld n, owned by that level).
 std::vector<Level> tree_factor(const double* A, long long ld, bool trans, int m, int n, double* rootR, double* rootQ,
-                               hipStream_t s) {
+                               hipStream_t s, const double** rootIn = nullptr, int* rootRows = nullptr) {
 	std::vector<Level> lv;
 	const double* in = A;
 	int rows = m;
@@ -663,6 +672,11 @@ std::vector<Level> tree_factor(const double* A, long long ld, bool trans, int m,
 		L.rows = rows;
 		level_geometry(rows, lv.empty(), n, L.blocks, L.rpb);
 		const bool root = L.blocks == 1;
+		if (root && rootIn) {
+			*rootIn = in;
+			*rootRows = rows;
+			break;
+		}
 		FactorArgs f{};
 		f.in = in; f.ld = lv.empty() ? ld : n; f.trans = lv.empty() && trans;
 		f.rows = rows; f.rpb = L.rpb; f.n = n;
@@ -816,9 +830,14 @@ void hh_qc(const double* A, size_t m, size_t n, double* Q, double* C, int* jpvt,
 		launch_factor(f, 1, true, s);
 		return;
 	}
-	DBuf<double> R1(n * n), Q2(n * n);
-	std::vector<Level> lv = tree_factor(A, (long long)n, false, (int)m, (int)n, R1.get(), nullptr, s);
-	f.in = R1.get(); f.ld = (long long)n; f.rows = f.rpb = (int)n;
+	// The levels below the root reduce A to the stacked R factors of its last level (at most HBR rows); the
+	// pivoted factorisation runs on that stack directly. Its column norms are A's, since every level's Q is
+	// orthogonal.
+	const double* rootIn = nullptr;
+	int rootRows = 0;
+	std::vector<Level> lv = tree_factor(A, (long long)n, false, (int)m, (int)n, nullptr, nullptr, s, &rootIn, &rootRows);
+	DBuf<double> Q2((size_t)rootRows * n);
+	f.in = rootIn; f.ld = (long long)n; f.rows = f.rpb = rootRows;
 	f.orig = A; f.origRows = (int)m;
 	f.Q = Q2.get();
 	launch_factor(f, 1, true, s);
