@@ -8,10 +8,11 @@
 //
 // Matrices taller than one block are factored as a reduction tree (TSQR): the leaves factor row blocks, each
 // level stacks the n×n R factors of up to 256/n children and factors the stack again, until one block is
-// left. Q is formed top-down: the root applies its reflectors to the identity (in the same launch as its
-// factorisation), every node applies its reflectors to its n×c slice of the parent's result. The result is a Householder QR
-// of the whole matrix (Q with orthonormal columns also for rank-deficient input); its diagonal signs follow
-// the tree, not LAPACK's single sweep, which only changes the TT gauge.
+// left. Q is formed top-down, each level in one launch of hh_wy_kernel: the root applies its reflectors to the
+// identity, every node applies its reflectors to its n×c slice of the parent's result, both in compact WY form
+// from the reflector Gram matrix that the factorisation leaves behind. The result is a Householder QR of the
+// whole matrix (Q with orthonormal columns also for rank-deficient input); its diagonal signs follow the tree,
+// not LAPACK's single sweep, which only changes the TT gauge.
 //
 // Column pivoting (xerus's QC, blasLapackWrapper.cpp:244-305 with dgeqp3) runs on one block: directly on
 // the matrix when it has at most 256 rows, else as the tree's root on the stacked R factors of the level
@@ -70,11 +71,9 @@ struct FactorArgs {
 	double* V;           // rows × n: R on/above, reflectors below the diagonal of each block (may be null)
 	double* tau;         // blocks × n (may be null when V is)
 	double* R;           // blocks × (n × n) upper triangles, ld n (may be null); block b writes min(rows_b, n) rows
-	// Q of a single-block level (FORMQ): Q = H_0…H_{k-1}·[init; 0], rows × nc (ld nc); init null = identity
-	double* Q;
-	const double* init;
-	int nc;
-	// column pivoting (single block); with FORMQ, nc = the rank
+	double* Y;           // blocks × (n × n): Y[l][i] = u_lᵀu_i for l < i (the reflectors' Gram matrix above the
+	                     // diagonal, for the compact WY form of their product); may be null, entries l >= i unset
+	// column pivoting (single block)
 	int* jpvt;           // n: original column of each position
 	int* rank;           // xerus's QC rank
 	double* C;           // rank × n, C[row][jpvt[col]] = R[row][col] (row <= col)
@@ -82,7 +81,7 @@ struct FactorArgs {
 	int origRows;
 };
 
-template <int RPT, bool PIV, bool FORMQ>
+template <int RPT, bool PIV>
 __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 	__shared__ double sW[2][HW][HN];   // per-wave Σ_{own r > i} a_ri·a_rj (lane i: the norm² partial)
 	__shared__ double sRow[2][HN];     // row i
@@ -108,6 +107,7 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 	}
 
 	double vn1 = 0.0, vn2 = 0.0;  // partial column norms, identical in every wave
+	double myScal = 1.0;          // lane i: the scale of reflector i (u_i = scal_i·a below the diagonal)
 	int jp = lane;
 	const double tol3z = sqrt(DBL_EPSILON * 0.5);
 	if (PIV) {
@@ -206,8 +206,13 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 					scal = 1.0 / (alpha - beta);
 				}
 			}
-			const double wj = (lane > i && lane < n) ? tau * fma(scal, s, sRow[p][lane]) : 0.0;
+			// f_j = a_ij + scal·d_j. Lanes j > i: the update's w_j = tau·f_j. Lanes l < i hold reflector l
+			// (unscaled) on rows > l, so u_lᵀu_i = scal_l·(a_il + scal_i·d_l) = scal_l·f_l.
+			const double f = fma(scal, s, sRow[p][lane]);
+			const double wj = (lane > i && lane < n) ? tau * f : 0.0;
 			const double rij = sRow[p][lane] - wj;  // row i after the update (lanes > i)
+			if (lane == i) myScal = scal;
+			if (a.Y && w == 0 && lane < i) a.Y[((size_t)b * n + lane) * n + i] = myScal * f;
 			if (w == 0) {
 				sR[i][lane] = lane > i ? rij : (lane == i ? beta : 0.0);
 				if (lane == 0) {
@@ -254,17 +259,14 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 	chunk(std::integral_constant<int, 7>{});
 	__syncthreads();
 
-	// compact output (R on and above the diagonal, scaled reflectors below); x becomes the reflector
-	// columns u (one on the diagonal, zero above)
+	// compact output: R on and above the diagonal, the scaled reflectors u below
 #pragma unroll
 	for (int t = 0; t < RPT; ++t) {
 		const int r = t * HW + w;
-		const double u = lane < k ? (r > lane ? x[t] * sScal[lane] : (r == lane ? 1.0 : 0.0)) : 0.0;
+		const double u = lane < k && r > lane ? x[t] * sScal[lane] : 0.0;
 		if (a.V && r < rows && lane < n) a.V[(size_t)(r0 + r) * n + lane] = (r < k && lane >= r) ? sR[r][lane] : u;
-		x[t] = u;
 	}
 	if (a.tau && threadIdx.x < k) a.tau[(size_t)b * n + threadIdx.x] = sTau[threadIdx.x];
-	int nc = a.nc;
 	if (!PIV) {
 		if (a.R) {
 			for (int r = w; r < k; r += HW)
@@ -293,66 +295,13 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 		const int rank = sRank;
 		for (int r = w; r < rank; r += HW)
 			if (lane < n) a.C[(size_t)r * n + jp] = sR[r][lane];
-		nc = rank;
-	}
-	if (!FORMQ) return;
-
-	// Q = H_0 … H_{k-1} · [init; 0] for this block (single-block levels only: r0 = 0)
-	double q[RPT];
-#pragma unroll
-	for (int t = 0; t < RPT; ++t) {
-		const int r = t * HW + w;
-		double z = 0.0;
-		if (r < k && lane < nc) z = a.init ? a.init[(size_t)r * nc + lane] : (r == lane ? 1.0 : 0.0);
-		q[t] = z;
-	}
-	int ph = 0;
-	// reflectors in reverse; in chunk C the rows of registers t < C lie above every reflector of the chunk
-	auto qchunk = [&](auto cc) {
-		constexpr int C = decltype(cc)::value;
-		if (C * HW >= k) return;
-		for (int i = min(k, (C + 1) * HW) - 1; i >= C * HW; --i) {
-			const double ti = sTau[i];
-			if (ti == 0.0) continue;
-			double c0 = 0.0, c1 = 0.0;
-#pragma unroll
-			for (int t = C; t < RPT; ++t) {
-				v[t] = lane_value(x[t], i);  // reflector i (lane i's column) on the wave's rows
-				if ((t - C) & 1) c1 = fma(v[t], q[t], c1);
-				else c0 = fma(v[t], q[t], c0);
-			}
-			sW[ph][w][lane] = c0 + c1;
-			__syncthreads();
-			double wa = 0.0, wb = 0.0;
-#pragma unroll
-			for (int s2 = 0; s2 < HW; s2 += 2) {
-				wa += sW[ph][s2][lane];
-				wb += sW[ph][s2 + 1][lane];
-			}
-			const double wj = (wa + wb) * ti;
-#pragma unroll
-			for (int t = C; t < RPT; ++t) q[t] = fma(-v[t], wj, q[t]);
-			ph ^= 1;
-		}
-	};
-	qchunk(std::integral_constant<int, 7>{});
-	qchunk(std::integral_constant<int, 6>{});
-	qchunk(std::integral_constant<int, 5>{});
-	qchunk(std::integral_constant<int, 4>{});
-	qchunk(std::integral_constant<int, 3>{});
-	qchunk(std::integral_constant<int, 2>{});
-	qchunk(std::integral_constant<int, 1>{});
-	qchunk(std::integral_constant<int, 0>{});
-#pragma unroll
-	for (int t = 0; t < RPT; ++t) {
-		const int r = t * HW + w;
-		if (r < rows && lane < nc) a.Q[(size_t)r * nc + lane] = q[t];
 	}
 }
 
 struct ApplyArgs {
 	const double* V;      // a factored level (same geometry as its FactorArgs)
 	const double* tau;
+	const double* Y;      // the level's reflector Gram matrices (FactorArgs::Y)
 	int rows, rpb, n;
 	const double* init;   // (blocks·n) × nc, ld nc: block b starts from rows [b·n, b·n+n); null: identity
 	double* out;          // rows × nc, ld nc
@@ -360,159 +309,242 @@ struct ApplyArgs {
 	const int* ncDev;     // when set, the column count is read here (<= nc)
 };
 
-// out_b = H_0 · … · H_{k-1} · [init_b; 0] for every block b of the level. Lane i holds the block's reflector
-// column u_i (zero above row i, one on it) for the wave's rows; the waves read it with v_readlane.
-template <int RPT>
-__global__ __launch_bounds__(HT) void hh_apply_kernel(ApplyArgs a) {
-	__shared__ double sW[2][HW][HN];
+typedef double wy_d4 __attribute__((ext_vector_type(4)));
+// v_mfma_f64_16x16x4f64: lane (li, lk) = (lane & 15, lane >> 4) supplies A[li][lk] and B[lk][li] and holds
+// D[4v + lk][li] in component v.
+__device__ inline wy_d4 mfma_f64(double a, double b, wy_d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+// out_b = H_0·…·H_{k-1}·[init_b; 0] for every block b of a factored level, in compact WY form (Schreiber–Van
+// Loan): H_0·…·H_{k-1} = I − U·T·Uᵀ, U the unit lower trapezoidal reflectors, T upper triangular with dlarft's
+// forward recurrence T[0:i, i] = −τ_i·T[0:i, 0:i]·Y[0:i, i], Y = UᵀU above the diagonal.
+//   staging: U's top k×k triangle (unit diagonal, zero above), Y, τ and init_b into LDS;
+//   phase 1: P = Uᵀ·[init_b; 0] = U_topᵀ·init_b (k × nc), 16×16 MFMA tiles on waves 0..6;
+//   phase 2: Z = T·P in chunks of 8 rows from the bottom, Z_C = T_CC·(P_C − Y_{C,>C}·Z_{>C}); the diagonal blocks
+//            T_CC are formed during phase 1 by one wave, a lane per row, from T[w][j] = −τ_j·Σ_{l=w}^{j-1} T[w][l]·Y[l][j]
+//            (only row w enters);
+//   phase 3: out = [init_b; 0] − U·Z, 16×16 MFMA tiles, U read per lane with its triangle masked.
+// τ_i = 0 (an identity reflector) zeroes row and column i of T.
+__global__ __launch_bounds__(HT) void hh_wy_kernel(ApplyArgs a) {
+	__shared__ double sU[HN][HN + 1];   // U's top triangle, zero outside k×k (odd stride: column reads spread over banks)
+	__shared__ double sY[HN][HN];
+	__shared__ double sI[HN][HN];       // init_b, zero outside k × nc
+	__shared__ double sP[HN + HW][HN];  // P, then Z; zero outside k × nc
+	__shared__ double sC[HW][HN];       // the chunk's right-hand sides
+	__shared__ double sT[HN][HW];       // the 8×8 diagonal blocks of T: row i holds T[i][8·(i/8) + j]
 	__shared__ double sTau[HN];
 	const int lane = threadIdx.x & 63;
 	const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int li = lane & 15, lk = lane >> 4;
 	const int b = blockIdx.x;
 	const int r0 = b * a.rpb;
 	const int rows = min(a.rpb, a.rows - r0);
 	const int n = a.n;
 	const int k = min(rows, n);
 	const int nc = a.ncDev ? __builtin_amdgcn_readfirstlane(*a.ncDev) : a.nc;
-	double vr[RPT];
+	const double* U = a.V + (size_t)r0 * n;
+	const double* Yg = a.Y + (size_t)b * n * n;
+	const bool ident = a.init == nullptr;
+	{  // staging: all loads of a thread in flight together
+		constexpr int E = HN * HN / HT;
+		double u[E], y[E], z[E];
 #pragma unroll
-	for (int t = 0; t < RPT; ++t) {
-		const int r = t * HW + w;
-		double z = 0.0;
-		if (r < rows && lane < k) z = r > lane ? a.V[(size_t)(r0 + r) * n + lane] : (r == lane ? 1.0 : 0.0);
-		vr[t] = z;
-	}
-	if (threadIdx.x < k) sTau[threadIdx.x] = a.tau[(size_t)b * n + threadIdx.x];
-	double q[RPT], u[RPT];
+		for (int q = 0; q < E; ++q) {
+			const int e = q * HT + threadIdx.x, r = e / HN, c = e % HN;
+			u[q] = (r < k && c < r) ? U[(size_t)r * n + c] : 0.0;
+			y[q] = (r < c && c < k) ? Yg[(size_t)r * n + c] : 0.0;
+			z[q] = (!ident && r < k && c < nc) ? a.init[((size_t)b * n + r) * nc + c] : 0.0;
+		}
 #pragma unroll
-	for (int t = 0; t < RPT; ++t) {
-		const int r = t * HW + w;
-		double z = 0.0;
-		if (r < k && lane < nc) z = a.init ? a.init[((size_t)b * n + r) * nc + lane] : (r == lane ? 1.0 : 0.0);
-		q[t] = z;
+		for (int q = 0; q < E; ++q) {
+			const int e = q * HT + threadIdx.x, r = e / HN, c = e % HN;
+			sU[r][c] = (c == r && r < k) ? 1.0 : u[q];
+			sY[r][c] = y[q];
+			sI[r][c] = z[q];
+			sP[r][c] = 0.0;
+		}
+		if (threadIdx.x < HW * HN) sP[HN + threadIdx.x / HN][threadIdx.x % HN] = 0.0;
+		if (threadIdx.x < k) sTau[threadIdx.x] = a.tau[(size_t)b * n + threadIdx.x];
 	}
 	__syncthreads();
-	int ph = 0;
-	auto chunk = [&](auto cc) {
-		constexpr int C = decltype(cc)::value;
-		if (C * HW >= k) return;
-		for (int i = min(k, (C + 1) * HW) - 1; i >= C * HW; --i) {
-			const double ti = sTau[i];
-			if (ti == 0.0) continue;
-			double c0 = 0.0, c1 = 0.0;
+	const int kq = (k + 3) >> 2;        // MFMA k-steps over the k reflectors
+	const int cT = (nc + 15) >> 4;      // 16-column tiles of the product
+	if (w == HW - 1) {
+		// T's diagonal blocks, lane i = 8C + w' forms row w' of block C:
+		// T[i][c0 + j] = −τ_{c0+j}·Σ_{l=w'}^{j-1} T[i][c0 + l]·Y[c0 + l][c0 + j], T[i][i] = τ_i
+		const int c0 = lane & ~(HW - 1), wr = lane & (HW - 1);
+		double yb[HW][HW];
 #pragma unroll
-			for (int t = C; t < RPT; ++t) {
-				u[t] = lane_value(vr[t], i);
-				if ((t - C) & 1) c1 = fma(u[t], q[t], c1);
-				else c0 = fma(u[t], q[t], c0);
-			}
-			sW[ph][w][lane] = c0 + c1;
-			__syncthreads();
-			double wa = 0.0, wb = 0.0;
+		for (int l = 0; l < HW; ++l)
 #pragma unroll
-			for (int s = 0; s < HW; s += 2) {
-				wa += sW[ph][s][lane];
-				wb += sW[ph][s + 1][lane];
-			}
-			const double wj = (wa + wb) * ti;
+			for (int j = l + 1; j < HW; ++j) yb[l][j] = sY[c0 + l][c0 + j];
+		double t[HW];
 #pragma unroll
-			for (int t = C; t < RPT; ++t) q[t] = fma(-u[t], wj, q[t]);
-			ph ^= 1;
+		for (int j = 0; j < HW; ++j) {
+			double acc = 0.0;
+#pragma unroll
+			for (int l = 0; l < j; ++l) acc = fma(t[l], yb[l][j], acc);  // t[l] = 0 for l < wr
+			const double tau = c0 + j < k ? sTau[c0 + j] : 0.0;
+			t[j] = j < wr ? 0.0 : (j == wr ? tau : -tau * acc);
 		}
-	};
-	chunk(std::integral_constant<int, 7>{});
-	chunk(std::integral_constant<int, 6>{});
-	chunk(std::integral_constant<int, 5>{});
-	chunk(std::integral_constant<int, 4>{});
-	chunk(std::integral_constant<int, 3>{});
-	chunk(std::integral_constant<int, 2>{});
-	chunk(std::integral_constant<int, 1>{});
-	chunk(std::integral_constant<int, 0>{});
 #pragma unroll
-	for (int t = 0; t < RPT; ++t) {
-		const int r = t * HW + w;
-		if (r < rows && lane < nc) a.out[(size_t)(r0 + r) * nc + lane] = q[t];
+		for (int j = 0; j < HW; ++j) sT[lane][j] = t[j];
+	} else {
+		// phase 1: P = U_topᵀ·init_b (waves 0..6)
+		if (ident) {
+			for (int e = threadIdx.x; e < HN * HN; e += (HW - 1) * 64) {
+				const int i = e >> 6, c = e & 63;
+				if (i < k && c < nc) sP[i][c] = sU[c][i];
+			}
+		} else {
+			const int kT = (k + 15) >> 4;
+			for (int tile = w; tile < kT * cT; tile += HW - 1) {
+				const int ti = tile / cT, tc = tile - ti * cT;
+				wy_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+				for (int q = 0; q < HN / 4; ++q) {
+					if (q < kq) {
+						const int r = 4 * q + lk;
+						acc = mfma_f64(sU[r][ti * 16 + li], sI[r][tc * 16 + li], acc);
+					}
+				}
+#pragma unroll
+				for (int v = 0; v < 4; ++v) sP[ti * 16 + 4 * v + lk][tc * 16 + li] = acc[v];
+			}
+		}
+	}
+	__syncthreads();
+	// phase 2: Z_C = T_CC·(P_C − Y_{C,>C}·Z_{>C}) from the last chunk up
+	for (int c0 = ((k - 1) / HW) * HW; c0 >= 0; c0 -= HW) {
+		const int i = c0 + w;
+		if (i < k) {
+			double a0 = sP[i][lane], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+			int l = c0 + HW;
+			for (; l + 3 < k; l += 4) {
+				a0 = fma(-sY[i][l], sP[l][lane], a0);
+				a1 = fma(-sY[i][l + 1], sP[l + 1][lane], a1);
+				a2 = fma(-sY[i][l + 2], sP[l + 2][lane], a2);
+				a3 = fma(-sY[i][l + 3], sP[l + 3][lane], a3);
+			}
+			for (; l < k; ++l) a0 = fma(-sY[i][l], sP[l][lane], a0);
+			sC[w][lane] = (a0 + a1) + (a2 + a3);
+		}
+		__syncthreads();
+		if (i < k) {
+			double z = 0.0;
+#pragma unroll
+			for (int j = 0; j < HW; ++j)
+				if (c0 + j < k) z = fma(sT[i][j], sC[j][lane], z);  // T[i][c0 + j] = 0 for j < w
+			sP[i][lane] = z;
+		}
+		__syncthreads();
+	}
+	// phase 3: out = [init_b; 0] − U·Z, two tiles per pass for independent MFMA chains
+	const int rT = (rows + 15) >> 4;
+	const int nt = rT * cT;
+	for (int t0 = w; t0 < nt; t0 += 2 * HW) {
+		const int t1 = t0 + HW;
+		const bool two = t1 < nt;
+		const int tr0 = t0 / cT, tc0 = t0 - tr0 * cT;
+		const int tr1 = two ? t1 / cT : tr0, tc1 = two ? t1 - tr1 * cT : tc0;
+		const int ra0 = tr0 * 16 + li, ra1 = tr1 * 16 + li;
+		const double* u0 = U + (size_t)min(ra0, rows - 1) * n;
+		const double* u1 = U + (size_t)min(ra1, rows - 1) * n;
+		// all k-steps of both tiles' U entries are loaded before the MFMA chains (kq <= HN/4)
+		double x0[HN / 4], x1[HN / 4];
+#pragma unroll
+		for (int q = 0; q < HN / 4; ++q) {
+			const int ic = min(4 * q + lk, k - 1);
+			x0[q] = q < kq ? u0[ic] : 0.0;
+			x1[q] = q < kq ? u1[ic] : 0.0;
+		}
+		wy_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+		for (int q = 0; q < HN / 4; ++q) {
+			if (q < kq) {
+				const int i = 4 * q + lk;
+				const double a0 = (ra0 < rows && i < k) ? (i < ra0 ? x0[q] : (i == ra0 ? 1.0 : 0.0)) : 0.0;
+				const double a1 = (ra1 < rows && i < k) ? (i < ra1 ? x1[q] : (i == ra1 ? 1.0 : 0.0)) : 0.0;
+				acc0 = mfma_f64(a0, sP[i][tc0 * 16 + li], acc0);
+				acc1 = mfma_f64(a1, sP[i][tc1 * 16 + li], acc1);
+			}
+		}
+#pragma unroll
+		for (int v = 0; v < 4; ++v) {
+			const int c0 = tc0 * 16 + li, row0 = tr0 * 16 + 4 * v + lk;
+			if (row0 < rows && c0 < nc) {
+				const double base = row0 < k ? (ident ? (row0 == c0 ? 1.0 : 0.0) : sI[row0][c0]) : 0.0;
+				a.out[(size_t)(r0 + row0) * nc + c0] = base - acc0[v];
+			}
+			const int c1 = tc1 * 16 + li, row1 = tr1 * 16 + 4 * v + lk;
+			if (two && row1 < rows && c1 < nc) {
+				const double base = row1 < k ? (ident ? (row1 == c1 ? 1.0 : 0.0) : sI[row1][c1]) : 0.0;
+				a.out[(size_t)(r0 + row1) * nc + c1] = base - acc1[v];
+			}
+		}
 	}
 }
 
 int rows_per_thread(int rpb) { return rpb <= 8 * HW ? 8 : (rpb <= 16 * HW ? 16 : 32); }
 
-// Householder flops of one level (dgeqrf's count per block: 2n²(m − n/3) for m >= n, 2m²(n − m/3) otherwise),
-// plus, with FORMQ, k reflectors applied to m × nc: 4·nc·(m·k − k²/2).
-double factor_flops(const FactorArgs& a, int blocks, bool formq) {
+// Householder flops of one level (dgeqrf's count per block: 2n²(m − n/3) for m >= n, 2m²(n − m/3) otherwise).
+double factor_flops(const FactorArgs& a, int blocks) {
 	double f = 0;
 	for (int b = 0; b < blocks; ++b) {
-		const double m = std::min(a.rpb, a.rows - b * a.rpb), n = a.n, k = std::min(m, n);
+		const double m = std::min(a.rpb, a.rows - b * a.rpb), n = a.n;
 		f += m >= n ? 2.0 * n * n * (m - n / 3.0) : 2.0 * m * m * (n - m / 3.0);
-		if (formq) f += 4.0 * a.nc * (m * k - k * k / 2.0);
 	}
 	return f;
 }
 
-template <bool PIV, bool FORMQ>
+template <bool PIV>
 void factor_variant(const FactorArgs& a, int blocks, hipStream_t s) {
 	const int rpt = rows_per_thread(a.rpb);
 	const dim3 g(blocks), t(HT);
-	static const char* tags[3][2] = {{"hh_factor_kernel<8,0,0>", "hh_factor_kernel<8,0,1>"},
-	                                 {"hh_factor_kernel<16,0,0>", "hh_factor_kernel<16,0,1>"},
-	                                 {"hh_factor_kernel<32,0,0>", "hh_factor_kernel<32,0,1>"}};
-	static const char* ptags[3][2] = {{"hh_factor_kernel<8,1,0>", "hh_factor_kernel<8,1,1>"},
-	                                  {"hh_factor_kernel<16,1,0>", "hh_factor_kernel<16,1,1>"},
-	                                  {"hh_factor_kernel<32,1,0>", "hh_factor_kernel<32,1,1>"}};
+	static const char* tags[3] = {"hh_factor_kernel<8,0>", "hh_factor_kernel<16,0>", "hh_factor_kernel<32,0>"};
+	static const char* ptags[3] = {"hh_factor_kernel<8,1>", "hh_factor_kernel<16,1>", "hh_factor_kernel<32,1>"};
 	const int ri = rpt == 8 ? 0 : (rpt == 16 ? 1 : 2);
-	std::string tag = (PIV ? ptags : tags)[ri][FORMQ ? 1 : 0];
+	std::string tag = (PIV ? ptags : tags)[ri];
 	if (prof_shapes()) tag += " rows=" + std::to_string(a.rows) + " n=" + std::to_string(a.n) + " blocks=" + std::to_string(blocks);
 	Work work;
 	if (prof_enabled()) {
-		// level matrix in; reflectors, τ, the blocks' R rows and (FORMQ) Q out
+		// level matrix in; reflectors, τ, Y and the blocks' R rows out
 		const double k = std::min(a.rpb, a.n);
-		work.flops = factor_flops(a, blocks, FORMQ);
+		work.flops = factor_flops(a, blocks);
 		work.bytes = 8.0 * ((double)a.rows * a.n + (a.V ? (double)a.rows * a.n + (double)blocks * a.n : 0.0) +
-		                    (a.R ? (double)blocks * k * a.n : 0.0) + (FORMQ ? (double)a.rows * a.nc : 0.0));
+		                    (a.R ? (double)blocks * k * a.n : 0.0) + (a.Y ? (double)blocks * k * (k - 1) / 2.0 : 0.0));
 	}
-	if (rpt == 8) launch_timed(tag.c_str(), work, hh_factor_kernel<8, PIV, FORMQ>, g, t, 0, s, a);
-	else if (rpt == 16) launch_timed(tag.c_str(), work, hh_factor_kernel<16, PIV, FORMQ>, g, t, 0, s, a);
-	else launch_timed(tag.c_str(), work, hh_factor_kernel<32, PIV, FORMQ>, g, t, 0, s, a);
+	if (rpt == 8) launch_timed(tag.c_str(), work, hh_factor_kernel<8, PIV>, g, t, 0, s, a);
+	else if (rpt == 16) launch_timed(tag.c_str(), work, hh_factor_kernel<16, PIV>, g, t, 0, s, a);
+	else launch_timed(tag.c_str(), work, hh_factor_kernel<32, PIV>, g, t, 0, s, a);
 }
 
 void launch_factor(const FactorArgs& a, int blocks, bool piv, hipStream_t s) {
-	const bool formq = a.Q != nullptr;
-	XTT_REQUIRE(!formq || blocks == 1, "hh_factor: Q formation in the factor launch needs a single block");
-	if (piv) {
-		if (formq) factor_variant<true, true>(a, blocks, s); else factor_variant<true, false>(a, blocks, s);
-	} else {
-		if (formq) factor_variant<false, true>(a, blocks, s); else factor_variant<false, false>(a, blocks, s);
-	}
+	XTT_REQUIRE(!piv || blocks == 1, "hh_factor: column pivoting needs a single block");
+	if (piv) factor_variant<true>(a, blocks, s);
+	else factor_variant<false>(a, blocks, s);
 	XTT_LAUNCH_CHECK();
 }
 
-template <int RPT>
-void apply_rpt(const ApplyArgs& a, int blocks, hipStream_t s) {
-	const size_t lds = 0;
+void launch_wy(const ApplyArgs& a, int blocks, hipStream_t s) {
 	double f = 0;
 	if (prof_enabled())
 		for (int b = 0; b < blocks; ++b) {
 			const double m = std::min(a.rpb, a.rows - b * a.rpb), k = std::min<double>(m, a.n);
-			f += 4.0 * a.nc * (m * k - k * k / 2.0);
+			f += 2.0 * a.nc * (m * k + k * k);
 		}
-	std::string tag = RPT == 8 ? "hh_apply_kernel<8>" : (RPT == 16 ? "hh_apply_kernel<16>" : "hh_apply_kernel<32>");
+	std::string tag = "hh_wy_kernel";
 	if (prof_shapes()) tag += " rows=" + std::to_string(a.rows) + " n=" + std::to_string(a.n) + " nc=" + std::to_string(a.nc);
-	// reflectors, τ and the blocks' start rows in; the level's Q rows out
-	const Work work(f, 8.0 * ((double)a.rows * a.n + (double)blocks * a.n + (double)blocks * a.n * a.nc + (double)a.rows * a.nc));
-	launch_timed(tag.c_str(), work, hh_apply_kernel<RPT>, dim3(blocks), dim3(HT), lds, s, a);
-}
-
-void launch_apply(const ApplyArgs& a, int blocks, hipStream_t s) {
-	const int rpt = rows_per_thread(a.rpb);
-	if (rpt == 8) apply_rpt<8>(a, blocks, s);
-	else if (rpt == 16) apply_rpt<16>(a, blocks, s);
-	else apply_rpt<32>(a, blocks, s);
+	// reflectors, τ, Y and the blocks' start rows in; the level's Q rows out
+	const double kk = std::min(a.rpb, a.n);
+	const Work work(f, 8.0 * ((double)a.rows * a.n + (double)blocks * (a.n + kk * (kk - 1) / 2.0) +
+	                          (a.init ? (double)blocks * a.n * a.nc : 0.0) + (double)a.rows * a.nc));
+	launch_timed(tag.c_str(), work, hh_wy_kernel, dim3(blocks), dim3(HT), 0, s, a);
 	XTT_LAUNCH_CHECK();
 }
 
 // C_b ← Q_bᵀ·C_b = H_{k-1}·…·H_0·C_b for every block b of a factored level, in place: C holds the level's rows
-// (ld ldc), 64 columns per grid.y chunk. The trailing-column update of a panel factorisation; reflectors are
-// held and read as in hh_apply_kernel and applied in forward order.
+// (ld ldc), 64 columns per grid.y chunk. The trailing-column update of a panel factorisation: lane i holds
+// reflector u_i on the wave's rows, the waves read it with v_readlane, reflectors applied in forward order.
 struct ApplyTArgs {
 	const double* V;
 	const double* tau;
@@ -655,14 +687,14 @@ void level_geometry(int rows, bool leaves, int n, int& blocks, int& rpb) {
 
 struct Level {
 	int rows = 0, rpb = 0, blocks = 0;
-	DBuf<double> V, tau, R;
+	DBuf<double> V, tau, Y, R;
 };
 
-// Factor the m×n matrix through the tree. The root's R (min(m,n)×n) goes to rootR when given; when rootQ is
-// given the root also forms its Q (root rows × min(m,n), identity start) in the same launch. With rootIn set,
-// the root is left to the caller: the levels below it are factored and *rootIn / *rootRows receive the root's
-// input (the stacked R factors of the last level, ld n, owned by that level).
-std::vector<Level> tree_factor(const double* A, long long ld, bool trans, int m, int n, double* rootR, double* rootQ,
+// Factor the m×n matrix through the tree. The root's R (min(m,n)×n) goes to rootR when given. With wantQ every
+// level keeps its reflectors, τ and Y for a later Q formation. With rootIn set, the root is left to the caller:
+// the levels below it are factored and *rootIn / *rootRows receive the root's input (the stacked R factors of
+// the last level, ld n, owned by that level).
+std::vector<Level> tree_factor(const double* A, long long ld, bool trans, int m, int n, double* rootR, bool wantQ,
                                hipStream_t s, const double** rootIn = nullptr, int* rootRows = nullptr) {
 	std::vector<Level> lv;
 	const double* in = A;
@@ -680,10 +712,11 @@ std::vector<Level> tree_factor(const double* A, long long ld, bool trans, int m,
 		FactorArgs f{};
 		f.in = in; f.ld = lv.empty() ? ld : n; f.trans = lv.empty() && trans;
 		f.rows = rows; f.rpb = L.rpb; f.n = n;
-		if (!root || !rootQ) {  // reflectors are read back by a later Q formation
+		if (wantQ) {
 			L.V.resize((size_t)rows * n);
 			L.tau.resize((size_t)L.blocks * n);
-			f.V = L.V.get(); f.tau = L.tau.get();
+			L.Y.resize((size_t)L.blocks * n * n);
+			f.V = L.V.get(); f.tau = L.tau.get(); f.Y = L.Y.get();
 		}
 		double* R = rootR;
 		if (!root || !rootR) {
@@ -691,7 +724,6 @@ std::vector<Level> tree_factor(const double* A, long long ld, bool trans, int m,
 			R = L.R.get();
 		}
 		f.R = R;
-		if (root && rootQ) { f.Q = rootQ; f.nc = std::min(rows, n); }
 		launch_factor(f, L.blocks, false, s);
 		lv.push_back(std::move(L));
 		if (root) break;
@@ -701,7 +733,8 @@ std::vector<Level> tree_factor(const double* A, long long ld, bool trans, int m,
 	return lv;
 }
 
-// Q (m × nc, ld nc) = (product of the levels [0, count)) · init, init the n × nc start of level count-1.
+// Q (m × nc, ld nc) = (product of the levels [0, count)) · init, init the (blocks·n) × nc start of level
+// count-1 (null: identity, nc = that level's min(rows, n)).
 void tree_apply(std::vector<Level>& lv, int count, int n, const double* init, int nc, const int* ncDev, double* Q,
                 hipStream_t s) {
 	DBuf<double> tmp[2];
@@ -716,9 +749,9 @@ void tree_apply(std::vector<Level>& lv, int count, int n, const double* init, in
 			which ^= 1;
 		}
 		ApplyArgs a{};
-		a.V = L.V.get(); a.tau = L.tau.get(); a.rows = L.rows; a.rpb = L.rpb; a.n = n;
+		a.V = L.V.get(); a.tau = L.tau.get(); a.Y = L.Y.get(); a.rows = L.rows; a.rpb = L.rpb; a.n = n;
 		a.init = cur; a.out = out; a.nc = nc; a.ncDev = ncDev;
-		launch_apply(a, L.blocks, s);
+		launch_wy(a, L.blocks, s);
 		cur = out;
 	}
 }
@@ -805,43 +838,33 @@ void hh_qr(const double* A, size_t m, size_t n, bool trans, double* Q, double* R
 	check_shape(m, n);
 	const int k = (int)std::min(m, n);
 	const long long ld = trans ? (long long)m : (long long)n;
-	if (!Q) {
-		tree_factor(A, ld, trans, (int)m, (int)n, R, nullptr, s);
-		return;
-	}
-	if (m <= (size_t)HBR) {
-		tree_factor(A, ld, trans, (int)m, (int)n, R, Q, s);
-		return;
-	}
-	DBuf<double> rootQ;
-	// the root has (blocks of the level below)·n rows, at most HBR
-	rootQ.resize((size_t)HBR * k);
-	std::vector<Level> lv = tree_factor(A, ld, trans, (int)m, (int)n, R, rootQ.get(), s);
-	tree_apply(lv, (int)lv.size() - 1, (int)n, rootQ.get(), k, nullptr, Q, s);
+	std::vector<Level> lv = tree_factor(A, ld, trans, (int)m, (int)n, R, Q != nullptr, s);
+	if (Q) tree_apply(lv, (int)lv.size(), (int)n, nullptr, k, nullptr, Q, s);
 }
 
 void hh_qc(const double* A, size_t m, size_t n, double* Q, double* C, int* jpvt, int* rank, hipStream_t s) {
 	check_shape(m, n);
+	// The pivoted factorisation runs on one block: A itself, or the stacked R factors that the levels below the
+	// root reduce A to (at most HBR rows). Their column norms are A's, since every level's Q is orthogonal.
+	const double* rootIn = A;
+	int rootRows = (int)m;
+	std::vector<Level> lv;
+	if (m > (size_t)HBR) lv = tree_factor(A, (long long)n, false, (int)m, (int)n, nullptr, true, s, &rootIn, &rootRows);
+	Level root;
+	root.rows = root.rpb = rootRows;
+	root.blocks = 1;
+	root.V.resize((size_t)rootRows * n);
+	root.tau.resize(n);
+	root.Y.resize(n * n);
 	FactorArgs f{};
-	f.n = (int)n; f.jpvt = jpvt; f.rank = rank; f.C = C;
-	if (m <= (size_t)HBR) {
-		f.in = A; f.ld = (long long)n; f.rows = f.rpb = (int)m;
-		f.Q = Q;
-		launch_factor(f, 1, true, s);
-		return;
-	}
-	// The levels below the root reduce A to the stacked R factors of its last level (at most HBR rows); the
-	// pivoted factorisation runs on that stack directly. Its column norms are A's, since every level's Q is
-	// orthogonal.
-	const double* rootIn = nullptr;
-	int rootRows = 0;
-	std::vector<Level> lv = tree_factor(A, (long long)n, false, (int)m, (int)n, nullptr, nullptr, s, &rootIn, &rootRows);
-	DBuf<double> Q2((size_t)rootRows * n);
-	f.in = rootIn; f.ld = (long long)n; f.rows = f.rpb = rootRows;
-	f.orig = A; f.origRows = (int)m;
-	f.Q = Q2.get();
+	f.in = rootIn; f.ld = (long long)n; f.rows = f.rpb = rootRows; f.n = (int)n;
+	f.V = root.V.get(); f.tau = root.tau.get(); f.Y = root.Y.get();
+	f.jpvt = jpvt; f.rank = rank; f.C = C;
+	if (m > (size_t)HBR) { f.orig = A; f.origRows = (int)m; }
 	launch_factor(f, 1, true, s);
-	tree_apply(lv, (int)lv.size(), (int)n, Q2.get(), (int)n, rank, Q, s);
+	lv.push_back(std::move(root));
+	// Q's column count is the rank, read on the device by every level
+	tree_apply(lv, (int)lv.size(), (int)n, nullptr, std::min(rootRows, (int)n), rank, Q, s);
 }
 
 }  // namespace xtt
