@@ -71,7 +71,7 @@ def _qr_checks(Q, R, A, tol=1e-13):
     assert np.abs(Q.T @ Q - np.eye(k)).max() < tol * 10
 
 
-@pytest.mark.parametrize("m,n", [(32, 8), (8, 32), (1024, 64), (64, 1024), (7, 7), (1, 5), (5, 1), (576, 36), (4096, 64)])
+@pytest.mark.parametrize("m,n", [(32, 8), (8, 32), (1024, 64), (64, 1024), (7, 7), (1, 5), (5, 1), (576, 36), (4096, 64), (150, 40)])
 def test_qr_rq(xe, oracle, m, n):
     from xerus_amd._lib import call
     print(f"qr/rq m={m} n={n}", flush=True)
@@ -97,7 +97,8 @@ def test_qr_rq(xe, oracle, m, n):
 
 
 @pytest.mark.parametrize("m,n,r", [(40, 12, 12), (40, 12, 5), (12, 40, 3), (1024, 64, 64), (1024, 64, 20), (64, 64, 1),
-                                   (576, 36, 36), (576, 36, 11), (700, 33, 33), (4096, 64, 30)])
+                                   (576, 36, 36), (576, 36, 11), (700, 33, 33), (4096, 64, 30),
+                                   (150, 40, 40), (170, 64, 20)])
 def test_qc_cq_rank(xe, oracle, m, n, r):
     """Rank decisions must equal the oracle's (xerus's 16·ε·R_00 rule with LAPACK signs)."""
     from xerus_amd._lib import call

@@ -484,7 +484,7 @@ __global__ __launch_bounds__(HT) void hh_wy_kernel(ApplyArgs a) {
 	}
 }
 
-int rows_per_thread(int rpb) { return rpb <= 8 * HW ? 8 : (rpb <= 16 * HW ? 16 : 32); }
+int rows_per_thread(int rpb) { return rpb <= 8 * HW ? 8 : (rpb <= 16 * HW ? 16 : (rpb <= 24 * HW ? 24 : 32)); }
 
 // Householder flops of one level (dgeqrf's count per block: 2n²(m − n/3) for m >= n, 2m²(n − m/3) otherwise).
 double factor_flops(const FactorArgs& a, int blocks) {
@@ -500,9 +500,9 @@ template <bool PIV>
 void factor_variant(const FactorArgs& a, int blocks, hipStream_t s) {
 	const int rpt = rows_per_thread(a.rpb);
 	const dim3 g(blocks), t(HT);
-	static const char* tags[3] = {"hh_factor_kernel<8,0>", "hh_factor_kernel<16,0>", "hh_factor_kernel<32,0>"};
-	static const char* ptags[3] = {"hh_factor_kernel<8,1>", "hh_factor_kernel<16,1>", "hh_factor_kernel<32,1>"};
-	const int ri = rpt == 8 ? 0 : (rpt == 16 ? 1 : 2);
+	static const char* tags[4] = {"hh_factor_kernel<8,0>", "hh_factor_kernel<16,0>", "hh_factor_kernel<24,0>", "hh_factor_kernel<32,0>"};
+	static const char* ptags[4] = {"hh_factor_kernel<8,1>", "hh_factor_kernel<16,1>", "hh_factor_kernel<24,1>", "hh_factor_kernel<32,1>"};
+	const int ri = rpt == 8 ? 0 : (rpt == 16 ? 1 : (rpt == 24 ? 2 : 3));
 	std::string tag = (PIV ? ptags : tags)[ri];
 	if (prof_shapes()) tag += " rows=" + std::to_string(a.rows) + " n=" + std::to_string(a.n) + " blocks=" + std::to_string(blocks);
 	Work work;
@@ -515,6 +515,7 @@ void factor_variant(const FactorArgs& a, int blocks, hipStream_t s) {
 	}
 	if (rpt == 8) launch_timed(tag.c_str(), work, hh_factor_kernel<8, PIV>, g, t, 0, s, a);
 	else if (rpt == 16) launch_timed(tag.c_str(), work, hh_factor_kernel<16, PIV>, g, t, 0, s, a);
+	else if (rpt == 24) launch_timed(tag.c_str(), work, hh_factor_kernel<24, PIV>, g, t, 0, s, a);
 	else launch_timed(tag.c_str(), work, hh_factor_kernel<32, PIV>, g, t, 0, s, a);
 }
 
@@ -657,7 +658,7 @@ void applyT_rpt(const ApplyTArgs& a, int blocks, hipStream_t s) {
 			const double m = std::min(a.rpb, a.rows - b * a.rpb), k = std::min<double>(m, a.n);
 			f += 4.0 * a.nc * (m * k - k * k / 2.0);
 		}
-	std::string tag = RPT == 8 ? "hh_applyT_kernel<8>" : (RPT == 16 ? "hh_applyT_kernel<16>" : "hh_applyT_kernel<32>");
+	std::string tag = "hh_applyT_kernel<" + std::to_string(RPT) + ">";
 	if (prof_shapes()) tag += " rows=" + std::to_string(a.rows) + " n=" + std::to_string(a.n) + " nc=" + std::to_string(a.nc);
 	const Work work(f, 8.0 * ((double)a.rows * a.n + (double)blocks * a.n + 2.0 * a.rows * a.nc));
 	launch_timed(tag.c_str(), work, hh_applyT_kernel<RPT>, dim3(blocks, (a.nc + HN - 1) / HN), dim3(HT), lds, s, a);
@@ -667,6 +668,7 @@ void launch_applyT(const ApplyTArgs& a, int blocks, hipStream_t s) {
 	const int rpt = rows_per_thread(a.rpb);
 	if (rpt == 8) applyT_rpt<8>(a, blocks, s);
 	else if (rpt == 16) applyT_rpt<16>(a, blocks, s);
+	else if (rpt == 24) applyT_rpt<24>(a, blocks, s);
 	else applyT_rpt<32>(a, blocks, s);
 }
 
