@@ -1,5 +1,7 @@
 // extern "C" surface of libxtt (declared in include/xtt.h). Exceptions become status codes here.
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <memory>
 
 #include "../../include/xtt.h"
@@ -494,6 +496,29 @@ struct RawDesc {
 		for (int k = 0; k < d; ++k) { desc.sumN += hn[k]; desc.maxN = std::max(desc.maxN, hn[k]); }
 	}
 };
+
+// Uploaded descriptors by (d, n, r, P): a repeated call (the per-iteration gradients of a solver) reuses the
+// device copy instead of uploading and synchronising again. Bounded; entries are immutable once built and held
+// by each call that uses them, so clearing the cache never frees a descriptor in use.
+std::shared_ptr<const RawDesc> raw_desc(int d, const int* hn, const int* hr, int P, hipStream_t s) {
+	XTT_REQUIRE(d > 0 && P >= 0, "invalid order / count");
+	std::vector<int> key;
+	key.reserve(2 * d + 3);
+	key.push_back(d);
+	key.push_back(P);
+	key.insert(key.end(), hn, hn + d);
+	key.insert(key.end(), hr, hr + d + 1);
+	static std::mutex mu;
+	static std::map<std::vector<int>, std::shared_ptr<const RawDesc>> cache;
+	std::lock_guard<std::mutex> lock(mu);
+	auto it = cache.find(key);
+	if (it == cache.end()) {
+		auto rd = std::make_shared<const RawDesc>(d, hn, hr, P, s);
+		if (cache.size() >= 64) cache.clear();
+		it = cache.emplace(std::move(key), std::move(rd)).first;
+	}
+	return it->second;
+}
 }  // namespace
 
 extern "C" {
@@ -502,12 +527,13 @@ int xt_tt_sample_f32(int d, const int* n, const int* r, const float* cores, cons
 	return guard([&] {
 		NN(n); NN(r); NN(cores); NN(values); XTT_REQUIRE(P == 0 || pos, "pos must not be NULL");
 		hipStream_t s = xtt::pick(stream);
-		RawDesc rd(d, n, r, P, s);
+		const auto held = raw_desc(d, n, r, P, s);
+		const xtt::AdfDesc& desc = held->desc;
 		if (P == 0) return;
 		xtt::DBuf<double> norms(2), partials(2 * 1024);
 		xtt::DBuf<unsigned> counter(1);
 		XTT_HIP(hipMemsetAsync(counter.get(), 0, 4, s));
-		xtt::adf_residual<float>(rd.desc, cores, pos, nullptr, values, norms.get(), partials.get(), counter.get(), s);
+		xtt::adf_residual<float>(desc, cores, pos, nullptr, values, norms.get(), partials.get(), counter.get(), s);
 		XTT_HIP(hipStreamSynchronize(s));
 	});
 }
@@ -516,14 +542,15 @@ int xt_tt_sample_residual_f32(int d, const int* n, const int* r, const float* co
 	return guard([&] {
 		NN(n); NN(r); NN(cores); NN(resid); XTT_REQUIRE(P == 0 || (pos && b), "pos/b must not be NULL");
 		hipStream_t s = xtt::pick(stream);
-		RawDesc rd(d, n, r, P, s);
+		const auto held = raw_desc(d, n, r, P, s);
+		const xtt::AdfDesc& desc = held->desc;
 		if (P == 0) { if (sums) sums[0] = sums[1] = 0; return; }
 		xtt::DBuf<double> norms(2), partials(2 * 1024);
 		xtt::DBuf<unsigned> counter(1);
 		XTT_HIP(hipMemsetAsync(counter.get(), 0, 4, s));
-		xtt::adf_residual<float>(rd.desc, cores, pos, b, resid, norms.get(), partials.get(), counter.get(), s);
-		if (sums) XTT_HIP(hipMemcpyAsync(sums, norms.get(), 16, hipMemcpyDeviceToHost, s));
-		XTT_HIP(hipStreamSynchronize(s));
+		xtt::adf_residual<float>(desc, cores, pos, b, resid, norms.get(), partials.get(), counter.get(), s);
+		if (sums) xtt::read_back(sums, norms.get(), 16, s);
+		else XTT_HIP(hipStreamSynchronize(s));
 	});
 }
 int xt_tt_sample_adjoint_f32(int d, const int* n, const int* r, const float* cores, const uint32_t* pos, int P, const float* seed,
@@ -531,10 +558,11 @@ int xt_tt_sample_adjoint_f32(int d, const int* n, const int* r, const float* cor
 	return guard([&] {
 		NN(n); NN(r); NN(cores); NN(grads); XTT_REQUIRE(P == 0 || (pos && seed), "pos/seed must not be NULL");
 		hipStream_t s = xtt::pick(stream);
-		RawDesc rd(d, n, r, P, s);
-		if (P == 0) { XTT_HIP(hipMemsetAsync(grads, 0, (size_t)rd.desc.gradSize * 4, s)); XTT_HIP(hipStreamSynchronize(s)); return; }
-		xtt::DBuf<float> partials((size_t)xtt::adf_grad_blocks(P) * rd.desc.gradSize);
-		xtt::adf_gradient<float>(rd.desc, cores, pos, seed, grads, partials.get(), -1, s);
+		const auto held = raw_desc(d, n, r, P, s);
+		const xtt::AdfDesc& desc = held->desc;
+		if (P == 0) { XTT_HIP(hipMemsetAsync(grads, 0, (size_t)desc.gradSize * 4, s)); XTT_HIP(hipStreamSynchronize(s)); return; }
+		xtt::DBuf<float> partials((size_t)xtt::adf_grad_blocks(P) * desc.gradSize);
+		xtt::adf_gradient<float>(desc, cores, pos, seed, grads, partials.get(), -1, s);
 		XTT_HIP(hipStreamSynchronize(s));
 	});
 }

@@ -62,11 +62,13 @@ std::mutex g_pinMu;
 std::vector<void*> g_pinFree;
 }  // namespace
 
-void read_back(void* host, const void* dev, size_t bytes) {
+void read_back(void* host, const void* dev, size_t bytes) { read_back(host, dev, bytes, stream()); }
+
+void read_back(void* host, const void* dev, size_t bytes, hipStream_t s) {
 	if (bytes == 0) return;
 	if (bytes > kPinnedBytes) {
-		XTT_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream()));
-		sync();
+		XTT_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s));
+		XTT_HIP(hipStreamSynchronize(s));
 		return;
 	}
 	void* buf = nullptr;
@@ -78,8 +80,8 @@ void read_back(void* host, const void* dev, size_t bytes) {
 		}
 	}
 	if (!buf) XTT_HIP(hipHostMalloc(&buf, kPinnedBytes, hipHostMallocDefault));
-	hipError_t e = hipMemcpyAsync(buf, dev, bytes, hipMemcpyDeviceToHost, stream());
-	if (e == hipSuccess) e = hipStreamSynchronize(stream());
+	hipError_t e = hipMemcpyAsync(buf, dev, bytes, hipMemcpyDeviceToHost, s);
+	if (e == hipSuccess) e = hipStreamSynchronize(s);
 	if (e == hipSuccess) std::memcpy(host, buf, bytes);
 	{
 		std::lock_guard<std::mutex> lk(g_pinMu);

@@ -39,9 +39,11 @@ struct Error : std::runtime_error {
 
 void set_error(const std::string& s);
 
-// Device → host copy of a small result (ranks, norms, solver state) on the calling thread's stream, waiting for
-// it: staged through pinned host memory, so the copy is one DMA without the runtime's pageable-memory staging.
+// Device → host copy of a small result (ranks, norms, solver state) on the calling thread's stream (or on s),
+// waiting for it: staged through pinned host memory, so the copy is one DMA without the runtime's pageable-memory
+// staging.
 void read_back(void* host, const void* dev, size_t bytes);
+void read_back(void* host, const void* dev, size_t bytes, hipStream_t s);
 
 // Optional launch profiler: while enabled, launches made through launch_timed hand a HIP event pair to the
 // dispatch itself (hipExtLaunchKernel), so the pair brackets the kernel's execution and not the host's
