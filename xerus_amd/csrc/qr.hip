@@ -324,92 +324,88 @@ __device__ inline wy_d4 mfma_f64(double a, double b, wy_d4 c) { return __builtin
 //            (only row w enters);
 //   phase 3: out = [init_b; 0] − U·Z, 16×16 MFMA tiles, U read per lane with its triangle masked.
 // τ_i = 0 (an identity reflector) zeroes row and column i of T.
-__global__ __launch_bounds__(HT) void hh_wy_kernel(ApplyArgs a) {
-	__shared__ double sU[HN][HN + 1];   // U's top triangle, zero outside k×k (odd stride: column reads spread over banks)
-	__shared__ double sY[HN][HN];
-	__shared__ double sI[HN][HN];       // init_b, zero outside k × nc
-	__shared__ double sP[HN + HW][HN];  // P, then Z; zero outside k × nc
-	__shared__ double sC[HW][HN];       // the chunk's right-hand sides
-	__shared__ double sT[HN][HW];       // the 8×8 diagonal blocks of T: row i holds T[i][8·(i/8) + j]
-	__shared__ double sTau[HN];
+struct WyLds {
+	double U[HN][HN + 1];   // U's top triangle, zero outside k×k (odd stride: column reads spread over banks)
+	double Y[HN][HN];
+	double I[HN][HN];       // init_b, zero outside k × nc
+	double P[HN + HW][HN];  // P, then Z; zero outside k × nc
+	double C[HW][HN];       // the chunk's right-hand sides
+	double T[HN][HW];       // the 8×8 diagonal blocks of T: row i holds T[i][8·(i/8) + j]
+	double tau[HN];
+};
+
+// Staging of one block (rows from U, ld n; k reflectors): U's triangle, Y, τ, P = 0 and, unless keepI, I from init
+// (k × nc, ld nc; null: zero). All loads of a thread are in flight together.
+__device__ __forceinline__ void wy_stage(WyLds& L, const double* U, const double* Yg, const double* tau, int n, int k, int nc,
+                         const double* init, bool keepI) {
+	constexpr int E = HN * HN / HT;
+	const int tid = threadIdx.x;
+	double u[E], y[E], z[E];
+#pragma unroll
+	for (int q = 0; q < E; ++q) {
+		const int e = q * HT + tid, r = e / HN, c = e % HN;
+		u[q] = (r < k && c < r) ? U[(size_t)r * n + c] : 0.0;
+		y[q] = (r < c && c < k) ? Yg[(size_t)r * n + c] : 0.0;
+		z[q] = (!keepI && init && r < k && c < nc) ? init[(size_t)r * nc + c] : 0.0;
+	}
+#pragma unroll
+	for (int q = 0; q < E; ++q) {
+		const int e = q * HT + tid, r = e / HN, c = e % HN;
+		L.U[r][c] = (c == r && r < k) ? 1.0 : u[q];
+		L.Y[r][c] = y[q];
+		if (!keepI) L.I[r][c] = z[q];
+		L.P[r][c] = 0.0;
+	}
+	if (tid < HW * HN) L.P[HN + tid / HN][tid % HN] = 0.0;
+	if (tid < k) L.tau[tid] = tau[tid];
+}
+
+// Z = T·U_topᵀ·I into L.P (ident: I = the identity's first nc columns); after wy_stage and a barrier.
+__device__ __forceinline__ void wy_solve(WyLds& L, int k, int nc, bool ident) {
 	const int lane = threadIdx.x & 63;
 	const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const int li = lane & 15, lk = lane >> 4;
-	const int b = blockIdx.x;
-	const int r0 = b * a.rpb;
-	const int rows = min(a.rpb, a.rows - r0);
-	const int n = a.n;
-	const int k = min(rows, n);
-	const int nc = a.ncDev ? __builtin_amdgcn_readfirstlane(*a.ncDev) : a.nc;
-	const double* U = a.V + (size_t)r0 * n;
-	const double* Yg = a.Y + (size_t)b * n * n;
-	const bool ident = a.init == nullptr;
-	{  // staging: all loads of a thread in flight together
-		constexpr int E = HN * HN / HT;
-		double u[E], y[E], z[E];
-#pragma unroll
-		for (int q = 0; q < E; ++q) {
-			const int e = q * HT + threadIdx.x, r = e / HN, c = e % HN;
-			u[q] = (r < k && c < r) ? U[(size_t)r * n + c] : 0.0;
-			y[q] = (r < c && c < k) ? Yg[(size_t)r * n + c] : 0.0;
-			z[q] = (!ident && r < k && c < nc) ? a.init[((size_t)b * n + r) * nc + c] : 0.0;
-		}
-#pragma unroll
-		for (int q = 0; q < E; ++q) {
-			const int e = q * HT + threadIdx.x, r = e / HN, c = e % HN;
-			sU[r][c] = (c == r && r < k) ? 1.0 : u[q];
-			sY[r][c] = y[q];
-			sI[r][c] = z[q];
-			sP[r][c] = 0.0;
-		}
-		if (threadIdx.x < HW * HN) sP[HN + threadIdx.x / HN][threadIdx.x % HN] = 0.0;
-		if (threadIdx.x < k) sTau[threadIdx.x] = a.tau[(size_t)b * n + threadIdx.x];
-	}
-	__syncthreads();
-	const int kq = (k + 3) >> 2;        // MFMA k-steps over the k reflectors
-	const int cT = (nc + 15) >> 4;      // 16-column tiles of the product
+	const int kq = (k + 3) >> 2;
+	const int cT = (nc + 15) >> 4;
 	if (w == HW - 1) {
 		// T's diagonal blocks, lane i = 8C + w' forms row w' of block C:
 		// T[i][c0 + j] = −τ_{c0+j}·Σ_{l=w'}^{j-1} T[i][c0 + l]·Y[c0 + l][c0 + j], T[i][i] = τ_i
 		const int c0 = lane & ~(HW - 1), wr = lane & (HW - 1);
-		double yb[HW][HW];
+		double yb[HW * (HW - 1) / 2];  // Y[c0 + l][c0 + j], l < j, at j(j-1)/2 + l
 #pragma unroll
-		for (int l = 0; l < HW; ++l)
+		for (int j = 1; j < HW; ++j)
 #pragma unroll
-			for (int j = l + 1; j < HW; ++j) yb[l][j] = sY[c0 + l][c0 + j];
+			for (int l = 0; l < j; ++l) yb[j * (j - 1) / 2 + l] = L.Y[c0 + l][c0 + j];
 		double t[HW];
 #pragma unroll
 		for (int j = 0; j < HW; ++j) {
 			double acc = 0.0;
 #pragma unroll
-			for (int l = 0; l < j; ++l) acc = fma(t[l], yb[l][j], acc);  // t[l] = 0 for l < wr
-			const double tau = c0 + j < k ? sTau[c0 + j] : 0.0;
+			for (int l = 0; l < j; ++l) acc = fma(t[l], yb[j * (j - 1) / 2 + l], acc);  // t[l] = 0 for l < wr
+			const double tau = c0 + j < k ? L.tau[c0 + j] : 0.0;
 			t[j] = j < wr ? 0.0 : (j == wr ? tau : -tau * acc);
 		}
 #pragma unroll
-		for (int j = 0; j < HW; ++j) sT[lane][j] = t[j];
-	} else {
-		// phase 1: P = U_topᵀ·init_b (waves 0..6)
-		if (ident) {
-			for (int e = threadIdx.x; e < HN * HN; e += (HW - 1) * 64) {
-				const int i = e >> 6, c = e & 63;
-				if (i < k && c < nc) sP[i][c] = sU[c][i];
-			}
-		} else {
-			const int kT = (k + 15) >> 4;
-			for (int tile = w; tile < kT * cT; tile += HW - 1) {
-				const int ti = tile / cT, tc = tile - ti * cT;
-				wy_d4 acc = {0.0, 0.0, 0.0, 0.0};
+		for (int j = 0; j < HW; ++j) L.T[lane][j] = t[j];
+	} else if (ident) {  // phase 1: P = U_topᵀ restricted to the first nc columns (waves 0..6)
+		for (int e = threadIdx.x; e < HN * HN; e += (HW - 1) * 64) {
+			const int i = e >> 6, c = e & 63;
+			if (i < k && c < nc) L.P[i][c] = L.U[c][i];
+		}
+	} else {  // phase 1: P = U_topᵀ·I (waves 0..6)
+		const int kT = (k + 15) >> 4;
+		for (int tile = w; tile < kT * cT; tile += HW - 1) {
+			const int ti = tile / cT, tc = tile - ti * cT;
+			wy_d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-				for (int q = 0; q < HN / 4; ++q) {
-					if (q < kq) {
-						const int r = 4 * q + lk;
-						acc = mfma_f64(sU[r][ti * 16 + li], sI[r][tc * 16 + li], acc);
-					}
+			for (int q = 0; q < HN / 4; ++q) {
+				if (q < kq) {
+					const int r = 4 * q + lk;
+					acc = mfma_f64(L.U[r][ti * 16 + li], L.I[r][tc * 16 + li], acc);
 				}
-#pragma unroll
-				for (int v = 0; v < 4; ++v) sP[ti * 16 + 4 * v + lk][tc * 16 + li] = acc[v];
 			}
+#pragma unroll
+			for (int v = 0; v < 4; ++v) L.P[ti * 16 + 4 * v + lk][tc * 16 + li] = acc[v];
 		}
 	}
 	__syncthreads();
@@ -417,28 +413,39 @@ __global__ __launch_bounds__(HT) void hh_wy_kernel(ApplyArgs a) {
 	for (int c0 = ((k - 1) / HW) * HW; c0 >= 0; c0 -= HW) {
 		const int i = c0 + w;
 		if (i < k) {
-			double a0 = sP[i][lane], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+			double a0 = L.P[i][lane], a1 = 0.0, a2 = 0.0, a3 = 0.0;
 			int l = c0 + HW;
 			for (; l + 3 < k; l += 4) {
-				a0 = fma(-sY[i][l], sP[l][lane], a0);
-				a1 = fma(-sY[i][l + 1], sP[l + 1][lane], a1);
-				a2 = fma(-sY[i][l + 2], sP[l + 2][lane], a2);
-				a3 = fma(-sY[i][l + 3], sP[l + 3][lane], a3);
+				a0 = fma(-L.Y[i][l], L.P[l][lane], a0);
+				a1 = fma(-L.Y[i][l + 1], L.P[l + 1][lane], a1);
+				a2 = fma(-L.Y[i][l + 2], L.P[l + 2][lane], a2);
+				a3 = fma(-L.Y[i][l + 3], L.P[l + 3][lane], a3);
 			}
-			for (; l < k; ++l) a0 = fma(-sY[i][l], sP[l][lane], a0);
-			sC[w][lane] = (a0 + a1) + (a2 + a3);
+			for (; l < k; ++l) a0 = fma(-L.Y[i][l], L.P[l][lane], a0);
+			L.C[w][lane] = (a0 + a1) + (a2 + a3);
 		}
 		__syncthreads();
 		if (i < k) {
 			double z = 0.0;
 #pragma unroll
 			for (int j = 0; j < HW; ++j)
-				if (c0 + j < k) z = fma(sT[i][j], sC[j][lane], z);  // T[i][c0 + j] = 0 for j < w
-			sP[i][lane] = z;
+				if (c0 + j < k) z = fma(L.T[i][j], L.C[j][lane], z);  // T[i][c0 + j] = 0 for j < w
+			L.P[i][lane] = z;
 		}
 		__syncthreads();
 	}
-	// phase 3: out = [init_b; 0] − U·Z, two tiles per pass for independent MFMA chains
+}
+
+// out (rows × nc, ld nc) = E − U_rows·Z: U's rows g0 .. g0 + rows - 1 (ld n, k reflectors: entry (g, i) is a
+// reflector entry for g > i, one for g == i, zero above), E[r][c] = (g0 + r == c) when ident, else I[r][c] for the
+// rows r < k. 16×16 MFMA tiles, two per pass for independent MFMA chains; each lane loads its own U entries.
+__device__ __forceinline__ void wy_product(const WyLds& L, const double* U, int n, int g0, int rows, int k, int nc, bool ident, double* out,
+                           bool toLds, double (*lds)[HN]) {
+	const int lane = threadIdx.x & 63;
+	const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int li = lane & 15, lk = lane >> 4;
+	const int kq = (k + 3) >> 2;
+	const int cT = (nc + 15) >> 4;
 	const int rT = (rows + 15) >> 4;
 	const int nt = rT * cT;
 	for (int t0 = w; t0 < nt; t0 += 2 * HW) {
@@ -446,10 +453,10 @@ __global__ __launch_bounds__(HT) void hh_wy_kernel(ApplyArgs a) {
 		const bool two = t1 < nt;
 		const int tr0 = t0 / cT, tc0 = t0 - tr0 * cT;
 		const int tr1 = two ? t1 / cT : tr0, tc1 = two ? t1 - tr1 * cT : tc0;
-		const int ra0 = tr0 * 16 + li, ra1 = tr1 * 16 + li;
-		const double* u0 = U + (size_t)min(ra0, rows - 1) * n;
-		const double* u1 = U + (size_t)min(ra1, rows - 1) * n;
-		// all k-steps of both tiles' U entries are loaded before the MFMA chains (kq <= HN/4)
+		const int ra0 = tr0 * 16 + li, ra1 = tr1 * 16 + li;  // rows of U relative to g0
+		const int ga0 = g0 + ra0, ga1 = g0 + ra1;
+		const double* u0 = U + (size_t)(g0 + min(ra0, rows - 1)) * n;
+		const double* u1 = U + (size_t)(g0 + min(ra1, rows - 1)) * n;
 		double x0[HN / 4], x1[HN / 4];
 #pragma unroll
 		for (int q = 0; q < HN / 4; ++q) {
@@ -462,26 +469,69 @@ __global__ __launch_bounds__(HT) void hh_wy_kernel(ApplyArgs a) {
 		for (int q = 0; q < HN / 4; ++q) {
 			if (q < kq) {
 				const int i = 4 * q + lk;
-				const double a0 = (ra0 < rows && i < k) ? (i < ra0 ? x0[q] : (i == ra0 ? 1.0 : 0.0)) : 0.0;
-				const double a1 = (ra1 < rows && i < k) ? (i < ra1 ? x1[q] : (i == ra1 ? 1.0 : 0.0)) : 0.0;
-				acc0 = mfma_f64(a0, sP[i][tc0 * 16 + li], acc0);
-				acc1 = mfma_f64(a1, sP[i][tc1 * 16 + li], acc1);
+				const double a0 = (ra0 < rows && i < k) ? (i < ga0 ? x0[q] : (i == ga0 ? 1.0 : 0.0)) : 0.0;
+				const double a1 = (ra1 < rows && i < k) ? (i < ga1 ? x1[q] : (i == ga1 ? 1.0 : 0.0)) : 0.0;
+				acc0 = mfma_f64(a0, L.P[i][tc0 * 16 + li], acc0);
+				acc1 = mfma_f64(a1, L.P[i][tc1 * 16 + li], acc1);
 			}
 		}
 #pragma unroll
 		for (int v = 0; v < 4; ++v) {
-			const int c0 = tc0 * 16 + li, row0 = tr0 * 16 + 4 * v + lk;
-			if (row0 < rows && c0 < nc) {
-				const double base = row0 < k ? (ident ? (row0 == c0 ? 1.0 : 0.0) : sI[row0][c0]) : 0.0;
-				a.out[(size_t)(r0 + row0) * nc + c0] = base - acc0[v];
-			}
-			const int c1 = tc1 * 16 + li, row1 = tr1 * 16 + 4 * v + lk;
-			if (two && row1 < rows && c1 < nc) {
-				const double base = row1 < k ? (ident ? (row1 == c1 ? 1.0 : 0.0) : sI[row1][c1]) : 0.0;
-				a.out[(size_t)(r0 + row1) * nc + c1] = base - acc1[v];
+#pragma unroll
+			for (int h = 0; h < 2; ++h) {
+				if (h == 1 && !two) continue;
+				const int c = (h ? tc1 : tc0) * 16 + li, r = (h ? tr1 : tr0) * 16 + 4 * v + lk;
+				if (r < rows && c < nc) {
+					const int g = g0 + r;
+					const double base = ident ? (g == c ? 1.0 : 0.0) : (r < k ? L.I[r][c] : 0.0);
+					const double val = base - (h ? acc1[v] : acc0[v]);
+					if (toLds) lds[r][c] = val;
+					else out[(size_t)r * nc + c] = val;
+				}
 			}
 		}
 	}
+}
+
+__global__ __launch_bounds__(HT) void hh_wy_kernel(ApplyArgs a) {
+	__shared__ WyLds L;
+	const int b = blockIdx.x;
+	const int r0 = b * a.rpb;
+	const int rows = min(a.rpb, a.rows - r0);
+	const int n = a.n;
+	const int k = min(rows, n);
+	const int nc = a.ncDev ? __builtin_amdgcn_readfirstlane(*a.ncDev) : a.nc;
+	const double* U = a.V + (size_t)r0 * n;
+	const bool ident = a.init == nullptr;
+	wy_stage(L, U, a.Y + (size_t)b * n * n, a.tau + (size_t)b * n, n, k, nc,
+	         ident ? nullptr : a.init + (size_t)b * n * nc, false);
+	__syncthreads();
+	wy_solve(L, k, nc, ident);
+	wy_product(L, U, n, 0, rows, k, nc, ident, a.out + (size_t)r0 * nc, false, nullptr);
+}
+
+// Two tree levels in one launch, the root started from the identity: leaf block b first forms rows
+// [b·n, b·n + n) of the root's Q itself (every leaf block repeats the root's staging and triangular solve), then
+// applies its own reflectors to them. Saves the root's launch and its round trip through global memory.
+__global__ __launch_bounds__(HT) void hh_wy2_kernel(ApplyArgs a, ApplyArgs root) {
+	__shared__ WyLds L;
+	const int b = blockIdx.x;
+	const int r0 = b * a.rpb;
+	const int rows = min(a.rpb, a.rows - r0);
+	const int n = a.n;
+	const int k = min(rows, n);
+	const int kr = min(root.rows, n);
+	const int nc = a.ncDev ? __builtin_amdgcn_readfirstlane(*a.ncDev) : a.nc;
+	wy_stage(L, root.V, root.Y, root.tau, n, kr, nc, nullptr, false);
+	__syncthreads();
+	wy_solve(L, kr, nc, true);
+	wy_product(L, root.V, n, b * n, k, kr, nc, true, nullptr, true, L.I);
+	__syncthreads();
+	const double* U = a.V + (size_t)r0 * n;
+	wy_stage(L, U, a.Y + (size_t)b * n * n, a.tau + (size_t)b * n, n, k, nc, nullptr, true);
+	__syncthreads();
+	wy_solve(L, k, nc, false);
+	wy_product(L, U, n, 0, rows, k, nc, false, a.out + (size_t)r0 * nc, false, nullptr);
 }
 
 int rows_per_thread(int rpb) { return rpb <= 8 * HW ? 8 : (rpb <= 16 * HW ? 16 : (rpb <= 24 * HW ? 24 : 32)); }
@@ -540,6 +590,26 @@ void launch_wy(const ApplyArgs& a, int blocks, hipStream_t s) {
 	const Work work(f, 8.0 * ((double)a.rows * a.n + (double)blocks * (a.n + kk * (kk - 1) / 2.0) +
 	                          (a.init ? (double)blocks * a.n * a.nc : 0.0) + (double)a.rows * a.nc));
 	launch_timed(tag.c_str(), work, hh_wy_kernel, dim3(blocks), dim3(HT), 0, s, a);
+	XTT_LAUNCH_CHECK();
+}
+
+// A leaf level (blocks blocks) under a single-block root, the root from the identity, in one launch.
+void launch_wy2(const ApplyArgs& a, int blocks, const ApplyArgs& root, hipStream_t s) {
+	double f = 0;
+	if (prof_enabled()) {
+		const double kr = std::min(root.rows, a.n);
+		for (int b = 0; b < blocks; ++b) {
+			const double m = std::min(a.rpb, a.rows - b * a.rpb), k = std::min<double>(m, a.n);
+			f += 2.0 * a.nc * (m * k + k * k) + 2.0 * a.nc * (k * kr + kr * kr);
+		}
+	}
+	std::string tag = "hh_wy2_kernel";
+	if (prof_shapes()) tag += " rows=" + std::to_string(a.rows) + " n=" + std::to_string(a.n) + " nc=" + std::to_string(a.nc);
+	// both levels' reflectors, τ and Y in; the leaf level's Q rows out
+	const double kk = std::min(a.rpb, a.n);
+	const Work work(f, 8.0 * ((double)a.rows * a.n + (double)root.rows * a.n + (double)(blocks + 1) * (a.n + kk * (kk - 1) / 2.0) +
+	                          (double)a.rows * a.nc));
+	launch_timed(tag.c_str(), work, hh_wy2_kernel, dim3(blocks), dim3(HT), 0, s, a, root);
 	XTT_LAUNCH_CHECK();
 }
 
@@ -739,6 +809,16 @@ std::vector<Level> tree_factor(const double* A, long long ld, bool trans, int m,
 // count-1 (null: identity, nc = that level's min(rows, n)).
 void tree_apply(std::vector<Level>& lv, int count, int n, const double* init, int nc, const int* ncDev, double* Q,
                 hipStream_t s) {
+	auto args = [&](Level& L, const double* in, double* out) {
+		ApplyArgs a{};
+		a.V = L.V.get(); a.tau = L.tau.get(); a.Y = L.Y.get(); a.rows = L.rows; a.rpb = L.rpb; a.n = n;
+		a.init = in; a.out = out; a.nc = nc; a.ncDev = ncDev;
+		return a;
+	};
+	if (count == 2 && !init && lv[1].blocks == 1) {  // leaves under the root: one launch
+		launch_wy2(args(lv[0], nullptr, Q), lv[0].blocks, args(lv[1], nullptr, nullptr), s);
+		return;
+	}
 	DBuf<double> tmp[2];
 	int which = 0;
 	const double* cur = init;
@@ -750,10 +830,7 @@ void tree_apply(std::vector<Level>& lv, int count, int n, const double* init, in
 			out = tmp[which].get();
 			which ^= 1;
 		}
-		ApplyArgs a{};
-		a.V = L.V.get(); a.tau = L.tau.get(); a.Y = L.Y.get(); a.rows = L.rows; a.rpb = L.rpb; a.n = n;
-		a.init = cur; a.out = out; a.nc = nc; a.ncDev = ncDev;
-		launch_wy(a, L.blocks, s);
+		launch_wy(args(L, cur, out), L.blocks, s);
 		cur = out;
 	}
 }
