@@ -179,6 +179,43 @@ def test_als_mixed_precision_config3(xe, oracle, prec, tol):
     assert err < tol
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_als_speculative_calls_match_normal(xe, oracle, prec, monkeypatch):
+    """An ALS call after one that needed no inner CG iteration (same shapes) first runs with every local solve taken
+    as converged and its residual tests on the device; if any test fails, x is restored and the call reruns. Calls
+    with b (speculation holds) and then with a changed b (speculation fails, rerun) must give exactly the result,
+    energies and statistics of the same calls with speculation off."""
+    print(f"als speculation {prec}", flush=True)
+    A, Ao, b, bo, x, xo = config1(xe, oracle)
+    als = xe.ALS_SPD.copy()
+    als.convergenceEpsilon = 0.0
+    als.localTolerance = 1e-9
+    als.precision = xe.XT_F64 if prec == "f64" else xe.XT_F32
+    for _ in range(40):  # converge until a call needs no inner iteration
+        als(A, x, b, 2)
+        if als.last_stats["cgIterations"] == 0:
+            break
+    assert als.last_stats["cgIterations"] == 0
+    b2 = b * 1.001
+
+    def sequence():
+        y = x.copy()
+        out = []
+        for rhs in (b, b, b2, b2):
+            e = als(A, y, rhs, 2)
+            out.append((e, y.full(), y.ranks(), y.corePosition, dict(als.last_stats)))
+        return out
+
+    monkeypatch.setenv("XTT_NO_SPECULATION", "1")
+    ref = sequence()
+    monkeypatch.delenv("XTT_NO_SPECULATION")
+    got = sequence()
+    assert ref[2][4]["cgIterations"] > 0  # the changed b needed iterations: the third call's speculation failed
+    for (e1, f1, r1, c1, s1), (e2, f2, r2, c2, s2) in zip(ref, got):
+        assert e1 == e2 and r1 == r2 and c1 == c2 and s1 == s2
+        assert np.array_equal(f1, f2)
+
+
 def test_als_identity(xe, oracle):
     """ALS:identity (als.cxx:28-67): with A = I the solution is b."""
     print("als identity", flush=True)

@@ -38,9 +38,10 @@ void scale_by_f64(double* m, const double* sv, size_t rows, size_t cols, bool by
 // one zero-initialised unsigned that the kernel returns to zero.
 size_t reduce_blocks(size_t n);
 void dot_f64(const double* x, const double* y, size_t n, double* out, double* partials, unsigned* counter, hipStream_t s);
-// r = b − q; out2[0] = ‖r‖², out2[1] = ‖b‖² (same partials/counter contract as dot_f64)
+// r = b − q; out2[0] = ‖r‖², out2[1] = ‖b‖² (same partials/counter contract as dot_f64). With fail set, the
+// last block also adds one to *fail unless ‖r‖² <= tol2·‖b‖² (a convergence test the host reads later).
 void residual_norms_f64(double* r, const double* b, const double* q, size_t n, double* out2, double* partials, unsigned* counter,
-                        hipStream_t s);
+                        hipStream_t s, double tol2 = 0.0, unsigned* fail = nullptr);
 void dot_f32_acc64(const float* x, const float* y, size_t n, double* out, double* partials, unsigned* counter, hipStream_t s);
 
 // ---- local CG on the device; all scalars live in CGState (device) -------------------------------------

@@ -125,7 +125,7 @@ __global__ __launch_bounds__(RB) void dot_kernel(const T* x, const T* y, size_t 
 
 // r = b − q with out[0] = ‖r‖², out[1] = ‖b‖² (one pass, deterministic)
 __global__ __launch_bounds__(RB) void residual_norms_kernel(double* r, const double* b, const double* q, size_t n, double* out,
-                                                            double* partials, unsigned* counter) {
+                                                            double* partials, unsigned* counter, double tol2, unsigned* fail) {
 	__shared__ double ws[RB / 64 * 2];
 	double rr = 0, bb = 0;
 	for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
@@ -137,6 +137,7 @@ __global__ __launch_bounds__(RB) void residual_norms_kernel(double* r, const dou
 	if (last_block_sum2(rr, bb, partials, counter, ws) && threadIdx.x == 0) {
 		out[0] = rr;
 		out[1] = bb;
+		if (fail && !(rr <= tol2 * bb)) atomicAdd(fail, 1u);  // NaN counts as a failure
 	}
 }
 
@@ -280,8 +281,9 @@ void dot_f64(const double* x, const double* y, size_t n, double* out, double* pa
 	XTT_LAUNCH_CHECK();
 }
 void residual_norms_f64(double* r, const double* b, const double* q, size_t n, double* out2, double* partials, unsigned* counter,
-                        hipStream_t s) {
-	hipLaunchKernelGGL(residual_norms_kernel, dim3((unsigned)reduce_blocks(n)), dim3(RB), 0, s, r, b, q, n, out2, partials, counter);
+                        hipStream_t s, double tol2, unsigned* fail) {
+	hipLaunchKernelGGL(residual_norms_kernel, dim3((unsigned)reduce_blocks(n)), dim3(RB), 0, s, r, b, q, n, out2, partials, counter,
+	                   tol2, fail);
 	XTT_LAUNCH_CHECK();
 }
 void dot_f32_acc64(const float* x, const float* y, size_t n, double* out, double* partials, unsigned* counter, hipStream_t s) {
