@@ -76,6 +76,12 @@ int xt_rq(const double* A, size_t m, size_t n, double* R, double* Q);
 int xt_qc(const double* A, size_t m, size_t n, double* Q, double* C, size_t* rank);
 /* C packed m×rank, Q packed rank×n: blasWrapper::cq, blasLapackWrapper.h:103-106 */
 int xt_cq(const double* A, size_t m, size_t n, double* C, double* Q, size_t* rank);
+/* blasWrapper::solve (blasLapackWrapper.cpp:501-651, xerus::solve tensor.cpp:1654-1704): X (n×nrhs) solves
+ * A·X = B for A m×n, B m×nrhs (device, row-major) with xerus's dispatch: m ≠ n → minimum-norm least squares
+ * (dgelsd, rcond = 8ε); A not symmetric by xerus's test → LU with partial pivoting; symmetric with a definite
+ * diagonal → Cholesky, falling back on failure; else symmetric indefinite (xerus: LDLᵀ; here LU). *method
+ * (nullable) = 0 LU, 1 Cholesky, 2 symmetric indefinite, 3 least squares. A singular system is an error. */
+int xt_solve(const double* A, size_t m, size_t n, const double* B, size_t nrhs, double* X, int* method);
 /* thin SVD, S descending (device, min(m,n)): blasWrapper::svd, blasLapackWrapper.h:88-91 */
 int xt_svd(const double* A, size_t m, size_t n, double* U, double* S, double* Vt);
 /* xerus::reshuffle (indexedTensor_tensor_evaluate.cpp:55-137): shuffle[i] = new position of mode i */

@@ -128,6 +128,64 @@ def test_qc_cq_rank(xe, oracle, m, n, r):
     assert np.abs(Q2 @ Q2.T - np.eye(rk)).max() < 1e-12
 
 
+@pytest.mark.parametrize("case", range(6))
+def test_solve_dispatch_matches_oracle(xe, oracle, case):
+    """xt_solve (blasWrapper::solve): the same method as the oracle's restatement of xerus's dispatch and the same
+    solution to 1e-10 relative."""
+    from test_oracle import _solve_cases
+    name, A, method = _solve_cases(np.random.default_rng(55))[case]
+    print(f"solve {name}", flush=True)
+    b = np.random.default_rng(56).standard_normal(A.shape[0])
+    x, m = xe.solve(A, b, return_method=True)
+    xo, mo = oracle.solve(A, b)
+    assert m == mo == method, name
+    assert np.abs(x - xo).max() < 1e-10 * max(1.0, np.abs(xo).max())
+
+
+@pytest.mark.parametrize("n,nrhs,kind", [(300, 3, "nonsym"), (257, 2, "spd"), (1, 1, "spd"), (2, 1, "nonsym")])
+def test_solve_sizes(xe, n, nrhs, kind):
+    """Several right-hand sides and sizes past one thread per row: the residual of the solution is at rounding level
+    (backward stability), and it matches numpy's solve to the conditioning."""
+    print(f"solve n={n} nrhs={nrhs} {kind}", flush=True)
+    rng = np.random.default_rng(n + nrhs)
+    if kind == "spd":
+        M = rng.standard_normal((n, n))
+        A = M @ M.T + n * np.eye(n)
+    else:
+        A = rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n)
+    B = rng.standard_normal((n, nrhs))
+    X, m = xe.solve(A, B, return_method=True)
+    assert m == (1 if kind == "spd" else 0)
+    assert np.abs(A @ X - B).max() < 1e-12 * np.abs(A).max() * np.abs(X).max() * n
+    ref = np.linalg.solve(A, B)
+    assert np.abs(X - ref).max() < 1e-9 * np.abs(ref).max()
+
+
+def test_solve_singular_is_an_error(xe):
+    A = np.ones((5, 5))
+    A[:, 0] = np.arange(5.0)  # rank 2, not symmetric → LU meets an exactly zero pivot
+    A[:, 1] = 2 * A[:, 0]
+    with pytest.raises(xe.XttError):
+        xe.solve(A, np.ones(5))
+
+
+@pytest.mark.parametrize("m,n,rank", [(120, 40, 40), (30, 70, 30), (60, 20, 10)])
+def test_solve_least_squares(xe, m, n, rank):
+    """m ≠ n: dgelsd's minimum-norm least-squares solution with rcond = 8ε, against numpy's pseudo-inverse with the
+    same cut-off: full-rank over- and underdetermined systems, and a rank-deficient one whose missing rank is
+    exactly zero columns (singular values exactly zero, clear of the cut-off)."""
+    print(f"lstsq m={m} n={n} rank={rank}", flush=True)
+    rng = np.random.default_rng(m * n + rank)
+    A = rng.standard_normal((m, n))
+    A[:, rank:] = 0.0
+    A = A[:, rng.permutation(n)]
+    B = rng.standard_normal((m, 2))
+    X, meth = xe.solve(A, B, return_method=True)
+    assert meth == 3
+    ref = np.linalg.pinv(A, rcond=8 * np.finfo(float).eps) @ B
+    assert np.abs(X - ref).max() < 1e-9 * max(1.0, np.abs(ref).max())
+
+
 @pytest.mark.parametrize("m,n", [(64, 64), (100, 30), (30, 100), (1, 1), (9, 4)])
 def test_svd(xe, oracle, m, n):
     from xerus_amd._lib import call

@@ -294,3 +294,32 @@ def test_rng_prefix_fixture(oracle):
     oracle.seed(fx["seed"])
     assert np.array_equal(oracle.normal_draws(len(fx["normal"])), np.array(fx["normal"]))
     assert [int(v) for v in oracle.uniform_draws(len(fx["uniform9"]), 9)] == fx["uniform9"]
+
+
+def _solve_cases(rng):
+    """Matrices that take each branch of xerus's dispatch (blasLapackWrapper.cpp:501-651)."""
+    n = 12
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    spd = Q @ np.diag(np.linspace(1.0, 5.0, n)) @ Q.T
+    indef = Q @ np.diag(np.linspace(-1.0, 5.0, n)) @ Q.T
+    indef = indef + np.diag(np.abs(np.diag(indef)) + 0.1 - np.diag(indef))  # positive diagonal, still indefinite
+    mixed = spd.copy()
+    mixed[3, 3] = -1.0                                                         # diagonal of both signs
+    neg = -(np.abs(rng.standard_normal((n, n))) + 0.1)
+    neg = neg + neg.T                                                          # symmetric, every entry < 0
+    return [("nonsym", rng.standard_normal((n, n)), 0), ("spd", spd, 1), ("indef_posdiag", indef, 2), ("mixed_diag", mixed, 2),
+            ("negdef", -spd, 2), ("all_negative", neg, 0)]
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_oracle_solve_dispatch(oracle, case):
+    """The oracle's dense solve: xerus's method choice on constructed matrices (the all-negative symmetric matrix
+    fails xerus's symmetry test because its maximum entry starts at 0) and the solution against numpy."""
+    name, A, method = _solve_cases(np.random.default_rng(55))[case]
+    if name == "indef_posdiag":
+        assert np.all(np.diag(A) > 0) and np.linalg.eigvalsh(A).min() < 0
+    b = np.random.default_rng(56).standard_normal(A.shape[0])
+    x, m = oracle.solve(A, b)
+    assert m == method, name
+    ref = np.linalg.solve(A, b)
+    assert np.abs(x - ref).max() < 1e-10 * max(1.0, np.abs(ref).max())

@@ -76,6 +76,7 @@ SIGNATURES = {
     "xt_rq": (C.c_int, [_vp, _sz, _sz, _vp, _vp]),
     "xt_qc": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _ps]),
     "xt_cq": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _ps]),
+    "xt_solve": (C.c_int, [_vp, _sz, _sz, _vp, _sz, _vp, C.POINTER(C.c_int)]),
     "xt_svd": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp]),
     "xt_reshuffle": (C.c_int, [_sz, _ps, _ps, _vp, _vp]),
     "xt_dot": (C.c_int, [_vp, _vp, _sz, _pd]),

@@ -138,6 +138,13 @@ int xt_cq(const double* A, size_t m, size_t n, double* C, double* Q, size_t* ran
 		xtt::DTensor q, c; xtt::calculate_cq(c, q, wrap_copy(A, m, n), 1); put(Q, q); put(C, c); *rank = c.dims[1]; xtt::sync();
 	});
 }
+int xt_solve(const double* A, size_t m, size_t n, const double* B, size_t nrhs, double* X, int* method) {
+	return guard([&] {
+		NN(A); NN(B); NN(X);
+		const int mth = xtt::dense_solve(A, m, n, B, nrhs, X, xtt::stream());
+		if (method) *method = mth;
+	});
+}
 int xt_svd(const double* A, size_t m, size_t n, double* U, double* S, double* Vt) {
 	return guard([&] {
 		NN(A); NN(U); NN(S); NN(Vt);

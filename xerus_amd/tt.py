@@ -291,6 +291,27 @@ def frob_norm(t) -> float:
     return t.frob_norm()
 
 
+def solve(A, b, return_method: bool = False):
+    """xerus::solve on dense matrices (tensor.cpp:1654-1704 → blasWrapper::solve, blasLapackWrapper.cpp:542-651), on
+    the device: X with A·X = b for A (m × n) and b (m,) or (m, nrhs). xerus's dispatch picks minimum-norm least
+    squares for m ≠ n, LU for a non-symmetric A, Cholesky for a symmetric A with a definite diagonal (falling
+    back on failure) and a symmetric indefinite solve otherwise. return_method also returns the method taken:
+    0 LU, 1 Cholesky, 2 symmetric indefinite, 3 least squares."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if A.ndim != 2 or b.shape[0] != A.shape[0]:
+        raise XttError("solve: A must be m × n and b must have m rows")
+    m, n = A.shape
+    B = np.ascontiguousarray(b.reshape(m, -1))
+    nrhs = B.shape[1]
+    dA, dB, dX = _lib.DeviceArray.from_numpy(A), _lib.DeviceArray.from_numpy(B), _lib.DeviceArray((n, nrhs), np.float64)
+    method = C.c_int()
+    call("xt_solve", dA.ptr, m, n, dB.ptr, nrhs, dX.ptr, C.byref(method))
+    X = dX.numpy()
+    X = X.reshape(n) if b.ndim == 1 else X
+    return (X, method.value) if return_method else X
+
+
 def residual(A: TTOperator, x: TTTensor, b: TTTensor) -> float:
     """‖A·x − b‖/‖b‖ computed stably on the device (TT of A·x − b, Householder R sweep)."""
     out = C.c_double()
