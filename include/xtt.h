@@ -146,6 +146,21 @@ typedef struct {
 int xt_als_default_options(xt_als_options* o, unsigned sites, int assumeSPD);
 /* ALSVariant::operator()(A, x, b, …) (als.h:152-207); A may be NULL (x ≈ b). Returns the energy. */
 int xt_als_solve(const xt_als_options* o, xt_tt A, xt_tt x, xt_tt b, double* energy, xt_als_stats* stats);
+/* ALS building blocks on device buffers (row-major): stacks L (a,p,a) and R (c,q,c), operator core A (p,n,n,q),
+ * site vectors X, x, y, b (a,n,c). Left stack push, localOperatorSlice (als.cpp:184-200) contracted onto the left
+ * stack as in prepare_stacks (:247): out (c,q,c) = Σ L(a,p,a')·X(a,n,c)·A(p,n,n',q)·X(a',n',c'). */
+int xt_als_env_left(const double* L, const double* X, const double* A, size_t a, size_t n, size_t c, size_t p, size_t q, double* out);
+/* Right stack push (als.cpp:239): out (a,p,a) = Σ R(c,q,c')·X(a,n,c)·A(p,n,n',q)·X(a',n',c'). */
+int xt_als_env_right(const double* R, const double* X, const double* A, size_t a, size_t n, size_t c, size_t p, size_t q, double* out);
+/* The local operator of construct_local_operator (als.cpp:383-402) applied without densifying it:
+ * y(a,n,c) = Σ L(a,p,a')·A(p,n,n',q)·R(c,q,c')·x(a',n',c'). */
+int xt_als_local_apply(const double* L, const double* A, const double* R, size_t a, size_t n, size_t c, size_t p, size_t q,
+                       const double* x, double* y);
+/* The local solve (localSolver, als.cpp:537) by CG on that operator (SPD): x holds the start value and receives
+ * the solution; stops when ‖b − A_loc·x‖ <= tol·‖b‖ or after max_iter iterations; *iterations (nullable) = the
+ * iterations taken. xerus's lapack_solver (als.cpp:43-75) densifies the same operator and solves it directly. */
+int xt_als_local_cg(const double* L, const double* A, const double* R, size_t a, size_t n, size_t c, size_t p, size_t q,
+                    const double* b, double* x, double tol, int max_iter, size_t* iterations);
 
 /* PerformanceData (performanceData.h:43-157, PerformanceData::add at performanceData.cpp:34-80). While a sink
  * is set on the calling thread, xt_als_solve adds a point after the initial energy (flags 3), after every local

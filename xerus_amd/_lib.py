@@ -77,6 +77,10 @@ SIGNATURES = {
     "xt_qc": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _ps]),
     "xt_cq": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _ps]),
     "xt_solve": (C.c_int, [_vp, _sz, _sz, _vp, _sz, _vp, C.POINTER(C.c_int)]),
+    "xt_als_env_left": (C.c_int, [_vp, _vp, _vp, _sz, _sz, _sz, _sz, _sz, _vp]),
+    "xt_als_env_right": (C.c_int, [_vp, _vp, _vp, _sz, _sz, _sz, _sz, _sz, _vp]),
+    "xt_als_local_apply": (C.c_int, [_vp, _vp, _vp, _sz, _sz, _sz, _sz, _sz, _vp, _vp]),
+    "xt_als_local_cg": (C.c_int, [_vp, _vp, _vp, _sz, _sz, _sz, _sz, _sz, _vp, _vp, C.c_double, C.c_int, C.POINTER(C.c_size_t)]),
     "xt_svd": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp]),
     "xt_reshuffle": (C.c_int, [_sz, _ps, _ps, _vp, _vp]),
     "xt_dot": (C.c_int, [_vp, _vp, _sz, _pd]),

@@ -615,6 +615,67 @@ struct ALSData {
 
 }  // namespace
 
+// ---- the ALS building blocks on raw device buffers (row-major): stacks L (a, p, a) and R (c, q, c), operator core
+// A (p, n, n, q), site vector x (a, n, c) --------------------------------------------------------------------------
+namespace {
+DTensor wrap_dev(const double* p, std::vector<size_t> dims) {
+	DTensor t(std::move(dims));
+	XTT_HIP(hipMemcpyAsync(t.data(), p, t.count() * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+	return t;
+}
+void put_dev(double* p, const DTensor& t) {
+	XTT_HIP(hipMemcpyAsync(p, t.data(), t.count() * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+}
+}  // namespace
+
+void als_env_left(const double* L, const double* X, const double* A, size_t a, size_t n, size_t c, size_t p, size_t q, double* out) {
+	XTT_REQUIRE(a && n && c && p && q, "env_update: empty dimension");
+	put_dev(out, op_left(wrap_dev(L, {a, p, a}), wrap_dev(X, {a, n, c}), wrap_dev(A, {p, n, n, q})));
+}
+void als_env_right(const double* R, const double* X, const double* A, size_t a, size_t n, size_t c, size_t p, size_t q, double* out) {
+	XTT_REQUIRE(a && n && c && p && q, "env_update: empty dimension");
+	put_dev(out, op_right(wrap_dev(R, {c, q, c}), wrap_dev(X, {a, n, c}), wrap_dev(A, {p, n, n, q})));
+}
+void als_local_apply(const double* L, const double* A, const double* R, size_t a, size_t n, size_t c, size_t p, size_t q, const double* x,
+                     double* y) {
+	XTT_REQUIRE(a && n && c && p && q, "local_apply: empty dimension");
+	const DTensor Ld = wrap_dev(L, {a, p, a}), Rd = wrap_dev(R, {c, q, c}), Ad = wrap_dev(A, {p, n, n, q});
+	SiteOp s;
+	site_from(s, Ld, Ad, Rd);
+	DBuf<double> T1(a * p * n * c), T2(a * n * q * c);
+	local_matvec<double>(Ld.data(), s.Am.data(), Rd.data(), a, n, c, p, q, x, y, T1.get(), T2.get(), nullptr, P_F64);
+}
+size_t als_local_cg(const double* L, const double* A, const double* R, size_t a, size_t n, size_t c, size_t p, size_t q, const double* b,
+                    double* x, double tol, int maxIter) {
+	XTT_REQUIRE(a && n && c && p && q, "local_cg: empty dimension");
+	XTT_REQUIRE(tol >= 0 && maxIter >= 0, "local_cg: tolerance and iteration limit must be non-negative");
+	const DTensor Ld = wrap_dev(L, {a, p, a}), Rd = wrap_dev(R, {c, q, c}), Ad = wrap_dev(A, {p, n, n, q});
+	SiteOp s;
+	site_from(s, Ld, Ad, Rd);
+	const size_t N = s.N();
+	DBuf<double> r(N), pv(N), qv(N), T1(a * p * n * c), T2(a * n * q * c), partials(2 * 1024);
+	DBuf<CGState> st(1);
+	auto mv = [&](const double* in, double* out, const int* skip) {
+		local_matvec<double>(Ld.data(), s.Am.data(), Rd.data(), a, n, c, p, q, in, out, T1.get(), T2.get(), skip, P_F64);
+	};
+	mv(x, qv.get(), nullptr);
+	cg_state_init(st.get(), tol * tol, maxIter, stream());
+	cg_start<double>(r.get(), pv.get(), b, qv.get(), N, st.get(), partials.get(), stream());
+	const int* done = &st.get()->done;
+	CGState h{};
+	read_back(&h, st.get(), sizeof(CGState));
+	while (!h.done) {
+		for (int i = 0; i < 16; ++i) {  // iterations after convergence are predicated off on the device
+			mv(pv.get(), qv.get(), done);
+			cg_alpha<double>(pv.get(), qv.get(), N, st.get(), partials.get(), stream());
+			cg_update<double>(x, r.get(), pv.get(), qv.get(), N, st.get(), partials.get(), stream());
+			cg_direction<double>(pv.get(), r.get(), N, st.get(), stream());
+		}
+		read_back(&h, st.get(), sizeof(CGState));
+	}
+	return (size_t)h.iter;
+}
+
 double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats* stats) {  // als.cpp:708-778
 	XTT_REQUIRE(o.sites > 0, "ALSVariant: sites must be > 0");
 	XTT_REQUIRE(x.order() > 0, "ALS: x must have degree > 0");

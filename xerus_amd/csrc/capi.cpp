@@ -335,6 +335,24 @@ int xt_als_round_batch(const xt_als_options* o, size_t count, const xt_tt* A, co
 	});
 }
 
+int xt_als_env_left(const double* L, const double* X, const double* A, size_t a, size_t n, size_t c, size_t p, size_t q, double* out) {
+	return guard([&] { NN(L); NN(X); NN(A); NN(out); xtt::als_env_left(L, X, A, a, n, c, p, q, out); });
+}
+int xt_als_env_right(const double* R, const double* X, const double* A, size_t a, size_t n, size_t c, size_t p, size_t q, double* out) {
+	return guard([&] { NN(R); NN(X); NN(A); NN(out); xtt::als_env_right(R, X, A, a, n, c, p, q, out); });
+}
+int xt_als_local_apply(const double* L, const double* A, const double* R, size_t a, size_t n, size_t c, size_t p, size_t q,
+                       const double* x, double* y) {
+	return guard([&] { NN(L); NN(A); NN(R); NN(x); NN(y); xtt::als_local_apply(L, A, R, a, n, c, p, q, x, y); });
+}
+int xt_als_local_cg(const double* L, const double* A, const double* R, size_t a, size_t n, size_t c, size_t p, size_t q,
+                    const double* b, double* x, double tol, int max_iter, size_t* iterations) {
+	return guard([&] {
+		NN(L); NN(A); NN(R); NN(b); NN(x);
+		const size_t it = xtt::als_local_cg(L, A, R, a, n, c, p, q, b, x, tol, max_iter);
+		if (iterations) *iterations = it;
+	});
+}
 int xt_als_solve(const xt_als_options* o, xt_tt A, xt_tt x, xt_tt b, double* energy, xt_als_stats* st) {
 	return guard([&] {
 		NN(o); NN(x); NN(b); NN(energy);

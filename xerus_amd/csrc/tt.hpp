@@ -100,6 +100,13 @@ struct ALSStats {
 	std::vector<double> energies;
 };
 double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats* stats);
+// ALS building blocks on device buffers: stacks L (a,p,a), R (c,q,c), operator core A (p,n,n,q), vectors (a,n,c)
+void als_env_left(const double* L, const double* X, const double* A, size_t a, size_t n, size_t c, size_t p, size_t q, double* out);
+void als_env_right(const double* R, const double* X, const double* A, size_t a, size_t n, size_t c, size_t p, size_t q, double* out);
+void als_local_apply(const double* L, const double* A, const double* R, size_t a, size_t n, size_t c, size_t p, size_t q, const double* x,
+                     double* y);
+size_t als_local_cg(const double* L, const double* A, const double* R, size_t a, size_t n, size_t c, size_t p, size_t q, const double* b,
+                    double* x, double tol, int maxIter);
 // problem p: `sweeps` × (als_solve, round(eps)) on one of `threads` worker threads with streams of their own
 void als_round_batch(const ALSOptions& o, const std::vector<const TT*>& A, const std::vector<TT*>& x,
                      const std::vector<const TT*>& b, size_t sweeps, double eps, int threads, double* residuals,
