@@ -79,6 +79,38 @@ def test_round_eps_and_maxrank(xe, oracle):
     assert rel(b.full(), bo.full()) < 1e-10
 
 
+def test_round_speculation_matches_normal(xe, monkeypatch):
+    """A round after one that kept every rank on the same shapes runs with its rank decisions assumed and checked
+    once at the end; a tensor of the same ranks whose round does truncate makes the check fail and the round rerun.
+    Every result (cores, ranks, core position) must equal the same rounds with speculation off."""
+    print("round speculation", flush=True)
+    dims, ranks = [4] * 8, [6] * 7
+    xe.seed(SEED)
+    x1 = xe.TTTensor.random(dims, ranks)                 # generic: round(1e-12) keeps its ranks
+    y = xe.TTTensor.random(dims, [2, 3, 3, 3, 3, 3, 2])
+    z = xe.TTTensor.random(dims, [2, 3, 3, 3, 3, 3, 2])
+    x2 = y + z * 1e-14   # x1's ranks [4, 6, …, 6, 4] (z is above the QC rule's 16ε), but round(1e-12) drops z
+    assert x2.ranks() == x1.ranks() and x2.corePosition == x1.corePosition
+
+    def sequence():
+        out = []
+        for src in (x1, x1, x2, x1):
+            z = src.copy()
+            z.round(1e-12)
+            out.append((z.ranks(), z.corePosition, [z.get_component(k) for k in range(z.order())]))
+        return out
+
+    monkeypatch.setenv("XTT_NO_SPECULATION", "1")
+    ref = sequence()
+    monkeypatch.delenv("XTT_NO_SPECULATION")
+    got = sequence()
+    assert ref[2][0] == [2, 3, 3, 3, 3, 3, 2]           # the third round truncates: its speculation fails
+    for (r1, c1, k1), (r2, c2, k2) in zip(ref, got):
+        assert r1 == r2 and c1 == c2
+        for a, b in zip(k1, k2):
+            assert np.array_equal(a, b)
+
+
 def test_tt_sum_and_difference(xe, oracle):
     """TT:real_difference pins (ttArithmetic.cxx:74-115): sum ranks {8,8,8,8}; a − a ≈ 0."""
     print("tt sum", flush=True)

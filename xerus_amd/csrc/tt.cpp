@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstring>
+#include <map>
 #include <random>
 
 namespace xtt {
@@ -392,6 +394,127 @@ void round_edge(TT& x, size_t from, size_t to, size_t maxRank, double eps, bool 
 	}
 }
 
+// ---- speculative round ---------------------------------------------------------------------------------------
+// Every rank decision of round (the QC sweep, each edge's CQ and SVD truncation) normally costs one read-back.
+// When the last round of this thread on the same shapes and parameters kept every rank, round first runs with
+// each decision assumed (QC and CQ keep full rank, the SVD keeps the current edge rank). Each kernel writes the
+// rank it decides to a device slot, and one read-back at the end compares them with the assumptions. A mismatch
+// restores x and runs the normal round, so the result is always the normal round's. (A wrong assumption leaves
+// unwritten columns in the pass being discarded; every kernel loop is bounded, so that pass still ends.)
+namespace {
+struct NotSpeculable {};  // a shape outside the register-resident kernels, or a canonicalisation detour
+struct RankSpec {
+	DBuf<int> slots;
+	std::vector<int> expect;
+	explicit RankSpec(size_t cap) : slots(cap) {}
+	int* slot(int expected) {
+		if (expect.size() >= slots.size()) throw NotSpeculable{};
+		expect.push_back(expected);
+		return slots.get() + (expect.size() - 1);
+	}
+	bool holds() {
+		std::vector<int> h(expect.size());
+		if (!h.empty()) read_back(h.data(), slots.get(), h.size() * sizeof(int));
+		return h == expect;
+	}
+};
+
+// calculate_qc with the rank assumed full (min(m, n))
+void qc_spec(DTensor& Q, DTensor& C, const DTensor& in, size_t split, RankSpec& rs) {
+	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree());
+	if (!hh_supported(m, n)) throw NotSpeculable{};
+	const size_t k = std::min(m, n);
+	DBuf<double> q(m * k), c(k * n);
+	DBuf<int> jp(n);
+	hh_qc(in.data(), m, n, q.get(), c.get(), jp.get(), rs.slot((int)k), stream());
+	Q = DTensor(cat(sl(in.dims, 0, split), {k}), std::move(q));
+	C = DTensor(cat({k}, sl(in.dims, split, in.degree())), std::move(c));
+}
+// calculate_cq with the rank assumed full
+void cq_spec(DTensor& C, DTensor& Q, const DTensor& in, size_t split, RankSpec& rs) {
+	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree());
+	DTensor At({n, m});
+	{ const size_t d2[2] = {m, n}, sh[2] = {1, 0}; permute_f64(2, d2, sh, in.data(), At.data(), stream()); }
+	DTensor Qt, Ct;
+	qc_spec(Qt, Ct, At, 1, rs);
+	const size_t r = Ct.dims[0];
+	C = DTensor(cat(sl(in.dims, 0, split), {r}));
+	Q = DTensor(cat({r}, sl(in.dims, split, in.degree())));
+	{ const size_t d2[2] = {r, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, Ct.data(), C.data(), stream()); }
+	{ const size_t d2[2] = {n, r}, sh[2] = {1, 0}; permute_f64(2, d2, sh, Qt.data(), Q.data(), stream()); }
+}
+// calculate_svd with the truncation rank assumed (rank), the decided one written to a slot
+void svd_spec(DTensor& U, DTensor& Vt, DBuf<double>& sDev, const DTensor& in, size_t split, size_t rank, size_t maxRank, double eps,
+              bool jacobiOnTranspose, RankSpec& rs) {
+	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree()), k = std::min(m, n);
+	if (rank == 0 || rank > k) throw NotSpeculable{};
+	DBuf<double> u(m * k), s(k), vt(k * n);
+	const size_t M = std::max(m, n), N = k, np = N + (N & 1);
+	DBuf<double> ws(M * np + np * np + 2 * np);
+	if (m > n || (m == n && !jacobiOnTranspose)) {
+		jacobi_svd(in.data(), m, n, u.get(), s.get(), vt.get(), ws.get(), stream());
+	} else {
+		DBuf<double> at(n * m), u2(n * m), vt2(m * m);
+		{ const size_t d2[2] = {m, n}, sh[2] = {1, 0}; permute_f64(2, d2, sh, in.data(), at.get(), stream()); }
+		jacobi_svd(at.get(), n, m, u2.get(), s.get(), vt2.get(), ws.get(), stream());
+		{ const size_t d2[2] = {m, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, vt2.get(), u.get(), stream()); }
+		{ const size_t d2[2] = {n, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, u2.get(), vt.get(), stream()); }
+	}
+	svd_rank_f64(s.get(), k, maxRank, eps, rs.slot((int)rank), stream());
+	U = DTensor(cat(sl(in.dims, 0, split), {rank}));
+	Vt = DTensor(cat({rank}, sl(in.dims, split, in.degree())));
+	XTT_HIP(hipMemcpy2DAsync(U.data(), rank * 8, u.get(), k * 8, rank * 8, m, hipMemcpyDeviceToDevice, stream()));
+	XTT_HIP(hipMemcpyAsync(Vt.data(), vt.get(), rank * n * 8, hipMemcpyDeviceToDevice, stream()));
+	sDev = std::move(s);
+}
+// move_core(x, pos, false) from a canonical position, QC / CQ speculative
+void move_core_spec(TT& x, size_t pos, RankSpec& rs) {
+	if (!x.canonicalized) throw NotSpeculable{};
+	for (size_t i = x.corePosition; i < pos; ++i) {  // transfer_core(i, i+1) with rank reduction
+		DTensor& F = x.cores[i];
+		DTensor Q, R;
+		qc_spec(Q, R, F, F.degree() - 1, rs);
+		F = std::move(Q);
+		x.cores[i + 1] = contract(R, false, x.cores[i + 1], false, 1);
+	}
+	for (size_t i = x.corePosition; i > pos; --i) {  // transfer_core(i, i-1)
+		DTensor& F = x.cores[i];
+		DTensor Q, R;
+		cq_spec(R, Q, F, 1, rs);
+		F = std::move(Q);
+		x.cores[i - 1] = contract(x.cores[i - 1], false, R, false, 1);
+	}
+	if (exceeds_maximal_ranks(x)) throw NotSpeculable{};
+	x.corePosition = pos;
+}
+// round_edge(x, from, from - 1, maxRank, eps, true) with the edge's current rank assumed
+void round_edge_spec(TT& x, size_t from, size_t to, size_t maxRank, double eps, RankSpec& rs) {
+	DTensor& F = x.cores[from];
+	DTensor& T = x.cores[to];
+	const size_t r = F.dims[0];
+	DBuf<double> sd;
+	DTensor U, Vt;
+	if (5.0 * (double)F.size() * (double)T.size() >= 6.0 * (double)r * r * r * r) {
+		DTensor coreA, Fq;
+		cq_spec(coreA, Fq, F, 1, rs);
+		DTensor X = transpose2(coreA, 1);
+		svd_spec(U, Vt, sd, X, 1, r, maxRank, eps, true, rs);
+		scale_by_f64(Vt.data(), sd.get(), r, Vt.size() / r, true, stream());
+		F = contract(U, true, Fq, false, 1);
+		T = contract(T, false, Vt, true, 1);
+	} else {
+		DTensor X = contract(F, true, T, true, 1);
+		svd_spec(U, Vt, sd, X, F.degree() - 1, r, maxRank, eps, false, rs);
+		scale_by_f64(Vt.data(), sd.get(), r, Vt.size() / r, true, stream());
+		T = transpose2(Vt, 1);
+		std::vector<size_t> back(U.degree());
+		for (size_t i = 0; i + 1 < U.degree(); ++i) back[i] = i + 1;
+		back[U.degree() - 1] = 0;
+		F = reshuffle(U, back);
+	}
+}
+}  // namespace
+
 void round(TT& x, const dims_t& maxRanks, double eps) {  // ttNetwork.cpp:643-665
 	const size_t d = x.order();
 	XTT_REQUIRE(eps < 1, "_eps must be smaller than one.");
@@ -399,12 +522,42 @@ void round(TT& x, const dims_t& maxRanks, double eps) {  // ttNetwork.cpp:643-66
 	for (size_t r : maxRanks) XTT_REQUIRE(r > 0, "Trying to round a TTTensor to rank 0 is not possible.");
 	const bool initCanon = x.canonicalized;
 	const size_t initPos = x.corePosition;
+	static thread_local std::map<std::vector<size_t>, bool> keptRanks;
+	std::vector<size_t> key = x.dims;
+	const dims_t r0 = x.ranks();
+	key.insert(key.end(), r0.begin(), r0.end());
+	key.insert(key.end(), maxRanks.begin(), maxRanks.end());
+	uint64_t eb;
+	std::memcpy(&eb, &eps, sizeof(eb));
+	key.push_back((size_t)eb);
+	key.push_back(x.op);
+	key.push_back(initCanon);
+	key.push_back(initPos);
+	auto kept = keptRanks.find(key);
+	if (d > 1 && initCanon && kept != keptRanks.end() && kept->second && !getenv("XTT_NO_SPECULATION")) {
+		TT saved = tt_copy(x);
+		bool ok = false;
+		try {
+			RankSpec rs(4 * d + 8);
+			move_core_spec(x, d - 1, rs);
+			for (size_t i = 0; i + 1 < d; ++i) round_edge_spec(x, d - 1 - i, d - 2 - i, maxRanks[d - 2 - i], eps, rs);
+			x.corePosition = 0;
+			x.canonicalized = true;
+			move_core_spec(x, initPos, rs);
+			ok = rs.holds();
+		} catch (const NotSpeculable&) {
+		}
+		if (ok) return;
+		x = std::move(saved);
+	}
 	move_core(x, d - 1, false);
 	// after move_core(d-1) every core left of the edge being rounded is left-orthogonal
 	for (size_t i = 0; i + 1 < d; ++i) round_edge(x, d - 1 - i, d - 2 - i, maxRanks[d - 2 - i], eps, true);
 	x.corePosition = 0;
 	x.canonicalized = true;
 	if (initCanon) move_core(x, initPos, false);
+	if (keptRanks.size() > 256) keptRanks.clear();
+	keptRanks[key] = x.ranks() == r0;
 }
 
 double tt_dot(const TT& a, const TT& b) {

@@ -49,6 +49,15 @@ __global__ void scale_by_kernel(double* m, const double* sv, size_t rows, size_t
 	for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
 		m[e] *= sv[byRow ? e / cols : e % cols];
 }
+// calculate_svd's truncation rank on the device: min(k, maxRank), cut at the first j with s_j <= eps·s_0
+__global__ void svd_rank_kernel(const double* sv, size_t k, size_t maxRank, double eps, int* out) {
+	if (threadIdx.x != 0 || blockIdx.x != 0) return;
+	size_t rank = k;
+	if (maxRank != 0 && maxRank < rank) rank = maxRank;
+	for (size_t j = 1; j < rank; ++j)
+		if (sv[j] <= eps * sv[0]) { rank = j; break; }
+	*out = (int)rank;
+}
 __global__ void axpy_kernel(double* y, double a, const double* x, size_t n) {
 	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] += a * x[i];
 }
@@ -269,6 +278,10 @@ void scale_f64(double* p, double a, size_t n, hipStream_t s) {
 void scale_by_f64(double* m, const double* sv, size_t rows, size_t cols, bool byRow, hipStream_t s) {
 	if (!rows || !cols) return;
 	hipLaunchKernelGGL(scale_by_kernel, dim3(grid_for(rows * cols)), dim3(256), 0, s, m, sv, rows, cols, byRow ? 1 : 0);
+	XTT_LAUNCH_CHECK();
+}
+void svd_rank_f64(const double* sv, size_t k, size_t maxRank, double eps, int* out, hipStream_t s) {
+	hipLaunchKernelGGL(svd_rank_kernel, dim3(1), dim3(64), 0, s, sv, k, maxRank, eps, out);
 	XTT_LAUNCH_CHECK();
 }
 void axpy_f64(double* y, double a, const double* x, size_t n, hipStream_t s) {
