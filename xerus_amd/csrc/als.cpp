@@ -234,7 +234,7 @@ struct ALSData {
 	TT& x;
 	const TT& b;
 	dims_t targetRank;
-	double normB;
+	double normB = -1;  // ‖b‖, taken on first use (only the non-SPD energy needs it)
 	bool canonAtEnd;
 	size_t corePosAtEnd;
 	std::vector<DTensor> opL, opR, rhsL, rhsR;
@@ -251,7 +251,7 @@ struct ALSData {
 	size_t cgIters = 0;  // inner CG iterations of this call
 
 	ALSData(const ALSOptions& o_, const TT* A_, TT& x_, const TT& b_, ALSStats* s)
-	    : o(o_), A(A_), x(x_), b(b_), targetRank(x_.ranks()), normB(tt_frob_norm(b_)), canonAtEnd(x_.canonicalized),
+	    : o(o_), A(A_), x(x_), b(b_), targetRank(x_.ranks()), canonAtEnd(x_.canonicalized),
 	      corePosAtEnd(x_.corePosition), stats(s) {
 		if (A) {
 			if (o.assumeSPD) { B = tt_copy(*A); rhs = tt_copy(b); } else { B = op_gram(*A); rhs = op_transpose_apply(*A, b); }
@@ -350,15 +350,18 @@ struct ALSData {
 		if (o.useResidualForEndCriterion && A && o.assumeSPD) return tt_residual(*A, x, b);
 		DTensor xm = merged(x);
 		DTensor bl = local_rhs();
-		const double xb = dt_dot(xm, bl);
-		if (!A) return 0.5 * dt_dot(xm, xm) - xb;
+		if (!A) {
+			const auto [xx, xb] = dt_dot2(xm, xm, xm, bl);
+			return 0.5 * xx - xb;
+		}
 		DTensor Bm = merged_op();
 		SiteOp s;
 		site_from(s, opL.back(), Bm, opR.back());
 		DTensor y(xm.dims);
 		matvec64(s, xm.data(), y.data(), nullptr);
-		const double xAx = dt_dot(xm, y);
+		const auto [xb, xAx] = dt_dot2(xm, bl, xm, y);  // one read-back for both
 		if (o.assumeSPD) return std::abs(0.5 * xAx - xb);
+		if (normB < 0) normB = tt_frob_norm(b);
 		return std::sqrt(xAx - 2 * xb + normB * normB) / normB;
 	}
 

@@ -231,6 +231,15 @@ double dt_dot(const DTensor& a, const DTensor& b) {
 	read_back(&r, s.scalar.get(), 8);
 	return r;
 }
+std::pair<double, double> dt_dot2(const DTensor& a, const DTensor& b, const DTensor& c, const DTensor& d) {
+	XTT_REQUIRE(a.size() == b.size() && c.size() == d.size(), "dot: size mismatch");
+	Scratch& s = scratch();
+	dot_f64(a.data(), b.data(), a.size(), s.scalar.get(), s.partials.get(), s.counter.get(), stream());
+	dot_f64(c.data(), d.data(), c.size(), s.scalar.get() + 1, s.partials.get(), s.counter.get(), stream());
+	double r[2] = {0, 0};
+	read_back(r, s.scalar.get(), 16);
+	return {r[0], r[1]};
+}
 double dt_norm(const DTensor& t) { return std::sqrt(std::max(0.0, dt_dot(t, t))); }
 void dt_scale(DTensor& t, double s) { scale_f64(t.data(), s, t.size(), stream()); }
 

@@ -3,6 +3,7 @@
 #pragma once
 #include <functional>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "kernels.hpp"
@@ -38,6 +39,8 @@ void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTenso
 DTensor qr_r_only(const DTensor& in, size_t split);
 double dt_norm(const DTensor& t);
 double dt_dot(const DTensor& a, const DTensor& b);
+// (a·b, c·d) with one read-back
+std::pair<double, double> dt_dot2(const DTensor& a, const DTensor& b, const DTensor& c, const DTensor& d);
 void dt_scale(DTensor& t, double s);
 
 // ---- TTNetwork ---------------------------------------------------------------------------------------
