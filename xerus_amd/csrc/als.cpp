@@ -172,6 +172,9 @@ struct Workspace {
 	DBuf<float> r32, p32, q32, e32, z32, T1f, T2f, L32, R32, Am32;
 	DBuf<CGState> st;
 	DBuf<double> partials;
+	DBuf<double> fcg;  // the fused CG's seven vectors (FP64, or FP32 in the first half)
+	DBuf<double> fsync;            // the fused CG's barrier slots
+	unsigned long long fepoch = 0;  // launches on fsync so far
 	std::map<std::vector<size_t>, hipGraphExec_t> graphs;
 	std::set<std::vector<size_t>> seen;
 
@@ -188,6 +191,7 @@ struct Workspace {
 		capL = std::max(capL, nl); capR = std::max(capR, nr); capAm = std::max(capAm, nam);
 		for (DBuf<double>* b : {&x, &bl, &r, &p, &q, &e64}) b->resize(capN);
 		for (DBuf<float>* b : {&r32, &p32, &q32, &e32, &z32}) b->resize(capN);
+		fcg.resize(7 * capN);
 		T1.resize(capT1); T2.resize(capT2); T1f.resize(capT1); T2f.resize(capT2);
 		L.resize(capL); L32.resize(capL); R.resize(capR); R32.resize(capR); Am.resize(capAm); Am32.resize(capAm);
 		if (!st.get()) { st.resize(1); partials.resize(2 * 1024); }
@@ -221,6 +225,10 @@ struct Workspace {
 	}
 };
 
+// Set on the batch driver's worker threads: their streams run concurrently, so a solve must not take a
+// device-wide grid (fused CG) there.
+thread_local bool t_batch_worker = false;
+
 Workspace& workspace() {
 	static thread_local Workspace w;  // per thread: its graphs replay on that thread's stream
 	return w;
@@ -249,6 +257,10 @@ struct ALSData {
 	bool spec = false;
 	unsigned* specFail = nullptr;
 	size_t cgIters = 0;  // inner CG iterations of this call
+	// Fused CG solves (one cooperative launch each) count their iterations and matvec flops on the device;
+	// finish_counts() reads them once at the end of the call.
+	DBuf<double> itDev;
+	bool usedFused = false;
 
 	ALSData(const ALSOptions& o_, const TT* A_, TT& x_, const TT& b_, ALSStats* s)
 	    : o(o_), A(A_), x(x_), b(b_), targetRank(x_.ranks()), canonAtEnd(x_.canonicalized),
@@ -389,6 +401,44 @@ struct ALSData {
 
 	static constexpr int kChunk = 16;
 
+	template <class T>
+	bool fused_ok(const SiteOp& s) const {
+		return !t_batch_worker && !getenv("XTT_NO_FUSED_CG") && fused_cg_fits<T>(s.a, s.n, s.c, s.P, s.Q);
+	}
+	template <class T>
+	void run_fused(const SiteOp& s, FusedCG d) {
+		if (!usedFused) {
+			itDev.resize(2);
+			fill_f64(itDev.get(), 0.0, 2, stream());
+			usedFused = true;
+		}
+		d.L = s.L->data(); d.Am = s.Am.data(); d.R = s.R->data();
+		d.vec = ws.fcg.get();
+		d.T2 = sizeof(T) == 8 ? (void*)ws.T2.get() : (void*)ws.T2f.get();
+		if (!ws.fsync.get()) {
+			ws.fsync.resize(FUSED_CG_SYNC);
+			XTT_HIP(hipMemsetAsync(ws.fsync.get(), 0, FUSED_CG_SYNC * sizeof(double), stream()));
+		}
+		d.sync = ws.fsync.get();
+		d.epoch = ++ws.fepoch;
+		d.st = ws.st.get();
+		d.iterAcc = itDev.get();
+		d.flopsPerIter = 2.0 * s.a * s.n * s.c * (s.a * s.P + s.n * s.P * s.Q + s.c * s.Q);
+		d.a = s.a; d.n = s.n; d.c = s.c; d.P = s.P; d.Q = s.Q;
+		fused_cg<T>(d, stream());
+	}
+	void finish_counts() {
+		if (!usedFused) return;
+		double h[2];
+		read_back(h, itDev.get(), sizeof h);
+		XTT_REQUIRE(!std::isnan(h[0]), "ALS: a fused CG grid barrier wait ran out of time");
+		const size_t it = (size_t)h[0];
+		cgIters += it;
+		if (!stats) return;
+		stats->cgIterations += it;
+		if (o.precision == P_F64) { stats->matvecs64 += it; stats->flopsMatvec += h[1]; } else stats->matvecsLow += it;
+	}
+
 	// FP64 CG on A_loc·x = b, warm start from x (device vector, overwritten). The start residual is read back
 	// first, so a site that is already converged costs one matvec and one synchronisation.
 	void cg64(SiteOp& s, double* xout, const double* blin) {
@@ -397,6 +447,15 @@ struct ALSData {
 		if (spec) {  // cg_start's test, on the device: converged iff ‖b − Ax‖² <= tol²·‖b‖² (or no iterations allowed)
 			matvec64(s, xout, ws.q.get(), nullptr);
 			scratch.check(ws.r.get(), blin, ws.q.get(), N, o.localTol * o.localTol, o.maxLocalIter > 0 ? specFail : nullptr);
+			return;
+		}
+		if (fused_ok<double>(s)) {  // no synchronisation: the kernel reads cg_start's verdict itself
+			matvec64(s, xout, ws.q.get(), nullptr);
+			cg_state_init(ws.st.get(), o.localTol * o.localTol, o.maxLocalIter, stream());
+			cg_start<double>(ws.r.get(), ws.p.get(), blin, ws.q.get(), N, ws.st.get(), ws.partials.get(), stream());
+			FusedCG d;
+			d.r0 = ws.r.get(); d.x = xout; d.mode = 0;
+			run_fused<double>(s, d);
 			return;
 		}
 		double* x = ws.x.get();
@@ -440,6 +499,7 @@ struct ALSData {
 		const int* done = &ws.st.get()->done;
 		const std::vector<size_t> key{(size_t)o.precision, s.a, s.n, s.c, s.P, s.Q};
 		bool staged = false;
+		const bool fused = o.precision == P_F32 && fused_ok<float>(s);
 		if (spec) {  // the first refinement test, on the device
 			matvec64(s, x, ws.q.get(), nullptr);
 			sc.check(ws.r.get(), bl, ws.q.get(), N, o.localTol * o.localTol, o.maxRefine > 0 ? specFail : nullptr);
@@ -451,6 +511,16 @@ struct ALSData {
 			sc.residual(ws.r.get(), bl, ws.q.get(), N, rb);
 			const double rr = rb[0], tol2 = o.localTol * o.localTol * rb[1];
 			if (rr <= tol2 || ref == o.maxRefine) break;
+			if (fused) {  // e from 0 for r/‖r‖ and x += ‖r‖·e, all inside one launch
+				if (stats) stats->refinements++;
+				const double rn = std::sqrt(rr);
+				cg_state_init(ws.st.get(), 0.0, 0, stream());
+				FusedCG d;
+				d.r0 = ws.r.get(); d.r0scale = 1.0 / rn; d.x = x; d.xscale = rn; d.mode = 1;
+				d.tol2 = inner * inner; d.maxIter = o.maxLocalIter;
+				run_fused<float>(s, d);
+				continue;
+			}
 			if (!staged) {
 				convert_f64_f32(s.L->data(), ws.L32.get(), s.L->size(), stream());
 				convert_f64_f32(s.R->data(), ws.R32.get(), s.R->size(), stream());
@@ -714,6 +784,7 @@ double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats*
 			if (data.end_of_sweep()) break;
 			data.next();
 		}
+		data.finish_counts();
 		cgIters = data.cgIters;
 		return data.energy;
 	};
@@ -760,6 +831,7 @@ void als_round_batch(const ALSOptions& o, const std::vector<const TT*>& A, const
 			XTT_HIP(hipSetDevice(dev));
 			XTT_HIP(hipStreamCreateWithFlags(&s, hipStreamDefault));
 			bind_thread_stream(s);
+			t_batch_worker = true;
 			ALSStats st;
 			for (size_t p = next++; p < count; p = next++) {
 				TT& xp = *x[p];

@@ -66,6 +66,35 @@ template <class T> void cg_update(T* x, T* r, const T* p, const T* q, size_t n, 
 template <class T> void cg_direction(T* p, const T* r, size_t n, CGState* st, hipStream_t s);
 void cg_state_init(CGState* st, double tol2, int maxIter, hipStream_t s);
 
+// Whole CG solve of an ALS local system in one launch with grid-wide barriers (cgfused.hip). Operator pieces
+// are FP64 in the layouts L (a,P,a), Am (n,Q,P,n), R (c,Q,c); the arithmetic type T is double or float.
+//   mode 0: warm start from x (FP64, in/out); r0 = b − A·x and the state (rr0, tol2, maxIter, done, the zeroed
+//           fail flag) come from cg_state_init + cg_start.
+//   mode 1: correction e from 0 for the right-hand side r0·r0scale; x += xscale·e on exit; tol2 and maxIter are
+//           taken from here, relative to ‖r0·r0scale‖²; the state only needs cg_state_init (the fail flag).
+// Not for concurrent use on several streams: the grid takes every compute unit.
+// iterAcc (optional, device): [0] += iterations, [1] += iterations·flopsPerIter; set to NaN if a barrier wait
+// ran out of time (then x is left unchanged and the state's `pad` flag is set).
+struct FusedCG {
+	const double* L = nullptr;
+	const double* Am = nullptr;
+	const double* R = nullptr;
+	const double* r0 = nullptr;
+	double* x = nullptr;
+	void* vec = nullptr;       // 7·a·n·c elements of T
+	void* T2 = nullptr;        // a·n·c·Q elements of T
+	double* sync = nullptr;      // FUSED_CG_SYNC doubles, zeroed once when allocated, then never cleared
+	unsigned long long epoch = 0;  // 1, 2, … per launch on the same `sync` buffer (below 2^47)
+	CGState* st = nullptr;
+	double* iterAcc = nullptr;
+	double r0scale = 1, xscale = 1, tol2 = 0, flopsPerIter = 0;
+	int maxIter = 0, mode = 0;
+	size_t a = 0, n = 0, c = 0, P = 0, Q = 0;
+};
+constexpr size_t FUSED_CG_SYNC = 1025 * 8;
+template <class T> bool fused_cg_fits(size_t a, size_t n, size_t c, size_t P, size_t Q);
+template <class T> void fused_cg(const FusedCG& d, hipStream_t s);
+
 // ---- dense factorisations (one workgroup per matrix, FP64, matrix in device memory) -------------------
 // In-place Householder QR (pivot = 0) or column-pivoted QR (pivot = 1) of a row-major m×n matrix, LAPACK
 // dgeqr2 / dlaqp2 conventions: R on and above the diagonal, reflectors below, tau[min(m,n)], jpvt[n]
