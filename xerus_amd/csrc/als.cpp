@@ -90,20 +90,36 @@ DTensor merge_vec(const DTensor& x1, const DTensor& x2) {  // (a, n1·n2, c)
 
 // Operator pieces of one site in the GEMM-friendly layouts, FP64 plus optional FP32 copies.
 struct SiteOp {
-	const DTensor* L = nullptr;  // (a, p, a')
-	const DTensor* R = nullptr;  // (c, q, c')
-	DTensor Am;                  // (n, q, p, n')
+	const DTensor* L = nullptr;    // (a, p, a')
+	const DTensor* R = nullptr;    // (c, q, c')
+	const DTensor* AmP = nullptr;  // (n, q, p, n'): AmOwn, or the caller's permuted copy of the operator core
+	DTensor AmOwn;
 	size_t a = 0, n = 0, c = 0, P = 0, Q = 0;
+	SiteOp() = default;
+	SiteOp(const SiteOp&) = delete;  // AmP may point into the object itself
 	size_t N() const { return a * n * c; }
+	const DTensor& am() const { return *AmP; }
 };
 
-void site_from(SiteOp& s, const DTensor& L, const DTensor& Acore, const DTensor& R) {
+// The operator core (p, n, n', q) in the layout of the local operator's second GEMM, (n, q, p, n').
+const std::vector<size_t> kAmPerm{2, 0, 3, 1};
+// The same for right stack pushes, which run the left push on the mirrored site: the core is first mirrored
+// to (q, n, n', p) and then laid out as above; both permutations together are this one, (n, p, q, n').
+const std::vector<size_t> kAmPermMirrored{1, 0, 3, 2};
+
+// amCached: the core already permuted by kAmPerm (reused across the visits of one site), or null
+void site_from(SiteOp& s, const DTensor& L, const DTensor& Acore, const DTensor& R, const DTensor* amCached = nullptr) {
 	s.L = &L; s.R = &R;
 	s.P = Acore.dims[0]; s.n = Acore.dims[1]; s.Q = Acore.dims[3];
 	s.a = L.dims[0]; s.c = R.dims[0];
 	XTT_REQUIRE(Acore.dims[1] == Acore.dims[2], "ALS: local operator must be square");
 	XTT_REQUIRE(L.dims[1] == s.P && R.dims[1] == s.Q, "ALS: stack/operator rank mismatch");
-	s.Am = reshuffle(Acore, {2, 0, 3, 1});
+	if (amCached) {
+		s.AmP = amCached;
+	} else {
+		s.AmOwn = reshuffle(Acore, kAmPerm);
+		s.AmP = &s.AmOwn;
+	}
 }
 GemmDesc stage2_desc(size_t a, size_t n, size_t c, size_t P, size_t Q, const int* skip) {
 	GemmDesc g = gemm_desc(0, 0, n * Q, c, P * n);
@@ -135,11 +151,12 @@ void local_matvec(const T* L, const T* Am, const T* R, size_t a, size_t n, size_
 	gemm_t<T>(g3, T2, R, y, prec);
 }
 
-// L'(c,q,c') = Σ L(a,p,a')·X(a,n,c)·A(p,n,n',q)·X(a',n',c')
-DTensor op_left(const DTensor& L, const DTensor& X, const DTensor& Acore) {
-	const size_t a = X.dims[0], n = X.dims[1], c = X.dims[2], P = Acore.dims[0], Q = Acore.dims[3];
-	XTT_REQUIRE(L.dims[0] == a && L.dims[2] == a && L.dims[1] == P, "stack push: shape mismatch");
-	DTensor Am = reshuffle(Acore, {2, 0, 3, 1});
+// L'(c,q,c') = Σ L(a,p,a')·X(a,n,c)·A(p,n,n',q)·X(a',n',c'), with the operator core given as
+// Am = reshuffle(A, kAmPerm) = (n, q, p, n')
+DTensor op_left_am(const DTensor& L, const DTensor& X, const DTensor& Am) {
+	const size_t a = X.dims[0], n = X.dims[1], c = X.dims[2], P = Am.dims[2], Q = Am.dims[1];
+	XTT_REQUIRE(L.dims[0] == a && L.dims[2] == a && L.dims[1] == P && Am.dims[0] == n && Am.dims[3] == n,
+	            "stack push: shape mismatch");
 	DBuf<double> T1(a * P * n * c), T2(a * n * Q * c);
 	gemm_f64(gemm_desc(0, 0, a * P, n * c, a), L.data(), X.data(), T1.get(), stream());
 	gemm_f64(stage2_desc(a, n, c, P, Q, nullptr), Am.data(), T1.get(), T2.get(), stream());
@@ -147,20 +164,15 @@ DTensor op_left(const DTensor& L, const DTensor& X, const DTensor& Acore) {
 	gemm_f64(gemm_desc(1, 0, c, Q * c, a * n), X.data(), T2.get(), out.data(), stream());
 	return out;
 }
+DTensor op_left(const DTensor& L, const DTensor& X, const DTensor& Acore) { return op_left_am(L, X, reshuffle(Acore, kAmPerm)); }
+// R'(a,p,a') from R(c,q,c'): the left push on the mirrored site (X reversed to (c, n, a))
 DTensor op_right(const DTensor& R, const DTensor& X, const DTensor& Acore) {
-	DTensor Xr = reshuffle(X, {2, 1, 0});
-	DTensor Ar = reshuffle(Acore, {3, 1, 2, 0});
-	return op_left(R, Xr, Ar);
+	return op_left_am(R, reshuffle(X, {2, 1, 0}), reshuffle(Acore, kAmPermMirrored));
 }
 // Lb'(β',c) = Σ Lb(β,a)·b(β,n,β')·X(a,n,c)
 DTensor rhs_left(const DTensor& Lb, const DTensor& b, const DTensor& X) {
 	DTensor T = contract(Lb, true, b, false, 1);   // (a, n, β')
 	return contract(T, true, X, false, 2);         // (β', c)
-}
-DTensor rhs_right(const DTensor& Rb, const DTensor& b, const DTensor& X) {
-	DTensor br = reshuffle(b, {2, 1, 0});
-	DTensor Xr = reshuffle(X, {2, 1, 0});
-	return rhs_left(Rb, br, Xr);                   // (β, a)
 }
 
 // Persistent CG workspace of the library (one per process). Every buffer a CG chunk touches has a fixed
@@ -237,8 +249,14 @@ Workspace& workspace() {
 struct ALSData {
 	const ALSOptions& o;
 	const TT* A;
-	TT B;      // operator of the stacks: A (SPD) or AᵀA
-	TT rhs;    // b or Aᵀb
+	TT Bown, rhsOwn;             // AᵀA and Aᵀb when not SPD (copies of b when it aliases x)
+	const TT* Bp = nullptr;      // operator of the stacks: A (SPD) or AᵀA
+	const TT* rhsp = nullptr;    // b or Aᵀb
+	const TT& B() const { return *Bp; }
+	const TT& rhs() const { return *rhsp; }
+	// Per core k, filled on first use: the operator core permuted for left pushes and local operators (kAmPerm)
+	// and for right pushes (kAmPermMirrored), and the right-hand side core mirrored to (β', n, β).
+	std::vector<DTensor> amLeft, amRight, rhsMirror;
 	TT& x;
 	const TT& b;
 	dims_t targetRank;
@@ -265,11 +283,17 @@ struct ALSData {
 	ALSData(const ALSOptions& o_, const TT* A_, TT& x_, const TT& b_, ALSStats* s)
 	    : o(o_), A(A_), x(x_), b(b_), targetRank(x_.ranks()), canonAtEnd(x_.canonicalized),
 	      corePosAtEnd(x_.corePosition), stats(s) {
-		if (A) {
-			if (o.assumeSPD) { B = tt_copy(*A); rhs = tt_copy(b); } else { B = op_gram(*A); rhs = op_transpose_apply(*A, b); }
+		// A and b are read only (const in xerus's signature); b is copied only when it is x itself
+		if (A && !o.assumeSPD) {
+			Bown = op_gram(*A);
+			rhsOwn = op_transpose_apply(*A, b);
+			Bp = &Bown;
+			rhsp = &rhsOwn;
 		} else {
-			rhs = tt_copy(b);
+			Bp = A;
+			if (&b == &x) { rhsOwn = tt_copy(b); rhsp = &rhsOwn; } else rhsp = &b;
 		}
+		amLeft.resize(x.order()); amRight.resize(x.order()); rhsMirror.resize(x.order());
 		prepare_x();
 		prepare_stacks();
 		curr = first;
@@ -319,34 +343,59 @@ struct ALSData {
 		opL.clear(); opR.clear(); rhsL.clear(); rhsR.clear();
 		opL.push_back(ones({1, 1, 1})); opR.push_back(ones({1, 1, 1}));
 		rhsL.push_back(ones({1, 1})); rhsR.push_back(ones({1, 1}));
-		for (size_t i = d - 1; i > first + o.sites - 1; --i) {
-			if (A) push_right(i);
-			rhsR.push_back(rhs_right(rhsR.back(), rhs.cores[i], x.cores[i]));
-		}
-		for (size_t i = 0; i < first; ++i) {
-			if (A) push_left(i);
-			rhsL.push_back(rhs_left(rhsL.back(), rhs.cores[i], x.cores[i]));
-		}
+		for (size_t i = d - 1; i > first + o.sites - 1; --i) push_right(i);
+		for (size_t i = 0; i < first; ++i) push_left(i);
 	}
 	void count_env(const DTensor& X, const DTensor& Acore) {
 		if (!stats) return;
 		const double a = (double)X.dims[0], n = (double)X.dims[1], c = (double)X.dims[2], p = (double)Acore.dims[0], q = (double)Acore.dims[3];
 		stats->flopsEnv += 2 * a * n * c * (a * p + n * p * q + c * q);
 	}
-	void push_left(size_t i) { opL.push_back(op_left(opL.back(), x.cores[i], B.cores[i])); count_env(x.cores[i], B.cores[i]); }
-	void push_right(size_t i) { opR.push_back(op_right(opR.back(), x.cores[i], B.cores[i])); count_env(x.cores[i], B.cores[i]); }
+	const DTensor& am_left(size_t k) {
+		if (amLeft[k].dims.empty()) amLeft[k] = reshuffle(B().cores[k], kAmPerm);
+		return amLeft[k];
+	}
+	const DTensor& am_right(size_t k) {
+		if (amRight[k].dims.empty()) amRight[k] = reshuffle(B().cores[k], kAmPermMirrored);
+		return amRight[k];
+	}
+	const DTensor& rhs_mirror(size_t k) {
+		if (rhsMirror[k].dims.empty()) rhsMirror[k] = reshuffle(rhs().cores[k], {2, 1, 0});
+		return rhsMirror[k];
+	}
+	// stack pushes of site i (operator and right-hand side); the right push mirrors x's core once for both
+	void push_left(size_t i) {
+		if (A) { opL.push_back(op_left_am(opL.back(), x.cores[i], am_left(i))); count_env(x.cores[i], B().cores[i]); }
+		rhsL.push_back(rhs_left(rhsL.back(), rhs().cores[i], x.cores[i]));
+	}
+	void push_right(size_t i) {
+		const DTensor Xr = reshuffle(x.cores[i], {2, 1, 0});
+		if (A) { opR.push_back(op_left_am(opR.back(), Xr, am_right(i))); count_env(x.cores[i], B().cores[i]); }
+		rhsR.push_back(rhs_left(rhsR.back(), rhs_mirror(i), Xr));
+	}
 
-	DTensor merged_op() const { DTensor m = dt_copy(B.cores[curr]); for (size_t p = 1; p < o.sites; ++p) m = merge_op(m, B.cores[curr + p]); return m; }
+	DTensor merged_op() const { DTensor m = dt_copy(B().cores[curr]); for (size_t p = 1; p < o.sites; ++p) m = merge_op(m, B().cores[curr + p]); return m; }
 	DTensor merged(const TT& t) const { DTensor m = dt_copy(t.cores[curr]); for (size_t p = 1; p < o.sites; ++p) m = merge_vec(m, t.cores[curr + p]); return m; }
+	// The site's operator pieces; with one site the operator core is the cached permutation (no copy).
+	void site_op(SiteOp& s, DTensor& keep) {
+		if (o.sites == 1) {
+			site_from(s, opL.back(), B().cores[curr], opR.back(), &am_left(curr));
+		} else {
+			keep = merged_op();
+			site_from(s, opL.back(), keep, opR.back());
+		}
+	}
 
 	DTensor local_rhs() const {  // als.cpp:629-648
-		DTensor bm = merged(rhs);
+		DTensor keep;
+		if (o.sites > 1) keep = merged(rhs());
+		const DTensor& bm = o.sites > 1 ? keep : rhs().cores[curr];
 		DTensor T = contract(rhsL.back(), true, bm, false, 1);  // (a, N, β')
 		return contract(T, false, rhsR.back(), false, 1);       // (a, N, c)
 	}
 
 	void reserve(const SiteOp& s) {
-		ws.ensure(s.N(), s.a * s.P * s.n * s.c, s.a * s.n * s.Q * s.c, s.L->size(), s.R->size(), s.Am.size());
+		ws.ensure(s.N(), s.a * s.P * s.n * s.c, s.a * s.n * s.Q * s.c, s.L->size(), s.R->size(), s.am().size());
 	}
 	void count_matvec(const SiteOp& s) {
 		if (stats) { stats->matvecs64++; stats->flopsMatvec += 2.0 * s.a * s.n * s.c * (s.a * s.P + s.n * s.P * s.Q + s.c * s.Q); }
@@ -354,21 +403,23 @@ struct ALSData {
 	// matvec with the site's own (non-workspace) operator pieces; used outside the CG loops
 	void matvec64(const SiteOp& s, const double* in, double* out, const int* skip) {
 		reserve(s);
-		local_matvec<double>(s.L->data(), s.Am.data(), s.R->data(), s.a, s.n, s.c, s.P, s.Q, in, out, ws.T1.get(), ws.T2.get(), skip, P_F64);
+		local_matvec<double>(s.L->data(), s.am().data(), s.R->data(), s.a, s.n, s.c, s.P, s.Q, in, out, ws.T1.get(), ws.T2.get(), skip, P_F64);
 		if (stats) { stats->matvecs64++; stats->flopsMatvec += 2.0 * s.a * s.n * s.c * (s.a * s.P + s.n * s.P * s.Q + s.c * s.Q); }
 	}
 
 	double energy_f() {  // als.cpp:480-545, evaluated from the local quantities at curr
 		if (o.useResidualForEndCriterion && A && o.assumeSPD) return tt_residual(*A, x, b);
-		DTensor xm = merged(x);
+		DTensor keepX;
+		if (o.sites > 1) keepX = merged(x);
+		const DTensor& xm = o.sites > 1 ? keepX : x.cores[curr];  // read only
 		DTensor bl = local_rhs();
 		if (!A) {
 			const auto [xx, xb] = dt_dot2(xm, xm, xm, bl);
 			return 0.5 * xx - xb;
 		}
-		DTensor Bm = merged_op();
+		DTensor keepOp;
 		SiteOp s;
-		site_from(s, opL.back(), Bm, opR.back());
+		site_op(s, keepOp);
 		DTensor y(xm.dims);
 		matvec64(s, xm.data(), y.data(), nullptr);
 		const auto [xb, xAx] = dt_dot2(xm, bl, xm, y);  // one read-back for both
@@ -388,11 +439,11 @@ struct ALSData {
 		reserve(s);
 		XTT_HIP(hipMemcpyAsync(ws.L.get(), s.L->data(), s.L->size() * 8, hipMemcpyDeviceToDevice, stream()));
 		XTT_HIP(hipMemcpyAsync(ws.R.get(), s.R->data(), s.R->size() * 8, hipMemcpyDeviceToDevice, stream()));
-		XTT_HIP(hipMemcpyAsync(ws.Am.get(), s.Am.data(), s.Am.size() * 8, hipMemcpyDeviceToDevice, stream()));
+		XTT_HIP(hipMemcpyAsync(ws.Am.get(), s.am().data(), s.am().size() * 8, hipMemcpyDeviceToDevice, stream()));
 		if (lowp) {
 			convert_f64_f32(ws.L.get(), ws.L32.get(), s.L->size(), stream());
 			convert_f64_f32(ws.R.get(), ws.R32.get(), s.R->size(), stream());
-			convert_f64_f32(ws.Am.get(), ws.Am32.get(), s.Am.size(), stream());
+			convert_f64_f32(ws.Am.get(), ws.Am32.get(), s.am().size(), stream());
 		}
 	}
 	void ws_matvec64(const SiteOp& s, const double* in, double* out, const int* skip) {
@@ -412,7 +463,7 @@ struct ALSData {
 			fill_f64(itDev.get(), 0.0, 2, stream());
 			usedFused = true;
 		}
-		d.L = s.L->data(); d.Am = s.Am.data(); d.R = s.R->data();
+		d.L = s.L->data(); d.Am = s.am().data(); d.R = s.R->data();
 		d.vec = ws.fcg.get();
 		d.T2 = sizeof(T) == 8 ? (void*)ws.T2.get() : (void*)ws.T2f.get();
 		if (!ws.fsync.get()) {
@@ -524,7 +575,7 @@ struct ALSData {
 			if (!staged) {
 				convert_f64_f32(s.L->data(), ws.L32.get(), s.L->size(), stream());
 				convert_f64_f32(s.R->data(), ws.R32.get(), s.R->size(), stream());
-				convert_f64_f32(s.Am.data(), ws.Am32.get(), s.Am.size(), stream());
+				convert_f64_f32(s.am().data(), ws.Am32.get(), s.am().size(), stream());
 				staged = true;
 			}
 			if (stats) stats->refinements++;
@@ -574,15 +625,26 @@ struct ALSData {
 	Scratch64 scratch;  // one per solve: the reduction counter is zeroed once and returned to zero by each use
 
 	void local_solve() {
+		// A speculative one-site solve only tests the start residual of x's core: no working copy of it
+		if (spec && A && o.sites == 1) {
+			DTensor bl = local_rhs();
+			SiteOp s;
+			DTensor unused;  // only a merged multi-site operator is kept there
+			site_op(s, unused);
+			double* xc = x.cores[curr].data();
+			if (o.precision == P_F64) cg64(s, xc, bl.data()); else cg_mixed(s, xc, bl.data());
+			if (stats) stats->localSolves++;
+			return;
+		}
 		DTensor xm = merged(x);
 		const size_t a = xm.dims[0], c = xm.dims[2];
 		DTensor bl = local_rhs();
 		if (!A) {
 			xm = std::move(bl);
 		} else {
-			DTensor Bm = merged_op();
+			DTensor keepOp;
 			SiteOp s;
-			site_from(s, opL.back(), Bm, opR.back());
+			site_op(s, keepOp);
 			if (o.precision == P_F64) cg64(s, xm.data(), bl.data()); else cg_mixed(s, xm.data(), bl.data());
 		}
 		if (stats) stats->localSolves++;
@@ -671,16 +733,16 @@ struct ALSData {
 	void next() {
 		if (increasing) {
 			move_core(x, curr + 1, true);
-			if (A) { opR.pop_back(); push_left(curr); }
+			if (A) opR.pop_back();
 			rhsR.pop_back();
-			rhsL.push_back(rhs_left(rhsL.back(), rhs.cores[curr], x.cores[curr]));
+			push_left(curr);
 			curr++;
 		} else {
 			const size_t leaving = curr + o.sites - 1;
 			move_core(x, leaving - 1, true);
-			if (A) { opL.pop_back(); push_right(leaving); }
+			if (A) opL.pop_back();
 			rhsL.pop_back();
-			rhsR.push_back(rhs_right(rhsR.back(), rhs.cores[leaving], x.cores[leaving]));
+			push_right(leaving);
 			curr--;
 		}
 	}
@@ -716,7 +778,7 @@ void als_local_apply(const double* L, const double* A, const double* R, size_t a
 	SiteOp s;
 	site_from(s, Ld, Ad, Rd);
 	DBuf<double> T1(a * p * n * c), T2(a * n * q * c);
-	local_matvec<double>(Ld.data(), s.Am.data(), Rd.data(), a, n, c, p, q, x, y, T1.get(), T2.get(), nullptr, P_F64);
+	local_matvec<double>(Ld.data(), s.am().data(), Rd.data(), a, n, c, p, q, x, y, T1.get(), T2.get(), nullptr, P_F64);
 }
 size_t als_local_cg(const double* L, const double* A, const double* R, size_t a, size_t n, size_t c, size_t p, size_t q, const double* b,
                     double* x, double tol, int maxIter) {
@@ -729,7 +791,7 @@ size_t als_local_cg(const double* L, const double* A, const double* R, size_t a,
 	DBuf<double> r(N), pv(N), qv(N), T1(a * p * n * c), T2(a * n * q * c), partials(2 * 1024);
 	DBuf<CGState> st(1);
 	auto mv = [&](const double* in, double* out, const int* skip) {
-		local_matvec<double>(Ld.data(), s.Am.data(), Rd.data(), a, n, c, p, q, in, out, T1.get(), T2.get(), skip, P_F64);
+		local_matvec<double>(Ld.data(), s.am().data(), Rd.data(), a, n, c, p, q, in, out, T1.get(), T2.get(), skip, P_F64);
 	};
 	mv(x, qv.get(), nullptr);
 	cg_state_init(st.get(), tol * tol, maxIter, stream());

@@ -215,7 +215,7 @@ __global__ __launch_bounds__(FT) void fused_cg_kernel(FcgArgs<T> g) {
 	extern __shared__ __align__(16) unsigned char lds_raw[];
 	T* lds = reinterpret_cast<T*>(lds_raw);
 	__shared__ double red[2 * FW];
-	const int tid = threadIdx.x, wg = blockIdx.x, G = gridDim.x;
+	const int tid = threadIdx.x, wg = blockIdx.x;
 	const int a = g.a, n = g.n, c = g.c, P = g.P, Q = g.Q, an = a * n, nQ = n * Q, Pn = P * n, cQ = c * Q;
 	const size_t N = (size_t)an * c;
 	T* const rb0 = g.vec;

@@ -30,6 +30,15 @@ void revtrans_f64(const double* in, size_t rows, size_t cols, double* out, hipSt
 void convert_f64_f32(const double* in, float* out, size_t n, hipStream_t s);
 void convert_f32_f64(const float* in, double* out, size_t n, hipStream_t s);
 void fill_f64(double* p, double v, size_t n, hipStream_t s);
+// dst[j][0..n[j]) = src[j][0..n[j]) for j < count, one launch (count <= kMultiCopy; empty copies allowed)
+constexpr int kMultiCopy = 32;
+struct MultiCopy {
+	const double* src[kMultiCopy];
+	double* dst[kMultiCopy];
+	size_t n[kMultiCopy];
+	int count = 0;
+};
+void multi_copy_f64(const MultiCopy& c, hipStream_t s);
 // *out = the truncation rank of calculate_svd for singular values sv[0..k) (device): min(k, maxRank unless 0), cut
 // at the first j >= 1 with sv[j] <= eps·sv[0]
 void svd_rank_f64(const double* sv, size_t k, size_t maxRank, double eps, int* out, hipStream_t s);

@@ -184,21 +184,43 @@ void calculate_cq(DTensor& C, DTensor& Q, const DTensor& in, size_t split) {
 	{ const size_t d2[2] = {n, r}, sh[2] = {1, 0}; permute_f64(2, d2, sh, Qt.data(), Q.data(), stream()); }
 }
 
+namespace {
+// U (m×rank) and Vt (rank×n) from the thin factors u (m×k) and vt (k×n): the buffers themselves at full rank,
+// else copies of the leading columns / rows.
+void take_svd_factors(DTensor& U, DTensor& Vt, DBuf<double>& u, DBuf<double>& vt, dims_t uDims, dims_t vDims, size_t m, size_t n,
+                      size_t k, size_t rank) {
+	if (rank == k) {
+		U = DTensor(std::move(uDims), std::move(u));
+		Vt = DTensor(std::move(vDims), std::move(vt));
+		return;
+	}
+	U = DTensor(std::move(uDims));
+	Vt = DTensor(std::move(vDims));
+	XTT_HIP(hipMemcpy2DAsync(U.data(), rank * 8, u.get(), k * 8, rank * 8, m, hipMemcpyDeviceToDevice, stream()));
+	XTT_HIP(hipMemcpyAsync(Vt.data(), vt.get(), rank * n * 8, hipMemcpyDeviceToDevice, stream()));
+}
+}  // namespace
+
 // tensor.cpp:1424-1489 (dense branch); truncation rank = min(k, maxRank) then first j with s_j <= eps·s_0.
 void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTensor& in, size_t split, size_t maxRank, double eps,
-                   DBuf<double>* sDev, bool jacobiOnTranspose) {
+                   DBuf<double>* sDev, bool jacobiOnTranspose, const double* inTransposed) {
 	XTT_REQUIRE(eps >= 0 && eps < 1, "Epsilon must be fullfill 0 <= _eps < 1.");
 	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree()), k = std::min(m, n);
 	XTT_REQUIRE(m > 0 && n > 0, "Dimension must be larger than zero");
 	DBuf<double> u(m * k), s(k), vt(k * n);
 	const size_t M = std::max(m, n), N = k, np = N + (N & 1);
 	DBuf<double> ws(M * np + np * np + 2 * np);
+	XTT_REQUIRE(!inTransposed || !(m > n || (m == n && !jacobiOnTranspose)), "svd: a transposed input is only used for m <= n");
 	if (m > n || (m == n && !jacobiOnTranspose)) {
 		jacobi_svd(in.data(), m, n, u.get(), s.get(), vt.get(), ws.get(), stream());
 	} else {  // Aᵀ = U'·S·V'ᵀ  ⇒  A = V'·S·U'ᵀ
-		DBuf<double> at(n * m), u2(n * m), vt2(m * m);
-		{ const size_t d2[2] = {m, n}, sh[2] = {1, 0}; permute_f64(2, d2, sh, in.data(), at.get(), stream()); }
-		jacobi_svd(at.get(), n, m, u2.get(), s.get(), vt2.get(), ws.get(), stream());
+		DBuf<double> at, u2(n * m), vt2(m * m);
+		if (!inTransposed) {
+			at.resize(n * m);
+			const size_t d2[2] = {m, n}, sh[2] = {1, 0};
+			permute_f64(2, d2, sh, in.data(), at.get(), stream());
+		}
+		jacobi_svd(inTransposed ? inTransposed : at.get(), n, m, u2.get(), s.get(), vt2.get(), ws.get(), stream());
 		{ const size_t d2[2] = {m, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, vt2.get(), u.get(), stream()); }
 		{ const size_t d2[2] = {n, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, u2.get(), vt.get(), stream()); }
 	}
@@ -207,10 +229,7 @@ void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTenso
 	size_t rank = k;
 	if (maxRank != 0) rank = std::min(rank, maxRank);
 	for (size_t j = 1; j < rank; ++j) if (sh[j] <= eps * sh[0]) { rank = j; break; }
-	U = DTensor(cat(sl(in.dims, 0, split), {rank}));
-	Vt = DTensor(cat({rank}, sl(in.dims, split, in.degree())));
-	XTT_HIP(hipMemcpy2DAsync(U.data(), rank * 8, u.get(), k * 8, rank * 8, m, hipMemcpyDeviceToDevice, stream()));
-	XTT_HIP(hipMemcpyAsync(Vt.data(), vt.get(), rank * n * 8, hipMemcpyDeviceToDevice, stream()));
+	take_svd_factors(U, Vt, u, vt, cat(sl(in.dims, 0, split), {rank}), cat({rank}, sl(in.dims, split, in.degree())), m, n, k, rank);
 	S.assign(sh.begin(), sh.begin() + rank);
 	if (sDev) *sDev = std::move(s);
 }
@@ -276,7 +295,14 @@ TT tt_zero(const dims_t& dims, bool op) {  // ttNetwork.cpp:57-108
 TT tt_copy(const TT& x) {
 	TT y;
 	y.op = x.op; y.dims = x.dims; y.canonicalized = x.canonicalized; y.corePosition = x.corePosition;
-	for (const auto& c : x.cores) y.cores.push_back(dt_copy(c));
+	// all cores in one launch per kMultiCopy cores (the per-core copies are host-bound at TT ranks <= 64)
+	MultiCopy mc;
+	for (const auto& c : x.cores) {
+		y.cores.emplace_back(c.dims);
+		if (mc.count == kMultiCopy) { multi_copy_f64(mc, stream()); mc.count = 0; }
+		mc.src[mc.count] = c.data(); mc.dst[mc.count] = y.cores.back().data(); mc.n[mc.count] = c.count(); mc.count++;
+	}
+	multi_copy_f64(mc, stream());
 	return y;
 }
 
@@ -375,9 +401,10 @@ void round_edge(TT& x, size_t from, size_t to, size_t maxRank, double eps, bool 
 		DTensor coreA, Fq, U, Vt, X;
 		calculate_cq(coreA, Fq, F, 1);
 		if (toIsLeftOrthogonal) {
-			// QC(T) = (T, I): X = coreAᵀ, and T·Vtᵀ replaces Tq·(coreB·…)
-			X = transpose2(coreA, 1);
-			calculate_svd(U, S, Vt, X, 1, maxRank, eps, &sd, true);
+			// QC(T) = (T, I): X = coreAᵀ, and T·Vtᵀ replaces Tq·(coreB·…). X has no more rows than columns, so
+			// the Jacobi SVD runs on its transpose, which is coreA itself (X's entries are never formed).
+			X.dims = {coreA.dims[1], coreA.dims[0]};
+			calculate_svd(U, S, Vt, X, 1, maxRank, eps, &sd, true, coreA.data());
 			scale_by_f64(Vt.data(), sd.get(), S.size(), Vt.size() / std::max<size_t>(S.size(), 1), true, stream());
 			F = contract(U, true, Fq, false, 1);
 			T = contract(T, false, Vt, true, 1);
@@ -454,26 +481,28 @@ void cq_spec(DTensor& C, DTensor& Q, const DTensor& in, size_t split, RankSpec& 
 }
 // calculate_svd with the truncation rank assumed (rank), the decided one written to a slot
 void svd_spec(DTensor& U, DTensor& Vt, DBuf<double>& sDev, const DTensor& in, size_t split, size_t rank, size_t maxRank, double eps,
-              bool jacobiOnTranspose, RankSpec& rs) {
+              bool jacobiOnTranspose, RankSpec& rs, const double* inTransposed = nullptr) {
 	const size_t m = prodr(in.dims, 0, split), n = prodr(in.dims, split, in.degree()), k = std::min(m, n);
 	if (rank == 0 || rank > k) throw NotSpeculable{};
 	DBuf<double> u(m * k), s(k), vt(k * n);
 	const size_t M = std::max(m, n), N = k, np = N + (N & 1);
 	DBuf<double> ws(M * np + np * np + 2 * np);
+	XTT_REQUIRE(!inTransposed || !(m > n || (m == n && !jacobiOnTranspose)), "svd: a transposed input is only used for m <= n");
 	if (m > n || (m == n && !jacobiOnTranspose)) {
 		jacobi_svd(in.data(), m, n, u.get(), s.get(), vt.get(), ws.get(), stream());
 	} else {
-		DBuf<double> at(n * m), u2(n * m), vt2(m * m);
-		{ const size_t d2[2] = {m, n}, sh[2] = {1, 0}; permute_f64(2, d2, sh, in.data(), at.get(), stream()); }
-		jacobi_svd(at.get(), n, m, u2.get(), s.get(), vt2.get(), ws.get(), stream());
+		DBuf<double> at, u2(n * m), vt2(m * m);
+		if (!inTransposed) {
+			at.resize(n * m);
+			const size_t d2[2] = {m, n}, sh[2] = {1, 0};
+			permute_f64(2, d2, sh, in.data(), at.get(), stream());
+		}
+		jacobi_svd(inTransposed ? inTransposed : at.get(), n, m, u2.get(), s.get(), vt2.get(), ws.get(), stream());
 		{ const size_t d2[2] = {m, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, vt2.get(), u.get(), stream()); }
 		{ const size_t d2[2] = {n, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, u2.get(), vt.get(), stream()); }
 	}
 	svd_rank_f64(s.get(), k, maxRank, eps, rs.slot((int)rank), stream());
-	U = DTensor(cat(sl(in.dims, 0, split), {rank}));
-	Vt = DTensor(cat({rank}, sl(in.dims, split, in.degree())));
-	XTT_HIP(hipMemcpy2DAsync(U.data(), rank * 8, u.get(), k * 8, rank * 8, m, hipMemcpyDeviceToDevice, stream()));
-	XTT_HIP(hipMemcpyAsync(Vt.data(), vt.get(), rank * n * 8, hipMemcpyDeviceToDevice, stream()));
+	take_svd_factors(U, Vt, u, vt, cat(sl(in.dims, 0, split), {rank}), cat({rank}, sl(in.dims, split, in.degree())), m, n, k, rank);
 	sDev = std::move(s);
 }
 // move_core(x, pos, false) from a canonical position, QC / CQ speculative
@@ -506,8 +535,9 @@ void round_edge_spec(TT& x, size_t from, size_t to, size_t maxRank, double eps, 
 	if (5.0 * (double)F.size() * (double)T.size() >= 6.0 * (double)r * r * r * r) {
 		DTensor coreA, Fq;
 		cq_spec(coreA, Fq, F, 1, rs);
-		DTensor X = transpose2(coreA, 1);
-		svd_spec(U, Vt, sd, X, 1, r, maxRank, eps, true, rs);
+		DTensor X;  // coreAᵀ, which is never formed: the Jacobi SVD runs on coreA (see round_edge)
+		X.dims = {coreA.dims[1], coreA.dims[0]};
+		svd_spec(U, Vt, sd, X, 1, r, maxRank, eps, true, rs, coreA.data());
 		scale_by_f64(Vt.data(), sd.get(), r, Vt.size() / r, true, stream());
 		F = contract(U, true, Fq, false, 1);
 		T = contract(T, false, Vt, true, 1);

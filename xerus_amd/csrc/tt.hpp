@@ -33,8 +33,9 @@ void calculate_cq(DTensor& C, DTensor& Q, const DTensor& in, size_t split);
 // sDev, when given, receives the kept singular values on the device as well. jacobiOnTranspose: for a
 // square matrix whose rows are nearly orthogonal (Rᵀ of a pivoted QR, as in round's precompression), the
 // one-sided Jacobi runs on the transpose, where it needs fewer sweeps (Drmač–Veselić preconditioning).
+// inTransposed, when given, is the transpose of `in` (n×m row-major) for that case, and in's data is not read.
 void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTensor& in, size_t split, size_t maxRank, double eps,
-                   DBuf<double>* sDev = nullptr, bool jacobiOnTranspose = false);
+                   DBuf<double>* sDev = nullptr, bool jacobiOnTranspose = false, const double* inTransposed = nullptr);
 // R factor only (k×n) of a Householder QR of the m×n matrix view (used by the stable residual).
 DTensor qr_r_only(const DTensor& in, size_t split);
 double dt_norm(const DTensor& t);

@@ -3,6 +3,8 @@
 // Reductions are deterministic: a fixed grid writes one partial per block, and the last block to finish
 // (ticket from an agent-scope atomic, after an agent-scope fence) sums the partials in index order. No
 // float atomics, so repeated runs are bit-identical.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "xtt_common.hpp"
 
@@ -36,6 +38,14 @@ __global__ void convert_kernel_64_32(const double* in, float* out, size_t n) {
 }
 __global__ void convert_kernel_32_64(const float* in, double* out, size_t n) {
 	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = (double)in[i];
+}
+// Up to kMultiCopy device-to-device copies in one launch: blockIdx.y picks the copy, the x-blocks stride over it.
+__global__ void multi_copy_kernel(MultiCopy c) {
+	const unsigned j = blockIdx.y;
+	const double* __restrict__ in = c.src[j];
+	double* __restrict__ out = c.dst[j];
+	const size_t n = c.n[j];
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = in[i];
 }
 __global__ void fill_kernel(double* p, double v, size_t n) {
 	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
@@ -263,6 +273,15 @@ void convert_f64_f32(const double* in, float* out, size_t n, hipStream_t s) {
 void convert_f32_f64(const float* in, double* out, size_t n, hipStream_t s) {
 	if (!n) return;
 	hipLaunchKernelGGL(convert_kernel_32_64, dim3(grid_for(n)), dim3(256), 0, s, in, out, n);
+	XTT_LAUNCH_CHECK();
+}
+void multi_copy_f64(const MultiCopy& c, hipStream_t s) {
+	if (c.count <= 0) return;
+	if (c.count > kMultiCopy) throw Error(ERR_INVALID, "multi_copy: too many copies in one launch");
+	size_t most = 0;
+	for (int j = 0; j < c.count; ++j) most = std::max(most, c.n[j]);
+	if (!most) return;
+	hipLaunchKernelGGL(multi_copy_kernel, dim3(std::min<size_t>(grid_for(most), 64), c.count), dim3(256), 0, s, c);
 	XTT_LAUNCH_CHECK();
 }
 void fill_f64(double* p, double v, size_t n, hipStream_t s) {
