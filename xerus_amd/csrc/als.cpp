@@ -170,9 +170,30 @@ DTensor op_right(const DTensor& R, const DTensor& X, const DTensor& Acore) {
 	return op_left_am(R, reshuffle(X, {2, 1, 0}), reshuffle(Acore, kAmPermMirrored));
 }
 // Lb'(β',c) = Σ Lb(β,a)·b(β,n,β')·X(a,n,c)
+// Contracted over a first (Y = Lb·X, K = a) and then over (β, n) (K = β·n): both products have short K, where
+// Lb·b first would leave a K = a·n reduction onto a β'×c result.
 DTensor rhs_left(const DTensor& Lb, const DTensor& b, const DTensor& X) {
-	DTensor T = contract(Lb, true, b, false, 1);   // (a, n, β')
-	return contract(T, true, X, false, 2);         // (β', c)
+	DTensor Y = contract(Lb, false, X, false, 1);  // (β, n, c)
+	return contract(b, true, Y, false, 2);         // (β', c)
+}
+// Both stacks of one site pushed together: the operator stack (op_left_am, when L is given) and the right-hand
+// side stack (rhs_left); their independent products share launches (3 instead of 5).
+void push_both(const DTensor* L, const DTensor& X, const DTensor* Am, const DTensor& Lb, const DTensor& bc, DTensor* opOut,
+               DTensor& rhsOut) {
+	if (!L) { rhsOut = rhs_left(Lb, bc, X); return; }
+	const size_t a = X.dims[0], n = X.dims[1], c = X.dims[2], P = Am->dims[2], Q = Am->dims[1];
+	const size_t beta = Lb.dims[0], beta2 = bc.dims[2];
+	XTT_REQUIRE(L->dims[0] == a && L->dims[2] == a && L->dims[1] == P && Am->dims[0] == n && Am->dims[3] == n,
+	            "stack push: shape mismatch");
+	XTT_REQUIRE(Lb.dims[1] == a && bc.dims[0] == beta && bc.dims[1] == n, "stack push: right-hand side shape mismatch");
+	DBuf<double> T1(a * P * n * c), T2(a * n * Q * c), Y(beta * n * c);
+	gemm_f64_pair(gemm_desc(0, 0, a * P, n * c, a), L->data(), X.data(), T1.get(),     // T1 = L·X
+	              gemm_desc(0, 0, beta, n * c, a), Lb.data(), X.data(), Y.get(), stream());  // Y = Lb·X, as rhs_left
+	rhsOut = DTensor({beta2, c});
+	gemm_f64_pair(stage2_desc(a, n, c, P, Q, nullptr), Am->data(), T1.get(), T2.get(),
+	              gemm_desc(1, 0, beta2, c, beta * n), bc.data(), Y.get(), rhsOut.data(), stream());  // bᵀ·Y
+	*opOut = DTensor({c, Q, c});
+	gemm_f64(gemm_desc(1, 0, c, Q * c, a * n), X.data(), T2.get(), opOut->data(), stream());
 }
 
 // Persistent CG workspace of the library (one per process). Every buffer a CG chunk touches has a fixed
@@ -365,13 +386,17 @@ struct ALSData {
 	}
 	// stack pushes of site i (operator and right-hand side); the right push mirrors x's core once for both
 	void push_left(size_t i) {
-		if (A) { opL.push_back(op_left_am(opL.back(), x.cores[i], am_left(i))); count_env(x.cores[i], B().cores[i]); }
-		rhsL.push_back(rhs_left(rhsL.back(), rhs().cores[i], x.cores[i]));
+		DTensor op, rb;
+		push_both(A ? &opL.back() : nullptr, x.cores[i], A ? &am_left(i) : nullptr, rhsL.back(), rhs().cores[i], &op, rb);
+		if (A) { opL.push_back(std::move(op)); count_env(x.cores[i], B().cores[i]); }
+		rhsL.push_back(std::move(rb));
 	}
 	void push_right(size_t i) {
 		const DTensor Xr = reshuffle(x.cores[i], {2, 1, 0});
-		if (A) { opR.push_back(op_left_am(opR.back(), Xr, am_right(i))); count_env(x.cores[i], B().cores[i]); }
-		rhsR.push_back(rhs_left(rhsR.back(), rhs_mirror(i), Xr));
+		DTensor op, rb;
+		push_both(A ? &opR.back() : nullptr, Xr, A ? &am_right(i) : nullptr, rhsR.back(), rhs_mirror(i), &op, rb);
+		if (A) { opR.push_back(std::move(op)); count_env(x.cores[i], B().cores[i]); }
+		rhsR.push_back(std::move(rb));
 	}
 
 	DTensor merged_op() const { DTensor m = dt_copy(B().cores[curr]); for (size_t p = 1; p < o.sites; ++p) m = merge_op(m, B().cores[curr + p]); return m; }
@@ -384,6 +409,24 @@ struct ALSData {
 			keep = merged_op();
 			site_from(s, opL.back(), keep, opR.back());
 		}
+	}
+
+	// One site: the local right-hand side (as local_rhs) and q = A_loc·x, their independent products paired.
+	DTensor rhs_and_matvec(const SiteOp& s, const double* xv, double* q) {
+		const DTensor &Lb = rhsL.back(), &Rb = rhsR.back(), &bc = rhs().cores[curr];
+		const size_t a = s.a, n = s.n, c = s.c, beta = Lb.dims[0], beta2 = Rb.dims[0];
+		XTT_REQUIRE(o.sites == 1 && Lb.dims[1] == a && bc.dims[0] == beta && bc.dims[1] == n && bc.dims[2] == beta2 && Rb.dims[1] == c,
+		            "ALS: local right-hand side shape mismatch");
+		reserve(s);
+		DTensor Tb({a, n, beta2});
+		gemm_f64_pair(gemm_desc(0, 0, a * s.P, n * c, a), s.L->data(), xv, ws.T1.get(),          // T1 = L·x
+		              gemm_desc(1, 0, a, n * beta2, beta), Lb.data(), bc.data(), Tb.data(), stream());  // Tb = Lbᵀ·b
+		DTensor bl({a, n, c});
+		gemm_f64_pair(stage2_desc(a, n, c, s.P, s.Q, nullptr), s.am().data(), ws.T1.get(), ws.T2.get(),
+		              gemm_desc(0, 0, a * n, c, beta2), Tb.data(), Rb.data(), bl.data(), stream());      // bl = Tb·Rb
+		gemm_f64(gemm_desc(0, 1, a * n, c, s.Q * c), ws.T2.get(), s.R->data(), q, stream());
+		count_matvec(s);
+		return bl;
 	}
 
 	DTensor local_rhs() const {  // als.cpp:629-648
@@ -627,12 +670,14 @@ struct ALSData {
 	void local_solve() {
 		// A speculative one-site solve only tests the start residual of x's core: no working copy of it
 		if (spec && A && o.sites == 1) {
-			DTensor bl = local_rhs();
 			SiteOp s;
 			DTensor unused;  // only a merged multi-site operator is kept there
 			site_op(s, unused);
-			double* xc = x.cores[curr].data();
-			if (o.precision == P_F64) cg64(s, xc, bl.data()); else cg_mixed(s, xc, bl.data());
+			// cg_start's test on the device, as cg64 / cg_mixed make it: converged iff ‖b − Ax‖² <= tol²·‖b‖²
+			// (or no iterations / refinements allowed); a failure counts into *specFail
+			const DTensor bl = rhs_and_matvec(s, x.cores[curr].data(), ws.q.get());
+			const bool canIterate = o.precision == P_F64 ? o.maxLocalIter > 0 : o.maxRefine > 0;
+			scratch.check(ws.r.get(), bl.data(), ws.q.get(), s.N(), o.localTol * o.localTol, canIterate ? specFail : nullptr);
 			if (stats) stats->localSolves++;
 			return;
 		}

@@ -79,15 +79,15 @@ __device__ inline void stage_kmajor(T* S, const T* X, int ld, int trans, int r0,
 	}
 }
 
+// One BM×BN output tile (bx, by) of product bz (batch entry or K slice). Called by the whole workgroup.
 template <class T, int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void gemm_x4_kernel(Args g) {
+__device__ inline void gemm_x4_tile(const Args& g, int bx, int by, long long bz) {
 	if (g.skip && *g.skip) return;
 	constexpr int NT = 64 * WM * WN, TM = BM / WM / 16, TN = BN / WN / 16, PAD = 1;
 	using Acc = typename AccOf<T>::type;
 	extern __shared__ __align__(16) unsigned char smem_raw[];
 	T* As = reinterpret_cast<T*>(smem_raw);
 	T* Bs = As + (size_t)g.kc * (BM + PAD);
-	const long long bz = blockIdx.z;
 	const T* A = static_cast<const T*>(g.A);
 	const T* B = static_cast<const T*>(g.B);
 	T* C = static_cast<T*>(g.C) + bz * g.sC;
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_x4_kernel(Args g) {
 		A += bz * g.sA;
 		B += bz * g.sB;
 	}
-	const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+	const int m0 = by * BM, n0 = bx * BN;
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int wm = wave / WN, wn = wave % WN;
 	const int li = lane & 15, lk = lane >> 4;
@@ -146,6 +146,27 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_x4_kernel(Args g) {
 					*c = (g.beta != 0.0) ? v + beta * *c : v;
 				}
 			}
+}
+
+template <class T, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_x4_kernel(Args g) {
+	gemm_x4_tile<T, BM, BN, WM, WN>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Two independent products in one launch. Workgroups [0, nb0) take problem 0's tiles, the rest problem 1's;
+// each problem's (gx × gy × batch) grid is flattened with x fastest.
+struct PairGrid {
+	int gx[2], gy[2];
+	int nb0;
+};
+template <class T, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_x4_pair_kernel(Args g0, Args g1, PairGrid pg) {
+	const int b = blockIdx.x;
+	const int p = b >= pg.nb0 ? 1 : 0;
+	const int l = p ? b - pg.nb0 : b;
+	const int gx = pg.gx[p], gy = pg.gy[p];
+	const int bx = l % gx, t = l / gx;
+	gemm_x4_tile<T, BM, BN, WM, WN>(p ? g1 : g0, bx, t % gy, t / gy);
 }
 
 // Split-K epilogue: C = alpha·Σ_s P_s + beta·C, slices summed in index order (deterministic).
@@ -345,7 +366,48 @@ void launch_x4(const GemmDesc& d, const T* A, const T* B, T* C, hipStream_t s) {
 	}
 }
 
+// Two products with small tile grids (no split-K) as one launch of 32×32 tiles; anything else as two launches.
+template <class T>
+void launch_x4_pair(const GemmDesc& d0, const T* A0, const T* B0, T* C0, const GemmDesc& d1, const T* A1, const T* B1, T* C1,
+                    hipStream_t s) {
+	auto small = [](const GemmDesc& d) { return ((d.m + 63) / 64) * ((d.n + 63) / 64) * d.batch < 256; };
+	if (d0.m == 0 || d0.n == 0 || d1.m == 0 || d1.n == 0 || !small(d0) || !small(d1)) {
+		launch_x4<T>(d0, A0, B0, C0, s);
+		launch_x4<T>(d1, A1, B1, C1, s);
+		return;
+	}
+	constexpr size_t BMp = 33;
+	size_t kmax = kLdsBudget / (2 * BMp * sizeof(T));
+	kmax &= ~size_t(3);
+	Args a[2] = {make_args(d0, A0, B0, C0), make_args(d1, A1, B1, C1)};
+	const GemmDesc* d[2] = {&d0, &d1};
+	PairGrid pg;
+	size_t blocks[2], lds = 0;
+	double flops = 0, bytes = 0;
+	for (int p = 0; p < 2; ++p) {
+		const GemmDesc& g = *d[p];
+		a[p].kc = (int)std::max<size_t>(4, std::min<size_t>((g.k + 3) & ~size_t(3), kmax));
+		lds = std::max(lds, (size_t)a[p].kc * BMp * 2 * sizeof(T));
+		pg.gx[p] = (int)((g.n + 31) / 32);
+		pg.gy[p] = (int)((g.m + 31) / 32);
+		blocks[p] = (size_t)pg.gx[p] * pg.gy[p] * g.batch;
+		flops += 2.0 * g.m * g.n * g.k * g.batch;
+		bytes += (double)sizeof(T) * g.batch * ((double)g.m * g.k + (double)g.k * g.n + (g.beta != 0.0 ? 2.0 : 1.0) * g.m * g.n);
+	}
+	pg.nb0 = (int)blocks[0];
+	std::string tag = sizeof(T) == 8 ? "gemm_x4_kernel<double,32,32> pair" : "gemm_x4_kernel<float,32,32> pair";
+	if (prof_shapes()) tag += shape_suffix(d0) + " +" + shape_suffix(d1);
+	auto kern = gemm_x4_pair_kernel<T, 32, 32, 2, 2>;
+	allow_lds(kern, lds);
+	launch_timed(tag.c_str(), Work(flops, bytes), kern, dim3((unsigned)(blocks[0] + blocks[1])), dim3(256), lds, s, a[0], a[1], pg);
+}
+
 }  // namespace
+
+void gemm_f64_pair(const GemmDesc& g0, const double* A0, const double* B0, double* C0, const GemmDesc& g1, const double* A1,
+                   const double* B1, double* C1, hipStream_t s) {
+	launch_x4_pair<double>(g0, A0, B0, C0, g1, A1, B1, C1, s);
+}
 
 GemmDesc gemm_desc(int ta, int tb, size_t m, size_t n, size_t k) {
 	GemmDesc d;

@@ -20,6 +20,9 @@ struct GemmDesc {
 };
 GemmDesc gemm_desc(int ta, int tb, size_t m, size_t n, size_t k);  // packed leading dimensions, batch 1
 void gemm_f64(const GemmDesc& g, const double* A, const double* B, double* C, hipStream_t s);
+// Two independent products in one launch when both have small tile grids (never split over K), else two launches.
+void gemm_f64_pair(const GemmDesc& g0, const double* A0, const double* B0, double* C0, const GemmDesc& g1, const double* A1,
+                   const double* B1, double* C1, hipStream_t s);
 void gemm_f32(const GemmDesc& g, const float* A, const float* B, float* C, hipStream_t s);
 // BF16 products with FP32 accumulation; operands are stored as FP32 and rounded to bf16 when staged in LDS.
 void gemm_bf16(const GemmDesc& g, const float* A, const float* B, float* C, hipStream_t s);
