@@ -395,7 +395,7 @@ void launch_x4_pair(const GemmDesc& d0, const T* A0, const T* B0, T* C0, const G
 		bytes += (double)sizeof(T) * g.batch * ((double)g.m * g.k + (double)g.k * g.n + (g.beta != 0.0 ? 2.0 : 1.0) * g.m * g.n);
 	}
 	pg.nb0 = (int)blocks[0];
-	std::string tag = sizeof(T) == 8 ? "gemm_x4_kernel<double,32,32> pair" : "gemm_x4_kernel<float,32,32> pair";
+	std::string tag = sizeof(T) == 8 ? "gemm_x4_pair_kernel<double,32,32>" : "gemm_x4_pair_kernel<float,32,32>";
 	if (prof_shapes()) tag += shape_suffix(d0) + " +" + shape_suffix(d1);
 	auto kern = gemm_x4_pair_kernel<T, 32, 32, 2, 2>;
 	allow_lds(kern, lds);
