@@ -170,28 +170,51 @@ DTensor op_right(const DTensor& R, const DTensor& X, const DTensor& Acore) {
 	return op_left_am(R, reshuffle(X, {2, 1, 0}), reshuffle(Acore, kAmPermMirrored));
 }
 // Lb'(β',c) = Σ Lb(β,a)·b(β,n,β')·X(a,n,c)
-// Contracted over a first (Y = Lb·X, K = a) and then over (β, n) (K = β·n): both products have short K, where
-// Lb·b first would leave a K = a·n reduction onto a β'×c result.
-DTensor rhs_left(const DTensor& Lb, const DTensor& b, const DTensor& X) {
-	DTensor Y = contract(Lb, false, X, false, 1);  // (β, n, c)
-	return contract(b, true, Y, false, 2);         // (β', c)
+// The right-hand-side half of a stack push: two products, (g1: A1·B1 → C1) then (g2: A2·B2 → C2).
+struct RhsPush {
+	GemmDesc g1, g2;
+	const double *A1 = nullptr, *B1 = nullptr, *A2 = nullptr, *B2 = nullptr;
+	double *C1 = nullptr, *C2 = nullptr;
+};
+// Left push of the right-hand-side stack, Lb'(β',c) = Σ Lb(β,a)·b(β,n,β')·X(a,n,c): Y = Lb·X (K = a), then
+// bᵀ·Y (K = β·n). Both products have short K, where Lb·b first would leave a K = a·n reduction onto a β'×c
+// result. `Y` holds β·n·c elements.
+RhsPush rhs_push_left(const DTensor& Lb, const DTensor& bc, const DTensor& X, double* Y, DTensor& out) {
+	const size_t a = X.dims[0], n = X.dims[1], c = X.dims[2], beta = Lb.dims[0], beta2 = bc.dims[2];
+	XTT_REQUIRE(Lb.dims[1] == a && bc.dims[0] == beta && bc.dims[1] == n, "stack push: right-hand side shape mismatch");
+	out = DTensor({beta2, c});
+	RhsPush r;
+	r.g1 = gemm_desc(0, 0, beta, n * c, a); r.A1 = Lb.data(); r.B1 = X.data(); r.C1 = Y;
+	r.g2 = gemm_desc(1, 0, beta2, c, beta * n); r.A2 = bc.data(); r.B2 = Y; r.C2 = out.data();
+	return r;
+}
+// Right push, Rb'(β,a) = Σ b(β,n,β')·X(a,n,c)·Rb(β',c), on the unmirrored cores: Z(n,β',a) = Rb·X(:,n,:)ᵀ for
+// every n (one batched product), then Rb' = b·Z with b as (β × n·β') and Z as (n·β' × a). `Z` holds n·β'·a.
+RhsPush rhs_push_right(const DTensor& Rb, const DTensor& bc, const DTensor& X, double* Z, DTensor& out) {
+	const size_t a = X.dims[0], n = X.dims[1], c = X.dims[2], beta = bc.dims[0], beta2 = bc.dims[2];
+	XTT_REQUIRE(Rb.dims[0] == beta2 && Rb.dims[1] == c && bc.dims[1] == n, "stack push: right-hand side shape mismatch");
+	out = DTensor({beta, a});
+	RhsPush r;
+	r.g1 = gemm_desc(0, 1, beta2, a, c);
+	r.g1.ldb = n * c; r.g1.sA = 0; r.g1.sB = (long long)c; r.g1.sC = (long long)(beta2 * a); r.g1.batch = n;
+	r.A1 = Rb.data(); r.B1 = X.data(); r.C1 = Z;
+	r.g2 = gemm_desc(0, 0, beta, a, n * beta2); r.A2 = bc.data(); r.B2 = Z; r.C2 = out.data();
+	return r;
 }
 // Both stacks of one site pushed together: the operator stack (op_left_am, when L is given) and the right-hand
-// side stack (rhs_left); their independent products share launches (3 instead of 5).
-void push_both(const DTensor* L, const DTensor& X, const DTensor* Am, const DTensor& Lb, const DTensor& bc, DTensor* opOut,
-               DTensor& rhsOut) {
-	if (!L) { rhsOut = rhs_left(Lb, bc, X); return; }
+// side stack; their independent products share launches (3 instead of 5).
+void push_both(const DTensor* L, const DTensor& X, const DTensor* Am, const RhsPush& rp, DTensor* opOut) {
+	if (!L) {
+		gemm_f64(rp.g1, rp.A1, rp.B1, rp.C1, stream());
+		gemm_f64(rp.g2, rp.A2, rp.B2, rp.C2, stream());
+		return;
+	}
 	const size_t a = X.dims[0], n = X.dims[1], c = X.dims[2], P = Am->dims[2], Q = Am->dims[1];
-	const size_t beta = Lb.dims[0], beta2 = bc.dims[2];
 	XTT_REQUIRE(L->dims[0] == a && L->dims[2] == a && L->dims[1] == P && Am->dims[0] == n && Am->dims[3] == n,
 	            "stack push: shape mismatch");
-	XTT_REQUIRE(Lb.dims[1] == a && bc.dims[0] == beta && bc.dims[1] == n, "stack push: right-hand side shape mismatch");
-	DBuf<double> T1(a * P * n * c), T2(a * n * Q * c), Y(beta * n * c);
-	gemm_f64_pair(gemm_desc(0, 0, a * P, n * c, a), L->data(), X.data(), T1.get(),     // T1 = L·X
-	              gemm_desc(0, 0, beta, n * c, a), Lb.data(), X.data(), Y.get(), stream());  // Y = Lb·X, as rhs_left
-	rhsOut = DTensor({beta2, c});
-	gemm_f64_pair(stage2_desc(a, n, c, P, Q, nullptr), Am->data(), T1.get(), T2.get(),
-	              gemm_desc(1, 0, beta2, c, beta * n), bc.data(), Y.get(), rhsOut.data(), stream());  // bᵀ·Y
+	DBuf<double> T1(a * P * n * c), T2(a * n * Q * c);
+	gemm_f64_pair(gemm_desc(0, 0, a * P, n * c, a), L->data(), X.data(), T1.get(), rp.g1, rp.A1, rp.B1, rp.C1, stream());
+	gemm_f64_pair(stage2_desc(a, n, c, P, Q, nullptr), Am->data(), T1.get(), T2.get(), rp.g2, rp.A2, rp.B2, rp.C2, stream());
 	*opOut = DTensor({c, Q, c});
 	gemm_f64(gemm_desc(1, 0, c, Q * c, a * n), X.data(), T2.get(), opOut->data(), stream());
 }
@@ -276,8 +299,8 @@ struct ALSData {
 	const TT& B() const { return *Bp; }
 	const TT& rhs() const { return *rhsp; }
 	// Per core k, filled on first use: the operator core permuted for left pushes and local operators (kAmPerm)
-	// and for right pushes (kAmPermMirrored), and the right-hand side core mirrored to (β', n, β).
-	std::vector<DTensor> amLeft, amRight, rhsMirror;
+	// and for right pushes (kAmPermMirrored).
+	std::vector<DTensor> amLeft, amRight;
 	TT& x;
 	const TT& b;
 	dims_t targetRank;
@@ -314,7 +337,7 @@ struct ALSData {
 			Bp = A;
 			if (&b == &x) { rhsOwn = tt_copy(b); rhsp = &rhsOwn; } else rhsp = &b;
 		}
-		amLeft.resize(x.order()); amRight.resize(x.order()); rhsMirror.resize(x.order());
+		amLeft.resize(x.order()); amRight.resize(x.order());
 		prepare_x();
 		prepare_stacks();
 		curr = first;
@@ -380,22 +403,30 @@ struct ALSData {
 		if (amRight[k].dims.empty()) amRight[k] = reshuffle(B().cores[k], kAmPermMirrored);
 		return amRight[k];
 	}
-	const DTensor& rhs_mirror(size_t k) {
-		if (rhsMirror[k].dims.empty()) rhsMirror[k] = reshuffle(rhs().cores[k], {2, 1, 0});
-		return rhsMirror[k];
-	}
 	// stack pushes of site i (operator and right-hand side); the right push mirrors x's core once for both
 	void push_left(size_t i) {
+		const DTensor& X = x.cores[i];
+		DBuf<double> Y(rhsL.back().dims[0] * X.dims[1] * X.dims[2]);
 		DTensor op, rb;
-		push_both(A ? &opL.back() : nullptr, x.cores[i], A ? &am_left(i) : nullptr, rhsL.back(), rhs().cores[i], &op, rb);
-		if (A) { opL.push_back(std::move(op)); count_env(x.cores[i], B().cores[i]); }
+		const RhsPush rp = rhs_push_left(rhsL.back(), rhs().cores[i], X, Y.get(), rb);
+		push_both(A ? &opL.back() : nullptr, X, A ? &am_left(i) : nullptr, rp, &op);
+		if (A) { opL.push_back(std::move(op)); count_env(X, B().cores[i]); }
 		rhsL.push_back(std::move(rb));
 	}
+	// the operator push runs as the left push on the mirrored site; the right-hand side needs no mirrored cores
 	void push_right(size_t i) {
-		const DTensor Xr = reshuffle(x.cores[i], {2, 1, 0});
+		const DTensor& X = x.cores[i];
+		DBuf<double> Z(X.dims[1] * rhsR.back().dims[0] * X.dims[0]);
 		DTensor op, rb;
-		push_both(A ? &opR.back() : nullptr, Xr, A ? &am_right(i) : nullptr, rhsR.back(), rhs_mirror(i), &op, rb);
-		if (A) { opR.push_back(std::move(op)); count_env(x.cores[i], B().cores[i]); }
+		const RhsPush rp = rhs_push_right(rhsR.back(), rhs().cores[i], X, Z.get(), rb);
+		if (A) {
+			const DTensor Xr = reshuffle(X, {2, 1, 0});
+			push_both(&opR.back(), Xr, &am_right(i), rp, &op);
+			opR.push_back(std::move(op));
+			count_env(X, B().cores[i]);
+		} else {
+			push_both(nullptr, X, nullptr, rp, nullptr);
+		}
 		rhsR.push_back(std::move(rb));
 	}
 

@@ -132,7 +132,9 @@ void hh_qr(const double* A, size_t m, size_t n, bool trans, double* Q, double* R
 void hh_qc(const double* A, size_t m, size_t n, double* Q, double* C, int* jpvt, int* rank, hipStream_t s);
 // One-sided Jacobi SVD of a row-major m×n matrix with m >= n: U (m×n), S (n, descending), Vt (n×n).
 // Workspace: m·n + n·n + 2n doubles.
-void jacobi_svd(const double* A, size_t m, size_t n, double* U, double* S, double* Vt, double* ws, hipStream_t s);
+// transposedOut: U is written as Uᵀ (n×m) and Vt as V (n×n), row-major.
+void jacobi_svd(const double* A, size_t m, size_t n, double* U, double* S, double* Vt, double* ws, hipStream_t s,
+                bool transposedOut = false);
 
 // ---- ADF sampled-entry kernels (FP32 cores / values, FP64 norms) ---------------------------------------
 struct AdfDesc {

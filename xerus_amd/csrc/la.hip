@@ -223,7 +223,7 @@ __global__ void extract_r_kernel(const double* A, int m, int n, double* R) {
 // to an even column count np, V (column-major np×np) as the identity; every round rotates np/2 disjoint
 // column pairs, one thread group per pair. Sweeps stop when no pair needed a rotation.
 __global__ __launch_bounds__(QT) void jacobi_kernel(const double* A, int m, int n, int np, double* W, double* V, double* sv, int* ord,
-                                                    double* U, double* S, double* Vt) {
+                                                    double* U, double* S, double* Vt, int tout) {
 	const int tid = threadIdx.x;
 	__shared__ int rotated;
 	for (size_t e = tid; e < (size_t)m * np; e += QT) {
@@ -286,12 +286,12 @@ __global__ __launch_bounds__(QT) void jacobi_kernel(const double* A, int m, int 
 	for (size_t e = tid; e < (size_t)np * m; e += QT) {
 		const int j = (int)(e / m), r = (int)(e % m);
 		const int rk = ord[j];
-		if (rk < n) U[(size_t)r * n + rk] = sv[j] > 0 ? W[e] / sv[j] : 0.0;
+		if (rk < n) U[tout ? (size_t)rk * m + r : (size_t)r * n + rk] = sv[j] > 0 ? W[e] / sv[j] : 0.0;
 	}
 	for (size_t e = tid; e < (size_t)np * np; e += QT) {
 		const int j = (int)(e / np), c = (int)(e % np);
 		const int rk = ord[j];
-		if (rk < n && c < n) Vt[(size_t)rk * n + c] = V[e];
+		if (rk < n && c < n) Vt[tout ? (size_t)c * n + rk : (size_t)rk * n + c] = V[e];
 	}
 	for (int j = tid; j < np; j += QT) if (ord[j] < n) S[ord[j]] = sv[j];
 }
@@ -320,7 +320,8 @@ constexpr int JMAXM = 128, JMAXN = 64;
 
 // One-sided Jacobi for m <= 128, n <= 64 with W (m×np, column-major) and V (np×np) in LDS. Same round-robin
 // ordering, rotation formula and stopping rule as jacobi_kernel; the pair sums are DPP butterflies.
-__global__ __launch_bounds__(JT) void jacobi_lds_kernel(const double* A, int m, int n, int np, double* U, double* S, double* Vt) {
+// tout: U and Vt are written transposed (U as n×m, Vt as n×n holding V).
+__global__ __launch_bounds__(JT) void jacobi_lds_kernel(const double* A, int m, int n, int np, double* U, double* S, double* Vt, int tout) {
 	extern __shared__ double sm[];
 	double* W = sm;                       // m × np, column stride m
 	double* V = sm + (size_t)m * np;      // np × np
@@ -385,12 +386,12 @@ __global__ __launch_bounds__(JT) void jacobi_lds_kernel(const double* A, int m, 
 	for (int e = tid; e < np * m; e += JT) {
 		const int j = e / m, r = e % m;
 		const int rk = ord[j];
-		if (rk < n) U[(size_t)r * n + rk] = sv[j] > 0 ? W[e] / sv[j] : 0.0;
+		if (rk < n) U[tout ? (size_t)rk * m + r : (size_t)r * n + rk] = sv[j] > 0 ? W[e] / sv[j] : 0.0;
 	}
 	for (int e = tid; e < np * np; e += JT) {
 		const int j = e / np, c = e % np;
 		const int rk = ord[j];
-		if (rk < n && c < n) Vt[(size_t)rk * n + c] = V[e];
+		if (rk < n && c < n) Vt[tout ? (size_t)c * n + rk : (size_t)rk * n + c] = V[e];
 	}
 	for (int j = tid; j < np; j += JT) if (ord[j] < n) S[ord[j]] = sv[j];
 }
@@ -429,7 +430,7 @@ void extract_r(const double* A, size_t m, size_t n, double* R, hipStream_t s) {
 	XTT_LAUNCH_CHECK();
 }
 
-void jacobi_svd(const double* A, size_t m, size_t n, double* U, double* S, double* Vt, double* ws, hipStream_t s) {
+void jacobi_svd(const double* A, size_t m, size_t n, double* U, double* S, double* Vt, double* ws, hipStream_t s, bool transposedOut) {
 	XTT_REQUIRE(m >= n && n > 0, "jacobi_svd expects m >= n > 0");
 	const size_t np = n + (n & 1);
 	if (m <= (size_t)JMAXM && np <= (size_t)JMAXN) {
@@ -440,14 +441,14 @@ void jacobi_svd(const double* A, size_t m, size_t n, double* U, double* S, doubl
 		});
 		// one sweep's rotations, 6m + 6n flops per pair and n(n-1)/2 pairs (the sweep count is data dependent)
 		launch_timed("jacobi_lds_kernel", Work(3.0 * (m + np) * np * (np - 1), 8.0 * (2.0 * m * n + n + (double)n * n)), jacobi_lds_kernel, dim3(1), dim3(JT),
-		             (m * np + np * np) * sizeof(double), s, A, (int)m, (int)n, (int)np, U, S, Vt);
+		             (m * np + np * np) * sizeof(double), s, A, (int)m, (int)n, (int)np, U, S, Vt, (int)transposedOut);
 		return;
 	}
 	double* W = ws;
 	double* V = W + m * np;
 	double* sv = V + np * np;
 	int* ord = reinterpret_cast<int*>(sv + np);
-	hipLaunchKernelGGL(jacobi_kernel, dim3(1), dim3(QT), 0, s, A, (int)m, (int)n, (int)np, W, V, sv, ord, U, S, Vt);
+	hipLaunchKernelGGL(jacobi_kernel, dim3(1), dim3(QT), 0, s, A, (int)m, (int)n, (int)np, W, V, sv, ord, U, S, Vt, (int)transposedOut);
 	XTT_LAUNCH_CHECK();
 }
 

@@ -214,15 +214,14 @@ void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTenso
 	if (m > n || (m == n && !jacobiOnTranspose)) {
 		jacobi_svd(in.data(), m, n, u.get(), s.get(), vt.get(), ws.get(), stream());
 	} else {  // Aᵀ = U'·S·V'ᵀ  ⇒  A = V'·S·U'ᵀ
-		DBuf<double> at, u2(n * m), vt2(m * m);
+		DBuf<double> at;
 		if (!inTransposed) {
 			at.resize(n * m);
 			const size_t d2[2] = {m, n}, sh[2] = {1, 0};
 			permute_f64(2, d2, sh, in.data(), at.get(), stream());
 		}
-		jacobi_svd(inTransposed ? inTransposed : at.get(), n, m, u2.get(), s.get(), vt2.get(), ws.get(), stream());
-		{ const size_t d2[2] = {m, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, vt2.get(), u.get(), stream()); }
-		{ const size_t d2[2] = {n, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, u2.get(), vt.get(), stream()); }
+		// written transposed by the kernel: U = V' (m×m) and Vt = U'ᵀ (m×n)
+		jacobi_svd(inTransposed ? inTransposed : at.get(), n, m, vt.get(), s.get(), u.get(), ws.get(), stream(), true);
 	}
 	std::vector<double> sh(k);
 	read_back(sh.data(), s.get(), k * 8);
@@ -491,15 +490,14 @@ void svd_spec(DTensor& U, DTensor& Vt, DBuf<double>& sDev, const DTensor& in, si
 	if (m > n || (m == n && !jacobiOnTranspose)) {
 		jacobi_svd(in.data(), m, n, u.get(), s.get(), vt.get(), ws.get(), stream());
 	} else {
-		DBuf<double> at, u2(n * m), vt2(m * m);
+		DBuf<double> at;
 		if (!inTransposed) {
 			at.resize(n * m);
 			const size_t d2[2] = {m, n}, sh[2] = {1, 0};
 			permute_f64(2, d2, sh, in.data(), at.get(), stream());
 		}
-		jacobi_svd(inTransposed ? inTransposed : at.get(), n, m, u2.get(), s.get(), vt2.get(), ws.get(), stream());
-		{ const size_t d2[2] = {m, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, vt2.get(), u.get(), stream()); }
-		{ const size_t d2[2] = {n, m}, sh[2] = {1, 0}; permute_f64(2, d2, sh, u2.get(), vt.get(), stream()); }
+		// written transposed by the kernel: U = V' (m×m) and Vt = U'ᵀ (m×n)
+		jacobi_svd(inTransposed ? inTransposed : at.get(), n, m, vt.get(), s.get(), u.get(), ws.get(), stream(), true);
 	}
 	svd_rank_f64(s.get(), k, maxRank, eps, rs.slot((int)rank), stream());
 	take_svd_factors(U, Vt, u, vt, cat(sl(in.dims, 0, split), {rank}), cat({rank}, sl(in.dims, split, in.degree())), m, n, k, rank);
