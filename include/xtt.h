@@ -225,6 +225,12 @@ int xt_tt_sample_residual_f32(int d, const int* n, const int* r, const float* co
                               float* resid, double* sums, void* stream);
 int xt_tt_sample_adjoint_f32(int d, const int* n, const int* r, const float* cores, const uint32_t* pos, int P,
                              const float* seed, float* grads, void* stream);
+/* Both of the above for one gradient step of ADF (calculate_residual adf.cpp:290-312 followed by
+ * calculate_projected_gradient adf.cpp:314-396, here for all cores): resid[i] = x(position_i) − b[i], sums (host,
+ * nullable) = {Σ resid², Σ b²} in FP64, grads = Σ_i resid_i·∂x_i/∂G. One pass over the samples where the ranks are
+ * <= 8 (the same resid and grads as the two calls; sums to FP64 rounding), else the two kernels in turn. */
+int xt_tt_sample_residual_adjoint_f32(int d, const int* n, const int* r, const float* cores, const uint32_t* pos, int P,
+                                      const float* b, float* resid, double* sums, float* grads, void* stream);
 
 #ifdef __cplusplus
 }

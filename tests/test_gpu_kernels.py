@@ -236,3 +236,73 @@ def test_reshuffle(xe, oracle):
              ptr(dT := dev(T)), ptr(out))
         ref = oracle.reshuffle(T, sh)
         assert np.array_equal(host(out), ref.ravel())  # data movement: bit-exact
+
+
+def _sampled_step(d, n, r, cores, pos, vals, fused):
+    """One ADF gradient step through the raw sampled-entry entry points: the fused call, or the residual call
+    followed by the adjoint call seeded with that residual."""
+    from xerus_amd._lib import call
+    flat = dev(np.concatenate([c.ravel() for c in cores]), np.float32)
+    dpos, db = dev(pos, np.uint32), dev(vals, np.float32)
+    P = len(vals)
+    resid, grad = empty(P, np.float32), empty(flat.shape, np.float32)
+    ns, rs = (ctypes.c_int * d)(*n), (ctypes.c_int * (d + 1))(*r)
+    sums = (ctypes.c_double * 2)()
+    if fused:
+        call("xt_tt_sample_residual_adjoint_f32", d, ns, rs, ptr(flat), ptr(dpos), P, ptr(db), ptr(resid), sums, ptr(grad), None)
+    else:
+        call("xt_tt_sample_residual_f32", d, ns, rs, ptr(flat), ptr(dpos), P, ptr(db), ptr(resid), sums, None)
+        call("xt_tt_sample_adjoint_f32", d, ns, rs, ptr(flat), ptr(dpos), P, ptr(resid), ptr(grad), None)
+    return host(resid), host(grad), np.array(sums[:])
+
+
+@pytest.mark.parametrize("d,n,r,P", [(10, 10, 8, 20000), (4, 3, 2, 77), (6, 16, 8, 3001), (3, 4, 3, 1), (5, 6, 12, 2000)])
+def test_sampled_residual_adjoint_fused(xe, oracle, d, n, r, P):
+    """xt_tt_sample_residual_adjoint_f32 against the two calls it replaces: residuals and gradients bit-identical
+    (the fused kernel's forward walk takes the residual kernel's steps in its order), the FP64 sums to rounding;
+    the gradient against the oracle's ADF gradient (adf.cpp:314-396, Δ = −∂½Σ(x−b)²/∂G, pinned to FP64 finite
+    differences in test_oracle). Ranks above 8 take the two-kernel path."""
+    from helpers import SEED
+    from test_gpu_tt import twin_random
+    print(f"fused sampled step d={d} n={n} r={r} P={P}", flush=True)
+    _, xo = twin_random(xe, oracle, [n] * d, [r] * (d - 1))
+    _, trutho = twin_random(xe, oracle, [n] * d, [r] * (d - 1), seed=SEED + 2)
+    oracle.seed(SEED + 3)
+    mo = oracle.Measurements.random(P, [n] * d)
+    mo.measure(trutho)
+    cores = [c.astype(np.float32) for c in xo.cores()]
+    rk = [1] + [c.shape[2] for c in cores]
+    pos, vals = mo.positions(d), mo.values()
+    rf, gf, sf = _sampled_step(d, [n] * d, rk, cores, pos, vals, True)
+    rs, gs, ss = _sampled_step(d, [n] * d, rk, cores, pos, vals, False)
+    np.testing.assert_array_equal(rf, rs)
+    np.testing.assert_array_equal(gf, gs)
+    assert np.abs(sf - ss).max() <= 1e-12 * np.abs(ss).max()
+    go, _ = oracle.adf_gradient(oracle.TT.from_cores([c.astype(np.float64) for c in cores]), mo)
+    o = 0
+    for k in range(d):
+        g = gf[o: o + go[k].size].reshape(go[k].shape)
+        o += go[k].size
+        assert np.abs(g + go[k]).max() <= 3e-5 * max(np.abs(go[k]).max(), 1e-30), k
+
+
+def test_sampled_residual_adjoint_fused_out_of_range(xe, oracle):
+    """A position outside its mode: residual NaN and no gradient contribution, in the fused call as in the two."""
+    from helpers import SEED
+    from test_gpu_tt import twin_random
+    print("fused sampled step, out-of-range position", flush=True)
+    d, n, r, P = 5, 6, 4, 500
+    _, xo = twin_random(xe, oracle, [n] * d, [r] * (d - 1))
+    oracle.seed(SEED + 4)
+    mo = oracle.Measurements.random(P, [n] * d)
+    pos = mo.positions(d).copy()
+    pos[17, 2] = n
+    vals = np.random.default_rng(SEED).normal(size=P)
+    cores = [c.astype(np.float32) for c in xo.cores()]
+    rk = [1] + [c.shape[2] for c in cores]
+    rf, gf, sf = _sampled_step(d, [n] * d, rk, cores, pos, vals, True)
+    rs, gs, ss = _sampled_step(d, [n] * d, rk, cores, pos, vals, False)
+    assert np.isnan(rf[17]) and np.isnan(rs[17]) and np.isnan(sf[0]) and np.isnan(ss[0])
+    np.testing.assert_array_equal(rf, rs)
+    np.testing.assert_array_equal(gf, gs)
+    assert np.all(np.isfinite(gf))

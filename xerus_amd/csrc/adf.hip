@@ -274,8 +274,14 @@ __host__ __device__ inline size_t fast_grad_floats(int d, int sumN) {
 
 typedef float gf4 __attribute__((ext_vector_type(4)));
 
+// With RES, the residual is computed here instead of read: the forward walkers take one more step (the last
+// core) to the sampled value x_s, the same operations in the same order as adf_residual_fast_kernel, and use
+// res_s = x_s − b_s (written to residOut; NaN for positions out of range, whose gradient contribution is 0). The
+// block's FP64 sums Σ res² and Σ b² go to normPart[2·block], [2·block + 1].
+template <bool RES>
 __global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const float* Gp, const long long* padOff, const uint32_t* pos,
-                                                                const float* resid, float* partials, int kSel) {
+                                                                const float* resid, float* partials, int kSel, const float* values,
+                                                                float* residOut, double* normPart) {
 	extern __shared__ __align__(16) float fsm[];
 	__shared__ int sN[FD], sRk[FD + 1], sSlot[FD + 1];
 	__shared__ long long sOff[FD], sPo[FD];
@@ -310,6 +316,7 @@ __global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const
 		for (int t = 0; t < GNT; ++t) acc[q][t] = gf4{0.f, 0.f, 0.f, 0.f};
 	const int per = (a.P + gridDim.x - 1) / gridDim.x;
 	const int s0 = blockIdx.x * per, s1 = min(a.P, s0 + per);
+	double rr = 0, bb = 0;  // RES: this lane's residual sums (forward walkers only)
 	__syncthreads();
 	for (int base = s0; base < s1; base += FS) {
 		const int cnt = min(FS, s1 - base);
@@ -325,7 +332,7 @@ __global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const
 			for (int k = 0; k < FD; ++k) valid = valid && (k >= d || pk[k] < (uint32_t)sN[k]);
 			if (fwd) {
 				for (int k = 0; k < d; ++k) Is[k * FS + s] = valid ? (int)pk[k] : 0;
-				Rs[s] = valid ? resid[base + s] : 0.f;
+				if (!RES) Rs[s] = valid ? resid[base + s] : 0.f;
 			}
 			float v[FR];
 #pragma unroll
@@ -337,6 +344,19 @@ __global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const
 						mul_row8(v, Gs + sPo[k] + (size_t)(valid ? pk[k] : 0) * PB);
 #pragma unroll
 						for (int c = 0; c < FR; ++c) Fs[(k * FR + c) * FSP + s] = v[c];
+					}
+				}
+				if (RES) {
+					mul_row8(v, Gs + sPo[d - 1] + (size_t)(valid ? pk[d - 1] : 0) * PB);
+					if (s < cnt) {
+						const float bi = values[base + s];
+						const float r = (valid ? v[0] : NAN) - bi;
+						residOut[base + s] = r;
+						rr += (double)r * r;
+						bb += (double)bi * bi;
+						Rs[s] = valid ? r : 0.f;
+					} else {
+						Rs[s] = 0.f;
 					}
 				}
 			} else {
@@ -397,6 +417,19 @@ __global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const
 			}
 		}
 		__syncthreads();
+	}
+	if (RES) {  // the block's residual sums: wave sums, then the waves in order
+		__shared__ double sNorm[2 * FW];
+		rr = wsum64(rr);
+		bb = wsum64(bb);
+		if (lane == 0) { sNorm[2 * w] = rr; sNorm[2 * w + 1] = bb; }
+		__syncthreads();
+		if (tid == 0) {
+			double t0 = 0, t1 = 0;
+			for (int q = 0; q < FW; ++q) { t0 += sNorm[2 * q]; t1 += sNorm[2 * q + 1]; }
+			normPart[2 * blockIdx.x] = t0;
+			normPart[2 * blockIdx.x + 1] = t1;
+		}
 	}
 	// the block's accumulators, written in the gradient layout: register r of tile t holds
 	// G_k[a = 4·lk + r, j = (16t + li)/8, c = li%8]
@@ -544,6 +577,15 @@ __global__ __launch_bounds__(256) void adf_grad_reduce_kernel(const T* partials,
 	if (q == 0 && e < gradSize) grad[e] = (part[0][o] + part[1][o]) + (part[2][o] + part[3][o]);
 }
 
+// norms[0..1] = Σ_b part[2b], Σ_b part[2b + 1]: lane l sums blocks l, l + 64, … in order, then a fixed tree.
+__global__ __launch_bounds__(64) void adf_norm_reduce_kernel(const double* part, int nblocks, double* norms) {
+	double s0 = 0, s1 = 0;
+	for (int b = threadIdx.x; b < nblocks; b += 64) { s0 += part[2 * b]; s1 += part[2 * b + 1]; }
+	s0 = wsum64(s0);
+	s1 = wsum64(s1);
+	if (threadIdx.x == 0) { norms[0] = s0; norms[1] = s1; }
+}
+
 GradLayout grad_layout(int d, int maxR, int gradSize, size_t E) {
 	GradLayout L{};
 	for (int SB = 256; SB >= 16; SB >>= 1) {
@@ -647,6 +689,44 @@ void adf_residual(const AdfDesc& a, const T* cores, const uint32_t* pos, const T
 
 int adf_grad_blocks(int P) { return std::max(1, std::min(512, (P + 255) / 256)); }
 
+namespace {
+void allow_fast_grad_lds() {
+	static std::once_flag once;
+	std::call_once(once, [&] {
+		for (const void* f : {reinterpret_cast<const void*>(adf_grad_fast_kernel<false>), reinterpret_cast<const void*>(adf_grad_fast_kernel<true>)})
+			XTT_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+	});
+}
+}  // namespace
+
+bool adf_residual_gradient_fused_ok(const AdfDesc& a) {
+	return fast_ok(a) && a.d >= 2 && a.maxN * FR <= GNT * 16 && fast_grad_floats(a.d, a.sumN) * sizeof(float) <= 150 * 1024;
+}
+
+// resid = x(pos) − values, norms = {‖resid‖², ‖values‖²} and grad (the adjoint with seed resid) in one gradient
+// launch where the fast kernel applies; otherwise adf_residual followed by adf_gradient.
+void adf_residual_gradient_f32(const AdfDesc& a, const float* cores, const uint32_t* pos, const float* values, float* resid, double* norms,
+                               float* grad, float* partials, double* normPart, unsigned* counter, hipStream_t s) {
+	if (!adf_residual_gradient_fused_ok(a)) {
+		adf_residual<float>(a, cores, pos, values, resid, norms, normPart, counter, s);
+		adf_gradient<float>(a, cores, pos, resid, grad, partials, -1, s);
+		return;
+	}
+	Padded pc = pad_cores(a, cores, s);
+	allow_fast_grad_lds();
+	const int blocks = std::min(adf_grad_blocks(a.P), 256);
+	// positions and values of every sample, residuals out, the cores in and the gradient out
+	launch_timed("adf_grad_fast_kernel<float> residual", Work(0.0, (double)a.P * (4.0 * a.d + 8.0) + 8.0 * a.gradSize),
+	             adf_grad_fast_kernel<true>, dim3(blocks), dim3(FW * 64), fast_grad_floats(a.d, a.sumN) * sizeof(float), s, a,
+	             (const float*)pc.Gp.get(), (const long long*)pc.off.get(), pos, (const float*)nullptr, partials, -1, values, resid,
+	             normPart);
+	hipLaunchKernelGGL(adf_grad_reduce_kernel<float>, dim3((a.gradSize + 63) / 64), dim3(256), 0, s, (const float*)partials, blocks,
+	                   a.gradSize, grad);
+	XTT_LAUNCH_CHECK();
+	hipLaunchKernelGGL(adf_norm_reduce_kernel, dim3(1), dim3(64), 0, s, (const double*)normPart, blocks, norms);
+	XTT_LAUNCH_CHECK();
+}
+
 template <class T>
 void adf_gradient(const AdfDesc& a, const T* cores, const uint32_t* pos, const T* resid, T* grad, T* partials, int kSel,
                   hipStream_t s) {
@@ -655,15 +735,12 @@ void adf_gradient(const AdfDesc& a, const T* cores, const uint32_t* pos, const T
 	const size_t fastLds = fast_grad_floats(a.d, a.sumN) * sizeof(float);
 	if (std::is_same<T, float>::value && fast_ok(a) && a.d >= 2 && a.maxN * FR <= GNT * 16 && fastLds <= 150 * 1024) {
 		Padded pc = pad_cores(a, reinterpret_cast<const float*>(cores), s);
-		static std::once_flag once;
-		std::call_once(once, [&] {
-			XTT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(adf_grad_fast_kernel), hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-		});
+		allow_fast_grad_lds();
 		const int blocks = std::min(adf_grad_blocks(a.P), 256);
 		// positions and residuals of every sample, the cores in and the gradient out
-		launch_timed("adf_grad_fast_kernel<float>", Work(0.0, (double)a.P * (4.0 * a.d + 4.0) + 8.0 * a.gradSize), adf_grad_fast_kernel, dim3(blocks), dim3(FW * 64), fastLds, s, a,
+		launch_timed("adf_grad_fast_kernel<float>", Work(0.0, (double)a.P * (4.0 * a.d + 4.0) + 8.0 * a.gradSize), adf_grad_fast_kernel<false>, dim3(blocks), dim3(FW * 64), fastLds, s, a,
 		             (const float*)pc.Gp.get(), (const long long*)pc.off.get(), pos, reinterpret_cast<const float*>(resid),
-		             reinterpret_cast<float*>(partials), kSel);
+		             reinterpret_cast<float*>(partials), kSel, (const float*)nullptr, (float*)nullptr, (double*)nullptr);
 		hipLaunchKernelGGL(adf_grad_reduce_kernel<T>, dim3((a.gradSize + 63) / 64), dim3(256), 0, s, partials, blocks, a.gradSize, grad);
 		XTT_LAUNCH_CHECK();
 		return;

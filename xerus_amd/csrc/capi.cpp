@@ -591,5 +591,27 @@ int xt_tt_sample_adjoint_f32(int d, const int* n, const int* r, const float* cor
 		XTT_HIP(hipStreamSynchronize(s));
 	});
 }
+int xt_tt_sample_residual_adjoint_f32(int d, const int* n, const int* r, const float* cores, const uint32_t* pos, int P,
+                                      const float* b, float* resid, double* sums, float* grads, void* stream) {
+	return guard([&] {
+		NN(n); NN(r); NN(cores); NN(resid); NN(grads); XTT_REQUIRE(P == 0 || (pos && b), "pos/b must not be NULL");
+		hipStream_t s = xtt::pick(stream);
+		const auto held = raw_desc(d, n, r, P, s);
+		const xtt::AdfDesc& desc = held->desc;
+		if (P == 0) {
+			if (sums) sums[0] = sums[1] = 0;
+			XTT_HIP(hipMemsetAsync(grads, 0, (size_t)desc.gradSize * 4, s));
+			XTT_HIP(hipStreamSynchronize(s));
+			return;
+		}
+		xtt::DBuf<double> norms(2), normPart(2 * 1024);
+		xtt::DBuf<unsigned> counter(1);
+		xtt::DBuf<float> partials((size_t)xtt::adf_grad_blocks(P) * desc.gradSize);
+		XTT_HIP(hipMemsetAsync(counter.get(), 0, 4, s));
+		xtt::adf_residual_gradient_f32(desc, cores, pos, b, resid, norms.get(), grads, partials.get(), normPart.get(), counter.get(), s);
+		if (sums) xtt::read_back(sums, norms.get(), 16, s);
+		else XTT_HIP(hipStreamSynchronize(s));
+	});
+}
 
 }  // extern "C"

@@ -158,6 +158,11 @@ int adf_grad_blocks(int P);
 template <class T>
 void adf_gradient(const AdfDesc& a, const T* cores, const uint32_t* pos, const T* resid, T* grad, T* partials, int kSel,
                   hipStream_t s);
+// resid = x(pos) − values and norms = {‖resid‖², ‖values‖²} (as adf_residual) together with grad = the adjoint
+// with seed resid (as adf_gradient, every core). `partials` holds adf_grad_blocks(P)·gradSize floats, `normPart`
+// 2·1024 doubles, `counter` one zeroed unsigned (used only when the fused kernel does not apply).
+void adf_residual_gradient_f32(const AdfDesc& a, const float* cores, const uint32_t* pos, const float* values, float* resid, double* norms,
+                               float* grad, float* partials, double* normPart, unsigned* counter, hipStream_t s);
 // Rank-one measurements (U: P × sumN, measurement i's vectors u_{i,0..d-1} concatenated):
 // value_i = x(u_i) with core kSel replaced by D (kSel < 0: none); resid_i = value_i − b_i (values/resid may be
 // null: b = 0 / not stored); norms2 = {Σ resid², Σ b²}. `partials` holds 2·r1_blocks(P) doubles.
