@@ -562,19 +562,25 @@ __global__ __launch_bounds__(AB) void adf_grad_kernel(AdfDesc a, const T* cores,
 	for (int e = tid; e < a.gradSize; e += AB) dst[e] = acc[e];
 }
 
-// grad[e] = Σ_b partials[b][e]: 64 entries per block, four threads per entry each summing every fourth block
-// partial, then the four sums in a fixed order (deterministic).
+// grad[e] = Σ_b partials[b][e]: RE entries per block, sixteen threads per entry each summing every sixteenth
+// block partial, then the sixteen sums by a fixed pairwise tree (deterministic).
+constexpr int RE = 16;
 template <class T>
 __global__ __launch_bounds__(256) void adf_grad_reduce_kernel(const T* partials, int nblocks, int gradSize, T* grad) {
-	__shared__ T part[4][64];
-	const int o = threadIdx.x & 63, q = threadIdx.x >> 6;
-	const int e = blockIdx.x * 64 + o;
+	__shared__ T part[16][RE + 1];
+	const int o = threadIdx.x % RE, q = threadIdx.x / RE;
+	const int e = blockIdx.x * RE + o;
 	T s = T(0);
 	if (e < gradSize)
-		for (int b = q; b < nblocks; b += 4) s += partials[(size_t)b * gradSize + e];
+		for (int b = q; b < nblocks; b += 16) s += partials[(size_t)b * gradSize + e];
 	part[q][o] = s;
 	__syncthreads();
-	if (q == 0 && e < gradSize) grad[e] = (part[0][o] + part[1][o]) + (part[2][o] + part[3][o]);
+	if (q == 0 && e < gradSize) {
+		T t[8];
+#pragma unroll
+		for (int i = 0; i < 8; ++i) t[i] = part[2 * i][o] + part[2 * i + 1][o];
+		grad[e] = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+	}
 }
 
 // norms[0..1] = Σ_b part[2b], Σ_b part[2b + 1]: lane l sums blocks l, l + 64, … in order, then a fixed tree.
@@ -720,7 +726,7 @@ void adf_residual_gradient_f32(const AdfDesc& a, const float* cores, const uint3
 	             adf_grad_fast_kernel<true>, dim3(blocks), dim3(FW * 64), fast_grad_floats(a.d, a.sumN) * sizeof(float), s, a,
 	             (const float*)pc.Gp.get(), (const long long*)pc.off.get(), pos, (const float*)nullptr, partials, -1, values, resid,
 	             normPart);
-	hipLaunchKernelGGL(adf_grad_reduce_kernel<float>, dim3((a.gradSize + 63) / 64), dim3(256), 0, s, (const float*)partials, blocks,
+	hipLaunchKernelGGL(adf_grad_reduce_kernel<float>, dim3((a.gradSize + RE - 1) / RE), dim3(256), 0, s, (const float*)partials, blocks,
 	                   a.gradSize, grad);
 	XTT_LAUNCH_CHECK();
 	hipLaunchKernelGGL(adf_norm_reduce_kernel, dim3(1), dim3(64), 0, s, (const double*)normPart, blocks, norms);
@@ -741,7 +747,7 @@ void adf_gradient(const AdfDesc& a, const T* cores, const uint32_t* pos, const T
 		launch_timed("adf_grad_fast_kernel<float>", Work(0.0, (double)a.P * (4.0 * a.d + 4.0) + 8.0 * a.gradSize), adf_grad_fast_kernel<false>, dim3(blocks), dim3(FW * 64), fastLds, s, a,
 		             (const float*)pc.Gp.get(), (const long long*)pc.off.get(), pos, reinterpret_cast<const float*>(resid),
 		             reinterpret_cast<float*>(partials), kSel, (const float*)nullptr, (float*)nullptr, (double*)nullptr);
-		hipLaunchKernelGGL(adf_grad_reduce_kernel<T>, dim3((a.gradSize + 63) / 64), dim3(256), 0, s, partials, blocks, a.gradSize, grad);
+		hipLaunchKernelGGL(adf_grad_reduce_kernel<T>, dim3((a.gradSize + RE - 1) / RE), dim3(256), 0, s, partials, blocks, a.gradSize, grad);
 		XTT_LAUNCH_CHECK();
 		return;
 	}
@@ -752,7 +758,7 @@ void adf_gradient(const AdfDesc& a, const T* cores, const uint32_t* pos, const T
 	const int blocks = adf_grad_blocks(a.P);
 	launch_timed(sizeof(T) == 8 ? "adf_grad_kernel<double>" : "adf_grad_kernel<float>", 0.0, adf_grad_kernel<T>, dim3(blocks),
 	             dim3(AB), lds, s, a, cores, pos, resid, partials, L, kSel);
-	hipLaunchKernelGGL(adf_grad_reduce_kernel<T>, dim3((a.gradSize + 63) / 64), dim3(256), 0, s, partials, blocks, a.gradSize, grad);
+	hipLaunchKernelGGL(adf_grad_reduce_kernel<T>, dim3((a.gradSize + RE - 1) / RE), dim3(256), 0, s, partials, blocks, a.gradSize, grad);
 	XTT_LAUNCH_CHECK();
 }
 
