@@ -607,7 +607,7 @@ int xt_tt_sample_residual_adjoint_f32(int d, const int* n, const int* r, const f
 		xtt::DBuf<double> norms(2), normPart(2 * 1024);
 		xtt::DBuf<unsigned> counter(1);
 		xtt::DBuf<float> partials((size_t)xtt::adf_grad_blocks(P) * desc.gradSize);
-		XTT_HIP(hipMemsetAsync(counter.get(), 0, 4, s));
+		if (!xtt::adf_residual_gradient_fused_ok(desc)) XTT_HIP(hipMemsetAsync(counter.get(), 0, 4, s));  // two-kernel path only
 		xtt::adf_residual_gradient_f32(desc, cores, pos, b, resid, norms.get(), grads, partials.get(), normPart.get(), counter.get(), s);
 		if (sums) xtt::read_back(sums, norms.get(), 16, s);
 		else XTT_HIP(hipStreamSynchronize(s));

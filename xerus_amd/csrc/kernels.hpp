@@ -161,6 +161,8 @@ void adf_gradient(const AdfDesc& a, const T* cores, const uint32_t* pos, const T
 // resid = x(pos) − values and norms = {‖resid‖², ‖values‖²} (as adf_residual) together with grad = the adjoint
 // with seed resid (as adf_gradient, every core). `partials` holds adf_grad_blocks(P)·gradSize floats, `normPart`
 // 2·1024 doubles, `counter` one zeroed unsigned (used only when the fused kernel does not apply).
+// whether adf_residual_gradient_f32 runs as one gradient launch (else the two kernels in turn)
+bool adf_residual_gradient_fused_ok(const AdfDesc& a);
 void adf_residual_gradient_f32(const AdfDesc& a, const float* cores, const uint32_t* pos, const float* values, float* resid, double* norms,
                                float* grad, float* partials, double* normPart, unsigned* counter, hipStream_t s);
 // Rank-one measurements (U: P × sumN, measurement i's vectors u_{i,0..d-1} concatenated):
