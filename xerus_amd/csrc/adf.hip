@@ -130,7 +130,7 @@ __global__ __launch_bounds__(AB) void adf_residual_kernel(AdfDesc a, const T* co
 // (a, ·, c) of every core and adds res_s·F_{k-1}[s][a]·B_{k+1}[s][c] to its own accumulator (k, i_k(s)) in the
 // wave's LDS region: every address has one owner lane, so the adds need no ordering between lanes and the
 // result is deterministic (samples in order, waves summed in order, blocks by the reduce kernel).
-constexpr int FW = 8;        // waves per gradient block (two per SIMD); the residual kernel uses AB threads
+constexpr int FW = 16;       // waves per gradient block (four per SIMD); the residual kernel uses AB threads
 constexpr int FR = 8;        // rank bound
 constexpr int PB = FR * FR + 4;  // floats per padded 8×8 block: the extra 4 spread blocks over the LDS banks
 constexpr int FD = 16;       // order bound
@@ -250,9 +250,10 @@ __global__ __launch_bounds__(AB) void adf_residual_fast_kernel(AdfDesc a, const 
 	}
 }
 
-// Gradient: one block of FW = 8 waves takes rounds of FS = 128 samples. Phase 1: waves 0-1 walk the left
+// Gradient: one block of FW = 16 waves takes rounds of FS = 128 samples. Phase 1: waves 0-1 walk the left
 // stacks F_k of the round's samples (one per lane), waves 2-3 the right stacks B_k, from the LDS copy of the
-// padded cores, into the block's staging area. Phase 2: wave w owns cores k ≡ w (mod 8) and adds the round's
+// padded cores, into the block's staging area. Phase 2: wave w owns core k = w (orders up to 16: one core per wave,
+// so the latency-bound accumulations of different cores interleave on a SIMD) and adds the round's
 // contribution to its gradient with MFMA: G_k[a, j, c] += Σ_{s: i_k(s) = j} (res_s·F_{k-1}[s][a]) · B_{k+1}[s][c].
 // The wave first counting-sorts the round's samples by i_k (ballot / mbcnt per bin, each bin padded to a
 // multiple of 4), so every v_mfma_f32_16x16x4_f32 step takes 4 samples of one bin and updates only that bin's
