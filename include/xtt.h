@@ -43,6 +43,8 @@ int xt_free(void* device_ptr);
 int xt_memcpy_htod(void* device_dst, const void* host_src, size_t bytes);
 int xt_memcpy_dtoh(void* host_dst, const void* device_src, size_t bytes);
 int xt_release_cache(void);
+/* device bytes held by live library buffers and by the allocator's cache of freed blocks (either nullable) */
+int xt_cache_bytes(size_t* in_use, size_t* cached);
 /* Launch profiler (no counterpart in xerus; replaces its XERUS_PA_START/END timers, misc/performanceAnalysis.h:33-56):
  * while enabled, GEMM, Householder, Jacobi and ADF launches get a HIP event pair handed to the dispatch on their
  * own stream. xt_profile_read synchronises the device, returns per-kernel (tag, launches, total ms, algorithmic
@@ -94,6 +96,16 @@ int xt_dot(const double* x, const double* y, size_t n, double* result);
 int xt_tt_new(int is_operator, size_t ndims, const size_t* dims, xt_tt* out);
 /* TTNetwork::random(dims, ranks) with the shared engine (ttNetwork.h:128-155) */
 int xt_tt_random(int is_operator, size_t ndims, const size_t* dims, const size_t* ranks, xt_tt* out);
+/* TTNetwork(const Tensor&, eps, maxRanks) (ttNetwork.cpp:112-160): TT-SVD of a dense host tensor (row-major; an
+ * operator's dims are [n_0..n_{d-1}, m_0..m_{d-1}]); max_ranks NULL = no cap (xerus's default eps is 8·DBL_EPSILON) */
+int xt_tt_from_tensor(int is_operator, size_t ndims, const size_t* dims, const double* host_data, double eps,
+                      const size_t* max_ranks, xt_tt* out);
+int xt_tt_ones(int is_operator, size_t ndims, const size_t* dims, xt_tt* out);       /* TTNetwork::ones, ttNetwork.cpp:170-191 */
+int xt_tt_identity(size_t ndims, const size_t* dims, xt_tt* out);                    /* TTOperator::identity, :194-221 */
+int xt_tt_kronecker(int is_operator, size_t ndims, const size_t* dims, xt_tt* out);  /* TTNetwork::kronecker, :224-254 */
+/* TTNetwork::dirac (:257-283): position has one entry per dimension */
+int xt_tt_dirac(int is_operator, size_t ndims, const size_t* dims, const size_t* position, xt_tt* out);
+int xt_tt_transpose(xt_tt t);                                 /* TTOperator::transpose, ttNetwork.h:443-448 */
 int xt_tt_copy(xt_tt src, xt_tt* out);
 int xt_tt_free(xt_tt t);
 int xt_tt_order(xt_tt t, size_t* order);
@@ -216,7 +228,10 @@ int xt_adf_solve_r1(xt_tt x, xt_r1meas m, const size_t* max_ranks, size_t max_it
                     double min_decrease, int precision, double* residual, size_t* iterations);
 
 /* Raw sampled-entry kernels on caller-owned device buffers (FP32 cores, all cores concatenated in TT order;
- * dims n[d] and ranks r[d+1] given on the host; positions P×d uint32 on the device).
+ * dims n[d] and ranks r[d+1] given on the host; positions P×d uint32 on the device). The rank list describes
+ * every core: core k is (r[k], n[k], r[k+1]) with r[0] = r[d] = 1, so `cores` and `grads` must each hold exactly
+ * Σ_k r[k]·n[k]·r[k+1] floats; the kernels take every core offset from that list and cannot see the buffers'
+ * real extent. Positions must satisfy pos[i·d + k] < n[k].
  * values[i] = x(position_i); grads = Σ_i seed_i·∂x_i/∂G (the adjoint of the sampling). */
 int xt_tt_sample_f32(int d, const int* n, const int* r, const float* cores, const uint32_t* pos, int P, float* values,
                      void* stream);

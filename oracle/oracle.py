@@ -84,6 +84,10 @@ def lib():
             "or_adf_plan": (C.c_int, [_vp, _ps, _ps, _pu32, _pu32, _ps]),
             "or_adf_gradient": (C.c_int, [_vp, _vp, _pd, _pd]),
             "or_adf_solve": (C.c_int, [_vp, _vp, _ps, _sz, C.c_double, C.c_double, _pd, _ps]),
+            "or_adf_core_step": (C.c_int, [_vp, _vp, _sz]),
+            "or_tt_from_tensor": (_vp, [C.c_int, _sz, _ps, _pd, C.c_double, _ps]),
+            "or_tt_special": (_vp, [C.c_int, C.c_int, _sz, _ps, _ps]),
+            "or_tt_transpose": (C.c_int, [_vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -256,6 +260,39 @@ class TT:
         t.set_state(canonicalized, core_position)
         return t
 
+    @staticmethod
+    def from_tensor(full, op=False, eps=8 * np.finfo(float).eps, max_ranks=None):
+        """TTNetwork(const Tensor&, eps, maxRanks) (ttNetwork.cpp:112-160); an operator's tensor is (n…, m…)."""
+        full = f64(full); dims, p = _s(full.shape)
+        mr = None
+        if max_ranks is not None:
+            d = full.ndim // (2 if op else 1)
+            arr, mr = _s([max_ranks] * (d - 1) if np.isscalar(max_ranks) else max_ranks)
+        return TT(lib().or_tt_from_tensor(int(op), full.ndim, p, _d(full), eps, mr))
+
+    @staticmethod
+    def ones(dims, op=False):
+        d, p = _s(dims)
+        return TT(lib().or_tt_special(0, int(op), len(dims), p, None))
+
+    @staticmethod
+    def identity(dims):
+        d, p = _s(dims)
+        return TT(lib().or_tt_special(1, 1, len(dims), p, None))
+
+    @staticmethod
+    def kronecker(dims, op=False):
+        d, p = _s(dims)
+        return TT(lib().or_tt_special(2, int(op), len(dims), p, None))
+
+    @staticmethod
+    def dirac(dims, position, op=False):
+        d, p = _s(dims); q, pq = _s(position)
+        return TT(lib().or_tt_special(3, int(op), len(dims), p, pq))
+
+    def transpose(self):
+        _check(lib().or_tt_transpose(self.h))
+
     def copy(self):
         return TT(lib().or_tt_copy(self.h))
 
@@ -415,6 +452,11 @@ def adf_gradient(x: TT, m: Measurements):
         grads.append(out[o: o + c.size].reshape(c.shape))
         o += c.size
     return grads, rn.value
+
+
+def adf_core_step(x: TT, m: Measurements, k: int) -> None:
+    """One ADF sweep step at component k on x as it is (gradient, slice norms, update_x; adf.cpp:360-478)."""
+    _check(lib().or_adf_core_step(x.h, m.h, k))
 
 
 def adf_solve(x: TT, m: Measurements, max_ranks=None, max_iterations=0, target=1e-8, min_decrease=0.999):

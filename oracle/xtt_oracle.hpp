@@ -88,6 +88,12 @@ void tt_scale(TT& a, double alpha);
 TT tt_apply(const TT& A, const TT& x);                                           // ttNetwork.cpp:888-967 + ttStack.cpp:197-310
 Tensor tt_full(const TT& x);
 double tt_residual(const TT& A, const TT& x, const TT& b);                       // als.cpp:483-486 (stable form)
+TT tt_from_tensor(const Tensor& full, bool op, double eps, const dims_t& maxRanks); // ttNetwork.cpp:112-160
+TT tt_ones(const dims_t& dims, bool op);                                         // ttNetwork.cpp:170-191
+TT tt_identity(const dims_t& dims);                                              // ttNetwork.cpp:194-221
+TT tt_kronecker(const dims_t& dims, bool op);                                    // ttNetwork.cpp:224-254
+TT tt_dirac(const dims_t& dims, const dims_t& position, bool op);               // ttNetwork.cpp:257-283
+void tt_transpose(TT& x);                                                        // ttNetwork.h:443-448
 
 // ---- ALS (als.cpp) ----
 struct ALSOptions {
@@ -123,5 +129,7 @@ ADFStacks adf_stack_plan(const SinglePointMeasurementSet& m);
 std::vector<Tensor> adf_full_gradient(const TT& x, const SinglePointMeasurementSet& m, double* residualNorm);
 struct ADFOptions { size_t maxIterations = 0; double targetResidualNorm = 1e-8; double minimalResidualNormDecrease = 0.999; };
 double adf_solve(const ADFOptions& o, TT& x, const SinglePointMeasurementSet& m, const dims_t& maxRanks, size_t* iterations); // adf.cpp:489-604
+// One step of the sweep at component k on x as it is (adf.cpp:360-478): gradient, slice norms, update_x.
+void adf_core_step(TT& x, const SinglePointMeasurementSet& m, size_t k);
 
 } // namespace orc

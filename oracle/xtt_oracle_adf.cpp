@@ -115,7 +115,41 @@ Tensor projected_gradient(const TT& x, const SinglePointMeasurementSet& m, const
 	return D;
 }
 double sqnorm(const std::vector<double>& v) { long double s = 0; for (double x : v) s += (long double)x * x; return (double)s; }
+
+// One ADF core step at component k (adf.cpp:360-396 gradient, :413-465 slice norms, :468-478 update_x): with the
+// residual res = b − x on the samples and the stacks F_{k-1}, B_{k+1} of x, Δ = projected gradient, and every
+// slice j of the core moves by ‖Δ_j‖²/‖P_j A Δ‖² · Δ_j, the exact minimiser of ½‖P_Ω(x) − b‖² along Δ_j (the
+// samples with i_k = j are the only ones the slice reaches).
+void core_step(TT& x, const SinglePointMeasurementSet& m, const std::vector<std::vector<double>>& F,
+               const std::vector<std::vector<double>>& B, const std::vector<double>& res, size_t k) {
+	const size_t P = m.size(), d = m.d;
+	Tensor D = projected_gradient(x, m, F, B, res, k);
+	const size_t n = D.dims[1], rl = D.dims[0], rr = D.dims[2];
+	std::vector<double> val;
+	evaluate_at(x, D, m, F, B, k, val);
+	std::vector<double> normA(n, 0.0);
+	for (size_t i = 0; i < P; ++i) normA[m.pos[i * d + k]] += val[i] * val[i];
+	Tensor& G = x.cores[k];
+	for (size_t j = 0; j < n; ++j) {
+		double pyr = 0;
+		for (size_t a = 0; a < rl; ++a) for (size_t c = 0; c < rr; ++c) { const double t = D.v[(a * n + j) * rr + c]; pyr += t * t; }
+		const double step = pyr / normA[j];
+		for (size_t a = 0; a < rl; ++a) for (size_t c = 0; c < rr; ++c) G.v[(a * n + j) * rr + c] += step * D.v[(a * n + j) * rr + c];
+	}
+}
 } // namespace
+
+void adf_core_step(TT& x, const SinglePointMeasurementSet& m, size_t k) {
+	const size_t d = x.order();
+	if (k >= d || m.d != d) throw std::invalid_argument("adf_core_step: component or degree mismatch");
+	std::vector<std::vector<double>> F(d), B(d);
+	for (size_t j = 0; j < k; ++j) forward_stack(x, m, F, j);
+	for (size_t j = d; j-- > k + 1;) backward_stack(x, m, B, j);
+	std::vector<double> val, res(m.size());
+	evaluate_at(x, x.cores[k], m, F, B, k, val);
+	for (size_t i = 0; i < m.size(); ++i) res[i] = m.values[i] - val[i];
+	core_step(x, m, F, B, res, k);
+}
 
 void meas_measure(SinglePointMeasurementSet& m, const TT& x) {  // measurments.cpp:125-153
 	std::vector<std::vector<double>> F(x.order()), B(x.order());
@@ -213,20 +247,7 @@ double adf_solve(const ADFOptions& o, TT& x, const SinglePointMeasurementSet& m,
 			if (residualNorm < o.targetResidualNorm || resDec1 * resDec2 * resDec3 * resDec4 > std::pow(o.minimalResidualNormDecrease, 4)) break;
 			for (size_t k = 0; k < d; ++k) {
 				if (k > 0) calculate_residual(k);
-				Tensor D = projected_gradient(x, m, F, B, res, k);
-				// slice-wise ‖P_j A Δ‖² (adf.cpp:413-465)
-				const size_t n = D.dims[1], rl = D.dims[0], rr = D.dims[2];
-				evaluate_at(x, D, m, F, B, k, val);
-				std::vector<double> normA(n, 0.0);
-				for (size_t i = 0; i < P; ++i) normA[m.pos[i * d + k]] += val[i] * val[i];
-				// update_x (adf.cpp:468-478)
-				Tensor& G = x.cores[k];
-				for (size_t j = 0; j < n; ++j) {
-					double pyr = 0;
-					for (size_t a = 0; a < rl; ++a) for (size_t c = 0; c < rr; ++c) { const double t = D.v[(a * n + j) * rr + c]; pyr += t * t; }
-					const double step = pyr / normA[j];
-					for (size_t a = 0; a < rl; ++a) for (size_t c = 0; c < rr; ++c) G.v[(a * n + j) * rr + c] += step * D.v[(a * n + j) * rr + c];
-				}
+				core_step(x, m, F, B, res, k);
 				if (k + 1 < d) { move_core(x, k + 1, true); forward_stack(x, m, F, k); }
 			}
 		}

@@ -245,6 +245,10 @@ struct ALSData {
 			for (size_t i = 0; i < n; ++i) r[i] = bl.v[i] - q[i];
 			const double bn = std::sqrt(dot(bl.v, bl.v));
 			double rr = dot(r, r);
+			// A warm start whose residual exceeds ‖b‖ is worse than zero, and CG's recursive residual cannot resolve
+			// the true one below ε·‖A‖·max‖x_k‖ (A. Greenbaum, SIAM J. Matrix Anal. Appl. 18 (1997) 535-551): such a
+			// solve starts from x = 0, r = b (the dense solve of lapack_solver has no start value at all).
+			if (rr > bn * bn) { std::fill(xv.begin(), xv.end(), 0.0); r = bl.v; rr = bn * bn; }
 			p = r;
 			size_t it = 0;
 			for (; it < o.cgMaxIter && std::sqrt(rr) > o.cgTol * bn; ++it) {

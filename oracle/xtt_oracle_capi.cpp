@@ -94,6 +94,24 @@ void or_tt_scale(void* h, double s) { tt_scale(*tt(h), s); }
 void* or_tt_apply(void* A, void* x) { try { return new TT(tt_apply(*tt(A), *tt(x))); } catch (const std::exception& e) { g_err = e.what(); return nullptr; } }
 int or_tt_full(void* h, double* out) { return guard([&] { Tensor t = tt_full(*tt(h)); std::memcpy(out, t.data(), t.size() * 8); }); }
 double or_tt_residual(void* A, void* x, void* b) { return tt_residual(*tt(A), *tt(x), *tt(b)); }
+void* or_tt_from_tensor(int op, size_t deg, const size_t* dims, const double* data, double eps, const size_t* maxRanks) {
+	try {
+		Tensor t(mk(dims, deg)); std::memcpy(t.data(), data, t.size() * 8);
+		const size_t d = deg / (op ? 2 : 1);
+		return new TT(tt_from_tensor(t, op, eps, maxRanks ? mk(maxRanks, d ? d - 1 : 0) : dims_t{}));
+	} catch (const std::exception& e) { g_err = e.what(); return nullptr; }
+}
+void* or_tt_special(int kind, int op, size_t nd, const size_t* dims, const size_t* position) {
+	try {
+		switch (kind) {
+		case 0: return new TT(tt_ones(mk(dims, nd), op));
+		case 1: return new TT(tt_identity(mk(dims, nd)));
+		case 2: return new TT(tt_kronecker(mk(dims, nd), op));
+		default: return new TT(tt_dirac(mk(dims, nd), mk(position, nd), op));
+		}
+	} catch (const std::exception& e) { g_err = e.what(); return nullptr; }
+}
+int or_tt_transpose(void* h) { return guard([&] { tt_transpose(*tt(h)); }); }
 
 int or_als(void* A, void* x, void* b, unsigned sites, size_t numHalfSweeps, double eps, int useResidual, int preserveCore,
            int assumeSPD, int localSolver, double cgTol, size_t cgMaxIter, double* energy, size_t* stats3) {
@@ -140,6 +158,7 @@ int or_adf_gradient(void* x, void* h, double* out, double* residualNorm) {
 		for (auto& t : g) { std::memcpy(out + o, t.data(), t.size() * 8); o += t.size(); }
 	});
 }
+int or_adf_core_step(void* x, void* h, size_t k) { return guard([&] { adf_core_step(*tt(x), *ms(h), k); }); }
 int or_adf_solve(void* x, void* h, const size_t* maxRanks, size_t maxIt, double target, double minDec, double* residual, size_t* iterations) {
 	return guard([&] {
 		ADFOptions o; o.maxIterations = maxIt; o.targetResidualNorm = target; o.minimalResidualNormDecrease = minDec;

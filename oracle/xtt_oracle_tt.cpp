@@ -346,4 +346,116 @@ double tt_residual(const TT& A, const TT& x, const TT& b) {
 	return tt_frob_norm(w) / std::sqrt(tt_dot(bb, bb));
 }
 
+
+// ---------------------------------------------------------------- TT constructors
+
+// TTNetwork(const Tensor&, eps, maxRanks) (ttNetwork.cpp:112-160): operator modes interleaved (n_0, m_0, n_1, …),
+// ghost modes of size 1 at both ends, then SVDs from the last component to the first, each keeping
+// min(maxRank, #{s_j > eps·s_0}) values and multiplying S into the remaining left part; canonical at 0.
+TT tt_from_tensor(const Tensor& full, bool op, double eps, const dims_t& maxRanksIn) {
+	const size_t N = op ? 2 : 1;
+	if (full.degree() % N) throw std::invalid_argument("Number of indicis must be even for TTOperator");
+	if (!(eps >= 0 && eps < 1)) throw std::invalid_argument("_eps must be positive and smaller than one.");
+	const size_t d = full.degree() / N;
+	if (d == 0) throw std::invalid_argument("a TT of degree 0 is not supported");
+	const dims_t maxRanks = maxRanksIn.empty() ? dims_t(d - 1, SIZE_MAX) : maxRanksIn;
+	if (maxRanks.size() != d - 1) throw std::invalid_argument("wrong number of maximal ranks");
+	for (size_t r : maxRanks) if (r == 0) throw std::invalid_argument("Maximal ranks must be strictly positive.");
+	TT x = tt_zero(full.dims, op);
+	Tensor remains;
+	if (op) {
+		std::vector<size_t> sh(full.degree());
+		for (size_t i = 0; i < d; ++i) { sh[i] = 2 * i; sh[d + i] = 2 * i + 1; }
+		remains = reshuffle(full, sh);
+	} else {
+		remains = full;
+	}
+	remains.dims = cat(cat({1}, remains.dims), {1});
+	for (size_t pos = d - 1; pos > 0; --pos) {
+		Tensor U, S, Vt;
+		calculate_svd(U, S, Vt, remains, 1 + pos * N, maxRanks[pos - 1], eps);
+		x.set_component(pos, std::move(Vt));
+		remains = contract(U, false, S, false, 1);
+	}
+	x.set_component(0, std::move(remains));
+	x.canonicalized = true;
+	x.corePosition = 0;
+	return x;
+}
+
+// TTNetwork::ones (ttNetwork.cpp:170-191): all-ones rank-1 components, then canonicalize_left (move_core(0)).
+TT tt_ones(const dims_t& dims, bool op) {
+	TT x = tt_zero(dims, op);
+	for (size_t k = 0; k < x.order(); ++k) {
+		Tensor t(x.cores[k].dims);
+		std::fill(t.v.begin(), t.v.end(), 1.0);
+		x.set_component(k, std::move(t));
+	}
+	move_core(x, 0, false);
+	return x;
+}
+
+// TTNetwork<true>::identity (ttNetwork.cpp:194-221): components δ(i_k, j_k), then canonicalize_left.
+TT tt_identity(const dims_t& dims) {
+	TT x = tt_zero(dims, true);
+	const size_t d = x.order();
+	for (size_t k = 0; k < d; ++k) {
+		Tensor t({1, dims[k], dims[d + k], 1});
+		for (size_t i = 0; i < std::min(dims[k], dims[d + k]); ++i) t.v[i * dims[d + k] + i] = 1.0;
+		x.set_component(k, std::move(t));
+	}
+	move_core(x, 0, false);
+	return x;
+}
+
+// TTNetwork::kronecker (ttNetwork.cpp:224-254): every component a Kronecker tensor (ones on its full diagonal) with
+// inner ranks min(dims), then canonicalize_left.
+TT tt_kronecker(const dims_t& dims, bool op) {
+	TT x = tt_zero(dims, op);
+	const size_t d = x.order();
+	size_t minN = SIZE_MAX;
+	for (size_t v : dims) minN = std::min(minN, v);
+	for (size_t k = 0; k < d; ++k) {
+		const size_t l = k == 0 ? 1 : minN, r = k + 1 == d ? 1 : minN;
+		dims_t cd = op ? dims_t{l, dims[k], dims[d + k], r} : dims_t{l, dims[k], r};
+		Tensor t(cd);
+		// the diagonal entries of the unreshaped component: every mode index equal (boundary ranks are ghosts)
+		for (size_t i = 0; i < minN; ++i) {
+			const size_t a = k == 0 ? 0 : i, c = k + 1 == d ? 0 : i;
+			const size_t idx = op ? ((a * cd[1] + i) * cd[2] + i) * cd[3] + c : (a * cd[1] + i) * cd[2] + c;
+			t.v[idx] = 1.0;
+		}
+		x.set_component(k, std::move(t));
+	}
+	move_core(x, 0, false);
+	return x;
+}
+
+// TTNetwork::dirac (ttNetwork.cpp:257-283): rank-1 components with a single 1 at the position (not canonicalized
+// afterwards, as in xerus: set_component of components other than the core clears the flag).
+TT tt_dirac(const dims_t& dims, const dims_t& position, bool op) {
+	if (dims.size() != position.size()) throw std::invalid_argument("Inconsitend number of entries in _dimensions and _position.");
+	for (size_t i = 0; i < dims.size(); ++i) if (position[i] >= dims[i]) throw std::invalid_argument("dirac position out of range");
+	TT x = tt_zero(dims, op);
+	const size_t d = x.order();
+	for (size_t k = 0; k < d; ++k) {
+		Tensor t(x.cores[k].dims);
+		t.v[op ? position[k] * dims[d + k] + position[d + k] : position[k]] = 1.0;
+		x.set_component(k, std::move(t));
+	}
+	if (d <= 1) { x.canonicalized = true; x.corePosition = 0; }  // the Tensor constructor's result
+	return x;
+}
+
+// TTOperator::transpose (ttNetwork.h:443-448): every component's two external modes swapped (row and column
+// dimensions exchanged with them).
+void tt_transpose(TT& x) {
+	if (!x.op) throw std::invalid_argument("transpose: not a TTOperator");
+	const size_t d = x.order();
+	for (size_t k = 0; k < d; ++k) {
+		x.cores[k] = reshuffle(x.cores[k], {0, 2, 1, 3});
+		std::swap(x.dims[k], x.dims[d + k]);
+	}
+}
+
 } // namespace orc

@@ -86,3 +86,19 @@ def test_refuses_foreign_hip_runtime():
                  "print('ok', '/opt/rocm' in maps and 'libxtt' in maps)\n") % ROOT
     out = subprocess.run([sys.executable, "-c", xtt_first], capture_output=True, text=True, timeout=300)
     assert "ok True" in out.stdout, out.stdout + out.stderr
+
+
+def test_torch_binding_rejects_inconsistent_ranks():
+    """The sampled-entry kernels take every core offset from the rank chain, so the torch binding refuses cores
+    whose neighbouring ranks disagree (cores (1,4,3),(2,4,1) would size the gradient 24 floats for 20) before any
+    device buffer is made."""
+    import subprocess
+    import sys
+    prog = ("import sys; sys.path.insert(0, %r)\n"
+            "from xerus_amd.autograd import tt_sample\nimport torch\n"
+            "pos = torch.zeros((3, 2), dtype=torch.int32)\n"
+            "for shapes in ([(1, 4, 3), (2, 4, 1)], [(2, 4, 3), (3, 4, 1)], [(1, 4, 3), (3, 4, 2)]):\n"
+            "    try:\n        tt_sample(pos, [torch.zeros(s) for s in shapes]); print('accepted', shapes)\n"
+            "    except ValueError as e:\n        print('refused', 'rank' in str(e))\n") % ROOT
+    out = subprocess.run([sys.executable, "-c", prog], capture_output=True, text=True, timeout=300)
+    assert out.stdout.split("\n")[:3] == ["refused True"] * 3, out.stdout + out.stderr

@@ -60,6 +60,7 @@ SIGNATURES = {
     "xt_version": (C.c_char_p, []),
     "xt_synchronize": (C.c_int, []),
     "xt_release_cache": (C.c_int, []),
+    "xt_cache_bytes": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "xt_malloc": (C.c_int, [_sz, C.POINTER(_vp)]),
     "xt_free": (C.c_int, [_vp]),
     "xt_memcpy_htod": (C.c_int, [_vp, _vp, _sz]),
@@ -86,6 +87,12 @@ SIGNATURES = {
     "xt_dot": (C.c_int, [_vp, _vp, _sz, _pd]),
     "xt_tt_new": (C.c_int, [C.c_int, _sz, _ps, C.POINTER(_vp)]),
     "xt_tt_random": (C.c_int, [C.c_int, _sz, _ps, _ps, C.POINTER(_vp)]),
+    "xt_tt_from_tensor": (C.c_int, [C.c_int, _sz, _ps, _pd, C.c_double, _ps, C.POINTER(_vp)]),
+    "xt_tt_ones": (C.c_int, [C.c_int, _sz, _ps, C.POINTER(_vp)]),
+    "xt_tt_identity": (C.c_int, [_sz, _ps, C.POINTER(_vp)]),
+    "xt_tt_kronecker": (C.c_int, [C.c_int, _sz, _ps, C.POINTER(_vp)]),
+    "xt_tt_dirac": (C.c_int, [C.c_int, _sz, _ps, _ps, C.POINTER(_vp)]),
+    "xt_tt_transpose": (C.c_int, [_vp]),
     "xt_tt_copy": (C.c_int, [_vp, C.POINTER(_vp)]),
     "xt_tt_free": (C.c_int, [_vp]),
     "xt_tt_order": (C.c_int, [_vp, _ps]),

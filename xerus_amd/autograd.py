@@ -37,6 +37,12 @@ def _check(positions, cores):
     for c in cores:
         if c.dtype != torch.float32 or c.is_cuda or c.dim() != 3:
             raise ValueError("cores must be CPU float32 tensors of shape (r_{k-1}, n_k, r_k)")
+    # the kernels take every core offset from the rank chain, so it must describe the cores exactly
+    if cores[0].shape[0] != 1 or cores[-1].shape[2] != 1:
+        raise ValueError("the first core's left rank and the last core's right rank must be 1")
+    for k in range(len(cores) - 1):
+        if cores[k].shape[2] != cores[k + 1].shape[0]:
+            raise ValueError(f"core {k} has right rank {cores[k].shape[2]} but core {k + 1} has left rank {cores[k + 1].shape[0]}")
 
 
 class TTSample(torch.autograd.Function):

@@ -50,6 +50,7 @@ void Measurements::upload() {
 
 template <class T>
 void Flat<T>::build(const TT& x) {
+	require_correct_format(x);  // strides below come from the rank chain
 	const size_t d = x.order();
 	std::vector<int> n(d), rank(d + 1);
 	std::vector<long long> off(d);

@@ -289,6 +289,9 @@ Workspace& workspace() {
 	static thread_local Workspace w;  // per thread: its graphs replay on that thread's stream
 	return w;
 }
+// A batch worker drops its recorded CG graphs before its stream goes (the buffers follow at thread exit, into the
+// pool of the library stream).
+void release_thread_graphs() { workspace().drop_graphs(); }
 
 struct ALSData {
 	const ALSOptions& o;
@@ -323,6 +326,8 @@ struct ALSData {
 	// finish_counts() reads them once at the end of the call.
 	DBuf<double> itDev;
 	bool usedFused = false;
+	bool noFused = false;        // the rerun after a fused solve ran out of time uses the chunked CG only
+	bool fusedTimedOut = false;
 
 	ALSData(const ALSOptions& o_, const TT* A_, TT& x_, const TT& b_, ALSStats* s)
 	    : o(o_), A(A_), x(x_), b(b_), targetRank(x_.ranks()), canonAtEnd(x_.canonicalized),
@@ -528,10 +533,11 @@ struct ALSData {
 
 	template <class T>
 	bool fused_ok(const SiteOp& s) const {
-		return !t_batch_worker && !getenv("XTT_NO_FUSED_CG") && fused_cg_fits<T>(s.a, s.n, s.c, s.P, s.Q);
+		return !noFused && !t_batch_worker && !getenv("XTT_NO_FUSED_CG") && fused_cg_fits<T>(s.a, s.n, s.c, s.P, s.Q);
 	}
+	// false when the runtime refused the cooperative launch (nothing was enqueued)
 	template <class T>
-	void run_fused(const SiteOp& s, FusedCG d) {
+	bool run_fused(const SiteOp& s, FusedCG d) {
 		if (!usedFused) {
 			itDev.resize(2);
 			fill_f64(itDev.get(), 0.0, 2, stream());
@@ -550,13 +556,16 @@ struct ALSData {
 		d.iterAcc = itDev.get();
 		d.flopsPerIter = 2.0 * s.a * s.n * s.c * (s.a * s.P + s.n * s.P * s.Q + s.c * s.Q);
 		d.a = s.a; d.n = s.n; d.c = s.c; d.P = s.P; d.Q = s.Q;
-		fused_cg<T>(d, stream());
+		return fused_cg<T>(d, stream());
 	}
+	// Reads the fused solves' counters once per call. A barrier wait that ran out of time (the device was shared
+	// and did not keep the grid resident) leaves that site's core unsolved: the call is then rerun on the chunked
+	// CG (als_solve), so fusedTimedOut is all that is recorded here.
 	void finish_counts() {
 		if (!usedFused) return;
 		double h[2];
 		read_back(h, itDev.get(), sizeof h);
-		XTT_REQUIRE(!std::isnan(h[0]), "ALS: a fused CG grid barrier wait ran out of time");
+		if (std::isnan(h[0])) { fusedTimedOut = true; return; }
 		const size_t it = (size_t)h[0];
 		cgIters += it;
 		if (!stats) return;
@@ -579,9 +588,8 @@ struct ALSData {
 			cg_state_init(ws.st.get(), o.localTol * o.localTol, o.maxLocalIter, stream());
 			cg_start<double>(ws.r.get(), ws.p.get(), blin, ws.q.get(), N, ws.st.get(), ws.partials.get(), stream());
 			FusedCG d;
-			d.r0 = ws.r.get(); d.x = xout; d.mode = 0;
-			run_fused<double>(s, d);
-			return;
+			d.r0 = ws.r.get(); d.b = blin; d.x = xout; d.mode = 0;
+			if (run_fused<double>(s, d)) return;
 		}
 		double* x = ws.x.get();
 		const double* bl = ws.bl.get();
@@ -590,7 +598,9 @@ struct ALSData {
 		matvec64(s, x, ws.q.get(), nullptr);
 		cg_state_init(ws.st.get(), o.localTol * o.localTol, o.maxLocalIter, stream());
 		cg_start<double>(ws.r.get(), ws.p.get(), bl, ws.q.get(), N, ws.st.get(), ws.partials.get(), stream());
-		if (read_state().done) return;  // x unchanged
+		const CGState h0 = read_state();
+		if (h0.done) return;  // x unchanged
+		if (h0.rr > h0.rr0) cg_zero_start(x, ws.r.get(), ws.p.get(), bl, N, ws.st.get(), stream());  // warm start worse than 0
 		stage_site(s, false);
 		const int* done = &ws.st.get()->done;
 		const std::vector<size_t> key{64, s.a, s.n, s.c, s.P, s.Q};
@@ -624,7 +634,7 @@ struct ALSData {
 		const int* done = &ws.st.get()->done;
 		const std::vector<size_t> key{(size_t)o.precision, s.a, s.n, s.c, s.P, s.Q};
 		bool staged = false;
-		const bool fused = o.precision == P_F32 && fused_ok<float>(s);
+		bool fused = o.precision == P_F32 && fused_ok<float>(s);
 		if (spec) {  // the first refinement test, on the device
 			matvec64(s, x, ws.q.get(), nullptr);
 			sc.check(ws.r.get(), bl, ws.q.get(), N, o.localTol * o.localTol, o.maxRefine > 0 ? specFail : nullptr);
@@ -634,8 +644,14 @@ struct ALSData {
 			matvec64(s, x, ws.q.get(), nullptr);
 			double rb[2];
 			sc.residual(ws.r.get(), bl, ws.q.get(), N, rb);
-			const double rr = rb[0], tol2 = o.localTol * o.localTol * rb[1];
+			double rr = rb[0];
+			const double tol2 = o.localTol * o.localTol * rb[1];
 			if (rr <= tol2 || ref == o.maxRefine) break;
+			if (ref == 0 && rr > rb[1]) {  // the warm start is worse than zero: refine from x = 0 (r = b)
+				fill_f64(x, 0.0, N, stream());
+				XTT_HIP(hipMemcpyAsync(ws.r.get(), bl, N * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+				rr = rb[1];
+			}
 			if (fused) {  // e from 0 for r/‖r‖ and x += ‖r‖·e, all inside one launch
 				if (stats) stats->refinements++;
 				const double rn = std::sqrt(rr);
@@ -643,8 +659,9 @@ struct ALSData {
 				FusedCG d;
 				d.r0 = ws.r.get(); d.r0scale = 1.0 / rn; d.x = x; d.xscale = rn; d.mode = 1;
 				d.tol2 = inner * inner; d.maxIter = o.maxLocalIter;
-				run_fused<float>(s, d);
-				continue;
+				if (run_fused<float>(s, d)) continue;
+				fused = false;  // launch refused: this and later refinements on the chunked CG
+				if (stats) stats->refinements--;
 			}
 			if (!staged) {
 				convert_f64_f32(s.L->data(), ws.L32.get(), s.L->size(), stream());
@@ -910,10 +927,12 @@ double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats*
 	for (const TT* t : {A, (const TT*)&x, &b})
 		if (t) { for (size_t r : t->ranks()) key.push_back(r); key.push_back(0); }
 	key.insert(key.end(), x.dims.begin(), x.dims.end());
-	auto run = [&](bool spec, unsigned* fail, size_t& cgIters) {
+	bool timedOut = false;
+	auto run = [&](bool spec, unsigned* fail, size_t& cgIters, bool noFused) {
 		ALSData data(o, A, x, b, stats);
 		data.spec = spec;
 		data.specFail = fail;
+		data.noFused = noFused;
 		data.clock.begin();
 		data.energy = data.energy_f();
 		data.perf_point(3, false);
@@ -924,6 +943,7 @@ double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats*
 		}
 		data.finish_counts();
 		cgIters = data.cgIters;
+		timedOut = data.fusedTimedOut;
 		return data.energy;
 	};
 	auto it = lastConverged.find(key);
@@ -933,7 +953,7 @@ double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats*
 		DBuf<unsigned> fail(1);
 		XTT_HIP(hipMemsetAsync(fail.get(), 0, sizeof(unsigned), stream()));
 		size_t cgIters = 0;
-		const double e = run(true, fail.get(), cgIters);
+		const double e = run(true, fail.get(), cgIters, false);
 		unsigned failed = 0;
 		read_back(&failed, fail.get(), sizeof(unsigned));
 		if (failed == 0) return e;
@@ -941,7 +961,18 @@ double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats*
 		if (stats) *stats = savedStats;
 	}
 	size_t cgIters = 0;
-	const double e = run(false, nullptr, cgIters);
+	// A fused (cooperative) local solve whose grid barrier ran out of time left its site unsolved: the whole call
+	// runs again from x's start on the chunked CG, so the result is that of a call without fused solves.
+	const bool mayFuse = A && !t_batch_worker && !getenv("XTT_NO_FUSED_CG");
+	TT start;
+	ALSStats startStats;
+	if (mayFuse) { start = tt_copy(x); if (stats) startStats = *stats; }
+	double e = run(false, nullptr, cgIters, false);
+	if (timedOut) {
+		x = std::move(start);
+		if (stats) *stats = startStats;
+		e = run(false, nullptr, cgIters, true);
+	}
 	if (lastConverged.size() > 256) lastConverged.clear();
 	lastConverged[key] = cgIters == 0;
 	return e;
@@ -960,6 +991,7 @@ void als_round_batch(const ALSOptions& o, const std::vector<const TT*>& A, const
 	int dev = 0;
 	XTT_HIP(hipGetDevice(&dev));
 	sync();  // inputs built on the library stream are complete before workers use them
+	park_stream_pool(stream());  // … and so the library stream's cached blocks can serve the workers
 	std::atomic<size_t> next{0};
 	std::mutex mu;
 	std::exception_ptr err;
@@ -996,6 +1028,9 @@ void als_round_batch(const ALSOptions& o, const std::vector<const TT*>& A, const
 		if (s) {
 			(void)hipStreamSynchronize(s);
 			bind_thread_stream(nullptr);
+			t_batch_worker = false;
+			release_thread_graphs();  // graph workspaces captured on s, and their buffers, go before s does
+			park_stream_pool(s);
 			(void)hipStreamDestroy(s);
 		}
 	};

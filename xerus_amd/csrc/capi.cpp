@@ -33,6 +33,9 @@ int guard(F&& f) {
 	}
 }
 #define NN(p) XTT_REQUIRE((p) != nullptr, #p " must not be NULL")
+// a TT handed to device work must be in xerus's format (TTNetwork::require_correct_format, ttNetwork.cpp:290-340):
+// the kernels derive strides from the rank chain, so a component set with an inconsistent rank is refused here
+#define FMT(h) do { NN(h); ::xtt::require_correct_format((h)->t); } while (0)
 xtt::dims_t mk(const size_t* p, size_t n) { return xtt::dims_t(p, p + n); }
 void sync_stream(void* s) { XTT_HIP(hipStreamSynchronize(xtt::pick(s))); }
 }  // namespace
@@ -43,6 +46,9 @@ const char* xt_last_error(void) { return xtt::last_error().c_str(); }
 const char* xt_version(void) { return "xtt 0.1 (gfx950)"; }
 int xt_synchronize(void) { return guard([] { xtt::sync(); }); }
 int xt_release_cache(void) { return guard([] { xtt::sync(); xtt::release_cache(); }); }
+int xt_cache_bytes(size_t* in_use, size_t* cached) {
+	return guard([&] { if (in_use) *in_use = xtt::bytes_in_use(); if (cached) *cached = xtt::bytes_cached(); });
+}
 int xt_profile_enable(int on) { return guard([&] { xtt::prof_enable(on != 0); }); }
 int xt_profile_read(size_t max_records, char* tags, size_t tag_len, size_t* launches, double* ms, double* flops, double* bytes,
                     size_t* count) {
@@ -196,6 +202,26 @@ int xt_tt_random(int op, size_t nd, const size_t* dims, const size_t* ranks, xt_
 		*out = new xt_tt_s{xtt::tt_random(mk(dims, nd), d > 1 ? mk(ranks, d - 1) : xtt::dims_t{}, op != 0)};
 	});
 }
+int xt_tt_from_tensor(int op, size_t nd, const size_t* dims, const double* host, double eps, const size_t* maxRanks, xt_tt* out) {
+	return guard([&] {
+		NN(out); NN(dims); NN(host);
+		const size_t d = nd / (op ? 2 : 1);
+		*out = new xt_tt_s{xtt::tt_from_tensor(mk(dims, nd), host, op != 0, eps, maxRanks && d > 1 ? mk(maxRanks, d - 1) : xtt::dims_t{})};
+	});
+}
+int xt_tt_ones(int op, size_t nd, const size_t* dims, xt_tt* out) {
+	return guard([&] { NN(out); NN(dims); *out = new xt_tt_s{xtt::tt_ones(mk(dims, nd), op != 0)}; });
+}
+int xt_tt_identity(size_t nd, const size_t* dims, xt_tt* out) {
+	return guard([&] { NN(out); NN(dims); *out = new xt_tt_s{xtt::tt_identity(mk(dims, nd))}; });
+}
+int xt_tt_kronecker(int op, size_t nd, const size_t* dims, xt_tt* out) {
+	return guard([&] { NN(out); NN(dims); *out = new xt_tt_s{xtt::tt_kronecker(mk(dims, nd), op != 0)}; });
+}
+int xt_tt_dirac(int op, size_t nd, const size_t* dims, const size_t* position, xt_tt* out) {
+	return guard([&] { NN(out); NN(dims); NN(position); *out = new xt_tt_s{xtt::tt_dirac(mk(dims, nd), mk(position, nd), op != 0)}; });
+}
+int xt_tt_transpose(xt_tt t) { return guard([&] { FMT(t); xtt::tt_transpose(t->t); xtt::sync(); }); }
 int xt_tt_copy(xt_tt src, xt_tt* out) { return guard([&] { NN(src); NN(out); *out = new xt_tt_s{xtt::tt_copy(src->t)}; }); }
 int xt_tt_free(xt_tt t) { return guard([&] { delete t; }); }
 int xt_tt_order(xt_tt t, size_t* o) { return guard([&] { NN(t); NN(o); *o = t->t.order(); }); }
@@ -229,24 +255,24 @@ int xt_tt_canonicalization(xt_tt t, int* c, size_t* pos) {
 int xt_tt_assume_core_position(xt_tt t, size_t pos) {
 	return guard([&] { NN(t); XTT_REQUIRE(pos < t->t.order(), "Invalid core position."); t->t.corePosition = pos; t->t.canonicalized = true; });
 }
-int xt_tt_move_core(xt_tt t, size_t pos, int keep) { return guard([&] { NN(t); xtt::move_core(t->t, pos, keep != 0); xtt::sync(); }); }
+int xt_tt_move_core(xt_tt t, size_t pos, int keep) { return guard([&] { FMT(t); xtt::move_core(t->t, pos, keep != 0); xtt::sync(); }); }
 int xt_tt_round(xt_tt t, const size_t* maxRanks, double eps) {
 	return guard([&] {
-		NN(t);
+		FMT(t);
 		const size_t d = t->t.order();
 		xtt::dims_t mr = maxRanks ? mk(maxRanks, d - 1) : xtt::dims_t(d - 1, SIZE_MAX);
 		xtt::round(t->t, mr, eps);
 		xtt::sync();
 	});
 }
-int xt_tt_frob_norm(xt_tt t, double* o) { return guard([&] { NN(t); NN(o); *o = xtt::tt_frob_norm(t->t); }); }
-int xt_tt_dot(xt_tt a, xt_tt b, double* o) { return guard([&] { NN(a); NN(b); NN(o); *o = xtt::tt_dot(a->t, b->t); }); }
-int xt_tt_add(xt_tt a, xt_tt b, xt_tt* out) { return guard([&] { NN(a); NN(b); NN(out); *out = new xt_tt_s{xtt::tt_add(a->t, b->t)}; xtt::sync(); }); }
+int xt_tt_frob_norm(xt_tt t, double* o) { return guard([&] { FMT(t); NN(o); *o = xtt::tt_frob_norm(t->t); }); }
+int xt_tt_dot(xt_tt a, xt_tt b, double* o) { return guard([&] { FMT(a); FMT(b); NN(o); *o = xtt::tt_dot(a->t, b->t); }); }
+int xt_tt_add(xt_tt a, xt_tt b, xt_tt* out) { return guard([&] { FMT(a); FMT(b); NN(out); *out = new xt_tt_s{xtt::tt_add(a->t, b->t)}; xtt::sync(); }); }
 int xt_tt_scale(xt_tt t, double s) { return guard([&] { NN(t); xtt::tt_scale(t->t, s); xtt::sync(); }); }
-int xt_tt_apply(xt_tt A, xt_tt x, xt_tt* out) { return guard([&] { NN(A); NN(x); NN(out); *out = new xt_tt_s{xtt::tt_apply(A->t, x->t)}; xtt::sync(); }); }
+int xt_tt_apply(xt_tt A, xt_tt x, xt_tt* out) { return guard([&] { FMT(A); FMT(x); NN(out); *out = new xt_tt_s{xtt::tt_apply(A->t, x->t)}; xtt::sync(); }); }
 int xt_tt_full(xt_tt t, double* host) {
 	return guard([&] {
-		NN(t); NN(host);
+		FMT(t); NN(host);
 		const xtt::TT& x = t->t;
 		xtt::DTensor acc = xtt::dt_copy(x.cores[0]);
 		for (size_t k = 1; k < x.order(); ++k) acc = xtt::contract(acc, false, x.cores[k], false, 1);
@@ -263,13 +289,13 @@ int xt_tt_full(xt_tt t, double* host) {
 	});
 }
 int xt_tt_save(xt_tt t, const char* filename, int format) {
-	return guard([&] { NN(t); NN(filename); xtt::tt_save(t->t, filename, format); });
+	return guard([&] { FMT(t); NN(filename); xtt::tt_save(t->t, filename, format); });
 }
 int xt_tt_load(const char* filename, xt_tt* out) {
 	return guard([&] { NN(filename); NN(out); *out = nullptr; *out = new xt_tt_s{xtt::tt_load(filename)}; });
 }
 int xt_tt_residual(xt_tt A, xt_tt x, xt_tt b, double* o) {
-	return guard([&] { NN(A); NN(x); NN(b); NN(o); *o = xtt::tt_residual(A->t, x->t, b->t); });
+	return guard([&] { FMT(A); FMT(x); FMT(b); NN(o); *o = xtt::tt_residual(A->t, x->t, b->t); });
 }
 
 int xt_als_default_options(xt_als_options* o, unsigned sites, int spd) {
@@ -324,7 +350,8 @@ int xt_als_round_batch(const xt_als_options* o, size_t count, const xt_tt* A, co
 		std::vector<const xtt::TT*> av(count), bv(count);
 		std::vector<xtt::TT*> xv(count);
 		for (size_t p = 0; p < count; ++p) {
-			NN(x[p]); NN(b[p]);
+			FMT(x[p]); FMT(b[p]);
+			if (A && A[p]) FMT(A[p]);
 			av[p] = A && A[p] ? &A[p]->t : nullptr;
 			xv[p] = &x[p]->t;
 			bv[p] = &b[p]->t;
@@ -355,7 +382,8 @@ int xt_als_local_cg(const double* L, const double* A, const double* R, size_t a,
 }
 int xt_als_solve(const xt_als_options* o, xt_tt A, xt_tt x, xt_tt b, double* energy, xt_als_stats* st) {
 	return guard([&] {
-		NN(o); NN(x); NN(b); NN(energy);
+		NN(o); FMT(x); FMT(b); NN(energy);
+		if (A) FMT(A);
 		XTT_REQUIRE(o->precision >= XT_F64 && o->precision <= XT_BF16, "unknown precision");
 		xtt::ALSOptions p;
 		p.sites = o->sites;
@@ -402,11 +430,11 @@ int xt_meas_positions(xt_meas m, uint32_t* o) { return guard([&] { NN(m); NN(o);
 int xt_meas_positions_ptr(xt_meas m, const uint32_t** p) { return guard([&] { NN(m); NN(p); *p = m->m.dpos.get(); }); }
 int xt_meas_values(xt_meas m, double* o) { return guard([&] { NN(m); NN(o); std::memcpy(o, m->m.values.data(), m->m.values.size() * 8); }); }
 int xt_meas_set_values(xt_meas m, const double* v) { return guard([&] { NN(m); NN(v); std::memcpy(m->m.values.data(), v, m->m.values.size() * 8); }); }
-int xt_meas_measure(xt_meas m, xt_tt x) { return guard([&] { NN(m); NN(x); xtt::meas_measure(m->m, x->t); }); }
+int xt_meas_measure(xt_meas m, xt_tt x) { return guard([&] { NN(m); FMT(x); xtt::meas_measure(m->m, x->t); }); }
 
 int xt_adf_gradient(xt_tt x, xt_meas m, int precision, double* host, double* rn) {
 	return guard([&] {
-		NN(x); NN(m); NN(host);
+		FMT(x); NN(m); NN(host);
 		XTT_REQUIRE(precision == XT_F64 || precision == XT_F32, "ADF gradient precision must be XT_F64 or XT_F32");
 		auto g = precision == XT_F64 ? xtt::adf_full_gradient<double>(x->t, m->m, rn) : xtt::adf_full_gradient<float>(x->t, m->m, rn);
 		size_t o = 0;
@@ -440,11 +468,11 @@ int xt_r1meas_set_values(xt_r1meas m, const double* v) {
 }
 int xt_r1meas_sort(xt_r1meas m, int positions_only) { return guard([&] { NN(m); xtt::r1_sort(m->m, positions_only != 0); }); }
 int xt_r1meas_normalize(xt_r1meas m) { return guard([&] { NN(m); xtt::r1_normalize(m->m); }); }
-int xt_r1meas_measure(xt_r1meas m, xt_tt x) { return guard([&] { NN(m); NN(x); xtt::r1_measure(m->m, x->t); }); }
-int xt_r1meas_test(xt_r1meas m, xt_tt x, double* out) { return guard([&] { NN(m); NN(x); NN(out); *out = xtt::r1_test(m->m, x->t); }); }
+int xt_r1meas_measure(xt_r1meas m, xt_tt x) { return guard([&] { NN(m); FMT(x); xtt::r1_measure(m->m, x->t); }); }
+int xt_r1meas_test(xt_r1meas m, xt_tt x, double* out) { return guard([&] { NN(m); FMT(x); NN(out); *out = xtt::r1_test(m->m, x->t); }); }
 int xt_adf_gradient_r1(xt_tt x, xt_r1meas m, int precision, double* host, double* rn) {
 	return guard([&] {
-		NN(x); NN(m); NN(host);
+		FMT(x); NN(m); NN(host);
 		XTT_REQUIRE(precision == XT_F64 || precision == XT_F32, "ADF gradient precision must be XT_F64 or XT_F32");
 		auto g = precision == XT_F64 ? xtt::adf_full_gradient<double>(x->t, m->m, rn) : xtt::adf_full_gradient<float>(x->t, m->m, rn);
 		size_t o = 0;
@@ -454,7 +482,7 @@ int xt_adf_gradient_r1(xt_tt x, xt_r1meas m, int precision, double* host, double
 int xt_adf_solve_r1(xt_tt x, xt_r1meas m, const size_t* maxRanks, size_t maxIt, double target, double minDec, int precision,
                     double* res, size_t* its) {
 	return guard([&] {
-		NN(x); NN(m); NN(res);
+		FMT(x); NN(m); NN(res);
 		XTT_REQUIRE(precision == XT_F64 || precision == XT_F32, "ADF precision must be XT_F64 or XT_F32");
 		xtt::ADFOptions o;
 		o.maxIterations = maxIt; o.targetResidualNorm = target; o.minimalResidualNormDecrease = minDec;
@@ -477,7 +505,7 @@ int xt_perf_sink(xt_perf_callback cb, void* user) {
 int xt_adf_solve(xt_tt x, xt_meas m, const size_t* maxRanks, size_t maxIt, double target, double minDec, int precision, double* res,
                  size_t* its) {
 	return guard([&] {
-		NN(x); NN(m); NN(res);
+		FMT(x); NN(m); NN(res);
 		XTT_REQUIRE(precision == XT_F64 || precision == XT_F32, "ADF precision must be XT_F64 or XT_F32");
 		xtt::ADFOptions o;
 		o.maxIterations = maxIt; o.targetResidualNorm = target; o.minimalResidualNormDecrease = minDec;

@@ -31,9 +31,10 @@
 // workgroup 0 polls the slots and publishes a broadcast slot, which the others poll. Tags carry a launch epoch,
 // so the slots are never cleared. Waits are bounded by the wall clock: a wait that exceeds the limit raises the
 // state's fail flag, every workgroup leaves at its next wait, and the host reports an error.
-// The grid never exceeds the number of workgroups the device holds at once, and the kernel's workgroups wait
-// only for each other, so every workgroup becomes resident even when other kernels share the device.
+// The launch is cooperative: the runtime starts it only with every workgroup resident at once (or refuses it, and
+// the caller solves on the chunked CG), and the kernel's workgroups wait only for each other.
 #include <algorithm>
+#include <cstdlib>
 
 #include "kernels.hpp"
 #include "xtt_common.hpp"
@@ -66,6 +67,7 @@ struct FcgArgs {
 	const double* Am;  // (n, Q, P, n)
 	const double* R;   // (c, Q, c)
 	const double* r0;  // start residual (a, n, c), scaled by r0scale
+	const double* b;   // mode 0: right-hand side (a, n, c)
 	double* x;         // (a, n, c): warm start and result (mode 0), or x += xscale·e with e from 0 (mode 1)
 	T* vec;            // 7·N: r[2], s[2], w, p, x in the layout (c, a, n)
 	T* T2g;            // (a, c, n·Q)
@@ -167,7 +169,7 @@ __device__ inline bool grid_sum(double* S, unsigned* fail, ull tag, double& x, d
 	ull* const bc = reinterpret_cast<ull*>(S + BCAST * SLOT);
 	if (blockIdx.x == 0 && tid < 64) {  // wave 0 of workgroup 0 gathers
 		const ull t0 = wall_clock64();
-		int bad = 0;
+		int bad = timeout == 0;  // a zero limit gives up at once (the host's way to exercise the failure path)
 		for (int u = tid; u < G && !bad; u += 64) {
 			while (ld_dev(reinterpret_cast<const ull*>(S + u * SLOT)) != tag) {
 				if (wall_clock64() - t0 > timeout) { bad = 1; break; }
@@ -189,7 +191,8 @@ __device__ inline bool grid_sum(double* S, unsigned* fail, ull tag, double& x, d
 	if (tid == 0) {
 		const ull t0 = wall_clock64();
 		int bad = 0;
-		while (ld_dev(bc) != tag) {
+		if (timeout == 0) st_dev(fail, 1u);
+		while (timeout != 0 && ld_dev(bc) != tag) {
 			if (wall_clock64() - t0 > timeout) {
 				st_dev(fail, 1u);
 				bad = 1;
@@ -229,13 +232,18 @@ __global__ __launch_bounds__(FT) void fused_cg_kernel(FcgArgs<T> g) {
 	int stopAtStart = 0;
 	double rr0 = 0, tol2 = g.tol2;
 	int maxIter = g.maxIter;
+	bool zeroStart = false;
 	if (g.mode == 0) {
 		stopAtStart = g.st->done;
 		rr0 = g.st->rr0;
 		tol2 = g.st->tol2;
 		maxIter = g.st->maxIter;
+		// A warm start whose residual exceeds ‖b‖ is worse than zero, and CG's recursive residual cannot resolve the
+		// true one below ε·‖A‖·‖x‖ (x of the start's size): such a solve starts from x = 0, r = b instead.
+		zeroStart = g.st->rr > rr0;
 	}
 	if (stopAtStart || maxIter <= 0) return;  // uniform: nothing to do, x stays as it is
+	const double* const rstart = zeroStart ? g.b : g.r0;
 
 	const bool has1 = wg < g.U1, has3 = wg < g.U3;
 	const int cp = has1 ? wg / g.nab : 0, a0 = has1 ? (wg - cp * g.nab) * UB : 0, ab = min(UB, a - a0);
@@ -271,12 +279,12 @@ __global__ __launch_bounds__(FT) void fused_cg_kernel(FcgArgs<T> g) {
 		for (int e = tid; e < ab * n; e += FT) {
 			const size_t src = (size_t)(a0 * n + e) * c + cp;
 			const size_t dst = (size_t)cp * an + a0 * n + e;
-			st_dev(rb0 + dst, (T)(g.r0[src] * g.r0scale));
+			st_dev(rb0 + dst, (T)(rstart[src] * g.r0scale));
 			st_dev(sb0 + dst, T(0));
-			xv[dst] = g.mode == 0 ? (T)g.x[src] : T(0);
+			xv[dst] = g.mode == 0 && !zeroStart ? (T)g.x[src] : T(0);
 			p[dst] = T(0);
 		}
-		for (int e = tid; e < an; e += FT) Vs[e] = (T)(g.r0[(size_t)e * c + cp] * g.r0scale);
+		for (int e = tid; e < an; e += FT) Vs[e] = (T)(rstart[(size_t)e * c + cp] * g.r0scale);
 	}
 	__syncthreads();
 
@@ -461,6 +469,7 @@ unsigned long long barrier_ticks() {
 		if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
 		ticks = (unsigned long long)khz * 2000ull;  // 2 s
 	}
+	if (const char* e = getenv("XTT_FUSED_CG_WAIT_TICKS")) return strtoull(e, nullptr, 10);
 	return ticks;
 }
 
@@ -474,12 +483,12 @@ bool fused_cg_fits(size_t a, size_t n, size_t c, size_t P, size_t Q) {
 }
 
 template <class T>
-void fused_cg(const FusedCG& d, hipStream_t s) {
+bool fused_cg(const FusedCG& d, hipStream_t s) {
 	FcgPlan f;
 	XTT_REQUIRE(plan<T>((int)d.a, (int)d.n, (int)d.c, (int)d.P, (int)d.Q, f) && f.G <= max_resident<T>(),
 	            "fused CG: local shape does not fit one cooperative launch");
 	FcgArgs<T> g{};
-	g.L = d.L; g.Am = d.Am; g.R = d.R; g.r0 = d.r0; g.x = d.x;
+	g.L = d.L; g.Am = d.Am; g.R = d.R; g.r0 = d.r0; g.b = d.b; g.x = d.x;
 	g.vec = static_cast<T*>(d.vec); g.T2g = static_cast<T*>(d.T2); g.sync = d.sync; g.epoch = d.epoch; g.st = d.st;
 	g.iterAcc = d.iterAcc;
 	g.r0scale = d.r0scale; g.xscale = d.xscale; g.tol2 = d.tol2; g.flopsPerIter = d.flopsPerIter; g.maxIter = d.maxIter; g.mode = d.mode;
@@ -488,13 +497,20 @@ void fused_cg(const FusedCG& d, hipStream_t s) {
 	g.ldL = f.ldL; g.ldAm = f.ldAm; g.ldR = f.ldR; g.ldT1 = f.ldT1; g.ldT2 = f.ldT2;
 	g.oL = f.oL; g.oAm = f.oAm; g.oR = f.oR; g.oV = f.oV; g.oT1 = f.oT1; g.oT2 = f.oT2; g.oS = f.oS; g.oRc = f.oRc;
 	XTT_REQUIRE(d.maxIter <= 30000 && d.epoch > 0 && d.epoch < (1ull << 47), "fused CG: iteration limit or epoch out of range");
-	hipLaunchKernelGGL(fused_cg_kernel<T>, dim3(f.G), dim3(FT), f.lds, s, g);
-	XTT_HIP(hipGetLastError());
+	void* args[] = {&g};
+	const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(fused_cg_kernel<T>), dim3(f.G), dim3(FT), args,
+	                                                (unsigned)f.lds, s);
+	if (e == hipErrorCooperativeLaunchTooLarge || e == hipErrorLaunchOutOfResources) {
+		(void)hipGetLastError();
+		return false;
+	}
+	XTT_HIP(e);
+	return true;
 }
 
 template bool fused_cg_fits<double>(size_t, size_t, size_t, size_t, size_t);
 template bool fused_cg_fits<float>(size_t, size_t, size_t, size_t, size_t);
-template void fused_cg<double>(const FusedCG&, hipStream_t);
-template void fused_cg<float>(const FusedCG&, hipStream_t);
+template bool fused_cg<double>(const FusedCG&, hipStream_t);
+template bool fused_cg<float>(const FusedCG&, hipStream_t);
 
 }  // namespace xtt

@@ -77,11 +77,14 @@ template <class T> void cg_alpha(const T* p, const T* q, size_t n, CGState* st, 
 template <class T> void cg_update(T* x, T* r, const T* p, const T* q, size_t n, CGState* st, double* partials, hipStream_t s);
 template <class T> void cg_direction(T* p, const T* r, size_t n, CGState* st, hipStream_t s);
 void cg_state_init(CGState* st, double tol2, int maxIter, hipStream_t s);
+// After cg_start with a warm start worse than zero (st->rr > st->rr0): x = 0, r = p = b, st->rr = st->rr0. CG's
+// recursive residual cannot resolve the true one below ε·‖A‖·max‖x_k‖, so a start of large norm caps the accuracy.
+void cg_zero_start(double* x, double* r, double* p, const double* b, size_t n, CGState* st, hipStream_t s);
 
 // Whole CG solve of an ALS local system in one launch with grid-wide barriers (cgfused.hip). Operator pieces
 // are FP64 in the layouts L (a,P,a), Am (n,Q,P,n), R (c,Q,c); the arithmetic type T is double or float.
 //   mode 0: warm start from x (FP64, in/out); r0 = b − A·x and the state (rr0, tol2, maxIter, done, the zeroed
-//           fail flag) come from cg_state_init + cg_start.
+//           fail flag) come from cg_state_init + cg_start. A warm start with ‖r0‖ > ‖b‖ is replaced by x = 0, r = b.
 //   mode 1: correction e from 0 for the right-hand side r0·r0scale; x += xscale·e on exit; tol2 and maxIter are
 //           taken from here, relative to ‖r0·r0scale‖²; the state only needs cg_state_init (the fail flag).
 // Not for concurrent use on several streams: the grid takes every compute unit.
@@ -92,6 +95,7 @@ struct FusedCG {
 	const double* Am = nullptr;
 	const double* R = nullptr;
 	const double* r0 = nullptr;
+	const double* b = nullptr;  // mode 0: the right-hand side, the start residual when the warm start is discarded
 	double* x = nullptr;
 	void* vec = nullptr;       // 7·a·n·c elements of T
 	void* T2 = nullptr;        // a·n·c·Q elements of T
@@ -105,7 +109,12 @@ struct FusedCG {
 };
 constexpr size_t FUSED_CG_SYNC = 1025 * 8;
 template <class T> bool fused_cg_fits(size_t a, size_t n, size_t c, size_t P, size_t Q);
-template <class T> void fused_cg(const FusedCG& d, hipStream_t s);
+// Cooperative launch (every workgroup co-resident or no launch): returns false, with nothing enqueued, when the
+// runtime refuses it (the device cannot hold the grid at once, e.g. while other work occupies it); the caller then
+// solves on the chunked CG. A barrier wait that runs out of time leaves x untouched and marks the solve failed
+// (iterAcc[0] = NaN). XTT_FUSED_CG_WAIT_TICKS (wall-clock ticks, read per launch) overrides the 2 s wait limit;
+// 0 makes every wait give up at once, which exercises the failure path with every workgroup exiting normally.
+template <class T> bool fused_cg(const FusedCG& d, hipStream_t s);
 
 // ---- dense factorisations (one workgroup per matrix, FP64, matrix in device memory) -------------------
 // In-place Householder QR (pivot = 0) or column-pivoted QR (pivot = 1) of a row-major m×n matrix, LAPACK

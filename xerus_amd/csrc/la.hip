@@ -231,7 +231,17 @@ __global__ __launch_bounds__(QT) void jacobi_kernel(const double* A, int m, int 
 		W[e] = c < n ? A[(size_t)r * n + c] : 0.0;
 	}
 	for (size_t e = tid; e < (size_t)np * np; e += QT) V[e] = ((int)(e / np) == (int)(e % np)) ? 1.0 : 0.0;
+	__shared__ double red[QT / 64];
+	double f2 = 0;
+	for (size_t e = tid; e < (size_t)m * n; e += QT) f2 = fma(A[e], A[e], f2);
+	for (int o = 32; o > 0; o >>= 1) f2 += __shfl_xor(f2, o);
+	if ((tid & 63) == 0) red[tid >> 6] = f2;
 	__syncthreads();
+	f2 = 0;
+	for (int w = 0; w < QT / 64; ++w) f2 += red[w];
+	// a column whose squared norm is below ε²·‖A‖²_F/n (norm below ε·s_0) is rounding noise: it takes part in no
+	// rotation (rotating noise against noise never meets the relative orthogonality test)
+	const double negligible = DBL_EPSILON * DBL_EPSILON * f2 / n;
 	const int npairs = np / 2;
 	int G = 64;
 	while (G > 1 && G * npairs > QT) G >>= 1;
@@ -255,7 +265,7 @@ __global__ __launch_bounds__(QT) void jacobi_kernel(const double* A, int m, int 
 					be += __shfl_xor(be, o, G);
 					ga += __shfl_xor(ga, o, G);
 				}
-				if (ga != 0.0 && fabs(ga) > DBL_EPSILON * sqrt(al * be)) {
+				if (fmin(al, be) > negligible && ga != 0.0 && fabs(ga) > DBL_EPSILON * sqrt(al * be)) {
 					if (gl == 0) rotated = 1;
 					const double zeta = (be - al) / (2.0 * ga);
 					const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
@@ -271,6 +281,7 @@ __global__ __launch_bounds__(QT) void jacobi_kernel(const double* A, int m, int 
 		if (!rotated) break;
 		__syncthreads();
 	}
+	const bool unconverged = rotated != 0;  // the 60th sweep still rotated: reported as NaN singular values
 	for (int j = tid; j < np; j += QT) {
 		double s = 0;
 		for (int r = 0; r < m; ++r) { const double x = W[(size_t)j * m + r]; s += x * x; }
@@ -293,7 +304,7 @@ __global__ __launch_bounds__(QT) void jacobi_kernel(const double* A, int m, int 
 		const int rk = ord[j];
 		if (rk < n && c < n) Vt[tout ? (size_t)c * n + rk : (size_t)rk * n + c] = V[e];
 	}
-	for (int j = tid; j < np; j += QT) if (ord[j] < n) S[ord[j]] = sv[j];
+	for (int j = tid; j < np; j += QT) if (ord[j] < n) S[ord[j]] = unconverged ? __builtin_nan("") : sv[j];
 }
 
 // DPP move of a 64-bit value inside 16-lane rows (butterfly partners only: every lane of a row ends with
@@ -335,9 +346,18 @@ __global__ __launch_bounds__(JT) void jacobi_lds_kernel(const double* A, int m, 
 	}
 	for (int e = tid; e < np * np; e += JT) V[e] = (e / np == e % np) ? 1.0 : 0.0;
 	if (tid == 0) rotated = 0;
+	__shared__ double red[JT / 64];
+	double f2 = 0;
+	for (int e = tid; e < m * n; e += JT) f2 = fma(A[e], A[e], f2);
+	for (int o = 32; o > 0; o >>= 1) f2 += __shfl_xor(f2, o);
+	if ((tid & 63) == 0) red[tid >> 6] = f2;
 	__syncthreads();
+	f2 = 0;
+	for (int w = 0; w < JT / 64; ++w) f2 += red[w];
+	const double negligible = DBL_EPSILON * DBL_EPSILON * f2 / n;  // as in jacobi_kernel
 	const int npairs = np / 2;
 	const int grp = tid / JG, gl = tid % JG;
+	bool unconverged = true;  // until a sweep without rotations; reported as NaN singular values (xerus: dgesdd info > 0 throws)
 	for (int sweep = 0; sweep < 60; ++sweep) {
 		for (int t = 0; t < np - 1; ++t) {
 			if (grp < npairs) {
@@ -352,7 +372,7 @@ __global__ __launch_bounds__(JT) void jacobi_lds_kernel(const double* A, int m, 
 				al = row16_sum(al);
 				be = row16_sum(be);
 				ga = row16_sum(ga);
-				if (ga != 0.0 && fabs(ga) > DBL_EPSILON * sqrt(al * be)) {
+				if (fmin(al, be) > negligible && ga != 0.0 && fabs(ga) > DBL_EPSILON * sqrt(al * be)) {
 					if (gl == 0) rotated = 1;
 					const double zeta = (be - al) / (2.0 * ga);
 					const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
@@ -367,7 +387,7 @@ __global__ __launch_bounds__(JT) void jacobi_lds_kernel(const double* A, int m, 
 		}
 		const int any = rotated;
 		__syncthreads();
-		if (!any) break;
+		if (!any) { unconverged = false; break; }
 		if (tid == 0) rotated = 0;
 		__syncthreads();
 	}
@@ -393,7 +413,7 @@ __global__ __launch_bounds__(JT) void jacobi_lds_kernel(const double* A, int m, 
 		const int rk = ord[j];
 		if (rk < n && c < n) Vt[tout ? (size_t)c * n + rk : (size_t)rk * n + c] = V[e];
 	}
-	for (int j = tid; j < np; j += JT) if (ord[j] < n) S[ord[j]] = sv[j];
+	for (int j = tid; j < np; j += JT) if (ord[j] < n) S[ord[j]] = unconverged ? __builtin_nan("") : sv[j];
 }
 
 }  // namespace

@@ -21,7 +21,8 @@ std::unordered_map<int, DeviceState> g_dev;
 
 // Allocation cache: (device, stream, size class rounded to 256 B below 1 MiB, to 64 KiB above) → free blocks.
 // A block freed while a stream is current goes back to that stream's pool, so reuse stays ordered by the
-// stream that last used it.
+// stream that last used it. Blocks under the null stream are idle (their stream had finished all work when they
+// were parked there) and serve any stream.
 struct Block { size_t bytes; int device; };
 using PoolKey = std::tuple<int, hipStream_t, size_t>;
 std::mutex g_amu;
@@ -99,6 +100,7 @@ void* dmalloc(size_t bytes) {
 	{
 		std::lock_guard<std::mutex> lk(g_amu);
 		auto it = g_free.find(PoolKey{dev, st, cls});
+		if (it == g_free.end()) it = g_free.find(PoolKey{dev, nullptr, cls});  // idle blocks: no work pending
 		if (it != g_free.end()) {
 			void* p = it->second;
 			g_free.erase(it);
@@ -133,6 +135,25 @@ void dfree(void* p) {
 	g_free.insert({PoolKey{it->second.device, st, it->second.bytes}, p});
 	g_inUse -= it->second.bytes;
 	g_live.erase(it);
+}
+
+void park_stream_pool(hipStream_t from) {
+	std::lock_guard<std::mutex> lk(g_amu);
+	for (auto it = g_free.begin(); it != g_free.end();) {
+		if (std::get<1>(it->first) == from) {
+			g_free.insert({PoolKey{std::get<0>(it->first), nullptr, std::get<2>(it->first)}, it->second});
+			it = g_free.erase(it);
+		} else {
+			++it;
+		}
+	}
+}
+
+size_t bytes_cached() {
+	std::lock_guard<std::mutex> lk(g_amu);
+	size_t b = 0;
+	for (auto& kv : g_free) b += std::get<2>(kv.first);
+	return b;
 }
 
 void release_cache() {

@@ -225,6 +225,7 @@ void calculate_svd(DTensor& U, std::vector<double>& S, DTensor& Vt, const DTenso
 	}
 	std::vector<double> sh(k);
 	read_back(sh.data(), s.get(), k * 8);
+	if (std::isnan(sh[0])) throw Error(ERR_NUMERIC, "SVD: the one-sided Jacobi iteration did not converge in 60 sweeps");
 	size_t rank = k;
 	if (maxRank != 0) rank = std::min(rank, maxRank);
 	for (size_t j = 1; j < rank; ++j) if (sh[j] <= eps * sh[0]) { rank = j; break; }
@@ -276,6 +277,23 @@ void TT::set_component(size_t k, DTensor t) {
 	if (op) dims[order() + k] = t.dims[2];
 	cores[k] = std::move(t);
 	canonicalized = canonicalized && (corePosition == k);
+}
+
+void require_correct_format(const TT& x) {  // ttNetwork.cpp:290-340 (the checks that apply to a core list)
+	using std::to_string;
+	const size_t d = x.order(), N = x.op ? 2 : 1;
+	XTT_REQUIRE(x.dims.size() == N * d, "TT format: " + to_string(x.dims.size()) + " dimensions for " + to_string(d) + " components");
+	XTT_REQUIRE(!x.canonicalized || d == 0 || x.corePosition < d, "TT format: invalid core position " + to_string(x.corePosition));
+	for (size_t k = 0; k < d; ++k) {
+		const DTensor& c = x.cores[k];
+		XTT_REQUIRE(c.degree() == N + 2, "TT format: component " + to_string(k) + " has degree " + to_string(c.degree()));
+		XTT_REQUIRE(c.dims[1] == x.dims[k] && (!x.op || c.dims[2] == x.dims[d + k]),
+		            "TT format: component " + to_string(k) + " does not match the external dimensions");
+		const size_t left = k == 0 ? 1 : x.cores[k - 1].dims.back();
+		XTT_REQUIRE(c.dims[0] == left, "TT format: component " + to_string(k) + " has left rank " + to_string(c.dims[0]) +
+		                                   (k == 0 ? ", the first component's must be 1" : ", its left neighbour's right rank is " + to_string(left)));
+	}
+	XTT_REQUIRE(d == 0 || x.cores[d - 1].dims.back() == 1, "TT format: the last component's right rank must be 1");
 }
 
 TT tt_zero(const dims_t& dims, bool op) {  // ttNetwork.cpp:57-108
@@ -687,6 +705,152 @@ double tt_residual(const TT& A, const TT& x, const TT& b) {
 		R = qr_r_only(c, c.degree() - 1);
 	}
 	return 0.0;
+}
+
+// ------------------------------------------------------------------------------- TT constructors
+
+// TTNetwork(const Tensor&, eps, maxRanks) (ttNetwork.cpp:112-160): operator modes interleaved (n_0, m_0, n_1, …),
+// ghost modes of size 1 at both ends, then SVDs from the last component to the first on the device, each keeping
+// min(maxRank, #{s_j > eps·s_0}) values and multiplying S into the remaining left part; canonical at 0.
+TT tt_from_tensor(const dims_t& fullDims, const double* host, bool op, double eps, const dims_t& maxRanksIn) {
+	const size_t N = op ? 2 : 1, deg = fullDims.size();
+	XTT_REQUIRE(deg % N == 0, "Number of indicis must be even for TTOperator");
+	XTT_REQUIRE(eps >= 0 && eps < 1, "_eps must be positive and smaller than one.");
+	const size_t d = deg / N;
+	XTT_REQUIRE(d > 0, "a TT of degree 0 is not supported");
+	const dims_t maxRanks = maxRanksIn.empty() ? dims_t(d - 1, SIZE_MAX) : maxRanksIn;
+	XTT_REQUIRE(maxRanks.size() == d - 1, "We need degree/N - 1 maximal ranks");
+	for (size_t r : maxRanks) XTT_REQUIRE(r > 0, "Maximal ranks must be strictly positive.");
+	TT x = tt_zero(fullDims, op);
+	DTensor remains;
+	if (op) {  // (n_0..n_{d-1}, m_0..m_{d-1}) → (n_0, m_0, n_1, m_1, …), on the host (the data arrives from there)
+		const size_t total = prodr(fullDims, 0, deg);
+		std::vector<double> inter(total);
+		dims_t idims(deg);
+		for (size_t i = 0; i < d; ++i) { idims[2 * i] = fullDims[i]; idims[2 * i + 1] = fullDims[d + i]; }
+		std::vector<size_t> idx(deg, 0);  // multi-index of the source, last mode fastest
+		for (size_t e = 0; e < total; ++e) {
+			size_t o = 0;
+			for (size_t i = 0; i < d; ++i) o = (o * idims[2 * i] + idx[i]) * idims[2 * i + 1] + idx[d + i];
+			inter[o] = host[e];
+			for (size_t j = deg; j-- > 0;) { if (++idx[j] < fullDims[j]) break; idx[j] = 0; }
+		}
+		remains = dt_upload(idims, inter.data());
+		sync();  // `inter` is a local
+	} else {
+		remains = dt_upload(fullDims, host);
+	}
+	remains.reshape(cat(cat({1}, remains.dims), {1}));
+	for (size_t pos = d - 1; pos > 0; --pos) {
+		DTensor U, Vt;
+		std::vector<double> S;
+		DBuf<double> sd;
+		calculate_svd(U, S, Vt, remains, 1 + pos * N, maxRanks[pos - 1], eps, &sd);
+		const size_t rk = S.size();
+		scale_by_f64(U.data(), sd.get(), U.size() / rk, rk, false, stream());  // U·S: column j times s_j
+		x.set_component(pos, std::move(Vt));
+		remains = std::move(U);
+	}
+	x.set_component(0, std::move(remains));
+	x.canonicalized = true;
+	x.corePosition = 0;
+	sync();
+	return x;
+}
+
+namespace {
+TT from_host_cores(const dims_t& dims, bool op, const std::vector<std::pair<dims_t, std::vector<double>>>& cores) {
+	TT x = tt_zero(dims, op);
+	for (size_t k = 0; k < cores.size(); ++k) x.set_component(k, dt_upload(cores[k].first, cores[k].second.data()));
+	sync();  // the host vectors are locals
+	return x;
+}
+void check_dims(const dims_t& dims, bool op) {
+	XTT_REQUIRE(dims.size() % (op ? 2 : 1) == 0, "Illegal number of dimensions for ttOperator");
+	XTT_REQUIRE(!dims.empty(), "a TT of degree 0 is not supported");
+	for (size_t n : dims) XTT_REQUIRE(n > 0, "Trying to construct a TTTensor with dimension 0 is not possible.");
+}
+}  // namespace
+
+// TTNetwork::ones (ttNetwork.cpp:170-191): all-ones rank-1 components, then canonicalize_left (move_core(0)).
+TT tt_ones(const dims_t& dims, bool op) {
+	check_dims(dims, op);
+	const size_t d = dims.size() / (op ? 2 : 1);
+	std::vector<std::pair<dims_t, std::vector<double>>> cores;
+	for (size_t k = 0; k < d; ++k) {
+		dims_t cd = op ? dims_t{1, dims[k], dims[d + k], 1} : dims_t{1, dims[k], 1};
+		cores.push_back({cd, std::vector<double>(prodr(cd, 0, cd.size()), 1.0)});
+	}
+	TT x = from_host_cores(dims, op, cores);
+	move_core(x, 0, false);
+	return x;
+}
+
+// TTNetwork<true>::identity (ttNetwork.cpp:194-221): components δ(i_k, j_k), then canonicalize_left.
+TT tt_identity(const dims_t& dims) {
+	check_dims(dims, true);
+	const size_t d = dims.size() / 2;
+	std::vector<std::pair<dims_t, std::vector<double>>> cores;
+	for (size_t k = 0; k < d; ++k) {
+		std::vector<double> h(dims[k] * dims[d + k], 0.0);
+		for (size_t i = 0; i < std::min(dims[k], dims[d + k]); ++i) h[i * dims[d + k] + i] = 1.0;
+		cores.push_back({{1, dims[k], dims[d + k], 1}, std::move(h)});
+	}
+	TT x = from_host_cores(dims, true, cores);
+	move_core(x, 0, false);
+	return x;
+}
+
+// TTNetwork::kronecker (ttNetwork.cpp:224-254): each component a Kronecker tensor (ones where every mode index is
+// equal) with inner ranks min(dims), then canonicalize_left.
+TT tt_kronecker(const dims_t& dims, bool op) {
+	check_dims(dims, op);
+	const size_t d = dims.size() / (op ? 2 : 1);
+	const size_t minN = *std::min_element(dims.begin(), dims.end());
+	std::vector<std::pair<dims_t, std::vector<double>>> cores;
+	for (size_t k = 0; k < d; ++k) {
+		const size_t l = k == 0 ? 1 : minN, r = k + 1 == d ? 1 : minN;
+		const dims_t cd = op ? dims_t{l, dims[k], dims[d + k], r} : dims_t{l, dims[k], r};
+		std::vector<double> h(prodr(cd, 0, cd.size()), 0.0);
+		for (size_t i = 0; i < minN; ++i) {
+			const size_t a = k == 0 ? 0 : i, c = k + 1 == d ? 0 : i;
+			h[op ? ((a * cd[1] + i) * cd[2] + i) * cd[3] + c : (a * cd[1] + i) * cd[2] + c] = 1.0;
+		}
+		cores.push_back({cd, std::move(h)});
+	}
+	TT x = from_host_cores(dims, op, cores);
+	move_core(x, 0, false);
+	return x;
+}
+
+// TTNetwork::dirac (ttNetwork.cpp:257-283): rank-1 components with a single 1 at the position; as in xerus the
+// result is not canonicalized (set_component of a non-core component clears the flag) unless it has one component.
+TT tt_dirac(const dims_t& dims, const dims_t& position, bool op) {
+	check_dims(dims, op);
+	XTT_REQUIRE(dims.size() == position.size(), "Inconsitend number of entries in _dimensions and _position.");
+	for (size_t i = 0; i < dims.size(); ++i) XTT_REQUIRE(position[i] < dims[i], "dirac position out of range");
+	const size_t d = dims.size() / (op ? 2 : 1);
+	std::vector<std::pair<dims_t, std::vector<double>>> cores;
+	for (size_t k = 0; k < d; ++k) {
+		const dims_t cd = op ? dims_t{1, dims[k], dims[d + k], 1} : dims_t{1, dims[k], 1};
+		std::vector<double> h(prodr(cd, 0, cd.size()), 0.0);
+		h[op ? position[k] * dims[d + k] + position[d + k] : position[k]] = 1.0;
+		cores.push_back({cd, std::move(h)});
+	}
+	TT x = from_host_cores(dims, op, cores);
+	if (d == 1) { x.canonicalized = true; x.corePosition = 0; }
+	return x;
+}
+
+// TTOperator::transpose (ttNetwork.h:443-448): each component's two external modes swapped; the row and column
+// dimensions are exchanged with them.
+void tt_transpose(TT& x) {
+	XTT_REQUIRE(x.op, "transpose: not a TTOperator");
+	const size_t d = x.order();
+	for (size_t k = 0; k < d; ++k) {
+		x.cores[k] = reshuffle(x.cores[k], {0, 2, 1, 3});
+		std::swap(x.dims[k], x.dims[d + k]);
+	}
 }
 
 }  // namespace xtt

@@ -57,6 +57,8 @@ struct TT {
 	dims_t ranks() const;
 	void set_component(size_t k, DTensor t);  // ttNetwork.cpp:475-492
 };
+// ttNetwork.cpp:290-340: ranks chain (first and last 1, neighbours equal), degrees and external dimensions
+void require_correct_format(const TT& x);
 TT tt_zero(const dims_t& dims, bool op);
 TT tt_copy(const TT& x);
 dims_t reduce_to_maximal_ranks(dims_t ranks, const dims_t& dims, bool op);
@@ -73,6 +75,13 @@ TT tt_add(const TT& a, const TT& b);
 void tt_scale(TT& x, double s);
 TT tt_apply(const TT& A, const TT& x);   // rank r_A·r_x, uncanonicalized
 double tt_residual(const TT& A, const TT& x, const TT& b);  // ‖Ax − b‖/‖b‖, stable (QR sweep of the TT of Ax − b)
+// TTNetwork(const Tensor&, eps, maxRanks) from a host tensor (an operator's as (n…, m…)); empty maxRanks: no cap
+TT tt_from_tensor(const dims_t& fullDims, const double* host, bool op, double eps, const dims_t& maxRanks);
+TT tt_ones(const dims_t& dims, bool op);
+TT tt_identity(const dims_t& dims);
+TT tt_kronecker(const dims_t& dims, bool op);
+TT tt_dirac(const dims_t& dims, const dims_t& position, bool op);
+void tt_transpose(TT& x);
 
 // xerus datafiles (misc::save_to_file / load_from_file of a TTNetwork): format 0 = BINARY, 1 = TSV.
 void tt_save(const TT& x, const std::string& filename, int format);

@@ -66,7 +66,7 @@ __global__ void svd_rank_kernel(const double* sv, size_t k, size_t maxRank, doub
 	if (maxRank != 0 && maxRank < rank) rank = maxRank;
 	for (size_t j = 1; j < rank; ++j)
 		if (sv[j] <= eps * sv[0]) { rank = j; break; }
-	*out = (int)rank;
+	*out = sv[0] != sv[0] ? 0 : (int)rank;  // an unconverged SVD (NaN values) matches no speculated rank
 }
 __global__ void axpy_kernel(double* y, double a, const double* x, size_t n) {
 	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] += a * x[i];
@@ -225,6 +225,14 @@ __global__ __launch_bounds__(RB) void cg_direction_kernel(T* p, const T* r, size
 	for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) p[i] = r[i] + beta * p[i];
 }
 
+__global__ void cg_zero_start_kernel(double* x, double* r, double* p, const double* b, size_t n, CGState* st) {
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		x[i] = 0.0;
+		r[i] = b[i];
+		p[i] = b[i];
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0) st->rr = st->rr0;
+}
 __global__ void cg_state_init_kernel(CGState* st, double tol2, int maxIter) {
 	if (threadIdx.x == 0 && blockIdx.x == 0) {
 		st->rr = 0; st->rr0 = 0; st->tol2 = tol2; st->alpha = 0; st->beta = 0; st->pq = 0;
@@ -233,6 +241,12 @@ __global__ void cg_state_init_kernel(CGState* st, double tol2, int maxIter) {
 }
 
 }  // namespace
+
+void cg_zero_start(double* x, double* r, double* p, const double* b, size_t n, CGState* st, hipStream_t s) {
+	if (!n) return;
+	hipLaunchKernelGGL(cg_zero_start_kernel, dim3(grid_for(n)), dim3(256), 0, s, x, r, p, b, n, st);
+	XTT_LAUNCH_CHECK();
+}
 
 size_t reduce_blocks(size_t n) {
 	size_t g = (n + RB * 4 - 1) / (RB * 4);

@@ -109,6 +109,10 @@ void* dmalloc(size_t bytes);
 void dfree(void* p);
 void release_cache();
 size_t bytes_in_use();
+size_t bytes_cached();  // free blocks held by the cache
+// Marks the free blocks of stream `from` idle, so that any stream may reuse them. The caller guarantees that
+// `from` has finished all its work (it was synchronised); a stream about to be destroyed parks its pool this way.
+void park_stream_pool(hipStream_t from);
 
 template <class T>
 struct DBuf {  // owning device buffer of T

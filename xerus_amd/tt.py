@@ -94,6 +94,50 @@ class _TTNetwork:
             t.assume_core_position(core_position)
         return t
 
+    @classmethod
+    def from_tensor(cls, tensor, eps: float = EPSILON, maxRanks=None):
+        """TTNetwork(const Tensor&, eps, maxRanks) (ttNetwork.cpp:112-160): TT-SVD of a dense array (an operator's
+        array is indexed (n_0..n_{d-1}, m_0..m_{d-1})); maxRanks a list, a number or None (no cap)."""
+        full = np.ascontiguousarray(tensor, dtype=np.float64)
+        _, pd = _sizes(full.shape)
+        d = full.ndim // (2 if cls._is_operator else 1)
+        pr = None
+        if maxRanks is not None:
+            ranks = [int(maxRanks)] * (d - 1) if np.isscalar(maxRanks) else list(maxRanks)
+            _keep, pr = _sizes(ranks or [0])
+        h = C.c_void_p()
+        call("xt_tt_from_tensor", int(cls._is_operator), full.ndim, pd, full.ctypes.data_as(C.POINTER(C.c_double)), float(eps),
+             pr, C.byref(h))
+        return cls(_handle=h)
+
+    @classmethod
+    def ones(cls, dims):
+        """TTNetwork::ones (ttNetwork.cpp:170-191): the all-ones tensor, rank 1, canonical at 0."""
+        _, pd = _sizes(list(dims))
+        h = C.c_void_p()
+        call("xt_tt_ones", int(cls._is_operator), len(list(dims)), pd, C.byref(h))
+        return cls(_handle=h)
+
+    @classmethod
+    def kronecker(cls, dims):
+        """TTNetwork::kronecker (ttNetwork.cpp:224-254): ones where every index is equal."""
+        _, pd = _sizes(list(dims))
+        h = C.c_void_p()
+        call("xt_tt_kronecker", int(cls._is_operator), len(list(dims)), pd, C.byref(h))
+        return cls(_handle=h)
+
+    @classmethod
+    def dirac(cls, dims, position):
+        """TTNetwork::dirac (ttNetwork.cpp:257-283): a single 1 at `position` (a multi-index or a linear index)."""
+        dims = list(dims)
+        if isinstance(position, (int, np.integer)):
+            position = list(np.unravel_index(int(position), dims))
+        _, pd = _sizes(dims)
+        _, pp = _sizes([int(v) for v in position])
+        h = C.c_void_p()
+        call("xt_tt_dirac", int(cls._is_operator), len(dims), pd, pp, C.byref(h))
+        return cls(_handle=h)
+
     def copy(self):
         h = C.c_void_p()
         call("xt_tt_copy", self._h, C.byref(h))
@@ -256,6 +300,18 @@ class TTTensor(_TTNetwork):
 
 class TTOperator(_TTNetwork):
     _is_operator = True
+
+    @classmethod
+    def identity(cls, dims):
+        """TTOperator::identity (ttNetwork.cpp:194-221): δ(i_k, j_k) in every component; dims = [n…, m…]."""
+        _, pd = _sizes(list(dims))
+        h = C.c_void_p()
+        call("xt_tt_identity", len(list(dims)), pd, C.byref(h))
+        return cls(_handle=h)
+
+    def transpose(self) -> None:
+        """TTOperator::transpose (ttNetwork.h:443-448): swaps the row and column modes of every component."""
+        call("xt_tt_transpose", self._h)
 
     def __call__(self, x: TTTensor) -> TTTensor:
         """A·x as a TT of rank r_A·r_x (ttStack.cpp:197-310), not canonicalized."""
