@@ -247,6 +247,16 @@ int xt_tt_sample_adjoint_f32(int d, const int* n, const int* r, const float* cor
 int xt_tt_sample_residual_adjoint_f32(int d, const int* n, const int* r, const float* cores, const uint32_t* pos, int P,
                                       const float* b, float* resid, double* sums, float* grads, void* stream);
 
+/* No counterpart in xerus. For callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid allocation): xt_tt_sample_residual_adjoint_f32's one-pass kernels with every scratch
+ * buffer (block partials, padded cores, descriptors) in a static 16 MiB device array per GPU, held until the process
+ * ends. Same inputs and outputs except that `sums` is a DEVICE pointer to 2 doubles (nullable) and the call does not
+ * synchronise: it is ordered on `stream` (NULL = the library stream). Limits (XT_ERR_UNSUPPORTED beyond them):
+ * order 2..16, ranks <= 8, n_k <= 16, Σ_k n_k <= 180 and Σ_k r_k·n_k·r_{k+1} <= 16314 floats. Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU (several streams or threads) race. */
+int xt_tt_sample_residual_adjoint_f32_noalloc(int d, const int* n, const int* r, const float* cores, const uint32_t* pos, int P,
+                                              const float* b, float* resid, double* sums, float* grads, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -150,6 +150,7 @@ SIGNATURES = {
     "xt_tt_sample_residual_f32": (C.c_int, [C.c_int, _pi, _pi, _vp, _vp, C.c_int, _vp, _vp, _pd, _vp]),
     "xt_tt_sample_adjoint_f32": (C.c_int, [C.c_int, _pi, _pi, _vp, _vp, C.c_int, _vp, _vp, _vp]),
     "xt_tt_sample_residual_adjoint_f32": (C.c_int, [C.c_int, _pi, _pi, _vp, _vp, C.c_int, _vp, _vp, _pd, _vp, _vp]),
+    "xt_tt_sample_residual_adjoint_f32_noalloc": (C.c_int, [C.c_int, _pi, _pi, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -710,6 +710,9 @@ bool adf_residual_gradient_fused_ok(const AdfDesc& a) {
 	return fast_ok(a) && a.d >= 2 && a.maxN * FR <= GNT * 16 && fast_grad_floats(a.d, a.sumN) * sizeof(float) <= 150 * 1024;
 }
 
+static_assert(ADF_PAD_OFFSETS >= FD, "padded core offsets");
+int adf_pad_floats(const AdfDesc& a) { return a.sumN * PB; }
+
 // resid = x(pos) − values, norms = {‖resid‖², ‖values‖²} and grad (the adjoint with seed resid) in one gradient
 // launch where the fast kernel applies; otherwise adf_residual followed by adf_gradient.
 void adf_residual_gradient_f32(const AdfDesc& a, const float* cores, const uint32_t* pos, const float* values, float* resid, double* norms,
@@ -719,14 +722,26 @@ void adf_residual_gradient_f32(const AdfDesc& a, const float* cores, const uint3
 		adf_gradient<float>(a, cores, pos, resid, grad, partials, -1, s);
 		return;
 	}
-	Padded pc = pad_cores(a, cores, s);
+	Padded pc;
+	pc.Gp.resize((size_t)adf_pad_floats(a));
+	pc.off.resize(ADF_PAD_OFFSETS);
+	adf_residual_gradient_f32_into(a, cores, pos, values, resid, norms, grad, partials, normPart, pc.Gp.get(), pc.off.get(), s);
+}
+
+void adf_residual_gradient_f32_into(const AdfDesc& a, const float* cores, const uint32_t* pos, const float* values, float* resid,
+                                    double* norms, float* grad, float* partials, double* normPart, float* Gp, long long* padOff,
+                                    hipStream_t s) {
+	XTT_REQUIRE(adf_residual_gradient_fused_ok(a), "ADF: shape outside the one-pass residual and gradient kernel");
+	const int total = adf_pad_floats(a);
+	hipLaunchKernelGGL(pad_cores_kernel, dim3(std::max(1, std::min(1024, (total + 255) / 256))), dim3(256), 0, s, a, cores, padOff, Gp,
+	                   total);
+	XTT_LAUNCH_CHECK();
 	allow_fast_grad_lds();
 	const int blocks = std::min(adf_grad_blocks(a.P), 256);
 	// positions and values of every sample, residuals out, the cores in and the gradient out
 	launch_timed("adf_grad_fast_kernel<float> residual", Work(0.0, (double)a.P * (4.0 * a.d + 8.0) + 8.0 * a.gradSize),
 	             adf_grad_fast_kernel<true>, dim3(blocks), dim3(FW * 64), fast_grad_floats(a.d, a.sumN) * sizeof(float), s, a,
-	             (const float*)pc.Gp.get(), (const long long*)pc.off.get(), pos, (const float*)nullptr, partials, -1, values, resid,
-	             normPart);
+	             (const float*)Gp, (const long long*)padOff, pos, (const float*)nullptr, partials, -1, values, resid, normPart);
 	hipLaunchKernelGGL(adf_grad_reduce_kernel<float>, dim3((a.gradSize + RE - 1) / RE), dim3(256), 0, s, (const float*)partials, blocks,
 	                   a.gradSize, grad);
 	XTT_LAUNCH_CHECK();

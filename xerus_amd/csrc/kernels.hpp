@@ -174,6 +174,14 @@ void adf_gradient(const AdfDesc& a, const T* cores, const uint32_t* pos, const T
 bool adf_residual_gradient_fused_ok(const AdfDesc& a);
 void adf_residual_gradient_f32(const AdfDesc& a, const float* cores, const uint32_t* pos, const float* values, float* resid, double* norms,
                                float* grad, float* partials, double* normPart, unsigned* counter, hipStream_t s);
+// The one-pass path alone (requires adf_residual_gradient_fused_ok) with caller-provided buffers for the padded cores:
+// Gp holds adf_pad_floats(a) floats, padOff ADF_PAD_OFFSETS long longs; partials min(adf_grad_blocks(P), 256)·gradSize
+// floats, normPart 2·256 doubles. Allocates nothing.
+constexpr int ADF_PAD_OFFSETS = 16;
+int adf_pad_floats(const AdfDesc& a);
+void adf_residual_gradient_f32_into(const AdfDesc& a, const float* cores, const uint32_t* pos, const float* values, float* resid,
+                                    double* norms, float* grad, float* partials, double* normPart, float* Gp, long long* padOff,
+                                    hipStream_t s);
 // Rank-one measurements (U: P × sumN, measurement i's vectors u_{i,0..d-1} concatenated):
 // value_i = x(u_i) with core kSel replaced by D (kSel < 0: none); resid_i = value_i − b_i (values/resid may be
 // null: b = 0 / not stored); norms2 = {Σ resid², Σ b²}. `partials` holds 2·r1_blocks(P) doubles.
