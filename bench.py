@@ -52,6 +52,41 @@ def run_als(xe, A, b, x0, precision, steps):
     return time.perf_counter() - t0, x, stats
 
 
+def per_step(xe, A, b, x0, precision, steps):
+    """The same steps one at a time, the device synchronised around ALS and around round: wall time of each part, inner
+    CG iterations and FP64 refinements per step (a separate pass; the timed value comes from run_als)."""
+    x = x0.copy()
+    als = xe.ALS_SPD.copy()
+    als.convergenceEpsilon = 0.0
+    als.precision = precision
+    out = []
+    sync = xe.lib().xt_synchronize
+    for _ in range(steps):
+        sync()
+        t0 = time.perf_counter()
+        als(A, x, b, 2)
+        sync()
+        t1 = time.perf_counter()
+        x.round(1e-12)
+        sync()
+        t2 = time.perf_counter()
+        out.append({"als_ms": round(1e3 * (t1 - t0), 3), "round_ms": round(1e3 * (t2 - t1), 3),
+                    "cg_iterations": als.last_stats["cgIterations"], "refinements": als.last_stats["refinements"],
+                    "max_rank": max(x.ranks())})
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def adf_workload(xe, np):
     d, n, r, P = 10, 10, 8, 200_000
     rng = np.random.default_rng(SEED + 5)
@@ -117,6 +152,56 @@ def bench_adf(xe, np, steps):
             "grad_kernel_ms": g[1] / g[0] if g else None, "residual_kernel_ms": rk[1] / rk[0] if rk else None}
 
 
+def bench_adf_solve(xe, np, iterations):
+    """ADFVariant(iterations, 0, 1) at d=10, n=10, r=8, P=200k (FP32 kernels, FP64 cores and norms): every iteration is a
+    full sweep of residual, gradient, slice norms and update_x over all cores plus the QR moves; ADF iterations/s."""
+    d, n, r, P, x, m = adf_workload(xe, np)
+    adf = xe.ADFVariant(iterations, 0.0, 1.0)  # no early stop: a fixed number of sweeps
+    adf.precision = xe.XT_F32
+    y = x.copy()
+    xe.ADFVariant(1, 0.0, 1.0)(y, m)  # warm-up (allocations, descriptors)
+    y = x.copy()
+    xe.lib().xt_synchronize()
+    t0 = time.perf_counter()
+    res = adf(y, m)
+    xe.lib().xt_synchronize()
+    dt = time.perf_counter() - t0
+    return {"iterations_per_s": adf.last_iterations / dt, "iterations": adf.last_iterations, "seconds": dt,
+            "residual": res, "precision": "f32 samples, f64 cores"}
+
+
+def adf_cpu_baseline(np, iterations):
+    """The oracle's ADF (FP64, OpenMP) on the same workload for a bounded number of iterations."""
+    from oracle import oracle as O
+    d, n, r, P = 10, 10, 8, 200_000
+    O.seed(SEED)
+
+    def tt(scale):
+        cores = []
+        for k in range(d):
+            rl = 1 if k == 0 else r
+            rr = 1 if k == d - 1 else r
+            cores.append(O.normal_draws(rl * n * rr).reshape(rl, n, rr) * scale)
+        return O.TT.from_cores(cores)
+
+    truth = tt(1 / np.sqrt(8))
+    x = tt(1 / np.sqrt(8))
+    m = O.Measurements.random(P, [n] * d)
+    m.measure(truth)
+    m.set_values(m.values() + np.random.default_rng(SEED + 5).normal(0.0, 1e-2, P))
+    t0 = time.perf_counter()
+    grads = 3
+    for _ in range(grads):
+        O.adf_gradient(x, m)
+    tg = (time.perf_counter() - t0) / grads
+    t0 = time.perf_counter()
+    O.adf_solve(x, m, max_iterations=iterations, target=0.0, min_decrease=1.0)
+    ts = (time.perf_counter() - t0) / iterations
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"grads_per_s": 1.0 / tg, "iterations_per_s": 1.0 / ts, "cores": threads, "kind": "port", "cpu": cpu_model(),
+            "sample": f"{grads} full gradients and {iterations} ADF iterations of the same d=10 n=10 r=8 P=200k workload"}
+
+
 def bench_batch(xe, np, count, sweeps, threads, precision):
     """Many independent ALS_SPD + round problems of uneven size (d ~ U{6..20}, n ∈ {2,4,8,16}, r ~ U{4..64},
     rank-3 Laplacian + potential operators, rank-4 right-hand sides) through the batched driver: worker threads
@@ -157,7 +242,7 @@ def cpu_baseline(np, steps):
         x.round(eps=1e-12)
     dt = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": steps / dt, "unit": "sweeps/s", "cores": threads, "kind": "port",
+    return {"value": steps / dt, "unit": "sweeps/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
             "sample": f"the full timed workload ({steps} steps of ALS_SPD(2 half-sweeps) + round(1e-12), d16 n16 r64, "
                       f"local CG tol 1e-12) on the host, {dt:.1f} s"}
 
@@ -184,6 +269,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=["f32", "bf16"], default="f32")
     ap.add_argument("--adf-steps", type=int, default=200)
+    ap.add_argument("--adf-iterations", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-problems", type=int, default=512)
@@ -231,13 +317,18 @@ def main():
     xe._lib.profile_enable(False)
     print(f"# mixed: {t_mixed:.2f} s, residual {res_mixed:.3e}, ranks {x_mixed.ranks()}", file=sys.stderr, flush=True)
 
+    steps_mixed = per_step(xe, A, b, x0, prec, args.steps)
+    print(f"# per step (mixed): {[(p['als_ms'], p['round_ms'], p['cg_iterations']) for p in steps_mixed]}", file=sys.stderr,
+          flush=True)
+
     fp64 = None
     if not args.no_fp64:
         t64, x64, st64 = run_als(xe, A, b, x0, xe.XT_F64, args.steps)
         res64 = xe.residual(A, x64, b)
         diff = (x_mixed - x64).frob_norm() / x64.frob_norm()
         fp64 = {"sweeps_per_s": args.steps / t64, "ms_per_step": 1e3 * t64 / args.steps, "rel_residual": res64,
-                "cg_iterations": st64["cgIterations"], "rel_diff_mixed_vs_fp64": diff}
+                "cg_iterations": st64["cgIterations"], "rel_diff_mixed_vs_fp64": diff,
+                "per_step": per_step(xe, A, b, x0, xe.XT_F64, args.steps)}
         print(f"# fp64: {t64:.2f} s, residual {res64:.3e}", file=sys.stderr, flush=True)
 
     # dominant kernel of the timed workload (largest total time among the profiled launches); its bound is the
@@ -258,10 +349,25 @@ def main():
                 "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_src,
                 "kernels_ms": {k: round(v[1], 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])[:8]}}
 
+    # MFMA use of the contraction kernels (the environment / stack contractions and local matvecs run there): their
+    # algorithmic flops over their summed kernel time, against the peak of their arithmetic type
+    mm = {"f64": [0.0, 0.0], "f32": [0.0, 0.0], "bf16": [0.0, 0.0]}
+    for tag, (_n, ms, fl, _by) in prof.items():
+        if tag.startswith("gemm"):
+            key = "bf16" if "bf16" in tag else ("f32" if "float" in tag else "f64")
+            mm[key][0] += fl
+            mm[key][1] += ms
+    contraction_mfma = {k: {"tflops": v[0] / (v[1] * 1e-3) / 1e12, "frac_of_peak": v[0] / (v[1] * 1e-3) / 1e12 / PEAK_TFLOPS[k],
+                            "kernel_ms": v[1]} for k, v in mm.items() if v[1] > 0}
+
     adf = bench_adf(xe, np, args.adf_steps) if args.adf_steps > 0 else None
     if adf and adf["grad_kernel_ms"]:
         adf["grad_kernel_GBs"] = adf["alg_bytes"] / (adf["grad_kernel_ms"] * 1e-3) / 1e9
         adf["grad_kernel_hbm_frac"] = adf["grad_kernel_GBs"] / PEAK_HBM_GBS
+
+    adf_solve_r = bench_adf_solve(xe, np, args.adf_iterations) if args.adf_iterations > 0 else None
+    if adf_solve_r:
+        print(f"# ADF solve: {adf_solve_r['iterations_per_s']:.1f} iterations/s", file=sys.stderr, flush=True)
 
     batch = None
     if args.batch_problems > 0:
@@ -269,6 +375,11 @@ def main():
         print(f"# batch: {batch['sweeps_per_s']:.1f} sweeps/s over {args.batch_problems} problems", file=sys.stderr, flush=True)
 
     cpu = None if args.no_cpu else cpu_baseline(np, args.steps)
+    adf_cpu = None if (args.no_cpu or not adf_solve_r) else adf_cpu_baseline(np, 2)
+    if adf and adf_cpu:
+        adf["cpu_grads_per_s"] = adf_cpu["grads_per_s"]
+    if adf_solve_r and adf_cpu:
+        adf_solve_r["cpu_baseline"] = adf_cpu
 
     value = args.steps / t_mixed
     out = {
@@ -287,10 +398,13 @@ def main():
         "config": {"workload": "ALS_SPD d=16 n=16 x-rank 64, A rank 3 (Laplacian + U(0,1) potential), b rank 4; "
                                "step = 2 half-sweeps + round(1e-12)",
                    "precision": args.precision, "rel_residual": res_mixed, "ranks_final": max(x_mixed.ranks()),
-                   "cg_iterations": st_mixed["cgIterations"], "refinements": st_mixed["refinements"]},
+                   "cg_iterations": st_mixed["cgIterations"], "refinements": st_mixed["refinements"],
+                   "per_step": steps_mixed},
         "fp64_gpu": fp64,
         "speedup_vs_fp64": (value / fp64["sweeps_per_s"]) if fp64 else None,
         "adf": adf,
+        "adf_solve": adf_solve_r,
+        "contraction_mfma": contraction_mfma,
         "batch": batch,
         "roofline": roofline,
         "cpu_baseline": cpu,

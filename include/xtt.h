@@ -90,6 +90,19 @@ int xt_svd(const double* A, size_t m, size_t n, double* U, double* S, double* Vt
 int xt_reshuffle(size_t degree, const size_t* dims, const size_t* shuffle, const double* in, double* out);
 /* blasWrapper::dot_product (blasLapackWrapper.h:55-58); result to host */
 int xt_dot(const double* x, const double* y, size_t n, double* result);
+/* blasWrapper::one_norm Σ|x_i| and two_norm ‖x‖₂ (blasLapackWrapper.h:42-46; dasum / dnrm2, the latter scaled by
+ * max|x_i| so the squares neither overflow nor underflow); results to host */
+int xt_one_norm(const double* x, size_t n, double* result);
+int xt_two_norm(const double* x, size_t n, double* result);
+/* blasWrapper::matrix_vector_product (blasLapackWrapper.h:52-53): x (m) = alpha·op(A)·y with A row-major m×n
+ * (transposed = 0) or n×m (transposed = 1, x = alpha·Aᵀ·y), y of length n */
+int xt_matrix_vector_product(double* x, size_t m, double alpha, const double* A, size_t n, int transposed, const double* y,
+                             void* stream);
+/* blasWrapper::dyadic_vector_product (blasLapackWrapper.h:55-56): A (m×n, row-major) = alpha·x·yᵀ (overwritten) */
+int xt_dyadic_vector_product(double* A, size_t m, size_t n, double alpha, const double* x, const double* y, void* stream);
+/* blasWrapper::solve_least_squares (blasLapackWrapper.h:136-137, .cpp:654-663): X (n×p) = the minimum-norm solution
+ * of min ‖A·X − B‖ (A m×n, B m×p) with dgelsd's cut-off rcond = 8·DBL_EPSILON, for square A as well */
+int xt_solve_least_squares(double* X, const double* A, size_t m, size_t n, const double* B, size_t p);
 
 /* ------------------------------------------------------ boundary 2: TTTensor / TTOperator (ttNetwork.h) */
 /* TTNetwork(dims): rank-1 zero TT, canonical at 0 (ttNetwork.cpp:57-108) */

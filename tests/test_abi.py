@@ -102,3 +102,23 @@ def test_torch_binding_rejects_inconsistent_ranks():
             "    except ValueError as e:\n        print('refused', 'rank' in str(e))\n") % ROOT
     out = subprocess.run([sys.executable, "-c", prog], capture_output=True, text=True, timeout=300)
     assert out.stdout.split("\n")[:3] == ["refused True"] * 3, out.stdout + out.stderr
+
+
+def test_cpp_faces_compile(tmp_path):
+    """The C++ face with xerus's names (include/xtt_xerus.hpp) and the blasWrapper shim shown in INTEGRATION.md compile
+    against include/xtt.h (syntax and types; linking and running are the GPU test's)."""
+    import subprocess
+    inc = os.path.join(ROOT, "include")
+    src = tmp_path / "face.cpp"
+    src.write_text('#include "xtt_xerus.hpp"\nint main() { xerus::ALSVariant v = xerus::DMRG_SPD; return (int)v.sites - 2; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-I", inc, str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```cpp\n(.*?)```", text, flags=re.S)
+    assert blocks
+    for i, b in enumerate(blocks):
+        f = tmp_path / f"shim{i}.cpp"
+        f.write_text(b)
+        r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-I", inc, str(f)], capture_output=True, text=True)
+        assert r.returncode == 0, f"INTEGRATION.md C++ block {i}:\n{r.stderr}"

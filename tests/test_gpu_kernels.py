@@ -306,3 +306,55 @@ def test_sampled_residual_adjoint_fused_out_of_range(xe, oracle):
     np.testing.assert_array_equal(rf, rs)
     np.testing.assert_array_equal(gf, gs)
     assert np.all(np.isfinite(gf))
+
+
+def test_blas_level1_level2(xe):
+    """one_norm (dasum), two_norm (dnrm2, scaled: entries of 1e200 and 1e-200 neither overflow nor vanish),
+    matrix_vector_product in both orientations and dyadic_vector_product (blasLapackWrapper.h:42-56) against numpy."""
+    from xerus_amd._lib import call
+    print("blas level 1 and 2", flush=True)
+    rng = np.random.default_rng(3)
+    for n in (1, 7, 1000, 100003):
+        x = rng.standard_normal(n)
+        out = ctypes.c_double()
+        dx = dev(x)
+        call("xt_one_norm", ptr(dx), n, ctypes.byref(out))
+        assert abs(out.value - np.abs(x).sum()) <= 1e-12 * np.abs(x).sum()
+        call("xt_two_norm", ptr(dx), n, ctypes.byref(out))
+        assert abs(out.value - np.linalg.norm(x)) <= 1e-13 * np.linalg.norm(x)
+    for scale in (1e200, 1e-200):
+        x = rng.standard_normal(513) * scale
+        out = ctypes.c_double()
+        dx = dev(x)  # device buffers stay referenced for the whole call (a temporary would be freed before it)
+        call("xt_two_norm", ptr(dx), 513, ctypes.byref(out))
+        ref = scale * np.linalg.norm(x / scale)
+        assert np.isfinite(out.value) and abs(out.value - ref) <= 1e-13 * ref
+    for m, n in ((1, 5), (5, 1), (37, 200), (300, 64)):
+        A = rng.standard_normal((m, n))
+        y = rng.standard_normal(n)
+        out, dA, dy = empty(m), dev(A), dev(y)
+        call("xt_matrix_vector_product", ptr(out), m, 2.5, ptr(dA), n, 0, ptr(dy), None)
+        assert np.abs(host(out) - 2.5 * A @ y).max() <= 1e-12 * max(1.0, np.abs(A @ y).max())
+        At = rng.standard_normal((n, m))
+        out, dAt = empty(m), dev(At)
+        call("xt_matrix_vector_product", ptr(out), m, -1.0, ptr(dAt), n, 1, ptr(dy), None)
+        assert np.abs(host(out) + At.T @ y).max() <= 1e-12 * max(1.0, np.abs(At.T @ y).max())
+        xv, yv = rng.standard_normal(m), rng.standard_normal(n)
+        D, dxv, dyv = dev(np.full((m, n), 9.0)), dev(xv), dev(yv)
+        call("xt_dyadic_vector_product", ptr(D), m, n, 0.5, ptr(dxv), ptr(dyv), None)
+        assert np.abs(host(D) - 0.5 * np.outer(xv, yv)).max() <= 1e-14 * max(1.0, np.abs(np.outer(xv, yv)).max())
+
+
+@pytest.mark.parametrize("m,n,p,rank", [(8, 8, 1, 8), (8, 8, 3, 5), (40, 12, 2, 12), (12, 40, 2, 12), (30, 20, 1, 7)])
+def test_solve_least_squares_entry(xe, m, n, p, rank):
+    """solve_least_squares (blasLapackWrapper.cpp:654-663, dgelsd): the minimum-norm least squares solution, for square
+    and rank-deficient systems too, equal to numpy's pseudo-inverse solution."""
+    from xerus_amd._lib import call
+    print(f"least squares {m}x{n} p={p} rank={rank}", flush=True)
+    rng = np.random.default_rng(m * 100 + n)
+    A = rng.standard_normal((m, rank)) @ rng.standard_normal((rank, n))
+    B = rng.standard_normal((m, p))
+    X, dA, dB = empty((n, p)), dev(A), dev(B)
+    call("xt_solve_least_squares", ptr(X), ptr(dA), m, n, ptr(dB), p)
+    ref = np.linalg.pinv(A) @ B
+    assert np.abs(host(X) - ref).max() <= 1e-9 * max(1.0, np.abs(ref).max())

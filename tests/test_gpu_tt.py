@@ -251,7 +251,7 @@ def test_als_speculative_calls_match_normal(xe, oracle, prec, monkeypatch):
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("shape", ["config1", "d16n16r64"])
 def test_als_fused_cg_matches_chunked(xe, oracle, prec, shape, monkeypatch):
-    """The fused local CG (one cooperative launch per solve, Chronopoulos–Gear recurrence) and the chunked
+    """The fused local CG (one grid-synchronised launch per solve, Chronopoulos–Gear recurrence) and the chunked
     standard CG (16-iteration graph chunks) solve the same local systems to the same tolerance: the ALS results
     agree to that tolerance and the inner iteration counts are close."""
     print(f"als fused cg {prec} {shape}", flush=True)

@@ -84,6 +84,11 @@ SIGNATURES = {
     "xt_als_local_cg": (C.c_int, [_vp, _vp, _vp, _sz, _sz, _sz, _sz, _sz, _vp, _vp, C.c_double, C.c_int, C.POINTER(C.c_size_t)]),
     "xt_svd": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp]),
     "xt_reshuffle": (C.c_int, [_sz, _ps, _ps, _vp, _vp]),
+    "xt_one_norm": (C.c_int, [_vp, _sz, _pd]),
+    "xt_two_norm": (C.c_int, [_vp, _sz, _pd]),
+    "xt_matrix_vector_product": (C.c_int, [_vp, _sz, C.c_double, _vp, _sz, C.c_int, _vp, _vp]),
+    "xt_dyadic_vector_product": (C.c_int, [_vp, _sz, _sz, C.c_double, _vp, _vp, _vp]),
+    "xt_solve_least_squares": (C.c_int, [_vp, _vp, _sz, _sz, _vp, _sz]),
     "xt_dot": (C.c_int, [_vp, _vp, _sz, _pd]),
     "xt_tt_new": (C.c_int, [C.c_int, _sz, _ps, C.POINTER(_vp)]),
     "xt_tt_random": (C.c_int, [C.c_int, _sz, _ps, _ps, C.POINTER(_vp)]),

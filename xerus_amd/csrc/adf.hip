@@ -790,6 +790,46 @@ void adf_slice_norms(const AdfDesc& a, const T* cores, const uint32_t* pos, int 
 	XTT_LAUNCH_CHECK();
 }
 
+namespace {
+// update_x of adf.cpp:468-487 on the device: with the gradient g of ½Σ(x − b)² (xerus's Δ = −g), slice j of the
+// core moves by (‖g_j‖²/normA[j])·Δ_j (single points), or the whole core by (‖g‖²/normA[0])·Δ (rank-one).
+template <class T>
+__global__ __launch_bounds__(256) void adf_update_kernel(double* core, const T* g, const double* normA, int rl, int n, int rr, int perSlice) {
+	__shared__ double red[4];
+	const int tid = threadIdx.x;
+	if (perSlice) {
+		for (int j = tid; j < n; j += 256) {
+			double pyr = 0;
+			for (int al = 0; al < rl; ++al)
+				for (int c = 0; c < rr; ++c) { const double t = (double)g[((size_t)al * n + j) * rr + c]; pyr += t * t; }
+			const double st = pyr / normA[j];
+			for (int al = 0; al < rl; ++al)
+				for (int c = 0; c < rr; ++c) {
+					const size_t e = ((size_t)al * n + j) * rr + c;
+					core[e] -= st * (double)g[e];
+				}
+		}
+		return;
+	}
+	const int total = rl * n * rr;
+	double pyr = 0;
+	for (int e = tid; e < total; e += 256) { const double t = (double)g[e]; pyr += t * t; }
+	for (int o = 32; o > 0; o >>= 1) pyr += __shfl_xor(pyr, o);
+	if ((tid & 63) == 0) red[tid >> 6] = pyr;
+	__syncthreads();
+	const double st = (red[0] + red[1] + red[2] + red[3]) / normA[0];
+	for (int e = tid; e < total; e += 256) core[e] -= st * (double)g[e];
+}
+}  // namespace
+
+template <class T>
+void adf_update_core(double* core, const T* g, const double* normA, int rl, int n, int rr, bool perSlice, hipStream_t s) {
+	hipLaunchKernelGGL(adf_update_kernel<T>, dim3(1), dim3(256), 0, s, core, g, normA, rl, n, rr, perSlice ? 1 : 0);
+	XTT_LAUNCH_CHECK();
+}
+template void adf_update_core<float>(double*, const float*, const double*, int, int, int, bool, hipStream_t);
+template void adf_update_core<double>(double*, const double*, const double*, int, int, int, bool, hipStream_t);
+
 template void adf_residual<float>(const AdfDesc&, const float*, const uint32_t*, const float*, float*, double*, double*, unsigned*, hipStream_t);
 template void adf_residual<double>(const AdfDesc&, const double*, const uint32_t*, const double*, double*, double*, double*, unsigned*, hipStream_t);
 template void adf_gradient<float>(const AdfDesc&, const float*, const uint32_t*, const float*, float*, float*, int, hipStream_t);

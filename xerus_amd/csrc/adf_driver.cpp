@@ -86,8 +86,16 @@ void Flat<T>::build(const TT& x) {
 }
 
 template <class T>
-void Flat<T>::refresh(const TT& x) {
-	for (size_t k = 0; k < x.order(); ++k) {
+bool Flat<T>::matches(const TT& x) const {
+	if (hn.size() != x.order()) return false;
+	for (size_t k = 0; k < x.order(); ++k)
+		if (hn[k] != (int)x.cores[k].dims[1] || hrank[k] != (int)x.cores[k].dims[0] || hrank[k + 1] != (int)x.cores[k].dims[2]) return false;
+	return true;
+}
+
+template <class T>
+void Flat<T>::refresh(const TT& x, size_t first, size_t last) {
+	for (size_t k = first; k < x.order() && k <= last; ++k) {
 		const DTensor& c = x.cores[k];
 		if constexpr (sizeof(T) == 8) {
 			XTT_HIP(hipMemcpyAsync(cores.get() + hoff[k], c.data(), c.size() * 8, hipMemcpyDeviceToDevice, stream()));
@@ -315,24 +323,11 @@ struct SinglePointOps {
 	void gradient(const AdfDesc& a, const Flat<T>& f, AdfWork<T>& w, int k) const {
 		adf_gradient<T>(a, f.cores.get(), m.dpos.get(), w.resid.get(), w.grad.get(), w.partials.get(), k, stream());
 	}
-	// x_k += Δ-step, from the gradient g of core k (host) and the slice norms
-	void step(const AdfDesc& a, const Flat<T>& f, AdfWork<T>& w, size_t k, const DTensor& core, const std::vector<T>& g,
-	          std::vector<double>& upd) const {
-		const size_t rl = core.dims[0], n = core.dims[1], rr = core.dims[2];
-		adf_slice_norms<T>(a, f.cores.get(), m.dpos.get(), (int)k, w.grad.get() + f.hoff[k], (int)n, w.slices.get(), w.dpart.get(),
-		                   stream());
-		std::vector<double> normA(n);
-		XTT_HIP(hipMemcpyAsync(normA.data(), w.slices.get(), n * 8, hipMemcpyDeviceToHost, stream()));
-		sync();
-		// update_x (adf.cpp:468-478): x_k[:, j, :] += (‖Δ_j‖²/‖P_j A Δ‖²)·Δ_j with Δ = −g
-		for (size_t j = 0; j < n; ++j) {
-			double pyr = 0;
-			for (size_t al = 0; al < rl; ++al)
-				for (size_t c = 0; c < rr; ++c) { const double t = (double)g[(al * n + j) * rr + c]; pyr += t * t; }
-			const double st = pyr / normA[j];
-			for (size_t al = 0; al < rl; ++al)
-				for (size_t c = 0; c < rr; ++c) upd[(al * n + j) * rr + c] = -st * (double)g[(al * n + j) * rr + c];
-		}
+	// update_x (adf.cpp:468-478) on the device: x_k[:, j, :] += (‖Δ_j‖²/‖P_j A Δ‖²)·Δ_j with Δ = −(the gradient)
+	void step(const AdfDesc& a, const Flat<T>& f, AdfWork<T>& w, size_t k, DTensor& core) const {
+		const int rl = (int)core.dims[0], n = (int)core.dims[1], rr = (int)core.dims[2];
+		adf_slice_norms<T>(a, f.cores.get(), m.dpos.get(), (int)k, w.grad.get() + f.hoff[k], n, w.slices.get(), w.dpart.get(), stream());
+		adf_update_core<T>(core.data(), w.grad.get() + f.hoff[k], w.slices.get(), rl, n, rr, true, stream());
 	}
 };
 
@@ -352,20 +347,13 @@ struct RankOneOps {
 			r1_gradient<T>(a, f.cores.get(), m.device_u<T>(), w.resid.get(), q, f.hrank[q], f.hn[q], f.hrank[q + 1], w.grad.get() + f.hoff[q],
 			               w.partials.get(), stream());
 	}
-	void step(const AdfDesc& a, const Flat<T>& f, AdfWork<T>& w, size_t k, const DTensor& core, const std::vector<T>& g,
-	          std::vector<double>& upd) const {
-		(void)core;
-		// Σ_i (x(u_i) with core k replaced by Δ)² (calculate_slicewise_norm_A_projGrad, adf.cpp:413-465, everything at slot 0)
-		DBuf<double> n2(2);
-		r1_residual<T>(a, f.cores.get(), m.device_u<T>(), nullptr, nullptr, n2.get(), w.dpart.get(), (int)k, w.grad.get() + f.hoff[k],
+	// update_x (adf.cpp:481-487) on the device: one step ‖Δ‖²/Σ_i (x(u_i) with core k replaced by Δ)² for the
+	// whole core (calculate_slicewise_norm_A_projGrad, adf.cpp:413-465, puts every measurement at slot 0)
+	void step(const AdfDesc& a, const Flat<T>& f, AdfWork<T>& w, size_t k, DTensor& core) const {
+		r1_residual<T>(a, f.cores.get(), m.device_u<T>(), nullptr, nullptr, w.slices.get(), w.dpart.get(), (int)k, w.grad.get() + f.hoff[k],
 		               stream());
-		double h[2];
-		XTT_HIP(hipMemcpyAsync(h, n2.get(), 16, hipMemcpyDeviceToHost, stream()));
-		sync();
-		double pyr = 0;
-		for (const T& t : g) pyr += (double)t * (double)t;
-		const double st = pyr / h[0];  // update_x (adf.cpp:481-487): ‖Δ‖² / Σ_j normAProjGrad[j]
-		for (size_t e = 0; e < g.size(); ++e) upd[e] = -st * (double)g[e];
+		adf_update_core<T>(core.data(), w.grad.get() + f.hoff[k], w.slices.get(), (int)core.dims[0], (int)core.dims[1], (int)core.dims[2],
+		                   false, stream());
 	}
 };
 
@@ -412,30 +400,30 @@ double solve(const ADFOptions& o, TT& x, const Ops& ops, const dims_t& maxRanksI
 	AdfWork<T> w;
 	upload_values<T>(ops.values(), w.vals);
 	Flat<T> f;
+	const size_t maxN = std::max<size_t>(2, *std::max_element(x.dims.begin(), x.dims.end()));  // slice norms per mode index
 	size_t iteration = 0;
 	double residualNorm = DBL_MAX, lastResidualNorm = DBL_MAX;
 	PerfClock clock;
 	clock.begin();
 
-	auto residual = [&]() {  // resid = x − b at the current x; returns Σ resid²
-		f.build(x);
-		w.ensure(P, f.desc.gradSize, 64);
+	// resid = x − b at the current x, after re-flattening cores first..last (the descriptor is rebuilt when a shape
+	// changed); nothing is read back
+	auto residual = [&](size_t first, size_t last) {
+		if (!f.matches(x)) { f.build(x); w.ensure(P, f.desc.gradSize, maxN); } else f.refresh(x, first, last);
 		AdfDesc a = f.desc;
 		a.P = (int)P;
 		ops.residual(a, f, w);
-		double norms[2];
-		XTT_HIP(hipMemcpyAsync(norms, w.norms.get(), 16, hipMemcpyDeviceToHost, stream()));
-		sync();
-		return norms[0];
 	};
 
 	auto solve_with_current_ranks = [&]() {
 		double resDec1 = 0.0, resDec2 = 0.0, resDec3 = 0.0;
 		for (; o.maxIterations == 0 || iteration < o.maxIterations; ++iteration) {
 			move_core(x, 0, true);
-			const double rr = residual();
+			residual(0, SIZE_MAX);
+			double norms[2];
+			read_back(norms, w.norms.get(), 16);  // the only synchronisation of an iteration
 			lastResidualNorm = residualNorm;
-			residualNorm = std::sqrt(rr) / normMeasured;
+			residualNorm = std::sqrt(norms[0]) / normMeasured;
 			if (PerfSink& ps = perf_sink(); ps.cb) {  // adf.cpp:514, perfData.add(iteration, residualNorm, x, 0)
 				const dims_t rk = x.ranks();
 				ps.cb(ps.user, iteration, clock.elapsed(), residualNorm, rk.data(), rk.size(), 0);
@@ -446,21 +434,12 @@ double solve(const ADFOptions& o, TT& x, const Ops& ops, const dims_t& maxRanksI
 			resDec1 = residualNorm / lastResidualNorm;
 			if (residualNorm < o.targetResidualNorm || resDec1 * resDec2 * resDec3 * resDec4 > std::pow(o.minimalResidualNormDecrease, 4)) break;
 			for (size_t k = 0; k < d; ++k) {
-				if (k > 0) residual();
+				if (k > 0) residual(k - 1, k);  // move_core(k) changed cores k−1 and k
 				AdfDesc a = f.desc;
 				a.P = (int)P;
-				const DTensor& core = x.cores[k];
-				const size_t sz = core.size();
-				// gradient of ½Σ(x−b)² w.r.t. core k; xerus's Δ (adf.cpp:359-396) is its negative
+				// gradient of ½Σ(x−b)² w.r.t. core k (xerus's Δ, adf.cpp:359-396, is its negative), then update_x
 				ops.gradient(a, f, w, (int)k);
-				std::vector<T> g(sz);
-				XTT_HIP(hipMemcpyAsync(g.data(), w.grad.get() + f.hoff[k], sz * sizeof(T), hipMemcpyDeviceToHost, stream()));
-				sync();
-				std::vector<double> upd(sz, 0.0);
-				ops.step(a, f, w, k, core, g, upd);
-				DTensor u = dt_upload(core.dims, upd.data());
-				axpy_f64(x.cores[k].data(), 1.0, u.data(), sz, stream());
-				sync();
+				ops.step(a, f, w, k, x.cores[k]);
 				if (k + 1 < d) move_core(x, k + 1, true);
 			}
 		}

@@ -50,7 +50,8 @@ struct Flat {
 	std::vector<long long> hoff;
 	AdfDesc desc{};
 	void build(const TT& x);
-	void refresh(const TT& x);
+	void refresh(const TT& x, size_t first = 0, size_t last = SIZE_MAX);  // cores first..last (inclusive) only
+	bool matches(const TT& x) const;  // the descriptor still describes x's shapes
 };
 
 struct ADFOptions { size_t maxIterations = 0; double targetResidualNorm = 1e-8; double minimalResidualNormDecrease = 0.999; };

@@ -322,7 +322,7 @@ struct ALSData {
 	bool spec = false;
 	unsigned* specFail = nullptr;
 	size_t cgIters = 0;  // inner CG iterations of this call
-	// Fused CG solves (one cooperative launch each) count their iterations and matvec flops on the device;
+	// Fused CG solves (one grid-synchronised launch each) count their iterations and matvec flops on the device;
 	// finish_counts() reads them once at the end of the call.
 	DBuf<double> itDev;
 	bool usedFused = false;
@@ -535,7 +535,7 @@ struct ALSData {
 	bool fused_ok(const SiteOp& s) const {
 		return !noFused && !t_batch_worker && !getenv("XTT_NO_FUSED_CG") && fused_cg_fits<T>(s.a, s.n, s.c, s.P, s.Q);
 	}
-	// false when the runtime refused the cooperative launch (nothing was enqueued)
+	// false when the launch could not be made (nothing was enqueued)
 	template <class T>
 	bool run_fused(const SiteOp& s, FusedCG d) {
 		if (!usedFused) {
@@ -961,7 +961,7 @@ double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats*
 		if (stats) *stats = savedStats;
 	}
 	size_t cgIters = 0;
-	// A fused (cooperative) local solve whose grid barrier ran out of time left its site unsolved: the whole call
+	// A fused (grid-synchronised) local solve whose grid barrier ran out of time left its site unsolved: the whole call
 	// runs again from x's start on the chunked CG, so the result is that of a call without fused solves.
 	const bool mayFuse = A && !t_batch_worker && !getenv("XTT_NO_FUSED_CG");
 	TT start;

@@ -188,6 +188,55 @@ int xt_dot(const double* x, const double* y, size_t n, double* result) {
 	});
 }
 
+int xt_one_norm(const double* x, size_t n, double* result) {
+	return guard([&] {
+		NN(result);
+		XTT_REQUIRE(n == 0 || x, "x must not be NULL");
+		if (n == 0) { *result = 0; return; }
+		xtt::DBuf<double> out(3), partials(2 * 1024);
+		xtt::DBuf<unsigned> counter(1);
+		XTT_HIP(hipMemsetAsync(counter.get(), 0, sizeof(unsigned), xtt::stream()));
+		xtt::norms_f64(x, n, out.get(), partials.get(), counter.get(), xtt::stream());
+		xtt::read_back(result, out.get(), 8);
+	});
+}
+int xt_two_norm(const double* x, size_t n, double* result) {
+	return guard([&] {
+		NN(result);
+		XTT_REQUIRE(n == 0 || x, "x must not be NULL");
+		if (n == 0) { *result = 0; return; }
+		xtt::DBuf<double> out(3), partials(2 * 1024);
+		xtt::DBuf<unsigned> counter(1);
+		XTT_HIP(hipMemsetAsync(counter.get(), 0, sizeof(unsigned), xtt::stream()));
+		xtt::norms_f64(x, n, out.get(), partials.get(), counter.get(), xtt::stream());
+		xtt::read_back(result, out.get() + 2, 8);
+	});
+}
+int xt_matrix_vector_product(double* x, size_t m, double alpha, const double* A, size_t n, int transposed, const double* y, void* stream) {
+	return guard([&] {
+		NN(x); NN(A); NN(y);
+		XTT_REQUIRE(m > 0 && n > 0, "Dimension must be larger than zero");
+		// x (m × 1) = alpha·op(A)·y: A m×n row-major, or n×m read transposed
+		xtt::GemmDesc g = xtt::gemm_desc(transposed ? 1 : 0, 0, m, 1, n);
+		g.alpha = alpha;
+		xtt::gemm_f64(g, A, y, x, xtt::pick(stream));
+		sync_stream(stream);
+	});
+}
+int xt_dyadic_vector_product(double* A, size_t m, size_t n, double alpha, const double* x, const double* y, void* stream) {
+	return guard([&] {
+		NN(A); NN(x); NN(y);
+		XTT_REQUIRE(m > 0 && n > 0, "Dimension must be larger than zero");
+		xtt::GemmDesc g = xtt::gemm_desc(0, 0, m, n, 1);  // (m × 1)·(1 × n)
+		g.alpha = alpha;
+		xtt::gemm_f64(g, x, y, A, xtt::pick(stream));
+		sync_stream(stream);
+	});
+}
+int xt_solve_least_squares(double* X, const double* A, size_t m, size_t n, const double* B, size_t p) {
+	return guard([&] { NN(X); NN(A); NN(B); xtt::least_squares_solve(A, m, n, B, p, X, xtt::stream()); xtt::sync(); });
+}
+
 // ---------------------------------------------------------------- boundary 2
 
 int xt_tt_new(int op, size_t nd, const size_t* dims, xt_tt* out) {

@@ -31,8 +31,9 @@
 // workgroup 0 polls the slots and publishes a broadcast slot, which the others poll. Tags carry a launch epoch,
 // so the slots are never cleared. Waits are bounded by the wall clock: a wait that exceeds the limit raises the
 // state's fail flag, every workgroup leaves at its next wait, and the host reports an error.
-// The launch is cooperative: the runtime starts it only with every workgroup resident at once (or refuses it, and
-// the caller solves on the chunked CG), and the kernel's workgroups wait only for each other.
+// The grid never exceeds the workgroups the device holds at once (occupancy query) and the workgroups wait only for
+// each other; if the device is shared and the grid is not resident after all, a wait runs out of time, every
+// workgroup leaves, and the ALS call reruns on the chunked CG.
 #include <algorithm>
 #include <cstdlib>
 
@@ -486,7 +487,7 @@ template <class T>
 bool fused_cg(const FusedCG& d, hipStream_t s) {
 	FcgPlan f;
 	XTT_REQUIRE(plan<T>((int)d.a, (int)d.n, (int)d.c, (int)d.P, (int)d.Q, f) && f.G <= max_resident<T>(),
-	            "fused CG: local shape does not fit one cooperative launch");
+	            "fused CG: local shape does not fit one grid-synchronised launch");
 	FcgArgs<T> g{};
 	g.L = d.L; g.Am = d.Am; g.R = d.R; g.r0 = d.r0; g.b = d.b; g.x = d.x;
 	g.vec = static_cast<T*>(d.vec); g.T2g = static_cast<T*>(d.T2); g.sync = d.sync; g.epoch = d.epoch; g.st = d.st;
@@ -497,14 +498,13 @@ bool fused_cg(const FusedCG& d, hipStream_t s) {
 	g.ldL = f.ldL; g.ldAm = f.ldAm; g.ldR = f.ldR; g.ldT1 = f.ldT1; g.ldT2 = f.ldT2;
 	g.oL = f.oL; g.oAm = f.oAm; g.oR = f.oR; g.oV = f.oV; g.oT1 = f.oT1; g.oT2 = f.oT2; g.oS = f.oS; g.oRc = f.oRc;
 	XTT_REQUIRE(d.maxIter <= 30000 && d.epoch > 0 && d.epoch < (1ull << 47), "fused CG: iteration limit or epoch out of range");
-	void* args[] = {&g};
-	const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(fused_cg_kernel<T>), dim3(f.G), dim3(FT), args,
-	                                                (unsigned)f.lds, s);
-	if (e == hipErrorCooperativeLaunchTooLarge || e == hipErrorLaunchOutOfResources) {
-		(void)hipGetLastError();
-		return false;
-	}
-	XTT_HIP(e);
+	// A plain launch: hipLaunchCooperativeKernel would refuse a grid the device cannot hold at once, but in this
+	// runtime a single cooperative launch slows every later multi-stream workload of the process (the 16-stream
+	// batch of 512 problems ran at 111 instead of 187 sweeps/s after one). Co-residency is instead sized from the
+	// occupancy query (f.G <= max_resident), and a grid that still is not resident (the device shared with other
+	// work) ends at the bounded wait, after which the caller reruns the call on the chunked CG.
+	hipLaunchKernelGGL(fused_cg_kernel<T>, dim3(f.G), dim3(FT), f.lds, s, g);
+	XTT_HIP(hipGetLastError());
 	return true;
 }
 

@@ -57,6 +57,8 @@ void dot_f64(const double* x, const double* y, size_t n, double* out, double* pa
 // last block also adds one to *fail unless ‖r‖² <= tol2·‖b‖² (a convergence test the host reads later).
 void residual_norms_f64(double* r, const double* b, const double* q, size_t n, double* out2, double* partials, unsigned* counter,
                         hipStream_t s, double tol2 = 0.0, unsigned* fail = nullptr);
+// out3 = {Σ|x_i|, max|x_i|, ‖x‖₂} (the two norm scaled by max|x_i| as dnrm2 does); same partials/counter contract
+void norms_f64(const double* x, size_t n, double* out3, double* partials, unsigned* counter, hipStream_t s);
 void dot_f32_acc64(const float* x, const float* y, size_t n, double* out, double* partials, unsigned* counter, hipStream_t s);
 
 // ---- local CG on the device; all scalars live in CGState (device) -------------------------------------
@@ -109,10 +111,10 @@ struct FusedCG {
 };
 constexpr size_t FUSED_CG_SYNC = 1025 * 8;
 template <class T> bool fused_cg_fits(size_t a, size_t n, size_t c, size_t P, size_t Q);
-// Cooperative launch (every workgroup co-resident or no launch): returns false, with nothing enqueued, when the
-// runtime refuses it (the device cannot hold the grid at once, e.g. while other work occupies it); the caller then
-// solves on the chunked CG. A barrier wait that runs out of time leaves x untouched and marks the solve failed
-// (iterAcc[0] = NaN). XTT_FUSED_CG_WAIT_TICKS (wall-clock ticks, read per launch) overrides the 2 s wait limit;
+// The grid is sized to the workgroups the device holds at once (occupancy query). Returns false, with nothing
+// enqueued, when the launch cannot be made (the caller then solves on the chunked CG). A barrier wait that runs out
+// of time (the grid was not resident because other work shared the device) leaves x untouched and marks the solve
+// failed (iterAcc[0] = NaN). XTT_FUSED_CG_WAIT_TICKS (wall-clock ticks, read per launch) overrides the 2 s wait limit;
 // 0 makes every wait give up at once, which exercises the failure path with every workgroup exiting normally.
 template <class T> bool fused_cg(const FusedCG& d, hipStream_t s);
 
@@ -198,5 +200,9 @@ int adf_slice_blocks(int P);
 template <class T>
 void adf_slice_norms(const AdfDesc& a, const T* cores, const uint32_t* pos, int k, const T* D, int nslices, double* out,
                      double* partials, hipStream_t s);
+// update_x (adf.cpp:468-487): core (FP64, rl × n × rr) −= step·g, where g is the gradient of ½Σ(x − b)² for that core
+// (xerus's Δ = −g) and step = ‖g_j‖²/normA[j] per slice j (single points) or ‖g‖²/normA[0] (rank-one, perSlice false).
+template <class T>
+void adf_update_core(double* core, const T* g, const double* normA, int rl, int n, int rr, bool perSlice, hipStream_t s);
 
 }  // namespace xtt

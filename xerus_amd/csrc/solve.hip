@@ -186,11 +186,12 @@ int square_solve(bool chol, const double* A, const double* B, size_t n, size_t n
 
 }  // namespace
 
-int dense_solve(const double* A, size_t m, size_t n, const double* B, size_t nrhs, double* X, hipStream_t s) {
+// blasWrapper::solve_least_squares (blasLapackWrapper.cpp:654-705, dgelsd): X = V·diag(1/s_j for s_j > rcond·s_0)·Uᵀ·B
+// with rcond = xerus::EPSILON (include/xerus/basic.h:50), the minimum-norm least squares solution. Returns 3.
+int least_squares_solve(const double* A, size_t m, size_t n, const double* B, size_t nrhs, double* X, hipStream_t s) {
 	XTT_REQUIRE(m > 0 && n > 0 && nrhs > 0, "solve: empty system");
 	XTT_REQUIRE(m <= (size_t)INT32_MAX && n <= (size_t)INT32_MAX && nrhs <= (size_t)INT32_MAX, "Dimension to large for BLAS/Lapack");
-	if (m != n) {
-		// dgelsd: x = V·diag(1/s_j for s_j > rcond·s_0)·Uᵀ·b, rcond = xerus::EPSILON (include/xerus/basic.h:50)
+	{
 		const double rcond = 8 * DBL_EPSILON;
 		DTensor Ad({m, n});
 		XTT_HIP(hipMemcpyAsync(Ad.data(), A, m * n * 8, hipMemcpyDeviceToDevice, s));
@@ -214,6 +215,12 @@ int dense_solve(const double* A, size_t m, size_t n, const double* B, size_t nrh
 		gemm_f64(gemm_desc(1, 0, n, nrhs, r), Vt.data(), T.get(), X, s);
 		return 3;
 	}
+}
+
+int dense_solve(const double* A, size_t m, size_t n, const double* B, size_t nrhs, double* X, hipStream_t s) {
+	XTT_REQUIRE(m > 0 && n > 0 && nrhs > 0, "solve: empty system");
+	XTT_REQUIRE(m <= (size_t)INT32_MAX && n <= (size_t)INT32_MAX && nrhs <= (size_t)INT32_MAX, "Dimension to large for BLAS/Lapack");
+	if (m != n) return least_squares_solve(A, m, n, B, nrhs, X, s);
 	DBuf<int> cls(2);
 	hipLaunchKernelGGL(solve_classify_kernel, dim3(1), dim3(ST), 0, s, A, (long long)n, cls.get());
 	XTT_LAUNCH_CHECK();

@@ -28,6 +28,8 @@ void calculate_qc(DTensor& Q, DTensor& C, const DTensor& in, size_t split);
 // blasWrapper::solve with xerus's dispatch (row-major A m×n, B m×nrhs, X n×nrhs, device); returns the method:
 // 0 LU, 1 Cholesky, 2 symmetric indefinite (solved by LU), 3 minimum-norm least squares
 int dense_solve(const double* A, size_t m, size_t n, const double* B, size_t nrhs, double* X, hipStream_t s);
+// blasWrapper::solve_least_squares: minimum-norm least squares (dgelsd's cut-off) for any m, n
+int least_squares_solve(const double* A, size_t m, size_t n, const double* B, size_t nrhs, double* X, hipStream_t s);
 void calculate_cq(DTensor& C, DTensor& Q, const DTensor& in, size_t split);
 // Thin SVD with xerus's truncation (tensor.cpp:1463-1474); S returned as host singular values (kept ones).
 // sDev, when given, receives the kept singular values on the device as well. jacobiOnTranspose: for a

@@ -142,6 +142,51 @@ __global__ __launch_bounds__(RB) void dot_kernel(const T* x, const T* y, size_t 
 	if (last_block_sum2(a, b, partials, counter, ws) && threadIdx.x == 0) *out = a;
 }
 
+// out[0] = Σ|x_i|, out[1] = max|x_i| (block partials reduced in block order by the last block)
+__global__ __launch_bounds__(RB) void asum_amax_kernel(const double* x, size_t n, double* out, double* partials, unsigned* counter) {
+	__shared__ double ws[RB / 64 * 2];
+	__shared__ int amLast;
+	double a = 0, m = 0;
+	for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+		const double v = fabs(x[i]);
+		a += v;
+		m = fmax(m, v);
+	}
+	for (int o = 32; o > 0; o >>= 1) { a += __shfl_xor(a, o); m = fmax(m, __shfl_xor(m, o)); }
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	if (lane == 0) { ws[2 * wave] = a; ws[2 * wave + 1] = m; }
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		a = 0; m = 0;
+		for (int w = 0; w < RB / 64; ++w) { a += ws[2 * w]; m = fmax(m, ws[2 * w + 1]); }
+		partials[2 * blockIdx.x] = a;
+		partials[2 * blockIdx.x + 1] = m;
+		__threadfence();
+		const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+		amLast = (t == gridDim.x - 1);
+	}
+	__syncthreads();
+	if (!amLast || threadIdx.x != 0) return;
+	__threadfence();
+	a = 0; m = 0;
+	for (unsigned b = 0; b < gridDim.x; ++b) {
+		a += __hip_atomic_load(partials + 2 * b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		m = fmax(m, __hip_atomic_load(partials + 2 * b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+	}
+	out[0] = a;
+	out[1] = m;
+	*counter = 0;
+}
+// out[2] = out[1]·√(Σ (x_i/out[1])²): the two norm without overflow or underflow of the squares (dnrm2's scaling)
+__global__ __launch_bounds__(RB) void scaled_nrm2_kernel(const double* x, size_t n, double* out, double* partials, unsigned* counter) {
+	__shared__ double ws[RB / 64 * 2];
+	const double scale = out[1];
+	double a = 0, b = 0;
+	if (scale > 0)
+		for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) { const double v = x[i] / scale; a = fma(v, v, a); }
+	if (last_block_sum2(a, b, partials, counter, ws) && threadIdx.x == 0) out[2] = scale > 0 ? scale * sqrt(a) : 0.0;
+}
+
 // r = b − q with out[0] = ‖r‖², out[1] = ‖b‖² (one pass, deterministic)
 __global__ __launch_bounds__(RB) void residual_norms_kernel(double* r, const double* b, const double* q, size_t n, double* out,
                                                             double* partials, unsigned* counter, double tol2, unsigned* fail) {
@@ -324,6 +369,13 @@ void axpy_f64(double* y, double a, const double* x, size_t n, hipStream_t s) {
 }
 void dot_f64(const double* x, const double* y, size_t n, double* out, double* partials, unsigned* counter, hipStream_t s) {
 	hipLaunchKernelGGL(dot_kernel<double>, dim3((unsigned)reduce_blocks(n)), dim3(RB), 0, s, x, y, n, out, partials, counter);
+	XTT_LAUNCH_CHECK();
+}
+void norms_f64(const double* x, size_t n, double* out3, double* partials, unsigned* counter, hipStream_t s) {
+	const unsigned g = (unsigned)reduce_blocks(n);
+	hipLaunchKernelGGL(asum_amax_kernel, dim3(g), dim3(RB), 0, s, x, n, out3, partials, counter);
+	XTT_LAUNCH_CHECK();
+	hipLaunchKernelGGL(scaled_nrm2_kernel, dim3(g), dim3(RB), 0, s, x, n, out3, partials, counter);
 	XTT_LAUNCH_CHECK();
 }
 void residual_norms_f64(double* r, const double* b, const double* q, size_t n, double* out2, double* partials, unsigned* counter,
