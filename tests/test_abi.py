@@ -71,21 +71,23 @@ def test_gpus_flag_refuses_multi_gpu():
     assert r.returncode == 2 and "one GPU" in r.stderr
 
 
-def test_refuses_foreign_hip_runtime():
-    """With torch (and its bundled HIP runtime) imported first, loading libxtt must fail loudly instead of binding
-    libxtt to that runtime; loading libxtt first keeps it on the ROCm runtime it was built for."""
+def test_hip_runtime_selection():
+    """libxtt binds to the HIP runtime mapped when it loads: torch's bundled copy when torch came first (then torch's
+    device tensors are libxtt pointers), the ROCm installation's when libxtt came first."""
     import subprocess
     import sys
     torch_first = ("import sys; sys.path.insert(0, %r); import torch\n"
-                   "try:\n    import xerus_amd\n    xerus_amd.lib()\nexcept RuntimeError as e:\n"
-                   "    print('refused', 'HIP runtime' in str(e))\n") % ROOT
+                   "import xerus_amd; from xerus_amd import _lib; xerus_amd.lib()\n"
+                   "maps = open('/proc/self/maps').read()\n"
+                   "print('torch-runtime', _lib.on_torch_runtime(), 'torch/lib/libamdhip64' in maps)\n") % ROOT
     out = subprocess.run([sys.executable, "-c", torch_first], capture_output=True, text=True, timeout=300)
-    assert "refused True" in out.stdout, out.stdout + out.stderr
+    assert "torch-runtime True True" in out.stdout, out.stdout + out.stderr
     xtt_first = ("import sys; sys.path.insert(0, %r); import xerus_amd; xerus_amd.lib(); import torch\n"
+                 "from xerus_amd import _lib\n"
                  "maps = open('/proc/self/maps').read()\n"
-                 "print('ok', '/opt/rocm' in maps and 'libxtt' in maps)\n") % ROOT
+                 "print('ok', '/opt/rocm' in maps and 'libxtt' in maps, _lib.on_torch_runtime())\n") % ROOT
     out = subprocess.run([sys.executable, "-c", xtt_first], capture_output=True, text=True, timeout=300)
-    assert "ok True" in out.stdout, out.stdout + out.stderr
+    assert "ok True False" in out.stdout, out.stdout + out.stderr
 
 
 def test_torch_binding_rejects_inconsistent_ranks():

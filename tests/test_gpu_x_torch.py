@@ -1,9 +1,14 @@
-"""GPU check of the torch.autograd.Function around the sampled TT contraction (xerus_amd.autograd).
+"""GPU checks of the torch.autograd.Function around the sampled TT contraction (xerus_amd.autograd).
 
-Kept in its own module, collected after the torch-free GPU tests: it is the only GPU test that loads torch
-into the same process as libxtt. libxtt is loaded first (the fixture), torch's tensors stay on the CPU and its
-own HIP runtime is never initialised; all device work runs inside libxtt.
+Kept in its own module, collected after the torch-free GPU tests. The first test runs in this process, where libxtt
+was loaded before torch (the fixture): torch's tensors stay on the CPU and are staged through libxtt buffers. The
+second runs tests/torch_device_check.py in a process of its own with torch imported first, so that libxtt binds to
+torch's HIP runtime and the Function takes device tensors in place.
 """
+import os
+import subprocess
+import sys
+
 import numpy as np
 import pytest
 
@@ -11,6 +16,7 @@ from helpers import SEED
 from test_gpu_tt import twin_random
 
 pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def test_autograd_adjoint(xe, oracle):
@@ -38,3 +44,20 @@ def test_autograd_adjoint(xe, oracle):
     for k in range(d):
         g = cores[k].grad.numpy()
         assert np.abs(g + go[k]).max() < 3e-5 * np.abs(go[k]).max()
+
+
+def test_autograd_device_tensors():
+    """Device tensors in place on torch's runtime (a process of its own, torch imported first): values and gradients
+    on the device against the oracle, central differences at d=3 through the torch path, CPU tensors moved there and
+    back."""
+    print("autograd on device tensors", flush=True)
+    out = subprocess.run([sys.executable, os.path.join(HERE, "torch_device_check.py")], capture_output=True, text=True,
+                         timeout=300)
+    print(out.stdout, out.stderr[-3000:], flush=True)
+    assert out.returncode == 0, out.stderr[-3000:]
+    vals = {line.split()[0]: float(line.split()[1]) for line in out.stdout.splitlines() if len(line.split()) == 2}
+    assert vals["torch_runtime"] == 1 and vals["values_on_device"] == 1 and vals["grads_on_device"] == 1
+    assert vals["values_rel_err"] < 1e-5
+    assert vals["grad_rel_err"] < 3e-5
+    assert vals["fd_rel_err"] < 1e-3
+    assert vals["cpu_roundtrip"] == 1

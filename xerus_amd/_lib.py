@@ -169,8 +169,7 @@ class XttError(RuntimeError):
 
 def _foreign_hip_runtime():
     """Path of a HIP runtime mapped into this process from outside the ROCm installation (torch bundles one),
-    or None. libxtt resolves libamdhip64.so.7 to an already mapped runtime of that name, so it must be loaded
-    before such a runtime appears."""
+    or None. libxtt resolves libamdhip64.so.7 to an already mapped runtime of that name."""
     try:
         with open("/proc/self/maps") as f:
             for line in f:
@@ -182,18 +181,22 @@ def _foreign_hip_runtime():
     return None
 
 
+_on_torch_runtime = False
+
+
 def lib() -> C.CDLL:
-    """Load libxtt.so once; raises if it has not been built, or if another HIP runtime (torch's bundled copy)
-    is already mapped, which libxtt would otherwise bind to."""
-    global _lib
+    """Load libxtt.so once; raises if it has not been built.
+
+    libxtt binds to the HIP runtime already mapped into the process when it is loaded. Loaded first (the usual
+    case for xerus_amd), that is the ROCm runtime it was built with; loaded after torch, it is torch's bundled
+    runtime, and then libxtt's kernels work on torch's device tensors and streams (xerus_amd.autograd relies on
+    it). A process keeps one runtime: torch imported after libxtt runs on the ROCm runtime without a usable
+    device, so tensors there must stay on the CPU."""
+    global _lib, _on_torch_runtime
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"libxtt.so not built: {LIB_PATH} missing (run `make -C xerus_amd/csrc`)")
-        foreign = _foreign_hip_runtime()
-        if foreign is not None:
-            raise RuntimeError(f"a HIP runtime from {foreign} is already loaded (torch imported first?); import xerus_amd "
-                               "and call xerus_amd.lib() before importing torch, so libxtt runs on the ROCm runtime it was "
-                               "built for")
+        _on_torch_runtime = _foreign_hip_runtime() is not None
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
@@ -201,6 +204,11 @@ def lib() -> C.CDLL:
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def on_torch_runtime() -> bool:
+    """Whether libxtt was loaded on torch's HIP runtime (device tensors of torch are then valid libxtt pointers)."""
+    return _lib is not None and _on_torch_runtime
 
 
 def check(rc: int) -> None:
