@@ -213,6 +213,14 @@ int xt_meas_positions_ptr(xt_meas m, const uint32_t** device_ptr);
 int xt_meas_values(xt_meas m, double* host_out);
 int xt_meas_set_values(xt_meas m, const double* host_values);
 int xt_meas_measure(xt_meas m, xt_tt x);                                       /* measurments.cpp:125-153 */
+/* The stack plan ADF builds for these positions (ADFVariant::InternalSolver::construct_stacks, adf.cpp:102-191): per
+ * level k, the forward entries (samples sharing positions 0..k) and the backward ones (sharing k..d-1), each entry
+ * listed by its representative, the group's smallest sample id, in ascending order (xerus's forwardUpdates[k] /
+ * backwardUpdates[k]). fwd_counts / bwd_counts: d entries each; fwd_ids / bwd_ids (optional): the concatenated
+ * representative lists; fwd_slots / bwd_slots (optional, P×d, [k·P + i]): sample i's entry within level k;
+ * num_unique (optional): xerus's numUniqueStackEntries (forward entries + 1). Built on the device on first use. */
+int xt_meas_stack_plan(xt_meas m, size_t* fwd_counts, size_t* bwd_counts, uint32_t* fwd_ids, uint32_t* bwd_ids, uint32_t* fwd_slots,
+                       uint32_t* bwd_slots, size_t* num_unique);
 /* xerus's projected gradient Δ_k = Σ (b_i − x_i)·∂x_i/∂G_k for all k (adf.cpp:314-396), concatenated cores */
 int xt_adf_gradient(xt_tt x, xt_meas m, int precision, double* host_grad, double* residual_norm);
 /* ADFVariant::operator() (adf.h:248-266); max_ranks NULL = x's ranks */

@@ -257,3 +257,24 @@ def test_adf_random_low_rank_reference_test(oracle):
     diff.scale(-1.0)
     err = (truth + diff).frob_norm() / truth.frob_norm()
     assert err < 1e-3, (err, res, it)
+
+
+def test_adf_stack_plan_unsorted_with_duplicates(oracle):
+    """The stack plan (adf.cpp:102-191) of an unsorted position set with repeated samples and long shared prefixes and
+    suffixes: per level, each group of equal prefix (suffix) is one entry represented by its smallest sample id,
+    listed in ascending order, as brute-force grouping gives."""
+    rng = np.random.default_rng(SEED)
+    pos = rng.integers(0, 2, size=(300, 7)).astype(np.uint32)
+    pos[:, 2:5] = 0
+    pos = pos[rng.permutation(len(pos))]
+    m = oracle.Measurements.from_arrays(pos, np.zeros(len(pos)))
+    fc, bc, fi, bi, nu = oracle.adf_plan(m, 7)
+    for k in range(7):
+        fwd, bwd = {}, {}
+        for i in range(len(pos)):
+            fwd.setdefault(tuple(pos[i, : k + 1]), i)
+            bwd.setdefault(tuple(pos[i, k:]), i)
+        assert fc[k] == len(fwd) and bc[k] == len(bwd)
+        assert list(fi[sum(fc[:k]): sum(fc[: k + 1])]) == sorted(fwd.values())
+        assert list(bi[sum(bc[:k]): sum(bc[: k + 1])]) == sorted(bwd.values())
+    assert nu == sum(fc) + 1

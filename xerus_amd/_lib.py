@@ -148,6 +148,7 @@ SIGNATURES = {
     "xt_meas_values": (C.c_int, [_vp, _pd]),
     "xt_meas_set_values": (C.c_int, [_vp, _pd]),
     "xt_meas_measure": (C.c_int, [_vp, _vp]),
+    "xt_meas_stack_plan": (C.c_int, [_vp, _ps, _ps, _pu32, _pu32, _pu32, _pu32, _ps]),
     "xt_adf_gradient": (C.c_int, [_vp, _vp, C.c_int, _pd, _pd]),
     "xt_adf_solve": (C.c_int, [_vp, _vp, _ps, _sz, C.c_double, C.c_double, C.c_int, _pd, _ps]),
     "xt_perf_sink": (C.c_int, [PERF_CALLBACK, _vp]),

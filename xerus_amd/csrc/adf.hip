@@ -790,6 +790,18 @@ void adf_slice_norms(const AdfDesc& a, const T* cores, const uint32_t* pos, int 
 	XTT_LAUNCH_CHECK();
 }
 
+template <class T>
+void adf_sum_partials(const T* partials, int nblocks, int size, T* out, hipStream_t s) {
+	hipLaunchKernelGGL(adf_grad_reduce_kernel<T>, dim3((size + RE - 1) / RE), dim3(256), 0, s, partials, nblocks, size, out);
+	XTT_LAUNCH_CHECK();
+}
+void adf_sum_slices(const double* partials, int nblocks, int nslices, double* out, hipStream_t s) {
+	hipLaunchKernelGGL(adf_slice_reduce_kernel, dim3(1), dim3(64), 0, s, partials, nblocks, nslices, out);
+	XTT_LAUNCH_CHECK();
+}
+template void adf_sum_partials<float>(const float*, int, int, float*, hipStream_t);
+template void adf_sum_partials<double>(const double*, int, int, double*, hipStream_t);
+
 namespace {
 // update_x of adf.cpp:468-487 on the device: with the gradient g of ½Σ(x − b)² (xerus's Δ = −g), slice j of the
 // core moves by (‖g_j‖²/normA[j])·Δ_j (single points), or the whole core by (‖g‖²/normA[0])·Δ (rank-one).

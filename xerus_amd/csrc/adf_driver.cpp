@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <set>
 
 namespace xtt {
@@ -43,9 +44,23 @@ void Measurements::check_against(const TT& x) const {
 }
 
 void Measurements::upload() {
+	plan.reset();
 	dpos.resize(pos.size());
 	if (!pos.empty()) XTT_HIP(hipMemcpyAsync(dpos.get(), pos.data(), pos.size() * 4, hipMemcpyHostToDevice, stream()));
 	sync();
+}
+
+const AdfPlan& Measurements::stack_plan() const {  // adf.cpp:102-191, once per position set
+	if (!plan) {
+		XTT_REQUIRE(size() > 0, "empty measurement set");
+		XTT_REQUIRE(size() <= (size_t)INT32_MAX, "ADF stacks: more than 2^31 - 1 measurements");
+		std::vector<uint32_t> maxPos(d, 0);
+		for (size_t i = 0; i < pos.size(); ++i) maxPos[i % d] = std::max(maxPos[i % d], pos[i]);
+		auto p = std::make_shared<AdfPlan>();
+		adf_plan_build(*p, dpos.get(), (int)size(), (int)d, maxPos, stream());
+		plan = std::move(p);
+	}
+	return *plan;
 }
 
 template <class T>
@@ -117,10 +132,11 @@ struct AdfWork {
 	void ensure(size_t P, size_t gradSize, size_t nslice) {
 		if (resid.size() < P) { resid.resize(P); vals.resize(P); }
 		if (grad.size() < gradSize) grad.resize(gradSize);
-		const size_t pb = (size_t)adf_grad_blocks((int)P) * gradSize;
+		const size_t pb = (size_t)std::max(adf_grad_blocks((int)P), adf_stack_blocks((int)P)) * gradSize;
 		if (partials.size() < pb) partials.resize(pb);
 		if (!norms.get()) { norms.resize(2); counter.resize(1); XTT_HIP(hipMemsetAsync(counter.get(), 0, 4, stream())); }
-		const size_t sp = std::max<size_t>((size_t)adf_slice_blocks((int)P) * std::max<size_t>(nslice, 1), 2 * 1024);
+		const size_t sp = std::max<size_t>((size_t)std::max(adf_slice_blocks((int)P), adf_stack_blocks((int)P)) * std::max<size_t>(nslice, 1),
+		                                   2 * 1024);
 		if (dpart.size() < sp) dpart.resize(sp);
 		if (slices.size() < nslice) slices.resize(std::max<size_t>(nslice, 1));
 	}
@@ -331,6 +347,74 @@ struct SinglePointOps {
 	}
 };
 
+// Single points with xerus's stacks (adf.cpp:102-312): backward stacks rebuilt at the start of an iteration,
+// forward level k−1 refreshed after each core move; the residual, gradient and slice norms at core k read F_{k-1}
+// and B_{k+1} through the plan's slots.
+template <class T>
+struct StackedSinglePointOps {
+	const Measurements& m;
+	const AdfPlan& plan;
+	DBuf<T> F, B;                       // stack values, level after level
+	std::vector<long long> fOff, bOff;  // first value of each level
+	std::vector<int> rk;                // ranks the layout was made for
+	void check(const TT& x) const { m.check_against(x); }
+	size_t size() const { return m.size(); }
+	const std::vector<double>& values() const { return m.values; }
+	// forward level k holds count × r_{k+1} values, backward level k count × r_k
+	void layout(const Flat<T>& f) {
+		if (rk == f.hrank) return;
+		rk = f.hrank;
+		const size_t d = f.hn.size();
+		fOff.assign(d, 0);
+		bOff.assign(d, 0);
+		long long fo = 0, bo = 0;
+		for (size_t k = 0; k < d; ++k) {
+			fOff[k] = fo;
+			bOff[k] = bo;
+			fo += plan.side[0].count[k] * f.hrank[k + 1];
+			bo += plan.side[1].count[k] * f.hrank[k];
+		}
+		if (F.size() < (size_t)fo) F.resize((size_t)fo);
+		if (B.size() < (size_t)bo) B.resize((size_t)bo);
+	}
+	const uint32_t* col(size_t k) const { return plan.posT.get() + k * m.size(); }  // positions of mode k
+	AdfSides<T> sides(size_t k) const {
+		const size_t d = fOff.size(), P = m.size();
+		AdfSides<T> sd{};
+		if (k > 0) { sd.fSlot = plan.side[0].slot.get() + (k - 1) * P; sd.F = F.get() + fOff[k - 1]; }
+		if (k + 1 < d) { sd.bSlot = plan.side[1].slot.get() + (k + 1) * P; sd.B = B.get() + bOff[k + 1]; }
+		return sd;
+	}
+	void forward(const Flat<T>& f, size_t k) {  // update_forward_stack (adf.cpp:255-270)
+		const int P = (int)m.size(), d = (int)fOff.size();
+		adf_forward_stack<T>(f.cores.get() + f.hoff[k], f.hrank[k], f.hn[k], f.hrank[k + 1], (int)k, d, col(k), P,
+		                     plan.side[0].rep.get() + plan.side[0].first[k], plan.side[0].count[k],
+		                     k > 0 ? plan.side[0].slot.get() + (k - 1) * (size_t)P : nullptr, k > 0 ? F.get() + fOff[k - 1] : nullptr,
+		                     F.get() + fOff[k], stream());
+	}
+	void backward(const Flat<T>& f, size_t k) {  // update_backward_stack (adf.cpp:217-232)
+		const int P = (int)m.size(), d = (int)fOff.size();
+		const bool last = (int)k + 1 == d;
+		adf_backward_stack<T>(f.cores.get() + f.hoff[k], f.hrank[k], f.hn[k], f.hrank[k + 1], (int)k, d, col(k), P,
+		                      plan.side[1].rep.get() + plan.side[1].first[k], plan.side[1].count[k],
+		                      last ? nullptr : plan.side[1].slot.get() + (k + 1) * (size_t)P, last ? nullptr : B.get() + bOff[k + 1],
+		                      B.get() + bOff[k], stream());
+	}
+	// calculate_residual + calculate_projected_gradient (adf.cpp:290-396) in one pass: resid, norms, and the gradient of
+	// core k in w.grad
+	// (norms only when wanted: the solve reads them at core 0)
+	void residual_gradient(const Flat<T>& f, AdfWork<T>& w, size_t k, bool norms) const {
+		adf_stack_residual_gradient<T>(f.cores.get() + f.hoff[k], f.hrank[k], f.hn[k], f.hrank[k + 1], (int)k, (int)fOff.size(), col(k),
+		                               (int)m.size(), sides(k), w.vals.get(), w.resid.get(), norms ? w.norms.get() : nullptr, w.grad.get(),
+		                               w.partials.get(), w.dpart.get(), stream());
+	}
+	// update_x (adf.cpp:468-478) with the slice-wise norms of adf.cpp:413-465
+	void step(AdfWork<T>& w, size_t k, DTensor& core) const {
+		adf_stack_step<T>(core.data(), w.grad.get(), (int)core.dims[0], (int)core.dims[1], (int)core.dims[2], (int)k, (int)fOff.size(), col(k),
+		                  (int)m.size(), sides(k), w.dpart.get(), stream());
+	}
+};
+
 template <class T>
 struct RankOneOps {
 	const RankOneMeasurements& m;
@@ -459,6 +543,79 @@ double solve(const ADFOptions& o, TT& x, const Ops& ops, const dims_t& maxRanksI
 	if (iterations) *iterations = iteration;
 	return residualNorm;
 }
+
+// adf.cpp:489-604 with xerus's stacks on the device (single points).
+template <class T>
+double solve_stacked(const ADFOptions& o, TT& x, const Measurements& m, const dims_t& maxRanksIn, size_t* iterations) {
+	const size_t d = x.order(), P = m.size();
+	XTT_REQUIRE(P > 0, "Need at very least one measurment.");
+	m.check_against(x);
+	const dims_t maxRanks = reduce_to_maximal_ranks(maxRanksIn, x.dims, false);
+	double normMeasured = 0;
+	for (double v : m.values) normMeasured += v * v;
+	normMeasured = std::sqrt(normMeasured);
+	StackedSinglePointOps<T> ops{m, m.stack_plan()};
+	AdfWork<T> w;
+	upload_values<T>(m.values, w.vals);
+	Flat<T> f;
+	const size_t maxN = std::max<size_t>(2, *std::max_element(x.dims.begin(), x.dims.end()));
+	size_t iteration = 0;
+	double residualNorm = DBL_MAX, lastResidualNorm = DBL_MAX;
+	PerfClock clock;
+	clock.begin();
+
+	auto solve_with_current_ranks = [&]() {
+		double resDec1 = 0.0, resDec2 = 0.0, resDec3 = 0.0;
+		for (; o.maxIterations == 0 || iteration < o.maxIterations; ++iteration) {
+			move_core(x, 0, true);
+			if (!f.matches(x)) {
+				f.build(x);
+				w.ensure(P, f.desc.gradSize, maxN);
+			} else {
+				f.refresh(x);
+			}
+			ops.layout(f);
+			for (size_t k = d - 1; k > 0; --k) ops.backward(f, k);  // rebuild the backward stacks
+			ops.residual_gradient(f, w, 0, true);
+			double norms[2];
+			read_back(norms, w.norms.get(), 16);  // the only synchronisation of an iteration
+			lastResidualNorm = residualNorm;
+			residualNorm = std::sqrt(norms[0]) / normMeasured;
+			if (PerfSink& ps = perf_sink(); ps.cb) {  // adf.cpp:514, perfData.add(iteration, residualNorm, x, 0)
+				const dims_t rk = x.ranks();
+				ps.cb(ps.user, iteration, clock.elapsed(), residualNorm, rk.data(), rk.size(), 0);
+			}
+			const double resDec4 = resDec3;
+			resDec3 = resDec2;
+			resDec2 = resDec1;
+			resDec1 = residualNorm / lastResidualNorm;
+			if (residualNorm < o.targetResidualNorm || resDec1 * resDec2 * resDec3 * resDec4 > std::pow(o.minimalResidualNormDecrease, 4)) break;
+			for (size_t k = 0; k < d; ++k) {
+				if (k > 0) {  // move_core(k) changed cores k−1 and k: forward level k−1, then residual and gradient at k
+					f.refresh(x, k - 1, k);
+					ops.forward(f, k - 1);
+					ops.residual_gradient(f, w, k, false);
+				}
+				ops.step(w, k, x.cores[k]);
+				if (k + 1 < d) move_core(x, k + 1, true);
+			}
+		}
+	};
+
+	move_core(x, 0, false);  // canonicalize_left
+	solve_with_current_ranks();
+	while (residualNorm > o.targetResidualNorm && x.ranks() != maxRanks && (o.maxIterations == 0 || iteration < o.maxIterations)) {
+		move_core(x, 0, true);  // adf.cpp:590-602
+		TT rnd = tt_random(x.dims, dims_t(d - 1, 1), false);
+		const double scale = 1e-6 * tt_frob_norm(x) / tt_frob_norm(rnd);
+		tt_scale(rnd, scale);
+		x = tt_add(x, rnd);
+		round(x, maxRanks, 8 * DBL_EPSILON);
+		solve_with_current_ranks();
+	}
+	if (iterations) *iterations = iteration;
+	return residualNorm;
+}
 }  // namespace
 
 template <class T>
@@ -471,6 +628,9 @@ std::vector<std::vector<double>> adf_full_gradient(const TT& x, const RankOneMea
 }
 template <class T>
 double adf_solve(const ADFOptions& o, TT& x, const Measurements& m, const dims_t& maxRanks, size_t* iterations) {
+	const bool stacked = !getenv("XTT_ADF_NO_STACKS") &&
+	                     std::all_of(x.dims.begin(), x.dims.end(), [](size_t n) { return n <= (size_t)ADF_STACK_MAX_N; });
+	if (stacked) return solve_stacked<T>(o, x, m, maxRanks, iterations);
 	return solve<T>(o, x, SinglePointOps<T>{m}, maxRanks, iterations);
 }
 template <class T>

@@ -9,8 +9,10 @@ struct Measurements {  // SinglePointMeasurementSet (include/xerus/measurments.h
 	std::vector<uint32_t> pos;     // P×d, lexicographically sorted
 	std::vector<double> values;
 	DBuf<uint32_t> dpos;           // device copy of pos
+	mutable std::shared_ptr<AdfPlan> plan;  // ADF's stack plan of these positions, built on first use
 	size_t size() const { return values.size(); }
 	void upload();
+	const AdfPlan& stack_plan() const;
 	void check_against(const TT& x) const;  // every position inside x's dimensions
 };
 Measurements meas_random(size_t P, const dims_t& dims);

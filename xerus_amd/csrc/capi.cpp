@@ -480,6 +480,22 @@ int xt_meas_positions_ptr(xt_meas m, const uint32_t** p) { return guard([&] { NN
 int xt_meas_values(xt_meas m, double* o) { return guard([&] { NN(m); NN(o); std::memcpy(o, m->m.values.data(), m->m.values.size() * 8); }); }
 int xt_meas_set_values(xt_meas m, const double* v) { return guard([&] { NN(m); NN(v); std::memcpy(m->m.values.data(), v, m->m.values.size() * 8); }); }
 int xt_meas_measure(xt_meas m, xt_tt x) { return guard([&] { NN(m); FMT(x); xtt::meas_measure(m->m, x->t); }); }
+int xt_meas_stack_plan(xt_meas m, size_t* fc, size_t* bc, uint32_t* fi, uint32_t* bi, uint32_t* fs, uint32_t* bs, size_t* nu) {
+	return guard([&] {
+		NN(m); NN(fc); NN(bc);
+		const xtt::AdfPlan& p = m->m.stack_plan();
+		const size_t d = (size_t)p.d, P = (size_t)p.P;
+		size_t all = 0;
+		for (size_t k = 0; k < d; ++k) { fc[k] = (size_t)p.side[0].count[k]; bc[k] = (size_t)p.side[1].count[k]; all += fc[k]; }
+		if (nu) *nu = all + 1;  // the slot shared by positions -1 and d (adf.cpp:162-166)
+		const size_t nf = (size_t)(p.side[0].first[d - 1] + p.side[0].count[d - 1]);
+		const size_t nb = (size_t)(p.side[1].first[d - 1] + p.side[1].count[d - 1]);
+		if (fi) xtt::read_back(fi, p.side[0].rep.get(), nf * 4);
+		if (bi) xtt::read_back(bi, p.side[1].rep.get(), nb * 4);
+		if (fs) xtt::read_back(fs, p.side[0].slot.get(), d * P * 4);
+		if (bs) xtt::read_back(bs, p.side[1].slot.get(), d * P * 4);
+	});
+}
 
 int xt_adf_gradient(xt_tt x, xt_meas m, int precision, double* host, double* rn) {
 	return guard([&] {

@@ -4,6 +4,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
+
+#include "xtt_common.hpp"
 
 namespace xtt {
 
@@ -200,6 +203,64 @@ int adf_slice_blocks(int P);
 template <class T>
 void adf_slice_norms(const AdfDesc& a, const T* cores, const uint32_t* pos, int k, const T* D, int nslices, double* out,
                      double* partials, hipStream_t s);
+// ---- ADF stacks (adf.cpp:102-312) ----------------------------------------------------------------------------
+// Stack plan of one direction. Forward entry (k, i) is the row G_0[i_0]·…·G_k[i_k] (r_{k+1} values), backward
+// entry (k, i) the column G_k[i_k]·…·G_{d-1}[i_{d-1}] (r_k values). Samples with equal positions 0..k (forward) or
+// k..d-1 (backward) share the entry of level k. Each shared entry is computed once, from its representative (the
+// group's smallest sample id); the entries of a level are ordered by representative, as xerus's _updates[k].
+struct AdfPlanSide {
+	std::vector<long long> count;  // entries per level
+	std::vector<long long> first;  // index in rep of each level's first entry (prefix sums of count)
+	DBuf<uint32_t> slot;           // [k·P + i]: the entry of sample i at level k
+	DBuf<uint32_t> rep;            // [first[k] + g]: the representative of level k's entry g
+};
+struct AdfPlan {
+	int d = 0, P = 0;
+	AdfPlanSide side[2];  // 0: forward (shared prefixes), 1: backward (shared suffixes)
+	DBuf<uint32_t> posT;  // [k·P + i]: the positions mode by mode (the sweep kernels read the column of their core)
+};
+// Builds both sides on the device from P×d positions (device, row-major); maxPos[j] = largest position of mode j.
+void adf_plan_build(AdfPlan& plan, const uint32_t* pos, int P, int d, const std::vector<uint32_t>& maxPos, hipStream_t s);
+// In the sweep kernels below `pos` is the positions' column of mode k (P entries, AdfPlan::posT + k·P).
+// Forward level k of the stacks: F_k[g] = F_{k-1}[slot_{k-1}(rep_g)]·G_k[:, i_k(rep_g), :] (F_{-1} = 1). `core` is
+// G_k (rl × n × rr), `prevSlot`/`prev` level k−1's slots and values (null at k = 0), out count × rr values.
+template <class T>
+void adf_forward_stack(const T* core, int rl, int n, int rr, int k, int d, const uint32_t* pos, int P, const uint32_t* rep, long long count,
+                       const uint32_t* prevSlot, const T* prev, T* out, hipStream_t s);
+// Backward level k: B_k[g] = G_k[:, i_k(rep_g), :]·B_{k+1}[slot_{k+1}(rep_g)] (B_d = 1), out count × rl values.
+template <class T>
+void adf_backward_stack(const T* core, int rl, int n, int rr, int k, int d, const uint32_t* pos, int P, const uint32_t* rep, long long count,
+                        const uint32_t* nextSlot, const T* next, T* out, hipStream_t s);
+// The stacks around core k: left = F_{k-1} (slots of level k−1, values; null at k = 0), right = B_{k+1} (null at
+// k = d−1).
+template <class T>
+struct AdfSides {
+	const uint32_t* fSlot;
+	const T* F;
+	const uint32_t* bSlot;
+	const T* B;
+};
+// Blocks of the residual-gradient and slice kernels (at most 512).
+int adf_stack_blocks(int P);
+// Residual and gradient at core k in one pass: resid_i = F·G_k[:, i_k, :]·B − b_i, norms = {Σ resid², Σ b²} (FP64,
+// fixed-order sums; skipped when norms is null) and grad (rl × n × rr) = Σ_i resid_i·F_iᵀ ⊗ e_{i_k} ⊗ B_iᵀ.
+// `partials` holds adf_stack_blocks(P)·rl·n·rr elements, `normPart` 2·adf_stack_blocks(P) doubles.
+template <class T>
+void adf_stack_residual_gradient(const T* core, int rl, int n, int rr, int k, int d, const uint32_t* pos, int P, const AdfSides<T>& sd,
+                                 const T* values, T* resid, double* norms, T* grad, T* partials, double* normPart, hipStream_t s);
+// update_x of core k (adf.cpp:413-478): normA_j = Σ_{i: i_k = j} (F_i·g[:, j, :]·B_i)², then slice j of the FP64 core
+// moves by −(‖g_j‖²/normA_j)·g_j, g the gradient of ½Σ(x − b)² (xerus's Δ = −g). `partials` holds
+// adf_stack_blocks(P)·n doubles.
+template <class T>
+void adf_stack_step(double* core, const T* g, int rl, int n, int rr, int k, int d, const uint32_t* pos, int P, const AdfSides<T>& sd,
+                    double* partials, hipStream_t s);
+// out[e] = Σ_b partials[b·size + e] in a fixed order (blocks' partial gradients); out[j] = Σ_b partials[b·nslices + j]
+// in block order (slice sums).
+template <class T>
+void adf_sum_partials(const T* partials, int nblocks, int size, T* out, hipStream_t s);
+void adf_sum_slices(const double* partials, int nblocks, int nslices, double* out, hipStream_t s);
+constexpr int ADF_STACK_MAX_N = 2048;  // largest mode size of the stack kernels (slice sums in LDS)
+
 // update_x (adf.cpp:468-487): core (FP64, rl × n × rr) −= step·g, where g is the gradient of ½Σ(x − b)² for that core
 // (xerus's Δ = −g) and step = ‖g_j‖²/normA[j] per slice j (single points) or ‖g‖²/normA[0] (rank-one, perSlice false).
 template <class T>

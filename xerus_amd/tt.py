@@ -598,6 +598,23 @@ class SinglePointMeasurementSet:
         call("xt_meas_positions_ptr", self._h, C.byref(p))
         return p.value
 
+    def stack_plan(self, order: int) -> dict:
+        """ADF's stack plan of these positions (adf.cpp:102-191), built on the device: per level k the representative
+        sample ids of the forward entries (shared positions 0..k) and backward entries (shared k..d-1), ascending,
+        and every sample's entry per level (slots[k, i])."""
+        ps = C.POINTER(C.c_size_t)
+        pu = C.POINTER(C.c_uint32)
+        fc, bc = np.zeros(order, dtype=np.uintp), np.zeros(order, dtype=np.uintp)
+        nu = C.c_size_t()
+        call("xt_meas_stack_plan", self._h, fc.ctypes.data_as(ps), bc.ctypes.data_as(ps), None, None, None, None, C.byref(nu))
+        P = self.size()
+        fi, bi = np.empty(int(fc.sum()), dtype=np.uint32), np.empty(int(bc.sum()), dtype=np.uint32)
+        fs, bs = np.empty((order, P), dtype=np.uint32), np.empty((order, P), dtype=np.uint32)
+        call("xt_meas_stack_plan", self._h, fc.ctypes.data_as(ps), bc.ctypes.data_as(ps), fi.ctypes.data_as(pu), bi.ctypes.data_as(pu),
+             fs.ctypes.data_as(pu), bs.ctypes.data_as(pu), C.byref(nu))
+        return {"fwd_counts": fc.astype(int), "bwd_counts": bc.astype(int), "fwd_ids": fi, "bwd_ids": bi, "fwd_slots": fs,
+                "bwd_slots": bs, "num_unique": nu.value}
+
     @property
     def measuredValues(self) -> np.ndarray:
         out = np.empty(self.size())
