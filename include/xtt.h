@@ -45,6 +45,11 @@ int xt_memcpy_dtoh(void* host_dst, const void* device_src, size_t bytes);
 int xt_release_cache(void);
 /* device bytes held by live library buffers and by the allocator's cache of freed blocks (either nullable) */
 int xt_cache_bytes(size_t* in_use, size_t* cached);
+/* Diagnostics, no xerus counterpart: replays of recorded calls accepted since the process started (all threads): the
+ * speculative steady ALS call (from the third speculative call with the same shapes, options and operands) and the
+ * ADF sweep (from the second sweep of a rank phase) run as one hipGraph launch each. XTT_NO_GRAPHS=1 turns recording
+ * off. */
+int xt_graph_replays(size_t* als, size_t* adf);
 /* Launch profiler (no counterpart in xerus; replaces its XERUS_PA_START/END timers, misc/performanceAnalysis.h:33-56):
  * while enabled, GEMM, Householder, Jacobi and ADF launches get a HIP event pair handed to the dispatch on their
  * own stream. xt_profile_read synchronises the device, returns per-kernel (tag, launches, total ms, algorithmic

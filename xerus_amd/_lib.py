@@ -61,6 +61,7 @@ SIGNATURES = {
     "xt_synchronize": (C.c_int, []),
     "xt_release_cache": (C.c_int, []),
     "xt_cache_bytes": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "xt_graph_replays": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "xt_malloc": (C.c_int, [_sz, C.POINTER(_vp)]),
     "xt_free": (C.c_int, [_vp]),
     "xt_memcpy_htod": (C.c_int, [_vp, _vp, _sz]),
@@ -273,3 +274,10 @@ class DeviceArray:
         if p and _lib is not None:
             _lib.xt_free(p)
             self.ptr = None
+
+
+def graph_replays():
+    """(ALS call, ADF sweep) replays of recorded calls accepted since the process started."""
+    out = [C.c_size_t() for _ in range(2)]
+    call("xt_graph_replays", *[C.byref(o) for o in out])
+    return tuple(o.value for o in out)

@@ -637,12 +637,15 @@ __global__ __launch_bounds__(AB) void adf_slice_kernel(AdfDesc a, const T* cores
 	}
 }
 
-__global__ void adf_slice_reduce_kernel(const double* partials, int nblocks, int nslices, double* out) {
-	for (int j = threadIdx.x; j < nslices; j += blockDim.x) {
-		double s = 0;
-		for (int b = 0; b < nblocks; ++b) s += partials[(size_t)b * nslices + j];
-		out[j] = s;
-	}
+// one wave per slice: lane l sums blocks l, l + 64, … in order, then a fixed butterfly (deterministic)
+__global__ __launch_bounds__(256) void adf_slice_reduce_kernel(const double* partials, int nblocks, int nslices, double* out) {
+	const int lane = threadIdx.x & 63;
+	const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+	if (j >= nslices) return;
+	double s = 0;
+	for (int b = lane; b < nblocks; b += 64) s += partials[(size_t)b * nslices + j];
+	s = wsum64(s);
+	if (lane == 0) out[j] = s;
 }
 
 }  // namespace
@@ -786,7 +789,7 @@ void adf_slice_norms(const AdfDesc& a, const T* cores, const uint32_t* pos, int 
 	const int blocks = adf_slice_blocks(a.P);
 	hipLaunchKernelGGL(adf_slice_kernel<T>, dim3(blocks), dim3(AB), 0, s, a, cores, pos, k, D, partials, nslices);
 	XTT_LAUNCH_CHECK();
-	hipLaunchKernelGGL(adf_slice_reduce_kernel, dim3(1), dim3(64), 0, s, partials, blocks, nslices, out);
+	hipLaunchKernelGGL(adf_slice_reduce_kernel, dim3((nslices + 3) / 4), dim3(256), 0, s, partials, blocks, nslices, out);
 	XTT_LAUNCH_CHECK();
 }
 
@@ -796,7 +799,7 @@ void adf_sum_partials(const T* partials, int nblocks, int size, T* out, hipStrea
 	XTT_LAUNCH_CHECK();
 }
 void adf_sum_slices(const double* partials, int nblocks, int nslices, double* out, hipStream_t s) {
-	hipLaunchKernelGGL(adf_slice_reduce_kernel, dim3(1), dim3(64), 0, s, partials, nblocks, nslices, out);
+	hipLaunchKernelGGL(adf_slice_reduce_kernel, dim3((nslices + 3) / 4), dim3(256), 0, s, partials, nblocks, nslices, out);
 	XTT_LAUNCH_CHECK();
 }
 template void adf_sum_partials<float>(const float*, int, int, float*, hipStream_t);

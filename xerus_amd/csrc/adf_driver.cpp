@@ -109,15 +109,19 @@ bool Flat<T>::matches(const TT& x) const {
 }
 
 template <class T>
-void Flat<T>::refresh(const TT& x, size_t first, size_t last) {
+void Flat<T>::refresh(const TT& x, size_t first, size_t last) {  // one launch per kMultiCopy cores
+	MultiCopy mc;
+	auto flush = [&] {
+		if constexpr (sizeof(T) == 8) multi_copy_f64(mc, stream()); else multi_convert_f64_f32(mc, stream());
+		mc.count = 0;
+	};
 	for (size_t k = first; k < x.order() && k <= last; ++k) {
-		const DTensor& c = x.cores[k];
-		if constexpr (sizeof(T) == 8) {
-			XTT_HIP(hipMemcpyAsync(cores.get() + hoff[k], c.data(), c.size() * 8, hipMemcpyDeviceToDevice, stream()));
-		} else {
-			convert_f64_f32(c.data(), reinterpret_cast<float*>(cores.get()) + hoff[k], c.size(), stream());
-		}
+		mc.src[mc.count] = x.cores[k].data();
+		mc.dst[mc.count] = reinterpret_cast<double*>(cores.get() + hoff[k]);
+		mc.n[mc.count] = x.cores[k].size();
+		if (++mc.count == kMultiCopy) flush();
 	}
+	flush();
 }
 
 template struct Flat<double>;
@@ -564,19 +568,47 @@ double solve_stacked(const ADFOptions& o, TT& x, const Measurements& m, const di
 	PerfClock clock;
 	clock.begin();
 
+	// one sweep of update_x over every core (adf.cpp:525-540)
+	auto sweep = [&](TT& y) {
+		for (size_t k = 0; k < d; ++k) {
+			if (k > 0) {  // move_core(k) changed cores k−1 and k: forward level k−1, then residual and gradient at k
+				f.refresh(y, k - 1, k);
+				ops.forward(f, k - 1);
+				ops.residual_gradient(f, w, k, false);
+			}
+			ops.step(w, k, y.cores[k]);
+			if (k + 1 < d) move_core(y, k + 1, true);
+		}
+	};
+	// the start of an iteration (adf.cpp:495-511): core to 0, backward stacks, residual (and gradient) at core 0
+	auto start = [&](TT& y) {
+		move_core(y, 0, true);
+		f.refresh(y);
+		for (size_t k = d - 1; k > 0; --k) ops.backward(f, k);
+		ops.residual_gradient(f, w, 0, true);
+	};
+
+	// Within a rank phase every sweep has the same kernels and shapes: from the second sweep on, the sweep and the
+	// next iteration's start are one recorded graph (TTGraph), launched once per iteration; the residual norms are
+	// read after it. The last allowed iteration's sweep runs alone, so x ends as the eager loop leaves it.
 	auto solve_with_current_ranks = [&]() {
 		double resDec1 = 0.0, resDec2 = 0.0, resDec3 = 0.0;
+		if (o.maxIterations != 0 && iteration >= o.maxIterations) return;
+		move_core(x, 0, true);
+		if (!f.matches(x)) {
+			f.build(x);
+			w.ensure(P, f.desc.gradSize, maxN);
+		} else {
+			f.refresh(x);
+		}
+		ops.layout(f);
+		for (size_t k = d - 1; k > 0; --k) ops.backward(f, k);
+		ops.residual_gradient(f, w, 0, true);
+		TTGraph g;
+		g.kind = 1;
+		bool recordFailed = !graphs_enabled();
+		size_t eagerSweeps = 0;
 		for (; o.maxIterations == 0 || iteration < o.maxIterations; ++iteration) {
-			move_core(x, 0, true);
-			if (!f.matches(x)) {
-				f.build(x);
-				w.ensure(P, f.desc.gradSize, maxN);
-			} else {
-				f.refresh(x);
-			}
-			ops.layout(f);
-			for (size_t k = d - 1; k > 0; --k) ops.backward(f, k);  // rebuild the backward stacks
-			ops.residual_gradient(f, w, 0, true);
 			double norms[2];
 			read_back(norms, w.norms.get(), 16);  // the only synchronisation of an iteration
 			lastResidualNorm = residualNorm;
@@ -590,14 +622,20 @@ double solve_stacked(const ADFOptions& o, TT& x, const Measurements& m, const di
 			resDec2 = resDec1;
 			resDec1 = residualNorm / lastResidualNorm;
 			if (residualNorm < o.targetResidualNorm || resDec1 * resDec2 * resDec3 * resDec4 > std::pow(o.minimalResidualNormDecrease, 4)) break;
-			for (size_t k = 0; k < d; ++k) {
-				if (k > 0) {  // move_core(k) changed cores k−1 and k: forward level k−1, then residual and gradient at k
-					f.refresh(x, k - 1, k);
-					ops.forward(f, k - 1);
-					ops.residual_gradient(f, w, k, false);
-				}
-				ops.step(w, k, x.cores[k]);
-				if (k + 1 < d) move_core(x, k + 1, true);
+			const bool last = o.maxIterations != 0 && iteration + 1 >= o.maxIterations;
+			if (last) {
+				sweep(x);
+			} else if (g.ready()) {
+				g.launch(x);
+				g.take(x);
+			} else if (eagerSweeps >= 1 && !recordFailed && g.record(x, [&](TT& y) { sweep(y); start(y); })) {
+				g.launch(x);
+				g.take(x);
+			} else {
+				if (eagerSweeps >= 1) recordFailed = true;
+				sweep(x);
+				start(x);
+				++eagerSweeps;
 			}
 		}
 	};

@@ -550,20 +550,27 @@ __global__ __launch_bounds__(SB) void stack_slice_kernel(const T* D, int rl, int
 
 // update_x (adf.cpp:468-478) from the blocks' slice partials: normA_j = Σ_b partials[b][j] in block order, then
 // slice j of the core moves by −(‖g_j‖²/normA_j)·g_j (g the gradient of ½Σ(x − b)², xerus's Δ = −g).
+// One wave per slice: lane l sums the partials of blocks l, l + 64, … in order and the lanes are added by a fixed
+// butterfly (deterministic); the same wave forms ‖g_j‖² and updates the slice's rl·rr entries.
 template <class T>
 __global__ __launch_bounds__(256) void stack_update_kernel(double* core, const T* g, const double* partials, int nblocks, int rl, int n, int rr) {
-	for (int j = threadIdx.x; j < n; j += 256) {
-		double na = 0;
-		for (int b = 0; b < nblocks; ++b) na += partials[(size_t)b * n + j];
-		double pyr = 0;
-		for (int al = 0; al < rl; ++al)
-			for (int c = 0; c < rr; ++c) { const double t = (double)g[((size_t)al * n + j) * rr + c]; pyr += t * t; }
-		const double st = pyr / na;
-		for (int al = 0; al < rl; ++al)
-			for (int c = 0; c < rr; ++c) {
-				const size_t e = ((size_t)al * n + j) * rr + c;
-				core[e] -= st * (double)g[e];
-			}
+	const int lane = threadIdx.x & 63;
+	const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+	if (j >= n) return;
+	double na = 0;
+	for (int b = lane; b < nblocks; b += 64) na += partials[(size_t)b * n + j];
+	na = wave_sum(na);
+	const int E = rl * rr;
+	double pyr = 0;
+	for (int e = lane; e < E; e += 64) {
+		const double t = (double)g[((size_t)(e / rr) * n + j) * rr + e % rr];
+		pyr += t * t;
+	}
+	pyr = wave_sum(pyr);
+	const double st = pyr / na;
+	for (int e = lane; e < E; e += 64) {
+		const size_t idx = ((size_t)(e / rr) * n + j) * rr + e % rr;
+		core[idx] -= st * (double)g[idx];
 	}
 }
 
@@ -665,7 +672,7 @@ void adf_stack_step(double* core, const T* g, int rl, int n, int rr, int k, int 
 	launch_timed(sizeof(T) == 8 ? "adf_stack_slice_kernel<double>" : "adf_stack_slice_kernel<float>",
 	             Work(2.0 * P * (rl * rr + rl), (double)P * (12.0 + sizeof(T) * (rl + rr))), stack_slice_kernel<T>, dim3(blocks), dim3(SB), lds, s,
 	             g, rl, n, rr, k, d, pos, P, sd, partials);
-	hipLaunchKernelGGL(stack_update_kernel<T>, dim3(1), dim3(256), 0, s, core, g, (const double*)partials, blocks, rl, n, rr);
+	hipLaunchKernelGGL(stack_update_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, s, core, g, (const double*)partials, blocks, rl, n, rr);
 	XTT_LAUNCH_CHECK();
 }
 

@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <cstdlib>
 #include <atomic>
 #include <exception>
@@ -219,6 +221,19 @@ void push_both(const DTensor* L, const DTensor& X, const DTensor* Am, const RhsP
 	gemm_f64(gemm_desc(1, 0, c, Q * c, a * n), X.data(), T2.get(), opOut->data(), stream());
 }
 
+// A speculative ALS call recorded as a graph (TTGraph): the device slots it leaves its results in (the failed-site
+// counter, then (x·b_loc, x·A_loc·x) per energy evaluation), the half-sweep count it was recorded with, and its
+// statistics. Replays are accepted when no site failed and the replayed energies stop the sweeps at the same count.
+struct ALSRecord {
+	TTGraph g;
+	void* slots = nullptr;        // arena memory: unsigned fail count, 4 bytes pad, then 2 doubles per energy
+	size_t energies = 0;          // energy evaluations per call (the start and each half-sweep end)
+	size_t halfSweeps = 0;        // of the eager speculative call the recording follows (0: none yet)
+	bool shapeStop = false;       // second − first <= sites: the range alone ends the sweep
+	bool failed = false;          // recording failed: stay eager for this key
+	ALSStats delta;               // statistics of one call (energies excluded)
+};
+
 // Persistent CG workspace of the library (one per process). Every buffer a CG chunk touches has a fixed
 // address while the capacities do not grow, so a chunk of CG iterations recorded once per local shape into a
 // hipGraph is replayed for every site of that shape (and across ALS calls). Growing a buffer drops the graphs.
@@ -234,13 +249,18 @@ struct Workspace {
 	std::map<std::vector<size_t>, hipGraphExec_t> graphs;
 	std::set<std::vector<size_t>> seen;
 
+	// Recorded speculative ALS calls (they read the workspace buffers in place), by call key.
+	std::map<std::vector<size_t>, std::unique_ptr<struct ALSRecord>> calls;
+
 	void drop_graphs() {
 		for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
 		graphs.clear();
 		seen.clear();
+		calls.clear();
 	}
 	void ensure(size_t N, size_t t1, size_t t2, size_t nl, size_t nr, size_t nam) {
 		if (N <= capN && t1 <= capT1 && t2 <= capT2 && nl <= capL && nr <= capR && nam <= capAm && st.get()) return;
+		ArenaScope persistent(nullptr);  // the workspace outlives any recorded call
 		sync();
 		drop_graphs();
 		capN = std::max(capN, N); capT1 = std::max(capT1, t1); capT2 = std::max(capT2, t2);
@@ -321,6 +341,12 @@ struct ALSData {
 	// residual checks run on the device and count the sites that were not into *specFail, read once at the end.
 	bool spec = false;
 	unsigned* specFail = nullptr;
+	// Recording a speculative call (graphMode): energies go to device slots (x·b_loc, x·A_loc·x at
+	// energySlots[2e], [2e + 1]) instead of being read back, and the sweep ends after plannedHalfSweeps.
+	bool graphMode = false;
+	double* energySlots = nullptr;
+	size_t energyCount = 0;
+	size_t plannedHalfSweeps = 0;
 	size_t cgIters = 0;  // inner CG iterations of this call
 	// Fused CG solves (one grid-synchronised launch each) count their iterations and matvec flops on the device;
 	// finish_counts() reads them once at the end of the call.
@@ -351,17 +377,20 @@ struct ALSData {
 	void prepare_x() {  // als.cpp:330-407
 		const size_t d = x.order();
 		size_t f = 0, dimProd = 1;
-		std::vector<double> h;
+		// A full-rank boundary core becomes the identity of its unfolding (dimProd·localDim = newProd), absorbed into
+		// its neighbour; built on the device (no host data, so a recorded call can contain it).
+		auto identity = [](dims_t dims, size_t n) {
+			DTensor t(std::move(dims));
+			eye_f64(t.data(), n, stream());
+			return t;
+		};
 		while (f + 1 < d) {
 			const size_t localDim = x.dims[f], newProd = dimProd * localDim;
 			if (x.rank(f) < newProd) break;
 			DTensor cur = dt_copy(x.cores[f]);
 			cur.reshape({cur.dims[0] * cur.dims[1], cur.dims[2]});
 			x.set_component(f + 1, contract(cur, false, x.cores[f + 1], false, 1));
-			h.assign(dimProd * localDim * newProd, 0.0);
-			for (size_t i = 0; i < dimProd; ++i) for (size_t j = 0; j < localDim; ++j) h[(i * localDim + j) * newProd + i * localDim + j] = 1.0;
-			x.set_component(f, dt_upload({dimProd, localDim, newProd}, h.data()));
-			sync();
+			x.set_component(f, identity({dimProd, localDim, newProd}, newProd));
 			f += 1; dimProd = newProd;
 		}
 		size_t l = d;
@@ -372,10 +401,7 @@ struct ALSData {
 			DTensor cur = dt_copy(x.cores[l - 1]);
 			cur.reshape({cur.dims[0], cur.dims[1] * cur.dims[2]});
 			x.set_component(l - 2, contract(x.cores[l - 2], false, cur, false, 1));
-			h.assign(newProd * localDim * dimProd, 0.0);
-			for (size_t j = 0; j < localDim; ++j) for (size_t k = 0; k < dimProd; ++k) h[((j * dimProd + k) * localDim + j) * dimProd + k] = 1.0;
-			x.set_component(l - 1, dt_upload({newProd, localDim, dimProd}, h.data()));
-			sync();
+			x.set_component(l - 1, identity({newProd, localDim, dimProd}, newProd));
 			l -= 1; dimProd = newProd;
 		}
 		if (canonAtEnd && corePosAtEnd < f) {
@@ -487,6 +513,19 @@ struct ALSData {
 	}
 
 	double energy_f() {  // als.cpp:480-545, evaluated from the local quantities at curr
+		if (graphMode) {  // one-site SPD: the two dot products into the next pair of slots, nothing read back
+			const DTensor& xm = x.cores[curr];
+			DTensor bl = local_rhs();
+			DTensor keepOp;
+			SiteOp s;
+			site_op(s, keepOp);
+			DTensor y(xm.dims);
+			matvec64(s, xm.data(), y.data(), nullptr);
+			double* dst = energySlots + 2 * energyCount++;
+			dot_f64(xm.data(), bl.data(), xm.size(), dst, scratch.partials.get(), scratch.counter.get(), stream());
+			dot_f64(xm.data(), y.data(), xm.size(), dst + 1, scratch.partials.get(), scratch.counter.get(), stream());
+			return std::nan("");
+		}
 		if (o.useResidualForEndCriterion && A && o.assumeSPD) return tt_residual(*A, x, b);
 		DTensor keepX;
 		if (o.sites > 1) keepX = merged(x);
@@ -702,7 +741,7 @@ struct ALSData {
 	struct Scratch64 {
 		DBuf<double> partials{2 * 1024}, out{2};
 		DBuf<unsigned> counter{1};
-		Scratch64() { XTT_HIP(hipMemsetAsync(counter.get(), 0, sizeof(unsigned), stream())); }
+		Scratch64() { zero_u32(counter.get(), 1, stream()); }  // a kernel: the speculative call may be recorded
 		// r = b − q; rb = {‖r‖², ‖b‖²}
 		void residual(double* r, const double* b, const double* q, size_t n, double* rb) {
 			residual_norms_f64(r, b, q, n, out.get(), partials.get(), counter.get(), stream());
@@ -809,8 +848,10 @@ struct ALSData {
 			energy = energy_f();
 			if (stats) { stats->halfSweeps = halfSweepCount; stats->energies.push_back(energy); }
 			perf_point(increasing ? 1 : 3, o.useResidualForEndCriterion);
-			if (halfSweepCount == o.numHalfSweeps || std::abs(lastEnergy - energy) < o.convergenceEpsilon ||
-			    std::abs(lastEnergy2 - energy) < o.convergenceEpsilon || second - first <= o.sites) {
+			const bool stop = graphMode ? halfSweepCount == plannedHalfSweeps
+			                            : (halfSweepCount == o.numHalfSweeps || std::abs(lastEnergy - energy) < o.convergenceEpsilon ||
+			                               std::abs(lastEnergy2 - energy) < o.convergenceEpsilon || second - first <= o.sites);
+			if (stop) {
 				if (canonAtEnd && o.preserveCorePosition) move_core(x, corePosAtEnd, true);
 				return true;
 			}
@@ -928,11 +969,18 @@ double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats*
 		if (t) { for (size_t r : t->ranks()) key.push_back(r); key.push_back(0); }
 	key.insert(key.end(), x.dims.begin(), x.dims.end());
 	bool timedOut = false;
-	auto run = [&](bool spec, unsigned* fail, size_t& cgIters, bool noFused) {
-		ALSData data(o, A, x, b, stats);
+	size_t lastHalfSweeps = 0;
+	bool lastShapeStop = false;
+	// graph: record a speculative call on the stand-in y (energies to device slots, plannedHalfSweeps sweeps)
+	auto run_on = [&](TT& y, ALSStats* st, bool spec, unsigned* fail, size_t& cgIters, bool noFused, double* energySlots,
+	                  size_t plannedHalfSweeps) {
+		ALSData data(o, A, y, b, st);
 		data.spec = spec;
 		data.specFail = fail;
 		data.noFused = noFused;
+		data.graphMode = energySlots != nullptr;
+		data.energySlots = energySlots;
+		data.plannedHalfSweeps = plannedHalfSweeps;
 		data.clock.begin();
 		data.energy = data.energy_f();
 		data.perf_point(3, false);
@@ -944,21 +992,102 @@ double als_solve(const ALSOptions& o, const TT* A, TT& x, const TT& b, ALSStats*
 		data.finish_counts();
 		cgIters = data.cgIters;
 		timedOut = data.fusedTimedOut;
+		lastHalfSweeps = data.halfSweepCount;
+		lastShapeStop = data.second - data.first <= o.sites;
 		return data.energy;
+	};
+	auto run = [&](bool spec, unsigned* fail, size_t& cgIters, bool noFused) {
+		return run_on(x, stats, spec, fail, cgIters, noFused, nullptr, 0);
 	};
 	auto it = lastConverged.find(key);
 	if (A && o.sites == 1 && !perf_sink().cb && it != lastConverged.end() && it->second && !getenv("XTT_NO_SPECULATION")) {
-		TT saved = tt_copy(x);
-		const ALSStats savedStats = stats ? *stats : ALSStats{};
-		DBuf<unsigned> fail(1);
-		XTT_HIP(hipMemsetAsync(fail.get(), 0, sizeof(unsigned), stream()));
-		size_t cgIters = 0;
-		const double e = run(true, fail.get(), cgIters, false);
-		unsigned failed = 0;
-		read_back(&failed, fail.get(), sizeof(unsigned));
-		if (failed == 0) return e;
-		x = std::move(saved);
-		if (stats) *stats = savedStats;
+		// From the third speculative call with the same key (shapes, options, the addresses of A's and b's cores, x's
+		// core position), the call is a recorded graph (TTGraph). Its energies are checked against the recorded
+		// half-sweep count; a failed site or a different count falls back to the normal run, with x untouched.
+		ALSRecord* rec = nullptr;
+		if (graphs_enabled() && o.assumeSPD && !o.useResidualForEndCriterion && &b != &x) {
+			std::vector<size_t> gkey = key;
+			for (const TT* t : {A, &b})
+				for (const DTensor& c : t->cores) gkey.push_back((size_t)c.data());
+			for (double v : {o.convergenceEpsilon, o.localTol, o.innerTol}) { uint64_t u; std::memcpy(&u, &v, 8); gkey.push_back((size_t)u); }
+			gkey.insert(gkey.end(), {(size_t)o.maxLocalIter, (size_t)o.maxRefine, (size_t)o.preserveCorePosition, (size_t)x.canonicalized,
+			                         x.corePosition});
+			auto& calls = workspace().calls;
+			if (calls.size() > 32) calls.clear();
+			auto& slot = calls[gkey];
+			if (!slot) { slot = std::make_unique<ALSRecord>(); slot->g.kind = 0; }
+			rec = slot.get();
+		}
+		// a replay: launch, read the slots once, accept when no site failed and the stop rule agrees
+		auto replay = [&](ALSRecord& r, double& energyOut) {
+			r.g.launch(x);
+			std::vector<double> h(1 + 2 * r.energies);
+			read_back(h.data(), r.slots, h.size() * sizeof(double));
+			unsigned failedSites;
+			std::memcpy(&failedSites, h.data(), sizeof(unsigned));
+			if (failedSites != 0) return false;
+			std::vector<double> E(r.energies);
+			for (size_t e = 0; e < r.energies; ++e) E[e] = std::abs(0.5 * h[2 + 2 * e] - h[1 + 2 * e]);
+			double le2 = 1e102, le = 1e101, en = E[0];
+			for (size_t hs = 1; hs <= r.halfSweeps; ++hs) {
+				le2 = le; le = en; en = E[hs];
+				const bool stop = hs == o.numHalfSweeps || std::abs(le - en) < o.convergenceEpsilon || std::abs(le2 - en) < o.convergenceEpsilon ||
+				                  r.shapeStop;
+				if (stop != (hs == r.halfSweeps)) return false;
+			}
+			r.g.take(x);
+			if (stats) {
+				stats->localSolves += r.delta.localSolves;
+				stats->matvecs64 += r.delta.matvecs64;
+				stats->flopsMatvec += r.delta.flopsMatvec;
+				stats->flopsEnv += r.delta.flopsEnv;
+				stats->halfSweeps = r.halfSweeps;
+				for (size_t hs = 1; hs <= r.halfSweeps; ++hs) stats->energies.push_back(E[hs]);
+			}
+			energyOut = E[r.halfSweeps];
+			return true;
+		};
+		double e = 0;
+		if (rec && rec->g.ready()) {
+			if (replay(*rec, e)) return e;
+		} else if (rec && rec->halfSweeps > 0 && !rec->failed) {
+			ALSStats local;
+			size_t cgIters = 0;
+			const size_t planned = rec->halfSweeps, nE = planned + 1;
+			void* slots = nullptr;
+			const bool recorded = rec->g.record(x, [&](TT& y) {
+				slots = dmalloc(8 * (1 + 2 * nE));  // the capture arena's
+				zero_u32(static_cast<unsigned*>(slots), 1, stream());
+				run_on(y, &local, true, static_cast<unsigned*>(slots), cgIters, false, static_cast<double*>(slots) + 1, planned);
+			});
+			if (recorded && lastHalfSweeps == planned) {
+				rec->slots = slots;
+				rec->energies = nE;
+				rec->shapeStop = lastShapeStop;
+				rec->delta = local;
+				if (replay(*rec, e)) return e;
+			} else {
+				if (recorded && getenv("XTT_DEBUG_GRAPHS"))
+					fprintf(stderr, "xtt: recorded ALS call ran %zu half-sweeps, planned %zu\n", lastHalfSweeps, planned);
+				rec->g.reset();
+				rec->failed = true;
+			}
+		} else {
+			TT saved = tt_copy(x);
+			const ALSStats savedStats = stats ? *stats : ALSStats{};
+			DBuf<unsigned> fail(1);
+			XTT_HIP(hipMemsetAsync(fail.get(), 0, sizeof(unsigned), stream()));
+			size_t cgIters = 0;
+			e = run(true, fail.get(), cgIters, false);
+			unsigned failed = 0;
+			read_back(&failed, fail.get(), sizeof(unsigned));
+			if (failed == 0) {
+				if (rec) rec->halfSweeps = lastHalfSweeps;
+				return e;
+			}
+			x = std::move(saved);
+			if (stats) *stats = savedStats;
+		}
 	}
 	size_t cgIters = 0;
 	// A fused (grid-synchronised) local solve whose grid barrier ran out of time left its site unsolved: the whole call

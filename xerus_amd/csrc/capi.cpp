@@ -49,6 +49,14 @@ int xt_release_cache(void) { return guard([] { xtt::sync(); xtt::release_cache()
 int xt_cache_bytes(size_t* in_use, size_t* cached) {
 	return guard([&] { if (in_use) *in_use = xtt::bytes_in_use(); if (cached) *cached = xtt::bytes_cached(); });
 }
+int xt_graph_replays(size_t* als, size_t* adf) {
+	return guard([&] {
+		size_t r[2];
+		xtt::graph_replays(r);
+		if (als) *als = r[0];
+		if (adf) *adf = r[1];
+	});
+}
 int xt_profile_enable(int on) { return guard([&] { xtt::prof_enable(on != 0); }); }
 int xt_profile_read(size_t max_records, char* tags, size_t tag_len, size_t* launches, double* ms, double* flops, double* bytes,
                     size_t* count) {

@@ -323,10 +323,11 @@ void launch_x4(const GemmDesc& d, const T* A, const T* B, T* C, hipStream_t s) {
 	if (prof_shapes()) tag += shape_suffix(d);
 	// Split-K for few output tiles over a long K (the stack contractions Xᵀ·T over a·n rows): K slices of >= 64
 	// run as blockIdx.z, partials go to a stream-ordered scratch buffer and one epilogue launch sums them.
-	// Not while the stream is being captured (the scratch buffer would outlive its allocation in the graph).
+	// While the stream is being captured only with a capture arena (its scratch then belongs to the graph); without
+	// one the scratch would be returned to the cache while the graph still uses it.
 	const size_t tiles32 = ((d.m + 31) / 32) * ((d.n + 31) / 32);
 	hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-	if (!big && d.batch == 1 && tiles32 < 32 && d.k >= 256) XTT_HIP(hipStreamIsCapturing(s, &cap));
+	if (!big && d.batch == 1 && tiles32 < 32 && d.k >= 256 && !arena_active()) XTT_HIP(hipStreamIsCapturing(s, &cap));
 	if (!big && d.batch == 1 && tiles32 < 32 && d.k >= 256 && cap == hipStreamCaptureStatusNone) {
 		const size_t want = std::max<size_t>(2, 128 / tiles32);
 		const size_t slices0 = std::min<size_t>(d.k / 64, want);

@@ -36,6 +36,8 @@ void revtrans_f64(const double* in, size_t rows, size_t cols, double* out, hipSt
 void convert_f64_f32(const double* in, float* out, size_t n, hipStream_t s);
 void convert_f32_f64(const float* in, double* out, size_t n, hipStream_t s);
 void fill_f64(double* p, double v, size_t n, hipStream_t s);
+void zero_u32(unsigned* p, size_t n, hipStream_t s);  // a kernel, so it may be recorded into a graph
+void eye_f64(double* p, size_t n, hipStream_t s);     // the n×n identity
 // dst[j][0..n[j]) = src[j][0..n[j]) for j < count, one launch (count <= kMultiCopy; empty copies allowed)
 constexpr int kMultiCopy = 32;
 struct MultiCopy {
@@ -45,6 +47,8 @@ struct MultiCopy {
 	int count = 0;
 };
 void multi_copy_f64(const MultiCopy& c, hipStream_t s);
+// the same with FP32 destinations (dst[j] points to floats) and conversion
+void multi_convert_f64_f32(const MultiCopy& c, hipStream_t s);
 // *out = the truncation rank of calculate_svd for singular values sv[0..k) (device): min(k, maxRank unless 0), cut
 // at the first j >= 1 with sv[j] <= eps·sv[0]
 void svd_rank_f64(const double* sv, size_t k, size_t maxRank, double eps, int* out, hipStream_t s);

@@ -1,4 +1,5 @@
 // libxtt runtime: thread-local error text, the per-device library stream and the caching device allocator.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <map>
@@ -91,7 +92,40 @@ void read_back(void* host, const void* dev, size_t bytes, hipStream_t s) {
 	XTT_HIP(e);
 }
 
+namespace {
+thread_local Arena* t_arena = nullptr;
+constexpr size_t kArenaChunk = 64ull << 20;
+}  // namespace
+
+void* Arena::alloc(size_t bytes) {
+	const size_t b = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+	if (chunks.empty() || used + b > chunks.back().second) {
+		const size_t sz = std::max(kArenaChunk, b);
+		void* p = nullptr;
+		const hipError_t e = hipMalloc(&p, sz);
+		if (e != hipSuccess) {
+			(void)hipGetLastError();
+			throw Error(e == hipErrorOutOfMemory ? ERR_NOMEM : ERR_HIP, std::string("hipMalloc (capture arena): ") + hipGetErrorString(e));
+		}
+		chunks.push_back({static_cast<char*>(p), sz});
+		used = 0;
+	}
+	char* p = chunks.back().first + used;
+	used += b;
+	total += b;
+	return p;
+}
+Arena::~Arena() {
+	if (chunks.empty()) return;
+	(void)hipDeviceSynchronize();
+	for (auto& c : chunks) (void)hipFree(c.first);
+}
+ArenaScope::ArenaScope(Arena* a) : prev(t_arena) { t_arena = a; }
+bool arena_active() { return t_arena != nullptr; }
+ArenaScope::~ArenaScope() { t_arena = prev; }
+
 void* dmalloc(size_t bytes) {
+	if (t_arena) return t_arena->alloc(bytes);
 	if (bytes == 0) bytes = 1;
 	int dev = 0;
 	XTT_HIP(hipGetDevice(&dev));

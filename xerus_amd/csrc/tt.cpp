@@ -22,7 +22,8 @@ struct Scratch {
 	DBuf<double> partials;
 	DBuf<unsigned> counter;
 	DBuf<double> scalar;
-	Scratch() : partials(2 * 1024), counter(1), scalar(4) {
+	Scratch() : Scratch(ArenaScope(nullptr)) {}
+	explicit Scratch(const ArenaScope&) : partials(2 * 1024), counter(1), scalar(4) {  // persistent: never from a capture arena
 		XTT_HIP(hipMemsetAsync(counter.get(), 0, sizeof(unsigned), stream()));
 	}
 };
@@ -590,15 +591,18 @@ void round(TT& x, const dims_t& maxRanks, double eps) {  // ttNetwork.cpp:643-66
 	key.push_back(initPos);
 	auto kept = keptRanks.find(key);
 	if (d > 1 && initCanon && kept != keptRanks.end() && kept->second && !getenv("XTT_NO_SPECULATION")) {
+		auto speculate = [&](TT& y, RankSpec& rs) {
+			move_core_spec(y, d - 1, rs);
+			for (size_t i = 0; i + 1 < d; ++i) round_edge_spec(y, d - 1 - i, d - 2 - i, maxRanks[d - 2 - i], eps, rs);
+			y.corePosition = 0;
+			y.canonicalized = true;
+			move_core_spec(y, initPos, rs);
+		};
 		TT saved = tt_copy(x);
 		bool ok = false;
 		try {
 			RankSpec rs(4 * d + 8);
-			move_core_spec(x, d - 1, rs);
-			for (size_t i = 0; i + 1 < d; ++i) round_edge_spec(x, d - 1 - i, d - 2 - i, maxRanks[d - 2 - i], eps, rs);
-			x.corePosition = 0;
-			x.canonicalized = true;
-			move_core_spec(x, initPos, rs);
+			speculate(x, rs);
 			ok = rs.holds();
 		} catch (const NotSpeculable&) {
 		}

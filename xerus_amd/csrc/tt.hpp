@@ -2,6 +2,7 @@
 // All FP64; cores stay in HBM, only integer rank decisions and scalar norms cross to the host.
 #pragma once
 #include <functional>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -84,6 +85,38 @@ TT tt_identity(const dims_t& dims);
 TT tt_kronecker(const dims_t& dims, bool op);
 TT tt_dirac(const dims_t& dims, const dims_t& position, bool op);
 void tt_transpose(TT& x);
+
+// One call of a TT algorithm recorded as a hipGraph, for calls whose kernels and shapes repeat (the speculative
+// steady ALS call, the ADF sweep): the launches of a call are replayed by one graph launch instead of being issued one by
+// one. record() runs `body` under stream capture on a stand-in TT whose cores are copies of x's cores in the
+// record's capture arena; every allocation the body makes comes from that arena, so the buffers the graph touches
+// keep their addresses and are never handed to anyone else. x is not modified. launch() copies x's cores into the
+// stand-in's input buffers and launches the graph; take() gives x fresh copies of the output cores. The body must not
+// synchronise or read device data back (record() then fails and returns false).
+struct TTGraph {
+	std::unique_ptr<Arena> arena;
+	hipGraphExec_t exec = nullptr;
+	std::vector<double*> in, out;
+	std::vector<dims_t> inDims, outDims;
+	bool canonicalized = true;
+	size_t corePosition = 0;
+	int kind = 0;  // 0 ALS call, 1 ADF sweep: the counter take() adds to (graph_replays)
+	std::string why;  // why the last record() failed (XTT_DEBUG_GRAPHS=1 prints it)
+	TTGraph() = default;
+	TTGraph(const TTGraph&) = delete;
+	TTGraph& operator=(const TTGraph&) = delete;
+	~TTGraph() { reset(); }
+	void reset();
+	bool ready() const { return exec != nullptr; }
+	bool record(const TT& x, const std::function<void(TT&)>& body);
+	void launch(const TT& x);
+	void take(TT& x) const;
+};
+// Recorded calls are used unless XTT_NO_GRAPHS is set or the launch profiler is on (read per call).
+bool graphs_enabled();
+// Accepted replays since the process started, per kind (all threads).
+void graph_replays(size_t out[2]);
+void count_replay(int kind);
 
 // xerus datafiles (misc::save_to_file / load_from_file of a TTNetwork): format 0 = BINARY, 1 = TSV.
 void tt_save(const TT& x, const std::string& filename, int format);

@@ -47,6 +47,14 @@ __global__ void multi_copy_kernel(MultiCopy c) {
 	const size_t n = c.n[j];
 	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = in[i];
 }
+// The same for FP64 → FP32 conversions (dst entries are float*)
+__global__ void multi_convert_kernel(MultiCopy c) {
+	const unsigned j = blockIdx.y;
+	const double* __restrict__ in = c.src[j];
+	float* __restrict__ out = reinterpret_cast<float*>(c.dst[j]);
+	const size_t n = c.n[j];
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = (float)in[i];
+}
 __global__ void fill_kernel(double* p, double v, size_t n) {
 	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -341,6 +349,35 @@ void multi_copy_f64(const MultiCopy& c, hipStream_t s) {
 	for (int j = 0; j < c.count; ++j) most = std::max(most, c.n[j]);
 	if (!most) return;
 	hipLaunchKernelGGL(multi_copy_kernel, dim3(std::min<size_t>(grid_for(most), 64), c.count), dim3(256), 0, s, c);
+	XTT_LAUNCH_CHECK();
+}
+void multi_convert_f64_f32(const MultiCopy& c, hipStream_t s) {
+	if (c.count <= 0) return;
+	if (c.count > kMultiCopy) throw Error(ERR_INVALID, "multi_convert: too many conversions in one launch");
+	size_t most = 0;
+	for (int j = 0; j < c.count; ++j) most = std::max(most, c.n[j]);
+	if (!most) return;
+	hipLaunchKernelGGL(multi_convert_kernel, dim3(std::min<size_t>(grid_for(most), 64), c.count), dim3(256), 0, s, c);
+	XTT_LAUNCH_CHECK();
+}
+namespace {
+__global__ void zero_u32_kernel(unsigned* p, size_t n) {
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+}  // namespace
+namespace {
+__global__ void eye_kernel(double* p, size_t n) {
+	for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n * n; e += (size_t)gridDim.x * blockDim.x) p[e] = (e / n == e % n) ? 1.0 : 0.0;
+}
+}  // namespace
+void eye_f64(double* p, size_t n, hipStream_t s) {
+	if (!n) return;
+	hipLaunchKernelGGL(eye_kernel, dim3(grid_for(n * n)), dim3(256), 0, s, p, n);
+	XTT_LAUNCH_CHECK();
+}
+void zero_u32(unsigned* p, size_t n, hipStream_t s) {
+	if (!n) return;
+	hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_for(n)), dim3(256), 0, s, p, n);
 	XTT_LAUNCH_CHECK();
 }
 void fill_f64(double* p, double v, size_t n, hipStream_t s) {

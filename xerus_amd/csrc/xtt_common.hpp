@@ -114,6 +114,26 @@ size_t bytes_cached();  // free blocks held by the cache
 // `from` has finished all its work (it was synchronised); a stream about to be destroyed parks its pool this way.
 void park_stream_pool(hipStream_t from);
 
+// Capture arena: while an ArenaScope is active on a thread, dmalloc on that thread hands out fresh memory from the
+// arena (bump allocation, never reused) and dfree ignores it. Work recorded into a hipGraph under the scope thus
+// keeps every buffer it touched at a fixed address that nothing else gets, for as long as the arena lives.
+struct Arena {
+	std::vector<std::pair<char*, size_t>> chunks;  // hipMalloc'ed chunks (the last one being filled)
+	size_t used = 0;                               // bytes used in the last chunk
+	size_t total = 0;                              // bytes handed out
+	void* alloc(size_t bytes);
+	Arena() = default;
+	Arena(const Arena&) = delete;
+	Arena& operator=(const Arena&) = delete;
+	~Arena();  // synchronises the device, then frees the chunks
+};
+struct ArenaScope {
+	Arena* prev;
+	explicit ArenaScope(Arena* a);
+	~ArenaScope();
+};
+bool arena_active();  // an ArenaScope with an arena is active on this thread
+
 template <class T>
 struct DBuf {  // owning device buffer of T
 	T* p = nullptr;
