@@ -74,6 +74,20 @@ int xt_gemm_f32(int transA, int transB, size_t m, size_t n, size_t k, float alph
                 void* stream);
 int xt_gemm_bf16(int transA, int transB, size_t m, size_t n, size_t k, float alpha, const float* A, const float* B, float* C,
                  void* stream);
+/* Tile autotuner of the GEMMs above and of every contraction inside libxtt (no xerus counterpart: xerus hands its
+ * products to a BLAS). A choice per shape (m, n, k, batch, transA, transB; precision XT_F64 or XT_F32) is the tile
+ * edge (32 or 64) and, for 32-tiles of a single product, the number of split-K slices (0: none); a shape with a
+ * choice no longer uses the built-in heuristic. xt_gemm_tune times the heuristic and each candidate over `iters`
+ * launches with HIP events, installs the fastest and reports both (tile, slices, µs per launch; outputs nullable).
+ * xt_gemm_tune_set installs a choice (a saved table), xt_gemm_tune_clear drops all. Process-wide. */
+int xt_gemm_tune(int precision, size_t m, size_t n, size_t k, size_t batch, int transA, int transB, int iters, int* tile, int* slices,
+                 double* best_us, int* heuristic_tile, int* heuristic_slices, double* heuristic_us);
+int xt_gemm_tune_set(int precision, size_t m, size_t n, size_t k, size_t batch, int transA, int transB, int tile, int slices);
+int xt_gemm_tune_clear(void);
+/* installs the choices of a table written by xerus_amd.tune_gemm (xerus_amd/tables/gemm_gfx950.tsv: tab-separated
+ * precision m n k batch transA transB launches tile slices µs heuristic_tile heuristic_slices heuristic_µs, '#'
+ * comments); rows whose choice is the heuristic's are skipped. installed (nullable): the number installed. */
+int xt_gemm_tune_load(const char* path, size_t* installed);
 /* Q (m×min(m,n)), R (min×n): blasWrapper::qr, blasLapackWrapper.h:109-112 */
 int xt_qr(const double* A, size_t m, size_t n, double* Q, double* R);
 /* R (m×min), Q (min×n): blasWrapper::rq, blasLapackWrapper.h:118-121 */

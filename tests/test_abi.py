@@ -124,3 +124,23 @@ def test_cpp_faces_compile(tmp_path):
         f.write_text(b)
         r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-I", inc, str(f)], capture_output=True, text=True)
         assert r.returncode == 0, f"INTEGRATION.md C++ block {i}:\n{r.stderr}"
+
+
+def test_gemm_tile_table_loads(xe):
+    """The committed measured GEMM tile table parses, every choice is one the kernels support, a choice that differs
+    from the heuristic's was measured faster, and xt_gemm_tune_load installs exactly those rows (host only)."""
+    import ctypes
+    from xerus_amd import tuning
+    rows = tuning.read()
+    assert len(rows) > 100
+    changed = 0
+    for r in rows:
+        prec, m, n, k, batch, ta, tb, launches, tile, slices, us, htile, hslices, hus = r
+        assert prec in (0, 1) and min(m, n, k, batch, launches) > 0 and ta in (0, 1) and tb in (0, 1)
+        assert tile in (32, 64) and (slices == 0 or (tile == 32 and 2 <= slices <= 256))
+        if (tile, slices) != (htile, hslices):
+            changed += 1
+            assert us < hus
+    n_inst = ctypes.c_size_t()
+    xe._lib.call("xt_gemm_tune_load", tuning.DEFAULT_TABLE.encode(), ctypes.byref(n_inst))
+    assert n_inst.value == changed > 0

@@ -358,3 +358,36 @@ def test_solve_least_squares_entry(xe, m, n, p, rank):
     call("xt_solve_least_squares", ptr(X), ptr(dA), m, n, ptr(dB), p)
     ref = np.linalg.pinv(A) @ B
     assert np.abs(host(X) - ref).max() <= 1e-9 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("m,n,k,ta,tb", [(36, 108, 576, 1, 0), (1024, 64, 192, 0, 1), (17, 33, 130, 0, 0), (300, 130, 64, 1, 1)])
+def test_gemm_tuned_choices(xe, prec, m, n, k, ta, tb):
+    """Every choice the tile autotuner can install (64-tiles, 32-tiles with 0..32 split-K slices) computes the same
+    product (float64 numpy), and xt_gemm_tune installs a choice no slower than the heuristic's."""
+    rng = np.random.default_rng(m * 7 + k)
+    dt = np.float64 if prec == "f64" else np.float32
+    A = rng.standard_normal((k, m) if ta else (m, k))
+    B = rng.standard_normal((n, k) if tb else (k, n))
+    ref = (A.T if ta else A) @ (B.T if tb else B)
+    P = xe.XT_F64 if prec == "f64" else xe.XT_F32
+    fn = "xt_gemm" if prec == "f64" else "xt_gemm_f32"
+    alpha = 1.0 if prec == "f64" else ctypes.c_float(1.0)
+    dA, dB, dC = dev(A, dt), dev(B, dt), empty((m, n), dt)
+    tol = 1e-13 if prec == "f64" else 1e-5
+    try:
+        for tile, slices in [(64, 0), (32, 0), (32, 2), (32, 4), (32, 32)]:
+            xe._lib.call("xt_gemm_tune_set", P, m, n, k, 1, ta, tb, tile, slices)
+            xe._lib.call(fn, ta, tb, m, n, k, alpha, ctypes.c_void_p(ptr(dA)), ctypes.c_void_p(ptr(dB)), ctypes.c_void_p(ptr(dC)), None)
+            err = np.abs(host(dC) - ref).max() / np.abs(ref).max()
+            assert err < tol, (tile, slices, err)
+        t, s, ht, hs = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        us, hus = ctypes.c_double(), ctypes.c_double()
+        xe._lib.call("xt_gemm_tune", P, m, n, k, 1, ta, tb, 20, ctypes.byref(t), ctypes.byref(s), ctypes.byref(us), ctypes.byref(ht),
+                     ctypes.byref(hs), ctypes.byref(hus))
+        assert us.value <= hus.value and t.value in (32, 64)
+    finally:
+        # back to the committed table for the tests that follow
+        xe._lib.call("xt_gemm_tune_clear")
+        from xerus_amd import tuning
+        tuning.load()

@@ -62,6 +62,10 @@ SIGNATURES = {
     "xt_release_cache": (C.c_int, []),
     "xt_cache_bytes": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "xt_graph_replays": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "xt_gemm_tune": (C.c_int, [C.c_int, _sz, _sz, _sz, _sz, C.c_int, C.c_int, C.c_int, _pi, _pi, _pd, _pi, _pi, _pd]),
+    "xt_gemm_tune_set": (C.c_int, [C.c_int, _sz, _sz, _sz, _sz, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "xt_gemm_tune_clear": (C.c_int, []),
+    "xt_gemm_tune_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_size_t)]),
     "xt_malloc": (C.c_int, [_sz, C.POINTER(_vp)]),
     "xt_free": (C.c_int, [_vp]),
     "xt_memcpy_htod": (C.c_int, [_vp, _vp, _sz]),
@@ -205,6 +209,10 @@ def lib() -> C.CDLL:
             f.restype = res
             f.argtypes = args
         _lib = L
+        if not os.environ.get("XTT_NO_TUNING"):  # the measured GEMM tile table, when one is committed
+            from . import tuning
+            if os.path.exists(tuning.DEFAULT_TABLE):
+                tuning.load()
     return _lib
 
 

@@ -258,6 +258,13 @@ struct Workspace {
 		seen.clear();
 		calls.clear();
 	}
+	// The executables are destroyed here, while the runtime (and any tool intercepting it) is still up, instead of
+	// being left to the runtime's own teardown at process exit.
+	~Workspace() {
+		if (graphs.empty() && calls.empty()) return;
+		(void)hipDeviceSynchronize();
+		drop_graphs();
+	}
 	void ensure(size_t N, size_t t1, size_t t2, size_t nl, size_t nr, size_t nam) {
 		if (N <= capN && t1 <= capT1 && t2 <= capT2 && nl <= capL && nr <= capR && nam <= capAm && st.get()) return;
 		ArenaScope persistent(nullptr);  // the workspace outlives any recorded call

@@ -1,5 +1,7 @@
 // extern "C" surface of libxtt (declared in include/xtt.h). Exceptions become status codes here.
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <map>
 #include <mutex>
 #include <memory>
@@ -38,6 +40,16 @@ int guard(F&& f) {
 #define FMT(h) do { NN(h); ::xtt::require_correct_format((h)->t); } while (0)
 xtt::dims_t mk(const size_t* p, size_t n) { return xtt::dims_t(p, p + n); }
 void sync_stream(void* s) { XTT_HIP(hipStreamSynchronize(xtt::pick(s))); }
+}  // namespace
+
+namespace {
+xtt::GemmDesc tune_desc(int precision, size_t m, size_t n, size_t k, size_t batch, int ta, int tb) {
+	XTT_REQUIRE(precision == XT_F64 || precision == XT_F32, "GEMM tuning: precision XT_F64 or XT_F32");
+	XTT_REQUIRE(m && n && k && batch, "GEMM tuning: empty shape");
+	xtt::GemmDesc g = xtt::gemm_desc(ta ? 1 : 0, tb ? 1 : 0, m, n, k);
+	g.batch = batch;
+	return g;
+}
 }  // namespace
 
 extern "C" {
@@ -98,6 +110,46 @@ int xt_gemm(int ta, int tb, size_t m, size_t n, size_t k, double alpha, const do
 		if (k == 0) { XTT_HIP(hipMemsetAsync(C, 0, m * n * 8, xtt::pick(stream))); }
 		else xtt::gemm_f64(g, A, B, C, xtt::pick(stream));
 		sync_stream(stream);
+	});
+}
+int xt_gemm_tune(int precision, size_t m, size_t n, size_t k, size_t batch, int ta, int tb, int iters, int* tile, int* slices, double* bestUs,
+                 int* hTile, int* hSlices, double* hUs) {
+	return guard([&] {
+		const xtt::GemmTuneResult r = xtt::gemm_tune(precision == XT_F64 ? 0 : 1, tune_desc(precision, m, n, k, batch, ta, tb), iters, xtt::stream());
+		if (tile) *tile = r.best.tile;
+		if (slices) *slices = r.best.slices;
+		if (bestUs) *bestUs = r.bestUs;
+		if (hTile) *hTile = r.heuristic.tile;
+		if (hSlices) *hSlices = r.heuristic.slices;
+		if (hUs) *hUs = r.heuristicUs;
+	});
+}
+int xt_gemm_tune_set(int precision, size_t m, size_t n, size_t k, size_t batch, int ta, int tb, int tile, int slices) {
+	return guard([&] { xtt::gemm_tune_set(precision == XT_F64 ? 0 : 1, tune_desc(precision, m, n, k, batch, ta, tb), xtt::GemmChoice{tile, slices}); });
+}
+int xt_gemm_tune_clear(void) { return guard([&] { xtt::gemm_tune_clear(); }); }
+int xt_gemm_tune_load(const char* path, size_t* installed) {
+	return guard([&] {
+		NN(path);
+		std::ifstream in(path);
+		XTT_REQUIRE(in.good(), std::string("GEMM tuning: cannot read ") + path);
+		std::string line;
+		size_t count = 0;
+		while (std::getline(in, line)) {
+			if (line.empty() || line[0] == '#' || line.compare(0, 9, "precision") == 0) continue;
+			std::istringstream ls(line);
+			long long v[14];
+			double us = 0, hus = 0;
+			// precision m n k batch ta tb launches tile slices us heuristic_tile heuristic_slices heuristic_us
+			for (int i = 0; i < 10; ++i) ls >> v[i];
+			ls >> us >> v[11] >> v[12] >> hus;
+			XTT_REQUIRE(!ls.fail(), "GEMM tuning: malformed table line: " + line);
+			if (v[8] == v[11] && v[9] == v[12]) continue;  // the heuristic's own choice
+			xtt::gemm_tune_set(v[0] == XT_F64 ? 0 : 1, tune_desc((int)v[0], (size_t)v[1], (size_t)v[2], (size_t)v[3], (size_t)v[4], (int)v[5], (int)v[6]),
+			                   xtt::GemmChoice{(int)v[8], (int)v[9]});
+			++count;
+		}
+		if (installed) *installed = count;
 	});
 }
 int xt_gemm_f32(int ta, int tb, size_t m, size_t n, size_t k, float alpha, const float* A, const float* B, float* C, void* stream) {

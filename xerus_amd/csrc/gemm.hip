@@ -11,9 +11,12 @@
 //   x4 shapes : A[i=l%16][k=l/16], B[k=l/16][j=l%16]
 //   x32 bf16  : A[i=l%16][k=8(l/16)+e], B[k=8(l/16)+e][j=l%16], e = 0..7
 //   result    : FP32 / BF16: D[i=4(l/16)+r][j=l%16]; FP64: D[i=4r+l/16][j=l%16], r = 0..3 (register r)
+#include <map>
 #include <mutex>
 #include <set>
 #include <string>
+#include <tuple>
+#include <vector>
 
 #include "kernels.hpp"
 #include "xtt_common.hpp"
@@ -303,14 +306,47 @@ std::string shape_suffix(const GemmDesc& d) {
 	       " batch=" + std::to_string(d.batch) + " t=" + std::to_string(d.ta) + std::to_string(d.tb);
 }
 
-// Tile choice: 64×64 tiles once they still give >= 256 workgroups, else 32×32. The K chunk is the largest
-// multiple of 4 (x4 MFMA) whose two operand tiles fit the LDS budget.
+// ---- tile autotuner: measured choices per shape override the heuristic below --------------------------------
+struct TuneKey {
+	int dtype;  // 0 FP64, 1 FP32
+	size_t m, n, k, batch;
+	int ta, tb;
+	bool operator<(const TuneKey& o) const {
+		return std::tie(dtype, m, n, k, batch, ta, tb) < std::tie(o.dtype, o.m, o.n, o.k, o.batch, o.ta, o.tb);
+	}
+};
+std::mutex g_tuneMu;
+std::map<TuneKey, GemmChoice> g_tune;
+TuneKey tune_key(int dtype, const GemmDesc& d) { return TuneKey{dtype, d.m, d.n, d.k, d.batch, d.ta, d.tb}; }
+bool tuned(int dtype, const GemmDesc& d, GemmChoice& c) {
+	std::lock_guard<std::mutex> lk(g_tuneMu);
+	if (g_tune.empty()) return false;
+	auto it = g_tune.find(tune_key(dtype, d));
+	if (it == g_tune.end()) return false;
+	c = it->second;
+	return true;
+}
+
+// The heuristic: 64×64 tiles once they still give >= 256 workgroups, else 32×32; split-K over
+// max(2, 128 / tiles) slices of >= 64 when the 32×32 grid has fewer than 32 tiles and K >= 256 (batch 1).
+GemmChoice heuristic_choice(const GemmDesc& d) {
+	const size_t tiles64 = ((d.m + 63) / 64) * ((d.n + 63) / 64) * d.batch;
+	if (tiles64 >= 256) return GemmChoice{64, 0};
+	const size_t tiles32 = ((d.m + 31) / 32) * ((d.n + 31) / 32);
+	if (d.batch == 1 && tiles32 < 32 && d.k >= 256) {
+		const size_t want = std::max<size_t>(2, 128 / tiles32);
+		return GemmChoice{32, (int)std::min<size_t>(d.k / 64, want)};
+	}
+	return GemmChoice{32, 0};
+}
+
+// One launch (or a split-K pair of launches) with the given choice. The K chunk is the largest multiple of 4
+// (x4 MFMA) whose two operand tiles fit the LDS budget.
 template <class T>
-void launch_x4(const GemmDesc& d, const T* A, const T* B, T* C, hipStream_t s) {
+void launch_x4_with(const GemmDesc& d, const T* A, const T* B, T* C, hipStream_t s, GemmChoice ch) {
 	if (d.m == 0 || d.n == 0) return;
 	Args a = make_args(d, A, B, C);
-	const size_t tiles64 = ((d.m + 63) / 64) * ((d.n + 63) / 64) * d.batch;
-	const bool big = tiles64 >= 256;
+	const bool big = ch.tile == 64;
 	const size_t BMp = big ? 65 : 33;
 	size_t kmax = kLdsBudget / (2 * BMp * sizeof(T));
 	kmax &= ~size_t(3);
@@ -321,16 +357,15 @@ void launch_x4(const GemmDesc& d, const T* A, const T* B, T* C, hipStream_t s) {
 	std::string tag = big ? (sizeof(T) == 8 ? "gemm_x4_kernel<double,64,64>" : "gemm_x4_kernel<float,64,64>")
 	                      : (sizeof(T) == 8 ? "gemm_x4_kernel<double,32,32>" : "gemm_x4_kernel<float,32,32>");
 	if (prof_shapes()) tag += shape_suffix(d);
-	// Split-K for few output tiles over a long K (the stack contractions Xᵀ·T over a·n rows): K slices of >= 64
+	// Split-K for few output tiles over a long K (the stack contractions Xᵀ·T over a·n rows): K slices
 	// run as blockIdx.z, partials go to a stream-ordered scratch buffer and one epilogue launch sums them.
 	// While the stream is being captured only with a capture arena (its scratch then belongs to the graph); without
 	// one the scratch would be returned to the cache while the graph still uses it.
-	const size_t tiles32 = ((d.m + 31) / 32) * ((d.n + 31) / 32);
+	const bool split = !big && d.batch == 1 && ch.slices >= 2 && d.k >= 8;
 	hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-	if (!big && d.batch == 1 && tiles32 < 32 && d.k >= 256 && !arena_active()) XTT_HIP(hipStreamIsCapturing(s, &cap));
-	if (!big && d.batch == 1 && tiles32 < 32 && d.k >= 256 && cap == hipStreamCaptureStatusNone) {
-		const size_t want = std::max<size_t>(2, 128 / tiles32);
-		const size_t slices0 = std::min<size_t>(d.k / 64, want);
+	if (split && !arena_active()) XTT_HIP(hipStreamIsCapturing(s, &cap));
+	if (split && cap == hipStreamCaptureStatusNone) {
+		const size_t slices0 = std::min<size_t>((size_t)ch.slices, d.k / 4);
 		const size_t ks = (((d.k + slices0 - 1) / slices0) + 3) & ~size_t(3);
 		const size_t slices = (d.k + ks - 1) / ks;
 		DBuf<T> part(slices * d.m * d.n);
@@ -365,6 +400,13 @@ void launch_x4(const GemmDesc& d, const T* A, const T* B, T* C, hipStream_t s) {
 		dim3 grid((unsigned)((d.n + 31) / 32), (unsigned)((d.m + 31) / 32), (unsigned)d.batch);
 		launch_timed(tag.c_str(), work, kern, grid, dim3(256), lds, s, a);
 	}
+}
+
+template <class T>
+void launch_x4(const GemmDesc& d, const T* A, const T* B, T* C, hipStream_t s) {
+	GemmChoice ch;
+	if (!tuned(sizeof(T) == 8 ? 0 : 1, d, ch)) ch = heuristic_choice(d);
+	launch_x4_with<T>(d, A, B, C, s, ch);
 }
 
 // Two products with small tile grids (no split-K) as one launch of 32×32 tiles; anything else as two launches.
@@ -419,6 +461,80 @@ GemmDesc gemm_desc(int ta, int tb, size_t m, size_t n, size_t k) {
 }
 
 void gemm_f64(const GemmDesc& g, const double* A, const double* B, double* C, hipStream_t s) { launch_x4<double>(g, A, B, C, s); }
+
+void gemm_tune_set(int dtype, const GemmDesc& d, GemmChoice c) {
+	XTT_REQUIRE(dtype == 0 || dtype == 1, "gemm tuning: FP64 (0) or FP32 (1)");
+	XTT_REQUIRE(c.tile == 32 || c.tile == 64, "gemm tuning: tile 32 or 64");
+	XTT_REQUIRE(c.slices == 0 || (c.tile == 32 && c.slices >= 2 && c.slices <= 256), "gemm tuning: split-K needs 32-tiles and 2..256 slices");
+	std::lock_guard<std::mutex> lk(g_tuneMu);
+	g_tune[tune_key(dtype, d)] = c;
+}
+void gemm_tune_clear() {
+	std::lock_guard<std::mutex> lk(g_tuneMu);
+	g_tune.clear();
+}
+GemmChoice gemm_heuristic(const GemmDesc& d) { return heuristic_choice(d); }
+
+namespace {
+__global__ void tune_fill_kernel(double* p, size_t n) {  // deterministic operand values in [-1, 1)
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		p[i] = (double)((i * 2654435761u) % 1000) * 0.002 - 1.0;
+}
+template <class T>
+double time_choice(const GemmDesc& d, const T* A, const T* B, T* C, hipStream_t s, GemmChoice ch, int iters) {
+	for (int i = 0; i < 3; ++i) launch_x4_with<T>(d, A, B, C, s, ch);
+	hipEvent_t e0, e1;
+	XTT_HIP(hipEventCreate(&e0));
+	XTT_HIP(hipEventCreate(&e1));
+	XTT_HIP(hipEventRecord(e0, s));
+	for (int i = 0; i < iters; ++i) launch_x4_with<T>(d, A, B, C, s, ch);
+	XTT_HIP(hipEventRecord(e1, s));
+	XTT_HIP(hipEventSynchronize(e1));
+	float ms = 0;
+	XTT_HIP(hipEventElapsedTime(&ms, e0, e1));
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	return 1e3 * ms / iters;
+}
+template <class T>
+GemmTuneResult tune_t(const GemmDesc& d, int iters, hipStream_t s) {
+	const size_t na = d.batch * (size_t)d.m * d.k, nb = d.batch * (size_t)d.k * d.n, nc = d.batch * (size_t)d.m * d.n;
+	// FP64 scratch of the largest operand, reinterpreted for FP32 (the values only feed the timing)
+	DBuf<double> a(na), b(nb), c(nc);
+	for (DBuf<double>* p : {&a, &b, &c}) {
+		hipLaunchKernelGGL(tune_fill_kernel, dim3(256), dim3(256), 0, s, p->get(), p->size());
+		XTT_LAUNCH_CHECK();
+	}
+	const T* A = reinterpret_cast<const T*>(a.get());
+	const T* B = reinterpret_cast<const T*>(b.get());
+	T* C = reinterpret_cast<T*>(c.get());
+	std::vector<GemmChoice> cand;
+	if (((d.m + 63) / 64) * ((d.n + 63) / 64) * d.batch >= 4) cand.push_back({64, 0});
+	cand.push_back({32, 0});
+	if (d.batch == 1 && d.k >= 128)
+		for (int sl = 2; sl <= 64 && (size_t)sl * 32 <= d.k; sl *= 2) cand.push_back({32, sl});
+	GemmTuneResult r;
+	r.heuristic = heuristic_choice(d);
+	r.heuristicUs = time_choice<T>(d, A, B, C, s, r.heuristic, iters);
+	r.best = r.heuristic;
+	r.bestUs = r.heuristicUs;
+	for (const GemmChoice& ch : cand) {
+		if (ch.tile == r.heuristic.tile && ch.slices == r.heuristic.slices) continue;
+		const double us = time_choice<T>(d, A, B, C, s, ch, iters);
+		if (us < r.bestUs) { r.bestUs = us; r.best = ch; }
+	}
+	return r;
+}
+}  // namespace
+
+GemmTuneResult gemm_tune(int dtype, const GemmDesc& d, int iters, hipStream_t s) {
+	XTT_REQUIRE(dtype == 0 || dtype == 1, "gemm tuning: FP64 (0) or FP32 (1)");
+	XTT_REQUIRE(d.m > 0 && d.n > 0 && d.k > 0 && d.batch > 0, "gemm tuning: empty shape");
+	XTT_REQUIRE(iters >= 1 && iters <= 10000, "gemm tuning: 1..10000 timed launches");
+	const GemmTuneResult r = dtype == 0 ? tune_t<double>(d, iters, s) : tune_t<float>(d, iters, s);
+	gemm_tune_set(dtype, d, r.best);
+	return r;
+}
 void gemm_f32(const GemmDesc& g, const float* A, const float* B, float* C, hipStream_t s) { launch_x4<float>(g, A, B, C, s); }
 
 void gemm_bf16(const GemmDesc& d, const float* A, const float* B, float* C, hipStream_t s) {

@@ -29,6 +29,16 @@ void gemm_f64_pair(const GemmDesc& g0, const double* A0, const double* B0, doubl
 void gemm_f32(const GemmDesc& g, const float* A, const float* B, float* C, hipStream_t s);
 // BF16 products with FP32 accumulation; operands are stored as FP32 and rounded to bf16 when staged in LDS.
 void gemm_bf16(const GemmDesc& g, const float* A, const float* B, float* C, hipStream_t s);
+// Tile autotuner of gemm_f64 / gemm_f32 (dtype 0 / 1): a choice is the tile (32 or 64) and, for 32-tiles of a
+// single product, the number of split-K slices (0: none). A choice set for a shape overrides the built-in heuristic
+// for exactly that shape (m, n, k, batch, ta, tb). gemm_tune times the heuristic and every candidate over `iters`
+// launches on stream s (HIP events), installs the fastest and returns both.
+struct GemmChoice { int tile = 32, slices = 0; };
+struct GemmTuneResult { GemmChoice heuristic, best; double heuristicUs = 0, bestUs = 0; };
+GemmTuneResult gemm_tune(int dtype, const GemmDesc& d, int iters, hipStream_t s);
+void gemm_tune_set(int dtype, const GemmDesc& d, GemmChoice c);
+void gemm_tune_clear();
+GemmChoice gemm_heuristic(const GemmDesc& d);
 
 // out = reshuffle(in, shuffle): shuffle[i] = new position of old mode i (xerus::reshuffle semantics).
 void permute_f64(size_t degree, const size_t* dims, const size_t* shuffle, const double* in, double* out, hipStream_t s);
