@@ -218,8 +218,12 @@ def bench_batch(xe, np, count, sweeps, threads, precision):
                                  threads=threads)
     dt = time.perf_counter() - t0
     shapes = np.array([p[3] for p in probs])
+    res = np.asarray(res)
+    worst = [{"problem": int(i), "d": int(shapes[i, 0]), "n": int(shapes[i, 1]), "r": int(shapes[i, 2]), "residual": float(res[i])}
+             for i in np.argsort(res)[::-1][:5]]
     return {"problems": count, "sweeps_per_problem": sweeps, "threads": threads, "seconds": dt,
             "sweeps_per_s": count * sweeps / dt, "residual_max": float(np.max(res)), "residual_median": float(np.median(res)),
+            "residual_p90": float(np.percentile(res, 90)), "above_1e-10": int((res > 1e-10).sum()), "worst": worst,
             "d_mean": float(shapes[:, 0].mean()), "n_mean": float(shapes[:, 1].mean()), "r_mean": float(shapes[:, 2].mean()),
             "cg_iterations": st["cgIterations"], "local_solves": st["localSolves"]}
 

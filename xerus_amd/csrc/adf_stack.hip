@@ -161,12 +161,29 @@ __global__ __launch_bounds__(PT) void mask_scan_kernel(const uint32_t* mask, int
 	}
 }
 
-// rmin[k·P + run] = smallest sample id of the run; inv[perm[t]] = t
-__global__ void run_min_kernel(const uint32_t* run, const uint32_t* perm, int P, int d, uint32_t* rmin, uint32_t* inv) {
-	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < P; t += gridDim.x * blockDim.x) {
-		const uint32_t id = perm[t];
-		inv[id] = (uint32_t)t;
-		for (int k = 0; k < d; ++k) atomicMin(rmin + (size_t)k * P + run[(size_t)k * P + t], id);
+// rmin[k·P + run] = smallest sample id of the run; inv[perm[t]] = t. The runs of a level are contiguous in sorted
+// order, so a wave first takes the minimum over each of its segments (a segmented scan with shuffles) and only a
+// segment's last lane issues the atomic: a level with few long runs does not serialise on a few addresses.
+__global__ __launch_bounds__(256) void run_min_kernel(const uint32_t* run, const uint32_t* perm, int P, int d, uint32_t* rmin, uint32_t* inv) {
+	const int lane = threadIdx.x & 63;
+	const int stride = gridDim.x * blockDim.x;
+	for (int base = blockIdx.x * blockDim.x; base < P; base += stride) {  // every lane runs the same trip count
+		const int t = base + threadIdx.x;
+		const bool in = t < P;
+		const uint32_t id = in ? perm[t] : 0xFFFFFFFFu;
+		if (in) inv[id] = (uint32_t)t;
+		for (int k = 0; k < d; ++k) {
+			const uint32_t r = in ? run[(size_t)k * P + t] : 0xFFFFFFFFu;
+			uint32_t v = id;
+#pragma unroll
+			for (int o = 1; o < 64; o <<= 1) {
+				const uint32_t ov = __shfl_up(v, o, 64), orun = __shfl_up(r, o, 64);
+				if (lane >= o && orun == r) v = min(v, ov);
+			}
+			const uint32_t nextRun = __shfl_down(r, 1, 64);
+			const bool lastOfSegment = lane == 63 || nextRun != r;
+			if (in && lastOfSegment) atomicMin(rmin + (size_t)k * P + r, v);
+		}
 	}
 }
 
@@ -430,9 +447,11 @@ __global__ __launch_bounds__(SB) void stack_resgrad_kernel(const T* core, int rl
 	T* sg = reinterpret_cast<T*>(gsmem);                   // padded core slices (when they fit)
 	const int nj = j1 - j0;                                 // the slices accumulated by this launch
 	T* acc = sg + (p.fits ? n * p.SS : 0);                 // [Q][nj][E]
-	T* sF = acc + (size_t)Q * nj * E;                      // [RG][rl]
-	T* sB = sF + RG * rl;                                  // [RG][rr]
-	T* sR = sB + RG * rr;                                  // [RG]
+	// staging entry-major with an odd stride (RG + 1): the writes of one entry index go to consecutive addresses and
+	// the owners' reads of entries a (or c) of one sample fall in different banks
+	T* sF = acc + (size_t)Q * nj * E;                      // [rl][RG + 1]
+	T* sB = sF + (RG + 1) * rl;                            // [rr][RG + 1]
+	T* sR = sB + (RG + 1) * rr;                            // [RG]
 	int* sJ = reinterpret_cast<int*>(sR + RG);             // [RG]
 	__shared__ double ws[2 * SB / 64];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -460,9 +479,9 @@ __global__ __launch_bounds__(SB) void stack_resgrad_kernel(const T* core, int rl
 				r2 += (double)r * r;
 				b2 += (double)bi * bi;
 #pragma unroll
-				for (int x = 0; x < SR; ++x) if (x < rl) sF[tid * rl + x] = f[x];
+				for (int x = 0; x < SR; ++x) if (x < rl) sF[x * (RG + 1) + tid] = f[x];
 #pragma unroll
-				for (int x = 0; x < SR; ++x) if (x < rr) sB[tid * rr + x] = b[x];
+				for (int x = 0; x < SR; ++x) if (x < rr) sB[x * (RG + 1) + tid] = b[x];
 				sR[tid] = r;
 				sJ[tid] = (int)j - j0;
 			}
@@ -472,7 +491,7 @@ __global__ __launch_bounds__(SB) void stack_resgrad_kernel(const T* core, int rl
 			T* my = acc + (size_t)q * nj * E + e;
 			for (int s = q; s < cnt; s += Q) {
 				const int jj = sJ[s];
-				if (jj >= 0 && jj < nj) my[(size_t)jj * E] += sR[s] * sF[s * rl + a] * sB[s * rr + c];
+				if (jj >= 0 && jj < nj) my[(size_t)jj * E] += sR[s] * sF[a * (RG + 1) + s] * sB[c * (RG + 1) + s];
 			}
 		}
 	}
@@ -619,7 +638,7 @@ template <class T>
 size_t resgrad_lds(int rl, int n, int rr, int nj) {
 	const int E = rl * rr, Q = SB / E, RG = resgrad_round<T>();
 	const SlicePad p = slice_pad<T>(rl, n, rr);
-	return ((p.fits ? (size_t)n * p.SS : 0) + (size_t)Q * nj * E + (size_t)RG * (rl + rr + 1)) * sizeof(T) + RG * sizeof(int);
+	return ((p.fits ? (size_t)n * p.SS : 0) + (size_t)Q * nj * E + (size_t)(RG + 1) * (rl + rr) + RG) * sizeof(T) + RG * sizeof(int);
 }
 // the kernels with dynamic LDS may take all of it (once per process; the attribute holds for every device)
 void allow_stack_lds() {
