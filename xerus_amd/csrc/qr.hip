@@ -17,7 +17,10 @@
 // Column pivoting (xerus's QC, blasLapackWrapper.cpp:244-305 with dgeqp3) runs on one block: directly on
 // the matrix when it has at most 256 rows, else as the tree's root on the stacked R factors of the level
 // below (same column norms and the same partial-norm downdating, since the levels' Q are orthogonal). The pivot rule and downdating are dlaqp2's: largest
-// partial norm, first index on ties, recompute when the downdated norm falls below sqrt(eps/2). The rank rule
+// partial norm, first index on ties, recompute when the downdated norm falls below sqrt(eps/2). Columns never
+// move between lanes: each lane tracks the position dlaqp2's swaps would give its column (the pivot and the
+// column at position i exchange positions), ties go to the smallest position, and the outputs (reflectors, τ,
+// Y, R's rows, jpvt, C) are written by position. The rank rule
 // is xerus's: the first r with |R_rr| < 16·ε·R_00, where R_00 carries dlarfg's sign for the original first
 // row (blasLapackWrapper.cpp:266-272).
 #include <algorithm>
@@ -86,7 +89,7 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 	__shared__ double sW[2][HW][HN];   // per-wave Σ_{own r > i} a_ri·a_rj (lane i: the norm² partial)
 	__shared__ double sRow[2][HN];     // row i
 	__shared__ double sR[HN][HN];      // finished rows of R (lanes >= row)
-	__shared__ double sScal[HN], sTau[HN];
+	__shared__ double sScal[HN], sTau[HN], sDiag[HN];
 	__shared__ double sNrm[HW][HN];    // per-wave partial column norms (pivoting)
 	__shared__ int sRank;
 	const int lane = threadIdx.x & 63;
@@ -107,8 +110,8 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 	}
 
 	double vn1 = 0.0, vn2 = 0.0;  // partial column norms, identical in every wave
-	double myScal = 1.0;          // lane i: the scale of reflector i (u_i = scal_i·a below the diagonal)
-	int jp = lane;
+	double myScal = 1.0;          // the pivot lane of step i: the scale of reflector i (u_i = scal_i·a below the diagonal)
+	int pos = lane;               // the column's position (pivoting), identical in every wave
 	const double tol3z = sqrt(DBL_EPSILON * 0.5);
 	if (PIV) {
 		double s0 = 0.0, s1 = 0.0;
@@ -135,33 +138,38 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 		for (int i = C * HW; i < iend; ++i) {
 			const int p = i & 1;
 			const int wi = i - C * HW;  // wave owning row i
+			int pl = i;  // the lane holding step i's pivot column
 			if (PIV) {
-				// dlaqp2's pivot: the largest partial norm over [i, n), the first index on ties
-				const double cand = (lane >= i && lane < n) ? vn1 : -1.0;
+				// dlaqp2's pivot: the largest partial norm over positions [i, n), the first position on ties; the
+				// pivot lane takes position i and the lane at position i takes the pivot's
+				const double cand = (pos >= i && lane < n) ? vn1 : -1.0;
 				const double mx = wave_max(cand);
-				const unsigned long long hit = __ballot(cand == mx);
-				const int pv = hit ? __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(hit)) : i;  // all-NaN: no swap
-				if (pv != i) {
-					const int src = lane == i ? pv : (lane == pv ? i : lane);
-#pragma unroll
-					for (int t = C; t < RPT; ++t) x[t] = __shfl(x[t], src, 64);
-					vn1 = __shfl(vn1, src, 64);
-					vn2 = __shfl(vn2, src, 64);
-					jp = __shfl(jp, src, 64);
-					if (w == 0 && lane < i) {  // the finished rows of R swap columns too (wave 0 owns sR)
-						const double tmp = sR[lane][i];
-						sR[lane][i] = sR[lane][pv];
-						sR[lane][pv] = tmp;
-					}
+				const bool hit = cand == mx && cand >= 0.0;
+				const unsigned long long hits = __ballot(hit);
+				const int at_i = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(__ballot(pos == i && lane < n)));
+				int P;
+				if (hits == 0) {  // all-NaN: position i
+					P = i;
+					pl = at_i;
+				} else if ((hits & (hits - 1)) == 0) {  // a single maximum
+					pl = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(hits));
+					P = __builtin_amdgcn_readlane(pos, pl);
+				} else {  // ties: the smallest position
+					P = (int)(-wave_max(hit ? -(double)pos : -1e9));
+					pl = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(__ballot(pos == P && lane < n)));
 				}
+				if (lane == pl) pos = i;
+				else if (lane == at_i) pos = P;
 			}
-			// phase A: d_j = Σ_{own r > i} a_ri·a_rj (unscaled reflector; lane i gets the norm² partial), row i.
-			// Column i of the wave's rows is read from lane i into scalar registers (v_readlane).
+			const bool act = (PIV ? pos > i : lane > i) && lane < n;  // columns the reflector updates
+			const bool isPiv = lane == pl;
+			// phase A: d_j = Σ_{own r > i} a_ri·a_rj (unscaled reflector; the pivot lane gets the norm² partial), row i.
+			// The pivot column on the wave's rows is read from its lane into scalar registers (v_readlane).
 			{
-				const double vc = lane_value(x[C], i);
+				const double vc = lane_value(x[C], pl);
 				v[C] = w > wi ? vc : 0.0;
 #pragma unroll
-				for (int t = C + 1; t < RPT; ++t) v[t] = lane_value(x[t], i);
+				for (int t = C + 1; t < RPT; ++t) v[t] = lane_value(x[t], pl);
 			}
 			double d0 = 0.0, d1 = 0.0;
 #pragma unroll
@@ -180,8 +188,8 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 				sb += sW[p][q + 1][lane];
 			}
 			const double s = sa + sb;
-			const double xn2 = lane_value(s, i);  // lane i's sum is ‖a(i+1:, i)‖²
-			const double alpha = sRow[p][i];
+			const double xn2 = lane_value(s, pl);  // the pivot lane's sum is ‖a(i+1:, i)‖²
+			const double alpha = sRow[p][pl];
 			double beta = alpha, tau = 0.0, scal = 1.0;
 			if (xn2 != 0.0) {
 				const double a2 = fma(alpha, alpha, xn2);
@@ -206,18 +214,20 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 					scal = 1.0 / (alpha - beta);
 				}
 			}
-			// f_j = a_ij + scal·d_j. Lanes j > i: the update's w_j = tau·f_j. Lanes l < i hold reflector l
-			// (unscaled) on rows > l, so u_lᵀu_i = scal_l·(a_il + scal_i·d_l) = scal_l·f_l.
+			// f_j = a_ij + scal·d_j. Updated columns: the update's w_j = tau·f_j. The lanes of earlier steps hold
+			// reflector l = their position (unscaled) on rows > l, so u_lᵀu_i = scal_l·(a_il + scal_i·d_l) = scal_l·f_l.
 			const double f = fma(scal, s, sRow[p][lane]);
-			const double wj = (lane > i && lane < n) ? tau * f : 0.0;
-			const double rij = sRow[p][lane] - wj;  // row i after the update (lanes > i)
-			if (lane == i) myScal = scal;
-			if (a.Y && w == 0 && lane < i) a.Y[((size_t)b * n + lane) * n + i] = myScal * f;
+			const double wj = act ? tau * f : 0.0;
+			const double rij = sRow[p][lane] - wj;  // row i after the update (updated columns)
+			if (isPiv) myScal = scal;
+			const bool earlier = (PIV ? pos < i : lane < i) && lane < n;
+			if (a.Y && w == 0 && earlier) a.Y[((size_t)b * n + (PIV ? pos : lane)) * n + i] = myScal * f;
 			if (w == 0) {
-				sR[i][lane] = lane > i ? rij : (lane == i ? beta : 0.0);
+				sR[i][lane] = act ? rij : (isPiv ? beta : 0.0);  // by lane: the column's own entry of row i
 				if (lane == 0) {
 					sTau[i] = tau;
 					sScal[i] = scal;
+					sDiag[i] = beta;
 				}
 			}
 			if (tau != 0.0) {
@@ -228,7 +238,7 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 			if (PIV) {
 				// dlaqp2's partial-norm downdating with the updated row i
 				bool need = false;
-				if (lane > i && lane < n && vn1 != 0.0) {
+				if (act && vn1 != 0.0) {
 					double temp = fabs(rij) / vn1;
 					temp = fmax(0.0, 1.0 - temp * temp);
 					const double ratio = vn1 / vn2;
@@ -259,12 +269,13 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 	chunk(std::integral_constant<int, 7>{});
 	__syncthreads();
 
-	// compact output: R on and above the diagonal, the scaled reflectors u below
+	// compact output by position: R on and above the diagonal, the scaled reflectors u below
+	const int P = PIV ? pos : lane;
 #pragma unroll
 	for (int t = 0; t < RPT; ++t) {
 		const int r = t * HW + w;
-		const double u = lane < k && r > lane ? x[t] * sScal[lane] : 0.0;
-		if (a.V && r < rows && lane < n) a.V[(size_t)(r0 + r) * n + lane] = (r < k && lane >= r) ? sR[r][lane] : u;
+		const double u = P < k && r > P ? x[t] * sScal[P] : 0.0;
+		if (a.V && r < rows && lane < n) a.V[(size_t)(r0 + r) * n + P] = (r < k && P >= r) ? sR[r][lane] : u;
 	}
 	if (a.tau && threadIdx.x < k) a.tau[(size_t)b * n + threadIdx.x] = sTau[threadIdx.x];
 	if (!PIV) {
@@ -273,28 +284,28 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 				if (lane < n) a.R[((size_t)b * n + r) * n + lane] = sR[r][lane];
 		}
 	} else {
-		double r00 = sR[0][0];
+		double r00 = sDiag[0];
 		if (a.orig) {
 			// R_00 as LAPACK's single sweep over the original matrix gives it: beta of the pivot column p0
-			const int p0 = __builtin_amdgcn_readfirstlane(__shfl(jp, 0, 64));
+			const int p0 = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(__ballot(pos == 0 && lane < n)));
 			int nz = 0;
 			for (int r = 1 + threadIdx.x; r < a.origRows; r += HT) nz |= a.orig[(size_t)r * n + p0] != 0.0;
 			nz = __syncthreads_or(nz);
 			const double alpha0 = a.orig[p0];
-			r00 = nz ? -copysign(fabs(sR[0][0]), alpha0) : alpha0;
+			r00 = nz ? -copysign(fabs(sDiag[0]), alpha0) : alpha0;
 		}
 		if (threadIdx.x == 0) {
 			int rk;
 			for (rk = 1; rk <= k; ++rk)
-				if (rk == k || fabs(sR[rk][rk]) < 16 * DBL_EPSILON * r00) break;
+				if (rk == k || fabs(sDiag[rk]) < 16 * DBL_EPSILON * r00) break;
 			sRank = rk;
 			*a.rank = rk;
 		}
-		if (w == 0 && lane < n) a.jpvt[lane] = jp;
+		if (w == 0 && lane < n) a.jpvt[pos] = lane;
 		__syncthreads();
 		const int rank = sRank;
 		for (int r = w; r < rank; r += HW)
-			if (lane < n) a.C[(size_t)r * n + jp] = sR[r][lane];
+			if (lane < n) a.C[(size_t)r * n + lane] = sR[r][lane];  // C[row][original column]
 	}
 }
 
