@@ -260,7 +260,10 @@ def measured_traffic(tag):
             table = json.load(f)
     except OSError:
         return None, None
-    entry = table.get("kernels", {}).get(tag.split(" ")[0])
+    kernels = table.get("kernels", {})
+    name = tag.split(" ")[0]
+    # the launch profiler prints a bool template argument as 0/1, rocprofv3's kernel names as false/true
+    entry = kernels.get(name) or kernels.get(name.replace(",0>", ",false>").replace(",1>", ",true>"))
     if not entry:
         return None, None
     return entry["fetch_bytes"] + entry["write_bytes"], os.path.relpath(path, ROOT)

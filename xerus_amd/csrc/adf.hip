@@ -151,7 +151,7 @@ __global__ void pad_cores_kernel(AdfDesc a, const float* cores, long long* padOf
 	}
 }
 
-// v ← v·G (row vector times the 8×8 block), or v ← G·v
+// v ← v·G (row vector times the 8×8 block)
 __device__ inline void mul_row8(float (&v)[FR], const float* G) {
 	float g[FR];
 #pragma unroll
@@ -166,16 +166,45 @@ __device__ inline void mul_row8(float (&v)[FR], const float* G) {
 #pragma unroll
 	for (int c = 0; c < FR; ++c) v[c] = g[c];
 }
-__device__ inline void mul_col8(float (&v)[FR], const float* G) {
-	float g[FR];
+
+// v ← v·G and v ← G·v with two adjacent lanes per sample (h = lane & 1): lane h holds components 4h..4h+3 of v
+// before and after, takes the other half from its partner (DPP quad_perm [1,0,3,2]) and computes its half of the
+// result; v·G with the operations of mul_row8 in the same order, so its values equal mul_row8's. Every lane of the
+// wave must take part.
+__device__ inline float pair_swap(float x) {
+	return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+__device__ inline void full8(const float (&v)[4], int h, float (&f)[FR]) {
+	float o[4];
+#pragma unroll
+	for (int c = 0; c < 4; ++c) o[c] = pair_swap(v[c]);
+#pragma unroll
+	for (int c = 0; c < 4; ++c) { f[c] = h ? o[c] : v[c]; f[4 + c] = h ? v[c] : o[c]; }
+}
+__device__ inline void mul_row8_half(float (&v)[4], const float* G, int h) {
+	float f[FR];
+	full8(v, h, f);
+	float g[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
 	for (int al = 0; al < FR; ++al) {
-		const float4 lo = *reinterpret_cast<const float4*>(G + al * FR);
-		const float4 hi = *reinterpret_cast<const float4*>(G + al * FR + 4);
-		g[al] = lo.x * v[0] + lo.y * v[1] + lo.z * v[2] + lo.w * v[3] + hi.x * v[4] + hi.y * v[5] + hi.z * v[6] + hi.w * v[7];
+		const float4 x = *reinterpret_cast<const float4*>(G + al * FR + 4 * h);
+		g[0] = fmaf(f[al], x.x, g[0]); g[1] = fmaf(f[al], x.y, g[1]); g[2] = fmaf(f[al], x.z, g[2]); g[3] = fmaf(f[al], x.w, g[3]);
 	}
 #pragma unroll
-	for (int al = 0; al < FR; ++al) v[al] = g[al];
+	for (int c = 0; c < 4; ++c) v[c] = g[c];
+}
+__device__ inline void mul_col8_half(float (&v)[4], const float* G, int h) {
+	float f[FR];
+	full8(v, h, f);
+	float g[4];
+#pragma unroll
+	for (int al = 0; al < 4; ++al) {
+		const float4 lo = *reinterpret_cast<const float4*>(G + (4 * h + al) * FR);
+		const float4 hi = *reinterpret_cast<const float4*>(G + (4 * h + al) * FR + 4);
+		g[al] = lo.x * f[0] + lo.y * f[1] + lo.z * f[2] + lo.w * f[3] + hi.x * f[4] + hi.y * f[5] + hi.z * f[6] + hi.w * f[7];
+	}
+#pragma unroll
+	for (int al = 0; al < 4; ++al) v[al] = g[al];
 }
 
 template <class T>
@@ -250,27 +279,29 @@ __global__ __launch_bounds__(AB) void adf_residual_fast_kernel(AdfDesc a, const 
 	}
 }
 
-// Gradient: one block of FW = 16 waves takes rounds of FS = 128 samples. Phase 1: waves 0-1 walk the left
-// stacks F_k of the round's samples (one per lane), waves 2-3 the right stacks B_k, from the LDS copy of the
-// padded cores, into the block's staging area. Phase 2: wave w owns core k = w (orders up to 16: one core per wave,
-// so the latency-bound accumulations of different cores interleave on a SIMD) and adds the round's
-// contribution to its gradient with MFMA: G_k[a, j, c] += Σ_{s: i_k(s) = j} (res_s·F_{k-1}[s][a]) · B_{k+1}[s][c].
-// The wave first counting-sorts the round's samples by i_k (ballot / mbcnt per bin, each bin padded to a
-// multiple of 4), so every v_mfma_f32_16x16x4_f32 step takes 4 samples of one bin and updates only that bin's
-// 16-column tile (bins 2t and 2t+1 share tile t). The summation order is fixed (bins in order, samples in round
-// order), so the result is deterministic; the blocks' partials are summed in block order by the reduce kernel.
+// Gradient: one block of FW = 16 waves takes rounds of FS = 128 samples. Phase 1: waves 0-3 walk the left stacks
+// F_k of the round's samples, waves 4-7 the right stacks B_k, two lanes per sample (each lane half of the rank
+// components), from the LDS copy of the padded cores, into the block's staging area. Phase 2: wave w owns core k = w
+// (orders up to 16: one core per wave, so the latency-bound accumulations of different cores interleave on a SIMD)
+// and adds the round's contribution to its gradient with MFMA: G_k[a, j, c] += Σ_{s: i_k(s) = j} (res_s·F_{k-1}[s][a])
+// · B_{k+1}[s][c]. The wave first counting-sorts the round's samples by i_k (ballot / mbcnt per bin); bins 2t and
+// 2t+1 form tile t, padded to a multiple of 4 samples. A v_mfma_f32_16x16x4_f32 step takes 4 samples of the tile and
+// is block diagonal: rows (g, a) and columns (g, c) of bin parity g, a sample entering only its own bin's block. The
+// summation order is fixed (samples in sorted round order, blocks in block order by the reduce kernel), so the
+// result is deterministic.
 constexpr int FS = 128;
+constexpr int WW = 2 * FS / 32;  // walker waves: FS/32 forward, FS/32 backward (32 samples per wave)
 constexpr int FSP = FS + 1;  // staging row stride: lanes reading rows a = 0..7 of one sample hit different banks
 
 constexpr int GNT = 8;       // 16-column MFMA tiles per core (8·n_k <= 128, n_k <= 16)
 constexpr int GKC = FD / FW; // cores per wave
 
-constexpr int FPERM = FS + 3 * 16;  // per-wave sorted sample slots: a round plus the padding of up to 16 bins
+constexpr int FPERM = FS + 3 * 16;  // sorted sample slots: a round, the tiles' padding and the prefetch past the last tile
 
 __host__ __device__ inline size_t fast_grad_floats(int d, int sumN) {
 	return (size_t)sumN * PB + (size_t)FR * FSP                      // padded cores, e_0 rows
-	       + (size_t)2 * (d - 1) * FR * FSP + (size_t)d * FS + FS    // F, B, positions, residuals
-	       + (size_t)FW * FPERM;                                     // per-wave sorted order
+	       + (size_t)2 * (d - 1) * FR * FSP + (size_t)d * FS + FSP   // F, B, positions, residuals (and a zero)
+	       + (size_t)2 * FW * FPERM;                                 // per-wave sorted order, one copy per bin parity
 }
 
 typedef float gf4 __attribute__((ext_vector_type(4)));
@@ -304,11 +335,15 @@ __global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const
 	float* Fs = Es + (size_t)FR * FSP;                                  // F_k at [(k·FR + c)·FSP + s], k = 0..d-2
 	float* Bs = Fs + (size_t)(d - 1) * FR * FSP;                        // B_k at [((k-1)·FR + a)·FSP + s], k = 1..d-1
 	int* Is = reinterpret_cast<int*>(Bs + (size_t)(d - 1) * FR * FSP);  // i_k at [k·FS + s]
-	float* Rs = reinterpret_cast<float*>(Is + (size_t)d * FS);         // res at [s]
-	int* Ps = reinterpret_cast<int*>(Rs + FS) + w * FPERM;              // this wave's sorted sample slots (-1: padding)
+	float* Rs = reinterpret_cast<float*>(Is + (size_t)d * FS);         // res at [s]; Rs[FS] = 0
+	// this wave's sorted sample slots, one copy per bin parity g: byte offset 4·s of the slot's sample when its bin
+	// has parity g, else 4·FS, the zero column (Rs, the staging rows and Es are 0 there)
+	int* Ps = reinterpret_cast<int*>(Rs + FSP) + w * 2 * FPERM;
 	for (int e = tid; e < nslots * PB / 4; e += FW * 64)
 		reinterpret_cast<float4*>(Gs)[e] = reinterpret_cast<const float4*>(Gp)[e];
-	for (int e = tid; e < FR * FSP; e += FW * 64) Es[e] = e < FSP ? 1.f : 0.f;
+	for (int e = tid; e < FR * FSP; e += FW * 64) Es[e] = e < FS ? 1.f : 0.f;
+	for (int e = tid; e < 2 * (d - 1) * FR; e += FW * 64) Fs[(size_t)e * FSP + FS] = 0.f;  // F and B rows, column FS
+	if (tid == 0) Rs[FS] = 0.f;
 	const int li = lane & 15, lk = lane >> 4;  // MFMA: A row / B column li, K index lk
 	gf4 acc[GKC][GNT];
 #pragma unroll
@@ -321,9 +356,10 @@ __global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const
 	__syncthreads();
 	for (int base = s0; base < s1; base += FS) {
 		const int cnt = min(FS, s1 - base);
-		if (w < 4) {  // phase 1 walkers (wave-uniform)
-			const int s = tid & (FS - 1);
-			const bool fwd = w < 2;
+		if (w < WW) {  // phase 1 walkers (wave-uniform): two lanes per sample, lane half h holds components 4h..4h+3
+			const bool fwd = w < WW / 2;
+			const int s = (fwd ? w : w - WW / 2) * 32 + (lane >> 1);
+			const int h = lane & 1;
 			bool valid = s < cnt;
 			const uint32_t* p = pos + (size_t)(base + (valid ? s : 0)) * d;
 			uint32_t pk[FD];
@@ -331,42 +367,44 @@ __global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const
 			for (int k = 0; k < FD; ++k) pk[k] = p[k < d ? k : d - 1];
 #pragma unroll
 			for (int k = 0; k < FD; ++k) valid = valid && (k >= d || pk[k] < (uint32_t)sN[k]);
-			if (fwd) {
+			if (fwd && h == 0) {
 				for (int k = 0; k < d; ++k) Is[k * FS + s] = valid ? (int)pk[k] : 0;
 				if (!RES) Rs[s] = valid ? resid[base + s] : 0.f;
 			}
-			float v[FR];
+			float v[4];
 #pragma unroll
-			for (int c = 0; c < FR; ++c) v[c] = c == 0 ? 1.f : 0.f;
+			for (int c = 0; c < 4; ++c) v[c] = (h == 0 && c == 0) ? 1.f : 0.f;
 			if (fwd) {
 #pragma unroll
 				for (int k = 0; k + 1 < FD; ++k) {
 					if (k + 1 < d) {
-						mul_row8(v, Gs + sPo[k] + (size_t)(valid ? pk[k] : 0) * PB);
+						mul_row8_half(v, Gs + sPo[k] + (size_t)(valid ? pk[k] : 0) * PB, h);
 #pragma unroll
-						for (int c = 0; c < FR; ++c) Fs[(k * FR + c) * FSP + s] = v[c];
+						for (int c = 0; c < 4; ++c) Fs[(k * FR + 4 * h + c) * FSP + s] = v[c];
 					}
 				}
 				if (RES) {
-					mul_row8(v, Gs + sPo[d - 1] + (size_t)(valid ? pk[d - 1] : 0) * PB);
-					if (s < cnt) {
-						const float bi = values[base + s];
-						const float r = (valid ? v[0] : NAN) - bi;
-						residOut[base + s] = r;
-						rr += (double)r * r;
-						bb += (double)bi * bi;
-						Rs[s] = valid ? r : 0.f;
-					} else {
-						Rs[s] = 0.f;
+					mul_row8_half(v, Gs + sPo[d - 1] + (size_t)(valid ? pk[d - 1] : 0) * PB, h);
+					if (h == 0) {
+						if (s < cnt) {
+							const float bi = values[base + s];
+							const float r = (valid ? v[0] : NAN) - bi;
+							residOut[base + s] = r;
+							rr += (double)r * r;
+							bb += (double)bi * bi;
+							Rs[s] = valid ? r : 0.f;
+						} else {
+							Rs[s] = 0.f;
+						}
 					}
 				}
 			} else {
 #pragma unroll
 				for (int k = FD - 1; k >= 1; --k) {
 					if (k < d) {
-						mul_col8(v, Gs + sPo[k] + (size_t)(valid ? pk[k] : 0) * PB);
+						mul_col8_half(v, Gs + sPo[k] + (size_t)(valid ? pk[k] : 0) * PB, h);
 #pragma unroll
-						for (int al = 0; al < FR; ++al) Bs[((k - 1) * FR + al) * FSP + s] = v[al];
+						for (int al = 0; al < 4; ++al) Bs[((k - 1) * FR + 4 * h + al) * FSP + s] = v[al];
 					}
 				}
 			}
@@ -377,43 +415,62 @@ __global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const
 			const int k = w + q * FW;
 			if (k >= d || (kSel >= 0 && k != kSel)) continue;
 			const int nk = sN[k];
-			const float* fk = (k > 0 ? Fs + (k - 1) * FR * FSP : Es) + (li < FR ? li : 0) * FSP;
+			const float* fk = (k > 0 ? Fs + (k - 1) * FR * FSP : Es) + (li & 7) * FSP;
 			const float* bk = (k + 1 < d ? Bs + k * FR * FSP : Es) + (li & 7) * FSP;
 			const int* ik = Is + k * FS;
-			// counting sort of the round's samples by i_k: lane holds samples lane and lane + 64
+			// counting sort of the round's samples by i_k: lane holds samples lane and lane + 64; the bins 2t and 2t+1
+			// of tile t are adjacent and the tile is padded to a multiple of 4 slots
 			const int ja = lane < cnt ? ik[lane] : -1;
 			const int jb = lane + 64 < cnt ? ik[lane + 64] : -1;
-			for (int e = lane; e < FPERM; e += 64) Ps[e] = -1;
+			for (int e = lane; e < 2 * FPERM; e += 64) Ps[e] = 4 * FS;
 			__builtin_amdgcn_wave_barrier();
-			int bstart[16], bend[16];
+			int tstart[GNT], tend[GNT];
 			int st = 0;
 #pragma unroll
-			for (int bin = 0; bin < 16; ++bin) {
-				const unsigned long long ma = __ballot(ja == bin), mb = __ballot(jb == bin);
-				const int ca = __popcll(ma), cb = __popcll(mb);
-				const int below_a = __builtin_amdgcn_mbcnt_hi((unsigned)(ma >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ma, 0));
-				const int below_b = __builtin_amdgcn_mbcnt_hi((unsigned)(mb >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mb, 0));
-				if (ja == bin) Ps[st + below_a] = lane;
-				if (jb == bin) Ps[st + ca + below_b] = lane + 64;
-				bstart[bin] = st;
-				bend[bin] = st + ((ca + cb + 3) & ~3);
-				st = bend[bin];
+			for (int t = 0; t < GNT; ++t) {
+				tstart[t] = st;
+				if (2 * t < nk) {
+#pragma unroll
+					for (int hb = 0; hb < 2; ++hb) {
+						const int bin = 2 * t + hb;
+						const unsigned long long ma = __ballot(ja == bin), mb = __ballot(jb == bin);
+						const int ca = __popcll(ma), cb = __popcll(mb);
+						const int below_a = __builtin_amdgcn_mbcnt_hi((unsigned)(ma >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ma, 0));
+						const int below_b = __builtin_amdgcn_mbcnt_hi((unsigned)(mb >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mb, 0));
+						if (ja == bin) Ps[hb * FPERM + st + below_a] = 4 * lane;
+						if (jb == bin) Ps[hb * FPERM + st + ca + below_b] = 4 * (lane + 64);
+						st += ca + cb;
+					}
+					st = tstart[t] + ((st - tstart[t] + 3) & ~3);
+				}
+				tend[t] = st;
 			}
 			__builtin_amdgcn_wave_barrier();
+			// Block-diagonal MFMA tiles: A row i = (g, a) and B column j = (g, c) with g = i >> 3 (j >> 3) the bin parity.
+			// The lanes of parity g read the slot copy of parity g, where samples of the other bin (and padding) point at
+			// the zero column, so a sample enters the rows and columns of its own bin only: the blocks (0,0) and (1,1)
+			// accumulate bins 2t and 2t + 1, and a step takes 4 samples of either bin. Two steps per iteration; the next
+			// iteration's slots are loaded with this one's operands (reads past the tile stay inside the wave's slots).
+			const int* Pp = Ps + (li >> 3) * FPERM;
+			const char* rB = reinterpret_cast<const char*>(Rs);
+			const char* fB = reinterpret_cast<const char*>(fk);
+			const char* bB = reinterpret_cast<const char*>(bk);
 #pragma unroll
 			for (int t = 0; t < GNT; ++t) {
 				if (2 * t >= nk) continue;
-#pragma unroll
-				for (int h = 0; h < 2; ++h) {
-					const int bin = 2 * t + h;
-					for (int ks = bstart[bin]; ks < bend[bin]; ks += 4) {
-						const int sidx = Ps[ks + lk];
-						const bool ok = sidx >= 0;
-						const int sx = ok ? sidx : 0;
-						const float av = (li < FR && ok) ? Rs[sx] * fk[sx] : 0.f;
-						const float bv = (ok && (li >> 3) == h) ? bk[sx] : 0.f;
-						acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[q][t], 0, 0, 0);
-					}
+				const int te = tend[t];
+				int ks = tstart[t];
+				int n0 = Pp[ks + lk], n1 = Pp[ks + 4 + lk];
+				for (; ks < te; ks += 8) {
+					const int x0 = n0, x1 = n1;
+					n0 = Pp[ks + 8 + lk];
+					n1 = Pp[ks + 12 + lk];
+					const float r0 = *reinterpret_cast<const float*>(rB + x0), f0 = *reinterpret_cast<const float*>(fB + x0);
+					const float b0 = *reinterpret_cast<const float*>(bB + x0);
+					const float r1 = *reinterpret_cast<const float*>(rB + x1), f1 = *reinterpret_cast<const float*>(fB + x1);
+					const float b1 = *reinterpret_cast<const float*>(bB + x1);
+					acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(r0 * f0, b0, acc[q][t], 0, 0, 0);
+					if (ks + 4 < te) acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(r1 * f1, b1, acc[q][t], 0, 0, 0);
 				}
 			}
 		}
@@ -432,8 +489,8 @@ __global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const
 			normPart[2 * blockIdx.x + 1] = t1;
 		}
 	}
-	// the block's accumulators, written in the gradient layout: register r of tile t holds
-	// G_k[a = 4·lk + r, j = (16t + li)/8, c = li%8]
+	// the block's accumulators, written in the gradient layout: register r of tile t holds row i = 4·lk + r and column
+	// li of the block-diagonal tile, i.e. G_k[a = i & 7, j = 2t + (i >> 3), c = li & 7] when i >> 3 == li >> 3
 	float* dst = partials + (size_t)blockIdx.x * a.gradSize;
 #pragma unroll
 	for (int q = 0; q < GKC; ++q) {
@@ -442,11 +499,10 @@ __global__ __launch_bounds__(FW * 64) void adf_grad_fast_kernel(AdfDesc a, const
 		const int rl = sRk[k], rr = sRk[k + 1], n = sN[k];
 #pragma unroll
 		for (int t = 0; t < GNT; ++t) {
-			const int j = (16 * t + li) >> 3, c = li & 7;
 #pragma unroll
 			for (int r = 0; r < 4; ++r) {
-				const int al = 4 * lk + r;
-				if (j < n && al < rl && c < rr) dst[sOff[k] + ((size_t)al * n + j) * rr + c] = acc[q][t][r];
+				const int i = 4 * lk + r, al = i & 7, j = 2 * t + (i >> 3), c = li & 7;
+				if ((i >> 3) == (li >> 3) && j < n && al < rl && c < rr) dst[sOff[k] + ((size_t)al * n + j) * rr + c] = acc[q][t][r];
 			}
 		}
 	}
