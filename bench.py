@@ -204,8 +204,10 @@ def adf_cpu_baseline(np, iterations):
 
 def bench_batch(xe, np, count, sweeps, threads, precision):
     """Many independent ALS_SPD + round problems of uneven size (d ~ U{6..20}, n ∈ {2,4,8,16}, r ~ U{4..64},
-    rank-3 Laplacian + potential operators, rank-4 right-hand sides) through the batched driver: worker threads
-    with a stream each. Returns aggregate sweeps/s (count·sweeps / wall time) and residual statistics."""
+    rank-3 Laplacian + potential operators, rank-4 right-hand sides) through the batched driver: threads=0 is the
+    lockstep driver (one grouped launch per step for all problems, FP64 local solves), threads > 0 worker threads
+    with a stream each. Returns aggregate sweeps/s (count·sweeps / wall time, residuals included) and residual
+    statistics."""
     from xerus_amd.workloads import batch_problems
     probs = batch_problems(count)
     als = xe.ALS_SPD.copy()
@@ -221,9 +223,12 @@ def bench_batch(xe, np, count, sweeps, threads, precision):
     res = np.asarray(res)
     worst = [{"problem": int(i), "d": int(shapes[i, 0]), "n": int(shapes[i, 1]), "r": int(shapes[i, 2]), "residual": float(res[i])}
              for i in np.argsort(res)[::-1][:5]]
-    return {"problems": count, "sweeps_per_problem": sweeps, "threads": threads, "seconds": dt,
-            "sweeps_per_s": count * sweeps / dt, "residual_max": float(np.max(res)), "residual_median": float(np.median(res)),
-            "residual_p90": float(np.percentile(res, 90)), "above_1e-10": int((res > 1e-10).sum()), "worst": worst,
+    return {"problems": count, "sweeps_per_problem": sweeps,
+            "driver": "lockstep (grouped launches)" if threads == 0 else f"{threads} worker threads",
+            "precision": {xe.XT_F64: "f64", xe.XT_F32: "f32+f64-refinement", xe.XT_BF16: "bf16+f64-refinement"}[precision],
+            "seconds": dt, "sweeps_per_s": count * sweeps / dt, "residual_max": float(np.max(res)),
+            "residual_median": float(np.median(res)), "residual_p90": float(np.percentile(res, 90)),
+            "above_1e-10": int((res > 1e-10).sum()), "worst": worst,
             "d_mean": float(shapes[:, 0].mean()), "n_mean": float(shapes[:, 1].mean()), "r_mean": float(shapes[:, 2].mean()),
             "cg_iterations": st["cgIterations"], "local_solves": st["localSolves"]}
 
@@ -378,8 +383,13 @@ def main():
 
     batch = None
     if args.batch_problems > 0:
-        batch = bench_batch(xe, np, args.batch_problems, args.batch_sweeps, args.batch_threads, prec)
-        print(f"# batch: {batch['sweeps_per_s']:.1f} sweeps/s over {args.batch_problems} problems", file=sys.stderr, flush=True)
+        batch = bench_batch(xe, np, args.batch_problems, args.batch_sweeps, 0, xe.XT_F64)
+        print(f"# batch (lockstep, f64): {batch['sweeps_per_s']:.1f} sweeps/s over {args.batch_problems} problems", file=sys.stderr, flush=True)
+        if args.batch_threads > 0:
+            ref = bench_batch(xe, np, args.batch_problems, args.batch_sweeps, args.batch_threads, prec)
+            batch["threads_reference"] = {k: ref[k] for k in ("driver", "precision", "seconds", "sweeps_per_s", "residual_max",
+                                                              "residual_median", "cg_iterations")}
+            print(f"# batch ({ref['driver']}, {ref['precision']}): {ref['sweeps_per_s']:.1f} sweeps/s", file=sys.stderr, flush=True)
 
     cpu = None if args.no_cpu else cpu_baseline(np, args.steps)
     adf_cpu = None if (args.no_cpu or not adf_solve_r) else adf_cpu_baseline(np, 2)

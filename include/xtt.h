@@ -218,7 +218,10 @@ int xt_perf_sink(xt_perf_callback cb, void* user);
 /* Many independent problems (no single xerus call; a loop over ALSVariant::operator() and TTNetwork::round,
  * als.h:152-207, ttNetwork.cpp:643-684): problem p runs `sweeps` × (ALS with options o, then round(eps)) on one
  * of `threads` worker threads, each with a stream of its own. A[p] may be NULL (x ≈ b). residuals (optional, one
- * per problem) receives ‖A x − b‖/‖b‖; st (optional) the summed counters. */
+ * per problem) receives ‖A x − b‖/‖b‖; st (optional) the summed counters.
+ * threads <= 0 runs the problems in lockstep on the library stream: every step of every problem (stack pushes,
+ * local products, CG steps, QR / QC moves, SVDs) is one grouped launch for all problems. It applies to one-site
+ * ALS_SPD with an operator and FP64 local solves (XT_F64); other options fall back to 16 worker threads. */
 int xt_als_round_batch(const xt_als_options* o, size_t count, const xt_tt* A, const xt_tt* x, const xt_tt* b, size_t sweeps,
                        double eps, int threads, double* residuals, xt_als_stats* st);
 

@@ -1,0 +1,106 @@
+"""The lockstep batch driver (als_round_batch with threads=0): every step of every problem is one grouped launch.
+
+It runs the same per-problem algorithm as the threaded driver (one-site ALS_SPD with FP64 local CG, then round(eps)),
+so both drivers and the oracle are compared problem by problem on the uneven batch of configuration 4 (d ~ U{6..20},
+n ∈ {2,4,8,16}, r ~ U{4..64}, rank-3 Laplacian + potential, rank-4 right-hand sides).
+"""
+import numpy as np
+import pytest
+
+from helpers import SEED, laplace_cores
+from test_gpu_reference_tests import round_parity
+
+pytestmark = pytest.mark.gpu
+
+
+def run_batch(xe, probs, threads):
+    als = xe.ALS_SPD.copy()
+    als.convergenceEpsilon = 0.0
+    als.numHalfSweeps = 2
+    als.precision = xe.XT_F64
+    res, st = xe.als_round_batch(als, [p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs], 2, 1e-12,
+                                 threads=threads)
+    return np.asarray(res), st
+
+
+def test_lockstep_small_matches_threads_bitwise(xe):
+    """Small problems (d 5..8, n 2 or 4, ranks 3..9): the lockstep driver issues the same kernels on the same shapes as
+    the threaded one (the grouped GEMM, Householder, Jacobi and CG kernels share the single-problem tile code), so the
+    results agree to rounding: TTs within 1e-14, equal ranks, equal CG iteration counts."""
+    print("lockstep small", flush=True)
+    from xerus_amd.workloads import batch_problems
+    kw = dict(d_range=(5, 8), n_choices=(2, 4), r_range=(3, 9))
+    pa, pb = batch_problems(8, **kw), batch_problems(8, **kw)
+    ra, sa = run_batch(xe, pa, 0)
+    rb, sb = run_batch(xe, pb, 4)
+    assert sa["cgIterations"] == sb["cgIterations"] and sa["localSolves"] == sb["localSolves"]
+    for (A, x, b, shape), (_, y, _, _), r1, r2 in zip(pa, pb, ra, rb):
+        assert x.ranks() == y.ranks(), shape
+        assert (x - y).frob_norm() <= 1e-14 * y.frob_norm() + 1e-300, shape
+        assert abs(r1 - r2) <= 1e-12 * r2 + 1e-15, shape
+
+
+def test_lockstep_matches_threads_full_shapes(xe, oracle):
+    """24 problems of configuration 4 at full size through both drivers. The CG reductions differ in summation order
+    (one workgroup per system against the chunked kernels), so trajectories agree to the local tolerance: TTs within
+    1e-8 + 10·ρ (ρ the residual, which bounds how well a rank-r trajectory is defined), residuals within 10% of ρ.
+    The rank lists of the two trajectories may differ where a singular value sits within the solvers' tolerance of the
+    1e-12·s_0 cut; round of the same TT gives bit-exact ranks against the oracle (checked on every result)."""
+    print("lockstep vs threads", flush=True)
+    from xerus_amd.workloads import batch_problems
+    pa, pb = batch_problems(24), batch_problems(24)
+    ra, sa = run_batch(xe, pa, 0)
+    rb, sb = run_batch(xe, pb, 8)
+    assert sa["localSolves"] == sb["localSolves"]
+    for p, ((A, x, b, shape), (_, y, _, _), r1, r2) in enumerate(zip(pa, pb, ra, rb)):
+        diff = (x - y).frob_norm() / y.frob_norm()
+        print(f"  problem {p} {shape}: residual {r1:.3e} (threads {r2:.3e}), rel diff {diff:.2e}", flush=True)
+        assert diff < 1e-8 + 10 * r2, p
+        assert abs(r1 - r2) <= 0.1 * r2 + 1e-11, p
+        round_parity(xe, oracle, x, 1e-12)
+
+
+def test_lockstep_config4_subset_matches_oracle(xe, oracle):
+    """The first 16 problems of configuration 4 through the lockstep driver, each against the oracle's FP64 ALS_SPD
+    (matrix-free CG to the same local tolerance) + round(1e-12) for 2 sweeps from the same start: TTs within
+    1e-8 + 10·ρ and residuals within 10% of ρ (ρ the oracle's residual)."""
+    print("lockstep config4 subset", flush=True)
+    from xerus_amd.workloads import batch_problems
+    count = 16
+    probs = batch_problems(count)
+    res, _ = run_batch(xe, probs, 0)
+    for p, (A, x, b, (d, n, r)) in enumerate(probs):
+        rng = np.random.default_rng(SEED + p)
+        assert (int(rng.integers(6, 21)), int(rng.choice((2, 4, 8, 16))), int(rng.integers(4, 65))) == (d, n, r)
+        pots = rng.random((d, n))
+        Ao = oracle.TT.from_cores(laplace_cores(d, n, list(pots), rank3=True), op=True)
+        oracle.seed(SEED + p)
+        bo = oracle.TT.random([n] * d, [4] * (d - 1))
+        xo = oracle.TT.random([n] * d, [r] * (d - 1))
+        for _ in range(2):
+            oracle.als(Ao, xo, bo, num_half_sweeps=2, eps=0.0, local_solver="cg", cg_tol=1e-13)
+            xo.round(eps=1e-12)
+        ro = oracle.residual(Ao, xo, bo)
+        xot = xe.TTTensor.from_components(xo.cores())
+        diff = (x - xot).frob_norm() / xot.frob_norm()
+        print(f"  problem {p} d={d} n={n} r={r}: residual {res[p]:.3e} (oracle {ro:.3e}), rel diff {diff:.2e}", flush=True)
+        assert diff < 1e-8 + 10 * ro, p
+        assert abs(res[p] - ro) <= 0.1 * ro + 1e-11, p
+
+
+def test_lockstep_falls_back_for_other_options(xe):
+    """Options outside the lockstep path (here FP32 local matvecs) run on worker threads with the same results as an
+    explicit thread count."""
+    print("lockstep fallback", flush=True)
+    from xerus_amd.workloads import batch_problems
+    kw = dict(d_range=(5, 7), n_choices=(2, 4), r_range=(3, 6))
+    pa, pb = batch_problems(4, **kw), batch_problems(4, **kw)
+    als = xe.ALS_SPD.copy()
+    als.convergenceEpsilon = 0.0
+    als.numHalfSweeps = 2
+    als.precision = xe.XT_F32
+    ra, _ = xe.als_round_batch(als, [p[0] for p in pa], [p[1] for p in pa], [p[2] for p in pa], 1, 1e-12, threads=0)
+    rb, _ = xe.als_round_batch(als, [p[0] for p in pb], [p[1] for p in pb], [p[2] for p in pb], 1, 1e-12, threads=2)
+    for (A, x, b, shape), (_, y, _, _), r1, r2 in zip(pa, pb, ra, rb):
+        assert x.ranks() == y.ranks(), shape
+        assert (x - y).frob_norm() <= 1e-10 * y.frob_norm(), shape

@@ -1123,6 +1123,13 @@ void als_round_batch(const ALSOptions& o, const std::vector<const TT*>& A, const
                      ALSStats* total) {
 	const size_t count = x.size();
 	XTT_REQUIRE(A.size() == count && b.size() == count, "batch: A, x and b lists differ in length");
+	if (threads <= 0) {  // lockstep: one grouped launch per step for all problems, where the options allow it
+		if (lockstep_supported(o, A)) {
+			als_round_lockstep(o, A, x, b, sweeps, eps, residuals, total);
+			return;
+		}
+		threads = 16;
+	}
 	XTT_REQUIRE(threads >= 1 && threads <= 64, "batch: 1..64 worker threads");
 	int dev = 0;
 	XTT_HIP(hipGetDevice(&dev));

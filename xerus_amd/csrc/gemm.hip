@@ -172,6 +172,21 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_x4_pair_kernel(Args g0, Arg
 	gemm_x4_tile<T, BM, BN, WM, WN>(p ? g1 : g0, bx, t % gy, t / gy);
 }
 
+// Many independent products in one launch (the lockstep batch driver): task t owns workgroups
+// [start[t], start[t+1]); its (gx × gy × batch) grid is flattened with x fastest.
+struct GroupArgs {
+	Args a;
+	int gx, gy;
+};
+template <class T>
+__global__ __launch_bounds__(256) void gemm_group_kernel(const GroupArgs* tasks, const int* start, int count) {
+	const int t = group_task(start, count, (int)blockIdx.x);
+	const GroupArgs g = tasks[t];
+	const int l = (int)blockIdx.x - start[t];
+	const int bx = l % g.gx, r = l / g.gx;
+	gemm_x4_tile<T, 32, 32, 2, 2>(g.a, bx, r % g.gy, r / g.gy);
+}
+
 // Split-K epilogue: C = alpha·Σ_s P_s + beta·C, slices summed in index order (deterministic).
 template <class T>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(const T* P, int slices, int m, int n, T* C, int ldc, double alpha,
@@ -445,7 +460,69 @@ void launch_x4_pair(const GemmDesc& d0, const T* A0, const T* B0, T* C0, const G
 	launch_timed(tag.c_str(), Work(flops, bytes), kern, dim3((unsigned)(blocks[0] + blocks[1])), dim3(256), lds, s, a[0], a[1], pg);
 }
 
+template <class T>
+void launch_group(const std::vector<GemmTask>& tasks, hipStream_t s, GemmGroupPlan* keep = nullptr) {
+	constexpr size_t BMp = 33;
+	size_t kmax = kLdsBudget / (2 * BMp * sizeof(T));
+	kmax &= ~size_t(3);
+	std::vector<GroupArgs> ga;
+	std::vector<int> start;
+	ga.reserve(tasks.size());
+	start.reserve(tasks.size());
+	int blocks = 0;
+	size_t lds = 0;
+	double flops = 0, bytes = 0;
+	for (const GemmTask& t : tasks) {
+		const GemmDesc& d = t.d;
+		if (d.m == 0 || d.n == 0 || d.batch == 0) continue;
+		GroupArgs g;
+		g.a = make_args(d, t.A, t.B, t.C);
+		g.a.kc = (int)std::max<size_t>(4, std::min<size_t>((d.k + 3) & ~size_t(3), kmax));
+		g.gx = (int)((d.n + 31) / 32);
+		g.gy = (int)((d.m + 31) / 32);
+		lds = std::max(lds, (size_t)g.a.kc * BMp * 2 * sizeof(T));
+		start.push_back(blocks);
+		blocks += g.gx * g.gy * (int)d.batch;
+		ga.push_back(g);
+		flops += 2.0 * d.m * d.n * d.k * d.batch;
+		bytes += (double)sizeof(T) * d.batch * ((double)d.m * d.k + (double)d.k * d.n + (d.beta != 0.0 ? 2.0 : 1.0) * d.m * d.n);
+	}
+	if (ga.empty()) {
+		if (keep) keep->count = 0;
+		return;
+	}
+	const GroupArgs* dt;
+	const int* ds;
+	if (keep) {  // tables in a buffer of the plan: replayed by gemm_group_run_f64 until the plan goes
+		const size_t tb = ga.size() * sizeof(GroupArgs), sb = start.size() * sizeof(int);
+		keep->buf.resize((tb + 255) / 256 * 256 + sb);
+		char* base = keep->buf.get();
+		table_upload_to(base, ga.data(), tb, s);
+		table_upload_to(base + (tb + 255) / 256 * 256, start.data(), sb, s);
+		dt = reinterpret_cast<const GroupArgs*>(base);
+		ds = reinterpret_cast<const int*>(base + (tb + 255) / 256 * 256);
+		keep->tasks = dt; keep->start = ds; keep->count = (int)ga.size(); keep->blocks = blocks; keep->lds = lds;
+		keep->flops = flops; keep->bytes = bytes;
+	} else {
+		dt = static_cast<const GroupArgs*>(table_upload(ga.data(), ga.size() * sizeof(GroupArgs), s));
+		ds = static_cast<const int*>(table_upload(start.data(), start.size() * sizeof(int), s));
+	}
+	auto kern = gemm_group_kernel<T>;
+	allow_lds(kern, lds);
+	launch_timed(sizeof(T) == 8 ? "gemm_group_kernel<double>" : "gemm_group_kernel<float>", Work(flops, bytes), kern, dim3((unsigned)blocks),
+	             dim3(256), lds, s, dt, ds, (int)ga.size());
+}
+
 }  // namespace
+
+void gemm_group_f64(const std::vector<GemmTask>& tasks, hipStream_t s) { launch_group<double>(tasks, s); }
+void gemm_group_plan_f64(const std::vector<GemmTask>& tasks, GemmGroupPlan& plan, hipStream_t s) { launch_group<double>(tasks, s, &plan); }
+void gemm_group_run_f64(const GemmGroupPlan& p, hipStream_t s) {
+	if (p.count == 0) return;
+	auto kern = gemm_group_kernel<double>;
+	launch_timed("gemm_group_kernel<double>", Work(p.flops, p.bytes), kern, dim3((unsigned)p.blocks), dim3(256), p.lds, s,
+	             static_cast<const GroupArgs*>(p.tasks), p.start, p.count);
+}
 
 void gemm_f64_pair(const GemmDesc& g0, const double* A0, const double* B0, double* C0, const GemmDesc& g1, const double* A1,
                    const double* B1, double* C1, hipStream_t s) {

@@ -164,6 +164,82 @@ void hh_qc(const double* A, size_t m, size_t n, double* Q, double* C, int* jpvt,
 void jacobi_svd(const double* A, size_t m, size_t n, double* U, double* S, double* Vt, double* ws, hipStream_t s,
                 bool transposedOut = false);
 
+// ---- grouped launches (lockstep batch driver): one launch, many independent problems ----------------------
+// Workgroup b belongs to the task t with start[t] <= b < start[t + 1] (start ascending, start[0] = 0).
+__device__ inline int group_task(const int* start, int count, int b) {
+	int lo = 0, hi = count - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (start[mid] <= b) lo = mid; else hi = mid - 1;
+	}
+	return lo;
+}
+// Copies a task table to device memory on stream s (through a per-thread pinned ring) and returns its device
+// address, valid for the launches that follow on s. table_ring_release(s) after s was synchronised frees the ring.
+const void* table_upload(const void* src, size_t bytes, hipStream_t s);
+void table_ring_release(hipStream_t s);
+// Copies a table to dst (device memory the caller owns) through the pinned ring, on stream s.
+void table_upload_to(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+enum MapOp { MAP_PERMUTE = 0, MAP_REVTRANS = 1, MAP_SCALE_BY = 2, MAP_EYE = 3, MAP_FILL = 4 };
+struct MapTask {
+	int op, deg;
+	long long n;
+	long long dims[4], outStride[4];
+	const double* in;
+	double* out;
+	const double* aux;
+	double alpha;
+};
+MapTask map_permute(size_t degree, const size_t* dims, const size_t* shuffle, const double* in, double* out);  // as permute_f64
+MapTask map_copy(const double* in, double* out, size_t n);
+MapTask map_revtrans(const double* in, size_t rows, size_t cols, double* out);                              // as revtrans_f64
+MapTask map_scale_by(double* m, const double* sv, size_t rows, size_t cols, bool byRow);                      // as scale_by_f64
+MapTask map_eye(double* out, size_t n);
+MapTask map_fill(double* out, double v, size_t n);
+void map_group(const std::vector<MapTask>& tasks, hipStream_t s);
+
+// FP64 CG of many local systems, one workgroup each; q = A·p (or A·x before the start) comes from grouped GEMMs.
+struct CGTask {
+	int n;
+	double* x;
+	double* r;
+	double* p;
+	const double* q;
+	const double* b;
+	CGState* st;  // tol2 and maxIter set beforehand (cg_state_init semantics)
+};
+void cg_group_start(const std::vector<CGTask>& tasks, hipStream_t s);  // as cg_start followed by the zero-start rule
+void cg_group_step(const std::vector<CGTask>& tasks, hipStream_t s);   // cg_alpha, cg_update and cg_direction
+void cg_group_step_run(const CGTask* devTasks, int count, hipStream_t s);  // the same on a table already on the device
+struct Dot2Task { int n; const double* a; const double* b; const double* c; const double* d; };
+void dot2_group(const std::vector<Dot2Task>& tasks, double* out, hipStream_t s);  // out[2t] = a·b, out[2t+1] = c·d
+
+// C = alpha·op(A)·op(B) + beta·C for many products in one launch (32×32 tiles, no split-K).
+struct GemmTask { GemmDesc d; const void* A; const void* B; void* C; };
+void gemm_group_f64(const std::vector<GemmTask>& tasks, hipStream_t s);
+// A grouped GEMM launch whose tables stay on the device (in buf) and that runs again with gemm_group_run_f64 (the
+// launch of a CG step's products, repeated every iteration). plan_f64 also runs it once.
+struct GemmGroupPlan {
+	DBuf<char> buf;
+	const void* tasks = nullptr;
+	const int* start = nullptr;
+	int count = 0, blocks = 0;
+	size_t lds = 0;
+	double flops = 0, bytes = 0;
+};
+void gemm_group_plan_f64(const std::vector<GemmTask>& tasks, GemmGroupPlan& plan, hipStream_t s);
+void gemm_group_run_f64(const GemmGroupPlan& plan, hipStream_t s);
+// Householder QR of many matrices (each as hh_qr): Q (m × k) and R (k × n), k = min(m, n), n <= 64.
+struct QRJob { const double* A; int m, n; bool trans; double* Q; double* R; };
+void hh_qr_group(const std::vector<QRJob>& jobs, hipStream_t s);
+// xerus's QC of many matrices (each as hh_qc): rank, jpvt, Q (m × rank, ld rank) and C (rank × n) on the device.
+struct QCJob { const double* A; int m, n; double* Q; double* C; int* jpvt; int* rank; };
+void hh_qc_group(const std::vector<QCJob>& jobs, hipStream_t s);
+// One-sided Jacobi SVD of many matrices (each as jacobi_svd, m >= n, workspace ws as there).
+struct SVDJob { const double* A; int m, n; double* U; double* S; double* Vt; double* ws; bool transposedOut; };
+void jacobi_svd_group(const std::vector<SVDJob>& jobs, hipStream_t s);
+
 // ---- ADF sampled-entry kernels (FP32 cores / values, FP64 norms) ---------------------------------------
 struct AdfDesc {
 	int d;                    // order

@@ -332,7 +332,7 @@ constexpr int JMAXM = 128, JMAXN = 64;
 // One-sided Jacobi for m <= 128, n <= 64 with W (m×np, column-major) and V (np×np) in LDS. Same round-robin
 // ordering, rotation formula and stopping rule as jacobi_kernel; the pair sums are DPP butterflies.
 // tout: U and Vt are written transposed (U as n×m, Vt as n×n holding V).
-__global__ __launch_bounds__(JT) void jacobi_lds_kernel(const double* A, int m, int n, int np, double* U, double* S, double* Vt, int tout) {
+__device__ __forceinline__ void jacobi_lds_body(const double* A, int m, int n, int np, double* U, double* S, double* Vt, int tout) {
 	extern __shared__ double sm[];
 	double* W = sm;                       // m × np, column stride m
 	double* V = sm + (size_t)m * np;      // np × np
@@ -415,6 +415,15 @@ __global__ __launch_bounds__(JT) void jacobi_lds_kernel(const double* A, int m, 
 	}
 	for (int j = tid; j < np; j += JT) if (ord[j] < n) S[ord[j]] = unconverged ? __builtin_nan("") : sv[j];
 }
+__global__ __launch_bounds__(JT) void jacobi_lds_kernel(const double* A, int m, int n, int np, double* U, double* S, double* Vt, int tout) {
+	jacobi_lds_body(A, m, n, np, U, S, Vt, tout);
+}
+struct JacobiTask { const double* A; int m, n, np; double* U; double* S; double* Vt; int tout; };
+// one matrix per workgroup
+__global__ __launch_bounds__(JT) void jacobi_lds_group_kernel(const JacobiTask* tasks) {
+	const JacobiTask t = tasks[blockIdx.x];
+	jacobi_lds_body(t.A, t.m, t.n, t.np, t.U, t.S, t.Vt, t.tout);
+}
 
 }  // namespace
 
@@ -470,6 +479,30 @@ void jacobi_svd(const double* A, size_t m, size_t n, double* U, double* S, doubl
 	int* ord = reinterpret_cast<int*>(sv + np);
 	hipLaunchKernelGGL(jacobi_kernel, dim3(1), dim3(QT), 0, s, A, (int)m, (int)n, (int)np, W, V, sv, ord, U, S, Vt, (int)transposedOut);
 	XTT_LAUNCH_CHECK();
+}
+
+
+void jacobi_svd_group(const std::vector<SVDJob>& jobs, hipStream_t s) {
+	std::vector<JacobiTask> tasks;
+	size_t lds = 0;
+	for (const SVDJob& j : jobs) {
+		XTT_REQUIRE(j.m >= j.n && j.n > 0, "jacobi_svd expects m >= n > 0");
+		const int np = j.n + (j.n & 1);
+		if (j.m <= JMAXM && np <= JMAXN) {
+			tasks.push_back(JacobiTask{j.A, j.m, j.n, np, j.U, j.S, j.Vt, j.transposedOut ? 1 : 0});
+			lds = std::max(lds, ((size_t)j.m * np + (size_t)np * np) * sizeof(double));
+		} else {
+			jacobi_svd(j.A, (size_t)j.m, (size_t)j.n, j.U, j.S, j.Vt, j.ws, s, j.transposedOut);
+		}
+	}
+	if (tasks.empty()) return;
+	static std::once_flag once;
+	std::call_once(once, [&] {
+		const size_t full = ((size_t)JMAXM * JMAXN + (size_t)JMAXN * JMAXN) * sizeof(double);
+		XTT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(jacobi_lds_group_kernel), hipFuncAttributeMaxDynamicSharedMemorySize, (int)full));
+	});
+	const JacobiTask* dt = static_cast<const JacobiTask*>(table_upload(tasks.data(), tasks.size() * sizeof(JacobiTask), s));
+	launch_timed("jacobi_lds_group_kernel", Work(), jacobi_lds_group_kernel, dim3((unsigned)tasks.size()), dim3(JT), lds, s, dt);
 }
 
 }  // namespace xtt

@@ -1,0 +1,888 @@
+// Lockstep batch driver: many independent ALS_SPD + round problems advance together, and every step of every
+// problem (a stack push, a local right-hand side, a local operator product, a CG step, a QR or RQ move, a pivoted
+// QC, a Jacobi SVD) is one grouped launch for all problems (group.hip and the grouped GEMM, Householder and Jacobi
+// kernels). Shapes differ from problem to problem (uneven d, n and ranks); each launch carries a task table.
+// Host decisions (ranks, convergence, energies, the stop rule) are read back once per step for all problems.
+//
+// The per-problem algorithm is the one of als_solve (one site, SPD operator, FP64 local CG with the same
+// tolerance, iteration cap and zero-start rule) and round (QC sweep to the last core, then round_edge from the
+// right with the SVD truncation rule). Trajectories agree with the per-problem drivers to the local solver's
+// tolerance; rank decisions are made by the same kernels and rules.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "tt.hpp"
+
+namespace xtt {
+
+namespace {
+
+const std::vector<size_t> kAm{2, 0, 3, 1};   // operator core (p,n,n',q) → (n,q,p,n'): left pushes, local operator
+const std::vector<size_t> kAmM{1, 0, 3, 2};  // → (n,p,q,n'): right pushes (the left push on the mirrored site)
+
+struct Lock {
+	const TT* A = nullptr;
+	TT* x = nullptr;
+	const TT* b = nullptr;
+	size_t d = 0;
+	std::vector<DTensor> opL, opR, rhsL, rhsR, am, amM;
+	size_t first = 0, second = 0, curr = 0;
+	bool increasing = true, active = false;
+	double le2 = 1e102, le = 1e101, en = 1e100;
+	size_t halfSweeps = 0;
+	bool canonAtEnd = true;
+	size_t corePosAtEnd = 0;
+	size_t localSolves = 0, cgIters = 0;
+};
+
+DTensor ones_dev(const dims_t& dims, std::vector<MapTask>& maps) {
+	DTensor t(dims);
+	maps.push_back(map_fill(t.data(), 1.0, t.size()));
+	return t;
+}
+
+GemmTask gt(const GemmDesc& d, const double* A, const double* B, double* C) { return GemmTask{d, A, B, C}; }
+
+GemmDesc stage2(size_t a, size_t n, size_t c, size_t P, size_t Q, const int* skip) {
+	GemmDesc g = gemm_desc(0, 0, n * Q, c, P * n);
+	g.batch = a;
+	g.sA = 0;
+	g.sB = (long long)(P * n * c);
+	g.sC = (long long)(n * Q * c);
+	g.skip = skip;
+	return g;
+}
+
+// ---- prepare_x_for_als (als.cpp:330-407): full-rank boundary cores become identities of their unfoldings, their
+// content absorbed into the neighbour; one grouped GEMM and one grouped identity fill per absorbed core ------------
+struct Absorb {
+	size_t j;
+	DTensor id, t;
+};
+void prepare_x_all(std::vector<Lock>& P, hipStream_t s) {
+	const size_t J = P.size();
+	std::vector<size_t> f(J, 0), l(J), dimF(J, 1), dimL(J, 1);
+	std::vector<char> goF(J, 1), goL(J, 1);
+	for (size_t j = 0; j < J; ++j) l[j] = P[j].d;
+	for (;;) {  // from the left
+		std::vector<GemmTask> g;
+		std::vector<MapTask> maps;
+		std::vector<Absorb> upd;
+		for (size_t j = 0; j < J; ++j) {
+			TT& x = *P[j].x;
+			if (!goF[j] || f[j] + 1 >= P[j].d) { goF[j] = 0; continue; }
+			const size_t localDim = x.dims[f[j]], newProd = dimF[j] * localDim;
+			if (x.rank(f[j]) < newProd) { goF[j] = 0; continue; }
+			const DTensor& cur = x.cores[f[j]];        // (dimProd·n) × newProd
+			const DTensor& nx = x.cores[f[j] + 1];     // newProd × (n'·r'')
+			DTensor t({cur.dims[0] * cur.dims[1], nx.dims[1], nx.dims[2]});
+			g.push_back(gt(gemm_desc(0, 0, cur.dims[0] * cur.dims[1], nx.dims[1] * nx.dims[2], cur.dims[2]), cur.data(), nx.data(), t.data()));
+			DTensor id({dimF[j], localDim, newProd});
+			maps.push_back(map_eye(id.data(), newProd));
+			upd.push_back(Absorb{j, std::move(id), std::move(t)});
+			dimF[j] = newProd;
+		}
+		if (upd.empty()) break;
+		gemm_group_f64(g, s);
+		map_group(maps, s);
+		for (Absorb& u : upd) {
+			TT& x = *P[u.j].x;
+			x.set_component(f[u.j] + 1, std::move(u.t));
+			x.set_component(f[u.j], std::move(u.id));
+			f[u.j] += 1;
+		}
+	}
+	for (;;) {  // from the right
+		std::vector<GemmTask> g;
+		std::vector<MapTask> maps;
+		std::vector<Absorb> upd;
+		for (size_t j = 0; j < J; ++j) {
+			TT& x = *P[j].x;
+			if (!goL[j] || l[j] <= f[j] + 1) { goL[j] = 0; continue; }
+			const size_t localDim = x.dims[l[j] - 1], newProd = dimL[j] * localDim;
+			if (x.rank(l[j] - 2) < newProd) { goL[j] = 0; continue; }
+			const DTensor& cur = x.cores[l[j] - 1];    // newProd × (n·dimProd)
+			const DTensor& pv = x.cores[l[j] - 2];     // (r''·n') × newProd
+			DTensor t({pv.dims[0], pv.dims[1], cur.dims[1] * cur.dims[2]});
+			g.push_back(gt(gemm_desc(0, 0, pv.dims[0] * pv.dims[1], cur.dims[1] * cur.dims[2], cur.dims[0]), pv.data(), cur.data(), t.data()));
+			DTensor id({newProd, localDim, dimL[j]});
+			maps.push_back(map_eye(id.data(), newProd));
+			upd.push_back(Absorb{j, std::move(id), std::move(t)});
+			dimL[j] = newProd;
+		}
+		if (upd.empty()) break;
+		gemm_group_f64(g, s);
+		map_group(maps, s);
+		for (Absorb& u : upd) {
+			TT& x = *P[u.j].x;
+			x.set_component(l[u.j] - 2, std::move(u.t));
+			x.set_component(l[u.j] - 1, std::move(u.id));
+			l[u.j] -= 1;
+		}
+	}
+	for (size_t j = 0; j < J; ++j) {
+		Lock& L = P[j];
+		TT& x = *L.x;
+		if (L.canonAtEnd && L.corePosAtEnd < f[j]) {
+			x.corePosition = f[j]; x.canonicalized = true;
+		} else {
+			if (L.canonAtEnd && L.corePosAtEnd >= l[j]) { x.corePosition = l[j] - 1; x.canonicalized = true; }
+			if (x.corePosition != f[j] || !x.canonicalized) move_core(x, f[j], true);
+		}
+		L.first = f[j];
+		L.second = l[j];
+	}
+}
+
+// ---- grouped stack pushes -----------------------------------------------------------------------------------
+struct Push {
+	Lock* L;
+	size_t i;
+	bool left;
+	DTensor Xr, op, rb;
+	DBuf<double> T1, T2, Y;
+};
+
+void pushes(std::vector<Push>& P, hipStream_t s) {
+	if (P.empty()) return;
+	std::vector<MapTask> maps;
+	for (Push& p : P) {
+		if (p.left) continue;
+		const DTensor& X = p.L->x->cores[p.i];
+		p.Xr = DTensor({X.dims[2], X.dims[1], X.dims[0]});
+		const size_t sh[3] = {2, 1, 0};
+		maps.push_back(map_permute(3, X.dims.data(), sh, X.data(), p.Xr.data()));
+	}
+	map_group(maps, s);
+	std::vector<GemmTask> g1, g2, g3;
+	for (Push& p : P) {
+		Lock& L = *p.L;
+		const DTensor& X = L.x->cores[p.i];
+		const DTensor& Xe = p.left ? X : p.Xr;
+		const DTensor& St = p.left ? L.opL.back() : L.opR.back();
+		const DTensor& Am = p.left ? L.am[p.i] : L.amM[p.i];
+		const DTensor& bc = L.b->cores[p.i];
+		const size_t a = Xe.dims[0], n = Xe.dims[1], c = Xe.dims[2], Pp = Am.dims[2], Q = Am.dims[1];
+		p.T1.resize(a * Pp * n * c);
+		p.T2.resize(a * n * Q * c);
+		p.op = DTensor({c, Q, c});
+		g1.push_back(gt(gemm_desc(0, 0, a * Pp, n * c, a), St.data(), Xe.data(), p.T1.get()));
+		g2.push_back(gt(stage2(a, n, c, Pp, Q, nullptr), Am.data(), p.T1.get(), p.T2.get()));
+		g3.push_back(gt(gemm_desc(1, 0, c, Q * c, a * n), Xe.data(), p.T2.get(), p.op.data()));
+		// right-hand-side stack (on the unmirrored core X = (a0, n, c0))
+		const size_t a0 = X.dims[0], c0 = X.dims[2], beta = bc.dims[0], beta2 = bc.dims[2];
+		if (p.left) {
+			const DTensor& Lb = L.rhsL.back();
+			p.Y.resize(beta * n * c0);
+			p.rb = DTensor({beta2, c0});
+			g1.push_back(gt(gemm_desc(0, 0, beta, n * c0, a0), Lb.data(), X.data(), p.Y.get()));
+			g2.push_back(gt(gemm_desc(1, 0, beta2, c0, beta * n), bc.data(), p.Y.get(), p.rb.data()));
+		} else {
+			const DTensor& Rb = L.rhsR.back();
+			p.Y.resize(n * beta2 * a0);
+			p.rb = DTensor({beta, a0});
+			GemmDesc z = gemm_desc(0, 1, beta2, a0, c0);
+			z.ldb = n * c0; z.sA = 0; z.sB = (long long)c0; z.sC = (long long)(beta2 * a0); z.batch = n;
+			g1.push_back(gt(z, Rb.data(), X.data(), p.Y.get()));
+			g2.push_back(gt(gemm_desc(0, 0, beta, a0, n * beta2), bc.data(), p.Y.get(), p.rb.data()));
+		}
+	}
+	gemm_group_f64(g1, s);
+	gemm_group_f64(g2, s);
+	gemm_group_f64(g3, s);
+	for (Push& p : P) {
+		Lock& L = *p.L;
+		if (p.left) { L.opL.push_back(std::move(p.op)); L.rhsL.push_back(std::move(p.rb)); }
+		else { L.opR.push_back(std::move(p.op)); L.rhsR.push_back(std::move(p.rb)); }
+	}
+}
+
+// ---- the local problem of a site: right-hand side and matrix-free operator -----------------------------------
+struct Site {
+	Lock* L;
+	size_t a, n, c, P, Q;
+	const DTensor *Lst, *Rst, *Am;
+	DTensor bl;              // local right-hand side (a, n, c)
+	DBuf<double> T1, T2, Tb;
+};
+void site_init(Site& S, Lock* L) {
+	S.L = L;
+	const size_t k = L->curr;
+	const DTensor& X = L->x->cores[k];
+	S.a = X.dims[0]; S.n = X.dims[1]; S.c = X.dims[2];
+	S.Lst = &L->opL.back(); S.Rst = &L->opR.back(); S.Am = &L->am[k];
+	S.P = S.Am->dims[2]; S.Q = S.Am->dims[1];
+	S.T1.resize(S.a * S.P * S.n * S.c);
+	S.T2.resize(S.a * S.n * S.Q * S.c);
+	S.bl = DTensor({S.a, S.n, S.c});
+}
+// y = A_loc·v for every site, three grouped launches (the first two also compute the local right-hand sides
+// when rhs is set); skip[j] (optional): site j's products are skipped while *skip[j] != 0
+void local_apply(std::vector<Site>& S, const std::vector<const double*>& v, const std::vector<double*>& y, bool rhs,
+                 const std::vector<const int*>* skip, hipStream_t s) {
+	std::vector<GemmTask> g1, g2, g3;
+	for (size_t j = 0; j < S.size(); ++j) {
+		Site& t = S[j];
+		const int* sk = skip ? (*skip)[j] : nullptr;
+		GemmDesc d1 = gemm_desc(0, 0, t.a * t.P, t.n * t.c, t.a); d1.skip = sk;
+		g1.push_back(gt(d1, t.Lst->data(), v[j], t.T1.get()));
+		g2.push_back(gt(stage2(t.a, t.n, t.c, t.P, t.Q, sk), t.Am->data(), t.T1.get(), t.T2.get()));
+		GemmDesc d3 = gemm_desc(0, 1, t.a * t.n, t.c, t.Q * t.c); d3.skip = sk;
+		g3.push_back(gt(d3, t.T2.get(), t.Rst->data(), y[j]));
+		if (rhs) {
+			const Lock& L = *t.L;
+			const DTensor& Lb = L.rhsL.back();
+			const DTensor& Rb = L.rhsR.back();
+			const DTensor& bc = L.b->cores[L.curr];
+			const size_t beta = Lb.dims[0], beta2 = Rb.dims[0];
+			XTT_REQUIRE(Lb.dims[1] == t.a && bc.dims[0] == beta && bc.dims[1] == t.n && bc.dims[2] == beta2 && Rb.dims[1] == t.c,
+			            "lockstep ALS: local right-hand side shape mismatch");
+			t.Tb.resize(t.a * t.n * beta2);
+			g1.push_back(gt(gemm_desc(1, 0, t.a, t.n * beta2, beta), Lb.data(), bc.data(), t.Tb.get()));
+			g2.push_back(gt(gemm_desc(0, 0, t.a * t.n, t.c, beta2), t.Tb.get(), Rb.data(), t.bl.data()));
+		}
+	}
+	gemm_group_f64(g1, s);
+	gemm_group_f64(g2, s);
+	gemm_group_f64(g3, s);
+}
+
+// energy of the SPD functional at each lock's current site: |½ xᵀA_loc x − xᵀb_loc|
+void energies(std::vector<Lock*>& Ls, std::vector<double>& E, hipStream_t s) {
+	E.assign(Ls.size(), 0.0);
+	if (Ls.empty()) return;
+	std::vector<Site> S(Ls.size());
+	std::vector<DTensor> y(Ls.size());
+	std::vector<const double*> v;
+	std::vector<double*> yo;
+	for (size_t j = 0; j < Ls.size(); ++j) {
+		site_init(S[j], Ls[j]);
+		y[j] = DTensor({S[j].a, S[j].n, S[j].c});
+		v.push_back(Ls[j]->x->cores[Ls[j]->curr].data());
+		yo.push_back(y[j].data());
+	}
+	local_apply(S, v, yo, true, nullptr, s);
+	std::vector<Dot2Task> dt;
+	for (size_t j = 0; j < Ls.size(); ++j) {
+		const DTensor& X = Ls[j]->x->cores[Ls[j]->curr];
+		dt.push_back(Dot2Task{(int)X.size(), X.data(), S[j].bl.data(), X.data(), y[j].data()});
+	}
+	DBuf<double> out(2 * Ls.size());
+	dot2_group(dt, out.get(), s);
+	std::vector<double> h(2 * Ls.size());
+	read_back(h.data(), out.get(), h.size() * sizeof(double), s);
+	table_ring_release(s);
+	for (size_t j = 0; j < Ls.size(); ++j) E[j] = std::abs(0.5 * h[2 * j + 1] - h[2 * j]);
+}
+
+// FP64 CG on every lock's local system (warm start from its core, which is updated in place)
+void local_solves(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t s) {
+	if (Ls.empty()) return;
+	const size_t J = Ls.size();
+	std::vector<Site> S(J);
+	std::vector<DBuf<double>> r(J), p(J), q(J);
+	DBuf<CGState> st(J);
+	std::vector<CGState> init(J);
+	std::vector<CGTask> tasks(J);
+	std::vector<const double*> xv(J), pv(J);
+	std::vector<double*> qv(J);
+	std::vector<const int*> skip(J);
+	for (size_t j = 0; j < J; ++j) {
+		site_init(S[j], Ls[j]);
+		const size_t N = S[j].a * S[j].n * S[j].c;
+		r[j].resize(N); p[j].resize(N); q[j].resize(N);
+		std::memset(&init[j], 0, sizeof(CGState));
+		init[j].tol2 = o.localTol * o.localTol;
+		init[j].maxIter = o.maxLocalIter;
+		double* x = Ls[j]->x->cores[Ls[j]->curr].data();
+		tasks[j] = CGTask{(int)N, x, r[j].get(), p[j].get(), q[j].get(), S[j].bl.data(), st.get() + j};
+		xv[j] = x; pv[j] = p[j].get(); qv[j] = q[j].get();
+		skip[j] = &st.get()[j].done;
+		Ls[j]->localSolves++;
+	}
+	const void* dInit = table_upload(init.data(), J * sizeof(CGState), s);
+	XTT_HIP(hipMemcpyAsync(st.get(), dInit, J * sizeof(CGState), hipMemcpyDeviceToDevice, s));
+	local_apply(S, xv, qv, true, nullptr, s);  // q = A_loc·x and the right-hand sides
+	cg_group_start(tasks, s);
+	// the three products of q = A_loc·p and the CG step for the systems still iterating, their tables kept on the
+	// device for every iteration of a chunk; the products of a system that converges inside a chunk exit at once
+	// (skip flag), and the tables are rebuilt without it once the open set has shrunk by a quarter
+	GemmGroupPlan g1, g2, g3;
+	DBuf<CGTask> dTasks;
+	int planned = 0;
+	auto plan = [&](const std::vector<size_t>& open) {
+		std::vector<GemmTask> t1, t2, t3;
+		std::vector<CGTask> tk;
+		for (size_t j : open) {
+			Site& t = S[j];
+			GemmDesc d1 = gemm_desc(0, 0, t.a * t.P, t.n * t.c, t.a); d1.skip = skip[j];
+			t1.push_back(gt(d1, t.Lst->data(), pv[j], t.T1.get()));
+			t2.push_back(gt(stage2(t.a, t.n, t.c, t.P, t.Q, skip[j]), t.Am->data(), t.T1.get(), t.T2.get()));
+			GemmDesc d3 = gemm_desc(0, 1, t.a * t.n, t.c, t.Q * t.c); d3.skip = skip[j];
+			t3.push_back(gt(d3, t.T2.get(), t.Rst->data(), qv[j]));
+			tk.push_back(tasks[j]);
+		}
+		gemm_group_plan_f64(t1, g1, s);  // each plan also runs once: the first product of the next iteration
+		gemm_group_plan_f64(t2, g2, s);
+		gemm_group_plan_f64(t3, g3, s);
+		dTasks.resize(tk.size());
+		table_upload_to(dTasks.get(), tk.data(), tk.size() * sizeof(CGTask), s);
+		cg_group_step_run(dTasks.get(), (int)tk.size(), s);
+		planned = (int)open.size();
+	};
+	std::vector<size_t> open(J);
+	for (size_t j = 0; j < J; ++j) open[j] = j;
+	plan(open);
+	std::vector<CGState> h(J);
+	int done = 1, chunk = 2;  // iterations run so far; chunks grow 2, 4, 8, 16
+	for (;;) {
+		read_back(h.data(), st.get(), J * sizeof(CGState), s);
+		std::vector<size_t> still;
+		for (size_t j = 0; j < J; ++j) if (!h[j].done) still.push_back(j);
+		if (still.empty() || done >= o.maxLocalIter) break;
+		int n = std::min(chunk, o.maxLocalIter - done);
+		if (4 * still.size() <= 3 * (size_t)planned) {
+			table_ring_release(s);
+			plan(still);
+			--n;
+			done += 1;
+		}
+		for (int c = 0; c < n; ++c) {
+			gemm_group_run_f64(g1, s);
+			gemm_group_run_f64(g2, s);
+			gemm_group_run_f64(g3, s);
+			cg_group_step_run(dTasks.get(), planned, s);
+		}
+		done += n;
+		chunk = std::min(16, 2 * chunk);
+	}
+	table_ring_release(s);
+	for (size_t j = 0; j < J; ++j) Ls[j]->cgIters += (size_t)std::max(0, h[j].iter);
+}
+
+// ---- core moves: QR to the right (x_k = Q, x_{k+1} = R·x_{k+1}), RQ to the left (x_k = Q, x_{k-1} = x_{k-1}·R) --
+struct Move {
+	TT* x;
+	size_t k;
+	bool right;
+};
+void moves(const std::vector<Move>& M, hipStream_t s) {
+	if (M.empty()) return;
+	const size_t J = M.size();
+	std::vector<DTensor> B(J), Qt(J), Rt(J), Q(J), R(J);
+	std::vector<QRJob> jobs;
+	std::vector<MapTask> maps;
+	for (size_t j = 0; j < J; ++j) {
+		const DTensor& F = M[j].x->cores[M[j].k];
+		const size_t a = F.dims[0], n = F.dims[1], c = F.dims[2];
+		if (M[j].right) {
+			const size_t m = a * n, kk = std::min(m, c);
+			Q[j] = DTensor({a, n, kk});
+			R[j] = DTensor({kk, c});
+			jobs.push_back(QRJob{F.data(), (int)m, (int)c, false, Q[j].data(), R[j].data()});
+		} else {  // QR(J·Fᵀ·J) = Q̃R̃ ⇒ F = (J·R̃ᵀ·J)(J·Q̃ᵀ·J)
+			const size_t m = a, nn = n * c, kk = std::min(m, nn);
+			B[j] = DTensor({nn, m});
+			maps.push_back(map_revtrans(F.data(), m, nn, B[j].data()));
+			Qt[j] = DTensor({nn, kk});
+			Rt[j] = DTensor({kk, m});
+			jobs.push_back(QRJob{B[j].data(), (int)nn, (int)m, false, Qt[j].data(), Rt[j].data()});
+		}
+	}
+	map_group(maps, s);
+	hh_qr_group(jobs, s);
+	maps.clear();
+	for (size_t j = 0; j < J; ++j) {
+		if (M[j].right) continue;
+		const DTensor& F = M[j].x->cores[M[j].k];
+		const size_t a = F.dims[0], n = F.dims[1], c = F.dims[2], nn = n * c, kk = std::min(a, nn);
+		R[j] = DTensor({a, kk});
+		Q[j] = DTensor({kk, n, c});
+		maps.push_back(map_revtrans(Rt[j].data(), kk, a, R[j].data()));
+		maps.push_back(map_revtrans(Qt[j].data(), nn, kk, Q[j].data()));
+	}
+	map_group(maps, s);
+	std::vector<GemmTask> g;
+	std::vector<DTensor> T(J);
+	for (size_t j = 0; j < J; ++j) {
+		TT& x = *M[j].x;
+		if (M[j].right) {
+			const DTensor& N = x.cores[M[j].k + 1];
+			const size_t kk = R[j].dims[0], c = R[j].dims[1];
+			T[j] = DTensor({kk, N.dims[1], N.dims[2]});
+			g.push_back(gt(gemm_desc(0, 0, kk, N.dims[1] * N.dims[2], c), R[j].data(), N.data(), T[j].data()));
+		} else {
+			const DTensor& N = x.cores[M[j].k - 1];
+			const size_t a = R[j].dims[0], kk = R[j].dims[1];
+			T[j] = DTensor({N.dims[0], N.dims[1], kk});
+			g.push_back(gt(gemm_desc(0, 0, N.dims[0] * N.dims[1], kk, a), N.data(), R[j].data(), T[j].data()));
+		}
+	}
+	gemm_group_f64(g, s);
+	for (size_t j = 0; j < J; ++j) {
+		TT& x = *M[j].x;
+		const size_t k = M[j].k;
+		x.cores[k] = std::move(Q[j]);
+		x.cores[M[j].right ? k + 1 : k - 1] = std::move(T[j]);
+		x.corePosition = M[j].right ? k + 1 : k - 1;
+		x.canonicalized = true;
+	}
+}
+
+// ---- one ALS_SPD call (sites = 1) for every problem --------------------------------------------------------------
+void als_lockstep(const ALSOptions& o, std::vector<Lock>& P, hipStream_t s) {
+	for (Lock& L : P) {
+		L.canonAtEnd = L.x->canonicalized;
+		L.corePosAtEnd = L.x->corePosition;
+		L.opL.clear(); L.opR.clear(); L.rhsL.clear(); L.rhsR.clear(); L.am.clear(); L.amM.clear();
+		L.le2 = 1e102; L.le = 1e101; L.en = 1e100; L.halfSweeps = 0; L.increasing = true; L.active = true;
+	}
+	prepare_x_all(P, s);
+	// permuted operator cores and the stacks' boundary entries
+	std::vector<MapTask> maps;
+	for (Lock& L : P) {
+		for (size_t k = 0; k < L.d; ++k) {
+			const DTensor& Ac = L.A->cores[k];
+			DTensor a({Ac.dims[1], Ac.dims[3], Ac.dims[0], Ac.dims[2]}), m({Ac.dims[1], Ac.dims[0], Ac.dims[3], Ac.dims[2]});
+			maps.push_back(map_permute(4, Ac.dims.data(), kAm.data(), Ac.data(), a.data()));
+			maps.push_back(map_permute(4, Ac.dims.data(), kAmM.data(), Ac.data(), m.data()));
+			L.am.push_back(std::move(a));
+			L.amM.push_back(std::move(m));
+		}
+		L.opL.push_back(ones_dev({1, 1, 1}, maps));
+		L.opR.push_back(ones_dev({1, 1, 1}, maps));
+		L.rhsL.push_back(ones_dev({1, 1}, maps));
+		L.rhsR.push_back(ones_dev({1, 1}, maps));
+	}
+	map_group(maps, s);
+	// prepare_stacks: right pushes from the last core down to first + 1, then left pushes below first
+	for (size_t step = 0;; ++step) {
+		std::vector<Push> pu;
+		for (Lock& L : P) {
+			if (L.d < 2 + step) continue;
+			const size_t i = L.d - 1 - step;
+			if (i > L.first) pu.push_back(Push{&L, i, false});
+		}
+		if (pu.empty()) break;
+		pushes(pu, s);
+	}
+	for (size_t i = 0;; ++i) {
+		std::vector<Push> pu;
+		for (Lock& L : P) if (i < L.first) pu.push_back(Push{&L, i, true});
+		if (pu.empty()) break;
+		pushes(pu, s);
+	}
+	// the start energy at the first site
+	std::vector<Lock*> act;
+	for (Lock& L : P) { L.curr = L.first; act.push_back(&L); }
+	std::vector<double> E;
+	energies(act, E, s);
+	for (size_t j = 0; j < act.size(); ++j) act[j]->en = E[j];
+	// sweeps: local solve, end-of-sweep test, move to the next site
+	while (!act.empty()) {
+		local_solves(act, o, s);
+		std::vector<Lock*> atEnd;
+		for (Lock* L : act)
+			if ((!L->increasing && L->curr == L->first) || (L->increasing && L->curr == L->second - 1)) atEnd.push_back(L);
+		energies(atEnd, E, s);
+		std::vector<Lock*> stay;
+		std::vector<size_t> endIdx(act.size(), SIZE_MAX);
+		for (size_t j = 0; j < atEnd.size(); ++j) {
+			Lock& L = *atEnd[j];
+			L.halfSweeps += 1;
+			L.le2 = L.le;
+			L.le = L.en;
+			L.en = E[j];
+			const bool stop = L.halfSweeps == o.numHalfSweeps || std::abs(L.le - L.en) < o.convergenceEpsilon ||
+			                  std::abs(L.le2 - L.en) < o.convergenceEpsilon || L.second - L.first <= 1;
+			if (stop) L.active = false;
+			else L.increasing = !L.increasing;
+		}
+		std::vector<Move> mv;
+		std::vector<Push> pu;
+		std::vector<Lock*> moving;
+		for (Lock* L : act) {
+			if (!L->active) continue;
+			mv.push_back(Move{L->x, L->curr, L->increasing});
+			moving.push_back(L);
+		}
+		moves(mv, s);
+		for (Lock* L : moving) {
+			if (L->increasing) {
+				L->opR.pop_back(); L->rhsR.pop_back();
+				pu.push_back(Push{L, L->curr, true});
+			} else {
+				L->opL.pop_back(); L->rhsL.pop_back();
+				pu.push_back(Push{L, L->curr, false});
+			}
+		}
+		pushes(pu, s);
+		for (Lock* L : moving) L->curr += L->increasing ? 1 : -1;
+		act = moving;
+	}
+	// preserveCorePosition: back to the entry position (keeping ranks)
+	for (;;) {
+		std::vector<Move> mv;
+		for (Lock& L : P) {
+			TT& x = *L.x;
+			if (!(o.preserveCorePosition && L.canonAtEnd) || x.corePosition == L.corePosAtEnd) continue;
+			const bool right = x.corePosition < L.corePosAtEnd;
+			mv.push_back(Move{&x, x.corePosition, right});
+		}
+		if (mv.empty()) break;
+		moves(mv, s);
+	}
+	for (Lock& L : P) { L.opL.clear(); L.opR.clear(); L.rhsL.clear(); L.rhsR.clear(); L.am.clear(); L.amM.clear(); }
+}
+
+// ---- round(eps) (ttNetwork.cpp:643-665) for every problem ------------------------------------------------------
+bool exceeds_max(const TT& x) {
+	for (size_t i = 0; i < x.order(); ++i) {
+		const DTensor& c = x.cores[i];
+		const size_t e = x.ext(i);
+		if (c.dims.front() > e * c.dims.back() || c.dims.back() > e * c.dims.front()) return true;
+	}
+	return false;
+}
+
+// QC moves k → k+1 (rank reduction allowed) for the listed problems
+void qc_moves(const std::vector<std::pair<TT*, size_t>>& M, hipStream_t s) {
+	const size_t J = M.size();
+	if (!J) return;
+	std::vector<DBuf<double>> q(J), c(J);
+	std::vector<DBuf<int>> jp(J);
+	DBuf<int> rk(J);
+	std::vector<QCJob> jobs;
+	for (size_t j = 0; j < J; ++j) {
+		const DTensor& F = M[j].first->cores[M[j].second];
+		const size_t m = F.dims[0] * F.dims[1], n = F.dims[2], k = std::min(m, n);
+		q[j].resize(m * k); c[j].resize(k * n); jp[j].resize(n);
+		jobs.push_back(QCJob{F.data(), (int)m, (int)n, q[j].get(), c[j].get(), jp[j].get(), rk.get() + j});
+	}
+	hh_qc_group(jobs, s);
+	std::vector<int> h(J);
+	read_back(h.data(), rk.get(), J * sizeof(int), s);
+	table_ring_release(s);
+	std::vector<GemmTask> g;
+	std::vector<DTensor> T(J);
+	for (size_t j = 0; j < J; ++j) {
+		TT& x = *M[j].first;
+		const size_t k = M[j].second, r = (size_t)h[j];
+		const DTensor& F = x.cores[k];
+		const DTensor& N = x.cores[k + 1];
+		T[j] = DTensor({r, N.dims[1], N.dims[2]});
+		g.push_back(gt(gemm_desc(0, 0, r, N.dims[1] * N.dims[2], F.dims[2]), c[j].get(), N.data(), T[j].data()));
+	}
+	gemm_group_f64(g, s);
+	for (size_t j = 0; j < J; ++j) {
+		TT& x = *M[j].first;
+		const size_t k = M[j].second, r = (size_t)h[j];
+		const DTensor& F = x.cores[k];
+		x.cores[k] = DTensor({F.dims[0], F.dims[1], r}, std::move(q[j]));
+		x.cores[k + 1] = std::move(T[j]);
+		x.corePosition = k + 1;
+		x.canonicalized = true;
+	}
+}
+
+// CQ moves k → k−1 (rank reduction allowed): F (r × n·r') = C·Q by a pivoted QR of Fᵀ; x_k = Q, x_{k-1} = x_{k-1}·C
+void cq_moves(const std::vector<std::pair<TT*, size_t>>& M, hipStream_t s) {
+	const size_t J = M.size();
+	if (!J) return;
+	std::vector<DTensor> Ft(J);
+	std::vector<DBuf<double>> q(J), c(J);
+	std::vector<DBuf<int>> jp(J);
+	DBuf<int> rk(J);
+	std::vector<MapTask> maps;
+	std::vector<QCJob> jobs;
+	for (size_t j = 0; j < J; ++j) {
+		const DTensor& F = M[j].first->cores[M[j].second];
+		const size_t r = F.dims[0], nr = F.dims[1] * F.dims[2], k = std::min(nr, r);
+		Ft[j] = DTensor({nr, r});
+		const size_t d2[2] = {r, nr}, sh[2] = {1, 0};
+		maps.push_back(map_permute(2, d2, sh, F.data(), Ft[j].data()));
+		q[j].resize(nr * k); c[j].resize(k * r); jp[j].resize(r);
+		jobs.push_back(QCJob{Ft[j].data(), (int)nr, (int)r, q[j].get(), c[j].get(), jp[j].get(), rk.get() + j});
+	}
+	map_group(maps, s);
+	hh_qc_group(jobs, s);
+	std::vector<int> hr(J);
+	read_back(hr.data(), rk.get(), J * sizeof(int), s);
+	table_ring_release(s);
+	std::vector<DTensor> C(J), Q(J), T(J);
+	maps.clear();
+	std::vector<GemmTask> g;
+	for (size_t j = 0; j < J; ++j) {
+		TT& x = *M[j].first;
+		const DTensor& F = x.cores[M[j].second];
+		const DTensor& N = x.cores[M[j].second - 1];
+		const size_t r = F.dims[0], nr = F.dims[1] * F.dims[2], rr = (size_t)hr[j];
+		C[j] = DTensor({r, rr});
+		Q[j] = DTensor({rr, F.dims[1], F.dims[2]});
+		{ const size_t d2[2] = {rr, r}, sh[2] = {1, 0}; maps.push_back(map_permute(2, d2, sh, c[j].get(), C[j].data())); }
+		{ const size_t d2[2] = {nr, rr}, sh[2] = {1, 0}; maps.push_back(map_permute(2, d2, sh, q[j].get(), Q[j].data())); }
+		T[j] = DTensor({N.dims[0], N.dims[1], rr});
+		g.push_back(gt(gemm_desc(0, 0, N.dims[0] * N.dims[1], rr, r), N.data(), C[j].data(), T[j].data()));
+	}
+	map_group(maps, s);
+	gemm_group_f64(g, s);
+	for (size_t j = 0; j < J; ++j) {
+		TT& x = *M[j].first;
+		const size_t k = M[j].second;
+		x.cores[k] = std::move(Q[j]);
+		x.cores[k - 1] = std::move(T[j]);
+		x.corePosition = k - 1;
+		x.canonicalized = true;
+	}
+}
+
+// move_core's sweeps while a core exceeds its maximal ranks (ttNetwork.cpp:609-622), in lockstep, for problems whose
+// core sits at `pos` = their last core: to the first core (CQ moves), to the last (QC moves), until no core exceeds
+void max_rank_sweeps(std::vector<Lock*> E, hipStream_t s) {
+	while (!E.empty()) {
+		for (;;) {
+			std::vector<std::pair<TT*, size_t>> M;
+			for (Lock* L : E) if (L->x->corePosition > 0) M.push_back({L->x, L->x->corePosition});
+			if (M.empty()) break;
+			cq_moves(M, s);
+		}
+		for (;;) {
+			std::vector<std::pair<TT*, size_t>> M;
+			for (Lock* L : E) if (L->x->corePosition + 1 < L->d) M.push_back({L->x, L->x->corePosition});
+			if (M.empty()) break;
+			qc_moves(M, s);
+		}
+		std::vector<Lock*> still;
+		for (Lock* L : E) if (exceeds_max(*L->x)) still.push_back(L);
+		E.swap(still);
+	}
+}
+
+// round_edge(k, k−1) with the left part orthogonal (tt.cpp round_edge, first branch) for the listed problems
+void edges(const std::vector<std::pair<TT*, size_t>>& M, double eps, hipStream_t s) {
+	const size_t J = M.size();
+	if (!J) return;
+	// CQ of F (r × n·r'): pivoted QR of Fᵀ
+	std::vector<DTensor> Ft(J);
+	std::vector<DBuf<double>> q(J), c(J);
+	std::vector<DBuf<int>> jp(J);
+	DBuf<int> rk(J);
+	std::vector<MapTask> maps;
+	std::vector<QCJob> jobs;
+	for (size_t j = 0; j < J; ++j) {
+		const DTensor& F = M[j].first->cores[M[j].second];
+		const size_t r = F.dims[0], nr = F.dims[1] * F.dims[2], k = std::min(nr, r);
+		Ft[j] = DTensor({nr, r});
+		const size_t d2[2] = {r, nr}, sh[2] = {1, 0};
+		maps.push_back(map_permute(2, d2, sh, F.data(), Ft[j].data()));
+		q[j].resize(nr * k); c[j].resize(k * r); jp[j].resize(r);
+		jobs.push_back(QCJob{Ft[j].data(), (int)nr, (int)r, q[j].get(), c[j].get(), jp[j].get(), rk.get() + j});
+	}
+	map_group(maps, s);
+	hh_qc_group(jobs, s);
+	std::vector<int> hr(J);
+	read_back(hr.data(), rk.get(), J * sizeof(int), s);
+	table_ring_release(s);
+	// coreA = Ctᵀ (r × rk), Fq = Qtᵀ (rk × n·r'); SVD of X = coreAᵀ (rk × r) by Jacobi on coreA (transposed output)
+	std::vector<DTensor> coreA(J), Fq(J);
+	std::vector<DBuf<double>> u(J), sv(J), vt(J), ws(J);
+	std::vector<SVDJob> sj;
+	maps.clear();
+	for (size_t j = 0; j < J; ++j) {
+		const DTensor& F = M[j].first->cores[M[j].second];
+		const size_t r = F.dims[0], nr = F.dims[1] * F.dims[2], rk = (size_t)hr[j];
+		coreA[j] = DTensor({r, rk});
+		Fq[j] = DTensor({rk, F.dims[1], F.dims[2]});
+		{ const size_t d2[2] = {rk, r}, sh[2] = {1, 0}; maps.push_back(map_permute(2, d2, sh, c[j].get(), coreA[j].data())); }
+		{ const size_t d2[2] = {nr, rk}, sh[2] = {1, 0}; maps.push_back(map_permute(2, d2, sh, q[j].get(), Fq[j].data())); }
+		// X is rk × r with rk <= r: U = V' (rk × rk) and Vt = U'ᵀ (rk × r)
+		const size_t m = rk, n = r, k = std::min(m, n);
+		u[j].resize(m * k); sv[j].resize(k); vt[j].resize(k * n);
+		const size_t M2 = std::max(m, n), N2 = k, np = N2 + (N2 & 1);
+		ws[j].resize(M2 * np + np * np + 2 * np);
+		sj.push_back(SVDJob{coreA[j].data(), (int)r, (int)rk, vt[j].get(), sv[j].get(), u[j].get(), ws[j].get(), true});
+	}
+	map_group(maps, s);
+	jacobi_svd_group(sj, s);
+	// singular values → truncation ranks (xerus: min(k, maxRank) then the first j >= 1 with s_j <= eps·s_0)
+	std::vector<size_t> off(J + 1, 0);
+	for (size_t j = 0; j < J; ++j) off[j + 1] = off[j] + sv[j].size();
+	std::vector<double> S(off[J]);
+	{
+		DBuf<double> all(off[J]);
+		maps.clear();
+		for (size_t j = 0; j < J; ++j) maps.push_back(map_copy(sv[j].get(), all.get() + off[j], sv[j].size()));
+		map_group(maps, s);
+		read_back(S.data(), all.get(), off[J] * sizeof(double), s);
+		table_ring_release(s);
+	}
+	std::vector<size_t> rank(J);
+	for (size_t j = 0; j < J; ++j) {
+		const double* sh = S.data() + off[j];
+		const size_t k = sv[j].size();
+		if (std::isnan(sh[0])) throw Error(ERR_NUMERIC, "SVD: the one-sided Jacobi iteration did not converge in 60 sweeps");
+		size_t rr = k;
+		for (size_t i = 1; i < rr; ++i) if (sh[i] <= eps * sh[0]) { rr = i; break; }
+		rank[j] = rr;
+	}
+	// U (rk × rank) = leading columns of u (read in place), Vt (rank × r) = leading rows of vt, scaled by S (a
+	// second launch: the tasks of one launch run concurrently)
+	std::vector<DTensor> Vt(J), Fn(J), Tn(J);
+	std::vector<MapTask> scale;
+	maps.clear();
+	std::vector<GemmTask> g;
+	for (size_t j = 0; j < J; ++j) {
+		TT& x = *M[j].first;
+		const size_t k0 = M[j].second;
+		const DTensor& F = x.cores[k0];
+		const DTensor& T = x.cores[k0 - 1];
+		const size_t r = F.dims[0], rk = (size_t)hr[j], kk = sv[j].size(), rr = rank[j];
+		Vt[j] = DTensor({rr, r});
+		maps.push_back(map_copy(vt[j].get(), Vt[j].data(), rr * r));
+		scale.push_back(map_scale_by(Vt[j].data(), sv[j].get(), rr, r, true));
+		Fn[j] = DTensor({rr, F.dims[1], F.dims[2]});
+		Tn[j] = DTensor({T.dims[0], T.dims[1], rr});
+		GemmDesc du = gemm_desc(1, 0, rr, F.dims[1] * F.dims[2], rk);
+		du.lda = kk;  // U = u[:, :rr] read in place (ld kk)
+		g.push_back(gt(du, u[j].get(), Fq[j].data(), Fn[j].data()));
+	}
+	map_group(maps, s);
+	map_group(scale, s);
+	std::vector<GemmTask> g2;
+	for (size_t j = 0; j < J; ++j) {
+		TT& x = *M[j].first;
+		const DTensor& T = x.cores[M[j].second - 1];
+		const size_t r = x.cores[M[j].second].dims[0], rr = rank[j];
+		g2.push_back(gt(gemm_desc(0, 1, T.dims[0] * T.dims[1], rr, r), T.data(), Vt[j].data(), Tn[j].data()));
+	}
+	gemm_group_f64(g, s);
+	gemm_group_f64(g2, s);
+	for (size_t j = 0; j < J; ++j) {
+		TT& x = *M[j].first;
+		const size_t k0 = M[j].second;
+		x.cores[k0] = std::move(Fn[j]);
+		x.cores[k0 - 1] = std::move(Tn[j]);
+	}
+}
+
+void round_lockstep(std::vector<Lock>& P, double eps, hipStream_t s) {
+	std::vector<char> initCanon(P.size());
+	std::vector<size_t> initPos(P.size());
+	for (size_t j = 0; j < P.size(); ++j) {
+		TT& x = *P[j].x;
+		initCanon[j] = x.canonicalized;
+		initPos[j] = x.corePosition;
+		XTT_REQUIRE(x.canonicalized, "lockstep round: x must be canonical");
+	}
+	// move_core(x, d − 1, false): QC moves from the core position to the last core
+	for (;;) {
+		std::vector<std::pair<TT*, size_t>> M;
+		for (Lock& L : P) if (L.x->corePosition + 1 < L.d) M.push_back({L.x, L.x->corePosition});
+		if (M.empty()) break;
+		qc_moves(M, s);
+	}
+	// a rank dropped by a QC can leave a neighbour above its maximal rank; move_core then sweeps until it is not
+	// (ttNetwork.cpp:609-622)
+	{
+		std::vector<Lock*> E;
+		for (Lock& L : P) if (exceeds_max(*L.x)) E.push_back(&L);
+		max_rank_sweeps(E, s);
+	}
+	// round_edge(d−1−i, d−2−i) from the right; problems whose edge falls outside the first branch of round_edge are
+	// rounded on their own
+	for (size_t i = 0;; ++i) {
+		std::vector<std::pair<TT*, size_t>> M;
+		for (Lock& L : P) {
+			if (i + 1 >= L.d) continue;
+			TT& x = *L.x;
+			const size_t from = L.d - 1 - i;
+			const DTensor& F = x.cores[from];
+			const DTensor& T = x.cores[from - 1];
+			const double r = (double)F.dims[0];
+			if (5.0 * (double)F.size() * (double)T.size() >= 6.0 * r * r * r * r) M.push_back({&x, from});
+			else round_edge(x, from, from - 1, 0, eps, true);
+		}
+		bool any = false;
+		for (Lock& L : P) any |= i + 1 < L.d;
+		if (!any) break;
+		edges(M, eps, s);
+	}
+	for (size_t j = 0; j < P.size(); ++j) {
+		TT& x = *P[j].x;
+		x.corePosition = 0;
+		x.canonicalized = true;
+		if (initCanon[j] && initPos[j] != 0) move_core(x, initPos[j], false);
+	}
+}
+
+}  // namespace
+
+bool lockstep_supported(const ALSOptions& o, const std::vector<const TT*>& A) {
+	if (o.sites != 1 || !o.assumeSPD || o.useResidualForEndCriterion || o.precision != P_F64) return false;
+	for (const TT* a : A) if (!a) return false;
+	return true;
+}
+
+void als_round_lockstep(const ALSOptions& o, const std::vector<const TT*>& A, const std::vector<TT*>& x,
+                        const std::vector<const TT*>& b, size_t sweeps, double eps, double* residuals, ALSStats* total) {
+	const size_t count = x.size();
+	std::vector<Lock> P(count);
+	for (size_t j = 0; j < count; ++j) {
+		XTT_REQUIRE(x[j]->order() > 0 && x[j]->dims == b[j]->dims && !x[j]->op && !b[j]->op, "lockstep: x and b must be TTTensors of equal dimensions");
+		XTT_REQUIRE(A[j]->op && A[j]->order() == x[j]->order(), "lockstep: operator degree mismatch");
+		require_correct_format(*x[j]);
+		require_correct_format(*b[j]);
+		require_correct_format(*A[j]);
+		P[j].A = A[j]; P[j].x = x[j]; P[j].b = b[j]; P[j].d = x[j]->order();
+	}
+	hipStream_t s = stream();
+	for (size_t sw = 0; sw < sweeps; ++sw) {
+		als_lockstep(o, P, s);
+		round_lockstep(P, eps, s);
+	}
+	sync();
+	if (residuals) {  // ‖Ax − b‖/‖b‖ per problem (a QR sweep over the TT of Ax − b each), on worker threads
+		int dev = 0;
+		XTT_HIP(hipGetDevice(&dev));
+		park_stream_pool(stream());
+		const size_t T = std::min<size_t>(16, count);
+		std::vector<std::thread> pool;
+		std::exception_ptr err;
+		std::mutex mu;
+		for (size_t t = 0; t < T; ++t)
+			pool.emplace_back([&, t] {
+				hipStream_t ws = nullptr;
+				try {
+					XTT_HIP(hipSetDevice(dev));
+					XTT_HIP(hipStreamCreateWithFlags(&ws, hipStreamDefault));
+					bind_thread_stream(ws);
+					for (size_t j = t; j < count; j += T) residuals[j] = tt_residual(*A[j], *x[j], *b[j]);
+				} catch (...) {
+					std::lock_guard<std::mutex> lk(mu);
+					if (!err) err = std::current_exception();
+				}
+				if (ws) {
+					(void)hipStreamSynchronize(ws);
+					bind_thread_stream(nullptr);
+					park_stream_pool(ws);
+					(void)hipStreamDestroy(ws);
+				}
+			});
+		for (auto& th : pool) th.join();
+		if (err) std::rethrow_exception(err);
+	}
+	if (total) {
+		for (const Lock& L : P) {
+			total->localSolves += L.localSolves;
+			total->cgIterations += L.cgIters;
+			total->halfSweeps += L.halfSweeps;
+		}
+	}
+}
+
+}  // namespace xtt

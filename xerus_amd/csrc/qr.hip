@@ -85,7 +85,7 @@ struct FactorArgs {
 };
 
 template <int RPT, bool PIV>
-__global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
+__device__ __forceinline__ void hh_factor_body(const FactorArgs& a, const int b) {
 	__shared__ double sW[2][HW][HN];   // per-wave Σ_{own r > i} a_ri·a_rj (lane i: the norm² partial)
 	__shared__ double sRow[2][HN];     // row i
 	__shared__ double sR[HN][HN];      // finished rows of R (lanes >= row)
@@ -94,7 +94,6 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 	__shared__ int sRank;
 	const int lane = threadIdx.x & 63;
 	const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-	const int b = blockIdx.x;
 	const int r0 = b * a.rpb;
 	const int rows = min(a.rpb, a.rows - r0);
 	const int n = a.n;
@@ -309,6 +308,16 @@ __global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) {
 	}
 }
 
+template <int RPT, bool PIV>
+__global__ __launch_bounds__(HT) void hh_factor_kernel(FactorArgs a) { hh_factor_body<RPT, PIV>(a, (int)blockIdx.x); }
+// many factorisations in one launch: task t owns workgroups [start[t], start[t+1])
+template <int RPT, bool PIV>
+__global__ __launch_bounds__(HT) void hh_factor_group_kernel(const FactorArgs* tasks, const int* start, int count) {
+	const int t = group_task(start, count, (int)blockIdx.x);
+	const FactorArgs a = tasks[t];
+	hh_factor_body<RPT, PIV>(a, (int)blockIdx.x - start[t]);
+}
+
 struct ApplyArgs {
 	const double* V;      // a factored level (same geometry as its FactorArgs)
 	const double* tau;
@@ -504,9 +513,8 @@ __device__ __forceinline__ void wy_product(const WyLds& L, const double* U, int 
 	}
 }
 
-__global__ __launch_bounds__(HT) void hh_wy_kernel(ApplyArgs a) {
+__device__ __forceinline__ void hh_wy_body(const ApplyArgs& a, const int b) {
 	__shared__ WyLds L;
-	const int b = blockIdx.x;
 	const int r0 = b * a.rpb;
 	const int rows = min(a.rpb, a.rows - r0);
 	const int n = a.n;
@@ -524,9 +532,8 @@ __global__ __launch_bounds__(HT) void hh_wy_kernel(ApplyArgs a) {
 // Two tree levels in one launch, the root started from the identity: leaf block b first forms rows
 // [b·n, b·n + n) of the root's Q itself (every leaf block repeats the root's staging and triangular solve), then
 // applies its own reflectors to them. Saves the root's launch and its round trip through global memory.
-__global__ __launch_bounds__(HT) void hh_wy2_kernel(ApplyArgs a, ApplyArgs root) {
+__device__ __forceinline__ void hh_wy2_body(const ApplyArgs& a, const ApplyArgs& root, const int b) {
 	__shared__ WyLds L;
-	const int b = blockIdx.x;
 	const int r0 = b * a.rpb;
 	const int rows = min(a.rpb, a.rows - r0);
 	const int n = a.n;
@@ -543,6 +550,20 @@ __global__ __launch_bounds__(HT) void hh_wy2_kernel(ApplyArgs a, ApplyArgs root)
 	__syncthreads();
 	wy_solve(L, k, nc, false);
 	wy_product(L, U, n, 0, rows, k, nc, false, a.out + (size_t)r0 * nc, false, nullptr);
+}
+
+__global__ __launch_bounds__(HT) void hh_wy_kernel(ApplyArgs a) { hh_wy_body(a, (int)blockIdx.x); }
+__global__ __launch_bounds__(HT) void hh_wy2_kernel(ApplyArgs a, ApplyArgs root) { hh_wy2_body(a, root, (int)blockIdx.x); }
+__global__ __launch_bounds__(HT) void hh_wy_group_kernel(const ApplyArgs* tasks, const int* start, int count) {
+	const int t = group_task(start, count, (int)blockIdx.x);
+	const ApplyArgs a = tasks[t];
+	hh_wy_body(a, (int)blockIdx.x - start[t]);
+}
+// tasks[2t] is a leaf level, tasks[2t + 1] its single-block root
+__global__ __launch_bounds__(HT) void hh_wy2_group_kernel(const ApplyArgs* tasks, const int* start, int count) {
+	const int t = group_task(start, count, (int)blockIdx.x);
+	const ApplyArgs a = tasks[2 * t], root = tasks[2 * t + 1];
+	hh_wy2_body(a, root, (int)blockIdx.x - start[t]);
 }
 
 int rows_per_thread(int rpb) { return rpb <= 8 * HW ? 8 : (rpb <= 16 * HW ? 16 : (rpb <= 24 * HW ? 24 : 32)); }
@@ -955,6 +976,200 @@ void hh_qc(const double* A, size_t m, size_t n, double* Q, double* C, int* jpvt,
 	lv.push_back(std::move(root));
 	// Q's column count is the rank, read on the device by every level
 	tree_apply(lv, (int)lv.size(), (int)n, nullptr, std::min(rootRows, (int)n), rank, Q, s);
+}
+
+
+namespace {
+// Factor launches of many levels: tasks bucketed by rows per thread, one grouped launch per bucket.
+void factor_group(const std::vector<std::pair<FactorArgs, int>>& tasks, bool piv, hipStream_t s) {
+	std::vector<FactorArgs> cls[4];
+	std::vector<int> st[4];
+	int nb[4] = {0, 0, 0, 0};
+	for (const auto& tb : tasks) {
+		const int rpt = rows_per_thread(tb.first.rpb);
+		const int c = rpt == 8 ? 0 : (rpt == 16 ? 1 : (rpt == 24 ? 2 : 3));
+		cls[c].push_back(tb.first);
+		st[c].push_back(nb[c]);
+		nb[c] += tb.second;
+	}
+	for (int c = 0; c < 4; ++c) {
+		if (cls[c].empty()) continue;
+		const FactorArgs* dt = static_cast<const FactorArgs*>(table_upload(cls[c].data(), cls[c].size() * sizeof(FactorArgs), s));
+		const int* ds = static_cast<const int*>(table_upload(st[c].data(), st[c].size() * sizeof(int), s));
+		const int cnt = (int)cls[c].size();
+		const dim3 g((unsigned)nb[c]), t(HT);
+		if (piv) {
+			if (c == 0) launch_timed("hh_factor_group_kernel<8,1>", Work(), hh_factor_group_kernel<8, true>, g, t, 0, s, dt, ds, cnt);
+			else if (c == 1) launch_timed("hh_factor_group_kernel<16,1>", Work(), hh_factor_group_kernel<16, true>, g, t, 0, s, dt, ds, cnt);
+			else if (c == 2) launch_timed("hh_factor_group_kernel<24,1>", Work(), hh_factor_group_kernel<24, true>, g, t, 0, s, dt, ds, cnt);
+			else launch_timed("hh_factor_group_kernel<32,1>", Work(), hh_factor_group_kernel<32, true>, g, t, 0, s, dt, ds, cnt);
+		} else {
+			if (c == 0) launch_timed("hh_factor_group_kernel<8,0>", Work(), hh_factor_group_kernel<8, false>, g, t, 0, s, dt, ds, cnt);
+			else if (c == 1) launch_timed("hh_factor_group_kernel<16,0>", Work(), hh_factor_group_kernel<16, false>, g, t, 0, s, dt, ds, cnt);
+			else if (c == 2) launch_timed("hh_factor_group_kernel<24,0>", Work(), hh_factor_group_kernel<24, false>, g, t, 0, s, dt, ds, cnt);
+			else launch_timed("hh_factor_group_kernel<32,0>", Work(), hh_factor_group_kernel<32, false>, g, t, 0, s, dt, ds, cnt);
+		}
+	}
+}
+// Q formation of many trees: a single level from the identity (hh_wy), or leaves under a single-block root
+// (hh_wy2); anything deeper one tree at a time.
+struct QTree {
+	std::vector<Level>* lv;
+	int n, nc;
+	const int* ncDev;
+	double* Q;
+};
+void qform_group(const std::vector<QTree>& trees, hipStream_t s) {
+	std::vector<ApplyArgs> one, two;
+	std::vector<int> s1, s2;
+	int b1 = 0, b2 = 0;
+	auto args = [](Level& L, int n, double* out, int nc, const int* ncDev) {
+		ApplyArgs a{};
+		a.V = L.V.get(); a.tau = L.tau.get(); a.Y = L.Y.get(); a.rows = L.rows; a.rpb = L.rpb; a.n = n;
+		a.init = nullptr; a.out = out; a.nc = nc; a.ncDev = ncDev;
+		return a;
+	};
+	for (const QTree& q : trees) {
+		std::vector<Level>& lv = *q.lv;
+		if (lv.size() == 1) {
+			one.push_back(args(lv[0], q.n, q.Q, q.nc, q.ncDev));
+			s1.push_back(b1);
+			b1 += lv[0].blocks;
+		} else if (lv.size() == 2 && lv[1].blocks == 1) {
+			two.push_back(args(lv[0], q.n, q.Q, q.nc, q.ncDev));
+			two.push_back(args(lv[1], q.n, nullptr, q.nc, q.ncDev));
+			s2.push_back(b2);
+			b2 += lv[0].blocks;
+		} else {
+			tree_apply(lv, (int)lv.size(), q.n, nullptr, q.nc, q.ncDev, q.Q, s);
+		}
+	}
+	if (!one.empty()) {
+		const ApplyArgs* dt = static_cast<const ApplyArgs*>(table_upload(one.data(), one.size() * sizeof(ApplyArgs), s));
+		const int* ds = static_cast<const int*>(table_upload(s1.data(), s1.size() * sizeof(int), s));
+		launch_timed("hh_wy_group_kernel", Work(), hh_wy_group_kernel, dim3((unsigned)b1), dim3(HT), 0, s, dt, ds, (int)one.size());
+	}
+	if (!two.empty()) {
+		const ApplyArgs* dt = static_cast<const ApplyArgs*>(table_upload(two.data(), two.size() * sizeof(ApplyArgs), s));
+		const int* ds = static_cast<const int*>(table_upload(s2.data(), s2.size() * sizeof(int), s));
+		launch_timed("hh_wy2_group_kernel", Work(), hh_wy2_group_kernel, dim3((unsigned)b2), dim3(HT), 0, s, dt, ds, (int)s2.size());
+	}
+}
+}  // namespace
+
+void hh_qr_group(const std::vector<QRJob>& jobs, hipStream_t s) {
+	const size_t J = jobs.size();
+	std::vector<std::vector<Level>> lv(J);
+	std::vector<const double*> in(J);
+	std::vector<int> rows(J);
+	std::vector<char> open(J, 1);
+	for (size_t j = 0; j < J; ++j) {
+		check_shape((size_t)jobs[j].m, (size_t)jobs[j].n);
+		in[j] = jobs[j].A;
+		rows[j] = jobs[j].m;
+	}
+	for (bool any = J > 0; any;) {  // one tree level of every open job per round
+		std::vector<std::pair<FactorArgs, int>> tasks;
+		for (size_t j = 0; j < J; ++j) {
+			if (!open[j]) continue;
+			const QRJob& q = jobs[j];
+			const int n = q.n;
+			Level L;
+			L.rows = rows[j];
+			level_geometry(rows[j], lv[j].empty(), n, L.blocks, L.rpb);
+			const bool root = L.blocks == 1;
+			FactorArgs f{};
+			f.in = in[j]; f.ld = lv[j].empty() ? (q.trans ? q.m : n) : n; f.trans = lv[j].empty() && q.trans;
+			f.rows = rows[j]; f.rpb = L.rpb; f.n = n;
+			if (q.Q) {
+				L.V.resize((size_t)rows[j] * n);
+				L.tau.resize((size_t)L.blocks * n);
+				L.Y.resize((size_t)L.blocks * n * n);
+				f.V = L.V.get(); f.tau = L.tau.get(); f.Y = L.Y.get();
+			}
+			double* R = q.R;
+			if (!root || !q.R) {
+				L.R.resize((size_t)L.blocks * n * n);
+				R = L.R.get();
+			}
+			f.R = R;
+			tasks.emplace_back(f, L.blocks);
+			lv[j].push_back(std::move(L));
+		}
+		factor_group(tasks, false, s);
+		any = false;
+		for (size_t j = 0; j < J; ++j) {
+			if (!open[j]) continue;
+			Level& L = lv[j].back();
+			if (L.blocks == 1) { open[j] = 0; continue; }
+			in[j] = L.R.get();
+			rows[j] = L.blocks * jobs[j].n;
+			any = true;
+		}
+	}
+	std::vector<QTree> trees;
+	for (size_t j = 0; j < J; ++j)
+		if (jobs[j].Q) trees.push_back(QTree{&lv[j], jobs[j].n, std::min(jobs[j].m, jobs[j].n), nullptr, jobs[j].Q});
+	qform_group(trees, s);
+}
+
+void hh_qc_group(const std::vector<QCJob>& jobs, hipStream_t s) {
+	const size_t J = jobs.size();
+	std::vector<std::vector<Level>> lv(J);
+	std::vector<const double*> rootIn(J);
+	std::vector<int> rootRows(J);
+	std::vector<char> grouped(J, 1);
+	// the levels below the root (matrices taller than one block), one level: leaves whose stacked R factors fit one block
+	std::vector<std::pair<FactorArgs, int>> leaves;
+	for (size_t j = 0; j < J; ++j) {
+		const QCJob& q = jobs[j];
+		check_shape((size_t)q.m, (size_t)q.n);
+		rootIn[j] = q.A;
+		rootRows[j] = q.m;
+		if (q.m <= HBR) continue;
+		Level L;
+		L.rows = q.m;
+		level_geometry(q.m, true, q.n, L.blocks, L.rpb);
+		if (L.blocks * q.n > HBR) { grouped[j] = 0; continue; }  // a deeper tree: one matrix at a time below
+		L.V.resize((size_t)q.m * q.n);
+		L.tau.resize((size_t)L.blocks * q.n);
+		L.Y.resize((size_t)L.blocks * q.n * q.n);
+		L.R.resize((size_t)L.blocks * q.n * q.n);
+		FactorArgs f{};
+		f.in = q.A; f.ld = q.n; f.rows = q.m; f.rpb = L.rpb; f.n = q.n;
+		f.V = L.V.get(); f.tau = L.tau.get(); f.Y = L.Y.get(); f.R = L.R.get();
+		leaves.emplace_back(f, L.blocks);
+		rootIn[j] = L.R.get();
+		rootRows[j] = L.blocks * q.n;
+		lv[j].push_back(std::move(L));
+	}
+	factor_group(leaves, false, s);
+	std::vector<std::pair<FactorArgs, int>> roots;
+	for (size_t j = 0; j < J; ++j) {
+		const QCJob& q = jobs[j];
+		if (!grouped[j]) {
+			hh_qc(q.A, (size_t)q.m, (size_t)q.n, q.Q, q.C, q.jpvt, q.rank, s);
+			continue;
+		}
+		Level root;
+		root.rows = root.rpb = rootRows[j];
+		root.blocks = 1;
+		root.V.resize((size_t)rootRows[j] * q.n);
+		root.tau.resize(q.n);
+		root.Y.resize((size_t)q.n * q.n);
+		FactorArgs f{};
+		f.in = rootIn[j]; f.ld = q.n; f.rows = f.rpb = rootRows[j]; f.n = q.n;
+		f.V = root.V.get(); f.tau = root.tau.get(); f.Y = root.Y.get();
+		f.jpvt = q.jpvt; f.rank = q.rank; f.C = q.C;
+		if (q.m > HBR) { f.orig = q.A; f.origRows = q.m; }
+		roots.emplace_back(f, 1);
+		lv[j].push_back(std::move(root));
+	}
+	factor_group(roots, true, s);
+	std::vector<QTree> trees;
+	for (size_t j = 0; j < J; ++j)
+		if (grouped[j]) trees.push_back(QTree{&lv[j], jobs[j].n, std::min(rootRows[j], jobs[j].n), jobs[j].rank, jobs[j].Q});
+	qform_group(trees, s);
 }
 
 }  // namespace xtt

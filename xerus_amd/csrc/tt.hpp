@@ -159,5 +159,10 @@ size_t als_local_cg(const double* L, const double* A, const double* R, size_t a,
 void als_round_batch(const ALSOptions& o, const std::vector<const TT*>& A, const std::vector<TT*>& x,
                      const std::vector<const TT*>& b, size_t sweeps, double eps, int threads, double* residuals,
                      ALSStats* total);
+// The same in lockstep (threads <= 0 in als_round_batch): every step of every problem is one grouped launch for all
+// problems. Supported for one-site ALS_SPD with an operator and FP64 local solves.
+bool lockstep_supported(const ALSOptions& o, const std::vector<const TT*>& A);
+void als_round_lockstep(const ALSOptions& o, const std::vector<const TT*>& A, const std::vector<TT*>& x,
+                        const std::vector<const TT*>& b, size_t sweeps, double eps, double* residuals, ALSStats* total);
 
 }  // namespace xtt

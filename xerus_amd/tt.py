@@ -522,8 +522,10 @@ class ALSVariant:
 
 def als_round_batch(variant: "ALSVariant", As, xs, bs, sweeps: int, eps: float, threads: int = 8, residuals: bool = True):
     """Many independent problems: problem p runs ``sweeps`` × (variant(As[p], xs[p], bs[p]), xs[p].round(eps)) on
-    one of ``threads`` worker threads, each with its own stream (xs updated in place). Returns the relative
-    residuals ‖A x − b‖/‖b‖ (or None) and the summed solver counters."""
+    one of ``threads`` worker threads, each with its own stream (xs updated in place). ``threads=0`` runs all problems
+    in lockstep instead: every step of every problem is one grouped launch (one-site ALS_SPD with an operator and
+    FP64 local solves; other options fall back to worker threads). Returns the relative residuals ‖A x − b‖/‖b‖ (or
+    None) and the summed solver counters."""
     n = len(xs)
     if not (len(As) == n and len(bs) == n):
         raise ValueError("As, xs and bs must have the same length")
