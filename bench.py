@@ -205,8 +205,7 @@ def adf_cpu_baseline(np, iterations):
 def bench_batch(xe, np, count, sweeps, threads, precision):
     """Many independent ALS_SPD + round problems of uneven size (d ~ U{6..20}, n ∈ {2,4,8,16}, r ~ U{4..64},
     rank-3 Laplacian + potential operators, rank-4 right-hand sides) through the batched driver: threads=0 is the
-    lockstep driver (one grouped launch per step for all problems, FP64 local solves), threads > 0 worker threads
-    with a stream each. Returns aggregate sweeps/s (count·sweeps / wall time, residuals included) and residual
+    lockstep driver (one grouped launch per step for all problems), threads > 0 worker threads with a stream each. Returns aggregate sweeps/s (count·sweeps / wall time, residuals included) and residual
     statistics."""
     from xerus_amd.workloads import batch_problems
     probs = batch_problems(count)
@@ -224,13 +223,14 @@ def bench_batch(xe, np, count, sweeps, threads, precision):
     worst = [{"problem": int(i), "d": int(shapes[i, 0]), "n": int(shapes[i, 1]), "r": int(shapes[i, 2]), "residual": float(res[i])}
              for i in np.argsort(res)[::-1][:5]]
     return {"problems": count, "sweeps_per_problem": sweeps,
-            "driver": "lockstep (grouped launches)" if threads == 0 else f"{threads} worker threads",
+            "driver": ("lockstep (grouped launches)" if precision in (xe.XT_F64, xe.XT_F32) else "16 worker threads (no lockstep for bf16)")
+            if threads <= 0 else f"{threads} worker threads",
             "precision": {xe.XT_F64: "f64", xe.XT_F32: "f32+f64-refinement", xe.XT_BF16: "bf16+f64-refinement"}[precision],
             "seconds": dt, "sweeps_per_s": count * sweeps / dt, "residual_max": float(np.max(res)),
             "residual_median": float(np.median(res)), "residual_p90": float(np.percentile(res, 90)),
             "above_1e-10": int((res > 1e-10).sum()), "worst": worst,
             "d_mean": float(shapes[:, 0].mean()), "n_mean": float(shapes[:, 1].mean()), "r_mean": float(shapes[:, 2].mean()),
-            "cg_iterations": st["cgIterations"], "local_solves": st["localSolves"]}
+            "cg_iterations": st["cgIterations"], "local_solves": st["localSolves"], "refinements": st["refinements"]}
 
 
 def cpu_baseline(np, steps):
@@ -383,12 +383,17 @@ def main():
 
     batch = None
     if args.batch_problems > 0:
-        batch = bench_batch(xe, np, args.batch_problems, args.batch_sweeps, 0, xe.XT_F64)
-        print(f"# batch (lockstep, f64): {batch['sweeps_per_s']:.1f} sweeps/s over {args.batch_problems} problems", file=sys.stderr, flush=True)
+        keys = ("driver", "precision", "seconds", "sweeps_per_s", "residual_max", "residual_median", "cg_iterations")
+        batch = bench_batch(xe, np, args.batch_problems, args.batch_sweeps, 0, prec)
+        print(f"# batch (lockstep, {batch['precision']}): {batch['sweeps_per_s']:.1f} sweeps/s over {args.batch_problems} problems",
+              file=sys.stderr, flush=True)
+        if prec != xe.XT_F64:
+            f64 = bench_batch(xe, np, args.batch_problems, args.batch_sweeps, 0, xe.XT_F64)
+            batch["lockstep_f64"] = {k: f64[k] for k in keys}
+            print(f"# batch (lockstep, f64): {f64['sweeps_per_s']:.1f} sweeps/s", file=sys.stderr, flush=True)
         if args.batch_threads > 0:
             ref = bench_batch(xe, np, args.batch_problems, args.batch_sweeps, args.batch_threads, prec)
-            batch["threads_reference"] = {k: ref[k] for k in ("driver", "precision", "seconds", "sweeps_per_s", "residual_max",
-                                                              "residual_median", "cg_iterations")}
+            batch["threads_reference"] = {k: ref[k] for k in keys}
             print(f"# batch ({ref['driver']}, {ref['precision']}): {ref['sweeps_per_s']:.1f} sweeps/s", file=sys.stderr, flush=True)
 
     cpu = None if args.no_cpu else cpu_baseline(np, args.steps)

@@ -221,7 +221,8 @@ int xt_perf_sink(xt_perf_callback cb, void* user);
  * per problem) receives ‖A x − b‖/‖b‖; st (optional) the summed counters.
  * threads <= 0 runs the problems in lockstep on the library stream: every step of every problem (stack pushes,
  * local products, CG steps, QR / QC moves, SVDs) is one grouped launch for all problems. It applies to one-site
- * ALS_SPD with an operator and FP64 local solves (XT_F64); other options fall back to 16 worker threads. */
+ * ALS_SPD with an operator and FP64 local solves (XT_F64) or FP32 CG inside FP64 refinement (XT_F32); other
+ * options fall back to 16 worker threads. */
 int xt_als_round_batch(const xt_als_options* o, size_t count, const xt_tt* A, const xt_tt* x, const xt_tt* b, size_t sweeps,
                        double eps, int threads, double* residuals, xt_als_stats* st);
 

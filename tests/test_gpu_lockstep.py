@@ -1,8 +1,9 @@
 """The lockstep batch driver (als_round_batch with threads=0): every step of every problem is one grouped launch.
 
-It runs the same per-problem algorithm as the threaded driver (one-site ALS_SPD with FP64 local CG, then round(eps)),
-so both drivers and the oracle are compared problem by problem on the uneven batch of configuration 4 (d ~ U{6..20},
-n ∈ {2,4,8,16}, r ~ U{4..64}, rank-3 Laplacian + potential, rank-4 right-hand sides).
+It runs the same per-problem algorithm as the threaded driver (one-site ALS_SPD with FP64 local CG, or FP32 CG inside
+FP64 iterative refinement, then round(eps)), so both drivers and the oracle are compared problem by problem on the
+uneven batch of configuration 4 (d ~ U{6..20}, n ∈ {2,4,8,16}, r ~ U{4..64}, rank-3 Laplacian + potential, rank-4
+right-hand sides).
 """
 import numpy as np
 import pytest
@@ -13,11 +14,11 @@ from test_gpu_reference_tests import round_parity
 pytestmark = pytest.mark.gpu
 
 
-def run_batch(xe, probs, threads):
+def run_batch(xe, probs, threads, precision=None):
     als = xe.ALS_SPD.copy()
     als.convergenceEpsilon = 0.0
     als.numHalfSweeps = 2
-    als.precision = xe.XT_F64
+    als.precision = xe.XT_F64 if precision is None else precision
     res, st = xe.als_round_batch(als, [p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs], 2, 1e-12,
                                  threads=threads)
     return np.asarray(res), st
@@ -88,8 +89,56 @@ def test_lockstep_config4_subset_matches_oracle(xe, oracle):
         assert abs(res[p] - ro) <= 0.1 * ro + 1e-11, p
 
 
+def test_lockstep_mixed_small_matches_threads(xe):
+    """FP32 CG inside FP64 refinement on small problems through both drivers. The FP32 reductions differ in summation
+    order (one workgroup per system against the chunked kernels), so each refinement's correction differs at FP32
+    rounding, which the refinement itself removes: TTs within 1e-9, equal ranks, residuals within 1e-6 relative."""
+    print("lockstep mixed small", flush=True)
+    from xerus_amd.workloads import batch_problems
+    kw = dict(d_range=(5, 8), n_choices=(2, 4), r_range=(3, 9))
+    pa, pb = batch_problems(8, **kw), batch_problems(8, **kw)
+    ra, sa = run_batch(xe, pa, 0, xe.XT_F32)
+    rb, sb = run_batch(xe, pb, 4, xe.XT_F32)
+    assert sa["localSolves"] == sb["localSolves"]
+    assert sa["refinements"] > 0
+    for (A, x, b, shape), (_, y, _, _), r1, r2 in zip(pa, pb, ra, rb):
+        assert x.ranks() == y.ranks(), shape
+        assert (x - y).frob_norm() <= 1e-9 * y.frob_norm(), shape
+        assert abs(r1 - r2) <= 1e-6 * r2 + 1e-13, shape
+
+
+def test_lockstep_mixed_config4_subset_matches_oracle(xe, oracle):
+    """Configuration 4 as specified (FP32 MFMA products with FP64 refinement): the first 12 problems through the
+    lockstep driver against the oracle's FP64 ALS_SPD + round(1e-12), 2 sweeps from the same start. The refinement
+    reaches the same local tolerance as the FP64 CG, so the bar is the FP64 one: TTs within 1e-8 + 10·ρ, residuals
+    within 10% of ρ (ρ the oracle's residual)."""
+    print("lockstep mixed config4 subset", flush=True)
+    from xerus_amd.workloads import batch_problems
+    count = 12
+    probs = batch_problems(count)
+    res, st = run_batch(xe, probs, 0, xe.XT_F32)
+    assert st["refinements"] > 0
+    for p, (A, x, b, (d, n, r)) in enumerate(probs):
+        rng = np.random.default_rng(SEED + p)
+        assert (int(rng.integers(6, 21)), int(rng.choice((2, 4, 8, 16))), int(rng.integers(4, 65))) == (d, n, r)
+        pots = rng.random((d, n))
+        Ao = oracle.TT.from_cores(laplace_cores(d, n, list(pots), rank3=True), op=True)
+        oracle.seed(SEED + p)
+        bo = oracle.TT.random([n] * d, [4] * (d - 1))
+        xo = oracle.TT.random([n] * d, [r] * (d - 1))
+        for _ in range(2):
+            oracle.als(Ao, xo, bo, num_half_sweeps=2, eps=0.0, local_solver="cg", cg_tol=1e-13)
+            xo.round(eps=1e-12)
+        ro = oracle.residual(Ao, xo, bo)
+        xot = xe.TTTensor.from_components(xo.cores())
+        diff = (x - xot).frob_norm() / xot.frob_norm()
+        print(f"  problem {p} d={d} n={n} r={r}: residual {res[p]:.3e} (oracle {ro:.3e}), rel diff {diff:.2e}", flush=True)
+        assert diff < 1e-8 + 10 * ro, p
+        assert abs(res[p] - ro) <= 0.1 * ro + 1e-11, p
+
+
 def test_lockstep_falls_back_for_other_options(xe):
-    """Options outside the lockstep path (here FP32 local matvecs) run on worker threads with the same results as an
+    """Options outside the lockstep path (here BF16 local matvecs) run on worker threads with the same results as an
     explicit thread count."""
     print("lockstep fallback", flush=True)
     from xerus_amd.workloads import batch_problems
@@ -98,7 +147,7 @@ def test_lockstep_falls_back_for_other_options(xe):
     als = xe.ALS_SPD.copy()
     als.convergenceEpsilon = 0.0
     als.numHalfSweeps = 2
-    als.precision = xe.XT_F32
+    als.precision = xe.XT_BF16
     ra, _ = xe.als_round_batch(als, [p[0] for p in pa], [p[1] for p in pa], [p[2] for p in pa], 1, 1e-12, threads=0)
     rb, _ = xe.als_round_batch(als, [p[0] for p in pb], [p[1] for p in pb], [p[2] for p in pb], 1, 1e-12, threads=2)
     for (A, x, b, shape), (_, y, _, _), r1, r2 in zip(pa, pb, ra, rb):

@@ -517,12 +517,19 @@ void launch_group(const std::vector<GemmTask>& tasks, hipStream_t s, GemmGroupPl
 
 void gemm_group_f64(const std::vector<GemmTask>& tasks, hipStream_t s) { launch_group<double>(tasks, s); }
 void gemm_group_plan_f64(const std::vector<GemmTask>& tasks, GemmGroupPlan& plan, hipStream_t s) { launch_group<double>(tasks, s, &plan); }
-void gemm_group_run_f64(const GemmGroupPlan& p, hipStream_t s) {
+void gemm_group_f32(const std::vector<GemmTask>& tasks, hipStream_t s) { launch_group<float>(tasks, s); }
+void gemm_group_plan_f32(const std::vector<GemmTask>& tasks, GemmGroupPlan& plan, hipStream_t s) { launch_group<float>(tasks, s, &plan); }
+namespace {
+template <class T>
+void run_group(const GemmGroupPlan& p, hipStream_t s) {
 	if (p.count == 0) return;
-	auto kern = gemm_group_kernel<double>;
-	launch_timed("gemm_group_kernel<double>", Work(p.flops, p.bytes), kern, dim3((unsigned)p.blocks), dim3(256), p.lds, s,
-	             static_cast<const GroupArgs*>(p.tasks), p.start, p.count);
+	auto kern = gemm_group_kernel<T>;
+	launch_timed(sizeof(T) == 8 ? "gemm_group_kernel<double>" : "gemm_group_kernel<float>", Work(p.flops, p.bytes), kern,
+	             dim3((unsigned)p.blocks), dim3(256), p.lds, s, static_cast<const GroupArgs*>(p.tasks), p.start, p.count);
 }
+}  // namespace
+void gemm_group_run_f64(const GemmGroupPlan& p, hipStream_t s) { run_group<double>(p, s); }
+void gemm_group_run_f32(const GemmGroupPlan& p, hipStream_t s) { run_group<float>(p, s); }
 
 void gemm_f64_pair(const GemmDesc& g0, const double* A0, const double* B0, double* C0, const GemmDesc& g1, const double* A1,
                    const double* B1, double* C1, hipStream_t s) {

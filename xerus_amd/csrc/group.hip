@@ -135,24 +135,26 @@ __device__ inline void block_sum2(double& x, double& y, double* red) {
 
 // r = b − q, p = r; rr = ‖r‖², rr0 = ‖b‖²; done when rr <= tol2·rr0 or no iteration is allowed. A warm start
 // worse than zero (rr > rr0) restarts from x = 0, r = p = b (CG's recursive residual cannot resolve the true one
-// below ε·‖A‖·‖x‖). The state's tol2 and maxIter are set beforehand.
-__global__ __launch_bounds__(CT) void cg_group_start_kernel(const CGTask* tasks) {
+// below ε·‖A‖·‖x‖). The state's tol2 and maxIter are set beforehand. q == nullptr: x = 0 (q = 0).
+template <class T>
+__global__ __launch_bounds__(CT) void cg_group_start_kernel(const CGTaskT<T>* tasks) {
 	__shared__ double red[2 * CT / 64];
-	const CGTask k = tasks[blockIdx.x];
+	const CGTaskT<T> k = tasks[blockIdx.x];
 	double rr = 0, bb = 0;
 	for (int i = threadIdx.x; i < k.n; i += CT) {
-		const double ri = k.b[i] - k.q[i];
+		const T ri = k.q ? k.b[i] - k.q[i] : k.b[i];
 		k.r[i] = ri;
 		k.p[i] = ri;
-		rr += ri * ri;
-		bb += k.b[i] * k.b[i];
+		if (!k.q) k.x[i] = T(0);
+		rr += (double)ri * (double)ri;
+		bb += (double)k.b[i] * (double)k.b[i];
 	}
 	block_sum2<CT>(rr, bb, red);
 	CGState* st = k.st;
 	const bool done = rr <= st->tol2 * bb || st->maxIter <= 0;
 	if (!done && rr > bb) {
 		for (int i = threadIdx.x; i < k.n; i += CT) {
-			k.x[i] = 0.0;
+			k.x[i] = T(0);
 			k.r[i] = k.b[i];
 			k.p[i] = k.b[i];
 		}
@@ -167,32 +169,34 @@ __global__ __launch_bounds__(CT) void cg_group_start_kernel(const CGTask* tasks)
 }
 
 // One CG step after q = A·p: α = rr/(p·q), x += α·p, r −= α·q, then the stop test and p = r + β·p.
-__global__ __launch_bounds__(CT) void cg_group_step_kernel(const CGTask* tasks) {
+template <class T>
+__global__ __launch_bounds__(CT) void cg_group_step_kernel(const CGTaskT<T>* tasks) {
 	__shared__ double red[2 * CT / 64];
-	const CGTask k = tasks[blockIdx.x];
+	const CGTaskT<T> k = tasks[blockIdx.x];
 	CGState* st = k.st;
 	if (st->done) return;
 	double pq = 0, z = 0;
-	for (int i = threadIdx.x; i < k.n; i += CT) pq += k.p[i] * k.q[i];
+	for (int i = threadIdx.x; i < k.n; i += CT) pq += (double)k.p[i] * (double)k.q[i];
 	block_sum2<CT>(pq, z, red);
 	if (!(pq > 0)) {  // breakdown (non-positive curvature or exact solution): stop where we are
 		if (threadIdx.x == 0) { st->alpha = 0; st->done = 1; }
 		return;
 	}
 	const double rrOld = st->rr, alpha = rrOld / pq;
+	const T al = (T)alpha;
 	double rr = 0;
 	z = 0;
 	for (int i = threadIdx.x; i < k.n; i += CT) {
-		k.x[i] += alpha * k.p[i];
-		const double ri = k.r[i] - alpha * k.q[i];
+		k.x[i] += al * k.p[i];
+		const T ri = k.r[i] - al * k.q[i];
 		k.r[i] = ri;
-		rr += ri * ri;
+		rr += (double)ri * (double)ri;
 	}
 	block_sum2<CT>(rr, z, red);
 	const int iter = st->iter + 1;
 	const bool done = rr <= st->tol2 * st->rr0 || iter >= st->maxIter;
 	if (!done) {
-		const double beta = rr / rrOld;
+		const T beta = (T)(rr / rrOld);
 		for (int i = threadIdx.x; i < k.n; i += CT) k.p[i] = k.r[i] + beta * k.p[i];
 	}
 	__syncthreads();  // every thread has read st before it changes
@@ -202,6 +206,37 @@ __global__ __launch_bounds__(CT) void cg_group_step_kernel(const CGTask* tasks) 
 		st->iter = iter;
 		st->done = done ? 1 : 0;
 	}
+}
+
+__global__ __launch_bounds__(GT) void mix_group_kernel(const MixTask* tasks, const int* start, int count) {
+	const int t = group_task(start, count, (int)blockIdx.x);
+	const MixTask m = tasks[t];
+	const long long e0 = (long long)(blockIdx.x - start[t]) * GE;
+	const long long e1 = min(m.n, e0 + GE);
+	for (long long e = e0 + threadIdx.x; e < e1; e += GT) {
+		if (m.op == MIX_CVT32) {
+			static_cast<float*>(m.out)[e] = (float)(m.alpha * static_cast<const double*>(m.a)[e]);
+		} else if (m.op == MIX_AXPY32) {
+			static_cast<double*>(m.out)[e] += m.alpha * (double)static_cast<const float*>(m.a)[e];
+		} else {  // MIX_ZERO
+			static_cast<double*>(m.out)[e] = 0.0;
+			static_cast<double*>(m.out2)[e] = static_cast<const double*>(m.a)[e];
+		}
+	}
+}
+
+__global__ __launch_bounds__(CT) void resid_group_kernel(const ResidTask* tasks, double* out) {
+	__shared__ double red[2 * CT / 64];
+	const ResidTask k = tasks[blockIdx.x];
+	double rr = 0, bb = 0;
+	for (int i = threadIdx.x; i < k.n; i += CT) {
+		const double ri = k.b[i] - k.q[i];
+		k.r[i] = ri;
+		rr += ri * ri;
+		bb += k.b[i] * k.b[i];
+	}
+	block_sum2<CT>(rr, bb, red);
+	if (threadIdx.x == 0) { out[2 * blockIdx.x] = rr; out[2 * blockIdx.x + 1] = bb; }
 }
 
 // out[2t] = a·b, out[2t + 1] = c·d (c may be null: 0)
@@ -277,19 +312,53 @@ MapTask map_fill(double* out, double v, size_t n) {
 	return m;
 }
 
-void cg_group_start(const std::vector<CGTask>& tasks, hipStream_t s) {
+template <class T>
+void cg_group_start(const std::vector<CGTaskT<T>>& tasks, hipStream_t s) {
 	if (tasks.empty()) return;
-	const CGTask* dt = static_cast<const CGTask*>(table_upload(tasks.data(), tasks.size() * sizeof(CGTask), s));
-	launch_timed("cg_group_start_kernel", Work(0.0, 0.0), cg_group_start_kernel, dim3((unsigned)tasks.size()), dim3(CT), 0, s, dt);
+	const CGTaskT<T>* dt = static_cast<const CGTaskT<T>*>(table_upload(tasks.data(), tasks.size() * sizeof(CGTaskT<T>), s));
+	launch_timed(sizeof(T) == 8 ? "cg_group_start_kernel<double>" : "cg_group_start_kernel<float>", Work(0.0, 0.0), cg_group_start_kernel<T>,
+	             dim3((unsigned)tasks.size()), dim3(CT), 0, s, dt);
 }
-void cg_group_step_run(const CGTask* devTasks, int count, hipStream_t s) {
-	launch_timed("cg_group_step_kernel", Work(0.0, 0.0), cg_group_step_kernel, dim3((unsigned)count), dim3(CT), 0, s, devTasks);
+template <class T>
+void cg_group_step_run(const CGTaskT<T>* devTasks, int count, hipStream_t s) {
+	launch_timed(sizeof(T) == 8 ? "cg_group_step_kernel<double>" : "cg_group_step_kernel<float>", Work(0.0, 0.0), cg_group_step_kernel<T>,
+	             dim3((unsigned)count), dim3(CT), 0, s, devTasks);
 }
-void cg_group_step(const std::vector<CGTask>& tasks, hipStream_t s) {
+template <class T>
+void cg_group_step(const std::vector<CGTaskT<T>>& tasks, hipStream_t s) {
 	if (tasks.empty()) return;
-	const CGTask* dt = static_cast<const CGTask*>(table_upload(tasks.data(), tasks.size() * sizeof(CGTask), s));
-	launch_timed("cg_group_step_kernel", Work(0.0, 0.0), cg_group_step_kernel, dim3((unsigned)tasks.size()), dim3(CT), 0, s, dt);
+	const CGTaskT<T>* dt = static_cast<const CGTaskT<T>*>(table_upload(tasks.data(), tasks.size() * sizeof(CGTaskT<T>), s));
+	cg_group_step_run<T>(dt, (int)tasks.size(), s);
 }
+template void cg_group_start<double>(const std::vector<CGTaskT<double>>&, hipStream_t);
+template void cg_group_start<float>(const std::vector<CGTaskT<float>>&, hipStream_t);
+template void cg_group_step<double>(const std::vector<CGTaskT<double>>&, hipStream_t);
+template void cg_group_step<float>(const std::vector<CGTaskT<float>>&, hipStream_t);
+template void cg_group_step_run<double>(const CGTaskT<double>*, int, hipStream_t);
+template void cg_group_step_run<float>(const CGTaskT<float>*, int, hipStream_t);
+
+void mix_group(const std::vector<MixTask>& tasks, hipStream_t s) {
+	std::vector<MixTask> live;
+	std::vector<int> start;
+	int blocks = 0;
+	for (const MixTask& m : tasks) {
+		if (m.n <= 0) continue;
+		live.push_back(m);
+		start.push_back(blocks);
+		blocks += (int)((m.n + GE - 1) / GE);
+	}
+	if (live.empty()) return;
+	const MixTask* dt = static_cast<const MixTask*>(table_upload(live.data(), live.size() * sizeof(MixTask), s));
+	const int* ds = static_cast<const int*>(table_upload(start.data(), start.size() * sizeof(int), s));
+	launch_timed("mix_group_kernel", Work(0.0, 0.0), mix_group_kernel, dim3(blocks), dim3(GT), 0, s, dt, ds, (int)live.size());
+}
+
+void resid_group(const std::vector<ResidTask>& tasks, double* out, hipStream_t s) {
+	if (tasks.empty()) return;
+	const ResidTask* dt = static_cast<const ResidTask*>(table_upload(tasks.data(), tasks.size() * sizeof(ResidTask), s));
+	launch_timed("resid_group_kernel", Work(0.0, 0.0), resid_group_kernel, dim3((unsigned)tasks.size()), dim3(CT), 0, s, dt, out);
+}
+
 void dot2_group(const std::vector<Dot2Task>& tasks, double* out, hipStream_t s) {
 	if (tasks.empty()) return;
 	const Dot2Task* dt = static_cast<const Dot2Task*>(table_upload(tasks.data(), tasks.size() * sizeof(Dot2Task), s));

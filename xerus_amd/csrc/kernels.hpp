@@ -199,19 +199,31 @@ MapTask map_eye(double* out, size_t n);
 MapTask map_fill(double* out, double v, size_t n);
 void map_group(const std::vector<MapTask>& tasks, hipStream_t s);
 
-// FP64 CG of many local systems, one workgroup each; q = A·p (or A·x before the start) comes from grouped GEMMs.
-struct CGTask {
+// CG of many local systems, one workgroup each (vectors in T = double or float, reductions in FP64); q = A·p (or
+// A·x before the start) comes from grouped GEMMs. q == nullptr at the start means x = 0 (q = 0).
+template <class T>
+struct CGTaskT {
 	int n;
-	double* x;
-	double* r;
-	double* p;
-	const double* q;
-	const double* b;
+	T* x;
+	T* r;
+	T* p;
+	const T* q;
+	const T* b;
 	CGState* st;  // tol2 and maxIter set beforehand (cg_state_init semantics)
 };
-void cg_group_start(const std::vector<CGTask>& tasks, hipStream_t s);  // as cg_start followed by the zero-start rule
-void cg_group_step(const std::vector<CGTask>& tasks, hipStream_t s);   // cg_alpha, cg_update and cg_direction
-void cg_group_step_run(const CGTask* devTasks, int count, hipStream_t s);  // the same on a table already on the device
+using CGTask = CGTaskT<double>;
+template <class T> void cg_group_start(const std::vector<CGTaskT<T>>& tasks, hipStream_t s);  // cg_start + the zero-start rule
+template <class T> void cg_group_step(const std::vector<CGTaskT<T>>& tasks, hipStream_t s);   // cg_alpha, cg_update, cg_direction
+template <class T> void cg_group_step_run(const CGTaskT<T>* devTasks, int count, hipStream_t s);  // on a table already on the device
+// Mixed-precision plumbing of the refinement loop, many vectors per launch:
+//   MIX_CVT32:  out (float) = alpha · a (double);  MIX_AXPY32: out (double) += alpha · a (float);
+//   MIX_ZERO:   out (double) = 0, out2 (double) = a (double) (a warm start worse than zero: x = 0, r = b).
+enum MixOp { MIX_CVT32 = 0, MIX_AXPY32 = 1, MIX_ZERO = 2 };
+struct MixTask { int op; long long n; const void* a; void* out; void* out2; double alpha; };
+void mix_group(const std::vector<MixTask>& tasks, hipStream_t s);
+// r = b − q with out[2t] = ‖r‖², out[2t + 1] = ‖b‖² (FP64), one workgroup per task
+struct ResidTask { int n; double* r; const double* b; const double* q; };
+void resid_group(const std::vector<ResidTask>& tasks, double* out, hipStream_t s);
 struct Dot2Task { int n; const double* a; const double* b; const double* c; const double* d; };
 void dot2_group(const std::vector<Dot2Task>& tasks, double* out, hipStream_t s);  // out[2t] = a·b, out[2t+1] = c·d
 
@@ -230,6 +242,10 @@ struct GemmGroupPlan {
 };
 void gemm_group_plan_f64(const std::vector<GemmTask>& tasks, GemmGroupPlan& plan, hipStream_t s);
 void gemm_group_run_f64(const GemmGroupPlan& plan, hipStream_t s);
+// the same with FP32 operands (v_mfma_f32_16x16x4_f32)
+void gemm_group_f32(const std::vector<GemmTask>& tasks, hipStream_t s);
+void gemm_group_plan_f32(const std::vector<GemmTask>& tasks, GemmGroupPlan& plan, hipStream_t s);
+void gemm_group_run_f32(const GemmGroupPlan& plan, hipStream_t s);
 // Householder QR of many matrices (each as hh_qr): Q (m × k) and R (k × n), k = min(m, n), n <= 64.
 struct QRJob { const double* A; int m, n; bool trans; double* Q; double* R; };
 void hh_qr_group(const std::vector<QRJob>& jobs, hipStream_t s);

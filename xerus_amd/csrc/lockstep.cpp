@@ -4,14 +4,15 @@
 // kernels). Shapes differ from problem to problem (uneven d, n and ranks); each launch carries a task table.
 // Host decisions (ranks, convergence, energies, the stop rule) are read back once per step for all problems.
 //
-// The per-problem algorithm is the one of als_solve (one site, SPD operator, FP64 local CG with the same
-// tolerance, iteration cap and zero-start rule) and round (QC sweep to the last core, then round_edge from the
-// right with the SVD truncation rule). Trajectories agree with the per-problem drivers to the local solver's
-// tolerance; rank decisions are made by the same kernels and rules.
+// The per-problem algorithm is the one of als_solve (one site, SPD operator, FP64 local CG, or FP32 CG inside FP64
+// iterative refinement, with the same tolerances, iteration caps and zero-start rule) and round (QC sweep to the
+// last core, then round_edge from the right with the SVD truncation rule). Trajectories agree with the per-problem
+// drivers to the local solver's tolerance; rank decisions are made by the same kernels and rules.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -37,7 +38,7 @@ struct Lock {
 	size_t halfSweeps = 0;
 	bool canonAtEnd = true;
 	size_t corePosAtEnd = 0;
-	size_t localSolves = 0, cgIters = 0;
+	size_t localSolves = 0, cgIters = 0, refinements = 0;
 };
 
 DTensor ones_dev(const dims_t& dims, std::vector<MapTask>& maps) {
@@ -223,10 +224,13 @@ void site_init(Site& S, Lock* L) {
 }
 // y = A_loc·v for every site, three grouped launches (the first two also compute the local right-hand sides
 // when rhs is set); skip[j] (optional): site j's products are skipped while *skip[j] != 0
+// (sel: only the listed sites)
 void local_apply(std::vector<Site>& S, const std::vector<const double*>& v, const std::vector<double*>& y, bool rhs,
-                 const std::vector<const int*>* skip, hipStream_t s) {
+                 const std::vector<const int*>* skip, hipStream_t s, const std::vector<size_t>* sel = nullptr) {
 	std::vector<GemmTask> g1, g2, g3;
-	for (size_t j = 0; j < S.size(); ++j) {
+	const size_t cnt = sel ? sel->size() : S.size();
+	for (size_t i = 0; i < cnt; ++i) {
+		const size_t j = sel ? (*sel)[i] : i;
 		Site& t = S[j];
 		const int* sk = skip ? (*skip)[j] : nullptr;
 		GemmDesc d1 = gemm_desc(0, 0, t.a * t.P, t.n * t.c, t.a); d1.skip = sk;
@@ -280,72 +284,51 @@ void energies(std::vector<Lock*>& Ls, std::vector<double>& E, hipStream_t s) {
 	for (size_t j = 0; j < Ls.size(); ++j) E[j] = std::abs(0.5 * h[2 * j + 1] - h[2 * j]);
 }
 
-// FP64 CG on every lock's local system (warm start from its core, which is updated in place)
-void local_solves(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t s) {
-	if (Ls.empty()) return;
-	const size_t J = Ls.size();
-	std::vector<Site> S(J);
-	std::vector<DBuf<double>> r(J), p(J), q(J);
-	DBuf<CGState> st(J);
-	std::vector<CGState> init(J);
-	std::vector<CGTask> tasks(J);
-	std::vector<const double*> xv(J), pv(J);
-	std::vector<double*> qv(J);
-	std::vector<const int*> skip(J);
-	for (size_t j = 0; j < J; ++j) {
-		site_init(S[j], Ls[j]);
-		const size_t N = S[j].a * S[j].n * S[j].c;
-		r[j].resize(N); p[j].resize(N); q[j].resize(N);
-		std::memset(&init[j], 0, sizeof(CGState));
-		init[j].tol2 = o.localTol * o.localTol;
-		init[j].maxIter = o.maxLocalIter;
-		double* x = Ls[j]->x->cores[Ls[j]->curr].data();
-		tasks[j] = CGTask{(int)N, x, r[j].get(), p[j].get(), q[j].get(), S[j].bl.data(), st.get() + j};
-		xv[j] = x; pv[j] = p[j].get(); qv[j] = q[j].get();
-		skip[j] = &st.get()[j].done;
-		Ls[j]->localSolves++;
-	}
-	const void* dInit = table_upload(init.data(), J * sizeof(CGState), s);
-	XTT_HIP(hipMemcpyAsync(st.get(), dInit, J * sizeof(CGState), hipMemcpyDeviceToDevice, s));
-	local_apply(S, xv, qv, true, nullptr, s);  // q = A_loc·x and the right-hand sides
-	cg_group_start(tasks, s);
-	// the three products of q = A_loc·p and the CG step for the systems still iterating, their tables kept on the
-	// device for every iteration of a chunk; the products of a system that converges inside a chunk exit at once
-	// (skip flag), and the tables are rebuilt without it once the open set has shrunk by a quarter
+void gemm_plan(const std::vector<GemmTask>& t, GemmGroupPlan& p, hipStream_t s, double) { gemm_group_plan_f64(t, p, s); }
+void gemm_plan(const std::vector<GemmTask>& t, GemmGroupPlan& p, hipStream_t s, float) { gemm_group_plan_f32(t, p, s); }
+void gemm_run(const GemmGroupPlan& p, hipStream_t s, double) { gemm_group_run_f64(p, s); }
+void gemm_run(const GemmGroupPlan& p, hipStream_t s, float) { gemm_group_run_f32(p, s); }
+
+// The three products of q = A_loc·p for system k (its skip flag given), appended to the three launch lists
+using ProdFn = std::function<void(size_t k, const int* skip, std::vector<GemmTask>&, std::vector<GemmTask>&, std::vector<GemmTask>&)>;
+
+// CG steps for every system of `tasks` (started already; state k in st[k]) until all are done or maxIter steps ran;
+// h receives the final states. The products and the CG step of the systems still iterating keep their tables on
+// the device for every iteration of a chunk; the products of a system that converges inside a chunk exit at once
+// (skip flag), and the tables are rebuilt without it once the open set has shrunk by a quarter.
+template <class T>
+void cg_iterate(const std::vector<CGTaskT<T>>& tasks, CGState* st, int maxIter, const ProdFn& prod, std::vector<CGState>& h,
+                hipStream_t s) {
+	const size_t K = tasks.size();
 	GemmGroupPlan g1, g2, g3;
-	DBuf<CGTask> dTasks;
+	DBuf<CGTaskT<T>> dTasks;
 	int planned = 0;
 	auto plan = [&](const std::vector<size_t>& open) {
 		std::vector<GemmTask> t1, t2, t3;
-		std::vector<CGTask> tk;
-		for (size_t j : open) {
-			Site& t = S[j];
-			GemmDesc d1 = gemm_desc(0, 0, t.a * t.P, t.n * t.c, t.a); d1.skip = skip[j];
-			t1.push_back(gt(d1, t.Lst->data(), pv[j], t.T1.get()));
-			t2.push_back(gt(stage2(t.a, t.n, t.c, t.P, t.Q, skip[j]), t.Am->data(), t.T1.get(), t.T2.get()));
-			GemmDesc d3 = gemm_desc(0, 1, t.a * t.n, t.c, t.Q * t.c); d3.skip = skip[j];
-			t3.push_back(gt(d3, t.T2.get(), t.Rst->data(), qv[j]));
-			tk.push_back(tasks[j]);
+		std::vector<CGTaskT<T>> tk;
+		for (size_t k : open) {
+			prod(k, &st[k].done, t1, t2, t3);
+			tk.push_back(tasks[k]);
 		}
-		gemm_group_plan_f64(t1, g1, s);  // each plan also runs once: the first product of the next iteration
-		gemm_group_plan_f64(t2, g2, s);
-		gemm_group_plan_f64(t3, g3, s);
+		gemm_plan(t1, g1, s, T());  // each plan also runs once: the first product of the next iteration
+		gemm_plan(t2, g2, s, T());
+		gemm_plan(t3, g3, s, T());
 		dTasks.resize(tk.size());
-		table_upload_to(dTasks.get(), tk.data(), tk.size() * sizeof(CGTask), s);
-		cg_group_step_run(dTasks.get(), (int)tk.size(), s);
+		table_upload_to(dTasks.get(), tk.data(), tk.size() * sizeof(CGTaskT<T>), s);
+		cg_group_step_run<T>(dTasks.get(), (int)tk.size(), s);
 		planned = (int)open.size();
 	};
-	std::vector<size_t> open(J);
-	for (size_t j = 0; j < J; ++j) open[j] = j;
+	std::vector<size_t> open(K);
+	for (size_t k = 0; k < K; ++k) open[k] = k;
 	plan(open);
-	std::vector<CGState> h(J);
+	h.resize(K);
 	int done = 1, chunk = 2;  // iterations run so far; chunks grow 2, 4, 8, 16
 	for (;;) {
-		read_back(h.data(), st.get(), J * sizeof(CGState), s);
+		read_back(h.data(), st, K * sizeof(CGState), s);
 		std::vector<size_t> still;
-		for (size_t j = 0; j < J; ++j) if (!h[j].done) still.push_back(j);
-		if (still.empty() || done >= o.maxLocalIter) break;
-		int n = std::min(chunk, o.maxLocalIter - done);
+		for (size_t k = 0; k < K; ++k) if (!h[k].done) still.push_back(k);
+		if (still.empty() || done >= maxIter) break;
+		int n = std::min(chunk, maxIter - done);
 		if (4 * still.size() <= 3 * (size_t)planned) {
 			table_ring_release(s);
 			plan(still);
@@ -353,16 +336,168 @@ void local_solves(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t s) {
 			done += 1;
 		}
 		for (int c = 0; c < n; ++c) {
-			gemm_group_run_f64(g1, s);
-			gemm_group_run_f64(g2, s);
-			gemm_group_run_f64(g3, s);
-			cg_group_step_run(dTasks.get(), planned, s);
+			gemm_run(g1, s, T());
+			gemm_run(g2, s, T());
+			gemm_run(g3, s, T());
+			cg_group_step_run<T>(dTasks.get(), planned, s);
 		}
 		done += n;
 		chunk = std::min(16, 2 * chunk);
 	}
 	table_ring_release(s);
+}
+
+DBuf<CGState> cg_states(size_t K, double tol2, int maxIter, hipStream_t s) {
+	DBuf<CGState> st(K);
+	std::vector<CGState> init(K);
+	for (CGState& c : init) {
+		std::memset(&c, 0, sizeof(CGState));
+		c.tol2 = tol2;
+		c.maxIter = maxIter;
+	}
+	const void* dInit = table_upload(init.data(), K * sizeof(CGState), s);
+	XTT_HIP(hipMemcpyAsync(st.get(), dInit, K * sizeof(CGState), hipMemcpyDeviceToDevice, s));
+	return st;
+}
+
+// FP64 CG on every lock's local system (warm start from its core, which is updated in place)
+void local_solves_f64(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t s) {
+	const size_t J = Ls.size();
+	std::vector<Site> S(J);
+	std::vector<DBuf<double>> r(J), p(J), q(J);
+	std::vector<CGTask> tasks(J);
+	std::vector<const double*> xv(J);
+	std::vector<double*> qv(J);
+	DBuf<CGState> st = cg_states(J, o.localTol * o.localTol, o.maxLocalIter, s);
+	for (size_t j = 0; j < J; ++j) {
+		site_init(S[j], Ls[j]);
+		const size_t N = S[j].a * S[j].n * S[j].c;
+		r[j].resize(N); p[j].resize(N); q[j].resize(N);
+		double* x = Ls[j]->x->cores[Ls[j]->curr].data();
+		tasks[j] = CGTask{(int)N, x, r[j].get(), p[j].get(), q[j].get(), S[j].bl.data(), st.get() + j};
+		xv[j] = x; qv[j] = q[j].get();
+		Ls[j]->localSolves++;
+	}
+	local_apply(S, xv, qv, true, nullptr, s);  // q = A_loc·x and the right-hand sides
+	cg_group_start(tasks, s);
+	const ProdFn prod = [&](size_t j, const int* skip, std::vector<GemmTask>& t1, std::vector<GemmTask>& t2, std::vector<GemmTask>& t3) {
+		Site& t = S[j];
+		GemmDesc d1 = gemm_desc(0, 0, t.a * t.P, t.n * t.c, t.a); d1.skip = skip;
+		t1.push_back(gt(d1, t.Lst->data(), p[j].get(), t.T1.get()));
+		t2.push_back(gt(stage2(t.a, t.n, t.c, t.P, t.Q, skip), t.Am->data(), t.T1.get(), t.T2.get()));
+		GemmDesc d3 = gemm_desc(0, 1, t.a * t.n, t.c, t.Q * t.c); d3.skip = skip;
+		t3.push_back(gt(d3, t.T2.get(), t.Rst->data(), q[j].get()));
+	};
+	std::vector<CGState> h;
+	cg_iterate<double>(tasks, st.get(), o.maxLocalIter, prod, h, s);
 	for (size_t j = 0; j < J; ++j) Ls[j]->cgIters += (size_t)std::max(0, h[j].iter);
+}
+
+// Mixed precision (als.cpp's cg_mixed, system by system): FP64 iterative refinement around an FP32 CG. Each round
+// forms r = b − A·x in FP64 for the systems still open (one grouped residual, one read-back), stops a system when
+// ‖r‖ <= tol·‖b‖ or the refinement cap is reached, restarts from x = 0 where the warm start is worse than zero,
+// solves A·e = r/‖r‖ from e = 0 in FP32 to the inner tolerance and adds ‖r‖·e to x in FP64.
+void local_solves_mixed(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t s) {
+	const size_t J = Ls.size();
+	const double inner = o.innerTol > 0 ? o.innerTol : 1e-5, tol2 = o.localTol * o.localTol;
+	struct Low {
+		DBuf<float> L, Am, R, rhs, e, r, p, q, T1, T2;
+		bool staged = false;
+	};
+	std::vector<Site> S(J);
+	std::vector<Low> lw(J);
+	std::vector<DBuf<double>> r(J), q(J);
+	std::vector<const double*> xv(J);
+	std::vector<double*> qv(J);
+	std::vector<size_t> N(J), open(J);
+	for (size_t j = 0; j < J; ++j) {
+		site_init(S[j], Ls[j]);
+		N[j] = S[j].a * S[j].n * S[j].c;
+		r[j].resize(N[j]); q[j].resize(N[j]);
+		xv[j] = Ls[j]->x->cores[Ls[j]->curr].data();
+		qv[j] = q[j].get();
+		open[j] = j;
+		Ls[j]->localSolves++;
+	}
+	local_apply(S, xv, qv, true, nullptr, s);  // q = A_loc·x and the right-hand sides
+	DBuf<double> norms(2 * J);
+	std::vector<double> hn(2 * J);
+	for (int ref = 0;; ++ref) {
+		std::vector<ResidTask> rt;
+		for (size_t j : open) rt.push_back(ResidTask{(int)N[j], r[j].get(), S[j].bl.data(), q[j].get()});
+		resid_group(rt, norms.get(), s);
+		read_back(hn.data(), norms.get(), 2 * open.size() * sizeof(double), s);
+		table_ring_release(s);
+		std::vector<size_t> next;
+		std::vector<double> rn;
+		std::vector<MixTask> mix;
+		for (size_t i = 0; i < open.size(); ++i) {
+			const size_t j = open[i];
+			double rr = hn[2 * i];
+			const double bb = hn[2 * i + 1];
+			if (rr <= tol2 * bb || ref == o.maxRefine) continue;
+			double* x = Ls[j]->x->cores[Ls[j]->curr].data();
+			const double* src = r[j].get();
+			if (ref == 0 && rr > bb) {  // the warm start is worse than zero: refine from x = 0 (r = b)
+				mix.push_back(MixTask{MIX_ZERO, (long long)N[j], S[j].bl.data(), x, r[j].get(), 0.0});
+				src = S[j].bl.data();
+				rr = bb;
+			}
+			Low& w = lw[j];
+			Site& t = S[j];
+			if (!w.staged) {
+				w.L.resize(t.Lst->size()); w.Am.resize(t.Am->size()); w.R.resize(t.Rst->size());
+				w.rhs.resize(N[j]); w.e.resize(N[j]); w.r.resize(N[j]); w.p.resize(N[j]); w.q.resize(N[j]);
+				w.T1.resize(t.a * t.P * t.n * t.c); w.T2.resize(t.a * t.n * t.Q * t.c);
+				mix.push_back(MixTask{MIX_CVT32, (long long)t.Lst->size(), t.Lst->data(), w.L.get(), nullptr, 1.0});
+				mix.push_back(MixTask{MIX_CVT32, (long long)t.Am->size(), t.Am->data(), w.Am.get(), nullptr, 1.0});
+				mix.push_back(MixTask{MIX_CVT32, (long long)t.Rst->size(), t.Rst->data(), w.R.get(), nullptr, 1.0});
+				w.staged = true;
+			}
+			mix.push_back(MixTask{MIX_CVT32, (long long)N[j], src, w.rhs.get(), nullptr, 1.0 / std::sqrt(rr)});
+			next.push_back(j);
+			rn.push_back(std::sqrt(rr));
+		}
+		if (next.empty()) break;
+		mix_group(mix, s);
+		const size_t K = next.size();
+		DBuf<CGState> st = cg_states(K, inner * inner, o.maxLocalIter, s);
+		std::vector<CGTaskT<float>> tasks(K), start(K);
+		for (size_t k = 0; k < K; ++k) {
+			Low& w = lw[next[k]];
+			tasks[k] = CGTaskT<float>{(int)N[next[k]], w.e.get(), w.r.get(), w.p.get(), w.q.get(), w.rhs.get(), st.get() + k};
+			start[k] = tasks[k];
+			start[k].q = nullptr;  // e = 0
+			Ls[next[k]]->refinements++;
+		}
+		cg_group_start(start, s);
+		const ProdFn prod = [&](size_t k, const int* skip, std::vector<GemmTask>& t1, std::vector<GemmTask>& t2, std::vector<GemmTask>& t3) {
+			Site& t = S[next[k]];
+			Low& w = lw[next[k]];
+			GemmDesc d1 = gemm_desc(0, 0, t.a * t.P, t.n * t.c, t.a); d1.skip = skip;
+			t1.push_back(GemmTask{d1, w.L.get(), w.p.get(), w.T1.get()});
+			t2.push_back(GemmTask{stage2(t.a, t.n, t.c, t.P, t.Q, skip), w.Am.get(), w.T1.get(), w.T2.get()});
+			GemmDesc d3 = gemm_desc(0, 1, t.a * t.n, t.c, t.Q * t.c); d3.skip = skip;
+			t3.push_back(GemmTask{d3, w.T2.get(), w.R.get(), w.q.get()});
+		};
+		std::vector<CGState> h;
+		cg_iterate<float>(tasks, st.get(), o.maxLocalIter, prod, h, s);
+		mix.clear();
+		for (size_t k = 0; k < K; ++k) {
+			const size_t j = next[k];
+			Ls[j]->cgIters += (size_t)std::max(0, h[k].iter);
+			mix.push_back(MixTask{MIX_AXPY32, (long long)N[j], lw[j].e.get(), Ls[j]->x->cores[Ls[j]->curr].data(), nullptr, rn[k]});
+		}
+		mix_group(mix, s);  // x += ‖r‖·e
+		local_apply(S, xv, qv, false, nullptr, s, &next);  // q = A_loc·x
+		open = std::move(next);
+	}
+	table_ring_release(s);
+}
+
+void local_solves(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t s) {
+	if (Ls.empty()) return;
+	if (o.precision == P_F64) local_solves_f64(Ls, o, s); else local_solves_mixed(Ls, o, s);
 }
 
 // ---- core moves: QR to the right (x_k = Q, x_{k+1} = R·x_{k+1}), RQ to the left (x_k = Q, x_{k-1} = x_{k-1}·R) --
@@ -823,7 +958,7 @@ void round_lockstep(std::vector<Lock>& P, double eps, hipStream_t s) {
 }  // namespace
 
 bool lockstep_supported(const ALSOptions& o, const std::vector<const TT*>& A) {
-	if (o.sites != 1 || !o.assumeSPD || o.useResidualForEndCriterion || o.precision != P_F64) return false;
+	if (o.sites != 1 || !o.assumeSPD || o.useResidualForEndCriterion || (o.precision != P_F64 && o.precision != P_F32)) return false;
 	for (const TT* a : A) if (!a) return false;
 	return true;
 }
@@ -880,6 +1015,7 @@ void als_round_lockstep(const ALSOptions& o, const std::vector<const TT*>& A, co
 		for (const Lock& L : P) {
 			total->localSolves += L.localSolves;
 			total->cgIterations += L.cgIters;
+			total->refinements += L.refinements;
 			total->halfSweeps += L.halfSweeps;
 		}
 	}

@@ -160,7 +160,7 @@ void als_round_batch(const ALSOptions& o, const std::vector<const TT*>& A, const
                      const std::vector<const TT*>& b, size_t sweeps, double eps, int threads, double* residuals,
                      ALSStats* total);
 // The same in lockstep (threads <= 0 in als_round_batch): every step of every problem is one grouped launch for all
-// problems. Supported for one-site ALS_SPD with an operator and FP64 local solves.
+// problems. Supported for one-site ALS_SPD with an operator and FP64 or FP32 (FP64-refined) local solves.
 bool lockstep_supported(const ALSOptions& o, const std::vector<const TT*>& A);
 void als_round_lockstep(const ALSOptions& o, const std::vector<const TT*>& A, const std::vector<TT*>& x,
                         const std::vector<const TT*>& b, size_t sweeps, double eps, double* residuals, ALSStats* total);

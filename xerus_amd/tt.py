@@ -524,7 +524,7 @@ def als_round_batch(variant: "ALSVariant", As, xs, bs, sweeps: int, eps: float, 
     """Many independent problems: problem p runs ``sweeps`` × (variant(As[p], xs[p], bs[p]), xs[p].round(eps)) on
     one of ``threads`` worker threads, each with its own stream (xs updated in place). ``threads=0`` runs all problems
     in lockstep instead: every step of every problem is one grouped launch (one-site ALS_SPD with an operator and
-    FP64 local solves; other options fall back to worker threads). Returns the relative residuals ‖A x − b‖/‖b‖ (or
+    XT_F64 or XT_F32 local solves; other options fall back to worker threads). Returns the relative residuals ‖A x − b‖/‖b‖ (or
     None) and the summed solver counters."""
     n = len(xs)
     if not (len(As) == n and len(bs) == n):
