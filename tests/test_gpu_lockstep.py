@@ -38,7 +38,26 @@ def test_lockstep_small_matches_threads_bitwise(xe):
     for (A, x, b, shape), (_, y, _, _), r1, r2 in zip(pa, pb, ra, rb):
         assert x.ranks() == y.ranks(), shape
         assert (x - y).frob_norm() <= 1e-14 * y.frob_norm() + 1e-300, shape
-        assert abs(r1 - r2) <= 1e-12 * r2 + 1e-15, shape
+        # the residuals come from different code paths (grouped sweep against tt_residual per problem): each is
+        # exact to a few ε·(‖A x‖ + ‖b‖)/‖b‖
+        assert abs(r1 - r2) <= 1e-6 * r2 + 1e-13, shape
+
+
+def test_lockstep_residuals_match_tt_residual(xe):
+    """The lockstep driver's residuals (w = A·x − b assembled core by core, grouped blocked R-only QR sweep, ‖b‖ from
+    b's transfer matrices) against tt_residual on every problem of a 48-problem batch of configuration 4: widths up to
+    3·64 + 4 = 196 columns take the blocked panels (64 columns each) and rows up to 196·16 the TSQR tree. Both are
+    backward stable, so they agree to a few ε·(‖A x‖ + ‖b‖)/‖b‖ in absolute terms."""
+    print("lockstep residuals", flush=True)
+    from xerus_amd.workloads import batch_problems
+    probs = batch_problems(48)
+    res, _ = run_batch(xe, probs, 0)
+    wide = 0
+    for p, ((A, x, b, shape), r1) in enumerate(zip(probs, res)):
+        r2 = xe.residual(A, x, b)
+        wide += max(x.ranks()) * 3 + 4 > 64
+        assert abs(r1 - r2) <= 1e-6 * r2 + 1e-13, (p, shape, r1, r2)
+    assert wide > 0
 
 
 def test_lockstep_matches_threads_full_shapes(xe, oracle):

@@ -252,7 +252,51 @@ __global__ __launch_bounds__(CT) void dot2_group_kernel(const Dot2Task* tasks, d
 	if (threadIdx.x == 0) { out[2 * blockIdx.x] = s0; out[2 * blockIdx.x + 1] = s1; }
 }
 
+__global__ __launch_bounds__(GT) void res_core_group_kernel(const ResCoreTask* tasks, const int* start, int count) {
+	const int t = group_task(start, count, (int)blockIdx.x);
+	const ResCoreTask k = tasks[t];
+	const int pl = k.rA * k.rx, pr = k.rA2 * k.rx2;
+	const int L = k.first ? 1 : pl + k.rb, R = k.last ? 1 : pr + k.rb2;
+	const long long total = (long long)L * k.n * R;
+	const long long e0 = (long long)(blockIdx.x - start[t]) * GE;
+	const long long e1 = min(total, e0 + GE);
+	for (long long e = e0 + threadIdx.x; e < e1; e += GT) {
+		const int r = (int)(e % R);
+		const long long lj = e / R;
+		const int j = (int)(lj % k.n), l = (int)(lj / k.n);
+		double v = 0.0;
+		if ((k.first || l < pl) && (k.last || r < pr)) {  // the product block
+			const int al = k.first ? 0 : l / k.rx, i = k.first ? 0 : l - (l / k.rx) * k.rx;
+			const int ar = k.last ? 0 : r / k.rx2, ir = k.last ? 0 : r - (r / k.rx2) * k.rx2;
+			const double* a = k.A + ((size_t)(al * k.n + j) * k.m) * k.rA2 + ar;
+			const double* x = k.x + (size_t)i * k.m * k.rx2 + ir;
+			for (int mm = 0; mm < k.m; ++mm) v = fma(a[(size_t)mm * k.rA2], x[(size_t)mm * k.rx2], v);
+		}
+		if ((k.first || l >= pl) && (k.last || r >= pr)) {  // the right-hand side block
+			const int bl = k.first ? 0 : l - pl, br = k.last ? 0 : r - pr;
+			const double bv = k.b[((size_t)bl * k.n + j) * k.rb2 + br];
+			v += k.first ? -bv : bv;
+		}
+		k.w[e] = v;
+	}
+}
+
 }  // namespace
+
+void res_core_group(const std::vector<ResCoreTask>& tasks, hipStream_t s) {
+	if (tasks.empty()) return;
+	std::vector<int> start;
+	int blocks = 0;
+	for (const ResCoreTask& k : tasks) {
+		start.push_back(blocks);
+		const long long L = k.first ? 1 : k.rA * k.rx + k.rb, R = k.last ? 1 : k.rA2 * k.rx2 + k.rb2;
+		blocks += (int)((L * k.n * R + GE - 1) / GE);
+	}
+	const ResCoreTask* dt = static_cast<const ResCoreTask*>(table_upload(tasks.data(), tasks.size() * sizeof(ResCoreTask), s));
+	const int* ds = static_cast<const int*>(table_upload(start.data(), start.size() * sizeof(int), s));
+	launch_timed("res_core_group_kernel", Work(0.0, 0.0), res_core_group_kernel, dim3((unsigned)blocks), dim3(GT), 0, s, dt, ds,
+	             (int)tasks.size());
+}
 
 void map_group(const std::vector<MapTask>& tasks, hipStream_t s) {
 	std::vector<MapTask> live;

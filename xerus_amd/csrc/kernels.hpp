@@ -224,6 +224,12 @@ void mix_group(const std::vector<MixTask>& tasks, hipStream_t s);
 // r = b − q with out[2t] = ‖r‖², out[2t + 1] = ‖b‖² (FP64), one workgroup per task
 struct ResidTask { int n; double* r; const double* b; const double* q; };
 void resid_group(const std::vector<ResidTask>& tasks, double* out, hipStream_t s);
+// Core k of w = A·x + (−b) as a TT (tt_apply, then tt_add with b's first core negated): rank index (α, i) of the
+// product, then b's ranks; the first core concatenates horizontally, the last vertically, the others block-diagonal.
+// A core (rA, n, m, rA2), x core (rx, m, rx2), b core (rb, n, rb2); w (L, n, R) with L = first ? 1 : rA·rx + rb,
+// R = last ? 1 : rA2·rx2 + rb2.
+struct ResCoreTask { const double* A; const double* x; const double* b; double* w; int rA, rA2, rx, rx2, rb, rb2, n, m, first, last; };
+void res_core_group(const std::vector<ResCoreTask>& tasks, hipStream_t s);
 struct Dot2Task { int n; const double* a; const double* b; const double* c; const double* d; };
 void dot2_group(const std::vector<Dot2Task>& tasks, double* out, hipStream_t s);  // out[2t] = a·b, out[2t+1] = c·d
 
@@ -250,6 +256,9 @@ void gemm_group_run_f32(const GemmGroupPlan& plan, hipStream_t s);
 struct QRJob { const double* A; int m, n; bool trans; double* Q; double* R; };
 void hh_qr_group(const std::vector<QRJob>& jobs, hipStream_t s);
 // xerus's QC of many matrices (each as hh_qc): rank, jpvt, Q (m × rank, ld rank) and C (rank × n) on the device.
+// R factor (min(m,n) × n) of a row-major m × n matrix of any width; A is overwritten
+struct RJob { double* A; long long m, n; double* R; };
+void hh_r_group(const std::vector<RJob>& jobs, hipStream_t s);
 struct QCJob { const double* A; int m, n; double* Q; double* C; int* jpvt; int* rank; };
 void hh_qc_group(const std::vector<QCJob>& jobs, hipStream_t s);
 // One-sided Jacobi SVD of many matrices (each as jacobi_svd, m >= n, workspace ws as there).

@@ -13,8 +13,6 @@
 #include <cstring>
 #include <exception>
 #include <functional>
-#include <mutex>
-#include <thread>
 #include <vector>
 
 #include "tt.hpp"
@@ -957,6 +955,85 @@ void round_lockstep(std::vector<Lock>& P, double eps, hipStream_t s) {
 
 }  // namespace
 
+// ---- ‖A x − b‖ / ‖b‖ per problem (tt_residual): w = A·x + (−b) as a TT, an R-only QR sweep from the left
+// (c_0 = w_0, c_k = R_{k-1}·w_k, R_k = R of c_k) and ‖c_{d-1}‖; ‖b‖² from the transfer matrices of b (tt_dot).
+// One step per core index for all problems: the cores of w are assembled, multiplied and factored in grouped
+// launches, and nothing is read back before the end.
+void residuals_lockstep(std::vector<Lock>& P, double* out, hipStream_t s) {
+	const size_t J = P.size();
+	size_t maxd = 0;
+	for (const Lock& L : P) maxd = std::max(maxd, L.d);
+	std::vector<DTensor> Rk(J), G(J), last(J);
+	DBuf<double> gfin(J);
+	for (size_t k = 0; k < maxd; ++k) {
+		std::vector<ResCoreTask> asmT;
+		std::vector<DTensor> W(J), C(J), M(J), Gn(J);
+		for (size_t j = 0; j < J; ++j) {
+			const Lock& L = P[j];
+			if (k >= L.d) continue;
+			const DTensor &Ac = L.A->cores[k], &Xc = L.x->cores[k], &Bc = L.b->cores[k];
+			ResCoreTask t{Ac.data(), Xc.data(), Bc.data(), nullptr, (int)Ac.dims[0], (int)Ac.dims[3], (int)Xc.dims[0], (int)Xc.dims[2],
+			              (int)Bc.dims[0], (int)Bc.dims[2], (int)Ac.dims[1], (int)Ac.dims[2], k == 0, k + 1 == L.d};
+			XTT_REQUIRE(Ac.dims[2] == Xc.dims[1] && Bc.dims[1] == Ac.dims[1], "lockstep residual: dimension mismatch");
+			const size_t Lw = t.first ? 1 : (size_t)(t.rA * t.rx + t.rb), Rw = t.last ? 1 : (size_t)(t.rA2 * t.rx2 + t.rb2);
+			W[j] = DTensor({Lw, (size_t)t.n, Rw});
+			t.w = W[j].data();
+			asmT.push_back(t);
+		}
+		res_core_group(asmT, s);
+		std::vector<GemmTask> g1, g2;
+		for (size_t j = 0; j < J; ++j) {
+			const Lock& L = P[j];
+			if (k >= L.d) continue;
+			const DTensor& Bc = L.b->cores[k];
+			const size_t beta = Bc.dims[0], n = Bc.dims[1], beta2 = Bc.dims[2];
+			double* gout;
+			if (k + 1 == L.d) {
+				gout = gfin.get() + j;
+			} else {
+				Gn[j] = DTensor({beta2, beta2});
+				gout = Gn[j].data();
+			}
+			if (k == 0) {
+				C[j] = std::move(W[j]);
+				g1.push_back(gt(gemm_desc(1, 0, beta2, beta2, n), Bc.data(), Bc.data(), gout));
+			} else {
+				const size_t kk = Rk[j].dims[0], Lw = W[j].dims[0], Rw = W[j].dims[2];
+				C[j] = DTensor({kk, n, Rw});
+				g1.push_back(gt(gemm_desc(0, 0, kk, n * Rw, Lw), Rk[j].data(), W[j].data(), C[j].data()));
+				M[j] = DTensor({beta, n, beta2});
+				g1.push_back(gt(gemm_desc(0, 0, beta, n * beta2, beta), G[j].data(), Bc.data(), M[j].data()));
+				g2.push_back(gt(gemm_desc(1, 0, beta2, beta2, beta * n), Bc.data(), M[j].data(), gout));
+			}
+		}
+		gemm_group_f64(g1, s);
+		gemm_group_f64(g2, s);
+		std::vector<RJob> qr;
+		for (size_t j = 0; j < J; ++j) {
+			const Lock& L = P[j];
+			if (k >= L.d) continue;
+			if (k + 1 == L.d) {
+				last[j] = std::move(C[j]);
+				continue;
+			}
+			const size_t cols = C[j].dims[2], rows = C[j].size() / cols;
+			Rk[j] = DTensor({std::min(rows, cols), cols});
+			qr.push_back(RJob{C[j].data(), (long long)rows, (long long)cols, Rk[j].data()});
+			G[j] = std::move(Gn[j]);
+		}
+		hh_r_group(qr, s);
+	}
+	std::vector<Dot2Task> dt;
+	for (size_t j = 0; j < J; ++j) dt.push_back(Dot2Task{(int)last[j].size(), last[j].data(), last[j].data(), nullptr, nullptr});
+	DBuf<double> sums(2 * J);
+	dot2_group(dt, sums.get(), s);
+	std::vector<double> h(2 * J), gb(J);
+	read_back(h.data(), sums.get(), 2 * J * sizeof(double), s);
+	read_back(gb.data(), gfin.get(), J * sizeof(double), s);
+	table_ring_release(s);
+	for (size_t j = 0; j < J; ++j) out[j] = std::sqrt(std::max(0.0, h[2 * j])) / std::sqrt(gb[j]);
+}
+
 bool lockstep_supported(const ALSOptions& o, const std::vector<const TT*>& A) {
 	if (o.sites != 1 || !o.assumeSPD || o.useResidualForEndCriterion || (o.precision != P_F64 && o.precision != P_F32)) return false;
 	for (const TT* a : A) if (!a) return false;
@@ -980,37 +1057,8 @@ void als_round_lockstep(const ALSOptions& o, const std::vector<const TT*>& A, co
 		als_lockstep(o, P, s);
 		round_lockstep(P, eps, s);
 	}
+	if (residuals) residuals_lockstep(P, residuals, s);
 	sync();
-	if (residuals) {  // ‖Ax − b‖/‖b‖ per problem (a QR sweep over the TT of Ax − b each), on worker threads
-		int dev = 0;
-		XTT_HIP(hipGetDevice(&dev));
-		park_stream_pool(stream());
-		const size_t T = std::min<size_t>(16, count);
-		std::vector<std::thread> pool;
-		std::exception_ptr err;
-		std::mutex mu;
-		for (size_t t = 0; t < T; ++t)
-			pool.emplace_back([&, t] {
-				hipStream_t ws = nullptr;
-				try {
-					XTT_HIP(hipSetDevice(dev));
-					XTT_HIP(hipStreamCreateWithFlags(&ws, hipStreamDefault));
-					bind_thread_stream(ws);
-					for (size_t j = t; j < count; j += T) residuals[j] = tt_residual(*A[j], *x[j], *b[j]);
-				} catch (...) {
-					std::lock_guard<std::mutex> lk(mu);
-					if (!err) err = std::current_exception();
-				}
-				if (ws) {
-					(void)hipStreamSynchronize(ws);
-					bind_thread_stream(nullptr);
-					park_stream_pool(ws);
-					(void)hipStreamDestroy(ws);
-				}
-			});
-		for (auto& th : pool) th.join();
-		if (err) std::rethrow_exception(err);
-	}
 	if (total) {
 		for (const Lock& L : P) {
 			total->localSolves += L.localSolves;

@@ -658,17 +658,16 @@ struct ApplyTArgs {
 };
 
 template <int RPT>
-__global__ __launch_bounds__(HT) void hh_applyT_kernel(ApplyTArgs a) {
+__device__ __forceinline__ void hh_applyT_body(const ApplyTArgs& a, const int b, const int cy) {
 	__shared__ double sW[2][HW][HN];
 	__shared__ double sTau[HN];
 	const int lane = threadIdx.x & 63;
 	const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-	const int b = blockIdx.x;
 	const int r0 = b * a.rpb;
 	const int rows = min(a.rpb, a.rows - r0);
 	const int n = a.n;
 	const int k = min(rows, n);
-	const int c0 = blockIdx.y * HN;
+	const int c0 = cy * HN;
 	const int ncc = min(HN, a.nc - c0);
 	double vr[RPT];
 #pragma unroll
@@ -730,25 +729,57 @@ __global__ __launch_bounds__(HT) void hh_applyT_kernel(ApplyTArgs a) {
 		if (r < rows && lane < ncc) a.C[(size_t)(r0 + r) * a.ldc + c0 + lane] = q[t];
 	}
 }
+template <int RPT>
+__global__ __launch_bounds__(HT) void hh_applyT_kernel(ApplyTArgs a) { hh_applyT_body<RPT>(a, (int)blockIdx.x, (int)blockIdx.y); }
+// many levels in one launch: task t owns workgroups [start[t], start[t+1]), row block fastest, then 64-column chunk
+template <int RPT>
+__global__ __launch_bounds__(HT) void hh_applyT_group_kernel(const ApplyTArgs* tasks, const int* start, int count) {
+	const int t = group_task(start, count, (int)blockIdx.x);
+	const ApplyTArgs a = tasks[t];
+	const int l = (int)blockIdx.x - start[t];
+	const int blocks = (a.rows + a.rpb - 1) / a.rpb;
+	hh_applyT_body<RPT>(a, l % blocks, l / blocks);
+}
 
 // After a level of a panel factorisation: level row r of block b (rb = r − b·rpb, k_b = min(rows_b, w) R rows)
 // goes to the next level as [R_b row rb | C row r] when rb < k_b, else its C part joins the next panel's rows.
 // Every block but the last is full, so the next-level row is b·w + rb and the remainder row b·(rpb − w) + rb − k_b.
+__device__ __forceinline__ void panel_split_elem(const double* Rb, int w, const double* C, long long ldc, int nT, int rows, int rpb,
+                                                 double* next, long long ldNext, double* rem, long long remRow0, long long e) {
+	const long long cols = w + nT;
+	const int r = (int)(e / cols), c = (int)(e - (long long)r * cols);
+	const int b = r / rpb, rb = r - b * rpb;
+	const int kb = min(min(rpb, rows - b * rpb), w);
+	if (rb < kb) {
+		const double v = c < w ? Rb[((size_t)b * w + rb) * w + c] : C[(size_t)r * ldc + (c - w)];
+		next[(size_t)(b * w + rb) * ldNext + c] = v;
+	} else if (c >= w) {
+		rem[(size_t)(remRow0 + (long long)b * (rpb - w) + (rb - kb)) * nT + (c - w)] = C[(size_t)r * ldc + (c - w)];
+	}
+}
 __global__ void panel_split_kernel(const double* Rb, int w, const double* C, long long ldc, int nT, int rows, int rpb,
                                    double* next, long long ldNext, double* rem, long long remRow0) {
-	const long long cols = w + nT;
-	const long long total = (long long)rows * cols;
-	for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
-		const int r = (int)(e / cols), c = (int)(e - (long long)r * cols);
-		const int b = r / rpb, rb = r - b * rpb;
-		const int kb = min(min(rpb, rows - b * rpb), w);
-		if (rb < kb) {
-			const double v = c < w ? Rb[((size_t)b * w + rb) * w + c] : C[(size_t)r * ldc + (c - w)];
-			next[(size_t)(b * w + rb) * ldNext + c] = v;
-		} else if (c >= w) {
-			rem[(size_t)(remRow0 + (long long)b * (rpb - w) + (rb - kb)) * nT + (c - w)] = C[(size_t)r * ldc + (c - w)];
-		}
-	}
+	const long long total = (long long)rows * (w + nT);
+	for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x)
+		panel_split_elem(Rb, w, C, ldc, nT, rows, rpb, next, ldNext, rem, remRow0, e);
+}
+struct SplitTask {
+	const double* Rb;
+	const double* C;
+	double* next;
+	double* rem;
+	long long ldc, ldNext, remRow0;
+	int w, nT, rows, rpb;
+};
+constexpr int kSplitPerBlock = 4096;
+__global__ __launch_bounds__(256) void panel_split_group_kernel(const SplitTask* tasks, const int* start, int count) {
+	const int t = group_task(start, count, (int)blockIdx.x);
+	const SplitTask a = tasks[t];
+	const long long total = (long long)a.rows * (a.w + a.nT);
+	const long long e0 = (long long)(blockIdx.x - start[t]) * kSplitPerBlock;
+	const long long e1 = min(total, e0 + kSplitPerBlock);
+	for (long long e = e0 + threadIdx.x; e < e1; e += 256)
+		panel_split_elem(a.Rb, a.w, a.C, a.ldc, a.nT, a.rows, a.rpb, a.next, a.ldNext, a.rem, a.remRow0, e);
 }
 
 template <int RPT>
@@ -1011,6 +1042,42 @@ void factor_group(const std::vector<std::pair<FactorArgs, int>>& tasks, bool piv
 		}
 	}
 }
+void applyT_group(const std::vector<std::pair<ApplyTArgs, int>>& tasks, hipStream_t s) {
+	std::vector<ApplyTArgs> cls[4];
+	std::vector<int> st[4];
+	int nb[4] = {0, 0, 0, 0};
+	for (const auto& tb : tasks) {
+		const int rpt = rows_per_thread(tb.first.rpb);
+		const int c = rpt == 8 ? 0 : (rpt == 16 ? 1 : (rpt == 24 ? 2 : 3));
+		cls[c].push_back(tb.first);
+		st[c].push_back(nb[c]);
+		nb[c] += tb.second * ((tb.first.nc + HN - 1) / HN);
+	}
+	for (int c = 0; c < 4; ++c) {
+		if (cls[c].empty()) continue;
+		const ApplyTArgs* dt = static_cast<const ApplyTArgs*>(table_upload(cls[c].data(), cls[c].size() * sizeof(ApplyTArgs), s));
+		const int* ds = static_cast<const int*>(table_upload(st[c].data(), st[c].size() * sizeof(int), s));
+		const int cnt = (int)cls[c].size();
+		const dim3 g((unsigned)nb[c]), t(HT);
+		if (c == 0) launch_timed("hh_applyT_group_kernel<8>", Work(), hh_applyT_group_kernel<8>, g, t, 0, s, dt, ds, cnt);
+		else if (c == 1) launch_timed("hh_applyT_group_kernel<16>", Work(), hh_applyT_group_kernel<16>, g, t, 0, s, dt, ds, cnt);
+		else if (c == 2) launch_timed("hh_applyT_group_kernel<24>", Work(), hh_applyT_group_kernel<24>, g, t, 0, s, dt, ds, cnt);
+		else launch_timed("hh_applyT_group_kernel<32>", Work(), hh_applyT_group_kernel<32>, g, t, 0, s, dt, ds, cnt);
+	}
+}
+void split_group(const std::vector<SplitTask>& tasks, hipStream_t s) {
+	if (tasks.empty()) return;
+	std::vector<int> start;
+	int blocks = 0;
+	for (const SplitTask& t : tasks) {
+		start.push_back(blocks);
+		blocks += (int)(((long long)t.rows * (t.w + t.nT) + kSplitPerBlock - 1) / kSplitPerBlock);
+	}
+	const SplitTask* dt = static_cast<const SplitTask*>(table_upload(tasks.data(), tasks.size() * sizeof(SplitTask), s));
+	const int* ds = static_cast<const int*>(table_upload(start.data(), start.size() * sizeof(int), s));
+	launch_timed("panel_split_group_kernel", Work(), panel_split_group_kernel, dim3((unsigned)blocks), dim3(256), 0, s, dt, ds,
+	             (int)tasks.size());
+}
 // Q formation of many trees: a single level from the identity (hh_wy), or leaves under a single-block root
 // (hh_wy2); anything deeper one tree at a time.
 struct QTree {
@@ -1170,6 +1237,105 @@ void hh_qc_group(const std::vector<QCJob>& jobs, hipStream_t s) {
 	for (size_t j = 0; j < J; ++j)
 		if (grouped[j]) trees.push_back(QTree{&lv[j], jobs[j].n, std::min(rootRows[j], jobs[j].n), jobs[j].rank, jobs[j].Q});
 	qform_group(trees, s);
+}
+
+// R factors of many row-major matrices of any width (blocked_r's panels and trees, all jobs one level per step:
+// one grouped factor, trailing-update and split launch per step). The inputs are overwritten.
+void hh_r_group(const std::vector<RJob>& jobs, hipStream_t s) {
+	struct St {
+		RJob j;
+		double* cur;
+		DBuf<double> curOwn, rem, lvl, nextLvl, V, tau, Rb;
+		long long mp = 0, c0 = 0, remRows = 0, remRow = 0;
+		int np = 0, w = 0, nT = 0, rows = 0, blocks = 0, rpb = 0, nextRows = 0;
+		const double* L = nullptr;
+		bool leaves = true, panel = true, done = false;
+	};
+	std::vector<St> S(jobs.size());
+	std::vector<MapTask> zero;
+	for (size_t i = 0; i < jobs.size(); ++i) {
+		const RJob& j = jobs[i];
+		XTT_REQUIRE(j.m > 0 && j.n > 0 && j.m <= INT32_MAX / HN && j.n <= INT32_MAX / HN, "blocked QR: matrix size");
+		S[i].j = j;
+		S[i].cur = j.A;
+		S[i].mp = j.m;
+		zero.push_back(map_fill(j.R, 0.0, (size_t)std::min(j.m, j.n) * j.n));
+	}
+	map_group(zero, s);
+	for (;;) {
+		std::vector<std::pair<FactorArgs, int>> fa;
+		std::vector<std::pair<ApplyTArgs, int>> ap;
+		std::vector<SplitTask> sp;
+		for (St& t : S) {
+			if (t.done) continue;
+			if (t.panel) {
+				t.np = (int)(t.j.n - t.c0);
+				t.w = std::min(t.np, HN);
+				t.nT = t.np - t.w;
+				t.remRows = t.mp - std::min<long long>(t.mp, t.w);
+				t.rem.resize((size_t)std::max<long long>(t.remRows, 1) * std::max(t.nT, 1));
+				t.remRow = 0;
+				t.L = t.cur;
+				t.rows = (int)t.mp;
+				t.leaves = true;
+				t.panel = false;
+			}
+			level_geometry(t.rows, t.leaves, t.w, t.blocks, t.rpb);
+			t.V.resize((size_t)t.rows * t.w);
+			t.tau.resize((size_t)t.blocks * t.w);
+			t.Rb.resize((size_t)t.blocks * t.w * t.w);
+			FactorArgs f{};
+			f.in = t.L; f.ld = t.np; f.trans = 0;
+			f.rows = t.rows; f.rpb = t.rpb; f.n = t.w;
+			f.V = t.V.get(); f.tau = t.tau.get(); f.R = t.Rb.get();
+			fa.push_back({f, t.blocks});
+			if (t.nT > 0) {
+				ApplyTArgs a{};
+				a.V = t.V.get(); a.tau = t.tau.get(); a.rows = t.rows; a.rpb = t.rpb; a.n = t.w;
+				a.C = const_cast<double*>(t.L) + t.w; a.ldc = t.np; a.nc = t.nT;
+				ap.push_back({a, t.blocks});
+			}
+			const bool root = t.blocks == 1;
+			t.nextRows = root ? std::min(t.rows, t.w) : (t.blocks - 1) * t.w + std::min(t.rows - (t.blocks - 1) * t.rpb, t.w);
+			SplitTask p{};
+			p.Rb = t.Rb.get(); p.C = t.L + t.w; p.ldc = t.np; p.w = t.w; p.nT = t.nT; p.rows = t.rows; p.rpb = t.rpb;
+			if (root) {
+				p.next = t.j.R + (size_t)t.c0 * t.j.n + t.c0;
+				p.ldNext = t.j.n;
+			} else {
+				t.nextLvl.resize((size_t)t.nextRows * t.np);
+				p.next = t.nextLvl.get();
+				p.ldNext = t.np;
+			}
+			p.rem = t.rem.get();
+			p.remRow0 = t.remRow;
+			sp.push_back(p);
+		}
+		if (fa.empty()) break;
+		factor_group(fa, false, s);
+		applyT_group(ap, s);
+		split_group(sp, s);
+		for (St& t : S) {
+			if (t.done) continue;
+			t.remRow += t.rows - t.nextRows;
+			if (t.blocks == 1) {  // the panel's root: the remainder rows form the next panel
+				XTT_REQUIRE(t.remRow == t.remRows, "blocked QR: row bookkeeping");
+				std::swap(t.curOwn, t.rem);
+				t.cur = t.curOwn.get();
+				t.mp = t.remRows;
+				t.c0 += HN;
+				t.panel = true;
+				t.done = t.c0 >= t.j.n || t.mp <= 0;
+			} else {
+				std::swap(t.lvl, t.nextLvl);
+				t.L = t.lvl.get();
+				t.rows = t.nextRows;
+				t.leaves = false;
+			}
+		}
+	}
+	// no table_ring_release here: the stream has not been synchronised, and the tables' host-to-device copies
+	// may still be queued behind the launches
 }
 
 }  // namespace xtt
