@@ -26,8 +26,17 @@ std::unordered_map<int, DeviceState> g_dev;
 // were parked there) and serve any stream.
 struct Block { size_t bytes; int device; };
 using PoolKey = std::tuple<int, hipStream_t, size_t>;
+struct PoolKeyHash {
+	size_t operator()(const PoolKey& k) const {
+		size_t h = std::hash<size_t>()(std::get<2>(k));
+		h ^= std::hash<const void*>()(std::get<1>(k)) + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+		h ^= (size_t)(unsigned)std::get<0>(k) * 0x100000001b3ull;
+		return h;
+	}
+};
 std::mutex g_amu;
-std::multimap<PoolKey, void*> g_free;
+// one free list per key: lookups are a hash and a vector pop, frees a hash and a push (no node allocation)
+std::unordered_map<PoolKey, std::vector<void*>, PoolKeyHash> g_free;
 std::unordered_map<void*, Block> g_live;
 size_t g_inUse = 0;
 
@@ -45,14 +54,25 @@ thread_local hipStream_t t_stream = nullptr;
 }
 void bind_thread_stream(hipStream_t s) { t_stream = s; }
 
+namespace {
+thread_local int t_lastDev = -1;                 // the device of the last stream() lookup on this thread
+thread_local hipStream_t t_lastStream = nullptr;  // and its library stream (streams live until the process ends)
+hipStream_t device_stream(int dev) {
+	if (dev == t_lastDev) return t_lastStream;
+	std::lock_guard<std::mutex> lk(g_mu);
+	DeviceState& d = g_dev[dev];
+	if (!d.stream) XTT_HIP(hipStreamCreateWithFlags(&d.stream, hipStreamDefault));
+	t_lastDev = dev;
+	t_lastStream = d.stream;
+	return d.stream;
+}
+}  // namespace
+
 hipStream_t stream() {
 	if (t_stream) return t_stream;
 	int dev = 0;
 	XTT_HIP(hipGetDevice(&dev));
-	std::lock_guard<std::mutex> lk(g_mu);
-	DeviceState& d = g_dev[dev];
-	if (!d.stream) XTT_HIP(hipStreamCreateWithFlags(&d.stream, hipStreamDefault));
-	return d.stream;
+	return device_stream(dev);
 }
 
 void sync() { XTT_HIP(hipStreamSynchronize(stream())); }
@@ -130,14 +150,14 @@ void* dmalloc(size_t bytes) {
 	int dev = 0;
 	XTT_HIP(hipGetDevice(&dev));
 	const size_t cls = size_class(bytes);
-	const hipStream_t st = stream();
+	const hipStream_t st = t_stream ? t_stream : device_stream(dev);
 	{
 		std::lock_guard<std::mutex> lk(g_amu);
 		auto it = g_free.find(PoolKey{dev, st, cls});
-		if (it == g_free.end()) it = g_free.find(PoolKey{dev, nullptr, cls});  // idle blocks: no work pending
-		if (it != g_free.end()) {
-			void* p = it->second;
-			g_free.erase(it);
+		if (it == g_free.end() || it->second.empty()) it = g_free.find(PoolKey{dev, nullptr, cls});  // idle blocks
+		if (it != g_free.end() && !it->second.empty()) {
+			void* p = it->second.back();
+			it->second.pop_back();
 			g_live[p] = {cls, dev};
 			g_inUse += cls;
 			return p;
@@ -166,38 +186,46 @@ void dfree(void* p) {
 	std::lock_guard<std::mutex> lk(g_amu);
 	auto it = g_live.find(p);
 	if (it == g_live.end()) return;
-	g_free.insert({PoolKey{it->second.device, st, it->second.bytes}, p});
+	g_free[PoolKey{it->second.device, st, it->second.bytes}].push_back(p);
 	g_inUse -= it->second.bytes;
 	g_live.erase(it);
 }
 
 void park_stream_pool(hipStream_t from) {
 	std::lock_guard<std::mutex> lk(g_amu);
+	std::vector<std::pair<PoolKey, std::vector<void*>>> moved;
 	for (auto it = g_free.begin(); it != g_free.end();) {
-		if (std::get<1>(it->first) == from) {
-			g_free.insert({PoolKey{std::get<0>(it->first), nullptr, std::get<2>(it->first)}, it->second});
+		if (std::get<1>(it->first) == from && from != nullptr) {
+			moved.push_back({PoolKey{std::get<0>(it->first), nullptr, std::get<2>(it->first)}, std::move(it->second)});
 			it = g_free.erase(it);
 		} else {
 			++it;
 		}
+	}
+	for (auto& m : moved) {
+		std::vector<void*>& dst = g_free[m.first];
+		dst.insert(dst.end(), m.second.begin(), m.second.end());
 	}
 }
 
 size_t bytes_cached() {
 	std::lock_guard<std::mutex> lk(g_amu);
 	size_t b = 0;
-	for (auto& kv : g_free) b += std::get<2>(kv.first);
+	for (auto& kv : g_free) b += std::get<2>(kv.first) * kv.second.size();
 	return b;
 }
 
 void release_cache() {
-	std::multimap<PoolKey, void*> blocks;
+	std::unordered_map<PoolKey, std::vector<void*>, PoolKeyHash> blocks;
 	{
 		std::lock_guard<std::mutex> lk(g_amu);
 		blocks.swap(g_free);
 	}
-	if (!blocks.empty()) (void)hipDeviceSynchronize();
-	for (auto& kv : blocks) (void)hipFree(kv.second);
+	bool any = false;
+	for (auto& kv : blocks) any |= !kv.second.empty();
+	if (any) (void)hipDeviceSynchronize();
+	for (auto& kv : blocks)
+		for (void* p : kv.second) (void)hipFree(p);
 }
 
 namespace {

@@ -64,3 +64,36 @@ def batch_problems(count, base_seed=0xBAADF00D, d_range=(6, 20), n_choices=(2, 4
         x = _tt.TTTensor.random([n] * d, [r] * (d - 1))
         out.append((A, x, b, (d, n, r)))
     return out
+
+
+def main(argv=None):
+    """Runs the uneven batch once and prints its wall time: ``python -m xerus_amd.workloads --problems 512
+    --sweeps 2 --precision f64 --threads 0`` (threads 0: the lockstep driver). Meant for profilers, which time the
+    kernels of exactly this call."""
+    import argparse
+    import time
+    from . import tt as _tt
+    from ._lib import XT_F32, XT_F64, lib
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--problems", type=int, default=512)
+    ap.add_argument("--sweeps", type=int, default=2)
+    ap.add_argument("--precision", choices=["f64", "f32"], default="f64")
+    ap.add_argument("--threads", type=int, default=0)
+    a = ap.parse_args(argv)
+    probs = batch_problems(a.problems)
+    als = _tt.ALS_SPD.copy()
+    als.convergenceEpsilon = 0.0
+    als.numHalfSweeps = 2
+    als.precision = XT_F64 if a.precision == "f64" else XT_F32
+    lib().xt_synchronize()
+    t0 = time.perf_counter()
+    res, st = _tt.als_round_batch(als, [p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs], a.sweeps, 1e-12,
+                                  threads=a.threads)
+    dt = time.perf_counter() - t0
+    res = np.asarray(res)
+    print(f"{a.problems} problems, {a.sweeps} sweeps, {a.precision}, threads {a.threads}: {dt:.3f} s, "
+          f"{a.problems * a.sweeps / dt:.1f} sweeps/s, residual median {np.median(res):.2e}, cg {st['cgIterations']}", flush=True)
+
+
+if __name__ == "__main__":
+    main()
