@@ -208,6 +208,69 @@ __global__ __launch_bounds__(CT) void cg_group_step_kernel(const CGTaskT<T>* tas
 	}
 }
 
+// Chunked CG step (cg_group_step_kernel's arithmetic over chunks of a system; see CGChunk).
+template <class T>
+__global__ __launch_bounds__(CT) void cg_chunk_dot_kernel(const CGChunk<T>* tasks) {
+	__shared__ double red[2 * CT / 64];
+	const CGChunk<T> t = tasks[blockIdx.x];
+	if (t.k.st->done) return;
+	double pq = 0, z = 0;
+	for (int i = t.c0 + threadIdx.x; i < t.c0 + t.len; i += CT) pq += (double)t.k.p[i] * (double)t.k.q[i];
+	block_sum2<CT>(pq, z, red);
+	if (threadIdx.x == 0) t.part[t.ci] = pq;
+}
+template <class T>
+__global__ __launch_bounds__(CT) void cg_chunk_update_kernel(const CGChunk<T>* tasks) {
+	__shared__ double red[2 * CT / 64];
+	const CGChunk<T> t = tasks[blockIdx.x];
+	CGState* st = t.k.st;
+	if (st->done) return;
+	double pq = 0;
+	for (int c = 0; c < t.nch; ++c) pq += t.part[c];
+	if (!(pq > 0)) {  // breakdown: every chunk sees the same sum and stops
+		if (t.ci == 0 && threadIdx.x == 0) { st->alpha = 0; st->done = 1; }
+		return;
+	}
+	const double rrOld = st->rr, alpha = rrOld / pq;
+	const T al = (T)alpha;
+	double rr = 0, z = 0;
+	for (int i = t.c0 + threadIdx.x; i < t.c0 + t.len; i += CT) {
+		t.k.x[i] += al * t.k.p[i];
+		const T ri = t.k.r[i] - al * t.k.q[i];
+		t.k.r[i] = ri;
+		rr += (double)ri * (double)ri;
+	}
+	block_sum2<CT>(rr, z, red);
+	if (threadIdx.x == 0) {
+		t.part[t.nch + t.ci] = rr;
+		if (t.ci == 0) {  // st->rr and st->iter are rewritten by the next launch, which reads these copies
+			st->alpha = alpha;
+			st->rrPrev = rrOld;
+			st->iterPrev = st->iter;
+		}
+	}
+}
+template <class T>
+__global__ __launch_bounds__(CT) void cg_chunk_direction_kernel(const CGChunk<T>* tasks) {
+	const CGChunk<T> t = tasks[blockIdx.x];
+	CGState* st = t.k.st;
+	// chunk 0 may set done in this launch: a chunk that reads it already set would have stopped too
+	if (st->done) return;
+	double rr = 0;
+	for (int c = 0; c < t.nch; ++c) rr += t.part[t.nch + c];
+	const int iter = st->iterPrev + 1;
+	const bool done = rr <= st->tol2 * st->rr0 || iter >= st->maxIter;
+	if (!done) {
+		const T beta = (T)(rr / st->rrPrev);
+		for (int i = t.c0 + threadIdx.x; i < t.c0 + t.len; i += CT) t.k.p[i] = t.k.r[i] + beta * t.k.p[i];
+	}
+	if (t.ci == 0 && threadIdx.x == 0) {
+		st->rr = rr;
+		st->iter = iter;
+		st->done = done ? 1 : 0;
+	}
+}
+
 __global__ __launch_bounds__(GT) void mix_group_kernel(const MixTask* tasks, const int* start, int count) {
 	const int t = group_task(start, count, (int)blockIdx.x);
 	const MixTask m = tasks[t];
@@ -374,6 +437,18 @@ void cg_group_step(const std::vector<CGTaskT<T>>& tasks, hipStream_t s) {
 	const CGTaskT<T>* dt = static_cast<const CGTaskT<T>*>(table_upload(tasks.data(), tasks.size() * sizeof(CGTaskT<T>), s));
 	cg_group_step_run<T>(dt, (int)tasks.size(), s);
 }
+template <class T>
+void cg_chunk_step_run(const CGChunk<T>* devTasks, int count, hipStream_t s) {
+	const bool f64 = sizeof(T) == 8;
+	launch_timed(f64 ? "cg_chunk_dot_kernel<double>" : "cg_chunk_dot_kernel<float>", Work(0.0, 0.0), cg_chunk_dot_kernel<T>,
+	             dim3((unsigned)count), dim3(CT), 0, s, devTasks);
+	launch_timed(f64 ? "cg_chunk_update_kernel<double>" : "cg_chunk_update_kernel<float>", Work(0.0, 0.0), cg_chunk_update_kernel<T>,
+	             dim3((unsigned)count), dim3(CT), 0, s, devTasks);
+	launch_timed(f64 ? "cg_chunk_direction_kernel<double>" : "cg_chunk_direction_kernel<float>", Work(0.0, 0.0),
+	             cg_chunk_direction_kernel<T>, dim3((unsigned)count), dim3(CT), 0, s, devTasks);
+}
+template void cg_chunk_step_run<double>(const CGChunk<double>*, int, hipStream_t);
+template void cg_chunk_step_run<float>(const CGChunk<float>*, int, hipStream_t);
 template void cg_group_start<double>(const std::vector<CGTaskT<double>>&, hipStream_t);
 template void cg_group_start<float>(const std::vector<CGTaskT<float>>&, hipStream_t);
 template void cg_group_step<double>(const std::vector<CGTaskT<double>>&, hipStream_t);

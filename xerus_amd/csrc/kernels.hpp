@@ -90,6 +90,9 @@ struct CGState {
 	int pad;
 	unsigned counter; // reduction ticket
 	unsigned pad2;
+	double rrPrev;    // chunked grouped CG: ‖r‖² and the iteration count before the running step
+	int iterPrev;
+	int pad3;
 };
 template <class T> void cg_start(T* r, T* p, const T* b, const T* q, size_t n, CGState* st, double* partials, hipStream_t s);
 template <class T> void cg_alpha(const T* p, const T* q, size_t n, CGState* st, double* partials, hipStream_t s);
@@ -215,6 +218,18 @@ using CGTask = CGTaskT<double>;
 template <class T> void cg_group_start(const std::vector<CGTaskT<T>>& tasks, hipStream_t s);  // cg_start + the zero-start rule
 template <class T> void cg_group_step(const std::vector<CGTaskT<T>>& tasks, hipStream_t s);   // cg_alpha, cg_update, cg_direction
 template <class T> void cg_group_step_run(const CGTaskT<T>* devTasks, int count, hipStream_t s);  // on a table already on the device
+// The same step for systems split into chunks of kCGChunk elements, one workgroup per chunk, in three launches
+// (p·q partials; α, x and r updates with ‖r‖² partials; the stop test and p = r + β·p). Each workgroup sums its
+// system's partials itself in chunk order, so every chunk computes the same α and β; a system of one chunk gets
+// the result of cg_group_step_run bit for bit. part: 2·nch doubles per system.
+constexpr int kCGChunk = 4096;
+template <class T>
+struct CGChunk {
+	CGTaskT<T> k;
+	int c0, len, nch, ci;
+	double* part;
+};
+template <class T> void cg_chunk_step_run(const CGChunk<T>* devTasks, int count, hipStream_t s);
 // Mixed-precision plumbing of the refinement loop, many vectors per launch:
 //   MIX_CVT32:  out (float) = alpha · a (double);  MIX_AXPY32: out (double) += alpha · a (float);
 //   MIX_ZERO:   out (double) = 0, out2 (double) = a (double) (a warm start worse than zero: x = 0, r = b).

@@ -300,21 +300,49 @@ void cg_iterate(const std::vector<CGTaskT<T>>& tasks, CGState* st, int maxIter, 
 	const size_t K = tasks.size();
 	GemmGroupPlan g1, g2, g3;
 	DBuf<CGTaskT<T>> dTasks;
-	int planned = 0;
+	DBuf<CGChunk<T>> dChunks;
+	DBuf<double> part;
+	int planned = 0, chunks = 0;
+	// systems above kCGChunk elements are split over several workgroups (the one-workgroup step of a 64·16·64
+	// system is bound by one CU's bandwidth); batches of small systems keep the single-launch step
+	auto step = [&] {
+		if (chunks) cg_chunk_step_run<T>(dChunks.get(), chunks, s);
+		else cg_group_step_run<T>(dTasks.get(), planned, s);
+	};
 	auto plan = [&](const std::vector<size_t>& open) {
 		std::vector<GemmTask> t1, t2, t3;
 		std::vector<CGTaskT<T>> tk;
+		bool split = false;
 		for (size_t k : open) {
 			prod(k, &st[k].done, t1, t2, t3);
 			tk.push_back(tasks[k]);
+			split |= tasks[k].n > kCGChunk;
 		}
 		gemm_plan(t1, g1, s, T());  // each plan also runs once: the first product of the next iteration
 		gemm_plan(t2, g2, s, T());
 		gemm_plan(t3, g3, s, T());
-		dTasks.resize(tk.size());
-		table_upload_to(dTasks.get(), tk.data(), tk.size() * sizeof(CGTaskT<T>), s);
-		cg_group_step_run<T>(dTasks.get(), (int)tk.size(), s);
 		planned = (int)open.size();
+		chunks = 0;
+		if (split) {
+			std::vector<CGChunk<T>> ch;
+			size_t parts = 0;
+			for (const CGTaskT<T>& t : tk) parts += 2 * (size_t)((t.n + kCGChunk - 1) / kCGChunk);
+			part.resize(std::max<size_t>(parts, 1));
+			size_t off = 0;
+			for (const CGTaskT<T>& t : tk) {
+				const int nch = std::max(1, (t.n + kCGChunk - 1) / kCGChunk);
+				for (int c = 0; c < nch; ++c)
+					ch.push_back(CGChunk<T>{t, c * kCGChunk, std::min(kCGChunk, t.n - c * kCGChunk), nch, c, part.get() + off});
+				off += 2 * (size_t)nch;
+			}
+			dChunks.resize(ch.size());
+			table_upload_to(dChunks.get(), ch.data(), ch.size() * sizeof(CGChunk<T>), s);
+			chunks = (int)ch.size();
+		} else {
+			dTasks.resize(tk.size());
+			table_upload_to(dTasks.get(), tk.data(), tk.size() * sizeof(CGTaskT<T>), s);
+		}
+		step();
 	};
 	std::vector<size_t> open(K);
 	for (size_t k = 0; k < K; ++k) open[k] = k;
@@ -337,7 +365,7 @@ void cg_iterate(const std::vector<CGTaskT<T>>& tasks, CGState* st, int maxIter, 
 			gemm_run(g1, s, T());
 			gemm_run(g2, s, T());
 			gemm_run(g3, s, T());
-			cg_group_step_run<T>(dTasks.get(), planned, s);
+			step();
 		}
 		done += n;
 		chunk = std::min(16, 2 * chunk);
