@@ -149,11 +149,12 @@ struct Push {
 
 void pushes(std::vector<Push>& P, hipStream_t s) {
 	if (P.empty()) return;
+	ScratchFrame frame;  // Xr, T1, T2, Y
 	std::vector<MapTask> maps;
 	for (Push& p : P) {
 		if (p.left) continue;
 		const DTensor& X = p.L->x->cores[p.i];
-		p.Xr = DTensor({X.dims[2], X.dims[1], X.dims[0]});
+		p.Xr = DTensor::scratch({X.dims[2], X.dims[1], X.dims[0]});
 		const size_t sh[3] = {2, 1, 0};
 		maps.push_back(map_permute(3, X.dims.data(), sh, X.data(), p.Xr.data()));
 	}
@@ -167,8 +168,8 @@ void pushes(std::vector<Push>& P, hipStream_t s) {
 		const DTensor& Am = p.left ? L.am[p.i] : L.amM[p.i];
 		const DTensor& bc = L.b->cores[p.i];
 		const size_t a = Xe.dims[0], n = Xe.dims[1], c = Xe.dims[2], Pp = Am.dims[2], Q = Am.dims[1];
-		p.T1.resize(a * Pp * n * c);
-		p.T2.resize(a * n * Q * c);
+		p.T1.scratch(a * Pp * n * c);
+		p.T2.scratch(a * n * Q * c);
 		p.op = DTensor({c, Q, c});
 		g1.push_back(gt(gemm_desc(0, 0, a * Pp, n * c, a), St.data(), Xe.data(), p.T1.get()));
 		g2.push_back(gt(stage2(a, n, c, Pp, Q, nullptr), Am.data(), p.T1.get(), p.T2.get()));
@@ -177,13 +178,13 @@ void pushes(std::vector<Push>& P, hipStream_t s) {
 		const size_t a0 = X.dims[0], c0 = X.dims[2], beta = bc.dims[0], beta2 = bc.dims[2];
 		if (p.left) {
 			const DTensor& Lb = L.rhsL.back();
-			p.Y.resize(beta * n * c0);
+			p.Y.scratch(beta * n * c0);
 			p.rb = DTensor({beta2, c0});
 			g1.push_back(gt(gemm_desc(0, 0, beta, n * c0, a0), Lb.data(), X.data(), p.Y.get()));
 			g2.push_back(gt(gemm_desc(1, 0, beta2, c0, beta * n), bc.data(), p.Y.get(), p.rb.data()));
 		} else {
 			const DTensor& Rb = L.rhsR.back();
-			p.Y.resize(n * beta2 * a0);
+			p.Y.scratch(n * beta2 * a0);
 			p.rb = DTensor({beta, a0});
 			GemmDesc z = gemm_desc(0, 1, beta2, a0, c0);
 			z.ldb = n * c0; z.sA = 0; z.sB = (long long)c0; z.sC = (long long)(beta2 * a0); z.batch = n;
@@ -216,9 +217,9 @@ void site_init(Site& S, Lock* L) {
 	S.a = X.dims[0]; S.n = X.dims[1]; S.c = X.dims[2];
 	S.Lst = &L->opL.back(); S.Rst = &L->opR.back(); S.Am = &L->am[k];
 	S.P = S.Am->dims[2]; S.Q = S.Am->dims[1];
-	S.T1.resize(S.a * S.P * S.n * S.c);
-	S.T2.resize(S.a * S.n * S.Q * S.c);
-	S.bl = DTensor({S.a, S.n, S.c});
+	S.T1.scratch(S.a * S.P * S.n * S.c);
+	S.T2.scratch(S.a * S.n * S.Q * S.c);
+	S.bl = DTensor::scratch({S.a, S.n, S.c});
 }
 // y = A_loc·v for every site, three grouped launches (the first two also compute the local right-hand sides
 // when rhs is set); skip[j] (optional): site j's products are skipped while *skip[j] != 0
@@ -244,7 +245,7 @@ void local_apply(std::vector<Site>& S, const std::vector<const double*>& v, cons
 			const size_t beta = Lb.dims[0], beta2 = Rb.dims[0];
 			XTT_REQUIRE(Lb.dims[1] == t.a && bc.dims[0] == beta && bc.dims[1] == t.n && bc.dims[2] == beta2 && Rb.dims[1] == t.c,
 			            "lockstep ALS: local right-hand side shape mismatch");
-			t.Tb.resize(t.a * t.n * beta2);
+			t.Tb.scratch(t.a * t.n * beta2);
 			g1.push_back(gt(gemm_desc(1, 0, t.a, t.n * beta2, beta), Lb.data(), bc.data(), t.Tb.get()));
 			g2.push_back(gt(gemm_desc(0, 0, t.a * t.n, t.c, beta2), t.Tb.get(), Rb.data(), t.bl.data()));
 		}
@@ -258,13 +259,14 @@ void local_apply(std::vector<Site>& S, const std::vector<const double*>& v, cons
 void energies(std::vector<Lock*>& Ls, std::vector<double>& E, hipStream_t s) {
 	E.assign(Ls.size(), 0.0);
 	if (Ls.empty()) return;
+	ScratchFrame frame;
 	std::vector<Site> S(Ls.size());
 	std::vector<DTensor> y(Ls.size());
 	std::vector<const double*> v;
 	std::vector<double*> yo;
 	for (size_t j = 0; j < Ls.size(); ++j) {
 		site_init(S[j], Ls[j]);
-		y[j] = DTensor({S[j].a, S[j].n, S[j].c});
+		y[j] = DTensor::scratch({S[j].a, S[j].n, S[j].c});
 		v.push_back(Ls[j]->x->cores[Ls[j]->curr].data());
 		yo.push_back(y[j].data());
 	}
@@ -274,7 +276,8 @@ void energies(std::vector<Lock*>& Ls, std::vector<double>& E, hipStream_t s) {
 		const DTensor& X = Ls[j]->x->cores[Ls[j]->curr];
 		dt.push_back(Dot2Task{(int)X.size(), X.data(), S[j].bl.data(), X.data(), y[j].data()});
 	}
-	DBuf<double> out(2 * Ls.size());
+	DBuf<double> out;
+	out.scratch(2 * Ls.size());
 	dot2_group(dt, out.get(), s);
 	std::vector<double> h(2 * Ls.size());
 	read_back(h.data(), out.get(), h.size() * sizeof(double), s);
@@ -327,7 +330,7 @@ void cg_iterate(const std::vector<CGTaskT<T>>& tasks, CGState* st, int maxIter, 
 			std::vector<CGChunk<T>> ch;
 			size_t parts = 0;
 			for (const CGTaskT<T>& t : tk) parts += 2 * (size_t)((t.n + kCGChunk - 1) / kCGChunk);
-			part.resize(std::max<size_t>(parts, 1));
+			part.scratch(std::max<size_t>(parts, 1));
 			size_t off = 0;
 			for (const CGTaskT<T>& t : tk) {
 				const int nch = std::max(1, (t.n + kCGChunk - 1) / kCGChunk);
@@ -335,11 +338,11 @@ void cg_iterate(const std::vector<CGTaskT<T>>& tasks, CGState* st, int maxIter, 
 					ch.push_back(CGChunk<T>{t, c * kCGChunk, std::min(kCGChunk, t.n - c * kCGChunk), nch, c, part.get() + off});
 				off += 2 * (size_t)nch;
 			}
-			dChunks.resize(ch.size());
+			dChunks.scratch(ch.size());
 			table_upload_to(dChunks.get(), ch.data(), ch.size() * sizeof(CGChunk<T>), s);
 			chunks = (int)ch.size();
 		} else {
-			dTasks.resize(tk.size());
+			dTasks.scratch(tk.size());
 			table_upload_to(dTasks.get(), tk.data(), tk.size() * sizeof(CGTaskT<T>), s);
 		}
 		step();
@@ -373,8 +376,9 @@ void cg_iterate(const std::vector<CGTaskT<T>>& tasks, CGState* st, int maxIter, 
 	table_ring_release(s);
 }
 
-DBuf<CGState> cg_states(size_t K, double tol2, int maxIter, hipStream_t s) {
-	DBuf<CGState> st(K);
+DBuf<CGState> cg_states(size_t K, double tol2, int maxIter, hipStream_t s) {  // in the caller's scratch frame
+	DBuf<CGState> st;
+	st.scratch(K);
 	std::vector<CGState> init(K);
 	for (CGState& c : init) {
 		std::memset(&c, 0, sizeof(CGState));
@@ -389,6 +393,7 @@ DBuf<CGState> cg_states(size_t K, double tol2, int maxIter, hipStream_t s) {
 // FP64 CG on every lock's local system (warm start from its core, which is updated in place)
 void local_solves_f64(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t s) {
 	const size_t J = Ls.size();
+	ScratchFrame frame;  // sites, CG vectors, states and tables
 	std::vector<Site> S(J);
 	std::vector<DBuf<double>> r(J), p(J), q(J);
 	std::vector<CGTask> tasks(J);
@@ -398,7 +403,7 @@ void local_solves_f64(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t s
 	for (size_t j = 0; j < J; ++j) {
 		site_init(S[j], Ls[j]);
 		const size_t N = S[j].a * S[j].n * S[j].c;
-		r[j].resize(N); p[j].resize(N); q[j].resize(N);
+		r[j].scratch(N); p[j].scratch(N); q[j].scratch(N);
 		double* x = Ls[j]->x->cores[Ls[j]->curr].data();
 		tasks[j] = CGTask{(int)N, x, r[j].get(), p[j].get(), q[j].get(), S[j].bl.data(), st.get() + j};
 		xv[j] = x; qv[j] = q[j].get();
@@ -426,6 +431,7 @@ void local_solves_f64(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t s
 void local_solves_mixed(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t s) {
 	const size_t J = Ls.size();
 	const double inner = o.innerTol > 0 ? o.innerTol : 1e-5, tol2 = o.localTol * o.localTol;
+	ScratchFrame frame;  // sites, FP32 operator copies, vectors, states and tables
 	struct Low {
 		DBuf<float> L, Am, R, rhs, e, r, p, q, T1, T2;
 		bool staged = false;
@@ -439,14 +445,15 @@ void local_solves_mixed(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t
 	for (size_t j = 0; j < J; ++j) {
 		site_init(S[j], Ls[j]);
 		N[j] = S[j].a * S[j].n * S[j].c;
-		r[j].resize(N[j]); q[j].resize(N[j]);
+		r[j].scratch(N[j]); q[j].scratch(N[j]);
 		xv[j] = Ls[j]->x->cores[Ls[j]->curr].data();
 		qv[j] = q[j].get();
 		open[j] = j;
 		Ls[j]->localSolves++;
 	}
 	local_apply(S, xv, qv, true, nullptr, s);  // q = A_loc·x and the right-hand sides
-	DBuf<double> norms(2 * J);
+	DBuf<double> norms;
+	norms.scratch(2 * J);
 	std::vector<double> hn(2 * J);
 	for (int ref = 0;; ++ref) {
 		std::vector<ResidTask> rt;
@@ -472,9 +479,9 @@ void local_solves_mixed(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t
 			Low& w = lw[j];
 			Site& t = S[j];
 			if (!w.staged) {
-				w.L.resize(t.Lst->size()); w.Am.resize(t.Am->size()); w.R.resize(t.Rst->size());
-				w.rhs.resize(N[j]); w.e.resize(N[j]); w.r.resize(N[j]); w.p.resize(N[j]); w.q.resize(N[j]);
-				w.T1.resize(t.a * t.P * t.n * t.c); w.T2.resize(t.a * t.n * t.Q * t.c);
+				w.L.scratch(t.Lst->size()); w.Am.scratch(t.Am->size()); w.R.scratch(t.Rst->size());
+				w.rhs.scratch(N[j]); w.e.scratch(N[j]); w.r.scratch(N[j]); w.p.scratch(N[j]); w.q.scratch(N[j]);
+				w.T1.scratch(t.a * t.P * t.n * t.c); w.T2.scratch(t.a * t.n * t.Q * t.c);
 				mix.push_back(MixTask{MIX_CVT32, (long long)t.Lst->size(), t.Lst->data(), w.L.get(), nullptr, 1.0});
 				mix.push_back(MixTask{MIX_CVT32, (long long)t.Am->size(), t.Am->data(), w.Am.get(), nullptr, 1.0});
 				mix.push_back(MixTask{MIX_CVT32, (long long)t.Rst->size(), t.Rst->data(), w.R.get(), nullptr, 1.0});
@@ -535,6 +542,7 @@ struct Move {
 void moves(const std::vector<Move>& M, hipStream_t s) {
 	if (M.empty()) return;
 	const size_t J = M.size();
+	ScratchFrame frame;  // B, Qt, Rt and R; the new cores Q and T are owned
 	std::vector<DTensor> B(J), Qt(J), Rt(J), Q(J), R(J);
 	std::vector<QRJob> jobs;
 	std::vector<MapTask> maps;
@@ -544,14 +552,14 @@ void moves(const std::vector<Move>& M, hipStream_t s) {
 		if (M[j].right) {
 			const size_t m = a * n, kk = std::min(m, c);
 			Q[j] = DTensor({a, n, kk});
-			R[j] = DTensor({kk, c});
+			R[j] = DTensor::scratch({kk, c});
 			jobs.push_back(QRJob{F.data(), (int)m, (int)c, false, Q[j].data(), R[j].data()});
 		} else {  // QR(J·Fᵀ·J) = Q̃R̃ ⇒ F = (J·R̃ᵀ·J)(J·Q̃ᵀ·J)
 			const size_t m = a, nn = n * c, kk = std::min(m, nn);
-			B[j] = DTensor({nn, m});
+			B[j] = DTensor::scratch({nn, m});
 			maps.push_back(map_revtrans(F.data(), m, nn, B[j].data()));
-			Qt[j] = DTensor({nn, kk});
-			Rt[j] = DTensor({kk, m});
+			Qt[j] = DTensor::scratch({nn, kk});
+			Rt[j] = DTensor::scratch({kk, m});
 			jobs.push_back(QRJob{B[j].data(), (int)nn, (int)m, false, Qt[j].data(), Rt[j].data()});
 		}
 	}
@@ -562,7 +570,7 @@ void moves(const std::vector<Move>& M, hipStream_t s) {
 		if (M[j].right) continue;
 		const DTensor& F = M[j].x->cores[M[j].k];
 		const size_t a = F.dims[0], n = F.dims[1], c = F.dims[2], nn = n * c, kk = std::min(a, nn);
-		R[j] = DTensor({a, kk});
+		R[j] = DTensor::scratch({a, kk});
 		Q[j] = DTensor({kk, n, c});
 		maps.push_back(map_revtrans(Rt[j].data(), kk, a, R[j].data()));
 		maps.push_back(map_revtrans(Qt[j].data(), nn, kk, Q[j].data()));
@@ -715,14 +723,16 @@ bool exceeds_max(const TT& x) {
 void qc_moves(const std::vector<std::pair<TT*, size_t>>& M, hipStream_t s) {
 	const size_t J = M.size();
 	if (!J) return;
+	ScratchFrame frame;  // C, pivots and ranks; Q becomes the core (owned)
 	std::vector<DBuf<double>> q(J), c(J);
 	std::vector<DBuf<int>> jp(J);
-	DBuf<int> rk(J);
+	DBuf<int> rk;
+	rk.scratch(J);
 	std::vector<QCJob> jobs;
 	for (size_t j = 0; j < J; ++j) {
 		const DTensor& F = M[j].first->cores[M[j].second];
 		const size_t m = F.dims[0] * F.dims[1], n = F.dims[2], k = std::min(m, n);
-		q[j].resize(m * k); c[j].resize(k * n); jp[j].resize(n);
+		q[j].resize(m * k); c[j].scratch(k * n); jp[j].scratch(n);
 		jobs.push_back(QCJob{F.data(), (int)m, (int)n, q[j].get(), c[j].get(), jp[j].get(), rk.get() + j});
 	}
 	hh_qc_group(jobs, s);
@@ -755,19 +765,21 @@ void qc_moves(const std::vector<std::pair<TT*, size_t>>& M, hipStream_t s) {
 void cq_moves(const std::vector<std::pair<TT*, size_t>>& M, hipStream_t s) {
 	const size_t J = M.size();
 	if (!J) return;
+	ScratchFrame frame;  // Fᵀ, its factors, C, pivots and ranks; the new cores are owned
 	std::vector<DTensor> Ft(J);
 	std::vector<DBuf<double>> q(J), c(J);
 	std::vector<DBuf<int>> jp(J);
-	DBuf<int> rk(J);
+	DBuf<int> rk;
+	rk.scratch(J);
 	std::vector<MapTask> maps;
 	std::vector<QCJob> jobs;
 	for (size_t j = 0; j < J; ++j) {
 		const DTensor& F = M[j].first->cores[M[j].second];
 		const size_t r = F.dims[0], nr = F.dims[1] * F.dims[2], k = std::min(nr, r);
-		Ft[j] = DTensor({nr, r});
+		Ft[j] = DTensor::scratch({nr, r});
 		const size_t d2[2] = {r, nr}, sh[2] = {1, 0};
 		maps.push_back(map_permute(2, d2, sh, F.data(), Ft[j].data()));
-		q[j].resize(nr * k); c[j].resize(k * r); jp[j].resize(r);
+		q[j].scratch(nr * k); c[j].scratch(k * r); jp[j].scratch(r);
 		jobs.push_back(QCJob{Ft[j].data(), (int)nr, (int)r, q[j].get(), c[j].get(), jp[j].get(), rk.get() + j});
 	}
 	map_group(maps, s);
@@ -783,7 +795,7 @@ void cq_moves(const std::vector<std::pair<TT*, size_t>>& M, hipStream_t s) {
 		const DTensor& F = x.cores[M[j].second];
 		const DTensor& N = x.cores[M[j].second - 1];
 		const size_t r = F.dims[0], nr = F.dims[1] * F.dims[2], rr = (size_t)hr[j];
-		C[j] = DTensor({r, rr});
+		C[j] = DTensor::scratch({r, rr});
 		Q[j] = DTensor({rr, F.dims[1], F.dims[2]});
 		{ const size_t d2[2] = {rr, r}, sh[2] = {1, 0}; maps.push_back(map_permute(2, d2, sh, c[j].get(), C[j].data())); }
 		{ const size_t d2[2] = {nr, rr}, sh[2] = {1, 0}; maps.push_back(map_permute(2, d2, sh, q[j].get(), Q[j].data())); }
@@ -828,20 +840,22 @@ void max_rank_sweeps(std::vector<Lock*> E, hipStream_t s) {
 void edges(const std::vector<std::pair<TT*, size_t>>& M, double eps, hipStream_t s) {
 	const size_t J = M.size();
 	if (!J) return;
+	ScratchFrame frame;  // everything but the new cores Fn and Tn
 	// CQ of F (r × n·r'): pivoted QR of Fᵀ
 	std::vector<DTensor> Ft(J);
 	std::vector<DBuf<double>> q(J), c(J);
 	std::vector<DBuf<int>> jp(J);
-	DBuf<int> rk(J);
+	DBuf<int> rk;
+	rk.scratch(J);
 	std::vector<MapTask> maps;
 	std::vector<QCJob> jobs;
 	for (size_t j = 0; j < J; ++j) {
 		const DTensor& F = M[j].first->cores[M[j].second];
 		const size_t r = F.dims[0], nr = F.dims[1] * F.dims[2], k = std::min(nr, r);
-		Ft[j] = DTensor({nr, r});
+		Ft[j] = DTensor::scratch({nr, r});
 		const size_t d2[2] = {r, nr}, sh[2] = {1, 0};
 		maps.push_back(map_permute(2, d2, sh, F.data(), Ft[j].data()));
-		q[j].resize(nr * k); c[j].resize(k * r); jp[j].resize(r);
+		q[j].scratch(nr * k); c[j].scratch(k * r); jp[j].scratch(r);
 		jobs.push_back(QCJob{Ft[j].data(), (int)nr, (int)r, q[j].get(), c[j].get(), jp[j].get(), rk.get() + j});
 	}
 	map_group(maps, s);
@@ -857,15 +871,15 @@ void edges(const std::vector<std::pair<TT*, size_t>>& M, double eps, hipStream_t
 	for (size_t j = 0; j < J; ++j) {
 		const DTensor& F = M[j].first->cores[M[j].second];
 		const size_t r = F.dims[0], nr = F.dims[1] * F.dims[2], rk = (size_t)hr[j];
-		coreA[j] = DTensor({r, rk});
-		Fq[j] = DTensor({rk, F.dims[1], F.dims[2]});
+		coreA[j] = DTensor::scratch({r, rk});
+		Fq[j] = DTensor::scratch({rk, F.dims[1], F.dims[2]});
 		{ const size_t d2[2] = {rk, r}, sh[2] = {1, 0}; maps.push_back(map_permute(2, d2, sh, c[j].get(), coreA[j].data())); }
 		{ const size_t d2[2] = {nr, rk}, sh[2] = {1, 0}; maps.push_back(map_permute(2, d2, sh, q[j].get(), Fq[j].data())); }
 		// X is rk × r with rk <= r: U = V' (rk × rk) and Vt = U'ᵀ (rk × r)
 		const size_t m = rk, n = r, k = std::min(m, n);
-		u[j].resize(m * k); sv[j].resize(k); vt[j].resize(k * n);
+		u[j].scratch(m * k); sv[j].scratch(k); vt[j].scratch(k * n);
 		const size_t M2 = std::max(m, n), N2 = k, np = N2 + (N2 & 1);
-		ws[j].resize(M2 * np + np * np + 2 * np);
+		ws[j].scratch(M2 * np + np * np + 2 * np);
 		sj.push_back(SVDJob{coreA[j].data(), (int)r, (int)rk, vt[j].get(), sv[j].get(), u[j].get(), ws[j].get(), true});
 	}
 	map_group(maps, s);
@@ -875,7 +889,8 @@ void edges(const std::vector<std::pair<TT*, size_t>>& M, double eps, hipStream_t
 	for (size_t j = 0; j < J; ++j) off[j + 1] = off[j] + sv[j].size();
 	std::vector<double> S(off[J]);
 	{
-		DBuf<double> all(off[J]);
+		DBuf<double> all;
+		all.scratch(off[J]);
 		maps.clear();
 		for (size_t j = 0; j < J; ++j) maps.push_back(map_copy(sv[j].get(), all.get() + off[j], sv[j].size()));
 		map_group(maps, s);
@@ -903,7 +918,7 @@ void edges(const std::vector<std::pair<TT*, size_t>>& M, double eps, hipStream_t
 		const DTensor& F = x.cores[k0];
 		const DTensor& T = x.cores[k0 - 1];
 		const size_t r = F.dims[0], rk = (size_t)hr[j], kk = sv[j].size(), rr = rank[j];
-		Vt[j] = DTensor({rr, r});
+		Vt[j] = DTensor::scratch({rr, r});
 		maps.push_back(map_copy(vt[j].get(), Vt[j].data(), rr * r));
 		scale.push_back(map_scale_by(Vt[j].data(), sv[j].get(), rr, r, true));
 		Fn[j] = DTensor({rr, F.dims[1], F.dims[2]});

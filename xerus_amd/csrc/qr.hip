@@ -1126,6 +1126,7 @@ void qform_group(const std::vector<QTree>& trees, hipStream_t s) {
 
 void hh_qr_group(const std::vector<QRJob>& jobs, hipStream_t s) {
 	const size_t J = jobs.size();
+	ScratchFrame frame;  // the levels' reflectors, τ, Y and inner R factors
 	std::vector<std::vector<Level>> lv(J);
 	std::vector<const double*> in(J);
 	std::vector<int> rows(J);
@@ -1149,14 +1150,14 @@ void hh_qr_group(const std::vector<QRJob>& jobs, hipStream_t s) {
 			f.in = in[j]; f.ld = lv[j].empty() ? (q.trans ? q.m : n) : n; f.trans = lv[j].empty() && q.trans;
 			f.rows = rows[j]; f.rpb = L.rpb; f.n = n;
 			if (q.Q) {
-				L.V.resize((size_t)rows[j] * n);
-				L.tau.resize((size_t)L.blocks * n);
-				L.Y.resize((size_t)L.blocks * n * n);
+				L.V.scratch((size_t)rows[j] * n);
+				L.tau.scratch((size_t)L.blocks * n);
+				L.Y.scratch((size_t)L.blocks * n * n);
 				f.V = L.V.get(); f.tau = L.tau.get(); f.Y = L.Y.get();
 			}
 			double* R = q.R;
 			if (!root || !q.R) {
-				L.R.resize((size_t)L.blocks * n * n);
+				L.R.scratch((size_t)L.blocks * n * n);
 				R = L.R.get();
 			}
 			f.R = R;
@@ -1182,6 +1183,7 @@ void hh_qr_group(const std::vector<QRJob>& jobs, hipStream_t s) {
 
 void hh_qc_group(const std::vector<QCJob>& jobs, hipStream_t s) {
 	const size_t J = jobs.size();
+	ScratchFrame frame;  // the levels' reflectors, τ, Y and leaf R factors
 	std::vector<std::vector<Level>> lv(J);
 	std::vector<const double*> rootIn(J);
 	std::vector<int> rootRows(J);
@@ -1198,10 +1200,10 @@ void hh_qc_group(const std::vector<QCJob>& jobs, hipStream_t s) {
 		L.rows = q.m;
 		level_geometry(q.m, true, q.n, L.blocks, L.rpb);
 		if (L.blocks * q.n > HBR) { grouped[j] = 0; continue; }  // a deeper tree: one matrix at a time below
-		L.V.resize((size_t)q.m * q.n);
-		L.tau.resize((size_t)L.blocks * q.n);
-		L.Y.resize((size_t)L.blocks * q.n * q.n);
-		L.R.resize((size_t)L.blocks * q.n * q.n);
+		L.V.scratch((size_t)q.m * q.n);
+		L.tau.scratch((size_t)L.blocks * q.n);
+		L.Y.scratch((size_t)L.blocks * q.n * q.n);
+		L.R.scratch((size_t)L.blocks * q.n * q.n);
 		FactorArgs f{};
 		f.in = q.A; f.ld = q.n; f.rows = q.m; f.rpb = L.rpb; f.n = q.n;
 		f.V = L.V.get(); f.tau = L.tau.get(); f.Y = L.Y.get(); f.R = L.R.get();
@@ -1221,9 +1223,9 @@ void hh_qc_group(const std::vector<QCJob>& jobs, hipStream_t s) {
 		Level root;
 		root.rows = root.rpb = rootRows[j];
 		root.blocks = 1;
-		root.V.resize((size_t)rootRows[j] * q.n);
-		root.tau.resize(q.n);
-		root.Y.resize((size_t)q.n * q.n);
+		root.V.scratch((size_t)rootRows[j] * q.n);
+		root.tau.scratch(q.n);
+		root.Y.scratch((size_t)q.n * q.n);
 		FactorArgs f{};
 		f.in = rootIn[j]; f.ld = q.n; f.rows = f.rpb = rootRows[j]; f.n = q.n;
 		f.V = root.V.get(); f.tau = root.tau.get(); f.Y = root.Y.get();

@@ -140,6 +140,72 @@ Arena::~Arena() {
 	(void)hipDeviceSynchronize();
 	for (auto& c : chunks) (void)hipFree(c.first);
 }
+namespace {
+struct ScratchState {
+	std::vector<std::pair<char*, size_t>> chunks;
+	size_t chunk = 0, off = 0;
+	int depth = 0;
+};
+constexpr size_t kScratchChunk = 256ull << 20;
+thread_local std::unordered_map<int, ScratchState> t_scratch;  // chunks are kept until the process ends
+ScratchState& scratch_state(int* devOut) {
+	int dev = 0;
+	XTT_HIP(hipGetDevice(&dev));
+	if (devOut) *devOut = dev;
+	return t_scratch[dev];
+}
+}  // namespace
+
+void* scratch_alloc(size_t bytes) {
+	ScratchState& S = scratch_state(nullptr);
+	XTT_REQUIRE(S.depth > 0, "scratch allocation outside a ScratchFrame");
+	const size_t b = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+	for (;;) {
+		if (S.chunk < S.chunks.size() && S.off + b <= S.chunks[S.chunk].second) break;
+		if (S.chunk < S.chunks.size() && S.off == 0) {
+			// an empty chunk too small for this request: a larger one takes its place
+			(void)hipStreamSynchronize(stream());  // earlier launches may still use it
+			(void)hipFree(S.chunks[S.chunk].first);
+			S.chunks.erase(S.chunks.begin() + (long)S.chunk);
+		} else if (S.chunk < S.chunks.size()) {
+			++S.chunk;
+			S.off = 0;
+			continue;
+		}
+		if (S.chunk >= S.chunks.size()) {
+			void* p = nullptr;
+			const size_t sz = std::max(kScratchChunk, b);
+			hipError_t e = hipMalloc(&p, sz);
+			if (e == hipErrorOutOfMemory) {
+				(void)hipGetLastError();
+				release_cache();
+				e = hipMalloc(&p, sz);
+			}
+			if (e != hipSuccess) {
+				(void)hipGetLastError();
+				throw Error(e == hipErrorOutOfMemory ? ERR_NOMEM : ERR_HIP, std::string("hipMalloc (scratch): ") + hipGetErrorString(e));
+			}
+			S.chunks.insert(S.chunks.begin() + (long)S.chunk, {static_cast<char*>(p), sz});
+			S.off = 0;
+		}
+	}
+	char* p = S.chunks[S.chunk].first + S.off;
+	S.off += b;
+	return p;
+}
+ScratchFrame::ScratchFrame() {
+	ScratchState& S = scratch_state(&dev);
+	chunk = S.chunk;
+	off = S.off;
+	S.depth++;
+}
+ScratchFrame::~ScratchFrame() {
+	ScratchState& S = t_scratch[dev];
+	S.chunk = chunk;
+	S.off = off;
+	S.depth--;
+}
+
 ArenaScope::ArenaScope(Arena* a) : prev(t_arena) { t_arena = a; }
 bool arena_active() { return t_arena != nullptr; }
 ArenaScope::~ArenaScope() { t_arena = prev; }

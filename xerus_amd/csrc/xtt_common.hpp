@@ -134,22 +134,45 @@ struct ArenaScope {
 };
 bool arena_active();  // an ArenaScope with an arena is active on this thread
 
+// Stream-ordered scratch of the calling thread (per device): bump allocation from chunks kept until the process
+// ends. Memory taken while a ScratchFrame lives is handed out again once the frame ends, so a frame may end only
+// after every launch that touches its buffers has been enqueued on the thread's stream: launches enqueued later on
+// that stream run after them. Frames nest; scratch_alloc outside any frame is an error.
+void* scratch_alloc(size_t bytes);
+struct ScratchFrame {
+	int dev;
+	size_t chunk, off;
+	ScratchFrame();
+	~ScratchFrame();
+	ScratchFrame(const ScratchFrame&) = delete;
+	ScratchFrame& operator=(const ScratchFrame&) = delete;
+};
+
 template <class T>
-struct DBuf {  // owning device buffer of T
+struct DBuf {  // device buffer of T: owned (pool) or a view of the thread's scratch (scratch())
 	T* p = nullptr;
 	size_t n = 0;
+	bool own = true;
 	DBuf() = default;
 	explicit DBuf(size_t count) { resize(count); }
 	DBuf(const DBuf&) = delete;
 	DBuf& operator=(const DBuf&) = delete;
-	DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
-	DBuf& operator=(DBuf&& o) noexcept { if (this != &o) { reset(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; } return *this; }
+	DBuf(DBuf&& o) noexcept : p(o.p), n(o.n), own(o.own) { o.p = nullptr; o.n = 0; o.own = true; }
+	DBuf& operator=(DBuf&& o) noexcept {
+		if (this != &o) { reset(); p = o.p; n = o.n; own = o.own; o.p = nullptr; o.n = 0; o.own = true; }
+		return *this;
+	}
 	~DBuf() { reset(); }
-	void reset() { if (p) dfree(p); p = nullptr; n = 0; }
+	void reset() { if (p && own) dfree(p); p = nullptr; n = 0; own = true; }
 	void resize(size_t count) {
-		if (count == n && p) return;
+		if (count == n && p && own) return;
 		reset();
 		if (count) { p = static_cast<T*>(dmalloc(count * sizeof(T))); n = count; }
+	}
+	// count elements from the scratch of the innermost ScratchFrame (valid until that frame ends)
+	void scratch(size_t count) {
+		reset();
+		if (count) { p = static_cast<T*>(scratch_alloc(count * sizeof(T))); n = count; own = false; }
 	}
 	T* get() const { return p; }
 	size_t size() const { return n; }
@@ -162,10 +185,18 @@ struct DTensor {
 	DBuf<double> buf;
 	DTensor() = default;
 	explicit DTensor(std::vector<size_t> d) : dims(std::move(d)) { buf.resize(count()); }
-	// Adopt a device buffer holding at least count() elements (the first count() are the tensor).
+	// Adopt an owned device buffer holding at least count() elements (the first count() are the tensor).
 	DTensor(std::vector<size_t> d, DBuf<double>&& b) : dims(std::move(d)), buf(std::move(b)) {
 		if (count() > buf.n) throw Error(ERR_INVALID, "DTensor: adopted buffer too small");
+		if (!buf.own) throw Error(ERR_INVALID, "DTensor: a scratch buffer cannot be adopted");
 		buf.n = count();
+	}
+	// A temporary in the thread's scratch (valid until the innermost ScratchFrame ends).
+	static DTensor scratch(std::vector<size_t> d) {
+		DTensor t;
+		t.dims = std::move(d);
+		t.buf.scratch(t.count());
+		return t;
 	}
 	size_t count() const { size_t p = 1; for (size_t x : dims) p *= x; return p; }
 	size_t size() const { return buf.size(); }
