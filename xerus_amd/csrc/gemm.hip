@@ -463,8 +463,9 @@ void launch_x4_pair(const GemmDesc& d0, const T* A0, const T* B0, T* C0, const G
 template <class T>
 void launch_group(const std::vector<GemmTask>& tasks, hipStream_t s, GemmGroupPlan* keep = nullptr) {
 	constexpr size_t BMp = 33;
-	size_t kmax = kLdsBudget / (2 * BMp * sizeof(T));
-	kmax &= ~size_t(3);
+	// K chunks of at most 64: one launch's LDS size is its largest task's, and 64·33·2 elements let four
+	// workgroups share a compute unit (the whole K of 192 would leave one). The MFMA order over K is unchanged.
+	const size_t kmax = std::min<size_t>(64, (kLdsBudget / (2 * BMp * sizeof(T))) & ~size_t(3));
 	std::vector<GroupArgs> ga;
 	std::vector<int> start;
 	ga.reserve(tasks.size());
@@ -524,8 +525,9 @@ template <class T>
 void run_group(const GemmGroupPlan& p, hipStream_t s) {
 	if (p.count == 0) return;
 	auto kern = gemm_group_kernel<T>;
-	launch_timed(sizeof(T) == 8 ? "gemm_group_kernel<double>" : "gemm_group_kernel<float>", Work(p.flops, p.bytes), kern,
-	             dim3((unsigned)p.blocks), dim3(256), p.lds, s, static_cast<const GroupArgs*>(p.tasks), p.start, p.count);
+	const char* tag = p.tag ? p.tag : (sizeof(T) == 8 ? "gemm_group_kernel<double>" : "gemm_group_kernel<float>");
+	launch_timed(tag, Work(p.flops, p.bytes), kern, dim3((unsigned)p.blocks), dim3(256), p.lds, s,
+	             static_cast<const GroupArgs*>(p.tasks), p.start, p.count);
 }
 }  // namespace
 void gemm_group_run_f64(const GemmGroupPlan& p, hipStream_t s) { run_group<double>(p, s); }

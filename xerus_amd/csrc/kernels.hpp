@@ -260,6 +260,7 @@ struct GemmGroupPlan {
 	int count = 0, blocks = 0;
 	size_t lds = 0;
 	double flops = 0, bytes = 0;
+	const char* tag = nullptr;  // launch profiler label of the replays (default: the kernel's name)
 };
 void gemm_group_plan_f64(const std::vector<GemmTask>& tasks, GemmGroupPlan& plan, hipStream_t s);
 void gemm_group_run_f64(const GemmGroupPlan& plan, hipStream_t s);

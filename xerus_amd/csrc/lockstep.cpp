@@ -302,6 +302,10 @@ void cg_iterate(const std::vector<CGTaskT<T>>& tasks, CGState* st, int maxIter, 
                 hipStream_t s) {
 	const size_t K = tasks.size();
 	GemmGroupPlan g1, g2, g3;
+	const bool f64 = sizeof(T) == 8;
+	g1.tag = f64 ? "gemm_group_kernel<double> CG L·p" : "gemm_group_kernel<float> CG L·p";
+	g2.tag = f64 ? "gemm_group_kernel<double> CG A·T1" : "gemm_group_kernel<float> CG A·T1";
+	g3.tag = f64 ? "gemm_group_kernel<double> CG T2·R" : "gemm_group_kernel<float> CG T2·R";
 	DBuf<CGTaskT<T>> dTasks;
 	DBuf<CGChunk<T>> dChunks;
 	DBuf<double> part;
