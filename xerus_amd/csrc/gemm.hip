@@ -178,13 +178,18 @@ struct GroupArgs {
 	Args a;
 	int gx, gy;
 };
-template <class T>
+// Grouped GEMM: 64×64 output tiles (each wave 32×32, four MFMA blocks per K step) and K chunks of at most 32, so a
+// launch's LDS size (set by its largest task) stays at 2·65·32 elements and several workgroups share a compute
+// unit. The MFMA order over K does not depend on either choice.
+constexpr int kGroupTile = 64;
+constexpr size_t kGroupKc = 32;
+template <class T, int BT>
 __global__ __launch_bounds__(256) void gemm_group_kernel(const GroupArgs* tasks, const int* start, int count) {
 	const int t = group_task(start, count, (int)blockIdx.x);
 	const GroupArgs g = tasks[t];
 	const int l = (int)blockIdx.x - start[t];
 	const int bx = l % g.gx, r = l / g.gx;
-	gemm_x4_tile<T, 32, 32, 2, 2>(g.a, bx, r % g.gy, r / g.gy);
+	gemm_x4_tile<T, BT, BT, 2, 2>(g.a, bx, r % g.gy, r / g.gy);
 }
 
 // Split-K epilogue: C = alpha·Σ_s P_s + beta·C, slices summed in index order (deterministic).
@@ -462,10 +467,9 @@ void launch_x4_pair(const GemmDesc& d0, const T* A0, const T* B0, T* C0, const G
 
 template <class T>
 void launch_group(const std::vector<GemmTask>& tasks, hipStream_t s, GemmGroupPlan* keep = nullptr) {
-	constexpr size_t BMp = 33;
-	// K chunks of at most 64: one launch's LDS size is its largest task's, and 64·33·2 elements let four
-	// workgroups share a compute unit (the whole K of 192 would leave one). The MFMA order over K is unchanged.
-	const size_t kmax = std::min<size_t>(64, (kLdsBudget / (2 * BMp * sizeof(T))) & ~size_t(3));
+	constexpr int BT = kGroupTile;
+	constexpr size_t BMp = BT + 1;
+	const size_t kmax = std::min<size_t>(kGroupKc, (kLdsBudget / (2 * BMp * sizeof(T))) & ~size_t(3));
 	std::vector<GroupArgs> ga;
 	std::vector<int> start;
 	ga.reserve(tasks.size());
@@ -479,8 +483,8 @@ void launch_group(const std::vector<GemmTask>& tasks, hipStream_t s, GemmGroupPl
 		GroupArgs g;
 		g.a = make_args(d, t.A, t.B, t.C);
 		g.a.kc = (int)std::max<size_t>(4, std::min<size_t>((d.k + 3) & ~size_t(3), kmax));
-		g.gx = (int)((d.n + 31) / 32);
-		g.gy = (int)((d.m + 31) / 32);
+		g.gx = (int)((d.n + BT - 1) / BT);
+		g.gy = (int)((d.m + BT - 1) / BT);
 		lds = std::max(lds, (size_t)g.a.kc * BMp * 2 * sizeof(T));
 		start.push_back(blocks);
 		blocks += g.gx * g.gy * (int)d.batch;
@@ -508,7 +512,7 @@ void launch_group(const std::vector<GemmTask>& tasks, hipStream_t s, GemmGroupPl
 		dt = static_cast<const GroupArgs*>(table_upload(ga.data(), ga.size() * sizeof(GroupArgs), s));
 		ds = static_cast<const int*>(table_upload(start.data(), start.size() * sizeof(int), s));
 	}
-	auto kern = gemm_group_kernel<T>;
+	auto kern = gemm_group_kernel<T, kGroupTile>;
 	allow_lds(kern, lds);
 	launch_timed(sizeof(T) == 8 ? "gemm_group_kernel<double>" : "gemm_group_kernel<float>", Work(flops, bytes), kern, dim3((unsigned)blocks),
 	             dim3(256), lds, s, dt, ds, (int)ga.size());
@@ -524,7 +528,7 @@ namespace {
 template <class T>
 void run_group(const GemmGroupPlan& p, hipStream_t s) {
 	if (p.count == 0) return;
-	auto kern = gemm_group_kernel<T>;
+	auto kern = gemm_group_kernel<T, kGroupTile>;
 	const char* tag = p.tag ? p.tag : (sizeof(T) == 8 ? "gemm_group_kernel<double>" : "gemm_group_kernel<float>");
 	launch_timed(tag, Work(p.flops, p.bytes), kern, dim3((unsigned)p.blocks), dim3(256), p.lds, s,
 	             static_cast<const GroupArgs*>(p.tasks), p.start, p.count);

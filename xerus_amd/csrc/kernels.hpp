@@ -248,7 +248,7 @@ void res_core_group(const std::vector<ResCoreTask>& tasks, hipStream_t s);
 struct Dot2Task { int n; const double* a; const double* b; const double* c; const double* d; };
 void dot2_group(const std::vector<Dot2Task>& tasks, double* out, hipStream_t s);  // out[2t] = a·b, out[2t+1] = c·d
 
-// C = alpha·op(A)·op(B) + beta·C for many products in one launch (32×32 tiles, no split-K).
+// C = alpha·op(A)·op(B) + beta·C for many products in one launch (64×64 tiles, no split-K).
 struct GemmTask { GemmDesc d; const void* A; const void* B; void* C; };
 void gemm_group_f64(const std::vector<GemmTask>& tasks, hipStream_t s);
 // A grouped GEMM launch whose tables stay on the device (in buf) and that runs again with gemm_group_run_f64 (the
