@@ -950,6 +950,95 @@ void edges(const std::vector<std::pair<TT*, size_t>>& M, double eps, hipStream_t
 	}
 }
 
+// round_edge(k, k−1)'s second branch (5|F||T| < 6r⁴: the SVD of the merged pair) for the listed problems:
+// X = Fᵀ·Tᵀ (n·r' × r''·n'), its Jacobi SVD (on Xᵀ when X is wider than tall, as calculate_svd does), the truncation
+// rank from the singular values, F = Uᵀ's leading rows (rank, n, r'), T = (S·Vt)ᵀ (r'', n', rank)
+void edges2(const std::vector<std::pair<TT*, size_t>>& M, double eps, hipStream_t s) {
+	const size_t J = M.size();
+	if (!J) return;
+	ScratchFrame frame;  // X, Xᵀ, the factors and the workspaces; the new cores are owned
+	std::vector<DTensor> X(J), Xt(J);
+	std::vector<DBuf<double>> u(J), sv(J), vt(J), ws(J);
+	std::vector<size_t> mm(J), nn(J), kk(J);
+	std::vector<GemmTask> g;
+	for (size_t j = 0; j < J; ++j) {
+		const DTensor& F = M[j].first->cores[M[j].second];
+		const DTensor& T = M[j].first->cores[M[j].second - 1];
+		const size_t r = F.dims[0], m = F.dims[1] * F.dims[2], n = T.dims[0] * T.dims[1], k = std::min(m, n);
+		XTT_REQUIRE(T.dims[2] == r, "lockstep round: rank mismatch");
+		mm[j] = m; nn[j] = n; kk[j] = k;
+		X[j] = DTensor::scratch({m, n});
+		g.push_back(gt(gemm_desc(1, 1, m, n, r), F.data(), T.data(), X[j].data()));
+		u[j].scratch(m * k); sv[j].scratch(k); vt[j].scratch(k * n);
+		const size_t np = k + (k & 1);
+		ws[j].scratch(std::max(m, n) * np + np * np + 2 * np);
+	}
+	gemm_group_f64(g, s);
+	std::vector<MapTask> maps;
+	std::vector<SVDJob> sj;
+	for (size_t j = 0; j < J; ++j) {
+		const size_t m = mm[j], n = nn[j];
+		if (m >= n) {
+			sj.push_back(SVDJob{X[j].data(), (int)m, (int)n, u[j].get(), sv[j].get(), vt[j].get(), ws[j].get(), false});
+		} else {  // Xᵀ = U'·S·V'ᵀ ⇒ X = V'·S·U'ᵀ, written transposed: U = V' (m × m), Vt = U'ᵀ (m × n)
+			Xt[j] = DTensor::scratch({n, m});
+			const size_t d2[2] = {m, n}, sh[2] = {1, 0};
+			maps.push_back(map_permute(2, d2, sh, X[j].data(), Xt[j].data()));
+			sj.push_back(SVDJob{Xt[j].data(), (int)n, (int)m, vt[j].get(), sv[j].get(), u[j].get(), ws[j].get(), true});
+		}
+	}
+	map_group(maps, s);
+	jacobi_svd_group(sj, s);
+	std::vector<size_t> off(J + 1, 0);
+	for (size_t j = 0; j < J; ++j) off[j + 1] = off[j] + kk[j];
+	std::vector<double> S(off[J]);
+	{
+		DBuf<double> all;
+		all.scratch(off[J]);
+		maps.clear();
+		for (size_t j = 0; j < J; ++j) maps.push_back(map_copy(sv[j].get(), all.get() + off[j], kk[j]));
+		map_group(maps, s);
+		read_back(S.data(), all.get(), off[J] * sizeof(double), s);
+		table_ring_release(s);
+	}
+	std::vector<size_t> rank(J);
+	for (size_t j = 0; j < J; ++j) {
+		const double* sh = S.data() + off[j];
+		if (std::isnan(sh[0])) throw Error(ERR_NUMERIC, "SVD: the one-sided Jacobi iteration did not converge in 60 sweeps");
+		size_t rr = kk[j];
+		for (size_t i = 1; i < rr; ++i) if (sh[i] <= eps * sh[0]) { rr = i; break; }
+		rank[j] = rr;
+	}
+	// F = the leading rank rows of uᵀ (k × m), T = (the leading rank rows of vt, scaled by S)ᵀ: the scaling is its
+	// own launch (the tasks of one launch run concurrently)
+	std::vector<DBuf<double>> fo(J);
+	maps.clear();
+	std::vector<MapTask> scale, tr;
+	std::vector<DTensor> Tn(J);
+	for (size_t j = 0; j < J; ++j) {
+		const size_t m = mm[j], n = nn[j], k = kk[j], rr = rank[j];
+		fo[j].resize(k * m);
+		const size_t du[2] = {m, k}, sh[2] = {1, 0};
+		maps.push_back(map_permute(2, du, sh, u[j].get(), fo[j].get()));
+		scale.push_back(map_scale_by(vt[j].get(), sv[j].get(), rr, n, true));
+		const DTensor& T = M[j].first->cores[M[j].second - 1];
+		Tn[j] = DTensor({T.dims[0], T.dims[1], rr});
+		const size_t dv[2] = {rr, n};
+		tr.push_back(map_permute(2, dv, sh, vt[j].get(), Tn[j].data()));
+	}
+	map_group(maps, s);
+	map_group(scale, s);
+	map_group(tr, s);
+	for (size_t j = 0; j < J; ++j) {
+		TT& x = *M[j].first;
+		const size_t k0 = M[j].second;
+		const DTensor& F = x.cores[k0];
+		const dims_t fd{rank[j], F.dims[1], F.dims[2]};
+		x.cores[k0] = DTensor(fd, std::move(fo[j]));
+		x.cores[k0 - 1] = std::move(Tn[j]);
+	}
+}
+
 void round_lockstep(std::vector<Lock>& P, double eps, hipStream_t s) {
 	std::vector<char> initCanon(P.size());
 	std::vector<size_t> initPos(P.size());
@@ -973,10 +1062,9 @@ void round_lockstep(std::vector<Lock>& P, double eps, hipStream_t s) {
 		for (Lock& L : P) if (exceeds_max(*L.x)) E.push_back(&L);
 		max_rank_sweeps(E, s);
 	}
-	// round_edge(d−1−i, d−2−i) from the right; problems whose edge falls outside the first branch of round_edge are
-	// rounded on their own
+	// round_edge(d−1−i, d−2−i) from the right, each edge in the branch round_edge takes for it
 	for (size_t i = 0;; ++i) {
-		std::vector<std::pair<TT*, size_t>> M;
+		std::vector<std::pair<TT*, size_t>> M, M2;
 		for (Lock& L : P) {
 			if (i + 1 >= L.d) continue;
 			TT& x = *L.x;
@@ -985,12 +1073,13 @@ void round_lockstep(std::vector<Lock>& P, double eps, hipStream_t s) {
 			const DTensor& T = x.cores[from - 1];
 			const double r = (double)F.dims[0];
 			if (5.0 * (double)F.size() * (double)T.size() >= 6.0 * r * r * r * r) M.push_back({&x, from});
-			else round_edge(x, from, from - 1, 0, eps, true);
+			else M2.push_back({&x, from});
 		}
 		bool any = false;
 		for (Lock& L : P) any |= i + 1 < L.d;
 		if (!any) break;
 		edges(M, eps, s);
+		edges2(M2, eps, s);
 	}
 	for (size_t j = 0; j < P.size(); ++j) {
 		TT& x = *P[j].x;
