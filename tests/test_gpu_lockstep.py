@@ -156,6 +156,45 @@ def test_lockstep_mixed_config4_subset_matches_oracle(xe, oracle):
         assert abs(res[p] - ro) <= 0.1 * ro + 1e-11, p
 
 
+def test_lockstep_energy_stop_matches_threads(xe):
+    """ALS_SPD as preset (numHalfSweeps = 0: half-sweeps until the energy test with convergenceEpsilon stops each
+    problem, so problems stop at different counts) on 16 small problems: both drivers run the same half-sweeps and
+    local solves per batch and agree per problem to rounding (1e-12 relative, equal ranks)."""
+    print("lockstep energy stop", flush=True)
+    from xerus_amd.workloads import batch_problems
+    kw = dict(d_range=(4, 9), n_choices=(2, 3, 4), r_range=(2, 12))
+    pa, pb = batch_problems(16, **kw), batch_problems(16, **kw)
+    als = xe.ALS_SPD.copy()
+    assert als.numHalfSweeps == 0 and als.convergenceEpsilon > 0
+    als.precision = xe.XT_F64
+    ra, sa = xe.als_round_batch(als, [p[0] for p in pa], [p[1] for p in pa], [p[2] for p in pa], 2, 1e-12, threads=0)
+    rb, sb = xe.als_round_batch(als, [p[0] for p in pb], [p[1] for p in pb], [p[2] for p in pb], 2, 1e-12, threads=4)
+    print(f"  half-sweeps {sa['halfSweeps']} / {sb['halfSweeps']}, local solves {sa['localSolves']} / {sb['localSolves']}", flush=True)
+    assert sa["halfSweeps"] == sb["halfSweeps"] and sa["localSolves"] == sb["localSolves"]
+    assert sa["halfSweeps"] > 2 * 2 * 16  # some problem ran more than the two half-sweeps of a fixed count
+    for (A, x, b, shape), (_, y, _, _), r1, r2 in zip(pa, pb, ra, rb):
+        assert x.ranks() == y.ranks(), shape
+        assert (x - y).frob_norm() <= 1e-12 * y.frob_norm() + 1e-300, shape
+        assert abs(r1 - r2) <= 1e-6 * r2 + 1e-13, shape
+
+
+def test_lockstep_short_trains_match_threads(xe):
+    """Orders 2 and 3 with n in {2, 3, 7} and ranks up to 30 (clipped to the maximal ranks n^k): the boundary cores
+    that prepare_x_for_als absorbs, stacks of length one, round with one or two edges and both branches of
+    round_edge. Both drivers agree per problem to 1e-12 with equal ranks."""
+    print("lockstep short trains", flush=True)
+    from xerus_amd.workloads import batch_problems
+    kw = dict(d_range=(2, 3), n_choices=(2, 3, 7), r_range=(1, 30))
+    pa, pb = batch_problems(24, **kw), batch_problems(24, **kw)
+    ra, sa = run_batch(xe, pa, 0)
+    rb, sb = run_batch(xe, pb, 4)
+    assert sa["localSolves"] == sb["localSolves"]
+    for (A, x, b, shape), (_, y, _, _), r1, r2 in zip(pa, pb, ra, rb):
+        assert x.ranks() == y.ranks(), shape
+        assert (x - y).frob_norm() <= 1e-12 * y.frob_norm() + 1e-300, shape
+        assert abs(r1 - r2) <= 1e-6 * r2 + 1e-13, shape
+
+
 def test_lockstep_falls_back_for_other_options(xe):
     """Options outside the lockstep path (here BF16 local matvecs) run on worker threads with the same results as an
     explicit thread count."""

@@ -1150,7 +1150,12 @@ void als_round_batch(const ALSOptions& o, const std::vector<const TT*>& A, const
 				TT& xp = *x[p];
 				const dims_t noCap(xp.order() > 0 ? xp.order() - 1 : 0, SIZE_MAX);
 				for (size_t sw = 0; sw < sweeps; ++sw) {
-					als_solve(o, A[p], xp, *b[p], &st);
+					ALSStats one;  // als_solve sets halfSweeps (the call's count); the others it adds to
+					als_solve(o, A[p], xp, *b[p], &one);
+					st.halfSweeps += one.halfSweeps; st.localSolves += one.localSolves;
+					st.cgIterations += one.cgIterations; st.refinements += one.refinements;
+					st.matvecs64 += one.matvecs64; st.matvecsLow += one.matvecsLow;
+					st.flopsMatvec += one.flopsMatvec; st.flopsEnv += one.flopsEnv;
 					round(xp, noCap, eps);
 				}
 				if (residuals) residuals[p] = A[p] ? tt_residual(*A[p], xp, *b[p]) : 0.0;

@@ -33,7 +33,8 @@ struct Lock {
 	size_t first = 0, second = 0, curr = 0;
 	bool increasing = true, active = false;
 	double le2 = 1e102, le = 1e101, en = 1e100;
-	size_t halfSweeps = 0;
+	size_t halfSweeps = 0;      // of the running ALS call
+	size_t halfSweepsAll = 0;   // over all calls
 	bool canonAtEnd = true;
 	size_t corePosAtEnd = 0;
 	size_t localSolves = 0, cgIters = 0, refinements = 0;
@@ -668,6 +669,7 @@ void als_lockstep(const ALSOptions& o, std::vector<Lock>& P, hipStream_t s) {
 		for (size_t j = 0; j < atEnd.size(); ++j) {
 			Lock& L = *atEnd[j];
 			L.halfSweeps += 1;
+			L.halfSweepsAll += 1;
 			L.le2 = L.le;
 			L.le = L.en;
 			L.en = E[j];
@@ -1200,7 +1202,7 @@ void als_round_lockstep(const ALSOptions& o, const std::vector<const TT*>& A, co
 			total->localSolves += L.localSolves;
 			total->cgIterations += L.cgIters;
 			total->refinements += L.refinements;
-			total->halfSweeps += L.halfSweeps;
+			total->halfSweeps += L.halfSweepsAll;
 		}
 	}
 }
