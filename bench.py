@@ -384,6 +384,8 @@ def main():
     batch = None
     if args.batch_problems > 0:
         keys = ("driver", "precision", "seconds", "sweeps_per_s", "residual_max", "residual_median", "cg_iterations")
+        if args.warmup > 0:  # one untimed batch: the allocator cache and the scratch reach their working size
+            bench_batch(xe, np, args.batch_problems, args.batch_sweeps, 0, prec)
         batch = bench_batch(xe, np, args.batch_problems, args.batch_sweeps, 0, prec)
         print(f"# batch (lockstep, {batch['precision']}): {batch['sweeps_per_s']:.1f} sweeps/s over {args.batch_problems} problems",
               file=sys.stderr, flush=True)
