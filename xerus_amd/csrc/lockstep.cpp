@@ -1104,6 +1104,7 @@ void residuals_lockstep(std::vector<Lock>& P, double* out, hipStream_t s) {
 	std::vector<DTensor> Rk(J), G(J), last(J);
 	DBuf<double> gfin(J);
 	for (size_t k = 0; k < maxd; ++k) {
+		ScratchFrame frame;  // this step's cores of w, the products M and the factorisations' work buffers
 		std::vector<ResCoreTask> asmT;
 		std::vector<DTensor> W(J), C(J), M(J), Gn(J);
 		for (size_t j = 0; j < J; ++j) {
@@ -1114,7 +1115,8 @@ void residuals_lockstep(std::vector<Lock>& P, double* out, hipStream_t s) {
 			              (int)Bc.dims[0], (int)Bc.dims[2], (int)Ac.dims[1], (int)Ac.dims[2], k == 0, k + 1 == L.d};
 			XTT_REQUIRE(Ac.dims[2] == Xc.dims[1] && Bc.dims[1] == Ac.dims[1], "lockstep residual: dimension mismatch");
 			const size_t Lw = t.first ? 1 : (size_t)(t.rA * t.rx + t.rb), Rw = t.last ? 1 : (size_t)(t.rA2 * t.rx2 + t.rb2);
-			W[j] = DTensor({Lw, (size_t)t.n, Rw});
+			// w_0 becomes c_0, kept to the end when it is also the last core
+			W[j] = k == 0 && t.last ? DTensor({Lw, (size_t)t.n, Rw}) : DTensor::scratch({Lw, (size_t)t.n, Rw});
 			t.w = W[j].data();
 			asmT.push_back(t);
 		}
@@ -1137,9 +1139,9 @@ void residuals_lockstep(std::vector<Lock>& P, double* out, hipStream_t s) {
 				g1.push_back(gt(gemm_desc(1, 0, beta2, beta2, n), Bc.data(), Bc.data(), gout));
 			} else {
 				const size_t kk = Rk[j].dims[0], Lw = W[j].dims[0], Rw = W[j].dims[2];
-				C[j] = DTensor({kk, n, Rw});
+				C[j] = k + 1 == L.d ? DTensor({kk, n, Rw}) : DTensor::scratch({kk, n, Rw});  // the last one is kept
 				g1.push_back(gt(gemm_desc(0, 0, kk, n * Rw, Lw), Rk[j].data(), W[j].data(), C[j].data()));
-				M[j] = DTensor({beta, n, beta2});
+				M[j] = DTensor::scratch({beta, n, beta2});
 				g1.push_back(gt(gemm_desc(0, 0, beta, n * beta2, beta), G[j].data(), Bc.data(), M[j].data()));
 				g2.push_back(gt(gemm_desc(1, 0, beta2, beta2, beta * n), Bc.data(), M[j].data(), gout));
 			}

@@ -1265,6 +1265,7 @@ void hh_r_group(const std::vector<RJob>& jobs, hipStream_t s) {
 	}
 	map_group(zero, s);
 	for (;;) {
+		ScratchFrame frame;  // the level's reflectors, τ and R blocks (the launches of this level are their last use)
 		std::vector<std::pair<FactorArgs, int>> fa;
 		std::vector<std::pair<ApplyTArgs, int>> ap;
 		std::vector<SplitTask> sp;
@@ -1283,9 +1284,9 @@ void hh_r_group(const std::vector<RJob>& jobs, hipStream_t s) {
 				t.panel = false;
 			}
 			level_geometry(t.rows, t.leaves, t.w, t.blocks, t.rpb);
-			t.V.resize((size_t)t.rows * t.w);
-			t.tau.resize((size_t)t.blocks * t.w);
-			t.Rb.resize((size_t)t.blocks * t.w * t.w);
+			t.V.scratch((size_t)t.rows * t.w);
+			t.tau.scratch((size_t)t.blocks * t.w);
+			t.Rb.scratch((size_t)t.blocks * t.w * t.w);
 			FactorArgs f{};
 			f.in = t.L; f.ld = t.np; f.trans = 0;
 			f.rows = t.rows; f.rpb = t.rpb; f.n = t.w;
