@@ -356,7 +356,7 @@ void cg_iterate(const std::vector<CGTaskT<T>>& tasks, CGState* st, int maxIter, 
 	for (size_t k = 0; k < K; ++k) open[k] = k;
 	plan(open);
 	h.resize(K);
-	int done = 1, chunk = 2;  // iterations run so far; chunks grow 2, 4, 8, 16
+	int done = 1, chunk = 8;  // iterations run so far; chunks of 8, then 16 (iterations of converged systems are skipped)
 	for (;;) {
 		read_back(h.data(), st, K * sizeof(CGState), s);
 		std::vector<size_t> still;
