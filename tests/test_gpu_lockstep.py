@@ -195,6 +195,34 @@ def test_lockstep_short_trains_match_threads(xe):
         assert abs(r1 - r2) <= 1e-6 * r2 + 1e-13, shape
 
 
+def test_lockstep_edge_batches(xe):
+    """An empty batch, a batch of one problem and a batch without residuals: the lockstep driver returns what the
+    threaded one does (the single problem to 1e-12, the same solver counters)."""
+    print("lockstep edge batches", flush=True)
+    from xerus_amd.workloads import batch_problems
+    als = xe.ALS_SPD.copy()
+    als.convergenceEpsilon = 0.0
+    als.numHalfSweeps = 2
+    als.precision = xe.XT_F64
+    res, st = xe.als_round_batch(als, [], [], [], 2, 1e-12, threads=0)
+    assert len(res) == 0 and st["localSolves"] == 0
+    kw = dict(d_range=(8, 8), n_choices=(4,), r_range=(16, 16))
+    pa, pb = batch_problems(1, **kw), batch_problems(1, **kw)
+    ra, sa = run_batch(xe, pa, 0)
+    rb, sb = run_batch(xe, pb, 1)
+    assert sa["localSolves"] == sb["localSolves"] and sa["cgIterations"] == sb["cgIterations"]
+    (_, x, _, _), (_, y, _, _) = pa[0], pb[0]
+    assert x.ranks() == y.ranks()
+    assert (x - y).frob_norm() <= 1e-12 * y.frob_norm()
+    assert abs(ra[0] - rb[0]) <= 1e-6 * rb[0] + 1e-13
+    pc = batch_problems(3, **kw)
+    rc, _ = xe.als_round_batch(als, [p[0] for p in pc], [p[1] for p in pc], [p[2] for p in pc], 2, 1e-12, threads=0,
+                               residuals=False)
+    assert rc is None
+    for (A, x, b, _) in pc:
+        assert xe.residual(A, x, b) < 1e-6
+
+
 def test_lockstep_falls_back_for_other_options(xe):
     """Options outside the lockstep path (here BF16 local matvecs) run on worker threads with the same results as an
     explicit thread count."""
