@@ -393,6 +393,23 @@ def main():
             f64 = bench_batch(xe, np, args.batch_problems, args.batch_sweeps, 0, xe.XT_F64)
             batch["lockstep_f64"] = {k: f64[k] for k in keys}
             print(f"# batch (lockstep, f64): {f64['sweeps_per_s']:.1f} sweeps/s", file=sys.stderr, flush=True)
+        # MFMA use of the grouped contractions (stack pushes, right-hand sides and local products of every problem in
+        # one launch each): one more lockstep batch under the launch profiler, algorithmic 2mnk of every task of a
+        # launch (converged systems' skipped products included, so an upper bound) over the launches' kernel time
+        from xerus_amd import _lib as xl
+        xl.profile_enable(True)
+        bench_batch(xe, np, args.batch_problems, args.batch_sweeps, 0, prec)
+        recs = xl.profile_read(4096)
+        xl.profile_enable(False)
+        gm = {"f64": [0.0, 0.0, 0], "f32": [0.0, 0.0, 0]}
+        for tag, (n_, ms, fl, _by) in recs.items():
+            if tag.startswith("gemm_group_kernel"):
+                k = "f32" if "float" in tag else "f64"
+                gm[k][0] += fl
+                gm[k][1] += ms
+                gm[k][2] += n_
+        batch["contraction_mfma"] = {k: {"tflops": v[0] / (v[1] * 1e-3) / 1e12, "frac_of_peak": v[0] / (v[1] * 1e-3) / 1e12 / PEAK_TFLOPS[k],
+                                         "kernel_ms": v[1], "launches": v[2]} for k, v in gm.items() if v[1] > 0}
         if args.batch_threads > 0:
             ref = bench_batch(xe, np, args.batch_problems, args.batch_sweeps, args.batch_threads, prec)
             batch["threads_reference"] = {k: ref[k] for k in keys}
