@@ -455,4 +455,6 @@ def test_batch_matches_sequential(xe):
             x2.round(1e-12)
         assert x.ranks() == x2.ranks(), shape
         assert rel(x.full(), x2.full()) < 1e-10, shape
-        assert abs(r - xe.residual(A2, x2, b2)) < 1e-9 * r + 1e-14  # residuals at round-off level differ there
+        # both stop their local solves somewhere below the FP64 bound, the FP32 inner solves only as far as that bound
+        # needs, so residuals at round-off level differ by a few 1e-14
+        assert abs(r - xe.residual(A2, x2, b2)) < 1e-9 * r + 5e-14

@@ -698,13 +698,14 @@ struct ALSData {
 				XTT_HIP(hipMemcpyAsync(ws.r.get(), bl, N * sizeof(double), hipMemcpyDeviceToDevice, stream()));
 				rr = rb[1];
 			}
+			const double innerEff = inner_tolerance(inner, tol2, rr);
 			if (fused) {  // e from 0 for r/‖r‖ and x += ‖r‖·e, all inside one launch
 				if (stats) stats->refinements++;
 				const double rn = std::sqrt(rr);
 				cg_state_init(ws.st.get(), 0.0, 0, stream());
 				FusedCG d;
 				d.r0 = ws.r.get(); d.r0scale = 1.0 / rn; d.x = x; d.xscale = rn; d.mode = 1;
-				d.tol2 = inner * inner; d.maxIter = o.maxLocalIter;
+				d.tol2 = innerEff * innerEff; d.maxIter = o.maxLocalIter;
 				if (run_fused<float>(s, d)) continue;
 				fused = false;  // launch refused: this and later refinements on the chunked CG
 				if (stats) stats->refinements--;
@@ -719,7 +720,7 @@ struct ALSData {
 			const double rn = std::sqrt(rr);
 			scale_f64(ws.r.get(), 1.0 / rn, N, stream());
 			convert_f64_f32(ws.r.get(), ws.q32.get(), N, stream());  // right-hand side r/‖r‖
-			cg_state_init(ws.st.get(), inner * inner, o.maxLocalIter, stream());
+			cg_state_init(ws.st.get(), innerEff * innerEff, o.maxLocalIter, stream());
 			// e = 0: r32 = p32 = rhs
 			cg_start<float>(ws.r32.get(), ws.p32.get(), ws.q32.get(), ws.z32.get(), N, ws.st.get(), ws.partials.get(), stream());
 			XTT_HIP(hipMemcpyAsync(ws.e32.get(), ws.z32.get(), N * 4, hipMemcpyDeviceToDevice, stream()));

@@ -381,13 +381,15 @@ void cg_iterate(const std::vector<CGTaskT<T>>& tasks, CGState* st, int maxIter, 
 	table_ring_release(s);
 }
 
-DBuf<CGState> cg_states(size_t K, double tol2, int maxIter, hipStream_t s) {  // in the caller's scratch frame
+// states of K systems in the caller's scratch frame: tolerance tol2[k] (one value for all when tol2 has one)
+DBuf<CGState> cg_states(size_t K, const std::vector<double>& tol2, int maxIter, hipStream_t s) {
 	DBuf<CGState> st;
 	st.scratch(K);
 	std::vector<CGState> init(K);
-	for (CGState& c : init) {
+	for (size_t k = 0; k < K; ++k) {
+		CGState& c = init[k];
 		std::memset(&c, 0, sizeof(CGState));
-		c.tol2 = tol2;
+		c.tol2 = tol2.size() == 1 ? tol2[0] : tol2[k];
 		c.maxIter = maxIter;
 	}
 	const void* dInit = table_upload(init.data(), K * sizeof(CGState), s);
@@ -404,7 +406,7 @@ void local_solves_f64(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t s
 	std::vector<CGTask> tasks(J);
 	std::vector<const double*> xv(J);
 	std::vector<double*> qv(J);
-	DBuf<CGState> st = cg_states(J, o.localTol * o.localTol, o.maxLocalIter, s);
+	DBuf<CGState> st = cg_states(J, {o.localTol * o.localTol}, o.maxLocalIter, s);
 	for (size_t j = 0; j < J; ++j) {
 		site_init(S[j], Ls[j]);
 		const size_t N = S[j].a * S[j].n * S[j].c;
@@ -432,7 +434,8 @@ void local_solves_f64(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t s
 // Mixed precision (als.cpp's cg_mixed, system by system): FP64 iterative refinement around an FP32 CG. Each round
 // forms r = b − A·x in FP64 for the systems still open (one grouped residual, one read-back), stops a system when
 // ‖r‖ <= tol·‖b‖ or the refinement cap is reached, restarts from x = 0 where the warm start is worse than zero,
-// solves A·e = r/‖r‖ from e = 0 in FP32 to the inner tolerance and adds ‖r‖·e to x in FP64.
+// solves A·e = r/‖r‖ from e = 0 in FP32 to the inner tolerance (inner_tolerance: no tighter than the outer test
+// needs) and adds ‖r‖·e to x in FP64.
 void local_solves_mixed(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t s) {
 	const size_t J = Ls.size();
 	const double inner = o.innerTol > 0 ? o.innerTol : 1e-5, tol2 = o.localTol * o.localTol;
@@ -467,7 +470,7 @@ void local_solves_mixed(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t
 		read_back(hn.data(), norms.get(), 2 * open.size() * sizeof(double), s);
 		table_ring_release(s);
 		std::vector<size_t> next;
-		std::vector<double> rn;
+		std::vector<double> rn, innerTol2;
 		std::vector<MixTask> mix;
 		for (size_t i = 0; i < open.size(); ++i) {
 			const size_t j = open[i];
@@ -495,11 +498,13 @@ void local_solves_mixed(std::vector<Lock*>& Ls, const ALSOptions& o, hipStream_t
 			mix.push_back(MixTask{MIX_CVT32, (long long)N[j], src, w.rhs.get(), nullptr, 1.0 / std::sqrt(rr)});
 			next.push_back(j);
 			rn.push_back(std::sqrt(rr));
+			const double it = inner_tolerance(inner, tol2 * bb, rr);
+			innerTol2.push_back(it * it);
 		}
 		if (next.empty()) break;
 		mix_group(mix, s);
 		const size_t K = next.size();
-		DBuf<CGState> st = cg_states(K, inner * inner, o.maxLocalIter, s);
+		DBuf<CGState> st = cg_states(K, innerTol2, o.maxLocalIter, s);
 		std::vector<CGTaskT<float>> tasks(K), start(K);
 		for (size_t k = 0; k < K; ++k) {
 			Low& w = lw[next[k]];

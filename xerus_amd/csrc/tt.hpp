@@ -1,6 +1,7 @@
 // Device-resident TT tensors / operators and the Tensor-level operations xerus's TT path is built from.
 // All FP64; cores stay in HBM, only integer rank decisions and scalar norms cross to the host.
 #pragma once
+#include <cmath>
 #include <functional>
 #include <memory>
 #include <string>
@@ -162,6 +163,12 @@ void als_round_batch(const ALSOptions& o, const std::vector<const TT*>& A, const
 // The same in lockstep (threads <= 0 in als_round_batch): every step of every problem is one grouped launch for all
 // problems. Supported for one-site ALS_SPD with an operator and FP64 or FP32 (FP64-refined) local solves.
 bool lockstep_supported(const ALSOptions& o, const std::vector<const TT*>& A);
+// Relative tolerance of the low-precision inner solve of one refinement step: no tighter than needed to bring the
+// FP64 residual (‖r‖² = rr) under the outer bound (tol2 = tol²·‖b‖²) with a factor two to spare, no looser than one
+// half, and never tighter than the configured inner tolerance (the floor). The outer test stays the FP64 one.
+inline double inner_tolerance(double inner, double tol2, double rr) {
+	return std::max(inner, std::min(0.5, 0.5 * std::sqrt(tol2 / rr)));
+}
 void als_round_lockstep(const ALSOptions& o, const std::vector<const TT*>& A, const std::vector<TT*>& x,
                         const std::vector<const TT*>& b, size_t sweeps, double eps, double* residuals, ALSStats* total);
 
